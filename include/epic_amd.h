@@ -1,0 +1,285 @@
+/*
+ * epic_amd.h -- C ABI of the MI355X (gfx950) implementation of EPIC's
+ * dynamical-core timestep.
+ *
+ * Two surfaces are exported by libepic_amd.so:
+ *
+ *  1. The drop-in.  `timestep(planet, action, Buff2D)` with the exact
+ *     signature of the reference (include/epic.h:654-656,
+ *     src/shared/epic_timestep.c:62-390).  It reads and writes the caller's
+ *     global `grid` and `var` structures (src/shared/epic_globals.c), whose
+ *     memory layout is restated below (struct epic_*_abi) from the measured
+ *     offsets of the reference headers (include/epic_datatypes.h:238-627,
+ *     EPIC_PRECISION=8, x86-64 SysV).  Only the members the step touches are
+ *     named; the rest of each struct is opaque padding of the right size.
+ *
+ *  2. The native batched engine (epic_engine_*): many independent
+ *     integrations ("members") of possibly different grid sizes held
+ *     resident in HBM and advanced together, in FP64 or in the mixed
+ *     FP32/FP64 precision mode.  It has no counterpart in the reference;
+ *     `timestep()` is its single-member special case.
+ *
+ * All entry points take plain pointers and sizes.  Errors are reported as
+ * negative return codes (engine) or, for the drop-in, as an EPIC-style
+ * message on stderr followed by exit(1), as the reference's epic_error()
+ * does (src/shared/epic_funcs_io.c:4877-4889).
+ */
+#ifndef EPIC_AMD_H
+#define EPIC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <time.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Constants shared with the reference (include/epic.h:294-295,
+ * include/epic_datatypes.h:72-188).
+ * ---------------------------------------------------------------------- */
+#define EPIC_STEP_PROGS_AND_SYNC_DIAGS 0
+#define EPIC_SYNC_DIAGS_ONLY           1
+
+#define EPIC_HELD_SUAREZ_INDEX 19
+#define EPIC_MAX_NU_ORDER       8
+
+/* ------------------------------------------------------------------------
+ * Reference struct layouts (EPIC_PRECISION == 8).  Offsets were measured
+ * with gcc 11.4 on the reference headers; each is pinned by a static
+ * assertion in epic_amd/csrc/abi_check.c.
+ * ---------------------------------------------------------------------- */
+
+/* planetspec, include/epic_datatypes.h:238-274 (2584 bytes) */
+struct epic_planet_abi {
+  int32_t index;                  /*    0 */
+  char    name[32];               /*    4 */
+  char    type[16];               /*   36 */
+  char    orbital_epoch[8];       /*   52 */
+  double  re, rp;                 /*   64, 72 */
+  double  omega_sidereal;         /*   80 */
+  double  omega_synodic;          /*   88 */
+  double  cp, cpr, rgas, kappa;   /*   96 .. 120 */
+  double  GM, J2;                 /*  128, 136 */
+  double  x_h2, x_he, x_3;        /*  144 .. 160 */
+  double  orbit[8];               /*  168: a,e,i,lon_asc,lon_peri,mean_lon,period,vernal_eq_anom */
+  double  kinvisc, dynvisc, k_a;  /*  232 .. 248 */
+  void   *u;                      /*  256: zonal-wind profile function */
+  unsigned char cloud[2320];      /*  264: microphysics_spec[10], unused by the step */
+};
+
+/* gridspec, include/epic_datatypes.h:279-394 (2024 bytes) */
+struct epic_grid_abi {
+  char    geometry[16];           /*    0 */
+  char    advection_scheme[256];  /*   16 */
+  char    uv_timestep_scheme[256];/*  272 */
+  char    turbulence_scheme[256]; /*  528 */
+  char    stability_factor[256];  /*  784 */
+  char    extract_append[256];    /* 1040 */
+  double  data_version;           /* 1296 */
+  double  globe_lonbot, globe_lontop, globe_latbot, globe_lattop; /* 1304 .. 1328 */
+  char    f_plane_map[16];        /* 1336 */
+  double  f_plane_lat0, f_plane_half_width; /* 1352, 1360 */
+  int32_t dt, cfl_dt, nk, nj, ni, jtp;      /* 1368 .. 1388 */
+  int32_t wrap[3], pad[3];        /* 1392, 1404 */
+  int32_t jlo, jfirst, jlast, we_num_nodes; /* 1416 .. 1428 */
+  double  dln, dlt, dy0;          /* 1432 .. 1448 */
+  double  sgth_bot, sgth_top;     /* 1456, 1464 */
+  double  pbot, press0, mont0;    /* 1472 .. 1488 */
+  double  zeta0, zeta1, hybrid_alpha, sigma_sigma; /* 1496 .. 1520 */
+  int32_t k_sponge, k_sigma;      /* 1528, 1532 */
+  int32_t newt_cool_on, newt_cool_adjust; /* 1536, 1540 */
+  int32_t microphysics_on, nmt_physics_on, diffusion_direction; /* 1544 .. 1552 */
+  int32_t _pad0;
+  double  du_vert;                /* 1560 */
+  char    eos[8];                 /* 1568 */
+  int32_t aux_a, aux_b, aux_c;    /* 1576 */
+  int32_t _pad1;
+  double  aux_fa, aux_fb, aux_fc; /* 1592 */
+  int32_t nq;                     /* 1616 */
+  int32_t _pad2;
+  int32_t *is, *ip;               /* 1624, 1632 */
+  double  *sigmatheta;            /* 1640 */
+  double  *p_ref, *theta_ref, *h_min; /* 1648 .. 1664 */
+  double  *rln, *rlt, *f, *m, *n, *mn, *g, *lat, *lon, *dsgth, *dsgth_inv; /* 1672 .. 1752 */
+  int32_t is_spole, is_npole, drag_v, drag_zonal_avg; /* 1760 .. 1772 */
+  double  *t_sponge_inv;          /* 1776 */
+  double  topo_scale, prandtl, tau_drag, nudiv_nondim; /* 1784 .. 1808 */
+  double  ab[3];                  /* 1816 */
+  double  nu_nondim[9];           /* 1840 */
+  double  nu[9];                  /* 1912 */
+  int32_t itback, itsave, itextract, itrun; /* 1984 .. 1996 */
+  int32_t it_uv, it_uv_dis, it_uv_tend, it_h; /* 2000 .. 2012 */
+  unsigned long itime;            /* 2016 */
+};
+
+/* wind_variable, include/epic_datatypes.h:536-545 (344 bytes) */
+struct epic_wind_abi {
+  int32_t on, extract_on;
+  double *value;                  /*   8 */
+  double *tendency;               /*  16 */
+  unsigned char info[320];        /*  24: id_information[4] */
+};
+
+/* thermo_variable, include/epic_datatypes.h:547-555 (352 bytes) */
+struct epic_thermo_abi {
+  int32_t on, extract_on;
+  double *value;                  /*   8 */
+  char    advection_scheme[256];  /*  16 */
+  unsigned char info[80];         /* 272 */
+};
+
+/* diagnostic_variable, include/epic_datatypes.h:557-563 (96 bytes) */
+struct epic_diag_abi {
+  int32_t on, extract_on;
+  double *value;                  /*   8 */
+  unsigned char info[80];         /*  16 */
+};
+
+/* variablespec, include/epic_datatypes.h:566-627 (20496 bytes) */
+struct epic_var_abi {
+  int32_t on_list[54];            /*     0 */
+  int32_t extract_on;             /*   216 */
+  int32_t extract_on_list[54];    /*   220 */
+  int32_t ntp, n_t_cool;          /*   436, 440 */
+  int32_t _pad0;
+  time_t  start_time, model_time; /*   448, 456 */
+  struct epic_wind_abi u, v;      /*   464, 808 */
+  struct epic_thermo_abi hdry, theta, fpara, nu_turb; /* 1152 .. 2208 */
+  unsigned char species[14080];   /*  2560: species_variable[16], dry runs only */
+  double *pdat, *tdat, *dtdat;    /* 16640 */
+  struct epic_diag_abi hdry3, pdry3, p2, p3, theta2, h2, h3, t2, t3, rho2, rho3,
+                       exner3, fgibb3, gz3, mont3, heat3, pv3, eddy_pv3, ri2,
+                       vort3, div_uv3, w3, dzdt3, u_spinup, diffusion_coef_uv,
+                       diffusion_coef_theta, diffusion_coef_mass, gz_surface;
+                                  /* 16664 .. 19256 */
+  unsigned char nmt_diag[960];    /* 19352: ten nmt diagnostic_variables */
+  void   *t_cool_table;           /* 20312 */
+  double  l_s_value;              /* 20320: time_variable l_s (value first) */
+  unsigned char l_s_info[80];     /* 20328 */
+  unsigned char info[80];         /* 20408 */
+  double  time_fp_bar;            /* 20488 */
+};
+
+/*
+ * Drop-in replacement for the reference's timestep()
+ * (include/epic.h:654-656; body src/shared/epic_timestep.c:62-390).
+ * Reads the caller's globals `grid`, `var`, IT_ZERO, IT_MINUS1, IT_MINUS2
+ * (include/epic.h:303-304), keeps the state resident on the GPU between
+ * calls and writes the post-conditions of the reference back into the
+ * caller's arrays (see epic_amd_set_sync_mode()).
+ */
+#ifndef EPIC_AMD_NO_DROPIN_DECL
+void timestep(struct epic_planet_abi *planet, int action, double **Buff2D);
+#endif
+
+/* Host write-back policy of the drop-in: 0 = after every call (default),
+ * 1 = only when epic_amd_flush() is called (e.g. from a var_write wrapper). */
+void epic_amd_set_sync_mode(int lazy);
+/* Copy the device-resident state and diagnostics into the caller's var.*. */
+void epic_amd_flush(void);
+/* Forget the device copy; the next timestep() re-uploads var.* and grid.*. */
+void epic_amd_invalidate(void);
+
+/* ------------------------------------------------------------------------
+ * Synthetic cases: planet + grid + seeded initial state, laid out exactly
+ * as the reference lays out its globals (padded (K,J,I), Kshift=0,
+ * Jshift=-1, Ishift=0; src/shared/epic_funcs_diag.c:654-664).
+ * ---------------------------------------------------------------------- */
+enum epic_planet_kind {
+  EPIC_PLANET_HELD_SUAREZ = 0, /* dry Earth, Held & Suarez (1994) forcing   */
+  EPIC_PLANET_JUPITER_LIKE = 1,/* ideal-gas gas giant, jets, no forcing      */
+  EPIC_PLANET_SATURN_LIKE = 2, /* ideal-gas gas giant, jets, no forcing      */
+  EPIC_PLANET_VENUS_LIKE = 3   /* slow rotator, Held-Suarez-type relaxation  */
+};
+
+typedef struct {
+  int    planet;          /* enum epic_planet_kind                         */
+  int    ni, nj, nk;      /* ni a power of two, nj >= 4, nk >= 4           */
+  int    dt;              /* seconds                                       */
+  uint64_t seed;          /* noise and jet coefficients                    */
+  double nu4_nondim;      /* del^4 hyperviscosity, fraction of maximum     */
+  double nudiv_nondim;    /* divergence damping, fraction of maximum       */
+  double noise_theta;     /* K, standard deviation of theta noise          */
+  double noise_uv;        /* m/s, standard deviation of u,v noise          */
+} epic_case_spec;
+
+typedef struct {
+  /* planet */
+  char   planet_name[32];
+  char   planet_type[16];
+  int    planet_index;
+  double re, rp, omega, cp, rgas, cpr, kappa, GM, J2, x_h2, kinvisc;
+  /* grid scalars */
+  int    ni, nj, nk, dt, jtp, k_sigma, k_sponge;
+  int    newt_cool_on, newt_cool_adjust, drag_v, drag_zonal_avg;
+  double dln, dlt, dy0, globe_lonbot, globe_lontop, globe_latbot, globe_lattop;
+  double press0, pbot, ptop, zeta0, zeta1, hybrid_alpha, sigma_sigma;
+  double topo_scale, prandtl, tau_drag, nudiv_nondim;
+  double nu[EPIC_MAX_NU_ORDER + 1], nu_nondim[EPIC_MAX_NU_ORDER + 1];
+  /* padded extents */
+  int    Iadim, Jadim, Kadim, Nelem2d, Nelem3d;
+  /* 1-D arrays (length 2*(ni+1)+1 for lon, 2*(nj+1)+1 for the J-arrays,
+   * 2*(nk+1)+1 for the kk-arrays, nk+2 for h_min, k_sponge+1 for the sponge) */
+  double *lon, *lat, *rln, *rlt, *f, *m, *n, *mn, *g;
+  double *sigmatheta, *dsgth, *dsgth_inv, *p_ref, *theta_ref, *h_min, *t_sponge_inv;
+  /* initial state, Nelem3d each (gz_surface: Nelem2d) */
+  double *u, *v, *hdry, *theta, *p3, *u_spinup, *gz_surface;
+} epic_case;
+
+/* Build a case from a spec: metric arrays (set_lonlat/set_fmn/set_gravity/
+ * set_dsgth restated), hybrid sigma-theta levels and a seeded initial
+ * state.  Returns 0 or a negative error code. */
+int  epic_case_build(const epic_case_spec *spec, epic_case *out);
+void epic_case_free(epic_case *c);
+/* Bytes of HBM one member of this size holds in the given precision mode. */
+size_t epic_member_footprint(int ni, int nj, int nk, int precision_mode);
+
+/* ------------------------------------------------------------------------
+ * Native engine.
+ * ---------------------------------------------------------------------- */
+enum epic_precision { EPIC_FP64 = 0, EPIC_MIXED = 1 };
+
+enum epic_field {
+  EPIC_F_U = 0, EPIC_F_V, EPIC_F_HDRY, EPIC_F_THETA, EPIC_F_P3, EPIC_F_P2,
+  EPIC_F_H3, EPIC_F_THETA2, EPIC_F_T2, EPIC_F_T3, EPIC_F_RHO2, EPIC_F_RHO3,
+  EPIC_F_EXNER3, EPIC_F_GZ3, EPIC_F_MONT3, EPIC_F_PV3, EPIC_F_DIV_UV3,
+  EPIC_F_W3, EPIC_F_HEAT3, EPIC_F_DZDT3, EPIC_F_DUDT0, EPIC_F_DUDT1,
+  EPIC_F_DUDT2, EPIC_F_DVDT0, EPIC_F_DVDT1, EPIC_F_DVDT2, EPIC_F_GZ_SURFACE,
+  EPIC_F_COUNT
+};
+
+typedef struct epic_engine epic_engine;
+
+/* Create an engine that may hold up to `max_members` members using at most
+ * `hbm_budget` bytes of device memory (0 = no limit). */
+epic_engine *epic_engine_create(int precision_mode, int max_members, size_t hbm_budget);
+void         epic_engine_destroy(epic_engine *e);
+/* Upload a case; primes the diagnostics (the reference's SYNC_DIAGS_ONLY).
+ * Returns the member slot (>= 0), or a negative code (-2: no memory). */
+int  epic_engine_add(epic_engine *e, const epic_case *c);
+int  epic_engine_remove(epic_engine *e, int slot);
+/* Advance every active member by nsteps (STEP_PROGS_AND_SYNC_DIAGS each). */
+int  epic_engine_step(epic_engine *e, int nsteps);
+/* Advance only the listed members. */
+int  epic_engine_step_members(epic_engine *e, const int *slots, int nslots, int nsteps);
+/* Copy one field of one member into host memory (Nelem3d doubles, or
+ * Nelem2d for EPIC_F_GZ_SURFACE; tendencies are indexed by time plane). */
+int  epic_engine_read(epic_engine *e, int slot, int field, double *out);
+/* Overwrite one prognostic field (U, V, HDRY, THETA) of a member. */
+int  epic_engine_write(epic_engine *e, int slot, int field, const double *in);
+long epic_engine_steps_done(epic_engine *e, int slot);
+size_t epic_engine_bytes_in_use(epic_engine *e);
+/* Wait for all queued device work. */
+int  epic_engine_sync(epic_engine *e);
+/* Record device timing of each kernel over the next `nsteps`; fills
+ * names/ms arrays (up to cap entries).  Returns the entry count. */
+int  epic_engine_profile(epic_engine *e, int nsteps, char (*names)[48], double *ms_total,
+                         int *launches, int cap);
+const char *epic_engine_last_error(epic_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EPIC_AMD_H */

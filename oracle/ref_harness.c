@@ -1,0 +1,240 @@
+/*
+ * ref_harness.c -- drives the reference's own FP64 timestep() (compiled
+ * from /root/reference by oracle/Makefile into oracle/_ref/) on a synthetic
+ * case and dumps the fields.  Test infrastructure only: nothing in the
+ * product links or loads it.
+ *
+ * It does what epic_main does between reading epic.nc and the time loop
+ * (src/shared/epic_main.c:248-303): make_arrays, set_lonlat, set_fmn,
+ * set_gravity, set_dsgth, HDRY from P3 via get_h, timestep(SYNC_DIAGS_ONLY);
+ * then timestep(STEP_PROGS_AND_SYNC_DIAGS) with grid.itime = 0, 1, 2, ...
+ * (src/shared/epic_main.c:336-427).  The initial state comes from the
+ * product's case builder, so both sides start from the same bits.
+ *
+ * Two points the reference leaves to its `initial` program are supplied
+ * here: the FLOAT_MAX Adams-Bashforth start-up flags
+ * (src/shared/epic_initial.c:1301-1316) and, since Held-Suarez forces
+ * k_sponge = 0 and make_arrays then allocates no t_sponge_inv
+ * (src/shared/epic_funcs_diag.c:859-861) while uv_drag still reads
+ * t_sponge_inv[0] (src/shared/epic_timestep.c:1363-1374), a zeroed
+ * one-element array (zero sponge).
+ *
+ * Usage: epic_ref PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT [DUMP_STEP ...]
+ * A DUMP_STEP of 0 dumps the state after the SYNC call; s > 0 after s steps.
+ * A negative DUMP_STEP -s dumps at s and prints the metric comparison only.
+ * Output: int32 header {magic, ni, nj, nk, ndumps, nfields, nelem3d, nelem2d,
+ * metric_mismatch}, then per dump: int32 {step, it_zero, it_minus1,
+ * it_minus2} followed by the fields in epic_field order.
+ */
+#include <epic.h>
+
+#define EPIC_AMD_NO_DROPIN_DECL
+#include "../include/epic_amd.h"
+
+#define MAGIC 0x45504943
+
+static int cmp_arr(const char *name, const double *a, const double *b, int n) {
+  int bad = 0;
+  for (int i = 0; i < n; i++)
+    if (memcmp(&a[i], &b[i], sizeof(double)) != 0) {
+      if (!bad) fprintf(stderr, "metric %s differs at %d: %.17g vs %.17g\n", name, i, a[i], b[i]);
+      bad++;
+    }
+  return bad;
+}
+
+static void write_fields(FILE *fo, int step) {
+  int hdr[4] = {step, IT_ZERO, IT_MINUS1, IT_MINUS2};
+  fwrite(hdr, sizeof(int), 4, fo);
+  double *f3[] = {var.u.value, var.v.value, var.hdry.value, var.theta.value, var.p3.value,
+                  var.p2.value, var.h3.value, var.theta2.value, var.t2.value, var.t3.value,
+                  var.rho2.value, var.rho3.value, var.exner3.value, var.gz3.value,
+                  var.mont3.value, var.pv3.value, var.div_uv3.value, var.w3.value,
+                  var.heat3.value, var.dzdt3.value};
+  for (int q = 0; q < 20; q++) fwrite(f3[q], sizeof(double), Nelem3d, fo);
+  for (int p = 0; p < 3; p++) fwrite(var.u.tendency + (size_t)p * Nelem3d, sizeof(double), Nelem3d, fo);
+  for (int p = 0; p < 3; p++) fwrite(var.v.tendency + (size_t)p * Nelem3d, sizeof(double), Nelem3d, fo);
+  fwrite(var.gz_surface.value, sizeof(double), Nelem2d, fo);
+}
+
+int main(int argc, char **argv) {
+  if (argc < 13) {
+    fprintf(stderr, "usage: %s PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT [DUMP ...]\n",
+            argv[0]);
+    return 2;
+  }
+  epic_case_spec sp;
+  memset(&sp, 0, sizeof(sp));
+  sp.planet = atoi(argv[1]);
+  sp.ni = atoi(argv[2]);
+  sp.nj = atoi(argv[3]);
+  sp.nk = atoi(argv[4]);
+  sp.dt = atoi(argv[5]);
+  sp.seed = strtoull(argv[6], NULL, 10);
+  sp.nu4_nondim = atof(argv[7]);
+  sp.nudiv_nondim = atof(argv[8]);
+  sp.noise_theta = atof(argv[9]);
+  sp.noise_uv = atof(argv[10]);
+  int nsteps = atoi(argv[11]);
+  const char *out = argv[12];
+  int ndump = argc - 13;
+  int dumps[256];
+  for (int q = 0; q < ndump && q < 256; q++) dumps[q] = atoi(argv[13 + q]);
+
+  epic_case c;
+  int rc = epic_case_build(&sp, &c);
+  if (rc) {
+    fprintf(stderr, "epic_case_build failed: %d\n", rc);
+    return 3;
+  }
+
+  static planetspec pl;
+  memset(&pl, 0, sizeof(pl));
+  pl.index = c.planet_index;
+  strcpy(pl.name, c.planet_name);
+  strcpy(pl.type, c.planet_type);
+  strcpy(pl.orbital_epoch, "J2000");
+  pl.re = c.re;
+  pl.rp = c.rp;
+  pl.omega_sidereal = c.omega;
+  pl.omega_synodic = c.omega;
+  pl.cp = c.cp;
+  pl.rgas = c.rgas;
+  pl.cpr = c.cpr;
+  pl.kappa = c.kappa;
+  pl.GM = c.GM;
+  pl.J2 = c.J2;
+  pl.x_h2 = 0.;
+  pl.x_he = 0.;
+  pl.x_3 = 1.;
+  pl.a = 1.;
+  pl.e = 0.0167;
+  pl.orbit_period = 1.;
+  pl.kinvisc = c.kinvisc;
+  planet = &pl;
+
+  memset(&grid, 0, sizeof(grid));
+  memset(&var, 0, sizeof(var));
+  strcpy(grid.geometry, "globe");
+  strcpy(grid.uv_timestep_scheme, "3rd-order Adams-Bashforth");
+  strcpy(grid.turbulence_scheme, "off");
+  strcpy(grid.eos, "ideal");
+  strcpy(grid.f_plane_map, "none");
+  grid.dt = c.dt;
+  grid.nk = c.nk;
+  grid.nj = c.nj;
+  grid.ni = c.ni;
+  grid.jtp = c.jtp;
+  grid.wrap[0] = 1;
+  grid.pad[0] = grid.pad[1] = grid.pad[2] = 1;
+  grid.jlo = 0;
+  grid.jfirst = 1;
+  grid.jlast = c.nj - 1;
+  grid.dln = c.dln;
+  grid.dlt = c.dlt;
+  grid.globe_lonbot = c.globe_lonbot;
+  grid.globe_lontop = c.globe_lontop;
+  grid.globe_latbot = c.globe_latbot;
+  grid.globe_lattop = c.globe_lattop;
+  grid.pbot = c.pbot;
+  grid.press0 = c.press0;
+  grid.zeta0 = c.zeta0;
+  grid.zeta1 = c.zeta1;
+  grid.hybrid_alpha = c.hybrid_alpha;
+  grid.sigma_sigma = c.sigma_sigma;
+  grid.k_sponge = c.k_sponge;
+  grid.k_sigma = c.k_sigma;
+  grid.newt_cool_on = c.newt_cool_on;
+  grid.newt_cool_adjust = c.newt_cool_adjust;
+  grid.topo_scale = c.topo_scale;
+  grid.prandtl = c.prandtl;
+  grid.tau_drag = c.tau_drag;
+  grid.nudiv_nondim = c.nudiv_nondim;
+  grid.drag_v = c.drag_v;
+  grid.drag_zonal_avg = c.drag_zonal_avg;
+  for (int q = 0; q <= MAX_NU_ORDER; q++) {
+    grid.nu[q] = c.nu[q];
+    grid.nu_nondim[q] = c.nu_nondim[q];
+  }
+  var.on_list[U_INDEX] = var.on_list[V_INDEX] = 1;
+  var.on_list[HDRY_INDEX] = var.on_list[THETA_INDEX] = 1;
+  strcpy(var.hdry.advection_scheme, "Predictor-corrector (Hsu, Konor, and Arakawa)");
+  strcpy(var.theta.advection_scheme, "Predictor-corrector (Hsu, Konor, and Arakawa)");
+  var.time_fp_bar = 3.e8 * 1000. * 100.;
+
+  make_arrays(planet);
+  grid.t_sponge_inv = fvector(0, grid.k_sponge > 0 ? grid.k_sponge : 0, "harness");
+
+  set_lonlat();
+  set_fmn(planet);
+  set_gravity(planet);
+  for (int kk = 0; kk <= 2 * (grid.nk + 1); kk++) {
+    grid.sigmatheta[kk] = c.sigmatheta[kk];
+    grid.p_ref[kk] = c.p_ref[kk];
+    grid.theta_ref[kk] = c.theta_ref[kk];
+  }
+  for (int K = 0; K <= grid.nk + 1; K++) grid.h_min[K] = c.h_min[K];
+  set_dsgth();
+
+  int nJ = 2 * (grid.nj + 1) + 1, nK = 2 * (grid.nk + 1) + 1;
+  int mism = 0;
+  mism += cmp_arr("lon", grid.lon, c.lon, 2 * (grid.ni + 1) + 1);
+  mism += cmp_arr("lat", grid.lat, c.lat, nJ);
+  mism += cmp_arr("rln", grid.rln, c.rln, nJ);
+  mism += cmp_arr("rlt", grid.rlt, c.rlt, nJ);
+  mism += cmp_arr("f", grid.f, c.f, nJ);
+  mism += cmp_arr("m", grid.m, c.m, nJ);
+  mism += cmp_arr("n", grid.n, c.n, nJ);
+  mism += cmp_arr("mn", grid.mn, c.mn, nJ);
+  mism += cmp_arr("g", grid.g, c.g, nJ);
+  mism += cmp_arr("dsgth", grid.dsgth, c.dsgth, nK);
+  mism += cmp_arr("dsgth_inv", grid.dsgth_inv, c.dsgth_inv, nK);
+  if (memcmp(&grid.dy0, &c.dy0, sizeof(double))) {
+    fprintf(stderr, "metric dy0 differs\n");
+    mism++;
+  }
+
+  memcpy(var.u.value, c.u, sizeof(double) * Nelem3d);
+  memcpy(var.v.value, c.v, sizeof(double) * Nelem3d);
+  memcpy(var.theta.value, c.theta, sizeof(double) * Nelem3d);
+  memcpy(var.p3.value, c.p3, sizeof(double) * Nelem3d);
+  memcpy(var.u_spinup.value, c.u_spinup, sizeof(double) * Nelem3d);
+  memcpy(var.gz_surface.value, c.gz_surface, sizeof(double) * Nelem2d);
+
+  timeplane_bookkeeping();
+  for (int K = KLO; K <= KHI; K++)
+    for (int J = JLO; J <= JHI; J++)
+      for (int I = ILO; I <= IHI; I++) HDRY(K, J, I) = get_h(planet, 2 * K, J, I, DRY);
+  bc_lateral(var.hdry.value, THREEDIM);
+  /* AB3 start-up flags */
+  for (int K = 1; K <= KHI; K++)
+    for (int J = JLO; J <= JHI; J++)
+      for (int I = ILO; I <= IHI; I++) {
+        DUDT(IT_MINUS2, K, J, I) = FLOAT_MAX;
+        DUDT(IT_MINUS1, K, J, I) = FLOAT_MAX;
+        DVDT(IT_MINUS2, K, J, I) = FLOAT_MAX;
+        DVDT(IT_MINUS1, K, J, I) = FLOAT_MAX;
+      }
+
+  EPIC_FLOAT *Buff2D[NUM_WORKING_BUFFERS];
+  for (int q = 0; q < NUM_WORKING_BUFFERS; q++) Buff2D[q] = fvector(0, Nelem2d - 1, "harness");
+
+  FILE *fo = fopen(out, "wb");
+  if (!fo) return 4;
+  int hdr[9] = {MAGIC, grid.ni, grid.nj, grid.nk, ndump, EPIC_F_COUNT, Nelem3d, Nelem2d, mism};
+  fwrite(hdr, sizeof(int), 9, fo);
+
+  grid.itime = 0;
+  timestep(planet, SYNC_DIAGS_ONLY, Buff2D);
+  for (int q = 0; q < ndump; q++)
+    if (dumps[q] == 0) write_fields(fo, 0);
+  for (int s = 1; s <= nsteps; s++) {
+    grid.itime = s - 1;
+    timestep(planet, STEP_PROGS_AND_SYNC_DIAGS, Buff2D);
+    for (int q = 0; q < ndump; q++)
+      if (dumps[q] == s) write_fields(fo, s);
+  }
+  fclose(fo);
+  epic_case_free(&c);
+  return mism ? 5 : 0;
+}
