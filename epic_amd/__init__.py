@@ -1,0 +1,178 @@
+"""epic_amd -- MI355X (gfx950) implementation of EPIC's dynamical-core
+timestep, driven through the C ABI of libepic_amd.so (include/epic_amd.h).
+
+The Python layer only marshals arguments: every computation of the step runs
+in the HIP kernels of the shared library.  If the library is missing the
+import fails loudly; there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libepic_amd.so")
+
+# planets (enum epic_planet_kind)
+HELD_SUAREZ, JUPITER_LIKE, SATURN_LIKE, VENUS_LIKE = 0, 1, 2, 3
+FP64, MIXED = 0, 1
+
+FIELDS = ["U", "V", "HDRY", "THETA", "P3", "P2", "H3", "THETA2", "T2", "T3", "RHO2", "RHO3",
+          "EXNER3", "GZ3", "MONT3", "PV3", "DIV_UV3", "W3", "HEAT3", "DZDT3", "DUDT0", "DUDT1",
+          "DUDT2", "DVDT0", "DVDT1", "DVDT2", "GZ_SURFACE"]
+FIELD = {n: i for i, n in enumerate(FIELDS)}
+
+
+class CaseSpec(ctypes.Structure):
+    """epic_case_spec of include/epic_amd.h."""
+    _fields_ = [("planet", ctypes.c_int), ("ni", ctypes.c_int), ("nj", ctypes.c_int),
+                ("nk", ctypes.c_int), ("dt", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("nu4_nondim", ctypes.c_double), ("nudiv_nondim", ctypes.c_double),
+                ("noise_theta", ctypes.c_double), ("noise_uv", ctypes.c_double)]
+
+
+def spec(planet=HELD_SUAREZ, ni=64, nj=32, nk=10, dt=600, seed=1, nu4=0.5, nudiv=0.0,
+         noise_theta=0.1, noise_uv=0.0):
+    return CaseSpec(planet, ni, nj, nk, dt, seed, nu4, nudiv, noise_theta, noise_uv)
+
+
+def dims(sp):
+    """(Iadim, Jadim, Kadim) of a spec: the reference's padded extents."""
+    return sp.ni + 2, sp.nj + 3, sp.nk + 2
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.epic_engine_create.restype = vp
+        L.epic_engine_create.argtypes = [ci, ci, ctypes.c_size_t]
+        L.epic_engine_destroy.argtypes = [vp]
+        L.epic_engine_add_spec.argtypes = [vp, ctypes.POINTER(CaseSpec)]
+        L.epic_engine_add_spec.restype = ci
+        L.epic_engine_remove.argtypes = [vp, ci]
+        L.epic_engine_step.argtypes = [vp, ci]
+        L.epic_engine_step_members.argtypes = [vp, ctypes.POINTER(ci), ci, ci]
+        L.epic_engine_read.argtypes = [vp, ci, ci, vp]
+        L.epic_engine_write.argtypes = [vp, ci, ci, vp]
+        L.epic_engine_sync.argtypes = [vp]
+        L.epic_engine_steps_done.argtypes = [vp, ci]
+        L.epic_engine_steps_done.restype = ctypes.c_long
+        L.epic_engine_bytes_in_use.argtypes = [vp]
+        L.epic_engine_bytes_in_use.restype = ctypes.c_size_t
+        L.epic_engine_last_error.argtypes = [vp]
+        L.epic_engine_last_error.restype = ctypes.c_char_p
+        L.epic_engine_time_planes.argtypes = [vp, ci, ctypes.POINTER(ci)]
+        L.epic_engine_profile.argtypes = [vp, ci, vp, ctypes.POINTER(cd), ctypes.POINTER(ci), ci]
+        L.epic_member_footprint.argtypes = [ci, ci, ci, ci]
+        L.epic_member_footprint.restype = ctypes.c_size_t
+        _lib = L
+    return _lib
+
+
+class EpicError(RuntimeError):
+    pass
+
+
+class Engine:
+    """Device-resident members advanced together (epic_engine_* of the C ABI)."""
+
+    def __init__(self, precision=FP64, max_members=1, hbm_budget=0):
+        L = lib()
+        self._e = L.epic_engine_create(precision, max_members, hbm_budget)
+        if not self._e:
+            raise EpicError("epic_engine_create failed")
+        self.specs = {}
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = lib().epic_engine_last_error(self._e).decode()
+            raise EpicError(f"{what} failed ({rc}): {msg}")
+        return rc
+
+    def add(self, sp):
+        slot = self._check(lib().epic_engine_add_spec(self._e, ctypes.byref(sp)), "add")
+        self.specs[slot] = sp
+        return slot
+
+    def remove(self, slot):
+        self._check(lib().epic_engine_remove(self._e, slot), "remove")
+        self.specs.pop(slot, None)
+
+    def step(self, n=1):
+        self._check(lib().epic_engine_step(self._e, n), "step")
+
+    def step_members(self, slots, n=1):
+        arr = (ctypes.c_int * len(slots))(*slots)
+        self._check(lib().epic_engine_step_members(self._e, arr, len(slots), n), "step_members")
+
+    def sync(self):
+        self._check(lib().epic_engine_sync(self._e), "sync")
+
+    def read(self, slot, field):
+        sp = self.specs[slot]
+        Ia, Ja, Ka = dims(sp)
+        f = FIELD[field] if isinstance(field, str) else field
+        shape = (Ja, Ia) if f == FIELD["GZ_SURFACE"] else (Ka, Ja, Ia)
+        out = np.empty(shape, dtype=np.float64)
+        self._check(lib().epic_engine_read(self._e, slot, f, out.ctypes.data), "read")
+        return out
+
+    def write_field(self, slot, field, arr):
+        """Overwrite a prognostic field (U, V, HDRY, THETA) of a member."""
+        sp = self.specs[slot]
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != tuple(reversed(dims(sp))):
+            raise ValueError(f"expected shape {tuple(reversed(dims(sp)))}, got {a.shape}")
+        f = FIELD[field] if isinstance(field, str) else field
+        self._check(lib().epic_engine_write(self._e, slot, f, a.ctypes.data), "write")
+
+    def time_planes(self, slot):
+        arr = (ctypes.c_int * 4)()
+        self._check(lib().epic_engine_time_planes(self._e, slot, arr), "time_planes")
+        return list(arr)
+
+    def tendency(self, slot, wind, back=0):
+        """Tendency plane of U or V: back=0 is the newest (IT_MINUS1 after a step)."""
+        it0, itm1, itm2, _ = self.time_planes(slot)
+        plane = [itm1, itm2, it0][back]
+        return self.read(slot, f"D{wind}DT{plane}")
+
+    def steps_done(self, slot):
+        return lib().epic_engine_steps_done(self._e, slot)
+
+    def bytes_in_use(self):
+        return lib().epic_engine_bytes_in_use(self._e)
+
+    def profile(self, nsteps, cap=128):
+        names = ctypes.create_string_buffer(48 * cap)
+        ms = (ctypes.c_double * cap)()
+        cnt = (ctypes.c_int * cap)()
+        n = self._check(lib().epic_engine_profile(self._e, nsteps, names, ms, cnt, cap), "profile")
+        raw = names.raw
+        out = []
+        for i in range(n):
+            nm = raw[48 * i:48 * (i + 1)].split(b"\0", 1)[0].decode()
+            out.append((nm, ms[i], cnt[i]))
+        return out
+
+    def close(self):
+        if self._e:
+            lib().epic_engine_destroy(self._e)
+            self._e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def footprint(ni, nj, nk, precision=FP64):
+    return lib().epic_member_footprint(ni, nj, nk, precision)

@@ -1,0 +1,183 @@
+// dev_common.h -- device-side data model of the MI355X EPIC engine.
+//
+// Every kernel is launched over a bucket of members that share (ni,nj,nk):
+// blockIdx.z selects the member, the remaining grid covers that member's
+// cells.  All per-member pointers and constants live in a MemberDesc held in
+// device memory; the per-member time-plane rotation of the Adams-Bashforth
+// tendencies lives in device memory too (MemberDyn), so a captured step
+// replays without host involvement.
+//
+// Layout of every 3-D field is the reference's padded (K,J,I):
+//   index = I + (J+1)*Iadim + K*Nelem2d,  K = 0..nk+1, J = -1..nj+1, I = 0..ni+1
+// (src/shared/epic_funcs_diag.c:654-664, include/epic.h:407-523).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace epic {
+
+constexpr int KMAX = 66;  // nk <= 64
+
+struct MemberDyn {
+  int it0, itm1, itm2;  // IT_ZERO, IT_MINUS1, IT_MINUS2
+  int itime;            // value of grid.itime during the next step
+  int err;              // sticky fatal-condition bits
+  int pad[3];
+};
+
+struct MemberDesc {
+  int ni, nj, nk, Ia, N2, N3;
+  int ks, k_sponge;  // k_sigma, k_sponge
+  int is_hs, is_gas, newt_cool_on, newt_cool_adjust, drag_v, drag_zonal_avg, jtp;
+  int nrow_h, nrow_v, nlogn;  // filtered rows (h- and v-grid), log2(ni)
+  int mixed;                  // precision mode
+  double dt, kappa, cp, cpr, rgas, press0, zeta0, zeta1, alpha, sigma_sigma;
+  double topo_scale, prandtl, tau_drag;
+  double coef2, coef4, nu4, m0z, nudiv, eps13, mu3, mu2, g_coeff;
+  // 1-D metrics and tables
+  const double *lat, *lon, *m, *n, *mn, *f, *g, *rln, *rlt, *sgth, *dsgth, *dsgth_inv, *h_min, *t_sponge;
+  const double *mn_inv, *hl_h, *hl_pv, *hcx, *hcy, *hcz, *vcx, *vcy, *vcz;
+  const double *tw_f, *tw_i, *tr_f, *tr_i;  // four1 / realft twiddles, forward and inverse
+  const float *tw_f32, *tw_i32, *tr_f32, *tr_i32;
+  const int *rows_h, *rows_v;
+  // state and diagnostics
+  double *U, *V, *DUDT, *DVDT, *HDRY, *THETA, *P3, *P2, *THETA2, *H3, *T2, *T3, *RHO2, *RHO3;
+  double *EXNER3, *GZ3, *MONT3, *HEAT3, *PV3, *DIV, *W3, *DZDT3, *USPIN, *GZS;
+  float *DUDTf, *DVDTf;  // tendency planes in the mixed mode
+  // scratch
+  double *gz, *varz, *sA, *sB, *sC, *sD, *unity, *old, *spl;
+  double *wstale;  // hsu column statics carried between calls (per member)
+  MemberDyn *dyn;
+};
+
+__device__ __forceinline__ size_t ix3(const MemberDesc &d, int K, int J, int I) {
+  return (size_t)I + (size_t)(J + 1) * d.Ia + (size_t)K * d.N2;
+}
+__device__ __forceinline__ size_t ix2(const MemberDesc &d, int J, int I) {
+  return (size_t)I + (size_t)(J + 1) * d.Ia;
+}
+
+__device__ __forceinline__ double dmax(double x, double y) { return x > y ? x : y; }
+__device__ __forceinline__ double dmin(double x, double y) { return x < y ? x : y; }
+
+// fcmp (src/shared/epic_funcs_util.c:4275-4346): compare to within one ulp of
+// the larger magnitude's binade.
+__device__ __forceinline__ int fcmp_d(double x1, double x2) {
+  int e;
+  frexp(fabs(x1) > fabs(x2) ? x1 : x2, &e);
+  double delta = ldexp(2.220446049250313e-16, e), d = x1 - x2;
+  if (d > delta) return 1;
+  if (d < -delta) return -1;
+  return 0;
+}
+
+// hybrid sigma-theta coordinate (src/shared/epic_funcs_diag.c:1238-1314,1751-1770)
+__device__ __forceinline__ double get_sigma(double pbot, double p, double ptop) {
+  return log(p / pbot) / log(ptop / pbot);
+}
+__device__ __forceinline__ double get_p_sigma(double pbot, double s, double ptop) {
+  return pbot * exp(s * log(ptop / pbot));
+}
+__device__ __forceinline__ double g_sig(const MemberDesc &d, double s) {
+  if (s <= d.sigma_sigma) return 0.;
+  if (s >= 1.) return 1.;
+  return d.g_coeff * (1. - exp(-d.alpha * (s - d.sigma_sigma)));
+}
+__device__ __forceinline__ double f_sig(const MemberDesc &d, double s) {
+  if (s < 0.) return d.zeta0;
+  if (s <= d.sigma_sigma) return d.zeta0 + s * (d.zeta1 - d.zeta0);
+  if (s < 1.) return (1. - g_sig(d, s)) * (d.zeta0 + s * (d.zeta1 - d.zeta0));
+  return 0.;
+}
+// return_sigmatheta; sets *bad when p is outside [ptop, pbot]
+__device__ __forceinline__ double sgth_of(const MemberDesc &d, double theta, double p, double pbot, double ptop,
+                                          int *bad) {
+  if (p <= ptop) return theta;
+  if (p <= pbot) {
+    double s = get_sigma(pbot, p, ptop);
+    return f_sig(d, s) + g_sig(d, s) * theta;
+  }
+  *bad = 1;
+  return 0.;
+}
+
+// bisection search of an ascending or descending table (find_place_in_table)
+__device__ __forceinline__ int table_place(int n, const double *x, double xx, double *dx) {
+  int il = -1, iu = n;
+  int ascend = (x[n - 1] >= x[0]);
+  int lo = fcmp_d(xx, x[0]), hi = fcmp_d(xx, x[n - 1]);
+  if (ascend) {
+    if (lo <= 0) { *dx = x[1] - x[0]; return 0; }
+    if (hi >= 0) { *dx = x[n - 1] - x[n - 2]; return n - 2; }
+  } else {
+    if (lo >= 0) { *dx = x[1] - x[0]; return 0; }
+    if (hi <= 0) { *dx = x[n - 1] - x[n - 2]; return n - 2; }
+  }
+  while (iu - il > 1) {
+    int im = (iu + il) >> 1;
+    if ((fcmp_d(xx, x[im]) >= 0) == ascend) il = im; else iu = im;
+  }
+  *dx = x[il + 1] - x[il];
+  return il;
+}
+
+// strided variant: table entries at x[k*stride]
+__device__ __forceinline__ int table_place_s(int n, const double *x, int st, double xx, double *dx) {
+  int il = -1, iu = n;
+  int ascend = (x[(n - 1) * st] >= x[0]);
+  int lo = fcmp_d(xx, x[0]), hi = fcmp_d(xx, x[(n - 1) * st]);
+  if (ascend) {
+    if (lo <= 0) { *dx = x[st] - x[0]; return 0; }
+    if (hi >= 0) { *dx = x[(n - 1) * st] - x[(n - 2) * st]; return n - 2; }
+  } else {
+    if (lo >= 0) { *dx = x[st] - x[0]; return 0; }
+    if (hi <= 0) { *dx = x[(n - 1) * st] - x[(n - 2) * st]; return n - 2; }
+  }
+  while (iu - il > 1) {
+    int im = (iu + il) >> 1;
+    if ((fcmp_d(xx, x[im * st]) >= 0) == ascend) il = im; else iu = im;
+  }
+  *dx = x[(il + 1) * st] - x[il * st];
+  return il;
+}
+
+// field selector: which 3-D array of the descriptor
+enum Fld : int {
+  F_U, F_V, F_HDRY, F_THETA, F_P3, F_P2, F_THETA2, F_H3, F_GZ3, F_HEAT3, F_PV3, F_DIV, F_W3, F_DZDT3,
+  F_GZ, F_VARZ, F_SA, F_SB, F_SC, F_SD, F_UNITY, F_OLD, F_DUDT_T, F_DVDT_T
+};
+
+__device__ __forceinline__ double *fld_ptr(const MemberDesc &d, int f) {
+  switch (f) {
+    case F_U: return d.U;
+    case F_V: return d.V;
+    case F_HDRY: return d.HDRY;
+    case F_THETA: return d.THETA;
+    case F_P3: return d.P3;
+    case F_P2: return d.P2;
+    case F_THETA2: return d.THETA2;
+    case F_H3: return d.H3;
+    case F_GZ3: return d.GZ3;
+    case F_HEAT3: return d.HEAT3;
+    case F_PV3: return d.PV3;
+    case F_DIV: return d.DIV;
+    case F_W3: return d.W3;
+    case F_DZDT3: return d.DZDT3;
+    case F_GZ: return d.gz;
+    case F_VARZ: return d.varz;
+    case F_SA: return d.sA;
+    case F_SB: return d.sB;
+    case F_SC: return d.sC;
+    case F_SD: return d.sD;
+    case F_UNITY: return d.unity;
+    case F_OLD: return d.old;
+    case F_DUDT_T: return d.DUDT + (size_t)d.dyn->it0 * d.N3;
+    case F_DVDT_T: return d.DVDT + (size_t)d.dyn->it0 * d.N3;
+  }
+  return nullptr;
+}
+
+// launch geometry shared by host and device
+constexpr int TPB = 256;
+
+}  // namespace epic

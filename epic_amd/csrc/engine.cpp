@@ -1,0 +1,673 @@
+// engine.cpp -- host side of the MI355X EPIC engine: member memory, the
+// run-once tables, shape buckets, the step and diagnostic-sync sequences,
+// hipGraph capture of the step, and per-kernel event timing.
+//
+// Tables (filter gains, FFT twiddles, d2dy2 weights, hyperviscosity
+// coefficients) are the callee-internal statics of the reference and are
+// formed here on the host with the host libm, with the same expressions as
+// their producers:
+//   zonal_filter init       src/shared/epic_adjust.c:148-242
+//   four1/realft twiddles   src/shared/epic_funcs_util.c:4023-4135
+//   d2dy2 init              src/shared/epic_subgrid.c:1049-1140
+//   set_max_nu / coef[]     src/shared/epic_subgrid.c:82-158, 627-650
+//   divergence damping      src/shared/epic_subgrid.c:1814-1847
+//   get_kin mn_inv          src/shared/epic_funcs_diag.c:2718-2732
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/epic_amd.h"
+#include "kernels.h"
+#include "tables.h"
+
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) {                                                            \
+      snprintf(E->err, sizeof(E->err), "%s: %s", #x, hipGetErrorString(e_));           \
+      return -5;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+namespace epic {
+
+bool g_prof_on = false;
+struct ProfRec {
+  hipEvent_t a, b;
+  const char *name;
+};
+static std::vector<ProfRec> g_prof;
+static std::vector<hipEvent_t> g_evpool;
+static size_t g_evnext = 0;
+static hipEvent_t ev_get() {
+  if (g_evnext < g_evpool.size()) return g_evpool[g_evnext++];
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  g_evpool.push_back(e);
+  g_evnext++;
+  return e;
+}
+void prof_begin(hipStream_t s, const char *name) {
+  ProfRec r;
+  r.a = ev_get();
+  r.b = nullptr;
+  r.name = name;
+  (void)hipEventRecord(r.a, s);
+  g_prof.push_back(r);
+}
+void prof_end(hipStream_t s) {
+  g_prof.back().b = ev_get();
+  (void)hipEventRecord(g_prof.back().b, s);
+}
+
+}  // namespace epic
+
+using namespace epic;
+
+struct Member {
+  int active;
+  MemberDesc h;         // host copy of the descriptor (device pointers)
+  void *blob;           // one device allocation for everything
+  size_t bytes;
+  MemberDyn *dyn;       // device
+  double nu4, nudiv;
+  int adjust;
+  long steps;
+};
+
+struct epic_engine {
+  int precision, max_members;
+  size_t budget, in_use;
+  std::vector<Member> m;
+  MemberDesc *dD;       // device descriptor array (bucket order)
+  std::vector<int> order;  // bucket order -> slot
+  std::vector<Bucket> buckets;
+  bool dirty;
+  hipStream_t stream;
+  hipGraph_t graph;
+  hipGraphExec_t gexec;
+  bool have_graph;
+  char err[512];
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ------------------------------------------------------------------ engine
+extern "C" epic_engine *epic_engine_create(int precision_mode, int max_members, size_t hbm_budget) {
+  epic_engine *E = new epic_engine();
+  E->precision = precision_mode;
+  E->max_members = max_members > 0 ? max_members : 1;
+  E->budget = hbm_budget;
+  E->in_use = 0;
+  E->m.resize(E->max_members);
+  for (auto &x : E->m) x.active = 0;
+  E->dD = nullptr;
+  E->dirty = true;
+  E->have_graph = false;
+  E->err[0] = 0;
+  if (hipStreamCreate(&E->stream) != hipSuccess) {
+    delete E;
+    return nullptr;
+  }
+  if (hipMalloc((void **)&E->dD, sizeof(MemberDesc) * E->max_members) != hipSuccess) {
+    (void)hipStreamDestroy(E->stream);
+    delete E;
+    return nullptr;
+  }
+  return E;
+}
+
+extern "C" const char *epic_engine_last_error(epic_engine *E) { return E ? E->err : "no engine"; }
+
+static void drop_graph(epic_engine *E) {
+  if (E->have_graph) {
+    (void)hipGraphExecDestroy(E->gexec);
+    (void)hipGraphDestroy(E->graph);
+    E->have_graph = false;
+  }
+}
+
+extern "C" void epic_engine_destroy(epic_engine *E) {
+  if (!E) return;
+  (void)hipStreamSynchronize(E->stream);
+  drop_graph(E);
+  for (auto &x : E->m)
+    if (x.active) (void)hipFree(x.blob);
+  (void)hipFree(E->dD);
+  (void)hipStreamDestroy(E->stream);
+  delete E;
+}
+
+static int rebuild_buckets(epic_engine *E) {
+  E->order.clear();
+  for (int s = 0; s < E->max_members; s++)
+    if (E->m[s].active) E->order.push_back(s);
+  std::stable_sort(E->order.begin(), E->order.end(), [&](int a, int b) {
+    const MemberDesc &x = E->m[a].h, &y = E->m[b].h;
+    if (x.ni != y.ni) return x.ni < y.ni;
+    if (x.nj != y.nj) return x.nj < y.nj;
+    return x.nk < y.nk;
+  });
+  std::vector<MemberDesc> hd;
+  for (int s : E->order) hd.push_back(E->m[s].h);
+  if (!hd.empty())
+    HIPCHK(hipMemcpyAsync(E->dD, hd.data(), sizeof(MemberDesc) * hd.size(), hipMemcpyHostToDevice, E->stream));
+  E->buckets.clear();
+  for (size_t i = 0; i < hd.size();) {
+    size_t j = i;
+    while (j < hd.size() && hd[j].ni == hd[i].ni && hd[j].nj == hd[i].nj && hd[j].nk == hd[i].nk) j++;
+    Bucket b;
+    b.D = E->dD;
+    b.mb = (int)i;
+    b.nm = (int)(j - i);
+    b.ni = hd[i].ni;
+    b.nj = hd[i].nj;
+    b.nk = hd[i].nk;
+    E->buckets.push_back(b);
+    i = j;
+  }
+  E->dirty = false;
+  drop_graph(E);
+  return 0;
+}
+
+// store_pgrad_vars (src/shared/epic_funcs_diag.c:3595-3950)
+static void rec_pgrad_vars(hipStream_t s, const Bucket &b) {
+  L_zero_dzdt(s, b);
+  L_thermo(s, b);
+  L_gz_from_u(s, b);
+  L_gz3(s, b);
+  L_dzdt3(s, b);
+  L_bc3(s, b, F_DZDT3);
+}
+
+// store_diag (src/shared/epic_funcs_diag.c:3963-4053)
+static void rec_store_diag(hipStream_t s, const Bucket &b) {
+  L_copy_planes(s, b, F_V, F_SA, 0, b.nk);
+  L_filter(s, b, F_SA, 0, b.nk, 1, 0);
+  L_vorticity(s, b, F_U, F_SA, 0, b.nk);
+  L_copy_planes(s, b, F_U, F_SB, 0, b.nk);
+  L_filter(s, b, F_SB, 0, b.nk, 0, 0);
+  L_divergence(s, b, F_SB, F_V, F_DIV, 0, b.nk);
+}
+
+static void rec_sync(hipStream_t s, const Bucket &b, bool adjust) {
+  L_set_p2etc(s, b, 1);
+  rec_pgrad_vars(s, b);
+  rec_store_diag(s, b);
+  L_heating(s, b, adjust);
+  L_calc_w(s, b, 0);
+}
+
+// timestep(STEP_PROGS_AND_SYNC_DIAGS) in the order of
+// src/shared/epic_timestep.c:104-327
+static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool adjust) {
+  L_zero_tend(s, b);
+  L_calc_w(s, b, 1);  // source_sink + calc_w
+  L_uv_core(s, b);
+  L_uv_drag(s, b);
+  if (hv) {
+    L_merid_hv(s, b, F_HDRY, 2);
+    L_zonal_hv(s, b, F_HDRY, 2);
+  }
+  L_restore_mass(s, b);
+  if (hv) {
+    L_merid_hv(s, b, F_THETA, 3);
+    L_zonal_hv(s, b, F_THETA, 3);
+  }
+  L_advection(s, b);
+  rec_pgrad_vars(s, b);
+  // filtered, monotone copy of GZ3
+  L_copy_planes(s, b, F_GZ3, F_GZ, 0, b.nk + 1);
+  L_filter(s, b, F_GZ, 0, b.nk - 1, 0, 0);
+  L_gz_mono(s, b);
+  L_filter3(s, b, F_DUDT_T, 0);
+  L_filter3(s, b, F_DVDT_T, 1);
+  L_filter3(s, b, F_HDRY, 2);
+  L_restore_mass(s, b);
+  L_filter3(s, b, F_THETA, 3);
+  L_set_p2etc(s, b, 1);
+  rec_pgrad_vars(s, b);
+  L_pgrad(s, b);
+  if (divd) L_divdamp(s, b);
+  L_ab(s, b);
+  if (hv) {
+    L_merid_hv(s, b, F_U, 0);
+    L_merid_hv(s, b, F_V, 1);
+    L_zonal_hv(s, b, F_U, 0);
+    L_zonal_hv(s, b, F_V, 1);
+  }
+  rec_store_diag(s, b);
+  L_heating(s, b, adjust);
+  L_rotate(s, b);
+}
+
+static void bucket_flags(epic_engine *E, const Bucket &b, bool &hv, bool &divd, bool &adj) {
+  hv = divd = adj = false;
+  for (int i = 0; i < b.nm; i++) {
+    const Member &x = E->m[E->order[b.mb + i]];
+    hv |= x.nu4 > 0.;
+    divd |= x.nudiv != 0.;
+    adj |= (x.adjust != 0);
+  }
+}
+
+static void record_all(epic_engine *E, bool sync_only, const std::vector<int> *only) {
+  for (const Bucket &b : E->buckets) {
+    bool hv, divd, adj;
+    bucket_flags(E, b, hv, divd, adj);
+    if (only) {
+      // run member by member for a subset (no graph)
+      for (int i = 0; i < b.nm; i++) {
+        int slot = E->order[b.mb + i];
+        if (std::find(only->begin(), only->end(), slot) == only->end()) continue;
+        Bucket one = b;
+        one.mb = b.mb + i;
+        one.nm = 1;
+        if (sync_only) rec_sync(E->stream, one, adj);
+        else rec_step(E->stream, one, hv, divd, adj);
+      }
+    } else {
+      if (sync_only) rec_sync(E->stream, b, adj);
+      else rec_step(E->stream, b, hv, divd, adj);
+    }
+  }
+}
+
+extern "C" size_t epic_engine_bytes_in_use(epic_engine *E) { return E ? E->in_use : 0; }
+
+// Upload a case into a free slot.  hdry_src: the prognostic h of the caller
+// (nullptr: reconstitute it from p3 as epic_main does).  prime: run the
+// SYNC_DIAGS_ONLY sequence for the new member.
+int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src, bool prime) {
+  if (!E || !c) return -1;
+  if (c->ni > 1024 || c->ni < 8 || (c->ni & (c->ni - 1)) || c->nk > 64 || c->nk < 4) {
+    snprintf(E->err, sizeof(E->err), "unsupported grid %dx%dx%d", c->ni, c->nj, c->nk);
+    return -1;
+  }
+  if (strcmp(c->planet_type, "gas-giant") == 0 && (c->jtp < 0 || c->jtp > c->nj)) {
+    snprintf(E->err, sizeof(E->err), "tie-point row jtp=%d outside 0..%d", c->jtp, c->nj);
+    return -1;
+  }
+  if (c->nu[6] != 0. || c->nu[8] != 0.) {
+    snprintf(E->err, sizeof(E->err), "only del^4 hyperviscosity is implemented");
+    return -1;
+  }
+  if (E->precision != EPIC_FP64) {
+    snprintf(E->err, sizeof(E->err), "precision mode %d not available", E->precision);
+    return -1;
+  }
+  int slot = -1;
+  for (int s = 0; s < E->max_members; s++)
+    if (!E->m[s].active) { slot = s; break; }
+  if (slot < 0) return -3;
+  epic_tables T;
+  epic_tables_build(c, &T);
+  struct TablesGuard { epic_tables *t; ~TablesGuard() { epic_tables_free(t); } } tg{&T};
+  const size_t N3 = (size_t)c->Nelem3d, N2 = (size_t)c->Nelem2d;
+  const int nJ = 2 * (c->nj + 1) + 1, nK = 2 * (c->nk + 1) + 1;
+  // layout of the member blob
+  struct Part { void **dst; size_t bytes; const void *src; };
+  Member &M = E->m[slot];
+  memset(&M.h, 0, sizeof(M.h));
+  MemberDesc &d = M.h;
+  std::vector<Part> parts;
+  double *dummy = nullptr;
+  (void)dummy;
+  auto add = [&](void *pp, size_t bytes, const void *src) { parts.push_back(Part{(void **)pp, bytes, src}); };
+  const double *nul = nullptr;
+  add(&d.lat, sizeof(double) * nJ, c->lat);
+  add(&d.lon, sizeof(double) * (2 * (c->ni + 1) + 1), c->lon);
+  add(&d.m, sizeof(double) * nJ, c->m);
+  add(&d.n, sizeof(double) * nJ, c->n);
+  add(&d.mn, sizeof(double) * nJ, c->mn);
+  add(&d.f, sizeof(double) * nJ, c->f);
+  add(&d.g, sizeof(double) * nJ, c->g);
+  add(&d.rln, sizeof(double) * nJ, c->rln);
+  add(&d.rlt, sizeof(double) * nJ, c->rlt);
+  add(&d.sgth, sizeof(double) * nK, c->sigmatheta);
+  add(&d.dsgth, sizeof(double) * nK, c->dsgth);
+  add(&d.dsgth_inv, sizeof(double) * nK, c->dsgth_inv);
+  add(&d.h_min, sizeof(double) * (c->nk + 2), c->h_min);
+  add(&d.t_sponge, sizeof(double) * (c->k_sponge + 1), c->t_sponge_inv);
+  add(&d.mn_inv, sizeof(double) * nJ, T.mn_inv);
+  add(&d.hl_h, sizeof(double) * T.n_hl, T.hl_h);
+  add(&d.hl_pv, sizeof(double) * T.n_hl, T.hl_pv);
+  add(&d.hcx, sizeof(double) * (c->nj + 2), T.hc[0]);
+  add(&d.hcy, sizeof(double) * (c->nj + 2), T.hc[1]);
+  add(&d.hcz, sizeof(double) * (c->nj + 2), T.hc[2]);
+  add(&d.vcx, sizeof(double) * (c->nj + 2), T.vc[0]);
+  add(&d.vcy, sizeof(double) * (c->nj + 2), T.vc[1]);
+  add(&d.vcz, sizeof(double) * (c->nj + 2), T.vc[2]);
+  add(&d.tw_f, sizeof(double) * T.n_tw, T.twf);
+  add(&d.tw_i, sizeof(double) * T.n_tw, T.twi);
+  add(&d.tr_f, sizeof(double) * T.n_tr, T.trf);
+  add(&d.tr_i, sizeof(double) * T.n_tr, T.tri);
+  add(&d.rows_h, sizeof(int) * (c->nj + 2), T.rows_h);
+  add(&d.rows_v, sizeof(int) * (c->nj + 2), T.rows_v);
+  add(&d.U, sizeof(double) * N3, c->u);
+  add(&d.V, sizeof(double) * N3, c->v);
+  add(&d.THETA, sizeof(double) * N3, c->theta);
+  add(&d.P3, sizeof(double) * N3, c->p3);
+  add(&d.USPIN, sizeof(double) * N3, c->u_spinup);
+  add(&d.GZS, sizeof(double) * N2, c->gz_surface);
+  double **fields[] = {&d.HDRY, &d.P2, &d.THETA2, &d.H3, &d.T2, &d.T3, &d.RHO2, &d.RHO3, &d.EXNER3, &d.GZ3,
+                       &d.MONT3, &d.HEAT3, &d.PV3, &d.DIV, &d.W3, &d.DZDT3, &d.gz, &d.varz, &d.sA, &d.sB,
+                       &d.sC, &d.sD, &d.unity, &d.old};
+  for (auto p : fields) add(p, sizeof(double) * N3, nul);
+  add(&d.DUDT, sizeof(double) * 3 * N3, nul);
+  add(&d.DVDT, sizeof(double) * 3 * N3, nul);
+  add(&d.spl, sizeof(double) * (size_t)(c->nj + 2) * 3 * (c->nk + 2), nul);
+  add(&d.wstale, sizeof(double) * 4, nul);
+  add(&d.dyn, sizeof(MemberDyn), nul);
+  size_t total = 0;
+  for (auto &p : parts) total += align256(p.bytes);
+  if (E->budget && E->in_use + total > E->budget) {
+    snprintf(E->err, sizeof(E->err), "HBM budget exceeded (%zu + %zu > %zu)", E->in_use, total, E->budget);
+    return -2;
+  }
+  void *blob = nullptr;
+  if (hipMalloc(&blob, total) != hipSuccess) {
+    snprintf(E->err, sizeof(E->err), "hipMalloc(%zu) failed", total);
+    return -2;
+  }
+  HIPCHK(hipMemsetAsync(blob, 0, total, E->stream));
+  size_t off = 0;
+  for (auto &p : parts) {
+    *p.dst = (char *)blob + off;
+    if (p.src) HIPCHK(hipMemcpyAsync(*p.dst, p.src, p.bytes, hipMemcpyHostToDevice, E->stream));
+    off += align256(p.bytes);
+  }
+  // HDRY reconstituted from P3 as epic_main does (src/shared/epic_main.c:290-297),
+  // and the FLOAT_MAX Adams-Bashforth start-up flags in planes 1 and 2
+  std::vector<double> h(N3, 0.), t1(N3, 0.);
+  const int Ia = c->Iadim, N2i = c->Nelem2d;
+  auto IX = [&](int K, int J, int I) { return (size_t)I + (size_t)(J + 1) * Ia + (size_t)K * N2i; };
+  if (hdry_src) {
+    memcpy(h.data(), hdry_src, sizeof(double) * N3);
+  } else {
+    for (int K = 1; K <= c->nk; K++)
+      for (int J = 0; J <= c->nj; J++)
+        for (int I = 1; I <= c->ni; I++)
+          h[IX(K, J, I)] = ((double)c->p3[IX(K, J, I)] - (double)c->p3[IX(K - 1, J, I)]) /
+                           ((double)c->g[2 * J + 1] * (double)c->dsgth[2 * K]);
+    for (int K = 0; K <= c->nk + 1; K++)
+      for (int J = 0; J <= c->nj + 1; J++) {
+        h[IX(K, J, 0)] = h[IX(K, J, c->ni)];
+        h[IX(K, J, c->ni + 1)] = h[IX(K, J, 1)];
+      }
+  }
+  for (int K = 1; K <= c->nk; K++)
+    for (int J = 0; J <= c->nj; J++)
+      for (int I = 1; I <= c->ni; I++) t1[IX(K, J, I)] = 1.7976931348623157e308;
+  HIPCHK(hipMemcpyAsync(d.HDRY, h.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
+  for (int p = 1; p <= 2; p++) {
+    HIPCHK(hipMemcpyAsync(d.DUDT + p * N3, t1.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
+    HIPCHK(hipMemcpyAsync(d.DVDT + p * N3, t1.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
+  }
+  MemberDyn dy;
+  memset(&dy, 0, sizeof(dy));
+  dy.it0 = 0; dy.itm1 = 1; dy.itm2 = 2; dy.itime = 0;
+  HIPCHK(hipMemcpyAsync(d.dyn, &dy, sizeof(dy), hipMemcpyHostToDevice, E->stream));
+  // scalars
+  d.ni = c->ni; d.nj = c->nj; d.nk = c->nk; d.Ia = c->Iadim; d.N2 = c->Nelem2d; d.N3 = c->Nelem3d;
+  d.ks = c->k_sigma; d.k_sponge = c->k_sponge;
+  d.is_hs = (strcmp(c->planet_name, "held_suarez") == 0);
+  d.is_gas = (strcmp(c->planet_type, "gas-giant") == 0);
+  d.newt_cool_on = c->newt_cool_on; d.newt_cool_adjust = c->newt_cool_adjust;
+  d.drag_v = c->drag_v; d.drag_zonal_avg = c->drag_zonal_avg; d.jtp = c->jtp;
+  d.nrow_h = T.nrow_h; d.nrow_v = T.nrow_v;
+  int lg = 0;
+  while ((1 << lg) < c->ni) lg++;
+  d.nlogn = lg;
+  d.mixed = 0;
+  d.dt = (double)c->dt; d.kappa = c->kappa; d.cp = c->cp; d.cpr = c->cpr; d.rgas = c->rgas;
+  d.press0 = c->press0; d.zeta0 = c->zeta0; d.zeta1 = c->zeta1; d.alpha = c->hybrid_alpha;
+  d.sigma_sigma = c->sigma_sigma; d.topo_scale = c->topo_scale; d.prandtl = c->prandtl; d.tau_drag = c->tau_drag;
+  d.coef2 = T.coef2; d.coef4 = T.coef4; d.nu4 = c->nu[4]; d.m0z = T.m0z; d.nudiv = T.nudiv; d.eps13 = T.eps13;
+  d.mu3 = T.mu3; d.mu2 = T.mu2; d.g_coeff = T.gcoef;
+  M.blob = blob;
+  M.bytes = total;
+  M.dyn = d.dyn;
+  M.nu4 = c->nu[4];
+  M.nudiv = T.nudiv;
+  M.adjust = (c->newt_cool_adjust || c->k_sponge >= 1) && c->newt_cool_on;
+  M.steps = 0;
+  M.active = 1;
+  E->in_use += total;
+  int rc = rebuild_buckets(E);
+  if (rc) return rc;
+  if (!prime) {
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return slot;
+  }
+  // prime the diagnostics of this member alone
+  std::vector<int> only{slot};
+  record_all(E, true, &only);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(E->stream));
+  MemberDyn chk;
+  HIPCHK(hipMemcpy(&chk, d.dyn, sizeof(chk), hipMemcpyDeviceToHost));
+  if (chk.err) {
+    snprintf(E->err, sizeof(E->err), "fatal condition %d while priming member %d", chk.err, slot);
+    return -4;
+  }
+  return slot;
+}
+
+extern "C" int epic_engine_add(epic_engine *E, const epic_case *c) { return engine_add_member(E, c, nullptr, true); }
+
+// SYNC_DIAGS_ONLY for one member (the drop-in's priming call)
+int engine_sync_member(epic_engine *E, int slot) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return -1;
+  if (E->dirty && rebuild_buckets(E)) return -5;
+  std::vector<int> only{slot};
+  record_all(E, true, &only);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+const MemberDesc *engine_member_desc(epic_engine *E, int slot) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return nullptr;
+  return &E->m[slot].h;
+}
+
+hipStream_t engine_stream(epic_engine *E) { return E->stream; }
+
+char *engine_errbuf(epic_engine *E) { return E->err; }
+
+extern "C" int epic_engine_remove(epic_engine *E, int slot) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return -1;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  HIPCHK(hipFree(E->m[slot].blob));
+  E->in_use -= E->m[slot].bytes;
+  E->m[slot].active = 0;
+  E->dirty = true;
+  return 0;
+}
+
+int engine_check_errors(epic_engine *E);
+static int check_errors(epic_engine *E) { return engine_check_errors(E); }
+int engine_check_errors(epic_engine *E) {
+  for (int s : E->order) {
+    MemberDyn chk;
+    HIPCHK(hipMemcpy(&chk, E->m[s].dyn, sizeof(chk), hipMemcpyDeviceToHost));
+    if (chk.err) {
+      snprintf(E->err, sizeof(E->err), "fatal condition %d in member %d (non-monotone GZ3=1, p range=2, spline=4)",
+               chk.err, s);
+      return -4;
+    }
+  }
+  return 0;
+}
+
+extern "C" int epic_engine_step(epic_engine *E, int nsteps) {
+  if (!E) return -1;
+  if (E->dirty && rebuild_buckets(E)) return -5;
+  if (E->order.empty()) return 0;
+  const bool use_graph = !g_prof_on && getenv("EPIC_AMD_NO_GRAPH") == nullptr;
+  if (use_graph && !E->have_graph) {
+    HIPCHK(hipStreamSynchronize(E->stream));
+    HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+    record_all(E, false, nullptr);
+    HIPCHK(hipStreamEndCapture(E->stream, &E->graph));
+    HIPCHK(hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0));
+    E->have_graph = true;
+  }
+  for (int n = 0; n < nsteps; n++) {
+    if (use_graph) HIPCHK(hipGraphLaunch(E->gexec, E->stream));
+    else record_all(E, false, nullptr);
+  }
+  HIPCHK(hipGetLastError());
+  for (int s : E->order) E->m[s].steps += nsteps;
+  return 0;
+}
+
+extern "C" int epic_engine_step_members(epic_engine *E, const int *slots, int nslots, int nsteps) {
+  if (!E) return -1;
+  if (E->dirty && rebuild_buckets(E)) return -5;
+  std::vector<int> only(slots, slots + nslots);
+  for (int n = 0; n < nsteps; n++) record_all(E, false, &only);
+  HIPCHK(hipGetLastError());
+  for (int s : only) E->m[s].steps += nsteps;
+  return 0;
+}
+
+extern "C" int epic_engine_sync(epic_engine *E) {
+  if (!E) return -1;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  return check_errors(E);
+}
+
+extern "C" long epic_engine_steps_done(epic_engine *E, int slot) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return -1;
+  return E->m[slot].steps;
+}
+
+static double *field_ptr(MemberDesc &d, int field, size_t &n, int itplanes[3]) {
+  n = (size_t)d.N3;
+  switch (field) {
+    case EPIC_F_U: return d.U;
+    case EPIC_F_V: return d.V;
+    case EPIC_F_HDRY: return d.HDRY;
+    case EPIC_F_THETA: return d.THETA;
+    case EPIC_F_P3: return d.P3;
+    case EPIC_F_P2: return d.P2;
+    case EPIC_F_H3: return d.H3;
+    case EPIC_F_THETA2: return d.THETA2;
+    case EPIC_F_T2: return d.T2;
+    case EPIC_F_T3: return d.T3;
+    case EPIC_F_RHO2: return d.RHO2;
+    case EPIC_F_RHO3: return d.RHO3;
+    case EPIC_F_EXNER3: return d.EXNER3;
+    case EPIC_F_GZ3: return d.GZ3;
+    case EPIC_F_MONT3: return d.MONT3;
+    case EPIC_F_PV3: return d.PV3;
+    case EPIC_F_DIV_UV3: return d.DIV;
+    case EPIC_F_W3: return d.W3;
+    case EPIC_F_HEAT3: return d.HEAT3;
+    case EPIC_F_DZDT3: return d.DZDT3;
+    case EPIC_F_DUDT0: case EPIC_F_DUDT1: case EPIC_F_DUDT2:
+      return d.DUDT + (size_t)(field - EPIC_F_DUDT0) * d.N3;
+    case EPIC_F_DVDT0: case EPIC_F_DVDT1: case EPIC_F_DVDT2:
+      return d.DVDT + (size_t)(field - EPIC_F_DVDT0) * d.N3;
+    case EPIC_F_GZ_SURFACE: n = (size_t)d.N2; return d.GZS;
+  }
+  (void)itplanes;
+  return nullptr;
+}
+
+extern "C" int epic_engine_read(epic_engine *E, int slot, int field, double *out) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active || !out) return -1;
+  size_t n;
+  double *p = field_ptr(E->m[slot].h, field, n, nullptr);
+  if (!p) return -1;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  HIPCHK(hipMemcpy(out, p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int epic_engine_write(epic_engine *E, int slot, int field, const double *in) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active || !in) return -1;
+  if (field != EPIC_F_U && field != EPIC_F_V && field != EPIC_F_HDRY && field != EPIC_F_THETA) return -1;
+  size_t n;
+  double *p = field_ptr(E->m[slot].h, field, n, nullptr);
+  HIPCHK(hipStreamSynchronize(E->stream));
+  HIPCHK(hipMemcpy(p, in, sizeof(double) * n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// time-plane state of a member: it0, itm1, itm2, itime
+extern "C" int epic_engine_time_planes(epic_engine *E, int slot, int *out4) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return -1;
+  MemberDyn chk;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  HIPCHK(hipMemcpy(&chk, E->m[slot].dyn, sizeof(chk), hipMemcpyDeviceToHost));
+  out4[0] = chk.it0; out4[1] = chk.itm1; out4[2] = chk.itm2; out4[3] = chk.itime;
+  return 0;
+}
+extern "C" int epic_engine_set_time_planes(epic_engine *E, int slot, const int *in4) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return -1;
+  MemberDyn chk;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  HIPCHK(hipMemcpy(&chk, E->m[slot].dyn, sizeof(chk), hipMemcpyDeviceToHost));
+  chk.it0 = in4[0]; chk.itm1 = in4[1]; chk.itm2 = in4[2]; chk.itime = in4[3];
+  HIPCHK(hipMemcpy(E->m[slot].dyn, &chk, sizeof(chk), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int epic_engine_profile(epic_engine *E, int nsteps, char (*names)[48], double *ms_total, int *launches,
+                                   int cap) {
+  if (!E) return -1;
+  if (E->dirty && rebuild_buckets(E)) return -5;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  g_prof.clear();
+  g_evnext = 0;
+  g_prof_on = true;
+  for (int n = 0; n < nsteps; n++) record_all(E, false, nullptr);
+  g_prof_on = false;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  for (int s : E->order) E->m[s].steps += nsteps;
+  std::map<std::string, std::pair<double, int>> agg;
+  std::vector<std::string> ord;
+  for (auto &r : g_prof) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    auto it = agg.find(r.name);
+    if (it == agg.end()) {
+      agg[r.name] = {ms, 1};
+      ord.push_back(r.name);
+    } else {
+      it->second.first += ms;
+      it->second.second += 1;
+    }
+  }
+  int k = 0;
+  for (auto &nm : ord) {
+    if (k >= cap) break;
+    snprintf(names[k], 48, "%s", nm.c_str());
+    ms_total[k] = agg[nm].first;
+    launches[k] = agg[nm].second;
+    k++;
+  }
+  return k;
+}
+
+extern "C" int epic_engine_add_spec(epic_engine *E, const epic_case_spec *sp) {
+  epic_case c;
+  int rc = epic_case_build(sp, &c);
+  if (rc) {
+    if (E) snprintf(E->err, sizeof(E->err), "case build failed (%d)", rc);
+    epic_case_free(&c);
+    return -1;
+  }
+  rc = epic_engine_add(E, &c);
+  epic_case_free(&c);
+  return rc;
+}

@@ -1,0 +1,633 @@
+// k_dyn.hip -- momentum tendencies, time step and hyperviscosity (FP64).
+//
+//   uv_core                 src/shared/epic_timestep.c:628-762 with the
+//                           Sadourny weights of include/epic_pv_schemes.h:48-61
+//   uv_drag                 src/shared/epic_timestep.c:1303-1509
+//   uv_pgrad_finite_volume  src/shared/epic_timestep.c:909-1273
+//   divergence_damping      src/shared/epic_subgrid.c:1792-1885
+//   adams_bashforth_step    src/shared/epic_timestep.c:406-498
+//   zonal/meridional hyperviscosity, d2dy2  src/shared/epic_subgrid.c:590-1232
+//   restore_mass            src/shared/epic_adjust.c:45-96
+//
+// The reference carries the finite-volume faces west->east and
+// south->north; here every lane recomputes the face shared with its
+// neighbour with the same expression, so the result is the same bits
+// without the carried dependence.
+#include "kernels.h"
+
+namespace epic {
+
+#define MEMBER const MemberDesc &d = D[mb + blockIdx.z]
+#define FLAT2(j0, nJ, i0, nI)                          \
+  int t_ = blockIdx.x * TPB + threadIdx.x;             \
+  if (t_ >= (nJ) * (nI)) return;                       \
+  const int J = (j0) + t_ / (nI);                      \
+  const int I = (i0) + t_ % (nI);
+
+static inline dim3 grid2(int nJ, int nI, int nK, int nm) {
+  return dim3((unsigned)((nJ * nI + TPB - 1) / TPB), (unsigned)nK, (unsigned)nm);
+}
+
+__device__ __forceinline__ double *tend(const MemberDesc &d, double *base, int plane) {
+  return base + (size_t)plane * d.N3;
+}
+
+// ------------------------------------------------------------- uv_core
+// UH (sA), VH (sB), KIN (sC) on planes K = 1..nk-1 with zonal pads; VH rows
+// 0 and nj+1 are zero as in the reference's cleared buffers
+__global__ void k_uvh_kin(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 2, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const double *u = d.U + (size_t)K * d.N2, *v = d.V + (size_t)K * d.N2, *h3 = d.H3 + (size_t)K * d.N2;
+  double *uh = d.sA + (size_t)K * d.N2, *vh = d.sB + (size_t)K * d.N2, *kin = d.sC + (size_t)K * d.N2;
+  const int ni = d.ni, nj = d.nj;
+  if (J <= nj) {
+    double havg = .5 * (h3[ix2(d, J, I)] + h3[ix2(d, J, I - 1)]);
+    double x = u[ix2(d, J, I)] * havg * (1. / d.n[2 * J + 1]);
+    uh[ix2(d, J, I)] = x;
+    if (I == ni) uh[ix2(d, J, 0)] = x;
+    if (I == 1) uh[ix2(d, J, ni + 1)] = x;
+    // get_kin (src/shared/epic_funcs_diag.c:2694-2847)
+    const double *mn = d.mn, *mni = d.mn_inv;
+    int jj = 2 * J;
+    double u2 = u[ix2(d, J, I)], u4 = u[ix2(d, J, I + 1)], v2 = v[ix2(d, J, I)], v4 = v[ix2(d, J + 1, I)];
+    double kc = (u2 * u2 + u4 * u4 + mn[jj + 1] * (v2 * v2 * mni[jj] + v4 * v4 * mni[jj + 2])) * .25, k;
+    if (J == 0 || J == nj) {
+      k = kc;
+    } else {
+      u2 = .5 * (u[ix2(d, J + 1, I)] + u[ix2(d, J - 1, I)]);
+      u4 = .5 * (u[ix2(d, J + 1, I + 1)] + u[ix2(d, J - 1, I + 1)]);
+      v2 = .5 * (v[ix2(d, J, I + 1)] + v[ix2(d, J, I - 1)]);
+      v4 = .5 * (v[ix2(d, J + 1, I + 1)] + v[ix2(d, J + 1, I - 1)]);
+      double ks = (u2 * u2 + u4 * u4 + mn[jj + 1] * (v2 * v2 * mni[jj] + v4 * v4 * mni[jj + 2])) * .25;
+      k = (5. / 6.) * kc + (1. / 6.) * ks;
+    }
+    kin[ix2(d, J, I)] = k;
+    if (I == ni) kin[ix2(d, J, 0)] = k;
+    if (I == 1) kin[ix2(d, J, ni + 1)] = k;
+  }
+  double y = 0.;
+  if (J >= 1 && J <= nj) {
+    double havg = .5 * (h3[ix2(d, J, I)] + h3[ix2(d, J - 1, I)]);
+    y = v[ix2(d, J, I)] * havg * (1. / d.m[2 * J]);
+  }
+  vh[ix2(d, J, I)] = y;
+  if (I == ni) vh[ix2(d, J, 0)] = y;
+  if (I == 1) vh[ix2(d, J, ni + 1)] = y;
+}
+
+// DVDT = (DVDT + PV*UH terms + dKIN/dy) - vertical advection
+__global__ void k_dvdt_core(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(1, d.nj, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const size_t o = (size_t)K * d.N2;
+  const double *uh = d.sA + o, *vh = d.sB + o, *kin = d.sC + o, *pv = d.PV3 + o;
+  double *dv = tend(d, d.DVDT, d.dyn->it0) + o;
+#define PV(j, i) pv[ix2(d, j, i)]
+  double ga = (PV(J, I) + PV(J, I + 1) + PV(J + 1, I + 1));
+  double de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
+  double al = (PV(J - 1, I) + PV(J, I) + PV(J, I + 1));
+  double be = (PV(J, I) + PV(J, I + 1) + PV(J - 1, I + 1));
+#undef PV
+  double x = dv[ix2(d, J, I)];
+  x += ((-ga * uh[ix2(d, J, I + 1)] - de * uh[ix2(d, J, I)] - al * uh[ix2(d, J - 1, I)] -
+         be * uh[ix2(d, J - 1, I + 1)] + 0. * vh[ix2(d, J - 1, I)] - 0. * vh[ix2(d, J + 1, I)]) *
+            (1. / 12.) +
+        kin[ix2(d, J - 1, I)] - kin[ix2(d, J, I)]) *
+       d.n[2 * J];
+  int kk = 2 * K + 1;
+  double ds = d.sgth[kk - 2] - d.sgth[kk + 2];
+  x -= .5 * (d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K, J - 1, I)]) *
+       (d.V[ix3(d, K - 1, J, I)] - d.V[ix3(d, K + 1, J, I)]) / ds;
+  dv[ix2(d, J, I)] = x;
+}
+
+// DUDT = (DUDT + PV*VH terms + dKIN/dx with filtered KIN) - vertical advection
+__global__ void k_dudt_core(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const size_t o = (size_t)K * d.N2;
+  const double *uh = d.sA + o, *vh = d.sB + o, *kin = d.sC + o, *pv = d.PV3 + o;
+  double *du = tend(d, d.DUDT, d.dyn->it0) + o;
+#define PV(j, i) pv[ix2(d, j, i)]
+  double al = (PV(J, I) + PV(J + 1, I) + PV(J + 1, I + 1));
+  double be = (PV(J + 1, I - 1) + PV(J + 1, I) + PV(J, I));
+  double ga = (PV(J, I - 1) + PV(J, I) + PV(J + 1, I));
+  double de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
+#undef PV
+  double x = du[ix2(d, J, I)];
+  x += ((al * vh[ix2(d, J + 1, I)] + be * vh[ix2(d, J + 1, I - 1)] + ga * vh[ix2(d, J, I - 1)] + de * vh[ix2(d, J, I)] +
+         0. * uh[ix2(d, J, I - 1)] - 0. * uh[ix2(d, J, I + 1)]) *
+            (1. / 12.) +
+        kin[ix2(d, J, I - 1)] - kin[ix2(d, J, I)]) *
+       d.m[2 * J + 1];
+  int kk = 2 * K + 1;
+  double ds = d.sgth[kk - 2] - d.sgth[kk + 2];
+  x -= .5 * (d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K, J, I - 1)]) *
+       (d.U[ix3(d, K - 1, J, I)] - d.U[ix3(d, K + 1, J, I)]) / ds;
+  du[ix2(d, J, I)] = x;
+}
+
+void L_uv_core(hipStream_t s, const Bucket &b) {
+  const int nK = b.nk - 1;
+  LAUNCH(k_uvh_kin, grid2(b.nj + 2, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  LAUNCH(k_dvdt_core, grid2(b.nj, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  L_filter(s, b, F_SC, 1, b.nk - 1, 0, 0);
+  LAUNCH(k_dudt_core, grid2(b.nj + 1, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+
+// ------------------------------------------------------------- uv_drag
+__device__ __forceinline__ double t_rad_hs(const MemberDesc &d, int K, int J, int I) {
+  const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
+  double p = d.P3[ix3(d, K, J, I)], kt = ka, pbot = d.P3[ix3(d, d.nk, J, I)];
+  double amp = (p / pbot - 0.7) / (1. - 0.7);
+  if (amp > 0.) kt += (ks - ka) * pow(cos(d.lat[2 * J + 1] * (M_PI / 180.)), 4.) * amp;
+  return 1. / kt;
+}
+
+// top sponge, K = 0..k_sponge (K = 0 runs even when k_sponge = 0)
+__global__ void k_sponge(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = blockIdx.y;
+  if (K > d.k_sponge) return;
+  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  size_t q = ix3(d, K, J, I);
+  double ts = d.t_sponge[K];
+  du[q] -= (ts) * (d.U[q] - d.USPIN[q]);
+  if (J >= 1) dv[q] -= (ts) * (d.V[q] - 0.);
+}
+
+// Rayleigh drag (tau_drag, or prandtl with Held-Suarez t_rad): one lane per
+// (K,J) row, sequential zonal means as in the reference
+__global__ void k_rayleigh(const MemberDesc *D, int mb) {
+  MEMBER;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int nrow = (d.nk - 1 - d.k_sponge) * (d.nj + 1);
+  if (t >= nrow || nrow <= 0) return;
+  if (!((d.newt_cool_on && d.prandtl > 0.) || (d.tau_drag > 0. && d.tau_drag < 1.e+20))) return;
+  const int K = d.k_sponge + 1 + t / (d.nj + 1), J = t % (d.nj + 1);
+  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  const int ni = d.ni;
+  const double za = (double)d.drag_zonal_avg, zb = (double)(1 - d.drag_zonal_avg);
+  double uavg = 0.;
+  for (int I = 1; I <= ni; I++) uavg += d.U[ix3(d, K, J, I)];
+  uavg /= (ni);
+  for (int I = 1; I <= ni; I++) {
+    size_t q = ix3(d, K, J, I);
+    double uray = za * uavg + zb * d.U[q];
+    double nu0 = d.newt_cool_on ? d.prandtl / (.5 * (t_rad_hs(d, K, J, I) + t_rad_hs(d, K, J, I - 1))) : 1. / d.tau_drag;
+    du[q] -= nu0 * (uray - d.USPIN[q]);
+  }
+  if (d.drag_v && J >= 1) {
+    double vavg = 0.;
+    for (int I = 1; I <= ni; I++) vavg += d.V[ix3(d, K, J, I)];
+    vavg /= ni;
+    for (int I = 1; I <= ni; I++) {
+      size_t q = ix3(d, K, J, I);
+      double vray = za * vavg + zb * d.V[q];
+      double nu0 =
+          d.newt_cool_on ? d.prandtl / (.5 * (t_rad_hs(d, K, J, I) + t_rad_hs(d, K, J - 1, I))) : 1. / d.tau_drag;
+      dv[q] -= nu0 * vray;
+    }
+  }
+}
+
+// Held-Suarez boundary-layer drag, one lane per u- and v-column
+__global__ void k_hs_pbl(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  if (!d.is_hs) return;
+  const double nu00 = (1. / (24. * 60. * 60.));
+  const int nk = d.nk;
+  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  const double *P = d.P3;
+  double pb = 0.5 * (P[ix3(d, nk, J, I)] + P[ix3(d, nk, J, I - 1)]);
+  for (int K = nk - 1; K >= 1; K--) {
+    double p3 = .5 * (P[ix3(d, K, J, I)] + P[ix3(d, K, J, I - 1)]);
+    double amp = (p3 / pb - 0.7) / (1. - 0.7);
+    if (amp < 0.) break;
+    du[ix3(d, K, J, I)] -= nu00 * amp * d.U[ix3(d, K, J, I)];
+  }
+  if (J >= 1) {
+    pb = 0.5 * (P[ix3(d, nk, J, I)] + P[ix3(d, nk, J - 1, I)]);
+    for (int K = nk - 1; K >= 1; K--) {
+      double p3 = .5 * (P[ix3(d, K, J, I)] + P[ix3(d, K, J - 1, I)]);
+      double amp = (p3 / pb - 0.7) / (1. - 0.7);
+      if (amp < 0.) break;
+      dv[ix3(d, K, J, I)] -= nu00 * amp * d.V[ix3(d, K, J, I)];
+    }
+  }
+}
+
+void L_uv_drag(hipStream_t s, const Bucket &b) {
+  LAUNCH(k_sponge, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  int nrow = b.nk * (b.nj + 1);
+  LAUNCH(k_rayleigh, dim3((nrow + 63) / 64, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
+  LAUNCH(k_hs_pbl, grid2(b.nj + 1, b.ni, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+
+// ------------------------------------------------------------- pgrad
+// GZ3 rows -1 and nj+1 take the zonal mean of the adjacent row, K = 0..nk
+__global__ void k_gz_polefill(const MemberDesc *D, int mb) {
+  MEMBER;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * (d.nk + 1)) return;
+  const int K = t >> 1, north = t & 1;
+  const int Js = north ? d.nj : 0, Jd = north ? d.nj + 1 : -1;
+  double tmp = 0.;
+  for (int I = 1; I <= d.ni; I++) tmp += d.GZ3[ix3(d, K, Js, I)];
+  tmp /= d.ni;
+  for (int I = 0; I <= d.ni + 1; I++) d.GZ3[ix3(d, K, Jd, I)] = tmp;
+}
+
+// finite-volume pressure-gradient force.  Longitudinal: the east face of
+// cell I-1 is this cell's west face.  Latitudinal: the north face of row
+// J-1 is this row's south face.
+__device__ __forceinline__ double face_lon(const MemberDesc &d, int K, int J, int I, double &zbse, double &zbne,
+                                           double &ztse, double &ztne, double &zbe, double &zte, double &pbe,
+                                           double &pte) {
+  const double *G = d.GZ3, *P2 = d.P2;
+#define Z(k, j, i) G[ix3(d, k, j, i)]
+  zbse = .25 * (Z(K, J, I) + Z(K, J - 1, I) + Z(K + 1, J, I) + Z(K + 1, J - 1, I));
+  zbne = .25 * (Z(K, J, I) + Z(K, J + 1, I) + Z(K + 1, J, I) + Z(K + 1, J + 1, I));
+  ztse = .25 * (Z(K, J, I) + Z(K, J - 1, I) + Z(K - 1, J, I) + Z(K - 1, J - 1, I));
+  ztne = .25 * (Z(K, J, I) + Z(K, J + 1, I) + Z(K - 1, J, I) + Z(K - 1, J + 1, I));
+  zbe = .5 * (Z(K, J, I) + Z(K + 1, J, I));
+  zte = .5 * (Z(K, J, I) + Z(K - 1, J, I));
+#undef Z
+  pbe = P2[ix3(d, K + 1, J, I)];
+  pte = P2[ix3(d, K, J, I)];
+  double a1 = ztne + ztse, b1 = ztne * ztse, a2 = zbne + zbse, b2 = zbne * zbse;
+  double t1 = a1 * a1 - b1 - (a2 * a2 - b2), t2 = 3. * (a1 - a2);
+  return (-t1 * (pbe - pte) + t2 * (zte * pbe - zbe * pte)) / (zte - zbe);
+}
+
+__device__ __forceinline__ double face_lat(const MemberDesc &d, int K, int J, int I, double rlnn, double &zbnw,
+                                           double &zbne, double &ztnw, double &ztne, double &zbn, double &ztn,
+                                           double &pbn, double &ptn) {
+  const double *G = d.GZ3, *P2 = d.P2;
+#define Z(k, j, i) G[ix3(d, k, j, i)]
+  zbnw = .25 * (Z(K, J, I) + Z(K, J, I - 1) + Z(K + 1, J, I) + Z(K + 1, J, I - 1));
+  zbne = .25 * (Z(K, J, I) + Z(K, J, I + 1) + Z(K + 1, J, I) + Z(K + 1, J, I + 1));
+  ztnw = .25 * (Z(K, J, I) + Z(K, J, I - 1) + Z(K - 1, J, I) + Z(K - 1, J, I - 1));
+  ztne = .25 * (Z(K, J, I) + Z(K, J, I + 1) + Z(K - 1, J, I) + Z(K - 1, J, I + 1));
+  zbn = .5 * (Z(K, J, I) + Z(K + 1, J, I));
+  ztn = .5 * (Z(K, J, I) + Z(K - 1, J, I));
+#undef Z
+  pbn = P2[ix3(d, K + 1, J, I)];
+  ptn = P2[ix3(d, K, J, I)];
+  double a1 = ztne + ztnw, b1 = ztne * ztnw, a2 = zbne + zbnw, b2 = zbne * zbnw;
+  double t1 = a1 * a1 - b1 - (a2 * a2 - b2), t2 = 3. * (a1 - a2);
+  return rlnn * (-t1 * (pbn - ptn) + t2 * (ztn * pbn - zbn * ptn)) / (ztn - zbn);
+}
+
+__global__ void k_pgrad(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const double TOL = 1.e-2, DG = M_PI / 180.;
+  const double *G = d.GZ3, *P2 = d.P2;
+  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+#define Z(k, j, i) G[ix3(d, k, j, i)]
+#define PZ(k, j, i) P2[ix3(d, k, j, i)]
+  {
+    // ---- longitudinal component at u-point (J,I)
+    double rlns = d.rln[2 * J], rlnn = d.rln[2 * J + 2];
+    double zbsw, zbnw, ztsw, ztnw, zbw, ztw, pbw, ptw;
+    double west = face_lon(d, K, J, I - 1, zbsw, zbnw, ztsw, ztnw, zbw, ztw, pbw, ptw);
+    double lnw = d.lon[2 * (I - 1) + 1] * DG;
+    double zbse, zbne, ztse, ztne, zbe, zte, pbe, pte;
+    double east = face_lon(d, K, J, I, zbse, zbne, ztse, ztne, zbe, zte, pbe, pte);
+    double lne = d.lon[2 * I + 1] * DG;
+    double rho = .5 * (d.RHO3[ix3(d, K, J, I)] + d.RHO3[ix3(d, K, J, I - 1)]);
+    double top, bottom, a1, a2, b1, b2, t1, t2;
+    a1 = ztne + ztse; a2 = ztnw + ztsw;
+    t2 = 3. * (a1 - a2);
+    if (fabs(2. * (zte - ztw) / (zte + ztw - zbe - zbw)) > TOL) {
+      b1 = ztne * ztse; b2 = ztnw * ztsw;
+      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+      top = (-t1 * (ptw - pte) + t2 * (zte * ptw - ztw * pte)) / (zte - ztw);
+    } else {
+      top = .5 * (ptw + pte) * t2;
+    }
+    a1 = zbne + zbse; a2 = zbnw + zbsw;
+    t2 = 3. * (a1 - a2);
+    if (fabs(2. * (zbe - zbw) / (zte + ztw - zbe - zbw)) > TOL) {
+      b1 = zbne * zbse; b2 = zbnw * zbsw;
+      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+      bottom = (-t1 * (pbw - pbe) + t2 * (zbe * pbw - zbw * pbe)) / (zbe - zbw);
+    } else {
+      bottom = .5 * (pbw + pbe) * t2;
+    }
+    double north = 0., south = 0.;
+    double pgf = top - bottom + south - north + west - east;
+    double dzn = ztne - zbne + ztnw - zbnw, dzs = ztse - zbse + ztsw - zbsw;
+    pgf /= (lne - lnw) * rho * (rlns * (dzs + .5 * dzn) + rlnn * (dzn + .5 * dzs));
+    du[ix3(d, K, J, I)] += pgf;
+  }
+  if (J >= 1) {
+    // ---- latitudinal component at v-point (J,I)
+    double rlns = d.rln[2 * J - 1], lts = d.lat[2 * J - 1] * DG;
+    double zbsw, zbse, ztsw, ztse, zbs, zts, pbs, pts;
+    double south = face_lat(d, K, J - 1, I, rlns, zbsw, zbse, ztsw, ztse, zbs, zts, pbs, pts);
+    double rlnn = d.rln[2 * J + 1], ltn = d.lat[2 * J + 1] * DG;
+    double zbnw, zbne, ztnw, ztne, zbn, ztn, pbn, ptn;
+    double north = face_lat(d, K, J, I, rlnn, zbnw, zbne, ztnw, ztne, zbn, ztn, pbn, ptn);
+    double rho = .5 * (d.RHO3[ix3(d, K, J, I)] + d.RHO3[ix3(d, K, J - 1, I)]);
+    double rlt = d.rlt[2 * J];
+    double a1, a2, b1, b2, b3, t1, t2;
+    double zbe = .125 * (Z(K, J, I) + Z(K, J, I + 1) + Z(K, J - 1, I) + Z(K, J - 1, I + 1) + Z(K + 1, J, I) +
+                         Z(K + 1, J, I + 1) + Z(K + 1, J - 1, I) + Z(K + 1, J - 1, I + 1));
+    double zte = .125 * (Z(K - 1, J, I) + Z(K - 1, J, I + 1) + Z(K - 1, J - 1, I) + Z(K - 1, J - 1, I + 1) +
+                         Z(K, J, I) + Z(K, J, I + 1) + Z(K, J - 1, I) + Z(K, J - 1, I + 1));
+    double pbe = .25 * (PZ(K + 1, J, I) + PZ(K + 1, J, I + 1) + PZ(K + 1, J - 1, I) + PZ(K + 1, J - 1, I + 1));
+    double pte = .25 * (PZ(K, J, I) + PZ(K, J, I + 1) + PZ(K, J - 1, I) + PZ(K, J - 1, I + 1));
+    a1 = ztne + ztse; b1 = ztne * ztse; a2 = zbne + zbse; b2 = zbne * zbse;
+    t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+    t2 = 3. * (a1 - a2);
+    double east = .5 * (rlns - rlnn) * (-t1 * (pbe - pte) + t2 * (zte * pbe - zbe * pte)) / (zte - zbe);
+    double zbw = .125 * (Z(K, J, I) + Z(K, J, I - 1) + Z(K, J - 1, I) + Z(K, J - 1, I - 1) + Z(K + 1, J, I) +
+                         Z(K + 1, J, I - 1) + Z(K + 1, J - 1, I) + Z(K + 1, J - 1, I - 1));
+    double ztw = .125 * (Z(K - 1, J, I) + Z(K - 1, J, I - 1) + Z(K - 1, J - 1, I) + Z(K - 1, J - 1, I - 1) +
+                         Z(K, J, I) + Z(K, J, I - 1) + Z(K, J - 1, I) + Z(K, J - 1, I - 1));
+    double pbw = .25 * (PZ(K + 1, J, I) + PZ(K + 1, J, I - 1) + PZ(K + 1, J - 1, I) + PZ(K + 1, J - 1, I - 1));
+    double ptw = .25 * (PZ(K, J, I) + PZ(K, J, I - 1) + PZ(K, J - 1, I) + PZ(K, J - 1, I - 1));
+    a1 = ztnw + ztsw; b1 = ztnw * ztsw; a2 = zbnw + zbsw; b2 = zbnw * zbsw;
+    t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+    t2 = 3. * (a1 - a2);
+    double west = .5 * (rlns - rlnn) * (-t1 * (pbw - ptw) + t2 * (ztw * pbw - zbw * ptw)) / (ztw - zbw);
+    double top, bottom;
+    a1 = ztne + ztnw; a2 = ztse + ztsw;
+    t2 = 3. * (a1 - a2);
+    if (fabs(2. * (ztn - zts) / (ztn + zts - zbn - zbs)) > TOL) {
+      b1 = ztne * ztnw; b2 = ztse * ztsw; b3 = ztne * ztse + ztnw * ztsw;
+      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+      top = .5 * ((ptn - pts) * ((rlnn + rlns) * t1 + (rlnn - rlns) * (b1 + b2 - b3)) +
+                  (rlnn + rlns) * t2 * (ztn * pts - zts * ptn)) / (ztn - zts);
+    } else {
+      top = .25 * (rlnn + rlns) * (pts + ptn) * t2;
+    }
+    a1 = zbne + zbnw; a2 = zbse + zbsw;
+    t2 = 3. * (a1 - a2);
+    if (fabs(2. * (zbn - zbs) / (ztn + zts - zbn - zbs)) > TOL) {
+      b1 = zbne * zbnw; b2 = zbse * zbsw; b3 = zbne * zbse + zbnw * zbsw;
+      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+      bottom = .5 * ((pbn - pbs) * ((rlnn + rlns) * t1 + (rlnn - rlns) * (b1 + b2 - b3)) +
+                     (rlnn + rlns) * t2 * (zbn * pbs - zbs * pbn)) / (zbn - zbs);
+    } else {
+      bottom = .25 * (rlnn + rlns) * (pbs + pbn) * t2;
+    }
+    double pgf = top - bottom + south - (north + west + east);
+    double dzn = ztne - zbne + ztnw - zbnw, dzs = ztse - zbse + ztsw - zbsw;
+    pgf /= (ltn - lts) * rho * rlt * (rlns * (dzs + .5 * dzn) + rlnn * (dzn + .5 * dzs));
+    dv[ix3(d, K, J, I)] += pgf;
+  }
+#undef Z
+#undef PZ
+}
+
+void L_pgrad(hipStream_t s, const Bucket &b) {
+  LAUNCH(k_gz_polefill, dim3((2 * (b.nk + 1) + 63) / 64, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
+  LAUNCH(k_pgrad, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+
+// ------------------------------------------------------ divergence damping
+__global__ void k_divdamp(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  if (d.nudiv == 0.) return;
+  const double *div = d.sB + (size_t)K * d.N2;
+  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  double coef = (d.lat[2 * J + 1] > 45.) ? d.nudiv * d.m0z : d.nudiv * d.m[2 * J + 1];
+  du[ix3(d, K, J, I)] += coef * (div[ix2(d, J, I)] - div[ix2(d, J, I - 1)]);
+  if (J >= 1) {
+    double cv = d.nudiv * d.n[2 * J];
+    dv[ix3(d, K, J, I)] += cv * (div[ix2(d, J, I)] - div[ix2(d, J - 1, I)]);
+  }
+}
+void L_divdamp(hipStream_t s, const Bucket &b) {
+  L_copy_planes(s, b, F_U, F_SA, 1, b.nk - 1);
+  L_filter(s, b, F_SA, 1, b.nk - 1, 0, 0);
+  L_divergence(s, b, F_SA, F_V, F_SB, 1, b.nk - 1);
+  LAUNCH(k_divdamp, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+
+// ------------------------------------------------------ Adams-Bashforth
+// Coefficients from the FLAT_MAX start-up flags of the older planes,
+// tested per wind at (KLO, jlo, ILO) as the reference does.
+__global__ void k_ab(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const MemberDyn dy = *d.dyn;
+  const double FMAX = 1.7976931348623157e308;
+  for (int w = 0; w < 2; w++) {
+    const int jlo = w;  // u: JLO = 0, v: JFIRST = 1
+    if (J < jlo) continue;
+    double *val = w ? d.V : d.U;
+    double *tb = w ? d.DVDT : d.DUDT;
+    double ab0, ab1, ab2;
+    if (tb[(size_t)dy.itm2 * d.N3 + ix3(d, 1, jlo, 1)] == FMAX) {
+      if (tb[(size_t)dy.itm1 * d.N3 + ix3(d, 1, jlo, 1)] == FMAX) { ab0 = 1.; ab1 = 0.; ab2 = 0.; }
+      else { ab0 = 3. / 2.; ab1 = -1. / 2.; ab2 = 0.; }
+    } else {
+      ab0 = 23. / 12.; ab1 = -16. / 12.; ab2 = 5. / 12.;
+    }
+    size_t q = ix3(d, K, J, I);
+    val[q] += d.dt * (ab0 * tb[(size_t)dy.it0 * d.N3 + q] + ab1 * tb[(size_t)dy.itm1 * d.N3 + q] +
+                      ab2 * tb[(size_t)dy.itm2 * d.N3 + q]);
+  }
+}
+// rotate IT_ZERO <- IT_MINUS2 <- IT_MINUS1 <- IT_ZERO and advance itime
+__global__ void k_rotate(const MemberDesc *D, int mb) {
+  MEMBER;
+  if (threadIdx.x) return;
+  MemberDyn *y = d.dyn;
+  int t = y->itm2;
+  y->itm2 = y->itm1;
+  y->itm1 = y->it0;
+  y->it0 = t;
+  y->itime += 1;
+}
+void L_ab(hipStream_t s, const Bucket &b) {
+  LAUNCH(k_ab, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  L_bc3(s, b, F_U);
+  L_bc3(s, b, F_V);
+}
+void L_rotate(hipStream_t s, const Bucket &b) {
+  LAUNCH(k_rotate, dim3(1, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
+}
+
+// zero the current tendency planes
+__global__ void k_zero_tend(const MemberDesc *D, int mb) {
+  MEMBER;
+  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  size_t i = (size_t)blockIdx.x * TPB + threadIdx.x, st = (size_t)gridDim.x * TPB;
+  for (; i < (size_t)d.N3; i += st) {
+    du[i] = 0.;
+    dv[i] = 0.;
+  }
+}
+void L_zero_tend(hipStream_t s, const Bucket &b) {
+  LAUNCH(k_zero_tend, dim3(256, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+
+// monotone filtered geopotential, K = nk-1 down to 0 (src/shared/epic_timestep.c:176-196)
+__global__ void k_gz_mono(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 0, d.ni + 2);
+  double *gz = d.gz;
+  double below = gz[ix3(d, d.nk, J, I)];
+  for (int K = d.nk - 1; K >= 0; K--) {
+    size_t q = ix3(d, K, J, I);
+    double v;
+    if (fcmp_d(below, 0.) == 0) v = 1.e-3;
+    else v = dmax(gz[q], below + fabs(below) * 1.e-5);
+    gz[q] = v;
+    below = v;
+  }
+}
+void L_gz_mono(hipStream_t s, const Bucket &b) {
+  LAUNCH(k_gz_mono, grid2(b.nj + 1, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+
+__global__ void k_restore_mass(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 0, d.ni + 2);
+  const int K = 1 + blockIdx.y;
+  size_t q = ix3(d, K, J, I);
+  d.HDRY[q] = dmax(d.h_min[K], d.HDRY[q]);
+}
+void L_restore_mass(hipStream_t s, const Bucket &b) {
+  LAUNCH(k_restore_mass, grid2(b.nj + 1, b.ni + 2, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+
+// ------------------------------------------------------ hyperviscosity
+struct HvRange { int k0, k1, jbot, jh; };
+__device__ __forceinline__ HvRange hv_range(const MemberDesc &d, int kind) {
+  HvRange r;
+  r.k0 = (kind == 3) ? d.ks - 1 : 1;
+  r.k1 = (kind <= 1) ? d.nk - 1 : d.nk;
+  r.jbot = (kind == 1) ? 1 : 0;
+  r.jh = (kind == 1) ? 0 : 1;
+  return r;
+}
+
+// d2dy2 with coefficient dc from src -> dst on planes of the kind's range
+__global__ void k_d2dy2(const MemberDesc *D, int mb, int src, int dst, int kind, int pass) {
+  MEMBER;
+  const int vg = (kind == 1);
+  const int jmax = vg ? d.nj + 1 : d.nj;
+  FLAT2(0, jmax + 1, 1, d.ni);
+  HvRange r = hv_range(d, kind);
+  const int K = r.k0 + blockIdx.y;
+  if (K > r.k1) return;
+  const double *hh = fld_ptr(d, src) + (size_t)K * d.N2;
+  double *out = fld_ptr(d, dst) + (size_t)K * d.N2;
+  const double dc = pass ? d.coef4 : d.coef2;
+  const double *cx = vg ? d.vcx : d.hcx, *cy = vg ? d.vcy : d.hcy, *cz = vg ? d.vcz : d.hcz;
+  int j0, j1, j2;
+  if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
+  else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
+  else { j0 = J - 1; j1 = J; j2 = J + 1; }
+  double a0 = dc * cx[J], a1 = dc * cy[J], a2 = dc * cz[J];
+  out[ix2(d, J, I)] = a0 * hh[ix2(d, j0, I)] + a1 * hh[ix2(d, j1, I)] + a2 * hh[ix2(d, j2, I)];
+}
+
+// A += DT*(-1)*D2 on the kind's rows, then zonal pads of every row 0..nj+1
+__global__ void k_hv_apply(const MemberDesc *D, int mb, int f, int src, int kind) {
+  MEMBER;
+  FLAT2(0, d.nj + 2, 1, d.ni);
+  HvRange r = hv_range(d, kind);
+  const int K = r.k0 + blockIdx.y;
+  if (K > r.k1 || d.nu4 == 0.) return;
+  double *a = fld_ptr(d, f) + (size_t)K * d.N2;
+  const double *dd = fld_ptr(d, src) + (size_t)K * d.N2;
+  double v = a[ix2(d, J, I)];
+  if (J >= r.jbot && J <= d.nj) {
+    v += d.dt * (double)(-1) * dd[ix2(d, J, I)];
+    a[ix2(d, J, I)] = v;
+  }
+  if (I == d.ni) a[ix2(d, J, 0)] = v;
+  if (I == 1) a[ix2(d, J, d.ni + 1)] = v;
+}
+
+void L_merid_hv(hipStream_t s, const Bucket &b, int f, int kind) {
+  const int jrows = (kind == 1) ? b.nj + 2 : b.nj + 1;
+  dim3 g = grid2(jrows, b.ni, b.nk, b.nm);
+  LAUNCH(k_d2dy2, g, dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SA, kind, 0);
+  LAUNCH(k_d2dy2, g, dim3(TPB), 0, s, b.D, b.mb, (int)F_SA, (int)F_SB, kind, 1);
+  // filter D2 on the kind's planes (planes outside the range are filtered
+  // too, harmlessly: they are scratch)
+  L_filter(s, b, F_SB, 1, b.nk, kind == 1, 0);
+  LAUNCH(k_hv_apply, grid2(b.nj + 2, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SB, kind);
+}
+
+// zonal hyperviscosity: one 64-lane wavefront per (K,J) row held in LDS;
+// the zonal means are sequential sums in lane 0, as the reference sums them
+__global__ void k_zonal_hv(const MemberDesc *D, int mb, int f, int kind) {
+  MEMBER;
+  extern __shared__ double lds[];
+  const int ni = d.ni, W = ni + 2;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rpb = blockDim.x >> 6;
+  HvRange r = hv_range(d, kind);
+  const int K = r.k0 + blockIdx.y;
+  const int J = blockIdx.x * rpb + wv;
+  if (K > r.k1 || d.nu4 == 0.) return;  // uniform per block
+  const bool live = (J <= d.nj + 1);
+  double *hh = lds + (size_t)wv * 2 * W, *dd = hh + W;
+  __shared__ double meanv[16];
+  double *a = fld_ptr(d, f) + (size_t)K * d.N2;
+  const bool upd = live && J >= r.jbot && J <= d.nj;
+  if (live)
+    for (int I = lane; I < W; I += 64) hh[I] = a[ix2(d, J, I)];
+  __syncthreads();
+  double dc[2] = {0., 0.};
+  if (upd) {
+    double m = d.m[2 * J + r.jh];
+    bool taper = fabs(d.lat[2 * J + r.jh]) > 45.;
+    dc[0] = taper ? d.coef2 * d.m0z * d.m0z : d.coef2 * m * m;
+    dc[1] = taper ? d.coef4 * d.m0z * d.m0z : d.coef4 * m * m;
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    double *src = pass ? dd : hh, *dst = pass ? hh : dd;
+    if (upd)
+      for (int I = 1 + lane; I <= ni; I += 64) dst[I] = dc[pass] * (src[I - 1] - 2. * src[I] + src[I + 1]);
+    __syncthreads();
+    if (upd && lane == 0) {
+      dst[ni + 1] = dst[1];
+      dst[0] = dst[ni];
+      double fm = 0.;
+      for (int I = 1; I <= ni; I++) fm += dst[I];
+      fm /= ni;
+      meanv[wv] = fm;
+    }
+    __syncthreads();
+    if (upd)
+      for (int I = lane; I < W; I += 64) dst[I] -= meanv[wv];
+    __syncthreads();
+  }
+  // pass 2 result is in hh; apply with sign -1, then pads
+  if (live) {
+    for (int I = 1 + lane; I <= ni; I += 64) {
+      double v = a[ix2(d, J, I)];
+      if (upd) v += d.dt * (double)(-1) * hh[I];
+      a[ix2(d, J, I)] = v;
+      if (I == ni) a[ix2(d, J, 0)] = v;
+      if (I == 1) a[ix2(d, J, ni + 1)] = v;
+    }
+  }
+}
+void L_zonal_hv(hipStream_t s, const Bucket &b, int f, int kind) {
+  const int rpb = b.ni >= 512 ? 2 : 4, W = b.ni + 2;
+  dim3 g((b.nj + 2 + rpb - 1) / rpb, b.nk, b.nm);
+  LAUNCH(k_zonal_hv, g, dim3(64 * rpb), sizeof(double) * 2 * W * rpb, s, b.D, b.mb, f, kind);
+}
+
+}  // namespace epic

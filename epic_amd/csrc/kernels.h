@@ -1,0 +1,70 @@
+// kernels.h -- host-side launchers of the engine's HIP kernels.
+// Every launcher covers the members [mb, mb+nm) of one bucket (same ni,nj,nk)
+// on stream s.  Field selectors name a 3-D array of the MemberDesc.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dev_common.h"
+
+namespace epic {
+
+// Optional per-launch timing: when enabled, every launch is bracketed by HIP
+// events on its stream and the durations are aggregated per kernel name.
+void prof_begin(hipStream_t s, const char *name);
+void prof_end(hipStream_t s);
+extern bool g_prof_on;
+#define LAUNCH(k, g, b, shm, s, ...)                     \
+  do {                                                   \
+    if (::epic::g_prof_on) ::epic::prof_begin(s, #k);    \
+    hipLaunchKernelGGL(k, g, b, shm, s, __VA_ARGS__);    \
+    if (::epic::g_prof_on) ::epic::prof_end(s);          \
+  } while (0)
+
+struct Bucket {
+  const MemberDesc *D;  // device array of descriptors
+  int mb, nm;           // first member, member count
+  int ni, nj, nk;
+};
+
+
+
+// ---- diagnostics (k_diag.hip)
+void L_fill(hipStream_t s, const Bucket &b, int fld, double v);
+void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta);
+void L_thermo(hipStream_t s, const Bucket &b);
+void L_gz_from_u(hipStream_t s, const Bucket &b);
+void L_gz3(hipStream_t s, const Bucket &b);
+void L_dzdt3(hipStream_t s, const Bucket &b);
+void L_bc3(hipStream_t s, const Bucket &b, int fld);
+void L_heating(hipStream_t s, const Bucket &b, int adjust);
+void L_calc_w(hipStream_t s, const Bucket &b, int with_source);
+void L_copy_planes(hipStream_t s, const Bucket &b, int src, int dst, int k0, int k1);
+void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1);
+void L_divergence(hipStream_t s, const Bucket &b, int uu, int vv, int dst, int k0, int k1);
+void L_zero_dzdt(hipStream_t s, const Bucket &b);
+
+// ---- zonal filter (k_filter.hip)
+// 2-D filter of planes k0..k1 of fld; vgrid selects the v/pv rows; mode 0
+// also refreshes the zonal pads of every row, mode 1 leaves pads alone.
+void L_filter(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode);
+// the 3-D filter with the PCHIP profile removal (kind: 0 U, 1 V, 2 HDRY, 3 THETA)
+void L_filter3(hipStream_t s, const Bucket &b, int fld, int kind);
+
+// ---- momentum (k_dyn.hip)
+void L_uv_core(hipStream_t s, const Bucket &b);
+void L_uv_drag(hipStream_t s, const Bucket &b);
+void L_pgrad(hipStream_t s, const Bucket &b);
+void L_divdamp(hipStream_t s, const Bucket &b);
+void L_ab(hipStream_t s, const Bucket &b);
+void L_rotate(hipStream_t s, const Bucket &b);
+void L_zero_tend(hipStream_t s, const Bucket &b);
+void L_gz_mono(hipStream_t s, const Bucket &b);
+void L_restore_mass(hipStream_t s, const Bucket &b);
+// hyperviscosity: kind 0 U, 1 V, 2 HDRY, 3 THETA
+void L_merid_hv(hipStream_t s, const Bucket &b, int fld, int kind);
+void L_zonal_hv(hipStream_t s, const Bucket &b, int fld, int kind);
+
+// ---- advection (k_adv.hip)
+void L_advection(hipStream_t s, const Bucket &b);
+
+}  // namespace epic

@@ -259,7 +259,13 @@ void         epic_engine_destroy(epic_engine *e);
 /* Upload a case; primes the diagnostics (the reference's SYNC_DIAGS_ONLY).
  * Returns the member slot (>= 0), or a negative code (-2: no memory). */
 int  epic_engine_add(epic_engine *e, const epic_case *c);
+/* epic_case_build() + epic_engine_add() + epic_case_free() in one call. */
+int  epic_engine_add_spec(epic_engine *e, const epic_case_spec *spec);
 int  epic_engine_remove(epic_engine *e, int slot);
+/* Adams-Bashforth time planes of a member: {IT_ZERO, IT_MINUS1, IT_MINUS2,
+ * itime of the next step} (include/epic.h:303-304, epic_timestep.c:329-345). */
+int  epic_engine_time_planes(epic_engine *e, int slot, int *out4);
+int  epic_engine_set_time_planes(epic_engine *e, int slot, const int *in4);
 /* Advance every active member by nsteps (STEP_PROGS_AND_SYNC_DIAGS each). */
 int  epic_engine_step(epic_engine *e, int nsteps);
 /* Advance only the listed members. */

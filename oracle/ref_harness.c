@@ -33,6 +33,87 @@
 
 #define MAGIC 0x45504943
 
+/* The restated layouts of include/epic_amd.h must equal the reference's. */
+#define SAME(R, A, m) _Static_assert(offsetof(R, m) == offsetof(struct A, m), #A "." #m)
+_Static_assert(sizeof(planetspec) == sizeof(struct epic_planet_abi), "planetspec size");
+_Static_assert(sizeof(gridspec) == sizeof(struct epic_grid_abi), "gridspec size");
+_Static_assert(sizeof(variablespec) == sizeof(struct epic_var_abi), "variablespec size");
+_Static_assert(sizeof(wind_variable) == sizeof(struct epic_wind_abi), "wind_variable size");
+_Static_assert(sizeof(thermo_variable) == sizeof(struct epic_thermo_abi), "thermo_variable size");
+_Static_assert(sizeof(diagnostic_variable) == sizeof(struct epic_diag_abi), "diagnostic_variable size");
+SAME(planetspec, epic_planet_abi, re);
+SAME(planetspec, epic_planet_abi, omega_sidereal);
+SAME(planetspec, epic_planet_abi, cp);
+SAME(planetspec, epic_planet_abi, rgas);
+SAME(planetspec, epic_planet_abi, kappa);
+SAME(planetspec, epic_planet_abi, GM);
+SAME(planetspec, epic_planet_abi, J2);
+SAME(planetspec, epic_planet_abi, kinvisc);
+SAME(planetspec, epic_planet_abi, u);
+SAME(gridspec, epic_grid_abi, dt);
+SAME(gridspec, epic_grid_abi, nk);
+SAME(gridspec, epic_grid_abi, nj);
+SAME(gridspec, epic_grid_abi, ni);
+SAME(gridspec, epic_grid_abi, jtp);
+SAME(gridspec, epic_grid_abi, jlo);
+SAME(gridspec, epic_grid_abi, dln);
+SAME(gridspec, epic_grid_abi, pbot);
+SAME(gridspec, epic_grid_abi, press0);
+SAME(gridspec, epic_grid_abi, zeta0);
+SAME(gridspec, epic_grid_abi, zeta1);
+SAME(gridspec, epic_grid_abi, hybrid_alpha);
+SAME(gridspec, epic_grid_abi, sigma_sigma);
+SAME(gridspec, epic_grid_abi, k_sponge);
+SAME(gridspec, epic_grid_abi, k_sigma);
+SAME(gridspec, epic_grid_abi, newt_cool_on);
+SAME(gridspec, epic_grid_abi, newt_cool_adjust);
+SAME(gridspec, epic_grid_abi, sigmatheta);
+SAME(gridspec, epic_grid_abi, h_min);
+SAME(gridspec, epic_grid_abi, rln);
+SAME(gridspec, epic_grid_abi, lon);
+SAME(gridspec, epic_grid_abi, dsgth_inv);
+SAME(gridspec, epic_grid_abi, drag_v);
+SAME(gridspec, epic_grid_abi, drag_zonal_avg);
+SAME(gridspec, epic_grid_abi, t_sponge_inv);
+SAME(gridspec, epic_grid_abi, prandtl);
+SAME(gridspec, epic_grid_abi, tau_drag);
+SAME(gridspec, epic_grid_abi, nudiv_nondim);
+SAME(gridspec, epic_grid_abi, ab);
+SAME(gridspec, epic_grid_abi, nu);
+SAME(gridspec, epic_grid_abi, itime);
+SAME(wind_variable, epic_wind_abi, value);
+SAME(wind_variable, epic_wind_abi, tendency);
+SAME(thermo_variable, epic_thermo_abi, value);
+SAME(diagnostic_variable, epic_diag_abi, value);
+SAME(variablespec, epic_var_abi, u);
+SAME(variablespec, epic_var_abi, v);
+SAME(variablespec, epic_var_abi, hdry);
+SAME(variablespec, epic_var_abi, theta);
+SAME(variablespec, epic_var_abi, pdat);
+SAME(variablespec, epic_var_abi, hdry3);
+SAME(variablespec, epic_var_abi, pdry3);
+SAME(variablespec, epic_var_abi, p2);
+SAME(variablespec, epic_var_abi, p3);
+SAME(variablespec, epic_var_abi, theta2);
+SAME(variablespec, epic_var_abi, h2);
+SAME(variablespec, epic_var_abi, h3);
+SAME(variablespec, epic_var_abi, t2);
+SAME(variablespec, epic_var_abi, t3);
+SAME(variablespec, epic_var_abi, rho2);
+SAME(variablespec, epic_var_abi, rho3);
+SAME(variablespec, epic_var_abi, exner3);
+SAME(variablespec, epic_var_abi, gz3);
+SAME(variablespec, epic_var_abi, mont3);
+SAME(variablespec, epic_var_abi, heat3);
+SAME(variablespec, epic_var_abi, pv3);
+SAME(variablespec, epic_var_abi, ri2);
+SAME(variablespec, epic_var_abi, div_uv3);
+SAME(variablespec, epic_var_abi, w3);
+SAME(variablespec, epic_var_abi, dzdt3);
+SAME(variablespec, epic_var_abi, u_spinup);
+SAME(variablespec, epic_var_abi, gz_surface);
+SAME(variablespec, epic_var_abi, t_cool_table);
+
 static int cmp_arr(const char *name, const double *a, const double *b, int n) {
   int bad = 0;
   for (int i = 0; i < n; i++)

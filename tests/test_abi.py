@@ -1,0 +1,74 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol that
+include/epic_amd.h declares, and the restated reference structs have the
+reference's sizes and offsets (compile-time assertions in abi_check.c)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import epic_amd as ea
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "epic_amd.h")
+
+
+def declared_functions():
+    src = open(HDR).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(\w+)\s*\([^;{]*\)\s*;", src, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "return")))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(ea.LIB_PATH)
+    names = declared_functions()
+    assert "timestep" in names and "epic_engine_step" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layout_assertions_compile():
+    r = subprocess.run(["gcc", "-fsyntax-only", "-std=gnu11", os.path.join(ROOT, "epic_amd", "csrc", "abi_check.c")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_case_builder_and_footprint():
+    sp = ea.spec(ea.HELD_SUAREZ, 64, 32, 10)
+    assert ea.footprint(64, 32, 10, ea.FP64) > 30 * 8 * 66 * 35 * 12
+    assert ea.footprint(64, 32, 10, ea.MIXED) < ea.footprint(64, 32, 10, ea.FP64)
+
+
+def _needed(exe):
+    r = subprocess.run(["readelf", "-d", "--dyn-syms", "-W", exe], capture_output=True, text=True)
+    return r.stdout
+
+
+@pytest.mark.parametrize("exe", ["tests/native/_bin/dropin_host", "oracle/_ref/epic_ref_dropin"])
+def test_host_programs_bind_timestep_to_the_library(exe):
+    """An EPIC-shaped executable leaves timestep() undefined (it binds to
+    libepic_amd.so) and exports the globals the drop-in reads."""
+    path = os.path.join(ROOT, exe)
+    if not os.path.exists(path):
+        pytest.skip(f"{exe} not built")
+    out = _needed(path)
+    assert "libepic_amd.so" in out
+    syms = {}
+    for line in out.splitlines():
+        p = line.split()
+        if len(p) >= 8 and p[0].rstrip(":").isdigit():
+            syms[p[7]] = p[6]
+    assert syms.get("timestep") == "UND"
+    for g in ("grid", "var", "IT_ZERO", "IT_MINUS1", "IT_MINUS2"):
+        assert g in syms and syms[g] != "UND", g
+
+
+def test_library_leaves_the_callers_globals_weak():
+    out = _needed(ea.LIB_PATH)
+    for line in out.splitlines():
+        p = line.split()
+        if len(p) >= 8 and p[7] in ("grid", "var", "IT_ZERO", "IT_MINUS1", "IT_MINUS2", "solar_longitude",
+                                    "epic_error"):
+            assert p[4] == "WEAK" and p[6] == "UND", line

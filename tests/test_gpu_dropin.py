@@ -1,0 +1,77 @@
+"""The drop-in timestep() on the GPU, called the way an EPIC executable
+calls it: from a host program that owns the globals grid, var and IT_*.
+
+tests/native/dropin_host.c is a minimal such program; oracle/_ref/
+epic_ref_dropin is the reference's own program (its globals, metric set-up
+and solar_longitude) with the reference's timestep() renamed away so that
+the call binds to libepic_amd.so (present where the reference tree was
+available at build time)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import epic_amd as ea
+from oracle.cpuref import CpuRef
+from tests.golden.make_golden import read_dump
+from tests.gpu_util import DIAG, PROG, rel_l2, say
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "tests", "native", "_bin", "dropin_host")
+REF = os.path.join(ROOT, "oracle", "_ref", "epic_ref")
+REF_DROPIN = os.path.join(ROOT, "oracle", "_ref", "epic_ref_dropin")
+
+ARGS = (0, 32, 16, 8, 600, 21, 0.5, 0.1, 0.3, 0.3)  # Held-Suarez with divergence damping
+NSTEPS = 12
+
+
+def run(exe, extra, tmp_path, name):
+    fn = str(tmp_path / name)
+    cmd = [exe] + [str(a) for a in ARGS] + [str(NSTEPS), fn] + [str(x) for x in extra]
+    say("run", os.path.basename(exe), *cmd[1:])
+    r = subprocess.run(cmd, cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    (hdr, f), = [d for d in read_dump(fn) if int(d[0][0]) == NSTEPS]
+    return hdr, f
+
+
+def oracle_state():
+    r = CpuRef(ea.spec(*ARGS))
+    r.sync()
+    r.step(NSTEPS)
+    out = {f: r.read(f) for f in ea.FIELDS}
+    tp = r.time_planes()[:3]
+    r.close()
+    return tp, out
+
+
+def compare(f, ref, tag):
+    for q in PROG:
+        e = rel_l2(f[q], ref[q])
+        assert e <= 1e-12, (tag, q, e)
+    for q in DIAG:
+        e = rel_l2(f[q], ref[q])
+        assert e <= 1e-10, (tag, q, e)
+
+
+@pytest.mark.skipif(not os.path.exists(HOST), reason="tests/native/_bin/dropin_host not built")
+def test_dropin_eager_and_lazy_match_oracle(tmp_path):
+    h_e, f_e = run(HOST, [0], tmp_path, "eager.bin")
+    h_l, f_l = run(HOST, [1], tmp_path, "lazy.bin")
+    for q in ea.FIELDS:
+        assert np.array_equal(f_e[q], f_l[q]), q
+    tp, ref = oracle_state()
+    assert [int(x) for x in h_e[1:4]] == tp
+    compare(f_e, ref, "dropin_host")
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_DROPIN) and os.path.exists(REF)),
+                    reason="reference programs (oracle/_ref) not built")
+def test_reference_program_with_dropin_matches_reference(tmp_path):
+    h_g, f_g = run(REF_DROPIN, [NSTEPS], tmp_path, "gpu.bin")
+    h_c, f_c = run(REF, [NSTEPS], tmp_path, "cpu.bin")
+    assert [int(x) for x in h_g[1:4]] == [int(x) for x in h_c[1:4]]
+    compare(f_g, f_c, "epic_ref_dropin")

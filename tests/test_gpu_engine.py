@@ -1,0 +1,91 @@
+"""Engine behaviour on the GPU: a member's result does not depend on what
+it is batched with, members come and go, the HBM budget is enforced."""
+import numpy as np
+import pytest
+
+import epic_amd as ea
+from tests.gpu_util import PROG, say
+
+pytestmark = pytest.mark.gpu
+
+A = ea.spec(ea.HELD_SUAREZ, 64, 32, 10, 600, 1, 0.5, 0.1, 0.1, 0.3)
+B = ea.spec(ea.JUPITER_LIKE, 32, 16, 8, 60, 2, 0.5, 0.0, 0.0, 0.5)
+C = ea.spec(ea.HELD_SUAREZ, 64, 32, 10, 600, 3, 0.3, 0.0, 0.2, 0.1)
+
+
+def run_alone(sp, n):
+    e = ea.Engine(ea.FP64, 1)
+    s = e.add(sp)
+    e.step(n)
+    out = {f: e.read(s, f) for f in PROG + ("PV3", "GZ3")}
+    e.close()
+    return out
+
+
+def test_member_alone_equals_member_in_mixed_batch():
+    say("alone vs batch")
+    n = 7
+    ref = {k: run_alone(sp, n) for k, sp in (("A", A), ("B", B), ("C", C))}
+    e = ea.Engine(ea.FP64, 4)
+    slots = {"A": e.add(A), "B": e.add(B), "C": e.add(C)}
+    e.step(n)
+    for k, s in slots.items():
+        for f, a in ref[k].items():
+            assert np.array_equal(e.read(s, f), a), (k, f)
+    e.close()
+
+
+def test_join_leave_mid_run():
+    """A member that joins late evolves exactly as if it had run alone."""
+    e = ea.Engine(ea.FP64, 3)
+    sa = e.add(A)
+    e.step(3)
+    sc = e.add(C)
+    e.step(4)
+    e.remove(sa)
+    e.step(2)
+    assert e.steps_done(sc) == 6
+    ref = run_alone(C, 6)
+    for f, a in ref.items():
+        assert np.array_equal(e.read(sc, f), a), f
+    e.close()
+
+
+def test_step_members_subset():
+    e = ea.Engine(ea.FP64, 2)
+    sa, sc = e.add(A), e.add(C)
+    e.step_members([sc], 5)
+    assert e.steps_done(sa) == 0 and e.steps_done(sc) == 5
+    ref = run_alone(C, 5)
+    assert np.array_equal(e.read(sc, "THETA"), ref["THETA"])
+    e.close()
+
+
+def test_hbm_budget_is_enforced():
+    fp = ea.footprint(64, 32, 10, ea.FP64)
+    e = ea.Engine(ea.FP64, 4, hbm_budget=int(fp * 1.5))
+    e.add(A)
+    with pytest.raises(ea.EpicError):
+        e.add(C)
+    assert e.bytes_in_use() <= fp * 1.5
+    e.close()
+
+
+def test_write_then_read_round_trip():
+    e = ea.Engine(ea.FP64, 1)
+    s = e.add(A)
+    th = e.read(s, "THETA")
+    th2 = th.copy()
+    th2[5, 10, 10] += 0.25
+    e.write_field(s, "THETA", th2)
+    assert np.array_equal(e.read(s, "THETA"), th2)
+    e.close()
+
+
+def test_rejects_unsupported_shapes():
+    e = ea.Engine(ea.FP64, 1)
+    with pytest.raises(ea.EpicError):
+        e.add(ea.spec(ea.HELD_SUAREZ, 2048, 32, 10))
+    with pytest.raises(ea.EpicError):
+        e.add(ea.spec(ea.HELD_SUAREZ, 64, 32, 80))
+    e.close()
