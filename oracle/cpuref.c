@@ -21,6 +21,9 @@
 
 #include <float.h>
 #include <math.h>
+#ifdef CPUREF_LIBM_NUDGE
+#include "libm_nudge.h"
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>

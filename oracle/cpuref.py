@@ -9,15 +9,18 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libcpuref.so")
-_L = None
+LIB_NUDGE = os.path.join(HERE, "libcpuref_nudge.so")
+_L = {}
 
 
-def lib():
-    global _L
-    if _L is None:
-        if not os.path.exists(LIB):
-            raise RuntimeError(f"{LIB} not built (make -C oracle)")
-        L = ctypes.CDLL(LIB)
+def lib(nudged=False):
+    """The oracle library; nudged=True gives the copy whose exp/log/pow
+    results are moved by one ulp 10% of the time (libm_nudge.h)."""
+    if nudged not in _L:
+        path = LIB_NUDGE if nudged else LIB
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (make -C oracle)")
+        L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         L.cpuref_create.restype = vp
         L.cpuref_create.argtypes = [vp]
@@ -31,27 +34,28 @@ def lib():
         L.cpuref_pchip_eval.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_double]
         L.cpuref_pchip_eval.restype = ctypes.c_double
         L.cpuref_fcmp.argtypes = [ctypes.c_double, ctypes.c_double]
-        _L = L
-    return _L
+        _L[nudged] = L
+    return _L[nudged]
 
 
 class CpuRef:
     """One integration on the CPU oracle, same field order as epic_amd.FIELDS."""
 
-    def __init__(self, sp):
+    def __init__(self, sp, nudged=False):
         self.sp = sp
-        self._r = lib().cpuref_create(ctypes.addressof(sp))
+        self._L = lib(nudged)
+        self._r = self._L.cpuref_create(ctypes.addressof(sp))
         if not self._r:
             raise RuntimeError("cpuref_create failed")
         self.shape3 = (sp.nk + 2, sp.nj + 3, sp.ni + 2)
 
     def sync(self):
-        return lib().cpuref_sync(self._r)
+        return self._L.cpuref_sync(self._r)
 
     def step(self, n=1):
         rc = 0
         for _ in range(n):
-            rc = lib().cpuref_step(self._r)
+            rc = self._L.cpuref_step(self._r)
         return rc
 
     def read(self, f):
@@ -59,21 +63,21 @@ class CpuRef:
         q = FIELD[f] if isinstance(f, str) else f
         shape = self.shape3[1:] if q == FIELD["GZ_SURFACE"] else self.shape3
         out = np.empty(shape, dtype=np.float64)
-        lib().cpuref_read(self._r, q, out.ctypes.data)
+        self._L.cpuref_read(self._r, q, out.ctypes.data)
         return out
 
     def time_planes(self):
         it = (ctypes.c_int * 3)()
         st = ctypes.c_long()
-        lib().cpuref_time_planes(self._r, it, ctypes.byref(st))
+        self._L.cpuref_time_planes(self._r, it, ctypes.byref(st))
         return list(it) + [st.value]
 
     def error(self):
-        return lib().cpuref_error(self._r)
+        return self._L.cpuref_error(self._r)
 
     def close(self):
         if self._r:
-            lib().cpuref_destroy(self._r)
+            self._L.cpuref_destroy(self._r)
             self._r = None
 
     def __del__(self):

@@ -152,3 +152,20 @@ def test_zonal_filter_gain_keeps_the_zonal_mean():
     L.cpuref_realft(d.ctypes.data, n, -1)
     y = d[1:] * 2. / n
     assert abs(y.mean() - x.mean()) < 1e-14
+
+
+def test_libm_sensitivity_floor_of_the_step():
+    """The nudged oracle (10% of exp/log/pow results moved by one ulp) stays
+    within a few ulps on h, theta, p but carries the perturbation into u, v
+    at 1e-12..1e-10 relative over 100 steps of Held-Suarez 64x32x10: the
+    reference algorithm's own floor for any FP64 implementation whose
+    transcendentals are not bit-identical to the host libm's."""
+    sp = spec_of("hs_c1")
+    a, b = cr.CpuRef(sp), cr.CpuRef(sp, nudged=True)
+    a.sync(); b.sync()
+    a.step(100); b.step(100)
+    e = {f: float(np.linalg.norm(a.read(f) - b.read(f)) / np.linalg.norm(a.read(f)))
+         for f in ("U", "V", "HDRY", "THETA", "P3")}
+    print(e)
+    assert 0 < e["U"] < 1e-10 and 0 < e["V"] < 1e-10
+    assert e["HDRY"] < 1e-12 and e["THETA"] < 1e-12 and e["P3"] < 1e-12
