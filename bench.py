@@ -2,16 +2,19 @@
 """Throughput of the EPIC dynamical-core step on one MI355X.
 
 Workload (default): the Jupiter-like globe ni=256 nj=128 nk=20, dt=60 s,
-ideal gas, eight seeded zonal jets plus 0.5 m/s noise, del^4
+ideal gas, eight seeded zonal jets plus 0.5 m/s u,v noise, del^4
 hyperviscosity, polar filter on.  A "step" is one
 timestep(STEP_PROGS_AND_SYNC_DIAGS) of every member, with the state resident
 in HBM.  The unit is grid-point-steps/s with N_pts = ni*(nj+1)*nk per member.
 
 Prints ONE JSON line.  `roofline` is for the kernel that takes the most
-device time, from HIP events around every launch of a profiled pass:
-achieved = its algorithmic bytes per launch / its mean launch duration.
-`cpu_baseline` is the FP64 CPU restatement of the step (oracle/) timed on
-this host, one thread, on a bounded number of steps of the same workload.
+device time in a profiled pass (HIP events around every launch, on the
+engine's stream): achieved = its algorithmic bytes per launch / its mean
+launch duration, with the per-cell byte counts of KERNEL_BYTES.
+`step_roofline` prices the whole step: N_pts*steps/s * 112 B (FP64; 88 B in
+the mixed mode) against 8 TB/s.  `cpu_baseline` is the FP64 CPU
+restatement of the step (oracle/) timed on this host, one thread, on a
+bounded number of steps of the same workload.
 """
 import argparse
 import json
@@ -29,45 +32,45 @@ WORKLOADS = {
     # name: (planet, ni, nj, nk, dt, members)
     "jupiter_256x128x20": (1, 256, 128, 20, 60, 1),
     "held_suarez_64x32x10": (0, 64, 32, 10, 600, 1),
-    "ensemble_64x128x64x16": (0, 128, 64, 16, 600, 64),
-    "hot_jupiter_1024x512x64": (1, 1024, 512, 64, 30, 1),
+    "hs_ensemble_64x128x64x16": (0, 128, 64, 16, 300, 64),
+    "jupiter_1024x512x64": (1, 1024, 512, 64, 30, 1),
 }
 
-# Algorithmic traffic of each kernel, in full passes over one member's
-# interior cells (8 bytes each): arrays read plus arrays written.
-KERNEL_PASSES = {
-    "k_hsu_vert": 3, "k_hsu_merid": 4, "k_hsu_zonal": 4, "k_pgrad": 7, "k_set_p2etc": 7,
-    "k_thermo": 10, "k_gz3": 5, "k_calc_w": 7, "k_zonal_hv": 2, "k_d2dy2": 2, "k_hv_apply": 3,
-    "k_uvh_kin": 6, "k_dvdt_core": 8, "k_dudt_core": 8, "k_spline_sub": 4, "k_spline_add": 3,
-    "k_vort_interior": 4, "k_divergence": 3, "k_copy_planes": 2, "k_ab": 10, "k_dzdt3": 8,
-    "k_restore_mass": 2, "k_gz_mono": 2, "k_zero_tend": 2, "k_fill": 1, "k_adv_elem": 2,
-    "k_heating": 3, "k_sponge": 3, "k_hs_pbl": 4,
+# Algorithmic bytes per cell of each kernel: the distinct arrays it must
+# read and write once (8 B each in FP64), over the cells of its launch
+# (interior h-cells ni*(nj+1) per plane, times the planes it covers).
+KERNEL_BYTES = {
+    "k_set_p2etc": 56, "k_thermo": 72, "k_gz3": 40, "k_calc_w": 56, "k_dzdt3": 72,
+    "k_hsu_vert": 24, "k_hsu_mer_ap": 24, "k_hsu_mer_ff": 32, "k_hsu_mer_fin": 32, "k_hsu_zonal": 32,
+    "k_filter3": 24, "k_fft_filter": 16, "k_pgrad": 48, "k_zonal_hv": 16, "k_d2dy2": 16, "k_hv_apply": 24,
+    "k_uvh_kin": 48, "k_dvdt_core": 40, "k_dudt_core": 40, "k_ab": 80, "k_divdamp": 40,
+    "k_restore_mass": 16, "k_gz_mono": 16, "k_copy_planes": 16, "k_zero_tend": 16, "k_adv_elem": 16,
+    "k_heating": 24, "k_sponge": 24, "k_hs_pbl": 32, "k_divergence": 24, "k_vort_interior": 32,
 }
 
 
-def filtered_fraction(nj):
-    """Fraction of h-rows with |lat| >= 45 deg on EPIC's globe grid."""
+def filtered_rows(nj):
+    """h-rows with |lat| >= 45 deg on EPIC's globe grid."""
     dlt = 180.0 / ((nj + 1) + 2 * math.sqrt(0.5))
     lat0 = -90.0 + dlt * math.sqrt(0.5)
-    rows = 0
-    for J in range(nj + 1):
-        lat = lat0 + (2 * J + 1) * dlt * 0.5
-        rows += abs(lat) >= 45.0
-    return rows / (nj + 1)
+    return sum(abs(lat0 + (2 * J + 1) * dlt * 0.5) >= 45.0 for J in range(nj + 1))
 
 
 def kernel_bytes(name, ni, nj, nk, members):
-    cells = ni * (nj + 1) * nk * members
+    """Algorithmic bytes of one launch of `name` (None if not tabulated)."""
+    if name not in KERNEL_BYTES:
+        return None
     if name == "k_fft_filter":
-        return 2 * 8 * cells * filtered_fraction(nj)
-    return KERNEL_PASSES.get(name, 0) * 8 * cells
+        cells = ni * filtered_rows(nj) * nk
+    else:
+        cells = ni * (nj + 1) * nk
+    return KERNEL_BYTES[name] * cells * members
 
 
 def cpu_baseline(planet, ni, nj, nk, dt, budget_s):
     from oracle.cpuref import CpuRef
     import epic_amd as ea
-    sp = ea.spec(planet, ni, nj, nk, dt, seed=1, nu4=0.5,
-                 noise_theta=0.1 if planet in (0, 3) else 0.0, noise_uv=0.0 if planet == 0 else 0.5)
+    sp = spec_of(ea, planet, ni, nj, nk, dt, 1)
     r = CpuRef(sp)
     r.sync()
     r.step(1)
@@ -82,6 +85,12 @@ def cpu_baseline(planet, ni, nj, nk, dt, budget_s):
     return ni * (nj + 1) * nk * n / el, n, el
 
 
+def spec_of(ea, planet, ni, nj, nk, dt, seed):
+    gas = planet in (1, 2)
+    return ea.spec(planet, ni, nj, nk, dt, seed=seed, nu4=0.5,
+                   noise_theta=0.0 if gas else 0.1, noise_uv=0.5 if gas else 0.0)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,8 +102,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported here "
-              "(members shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
+              "(this build runs on one GPU; members would shard across GPUs with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
 
     import epic_amd as ea
@@ -102,9 +112,7 @@ def main():
     prec = ea.FP64 if args.precision == "fp64" else ea.MIXED
     eng = ea.Engine(prec, members)
     for m in range(members):
-        eng.add(ea.spec(planet, ni, nj, nk, dt, seed=1 + m, nu4=0.5,
-                        noise_theta=0.1 if planet in (0, 3) else 0.0,
-                        noise_uv=0.0 if planet == 0 else 0.5))
+        eng.add(spec_of(ea, planet, ni, nj, nk, dt, 1 + m))
     eng.step(args.warmup)
     eng.sync()
     t0 = time.perf_counter()
@@ -116,10 +124,11 @@ def main():
 
     prof = eng.profile(3)
     tot = sum(ms for _, ms, _ in prof)
-    name, ms, cnt = max(prof, key=lambda x: x[1])
+    ranked = sorted(prof, key=lambda x: -x[1])
+    name, ms, cnt = next(x for x in ranked if kernel_bytes(x[0], ni, nj, nk, members) is not None)
     avg_s = ms / cnt / 1e3
     nbytes = kernel_bytes(name, ni, nj, nk, members)
-    achieved = nbytes / avg_s if avg_s > 0 else 0.0
+    achieved = nbytes / avg_s
     step_bytes = 112 if prec == ea.FP64 else 88
 
     out = {
@@ -137,11 +146,13 @@ def main():
         "data": "synthetic (seeded initial state of the workload's planet)",
         "config": {"workload": args.workload, "ni": ni, "nj": nj, "nk": nk, "members": members, "dt": dt,
                    "precision": args.precision, "n_pts": npts},
-        "step_hbm_fraction": value * step_bytes / PEAK_HBM,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
                      "launch_us": avg_s * 1e6, "bytes_per_launch": nbytes,
-                     "share_of_step": ms / tot if tot else None},
+                     "share_of_step": ms / tot if tot else None,
+                     "top_kernels": [[n, round(m / 3 * 1e3, 1), c // 3] for n, m, c in ranked[:8]]},
+        "step_roofline": {"bytes_per_pt": step_bytes, "achieved": value * step_bytes / 1e9,
+                          "frac": value * step_bytes / PEAK_HBM, "unit": "GB/s"},
     }
     eng.close()
     if not args.no_cpu:

@@ -40,6 +40,7 @@ struct MemberDesc {
   const double *tw_f, *tw_i, *tr_f, *tr_i;  // four1 / realft twiddles, forward and inverse
   const float *tw_f32, *tw_i32, *tr_f32, *tr_i32;
   const int *rows_h, *rows_v;
+  const int *rflag_h, *rflag_v;  // 1 where row J is in the filter band
   // state and diagnostics
   double *U, *V, *DUDT, *DVDT, *HDRY, *THETA, *P3, *P2, *THETA2, *H3, *T2, *T3, *RHO2, *RHO3;
   double *EXNER3, *GZ3, *MONT3, *HEAT3, *PV3, *DIV, *W3, *DZDT3, *USPIN, *GZS;
@@ -47,6 +48,7 @@ struct MemberDesc {
   // scratch
   double *gz, *varz, *sA, *sB, *sC, *sD, *unity, *old, *spl;
   double *wstale;  // hsu column statics carried between calls (per member)
+  double *rowbuf;  // 8*(nj+3) doubles: zonal means and meridional profiles
   MemberDyn *dyn;
 };
 

@@ -184,7 +184,6 @@ static void rec_pgrad_vars(hipStream_t s, const Bucket &b) {
   L_gz_from_u(s, b);
   L_gz3(s, b);
   L_dzdt3(s, b);
-  L_bc3(s, b, F_DZDT3);
 }
 
 // store_diag (src/shared/epic_funcs_diag.c:3963-4053)
@@ -351,6 +350,11 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   add(&d.tr_i, sizeof(double) * T.n_tr, T.tri);
   add(&d.rows_h, sizeof(int) * (c->nj + 2), T.rows_h);
   add(&d.rows_v, sizeof(int) * (c->nj + 2), T.rows_v);
+  std::vector<int> flag_h(c->nj + 2, 0), flag_v(c->nj + 2, 0);
+  for (int r = 0; r < T.nrow_h; r++) flag_h[T.rows_h[r]] = 1;
+  for (int r = 0; r < T.nrow_v; r++) flag_v[T.rows_v[r]] = 1;
+  add(&d.rflag_h, sizeof(int) * (c->nj + 2), flag_h.data());
+  add(&d.rflag_v, sizeof(int) * (c->nj + 2), flag_v.data());
   add(&d.U, sizeof(double) * N3, c->u);
   add(&d.V, sizeof(double) * N3, c->v);
   add(&d.THETA, sizeof(double) * N3, c->theta);
@@ -365,6 +369,7 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   add(&d.DVDT, sizeof(double) * 3 * N3, nul);
   add(&d.spl, sizeof(double) * (size_t)(c->nj + 2) * 3 * (c->nk + 2), nul);
   add(&d.wstale, sizeof(double) * 4, nul);
+  add(&d.rowbuf, sizeof(double) * 8 * (c->nj + 3), nul);
   add(&d.dyn, sizeof(MemberDyn), nul);
   size_t total = 0;
   for (auto &p : parts) total += align256(p.bytes);

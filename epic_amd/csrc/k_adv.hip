@@ -27,107 +27,143 @@ __device__ __forceinline__ double *adv_field(const MemberDesc &d, int which) {
 }
 
 // ------------------------------------------------------------ vertical
-__global__ void k_hsu_vert(const MemberDesc *D, int mb, int which) {
+// Column tiles of C columns (J = 0..nj, I = 0..ni+1) with every level in LDS.
+// The pass is a chain of K-stencils (first-guess flux, predictor, centred
+// flux, limiter correction, update) with no recurrence, so each stage runs
+// on all lanes over (column, level) pairs, separated by barriers.
+__device__ __forceinline__ double tt_layer(const double *wa, int K, int nk, double ep) {
+  double tt;
+  if (K == 1) tt = fabs(wa[K + 1] - wa[K]) + ep;
+  else if (K == nk) tt = fabs(-wa[K] + wa[K - 1]) + ep;
+  else tt = fabs(wa[K + 1] - 2. * wa[K] + wa[K - 1]) + ep;
+  return tt * tt;
+}
+__device__ __forceinline__ double tt_iface(const double *wa, int K, int nk, double ep) {
+  double tt;
+  if (K == 0) tt = fabs(wa[K + 1] - wa[K]) + ep;
+  else if (K == nk) tt = fabs(-wa[K] + wa[K - 1]) + ep;
+  else tt = fabs(wa[K + 1] - 2. * wa[K] + wa[K - 1]) + ep;
+  return tt * tt;
+}
+
+__global__ void k_hsu_vert(const MemberDesc *D, int mb, int which, int C) {
   MEMBER;
-  int t = blockIdx.x * TPB + threadIdx.x;
-  const int nI = d.ni + 2;
-  if (t >= (d.nj + 1) * nI) return;
-  const int J = t / nI, I = t % nI;
-  const int nk = d.nk;
+  extern __shared__ double lds[];
+  const int c = threadIdx.x % C, kk = threadIdx.x / C, KS = blockDim.x / C;
+  const int nI = d.ni + 2, col = blockIdx.x * C + c;
+  const bool live = col < (d.nj + 1) * nI;
+  const int J = live ? col / nI : 0, I = live ? col % nI : 0;
+  const int nk = d.nk, L = nk + 2;
   const int layer = (which == AV_HDRY);
   const int kf = (which == AV_THETA) ? d.ks - 1 : 1;
   const double ep = 1.e-10, DT = d.dt;
   const double *dsi = d.dsgth_inv;
   double *b = adv_field(d, which);
-  double wa[KMAX + 2], wm[KMAX + 2], wp[KMAX + 2], pr[KMAX + 2], ff[KMAX + 2], d1[KMAX + 2], d2[KMAX + 2],
-      aaa[KMAX + 2], amin[KMAX + 2];
-  for (int K = 0; K <= nk + 1; K++) amin[K] = 0.;
-  if (layer)
-    for (int K = kf; K <= nk; K++) amin[K] = d.h_min[K];
-  if (which == AV_THETA) {
-    if (kf - 1 >= 1) wm[kf - 1] = d.wstale[0];
-    if (kf - 2 >= 0) wa[kf - 2] = d.wstale[1];
-  }
-  for (int K = layer ? kf : kf - 1; K <= nk; K++) wa[K] = b[ix3(d, K, J, I)];
-  if (layer) {
-    for (int K = kf - 1; K <= nk; K++) {
-      double w = d.W3[ix3(d, K, J, I)];
-      wm[K] = dmin(0., w);
-      wp[K] = dmax(0., w);
-    }
-  } else {
-    for (int K = kf; K <= nk; K++) {
-      double w = (K == 1 || K == nk) ? 0. : .5 * (d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K - 1, J, I)]);
-      wm[K] = dmin(0., w);
-      wp[K] = dmax(0., w);
+  double *wa = lds + c * L, *wv = lds + (C + c) * L, *pr = lds + (2 * C + c) * L, *ff = lds + (3 * C + c) * L;
+  // stage 1: field and vertical velocity (wv holds w on the points the
+  // field lives on: W3 itself for layers, the interface average otherwise)
+  if (live) {
+    for (int K = kk; K <= nk + 1; K += KS) {
+      double x = 0., w = 0.;
+      if (layer) {
+        if (K >= kf && K <= nk) x = b[ix3(d, K, J, I)];
+        if (K >= kf - 1 && K <= nk) w = d.W3[ix3(d, K, J, I)];
+      } else {
+        if (K >= kf - 1 && K <= nk) x = b[ix3(d, K, J, I)];
+        if (K >= kf && K <= nk) w = (K == 1 || K == nk) ? 0. : .5 * (d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K - 1, J, I)]);
+        // statics left by the unity pass (wm[kf-1], wa[kf-2])
+        if (which == AV_THETA && K == kf - 1 && K >= 1) w = d.wstale[0];
+        if (which == AV_THETA && K == kf - 2) x = d.wstale[1];
+      }
+      wa[K] = x;
+      wv[K] = w;
     }
   }
-  if (which == AV_P3 && J == d.nj && I == d.ni + 1) {
+  __syncthreads();
+  if (which == AV_P3 && live && kk == 0 && J == d.nj && I == d.ni + 1) {
     // hand the statics the theta call will read to it
     const int kft = d.ks - 1;
-    if (kft - 1 >= 1) d.wstale[0] = wm[kft - 1];
+    if (kft - 1 >= 1) d.wstale[0] = dmin(0., wv[kft - 1]);
     if (kft - 2 >= 0) d.wstale[1] = wa[kft - 2];
   }
+#define WM(k) dmin(0., wv[k])
+#define WP(k) dmax(0., wv[k])
   if (layer) {
-    ff[kf - 1] = 0.;
-    ff[nk] = 0.;
-    for (int K = kf; K < nk; K++) ff[K] = wp[K] * wa[K + 1] + wm[K] * wa[K];
-    for (int K = kf; K <= nk; K++) pr[K] = wa[K] + (ff[K] - ff[K - 1]) * dsi[2 * K] * DT;
-    for (int K = kf; K < nk; K++) ff[K] = .5 * (wp[K] * (pr[K] + wa[K + 1]) + wm[K] * (pr[K + 1] + wa[K]));
-    for (int K = kf; K <= nk; K++) {
-      double tt;
-      if (K == 1) { d1[K] = pr[K] - wa[K + 1]; d2[K] = 0.; tt = fabs(wa[K + 1] - wa[K]) + ep; }
-      else if (K == nk) { d1[K] = 0.; d2[K] = pr[K] - wa[K - 1]; tt = fabs(-wa[K] + wa[K - 1]) + ep; }
-      else { d1[K] = pr[K] - wa[K + 1]; d2[K] = pr[K] - wa[K - 1]; tt = fabs(wa[K + 1] - 2. * wa[K] + wa[K - 1]) + ep; }
-      aaa[K] = tt * tt;
-    }
-    for (int K = kf; K < nk; K++) {
-      double whp = sqrt(wp[K] * wp[K + 1]), whm = -sqrt(wm[K] * wm[K - 1]);
-      double mu = (wp[K] - wm[K]) * dsi[2 * K + 1] * DT, al = (1. + mu) / 6.;
-      double gp = aaa[K + 1] / (aaa[K + 1] + (wa[K + 1] - amin[K + 1]) * (wa[K] - amin[K]));
-      gp = gp * gp;
-      double gm = aaa[K] / (aaa[K] + (wa[K + 1] - amin[K + 1]) * (wa[K] - amin[K]));
-      gm = gm * gm;
-      double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
-      double bhp = 1. - gp, bhm = 1. - gm;
-      ff[K] += -al * (wp[K] * bp * d1[K] - whp * bhp * d1[K + 1] + wm[K] * bm * d2[K + 1] - whm * bhm * d2[K]);
-    }
-    for (int K = kf; K <= nk; K++) b[ix3(d, K, J, I)] += (ff[K] - ff[K - 1]) * dsi[2 * K] * DT;
+    // first-guess flux on interfaces kf..nk-1 (zero at kf-1 and nk)
+    if (live)
+      for (int K = kf - 1 + kk; K <= nk; K += KS)
+        ff[K] = (K == kf - 1 || K == nk) ? 0. : WP(K) * wa[K + 1] + WM(K) * wa[K];
+    __syncthreads();
+    if (live)
+      for (int K = kf + kk; K <= nk; K += KS) pr[K] = wa[K] + (ff[K] - ff[K - 1]) * dsi[2 * K] * DT;
+    __syncthreads();
+    if (live)
+      for (int K = kf + kk; K < nk; K += KS) ff[K] = .5 * (WP(K) * (pr[K] + wa[K + 1]) + WM(K) * (pr[K + 1] + wa[K]));
+    __syncthreads();
+    if (live)
+      for (int K = kf + kk; K < nk; K += KS) {
+        const double d1K = pr[K] - wa[K + 1];
+        const double d1P = (K + 1 == nk) ? 0. : pr[K + 1] - wa[K + 2];
+        const double d2K = (K == 1) ? 0. : pr[K] - wa[K - 1];
+        const double d2P = pr[K + 1] - wa[K];
+        const double aaK = tt_layer(wa, K, nk, ep), aaP = tt_layer(wa, K + 1, nk, ep);
+        const double aminK = (K >= kf) ? d.h_min[K] : 0., aminP = d.h_min[K + 1];
+        const double wpK = WP(K), wmK = WM(K);
+        double whp = sqrt(wpK * WP(K + 1)), whm = -sqrt(wmK * WM(K - 1));
+        double mu = (wpK - wmK) * dsi[2 * K + 1] * DT, al = (1. + mu) / 6.;
+        double gp = aaP / (aaP + (wa[K + 1] - aminP) * (wa[K] - aminK));
+        gp = gp * gp;
+        double gm = aaK / (aaK + (wa[K + 1] - aminP) * (wa[K] - aminK));
+        gm = gm * gm;
+        double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
+        double bhp = 1. - gp, bhm = 1. - gm;
+        ff[K] += -al * (wpK * bp * d1K - whp * bhp * d1P + wmK * bm * d2P - whm * bhm * d2K);
+      }
+    __syncthreads();
+    if (live)
+      for (int K = kf + kk; K <= nk; K += KS) b[ix3(d, K, J, I)] += (ff[K] - ff[K - 1]) * dsi[2 * K] * DT;
   } else {
-    for (int K = kf; K <= nk; K++) ff[K] = wp[K] * wa[K] + wm[K] * wa[K - 1];
-    ff[nk + 1] = 0.;
-    pr[kf - 1] = wa[kf - 1];
-    for (int K = kf; K <= nk; K++) pr[K] = wa[K] + (ff[K + 1] - ff[K]) * dsi[2 * K + 1] * DT;
-    for (int K = kf; K <= nk; K++) ff[K] = .5 * (wp[K] * (pr[K - 1] + wa[K]) + wm[K] * (pr[K] + wa[K - 1]));
-    for (int K = kf - 1; K <= nk; K++) {
-      double tt;
-      if (K == 0) tt = fabs(wa[K + 1] - wa[K]) + ep;
-      else if (K == nk) tt = fabs(-wa[K] + wa[K - 1]) + ep;
-      else tt = fabs(wa[K + 1] - 2. * wa[K] + wa[K - 1]) + ep;
-      aaa[K] = tt * tt;
-    }
-    d2[kf - 1] = 0.;
-    for (int K = kf; K <= nk; K++) {
-      d1[K] = pr[K - 1] - wa[K];
-      d2[K] = pr[K] - wa[K - 1];
-    }
-    for (int K = kf; K <= nk; K++) {
-      double whm = (K == 1) ? 0. : -sqrt(wm[K] * wm[K - 1]);
-      double whp = (K == nk) ? 0. : sqrt(wp[K] * wp[K + 1]);
-      double mu = (wp[K] - wm[K]) * dsi[2 * K] * DT, al = (1. + mu) / 6.;
-      double gp = aaa[K] / (aaa[K] + (wa[K] - amin[K]) * (wa[K - 1] - amin[K - 1]));
-      gp = gp * gp;
-      double gm = aaa[K - 1] / (aaa[K - 1] + (wa[K] - amin[K]) * (wa[K - 1] - amin[K - 1]));
-      gm = gm * gm;
-      double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
-      double bhp = 1. - gp, bhm = 1. - gm, g;
-      if (K < nk) g = -al * (wp[K] * bp * d1[K] - whp * bhp * d1[K + 1] + wm[K] * bm * d2[K] - whm * bhm * d2[K - 1]);
-      else g = -al * (wp[K] * bp * d1[K] + wm[K] * bm * d2[K] - whm * bhm * d2[K - 1]);
-      ff[K] += g;
-    }
-    ff[nk + 1] = 0.;
-    for (int K = kf; K <= nk; K++) b[ix3(d, K, J, I)] += (ff[K + 1] - ff[K]) * dsi[2 * K + 1] * DT;
+    // first-guess flux on layers kf..nk (zero below the bottom)
+    if (live)
+      for (int K = kf + kk; K <= nk + 1; K += KS) ff[K] = (K == nk + 1) ? 0. : WP(K) * wa[K] + WM(K) * wa[K - 1];
+    __syncthreads();
+    if (live)
+      for (int K = kf - 1 + kk; K <= nk; K += KS)
+        pr[K] = (K == kf - 1) ? wa[K] : wa[K] + (ff[K + 1] - ff[K]) * dsi[2 * K + 1] * DT;
+    __syncthreads();
+    if (live)
+      for (int K = kf + kk; K <= nk; K += KS) ff[K] = .5 * (WP(K) * (pr[K - 1] + wa[K]) + WM(K) * (pr[K] + wa[K - 1]));
+    __syncthreads();
+    if (live)
+      for (int K = kf + kk; K <= nk; K += KS) {
+        const double d1K = pr[K - 1] - wa[K];
+        const double d2K = pr[K] - wa[K - 1];
+        const double d2M = (K == kf) ? 0. : pr[K - 1] - wa[K - 2];
+        const double aaK = tt_iface(wa, K, nk, ep), aaM = tt_iface(wa, K - 1, nk, ep);
+        const double wpK = WP(K), wmK = WM(K);
+        double whm = (K == 1) ? 0. : -sqrt(wmK * WM(K - 1));
+        double whp = (K == nk) ? 0. : sqrt(wpK * WP(K + 1));
+        double mu = (wpK - wmK) * dsi[2 * K] * DT, al = (1. + mu) / 6.;
+        double gp = aaK / (aaK + (wa[K] - 0.) * (wa[K - 1] - 0.));
+        gp = gp * gp;
+        double gm = aaM / (aaM + (wa[K] - 0.) * (wa[K - 1] - 0.));
+        gm = gm * gm;
+        double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
+        double bhp = 1. - gp, bhm = 1. - gm, g;
+        if (K < nk) {
+          const double d1P = pr[K] - wa[K + 1];
+          g = -al * (wpK * bp * d1K - whp * bhp * d1P + wmK * bm * d2K - whm * bhm * d2M);
+        } else {
+          g = -al * (wpK * bp * d1K + wmK * bm * d2K - whm * bhm * d2M);
+        }
+        ff[K] += g;
+      }
+    __syncthreads();
+    if (live)
+      for (int K = kf + kk; K <= nk; K += KS) b[ix3(d, K, J, I)] += (ff[K + 1] - ff[K]) * dsi[2 * K + 1] * DT;
   }
+#undef WM
+#undef WP
 }
 
 // ---------------------------------------------------------- meridional
@@ -186,105 +222,146 @@ struct MerCol {
   }
 };
 
-__global__ void k_hsu_merid(const MemberDesc *D, int mb, int which) {
+// Three launches per field, each one lane per cell: the predictor ap(J)
+// (into sC), the corrected flux ff(J) on v-rows 1..nj (into sD), and the
+// update into sA and its copy sB (which the zonal pass filters).
+#define MER_SETUP                                                   \
+  const int K = 1 + blockIdx.y;                                     \
+  if (which == AV_THETA && K <= d.ks - 2) return;                   \
+  MerCol c;                                                         \
+  c.d = &d;                                                         \
+  c.A = adv_field(d, which) + (size_t)K * d.N2;                     \
+  c.K = K;                                                          \
+  c.I = I;                                                          \
+  c.which = which;                                                  \
+  c.amin = (which == AV_HDRY) ? d.h_min[K] : 0.;                    \
+  c.DT = d.dt;                                                      \
+  const size_t o = (size_t)K * d.N2;
+
+__global__ void k_hsu_mer_ap(const MemberDesc *D, int mb, int which) {
   MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= (d.nj + 1) * d.ni) return;
   const int J = t / d.ni, I = 1 + t % d.ni;
-  const int K = 1 + blockIdx.y;
-  if (which == AV_THETA && K <= d.ks - 2) return;
-  MerCol c;
-  c.d = &d;
-  double *F = adv_field(d, which);
-  c.A = F + (size_t)K * d.N2;
-  c.K = K;
-  c.I = I;
-  c.which = which;
-  c.amin = (which == AV_HDRY) ? d.h_min[K] : 0.;
-  c.DT = d.dt;
-  double v = c.a(J) + (c.ff(J) - c.ff(J + 1)) * d.mn[2 * J + 1] * c.DT;
-  size_t q = ix3(d, K, J, I);
+  MER_SETUP;
+  d.sC[o + ix2(d, J, I)] = c.ap(J);
+}
+__global__ void k_hsu_mer_ff(const MemberDesc *D, int mb, int which) {
+  MEMBER;
+  int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= d.nj * d.ni) return;
+  const int J = 1 + t / d.ni, I = 1 + t % d.ni;
+  MER_SETUP;
+  const double *AP = d.sC + o;
+  const int nj = d.nj;
+  double VP = c.vp(J), VM = c.vm(J), APj = AP[ix2(d, J, I)], APm = AP[ix2(d, J - 1, I)], Am = c.a(J - 1), Aj = c.a(J);
+  double fc = .5 * (VP * (APj + Am) + VM * (APm + Aj));
+  double vhp = sqrt(VP * c.vp(J - 1)), vhm = -sqrt(VM * c.vm(J + 1));
+  double mu = (VP - VM) * d.mn[2 * J] * c.DT, al = (1. + mu) / 6.;
+  double q0 = c.aq(J - 1), q1 = c.aq(J);
+  double gp = q0 / (q0 + (Am - c.amin) * (Aj - c.amin));
+  gp = gp * gp;
+  double gm = q1 / (q1 + (Am - c.amin) * (Aj - c.amin));
+  gm = gm * gm;
+  double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
+  double bhp = 1. - gp, bhm = 1. - gm;
+  double d1J = (APj - Am);
+  double d1m = (J - 1 == 0) ? 0. : APm - c.a(J - 2);
+  double d2m = APm - c.a(J);
+  double d2J = (J == nj) ? 0. : APj - c.a(J + 1);
+  d.sD[o + ix2(d, J, I)] = fc + -al * (VP * bp * d1J - vhp * bhp * d1m + VM * bm * d2m - vhm * bhm * d2J);
+}
+__global__ void k_hsu_mer_fin(const MemberDesc *D, int mb, int which) {
+  MEMBER;
+  int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= (d.nj + 1) * d.ni) return;
+  const int J = t / d.ni, I = 1 + t % d.ni;
+  MER_SETUP;
+  const double *FF = d.sD + o;
+  double f0 = (J >= 1) ? FF[ix2(d, J, I)] : 0., f1 = (J + 1 <= d.nj) ? FF[ix2(d, J + 1, I)] : 0.;
+  double v = c.a(J) + (f0 - f1) * d.mn[2 * J + 1] * c.DT;
+  const size_t q = o + ix2(d, J, I);
   d.sA[q] = v;
   d.sB[q] = v;
+  // the copy the zonal pass filters carries its zonal pads
+  if (I == d.ni) d.sB[o + ix2(d, J, 0)] = v;
+  if (I == 1) d.sB[o + ix2(d, J, d.ni + 1)] = v;
 }
+#undef MER_SETUP
 
 // --------------------------------------------------------------- zonal
-struct ZonRow {
-  const MemberDesc *d;
-  const double *af;  // filtered copy, plane K, with pads
-  int K, J, which, ni;
-  double amin, DT, ninv, mn;
-  __device__ int w(int I) const { return I < 1 ? I + ni : (I > ni ? I - ni : I); }
-  __device__ double a(int I) const { return af[ix2(*d, J, I)]; }
-  __device__ double u2(int I) const {
-    const MemberDesc &m = *d;
+// One block per (K,J) row of the filtered copy sB, staged in LDS with its
+// pads: velocities, first-guess fluxes, predictor, corrected fluxes and the
+// update sA + d(flux) each run on all lanes over the row, periodic in I.
+__global__ void k_hsu_zonal(const MemberDesc *D, int mb, int which) {
+  MEMBER;
+  const int J = blockIdx.x, K = 1 + blockIdx.y;
+  if (which == AV_THETA && K <= d.ks - 2) return;
+  extern __shared__ double sh[];
+  const int ni = d.ni, W = ni + 2;
+  double *A = sh, *UP = sh + W, *UM = sh + 2 * W, *FP = sh + 3 * W, *AP = sh + 4 * W, *FF = sh + 5 * W;
+  const double *af = d.sB + (size_t)K * d.N2;
+  const double ninv = 1. / d.n[2 * J + 1], mn = d.mn[2 * J + 1], DT = d.dt;
+  const double amin = (which == AV_HDRY) ? d.h_min[K] : 0.;
+  auto wr = [ni](int I) { return I < 1 ? I + ni : (I > ni ? I - ni : I); };
+  for (int I = threadIdx.x; I < W; I += blockDim.x) {
+    A[I] = af[ix2(d, J, I)];
+    double u2;
     if (which == AV_HDRY) {
-      int Ii = w(I);
-      return .5 * (m.U[ix3(m, K, J, Ii)] + m.U[ix3(m, K - 1, J, Ii)]);
+      int Ii = wr(I);
+      u2 = .5 * (d.U[ix3(d, K, J, Ii)] + d.U[ix3(d, K - 1, J, Ii)]);
+    } else {
+      u2 = d.U[ix3(d, K, J, I)];
     }
-    return m.U[ix3(m, K, J, I)];
+    UM[I] = dmin(0., u2 * ninv);
+    UP[I] = dmax(0., u2 * ninv);
   }
-  __device__ double um(int I) const { return dmin(0., u2(I) * ninv); }
-  __device__ double up(int I) const { return dmax(0., u2(I) * ninv); }
-  // intermediates at c in 1..ni
-  __device__ double ffp(int c) const { return up(c) * a(c - 1) + um(c) * a(c); }
-  __device__ double ap(int c) const { return a(c) + (ffp(c) - ffp(w(c + 1))) * mn * DT; }
-  __device__ double aq(int c) const {
-    double t = fabs(a(c - 1) - 2. * a(c) + a(c + 1)) + 1.e-10;
-    return t * t;
-  }
-  __device__ double ff(int c) const {
-    double UP = up(c), UM = um(c), APc = ap(c), APm = ap(w(c - 1)), Am = a(c - 1), Ac = a(c);
-    double fc = .5 * (UP * (APc + Am) + UM * (APm + Ac));
-    double d1c = APc - Am, d2c = APc - a(c + 1);
-    int cm = w(c - 1);
-    double d1m = APm - a(cm - 1), d2m = APm - a(cm + 1);
-    double uhp = sqrt(UP * up(c - 1)), uhm = -sqrt(UM * um(c + 1));
-    double mu = (UP - UM) * mn * DT, al = (1. + mu) / 6.;
-    double q0 = aq(cm), q1 = aq(c);
+  __syncthreads();
+  for (int c = 1 + threadIdx.x; c <= ni; c += blockDim.x) FP[c] = UP[c] * A[c - 1] + UM[c] * A[c];
+  __syncthreads();
+  for (int c = 1 + threadIdx.x; c <= ni; c += blockDim.x) AP[c] = A[c] + (FP[c] - FP[wr(c + 1)]) * mn * DT;
+  __syncthreads();
+  for (int c = 1 + threadIdx.x; c <= ni; c += blockDim.x) {
+    const int cm = wr(c - 1);
+    double UPc = UP[c], UMc = UM[c], APc = AP[c], APm = AP[cm], Am = A[c - 1], Ac = A[c];
+    double fc = .5 * (UPc * (APc + Am) + UMc * (APm + Ac));
+    double d1c = APc - Am, d2c = APc - A[c + 1];
+    double d1m = APm - A[cm - 1], d2m = APm - A[cm + 1];
+    double uhp = sqrt(UPc * UP[c - 1]), uhm = -sqrt(UMc * UM[c + 1]);
+    double mu = (UPc - UMc) * mn * DT, al = (1. + mu) / 6.;
+    double t0 = fabs(A[cm - 1] - 2. * A[cm] + A[cm + 1]) + 1.e-10, q0 = t0 * t0;
+    double t1 = fabs(A[c - 1] - 2. * A[c] + A[c + 1]) + 1.e-10, q1 = t1 * t1;
     double gp = q0 / (q0 + (Am - amin) * (Ac - amin));
     gp = gp * gp;
     double gm = q1 / (q1 + (Am - amin) * (Ac - amin));
     gm = gm * gm;
     double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
     double bhp = 1. - gp, bhm = 1. - gm;
-    return fc + -al * (UP * bp * d1c - uhp * bhp * d1m + UM * bm * d2m - uhm * bhm * d2c);
+    FF[c] = fc + -al * (UPc * bp * d1c - uhp * bhp * d1m + UMc * bm * d2m - uhm * bhm * d2c);
   }
-};
-
-__global__ void k_hsu_zonal(const MemberDesc *D, int mb, int which) {
-  MEMBER;
-  int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= (d.nj + 1) * d.ni) return;
-  const int J = t / d.ni, I = 1 + t % d.ni;
-  const int K = 1 + blockIdx.y;
-  if (which == AV_THETA && K <= d.ks - 2) return;
-  ZonRow z;
-  z.d = &d;
-  z.af = d.sB + (size_t)K * d.N2;
-  z.K = K;
-  z.J = J;
-  z.which = which;
-  z.ni = d.ni;
-  z.amin = (which == AV_HDRY) ? d.h_min[K] : 0.;
-  z.DT = d.dt;
-  z.ninv = 1. / d.n[2 * J + 1];
-  z.mn = d.mn[2 * J + 1];
-  double da = (z.ff(I) - z.ff(z.w(I + 1))) * z.mn * z.DT;
-  size_t q = ix3(d, K, J, I);
-  adv_field(d, which)[q] = d.sA[q] + da;
+  __syncthreads();
+  double *out = adv_field(d, which);
+  for (int I = 1 + threadIdx.x; I <= ni; I += blockDim.x) {
+    const size_t q = ix3(d, K, J, I);
+    const double y = d.sA[q] + (FF[I] - FF[wr(I + 1)]) * mn * DT;
+    out[q] = y;
+    if (I == ni) out[ix3(d, K, J, 0)] = y;
+    if (I == 1) out[ix3(d, K, J, ni + 1)] = y;
+  }
 }
 
 static void hsu(hipStream_t s, const Bucket &b, int which, int fld) {
-  const int TP = TPB;
   int ncol = (b.nj + 1) * (b.ni + 2);
-  LAUNCH(k_hsu_vert, dim3((ncol + TP - 1) / TP, 1, b.nm), dim3(TP), 0, s, b.D, b.mb, which);
-  int ncell = (b.nj + 1) * b.ni;
-  dim3 g((ncell + TP - 1) / TP, b.nk, b.nm);
-  LAUNCH(k_hsu_merid, g, dim3(TP), 0, s, b.D, b.mb, which);
+  const int C = b.nk <= 30 ? 64 : 32;
+  LAUNCH(k_hsu_vert, dim3((ncol + C - 1) / C, 1, b.nm), dim3(256), sizeof(double) * 4 * C * (b.nk + 2), s, b.D, b.mb,
+         which, C);
+  dim3 g(((b.nj + 1) * b.ni + TPB - 1) / TPB, b.nk, b.nm);
+  LAUNCH(k_hsu_mer_ap, g, dim3(TPB), 0, s, b.D, b.mb, which);
+  LAUNCH(k_hsu_mer_ff, dim3((b.nj * b.ni + TPB - 1) / TPB, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb, which);
+  LAUNCH(k_hsu_mer_fin, g, dim3(TPB), 0, s, b.D, b.mb, which);
   L_filter(s, b, F_SB, 1, b.nk, 0, 0);
-  LAUNCH(k_hsu_zonal, g, dim3(TP), 0, s, b.D, b.mb, which);
-  L_bc3(s, b, fld);
+  LAUNCH(k_hsu_zonal, dim3(b.nj + 1, b.nk, b.nm), dim3(256), sizeof(double) * 6 * (b.ni + 2), s, b.D, b.mb, which);
+  (void)fld;
 }
 
 // unity = 1 on K = 0..nk+1, J = 0..nj, I = 0..ni+1; OLD = THETA and the

@@ -2,16 +2,17 @@
 //
 //   set_p2etc         src/shared/epic_funcs_diag.c:3352-3569  (column scan)
 //   store_pgrad_vars  src/shared/epic_funcs_diag.c:3595-3950  (column scans)
-//   gz_from_u         src/shared/epic_funcs_diag.c:4460-4610  (one block per member)
+//   gz_from_u         src/shared/epic_funcs_diag.c:4460-4610
 //   calc_heating      src/shared/epic_funcs_diag.c:4793-4869 +
 //                     src/shared/epic_sensible_heating.c:51-463 (Held-Suarez)
 //   calc_w            src/shared/epic_flux.c:123-335           (column scan)
 //   vorticity         src/shared/epic_funcs_diag.c:4131-4445
 //   divergence        src/shared/epic_funcs_diag.c:4064-4117
 //
-// Column kernels run one lane per (J,I) column with the K recurrence in the
-// lane, so every load and store of a K-level is coalesced along I.  The
-// arithmetic of each expression follows the reference's operation order.
+// Column work runs in LDS column tiles (per-level arithmetic on every lane,
+// the K recurrences in one lane per column); zonal means stage the row in
+// LDS and sum it in one lane in the reference's order.  The arithmetic of
+// each expression follows the reference's operation order.
 #include "kernels.h"
 
 namespace epic {
@@ -86,114 +87,168 @@ void L_copy_planes(hipStream_t s, const Bucket &b, int src, int dst, int k0, int
                      src, dst, k0);
 }
 
-// ------------------------------------------------------------ set_p2etc
-__device__ __forceinline__ double h_layer(const MemberDesc &d, const double *P3, int K, int J, int I) {
-  return ((double)P3[ix3(d, K, J, I)] - (double)P3[ix3(d, K - 1, J, I)]) /
-         ((double)d.g[2 * J + 1] * (double)d.dsgth[2 * K]);
+// ------------------------------------------------------------ column tiles
+// A block holds COLS consecutive (J,I) columns (J = 0..nj, I = 0..ni+1) and
+// all their levels.  The per-level work (transcendentals, Leslie-Purser
+// weights) runs on all lanes, (column, level) pairs strided over the block;
+// the recurrences in K (pressure from the top, geopotential from the
+// bottom, the running SUM_DIV) run in one lane per column over values staged
+// in LDS, in the reference's order.
+constexpr int COLS = 64;
+struct Tile {
+  int c, kk, KS, J, I;
+  bool live;
+};
+__device__ __forceinline__ Tile tile_of(const MemberDesc &d) {
+  Tile t;
+  t.c = threadIdx.x % COLS;
+  t.kk = threadIdx.x / COLS;
+  t.KS = blockDim.x / COLS;
+  const int nI = d.ni + 2, col = blockIdx.x * COLS + t.c;
+  t.live = col < (d.nj + 1) * nI;
+  t.J = t.live ? col / nI : 0;
+  t.I = t.live ? col % nI : 0;
+  return t;
+}
+static inline dim3 tile_grid(const Bucket &b) {
+  return dim3((unsigned)(((b.nj + 1) * (b.ni + 2) + COLS - 1) / COLS), 1, (unsigned)b.nm);
 }
 
+// ------------------------------------------------------------ set_p2etc
+// P3 from the top down (prefix of dsgth*g*h), the sigma-region pressures,
+// h = max(h_min, dp/(g dsgth)), and the layer diagnostics P2, H3, THETA2
+// (and the theta of the sigma region when update_theta)
 __global__ void k_set_p2etc(const MemberDesc *D, int mb, int update_theta) {
   MEMBER;
-  FLAT2(0, d.nj + 1, 0, d.ni + 2);
-  const int nk = d.nk, ks = d.ks;
+  extern __shared__ double lds[];
+  const Tile T = tile_of(d);
+  const int nk = d.nk, ks = d.ks, W = nk + 1, J = T.J, I = T.I;
+  double *a = lds + T.c * W, *p2 = lds + COLS * W + T.c * W, *pw = lds + 2 * COLS * W + T.c * W;
+  double *aux = lds + 3 * COLS * W + 2 * T.c;
   double *P3 = d.P3, *H = d.HDRY;
   const double g = d.g[2 * J + 1];
-  for (int K = 1; K <= nk; K++)
-    P3[ix3(d, K, J, I)] = P3[ix3(d, K - 1, J, I)] + d.dsgth[2 * K] * g * H[ix3(d, K, J, I)];
-  const double ptop = P3[ix3(d, 0, J, I)], pbot = P3[ix3(d, nk, J, I)];
-  for (int K = ks - 1; K < nk; K++) {
-    double sigma = (d.sgth[2 * K + 1] - d.zeta0) / (d.zeta1 - d.zeta0);
-    P3[ix3(d, K, J, I)] = get_p_sigma(pbot, sigma, ptop);
-  }
-  for (int K = 1; K <= nk; K++) H[ix3(d, K, J, I)] = dmax(d.h_min[K], h_layer(d, P3, K, J, I));
-  if (update_theta)
-    for (int K = 0; K < ks - 1; K++) {
-      double sigma = get_sigma(pbot, P3[ix3(d, K, J, I)], ptop);
-      d.THETA[ix3(d, K, J, I)] = ((double)d.sgth[2 * K + 1] - f_sig(d, sigma)) / g_sig(d, sigma);
+  if (T.live)
+    for (int K = T.kk; K <= nk; K += T.KS)
+      a[K] = (K == 0) ? P3[ix3(d, 0, J, I)] : d.dsgth[2 * K] * g * H[ix3(d, K, J, I)];
+  __syncthreads();
+  if (T.live && T.kk == 0)
+    for (int K = 1; K <= nk; K++) a[K] = a[K - 1] + a[K];
+  __syncthreads();
+  if (T.live) {
+    const double ptop = a[0], pbot = a[nk];
+    for (int K = ks - 1 + T.kk; K < nk; K += T.KS) {
+      double sigma = (d.sgth[2 * K + 1] - d.zeta0) / (d.zeta1 - d.zeta0);
+      a[K] = get_p_sigma(pbot, sigma, ptop);
     }
-  const double kp1 = d.kappa + 1.;
-  for (int K = 1; K <= nk; K++) {
-    double top = P3[ix3(d, K - 1, J, I)], bot = P3[ix3(d, K, J, I)], p2;
-    if (fcmp_d(bot, top) == 0) p2 = .5 * (bot + top);
-    else if (top < 0. || bot < 0.) { atomicOr(&d.dyn->err, 2); p2 = 0.; }
-    else p2 = pow((pow(bot, kp1) - pow(top, kp1)) / (kp1 * (bot - top)), 1. / d.kappa);
-    d.P2[ix3(d, K, J, I)] = p2;
   }
-  d.P2[ix3(d, 0, J, I)] = P3[ix3(d, 0, J, I)] * P3[ix3(d, 0, J, I)] / d.P2[ix3(d, 1, J, I)];
-  for (int K = 1; K <= nk; K++) {
-    double h;
-    if (K < nk) {
-      int kk = 2 * K + 1;
-      double topwt = d.dsgth[kk - 1], botwt = d.dsgth[kk + 1];
-      h = (topwt * h_layer(d, P3, K, J, I) + botwt * h_layer(d, P3, K + 1, J, I)) / (topwt + botwt);
-    } else {
-      h = h_layer(d, P3, K, J, I);
-    }
-    d.H3[ix3(d, K, J, I)] = h;
-  }
-  d.H3[ix3(d, 0, J, I)] = d.H3[ix3(d, 1, J, I)];
-  if (update_theta) {
-    for (int K = 1; K < ks - 1; K++) {
-      double sigma = get_sigma(pbot, d.P2[ix3(d, K, J, I)], ptop);
-      d.THETA2[ix3(d, K, J, I)] = ((double)d.sgth[2 * K] - f_sig(d, sigma)) / g_sig(d, sigma);
-    }
-    d.THETA2[ix3(d, ks - 1, J, I)] = .5 * (d.THETA[ix3(d, ks - 1, J, I)] + d.THETA[ix3(d, ks - 2, J, I)]);
-    // linear interpolation of theta against sigmatheta, table ordered from
-    // the bottom interface up (src/shared/epic_funcs_diag.c:3522-3562)
+  __syncthreads();
+  // p^(kappa+1) once per interface (each is the top of one layer and the
+  // bottom of the next)
+  if (T.live)
+    for (int K = T.kk; K <= nk; K += T.KS) pw[K] = pow(a[K], d.kappa + 1.);
+  __syncthreads();
+  if (T.live) {
+    const double ptop = a[0], pbot = a[nk], kp1 = d.kappa + 1.;
     const int nkay = nk - (ks - 1) + 1;
     const double *tx = d.sgth + 2 * nk + 1;  // tx[kay*(-2)] = sgth[2*(nk-kay)+1]
-    for (int K = ks; K <= nk; K++) {
-      double sg = d.sgth[2 * K], h;
-      int kay = table_place_s(nkay, tx, -2, sg, &h);
-      double y0 = d.THETA[ix3(d, nk - kay, J, I)], y1 = d.THETA[ix3(d, nk - kay - 1, J, I)];
-      d.THETA2[ix3(d, K, J, I)] = y0 + (y1 - y0) * (sg - tx[-2 * kay]) / h;
+    for (int K = T.kk; K <= nk; K += T.KS) {
+      const size_t q = ix3(d, K, J, I);
+      P3[q] = a[K];
+      if (update_theta && K < ks - 1) {
+        double sigma = get_sigma(pbot, a[K], ptop);
+        double th = ((double)d.sgth[2 * K + 1] - f_sig(d, sigma)) / g_sig(d, sigma);
+        d.THETA[q] = th;
+        if (K == ks - 2) aux[0] = th;
+      }
+      if (K == 0) continue;
+      double hl = (a[K] - a[K - 1]) / (g * d.dsgth[2 * K]);
+      H[q] = dmax(d.h_min[K], hl);
+      double top = a[K - 1], bot = a[K], pp;
+      if (fcmp_d(bot, top) == 0) pp = .5 * (bot + top);
+      else if (top < 0. || bot < 0.) { atomicOr(&d.dyn->err, 2); pp = 0.; }
+      else pp = pow((pw[K] - pw[K - 1]) / (kp1 * (bot - top)), 1. / d.kappa);
+      p2[K] = pp;
+      d.P2[q] = pp;
+      double h3;
+      if (K < nk) {
+        int kk = 2 * K + 1;
+        double topwt = d.dsgth[kk - 1], botwt = d.dsgth[kk + 1];
+        double hl1 = (a[K + 1] - a[K]) / (g * d.dsgth[2 * (K + 1)]);
+        h3 = (topwt * hl + botwt * hl1) / (topwt + botwt);
+      } else {
+        h3 = hl;
+      }
+      d.H3[q] = h3;
+      if (K == 1) aux[1] = h3;
+      if (update_theta) {
+        if (K < ks - 1) {
+          double sigma = get_sigma(pbot, pp, ptop);
+          d.THETA2[q] = ((double)d.sgth[2 * K] - f_sig(d, sigma)) / g_sig(d, sigma);
+        } else if (K >= ks) {
+          // linear interpolation of theta against sigmatheta, table ordered
+          // from the bottom interface up (src/shared/epic_funcs_diag.c:3522-3562)
+          double sg = d.sgth[2 * K], h;
+          int kay = table_place_s(nkay, tx, -2, sg, &h);
+          double y0 = d.THETA[ix3(d, nk - kay, J, I)], y1 = d.THETA[ix3(d, nk - kay - 1, J, I)];
+          d.THETA2[q] = y0 + (y1 - y0) * (sg - tx[-2 * kay]) / h;
+        }
+      }
     }
+  }
+  __syncthreads();
+  if (T.live && T.kk == 0) {
+    const size_t q0 = ix3(d, 0, J, I);
+    d.P2[q0] = a[0] * a[0] / p2[1];
+    d.H3[q0] = aux[1];
+    if (update_theta) d.THETA2[ix3(d, ks - 1, J, I)] = .5 * (d.THETA[ix3(d, ks - 1, J, I)] + aux[0]);
   }
 }
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
-  LAUNCH(k_set_p2etc, grid2(b.nj + 1, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb, update_theta);
+  size_t shm = sizeof(double) * (3 * COLS * (b.nk + 1) + 2 * COLS);
+  LAUNCH(k_set_p2etc, tile_grid(b), dim3(TPB), shm, s, b.D, b.mb, update_theta);
 }
 
 // ------------------------------------------------- store_pgrad_vars parts
-// T3, EXNER3, RHO3, T2, RHO2 and the top values; DZDT3 <- GZ3 when itime > 1
+// T3, EXNER3, RHO3, T2, RHO2 per cell and the top values; DZDT3 <- GZ3 when
+// itime > 1.  One lane per (K,J,I), K = 0..nk.
 __global__ void k_thermo(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 0, d.ni + 2);
-  const int nk = d.nk;
-  if (d.dyn->itime > 1 && I >= 1 && I <= d.ni)
-    for (int K = 1; K <= nk; K++) d.DZDT3[ix3(d, K, J, I)] = d.GZ3[ix3(d, K, J, I)];
+  const int K = blockIdx.y;
+  const size_t q = ix3(d, K, J, I);
+  if (K >= 1 && d.dyn->itime > 1 && I >= 1 && I <= d.ni) d.DZDT3[q] = d.GZ3[q];
   const double R_GAS = 8.314472e+3;
-  for (int K = 0; K <= nk; K++) {
-    size_t q = ix3(d, K, J, I);
-    double th = d.THETA[q], p = d.P3[q];
-    double t = th * pow(p / d.press0, d.kappa);
-    d.T3[q] = t;
-    d.EXNER3[q] = d.cp * t / th;
-    d.RHO3[q] = p * d.mu3 / (R_GAS * t);
-  }
-  for (int K = 1; K <= nk; K++) {
-    size_t q = ix3(d, K, J, I);
-    double p = d.P2[q], th = d.THETA2[q];
-    double t = th * pow(p / d.press0, d.kappa);
+  double th = d.THETA[q], p = d.P3[q];
+  double t = th * pow(p / d.press0, d.kappa);
+  double rho3 = p * d.mu3 / (R_GAS * t);
+  d.T3[q] = t;
+  d.EXNER3[q] = d.cp * t / th;
+  d.RHO3[q] = rho3;
+  if (K >= 1) {
+    double p2 = d.P2[q], th2 = d.THETA2[q];
+    double t2 = th2 * pow(p2 / d.press0, d.kappa);
+    d.T2[q] = t2;
+    d.RHO2[q] = p2 * d.mu2 / (R_GAS * t2);
+  } else {
+    const size_t q1 = ix3(d, 1, J, I);
+    double p21 = d.P2[q1], th21 = d.THETA2[q1];
+    double t21 = th21 * pow(p21 / d.press0, d.kappa);
+    double rho21 = p21 * d.mu2 / (R_GAS * t21);
+    d.RHO2[q] = rho3 * rho3 / rho21;
+    d.THETA2[q] = d.sgth[0];
     d.T2[q] = t;
-    d.RHO2[q] = p * d.mu2 / (R_GAS * t);
   }
-  size_t q0 = ix3(d, 0, J, I);
-  d.RHO2[q0] = d.RHO3[q0] * d.RHO3[q0] / d.RHO2[ix3(d, 1, J, I)];
-  d.THETA2[q0] = d.sgth[0];
-  d.T2[q0] = d.T3[q0];
 }
 void L_thermo(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_thermo, grid2(b.nj + 1, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  LAUNCH(k_thermo, grid2(b.nj + 1, b.ni + 2, b.nk + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
 // kinetic energy of a zonally uniform u row (get_kin with v = 0)
-__device__ double kin_uniform(const MemberDesc &d, const double *u1, int J) {
-  const double *mn = d.mn, *mni = d.mn_inv;
+__device__ double kin_uniform(const double *mn, const double *mni, int nj, const double *u1, int J) {
   int jj = 2 * J;
   double u2 = u1[J], u4 = u1[J], v2 = 0., v4 = 0.;
   double kc = (u2 * u2 + u4 * u4 + mn[jj + 1] * (v2 * v2 * mni[jj] + v4 * v4 * mni[jj + 2])) * .25;
-  if (J == 0 || J == d.nj) return kc;
+  if (J == 0 || J == nj) return kc;
   u2 = .5 * (u1[J + 1] + u1[J - 1]);
   u4 = .5 * (u1[J + 1] + u1[J - 1]);
   v2 = .5 * (0. + 0.);
@@ -203,130 +258,168 @@ __device__ double kin_uniform(const MemberDesc &d, const double *u1, int J) {
 }
 
 // Gas giants: GZ_SURFACE from gradient balance with the zonal-mean bottom
-// wind.  Zonal means are sequential sums, as in the reference.
-__global__ void k_gz_from_u(const MemberDesc *D, int mb) {
+// wind, in three launches: the zonal means of u, p, rho on the bottom
+// interface (one wavefront per row; lanes 0-2 sum in the reference's
+// sequential order), the meridional integral (one block), the fills.
+__global__ void k_gzu_means(const MemberDesc *D, int mb) {
+  MEMBER;
+  if (!d.is_gas) return;
+  extern __shared__ double row[];
+  const int J = blockIdx.x, ni = d.ni, nk = d.nk, R = d.nj + 3;
+  for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+    const size_t q = ix3(d, nk, J, i + 1);
+    row[i] = d.U[q];
+    row[ni + i] = d.P3[q];
+    row[2 * ni + i] = d.RHO3[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const double *v = row + threadIdx.x * ni;
+    double sum = 0.;
+    for (int i = 0; i < ni; i++) sum += v[i];
+    d.rowbuf[threadIdx.x * R + J] = sum / ni;
+  }
+}
+__global__ void k_gzu_core(const MemberDesc *D, int mb) {
   MEMBER;
   if (!d.is_gas) return;
   extern __shared__ double sh[];
-  const int nj = d.nj, ni = d.ni, nk = d.nk;
-  double *u1 = sh, *p1 = sh + (nj + 2), *r1 = sh + 2 * (nj + 2), *pvr = sh + 3 * (nj + 2), *gz1 = sh + 4 * (nj + 3);
+  const int nj = d.nj, ni = d.ni, R = nj + 3, NJ = 2 * (nj + 1) + 1;
+  double *u1 = sh, *p1 = sh + R, *r1 = sh + 2 * R, *pvr = sh + 3 * R, *pvh = sh + 4 * R, *bern = sh + 5 * R;
+  double *kn = sh + 6 * R;
+  // the metric columns the serial integral reads, staged once
+  double *f = sh + 7 * R, *m = f + NJ, *n = m + NJ, *mn = n + NJ, *mni = mn + NJ;
   for (int J = threadIdx.x; J <= nj; J += blockDim.x) {
-    double su = 0., sp = 0., sr = 0.;
-    for (int I = 1; I <= ni; I++) {
-      su += d.U[ix3(d, nk, J, I)];
-      sp += d.P3[ix3(d, nk, J, I)];
-      sr += d.RHO3[ix3(d, nk, J, I)];
-    }
-    u1[J] = su / ni;
-    p1[J] = sp / ni;
-    r1[J] = sr / ni;
+    u1[J] = d.rowbuf[J];
+    p1[J] = d.rowbuf[R + J];
+    r1[J] = d.rowbuf[2 * R + J];
+  }
+  for (int jj = threadIdx.x; jj < NJ; jj += blockDim.x) {
+    f[jj] = d.f[jj];
+    m[jj] = d.m[jj];
+    n[jj] = d.n[jj];
+    mn[jj] = d.mn[jj];
+    mni[jj] = d.mn_inv[jj];
   }
   __syncthreads();
-  // absolute vorticity of the uniform row field, stored in PV3(nk+1)
-  double *pv = d.PV3 + (size_t)(nk + 1) * d.N2;
+  // absolute vorticity of the uniform row field
   for (int J = 1 + threadIdx.x; J <= nj; J += blockDim.x) {
-    double f2j = d.f[2 * J], m2j = d.m[2 * J], n2j = d.n[2 * J];
-    double mm1i = 1. / d.m[2 * J - 1], mp1i = 1. / d.m[2 * J + 1];
+    double f2j = f[2 * J], m2j = m[2 * J], n2j = n[2 * J];
+    double mm1i = 1. / m[2 * J - 1], mp1i = 1. / m[2 * J + 1];
     double ze = m2j * ((0. - 0.) + n2j * (u1[J - 1] * mm1i - u1[J] * mp1i));
     pvr[J] = (ze + f2j);
   }
   if (threadIdx.x == 0) {
     double ze = 0.;
     for (int I = 1; I <= ni; I++) ze -= u1[0];
-    ze *= d.mn[0] / ((d.m[1]) * (double)(ni));
-    pvr[0] = ze + d.f[0];
-    ze = 0.;
+    ze *= mn[0] / ((m[1]) * (double)(ni));
+    pvr[0] = ze + f[0];
+  } else if (threadIdx.x == 64) {
+    double ze = 0.;
     for (int I = 1; I <= ni; I++) ze += u1[nj];
-    ze *= d.mn[2 * (nj + 1)] / (d.m[2 * nj + 1] * (double)(ni));
-    pvr[nj + 1] = ze + d.f[2 * (nj + 1)];
+    ze *= mn[2 * (nj + 1)] / (m[2 * nj + 1] * (double)(ni));
+    pvr[nj + 1] = ze + f[2 * (nj + 1)];
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < (nj + 2) * (ni + 2); t += blockDim.x) {
-    int J = t / (ni + 2), I = t % (ni + 2);
-    pv[ix2(d, J, I)] = pvr[J];
-  }
-  if (threadIdx.x == 0) {
-    const double PVC = 1. / 12.;
-    const int j0 = d.jtp;
-    double pvh_prev = 0.;
-    // pvhudy[J], then the meridional integral from j0 both ways
-    double *pvh = gz1;  // reuse as pvhudy[0..nj+1]
-    for (int J = 0; J <= nj + 1; J++) pvh[J] = 0.;
-    for (int J = 1; J <= nj; J++) {
-      double udyJ = u1[J] / d.n[2 * J + 1], udyM = u1[J - 1] / d.n[2 * J - 1];
+  for (int J = threadIdx.x; J <= nj + 1; J += blockDim.x) {
+    double v = 0.;
+    if (J >= 1 && J <= nj) {
+      const double PVC = 1. / 12.;
+      double udyJ = u1[J] / n[2 * J + 1], udyM = u1[J - 1] / n[2 * J - 1];
       double ga = (pvr[J] + pvr[J] + pvr[J + 1]);
       double de = (pvr[J] + pvr[J + 1] + pvr[J]);
       double al = (pvr[J - 1] + pvr[J] + pvr[J]);
       double be = (pvr[J] + pvr[J] + pvr[J - 1]);
-      pvh[J] = (ga * udyJ + de * udyJ + al * udyM + be * udyM) * PVC;
-      pvh[J] += (p1[J] - p1[J - 1]) * 2. / (r1[J] + r1[J - 1]);
+      v = (ga * udyJ + de * udyJ + al * udyM + be * udyM) * PVC;
+      v += (p1[J] - p1[J - 1]) * 2. / (r1[J] + r1[J - 1]);
     }
-    (void)pvh_prev;
-    double *bern = sh + 5 * (nj + 3);
-    double kin0 = 0.;
-    if (j0 >= 0 && j0 <= nj) kin0 = kin_uniform(d, u1, j0);
-    bern[j0] = 0. + kin0;
-    for (int J = j0 + 1; J <= nj; J++) bern[J] = bern[J - 1] - pvh[J];
-    for (int J = j0 - 1; J >= 0; J--) bern[J] = bern[J + 1] + pvh[J + 1];
-    for (int J = 0; J <= nj; J++) bern[J] = bern[J] - kin_uniform(d, u1, J);
+    pvh[J] = v;
+    if (J <= nj) kn[J] = kin_uniform(mn, mni, nj, u1, J);
   }
   __syncthreads();
-  double *gzr = sh + 5 * (nj + 3);
-  for (int t = threadIdx.x; t < (nj + 1) * (ni + 2); t += blockDim.x) {
-    int J = t / (ni + 2), I = t % (ni + 2);
-    d.GZS[ix2(d, J, I)] = gzr[J];
+  if (threadIdx.x == 0) {
+    // Bernoulli function integrated from the tie-point row both ways
+    const int j0 = d.jtp;
+    bern[j0] = 0. + kn[j0];
+    for (int J = j0 + 1; J <= nj; J++) bern[J] = bern[J - 1] - pvh[J];
+    for (int J = j0 - 1; J >= 0; J--) bern[J] = bern[J + 1] + pvh[J + 1];
   }
+  __syncthreads();
+  for (int J = threadIdx.x; J <= nj + 1; J += blockDim.x) {
+    d.rowbuf[3 * R + J] = pvr[J];
+    if (J <= nj) d.rowbuf[4 * R + J] = bern[J] - kn[J];
+  }
+}
+// PV3 plane nk+1 holds the absolute vorticity of the uniform rows; GZ_SURFACE
+__global__ void k_gzu_fill(const MemberDesc *D, int mb) {
+  MEMBER;
+  if (!d.is_gas) return;
+  FLAT2(0, d.nj + 2, 0, d.ni + 2);
+  const int R = d.nj + 3;
+  d.PV3[(size_t)(d.nk + 1) * d.N2 + ix2(d, J, I)] = d.rowbuf[3 * R + J];
+  if (J <= d.nj) d.GZS[ix2(d, J, I)] = d.rowbuf[4 * R + J];
 }
 void L_gz_from_u(hipStream_t s, const Bucket &b) {
-  size_t shm = sizeof(double) * 6 * (b.nj + 3);
-  LAUNCH(k_gz_from_u, dim3(1, 1, b.nm), dim3(256), shm, s, b.D, b.mb);
+  LAUNCH(k_gzu_means, dim3(b.nj + 1, 1, b.nm), dim3(64), sizeof(double) * 3 * b.ni, s, b.D, b.mb);
+  LAUNCH(k_gzu_core, dim3(1, 1, b.nm), dim3(256), sizeof(double) * (7 * (b.nj + 3) + 5 * (2 * b.nj + 3)), s, b.D,
+         b.mb);
+  LAUNCH(k_gzu_fill, grid2(b.nj + 2, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
-// hydrostatic GZ3 and MONT3 (Leslie-Purser 4-point integral in ln p) and the
-// monotonicity screen
+// hydrostatic GZ3 and MONT3 from the bottom up (Leslie-Purser 4-point
+// integral in ln p) and the monotonicity screen
 __global__ void k_gz3(const MemberDesc *D, int mb) {
   MEMBER;
-  FLAT2(0, d.nj + 1, 0, d.ni + 2);
-  const int nk = d.nk;
-  const double cpr = d.cpr, rgas = d.rgas;
-  double *G = d.GZ3, *M = d.MONT3;
-  const double *P = d.P3, *RH = d.RHO3, *T = d.T3;
+  extern __shared__ double lds[];
+  const Tile T = tile_of(d);
+  const int nk = d.nk, W = nk + 1, J = T.J, I = T.I;
+  double *inc = lds + T.c * W, *tt = lds + COLS * W + T.c * W;
+  const double *P = d.P3, *RH = d.RHO3;
+  if (T.live)
+    for (int K = T.kk; K <= nk; K += T.KS) tt[K] = d.T3[ix3(d, K, J, I)];
 #define PP(k) P[ix3(d, k, J, I)]
 #define RR(k) RH[ix3(d, k, J, I)]
-  G[ix3(d, nk, J, I)] = d.topo_scale * d.GZS[ix2(d, J, I)];
-  M[ix3(d, nk, J, I)] = G[ix3(d, nk, J, I)] + cpr * T[ix3(d, nk, J, I)] * rgas;
-  int K = nk;
-  double gzk = G[ix3(d, K, J, I)];
-  double gzm = gzk + ((PP(K) + PP(K - 1)) / (RR(K) + RR(K - 1))) * log(PP(K) / PP(K - 1));
-  G[ix3(d, K - 1, J, I)] = gzm;
-  M[ix3(d, K - 1, J, I)] = gzm + cpr * T[ix3(d, K - 1, J, I)] * rgas;
-  int bad = 0;
-  if (I >= 1 && I <= d.ni && gzk >= gzm) bad = 1;
-  for (K = nk - 1; K >= 2; K--) {
-    double dz10 = log(PP(K + 1) / PP(K)), dz21 = log(PP(K) / PP(K - 1)), dz32 = log(PP(K - 1) / PP(K - 2));
-    double dz20 = dz21 + dz10, dz30 = dz32 + dz20, dz31 = dz32 + dz21;
-    double df0 = PP(K + 1) / RR(K + 1), df1 = PP(K) / RR(K), df2 = PP(K - 1) / RR(K - 1), df3 = PP(K - 2) / RR(K - 2);
-    double dgz = -(df0 * (dz21 * dz21 * dz21 * (dz31 + dz32)) / (dz10 * dz20 * dz30) +
-                   df1 * (dz21 * (dz21 * (dz21 + 2. * (dz10 - dz31)) - 6. * dz10 * dz31)) / (dz10 * dz31) +
-                   df2 * (dz21 * (dz21 * (dz21 + 2. * (dz32 - dz20)) - 6. * dz20 * dz32)) / (dz20 * dz32) +
-                   df3 * (dz21 * dz21 * dz21 * (dz10 + dz20)) / (dz30 * dz31 * dz32)) / 12.;
-    double gn = gzm + dgz;
-    G[ix3(d, K - 1, J, I)] = gn;
-    M[ix3(d, K - 1, J, I)] = gn + cpr * T[ix3(d, K - 1, J, I)] * rgas;
-    if (I >= 1 && I <= d.ni && gzm >= gn) bad = 1;
-    gzm = gn;
-  }
-  K = 1;
-  double g0 = gzm + ((PP(K) + PP(K - 1)) / (RR(K) + RR(K - 1))) * log(PP(K) / PP(K - 1));
-  G[ix3(d, 0, J, I)] = g0;
-  M[ix3(d, 0, J, I)] = g0 + cpr * T[ix3(d, 0, J, I)] * rgas;
-  if (I >= 1 && I <= d.ni && gzm >= g0) bad = 1;
-  if (bad) atomicOr(&d.dyn->err, 1);
+  if (T.live)
+    for (int K = 1 + T.kk; K <= nk; K += T.KS) {
+      double v;
+      if (K == nk || K == 1) {
+        v = ((PP(K) + PP(K - 1)) / (RR(K) + RR(K - 1))) * log(PP(K) / PP(K - 1));
+      } else {
+        double dz10 = log(PP(K + 1) / PP(K)), dz21 = log(PP(K) / PP(K - 1)), dz32 = log(PP(K - 1) / PP(K - 2));
+        double dz20 = dz21 + dz10, dz30 = dz32 + dz20, dz31 = dz32 + dz21;
+        double df0 = PP(K + 1) / RR(K + 1), df1 = PP(K) / RR(K), df2 = PP(K - 1) / RR(K - 1),
+               df3 = PP(K - 2) / RR(K - 2);
+        v = -(df0 * (dz21 * dz21 * dz21 * (dz31 + dz32)) / (dz10 * dz20 * dz30) +
+              df1 * (dz21 * (dz21 * (dz21 + 2. * (dz10 - dz31)) - 6. * dz10 * dz31)) / (dz10 * dz31) +
+              df2 * (dz21 * (dz21 * (dz21 + 2. * (dz32 - dz20)) - 6. * dz20 * dz32)) / (dz20 * dz32) +
+              df3 * (dz21 * dz21 * dz21 * (dz10 + dz20)) / (dz30 * dz31 * dz32)) /
+            12.;
+      }
+      inc[K] = v;
+    }
 #undef PP
 #undef RR
+  __syncthreads();
+  if (T.live && T.kk == 0) {
+    const double cpr = d.cpr, rgas = d.rgas;
+    double *G = d.GZ3, *M = d.MONT3;
+    double gprev = d.topo_scale * d.GZS[ix2(d, J, I)];
+    G[ix3(d, nk, J, I)] = gprev;
+    M[ix3(d, nk, J, I)] = gprev + cpr * tt[nk] * rgas;
+    const bool inner = (I >= 1 && I <= d.ni);
+    int bad = 0;
+    for (int K = nk; K >= 1; K--) {
+      double gn = gprev + inc[K];
+      G[ix3(d, K - 1, J, I)] = gn;
+      M[ix3(d, K - 1, J, I)] = gn + cpr * tt[K - 1] * rgas;
+      if (inner && gprev >= gn) bad = 1;
+      gprev = gn;
+    }
+    if (bad) atomicOr(&d.dyn->err, 1);
+  }
 }
 void L_gz3(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_gz3, grid2(b.nj + 1, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  LAUNCH(k_gz3, tile_grid(b), dim3(TPB), sizeof(double) * 2 * COLS * (b.nk + 1), s, b.D, b.mb);
 }
 
 // finish DZDT3 (diagnostic vertical velocity)
@@ -352,6 +445,8 @@ __global__ void k_dzdt3(const MemberDesc *D, int mb) {
        d.U[q] * (G[q] - G[ix3(d, K, J, I - 1)]) * gx3 +
        d.U[ix3(d, K, J, I + 1)] * (G[ix3(d, K, J, I + 1)] - G[q]) * gx3;
   d.DZDT3[q] = z;
+  if (I == d.ni) d.DZDT3[ix3(d, K, J, 0)] = z;
+  if (I == 1) d.DZDT3[ix3(d, K, J, d.ni + 1)] = z;
 }
 void L_dzdt3(hipStream_t s, const Bucket &b) {
   LAUNCH(k_dzdt3, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
@@ -420,63 +515,80 @@ void L_heating(hipStream_t s, const Bucket &b, int adjust) {
 }
 
 // ------------------------------------------------------------ calc_w
-// One lane per column.  SUM_DIV(1) and SUM_DIV(nk) are zero (the
+// Column tiles.  Per level: the theta heating of source_sink
+// (src/shared/epic_funcs_diag.c:4768-4778, with_source), the heating part
+// of W3, the Leslie-Purser increment of SUM_DIV and dF/dp; then the running
+// SUM_DIV in one lane per column.  SUM_DIV(1) and SUM_DIV(nk) are zero (the
 // reference's top/bottom statements write stale-index pad cells), so the
-// dF/dpbot term vanishes and is not formed.  With with_source the theta
-// heating of source_sink (src/shared/epic_funcs_diag.c:4768-4778) is applied
-// first.
+// dF/dpbot term vanishes and is not formed.
 __global__ void k_calc_w(const MemberDesc *D, int mb, int with_source) {
   MEMBER;
-  FLAT2(0, d.nj + 1, 0, d.ni + 2);
-  const int nk = d.nk, ks = d.ks;
+  extern __shared__ double lds[];
+  const Tile T = tile_of(d);
+  const int nk = d.nk, ks = d.ks, W = nk + 1, J = T.J, I = T.I;
+  double *dsv = lds + T.c * W, *dfv = lds + COLS * W + T.c * W, *wbv = lds + 2 * COLS * W + T.c * W;
   const double *P = d.P3;
-  double *W = d.W3, *TH = d.THETA;
-  if (with_source)
-    for (int K = ks - 1; K <= nk; K++) {
-      size_t q = ix3(d, K, J, I);
-      TH[q] += d.dt * d.HEAT3[q] / d.EXNER3[q];
-    }
-  W[ix3(d, 0, J, I)] = 0.;
-  W[ix3(d, nk, J, I)] = 0.;
-  const double ptop = P[ix3(d, 0, J, I)], pbot = P[ix3(d, nk, J, I)], psig = P[ix3(d, ks - 1, J, I)];
-  for (int K = 1; K < ks - 1; K++) {
-    size_t q = ix3(d, K, J, I);
-    double sigma = get_sigma(pbot, P[q], ptop);
-    W[q] = g_sig(d, sigma) * d.HEAT3[q] / d.EXNER3[q];
-  }
-  for (int K = ks - 1; K < nk; K++) W[ix3(d, K, J, I)] = 0.;
-  double S = 0.;
+  double *TH = d.THETA;
   int bad = 0;
-  const double eps = d.eps13;
+  if (T.live) {
+    const double ptop = P[ix3(d, 0, J, I)], pbot = P[ix3(d, nk, J, I)], psig = P[ix3(d, ks - 1, J, I)];
+    const double eps = d.eps13;
 #define PP(k) P[ix3(d, k, J, I)]
 #define DV(k) d.DIV[ix3(d, k, J, I)]
-  for (int K = 1; K < nk; K++) {
-    if (K >= 2) {
-      double dz10 = log(PP(K - 1) / PP(K - 2)), dz21 = log(PP(K) / PP(K - 1)), dz32 = log(PP(K + 1) / PP(K));
-      double dz20 = dz21 + dz10, dz30 = dz32 + dz20, dz31 = dz32 + dz21;
-      double df0 = PP(K - 2) * DV(K - 2), df1 = PP(K - 1) * DV(K - 1), df2 = PP(K) * DV(K), df3 = PP(K + 1) * DV(K + 1);
-      double ds = -(df0 * (dz21 * dz21 * dz21 * (dz31 + dz32)) / (dz10 * dz20 * dz30) +
-                    df1 * (dz21 * (dz21 * (dz21 + 2. * (dz10 - dz31)) - 6. * dz10 * dz31)) / (dz10 * dz31) +
-                    df2 * (dz21 * (dz21 * (dz21 + 2. * (dz32 - dz20)) - 6. * dz20 * dz32)) / (dz20 * dz32) +
-                    df3 * (dz21 * dz21 * dz21 * (dz10 + dz20)) / (dz30 * dz31 * dz32)) / 12.;
-      S = S + ds;
+    for (int K = 1 + T.kk; K <= nk; K += T.KS) {
+      const size_t q = ix3(d, K, J, I);
+      double th = TH[q];
+      if (with_source && K >= ks - 1) {
+        th += d.dt * d.HEAT3[q] / d.EXNER3[q];
+        TH[q] = th;
+      }
+      if (K == nk) continue;
+      double wb = 0.;
+      if (K < ks - 1) {
+        double sigma = get_sigma(pbot, P[q], ptop);
+        wb = g_sig(d, sigma) * d.HEAT3[q] / d.EXNER3[q];
+      }
+      double ds = 0.;
+      if (K >= 2) {
+        double dz10 = log(PP(K - 1) / PP(K - 2)), dz21 = log(PP(K) / PP(K - 1)), dz32 = log(PP(K + 1) / PP(K));
+        double dz20 = dz21 + dz10, dz30 = dz32 + dz20, dz31 = dz32 + dz21;
+        double df0 = PP(K - 2) * DV(K - 2), df1 = PP(K - 1) * DV(K - 1), df2 = PP(K) * DV(K),
+               df3 = PP(K + 1) * DV(K + 1);
+        ds = -(df0 * (dz21 * dz21 * dz21 * (dz31 + dz32)) / (dz10 * dz20 * dz30) +
+               df1 * (dz21 * (dz21 * (dz21 + 2. * (dz10 - dz31)) - 6. * dz10 * dz31)) / (dz10 * dz31) +
+               df2 * (dz21 * (dz21 * (dz21 + 2. * (dz32 - dz20)) - 6. * dz20 * dz32)) / (dz20 * dz32) +
+               df3 * (dz21 * dz21 * dz21 * (dz10 + dz20)) / (dz30 * dz31 * dz32)) /
+             12.;
+      }
+      double p3 = P[q], pb3 = pbot;
+      double dp = p3 * eps;
+      double p2 = dmax(p3 - dp * .5, ptop), p4 = dmin(p3 + dp * .5, pbot);
+      if (p2 < psig && p4 >= psig) p2 = p3;
+      double dFdp = 0.;
+      if (p2 < p4) dFdp = (sgth_of(d, th, p2, pb3, ptop, &bad) - sgth_of(d, th, p4, pb3, ptop, &bad)) / (p2 - p4);
+      else bad = 1;
+      dsv[K] = ds;
+      dfv[K] = dFdp;
+      wbv[K] = wb;
     }
-    size_t q = ix3(d, K, J, I);
-    double th = TH[q], p3 = P[q], pb3 = pbot;
-    double dp = p3 * eps;
-    double p2 = dmax(p3 - dp * .5, ptop), p4 = dmin(p3 + dp * .5, pbot);
-    if (p2 < psig && p4 >= psig) p2 = p3;
-    double dFdp = 0.;
-    if (p2 < p4) dFdp = (sgth_of(d, th, p2, pb3, ptop, &bad) - sgth_of(d, th, p4, pb3, ptop, &bad)) / (p2 - p4);
-    else bad = 1;
-    W[q] -= S * dFdp;
-  }
 #undef PP
 #undef DV
+  }
+  __syncthreads();
+  if (T.live && T.kk == 0) {
+    double *Wf = d.W3;
+    Wf[ix3(d, 0, J, I)] = 0.;
+    Wf[ix3(d, nk, J, I)] = 0.;
+    double S = 0.;
+    for (int K = 1; K < nk; K++) {
+      if (K >= 2) S = S + dsv[K];
+      Wf[ix3(d, K, J, I)] = wbv[K] - S * dfv[K];
+    }
+  }
   if (bad) atomicOr(&d.dyn->err, 2);
 }
 void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
-  LAUNCH(k_calc_w, grid2(b.nj + 1, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb, with_source);
+  LAUNCH(k_calc_w, tile_grid(b), dim3(TPB), sizeof(double) * 3 * COLS * (b.nk + 1), s, b.D, b.mb, with_source);
 }
 
 // ------------------------------------------------ vorticity / divergence
@@ -501,37 +613,48 @@ __global__ void k_vort_interior(const MemberDesc *D, int mb, int uf, int vf, int
   if (I == d.ni) pv[ix2(d, J, 0)] = v;
   if (I == 1) pv[ix2(d, J, d.ni + 1)] = v;
 }
-// pole PV from the circulation of the adjacent u-row, one lane per (pole, K)
-__global__ void k_vort_poles(const MemberDesc *D, int mb, int uf, int k0, int nK) {
+// pole PV from the circulation of the adjacent u-row: one wavefront per
+// (pole, K); the row is staged in LDS and lanes 0/1 form the two zonal sums
+// in the reference's sequential order
+__global__ void k_vort_poles(const MemberDesc *D, int mb, int uf, int k0) {
   MEMBER;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * nK) return;
-  const int K = k0 + t / 2, north = t & 1;
+  extern __shared__ double row[];
+  const int K = k0 + (blockIdx.x >> 1), north = blockIdx.x & 1;
   const double *uu = fld_ptr(d, uf) + (size_t)K * d.N2;
   const double *hh = d.H3 + (size_t)K * d.N2;
   double *pv = d.PV3 + (size_t)K * d.N2;
-  const int ni = d.ni, nj = d.nj;
-  double ze = 0., h = 0., val;
-  if (!north) {
-    for (int I = 1; I <= ni; I++) ze -= uu[ix2(d, 0, I)];
-    ze *= d.mn[0] / ((d.m[1]) * (double)(ni));
-    for (int I = 1; I <= ni; I++) h += hh[ix2(d, 0, I)];
-    h /= (double)(ni);
-    val = (ze + d.f[0]) / h;
-    for (int I = 0; I <= ni + 1; I++) pv[ix2(d, 0, I)] = val;
-  } else {
-    for (int I = 1; I <= ni; I++) ze += uu[ix2(d, nj, I)];
-    ze *= d.mn[2 * (nj + 1)] / (d.m[2 * nj + 1] * (double)(ni));
-    for (int I = 1; I <= ni; I++) h += hh[ix2(d, nj, I)];
-    h /= (double)(ni);
-    val = (ze + d.f[2 * (nj + 1)]) / h;
-    for (int I = 0; I <= ni + 1; I++) pv[ix2(d, nj + 1, I)] = val;
+  const int ni = d.ni, nj = d.nj, Jr = north ? nj : 0;
+  for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+    row[i] = uu[ix2(d, Jr, i + 1)];
+    row[ni + i] = hh[ix2(d, Jr, i + 1)];
   }
+  __syncthreads();
+  double *res = row + 2 * ni;
+  if (threadIdx.x == 0) {
+    double ze = 0.;
+    if (!north) {
+      for (int i = 0; i < ni; i++) ze -= row[i];
+      ze *= d.mn[0] / ((d.m[1]) * (double)(ni));
+    } else {
+      for (int i = 0; i < ni; i++) ze += row[i];
+      ze *= d.mn[2 * (nj + 1)] / (d.m[2 * nj + 1] * (double)(ni));
+    }
+    res[0] = ze;
+  } else if (threadIdx.x == 1) {
+    double h = 0.;
+    for (int i = 0; i < ni; i++) h += row[ni + i];
+    h /= (double)(ni);
+    res[1] = h;
+  }
+  __syncthreads();
+  const double val = (res[0] + (north ? d.f[2 * (nj + 1)] : d.f[0])) / res[1];
+  const int Jd = north ? nj + 1 : 0;
+  for (int I = threadIdx.x; I <= ni + 1; I += blockDim.x) pv[ix2(d, Jd, I)] = val;
 }
 void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1) {
   int nK = k1 - k0 + 1;
   LAUNCH(k_vort_interior, grid2(b.nj, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb, uu, vv, k0);
-  LAUNCH(k_vort_poles, dim3((2 * nK + 63) / 64, 1, b.nm), dim3(64), 0, s, b.D, b.mb, uu, k0, nK);
+  LAUNCH(k_vort_poles, dim3(2 * nK, 1, b.nm), dim3(64), sizeof(double) * (2 * b.ni + 2), s, b.D, b.mb, uu, k0);
 }
 
 __global__ void k_divergence(const MemberDesc *D, int mb, int uf, int vf, int df, int k0) {

@@ -231,17 +231,24 @@ void L_uv_drag(hipStream_t s, const Bucket &b) {
 }
 
 // ------------------------------------------------------------- pgrad
-// GZ3 rows -1 and nj+1 take the zonal mean of the adjacent row, K = 0..nk
+// GZ3 rows -1 and nj+1 take the zonal mean of the adjacent row, K = 0..nk:
+// one wavefront per (K, pole), the row staged in LDS and summed in order
 __global__ void k_gz_polefill(const MemberDesc *D, int mb) {
   MEMBER;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * (d.nk + 1)) return;
-  const int K = t >> 1, north = t & 1;
+  extern __shared__ double row[];
+  const int K = blockIdx.x >> 1, north = blockIdx.x & 1, ni = d.ni;
   const int Js = north ? d.nj : 0, Jd = north ? d.nj + 1 : -1;
-  double tmp = 0.;
-  for (int I = 1; I <= d.ni; I++) tmp += d.GZ3[ix3(d, K, Js, I)];
-  tmp /= d.ni;
-  for (int I = 0; I <= d.ni + 1; I++) d.GZ3[ix3(d, K, Jd, I)] = tmp;
+  for (int i = threadIdx.x; i < ni; i += blockDim.x) row[i] = d.GZ3[ix3(d, K, Js, i + 1)];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tmp = 0.;
+    for (int i = 0; i < ni; i++) tmp += row[i];
+    tmp /= ni;
+    row[ni] = tmp;
+  }
+  __syncthreads();
+  const double v = row[ni];
+  for (int I = threadIdx.x; I <= ni + 1; I += blockDim.x) d.GZ3[ix3(d, K, Jd, I)] = v;
 }
 
 // finite-volume pressure-gradient force.  Longitudinal: the east face of
@@ -391,7 +398,7 @@ __global__ void k_pgrad(const MemberDesc *D, int mb) {
 }
 
 void L_pgrad(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_gz_polefill, dim3((2 * (b.nk + 1) + 63) / 64, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
+  LAUNCH(k_gz_polefill, dim3(2 * (b.nk + 1), 1, b.nm), dim3(64), sizeof(double) * (b.ni + 1), s, b.D, b.mb);
   LAUNCH(k_pgrad, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
@@ -439,8 +446,12 @@ __global__ void k_ab(const MemberDesc *D, int mb) {
       ab0 = 23. / 12.; ab1 = -16. / 12.; ab2 = 5. / 12.;
     }
     size_t q = ix3(d, K, J, I);
-    val[q] += d.dt * (ab0 * tb[(size_t)dy.it0 * d.N3 + q] + ab1 * tb[(size_t)dy.itm1 * d.N3 + q] +
-                      ab2 * tb[(size_t)dy.itm2 * d.N3 + q]);
+    const double y = val[q] + d.dt * (ab0 * tb[(size_t)dy.it0 * d.N3 + q] + ab1 * tb[(size_t)dy.itm1 * d.N3 + q] +
+                                      ab2 * tb[(size_t)dy.itm2 * d.N3 + q]);
+    val[q] = y;
+    // zonal pads (bc_lateral after the update)
+    if (I == d.ni) val[ix3(d, K, J, 0)] = y;
+    if (I == 1) val[ix3(d, K, J, d.ni + 1)] = y;
   }
 }
 // rotate IT_ZERO <- IT_MINUS2 <- IT_MINUS1 <- IT_ZERO and advance itime
@@ -456,8 +467,6 @@ __global__ void k_rotate(const MemberDesc *D, int mb) {
 }
 void L_ab(hipStream_t s, const Bucket &b) {
   LAUNCH(k_ab, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
-  L_bc3(s, b, F_U);
-  L_bc3(s, b, F_V);
 }
 void L_rotate(hipStream_t s, const Bucket &b) {
   LAUNCH(k_rotate, dim3(1, 1, b.nm), dim3(64), 0, s, b.D, b.mb);

@@ -13,7 +13,7 @@
 //
 // The 3-D form removes a PCHIP profile of the variable against the filtered
 // geopotential before filtering and adds it back afterwards, on every row
-// (src/shared/epic_adjust.c:258-397, 463-474).
+// (src/shared/epic_adjust.c:258-397, 463-474), fused with the transform.
 #include "kernels.h"
 
 namespace epic {
@@ -22,15 +22,39 @@ namespace epic {
 
 __device__ __forceinline__ int brev(int k, int bits) { return bits ? (int)(__brev((unsigned)k) >> (32 - bits)) : 0; }
 
+// The lane's twiddles for every stage are loaded into registers before the
+// transforms start (one round of loads instead of one per stage).
+constexpr int MAX_STAGES = 9;  // n <= 1024: stages with 2 << s < n
+
+struct LaneTw {
+  double wr[MAX_STAGES], wi[MAX_STAGES];
+  double sr, si;  // split-step twiddle of this lane
+};
+
+__device__ __forceinline__ void load_tw(LaneTw &t, const double *tw, const double *tr, int n, int lane) {
+#pragma unroll
+  for (int s = 0; s < MAX_STAGES; s++) {
+    const int half = 1 << s, mm = lane & (half - 1);
+    const bool on = (2 << s) < n;
+    t.wr[s] = on ? tw[2 * (half - 1 + mm)] : 0.;
+    t.wi[s] = on ? tw[2 * (half - 1 + mm) + 1] : 0.;
+  }
+  const bool on = lane < n / 4 - 1;
+  t.sr = on ? tr[2 * lane] : 0.;
+  t.si = on ? tr[2 * lane + 1] : 0.;
+}
+
 // complex radix-2 stages on buf[0..n) (n doubles = n/2 complex), one
-// butterfly per lane; tw holds (wr,wi) of stage s at [(2^s - 1) + mm]
-__device__ __forceinline__ void four1_stages(double *buf, int n, int lane, const double *tw, bool active) {
-  for (int s = 0; (2 << s) < n; s++) {
+// butterfly per lane, in the reference's stage order
+__device__ __forceinline__ void four1_stages(double *buf, int n, int lane, const LaneTw &t, bool active) {
+#pragma unroll
+  for (int s = 0; s < MAX_STAGES; s++) {
+    if ((2 << s) >= n) break;
     const int mmax = 2 << s, half = 1 << s;
     if (active) {
       int g = lane >> s, mm = lane & (half - 1);
       int i0 = 2 * mm + g * 2 * mmax, j0 = i0 + mmax;
-      double wr = tw[2 * (half - 1 + mm)], wi = tw[2 * (half - 1 + mm) + 1];
+      double wr = t.wr[s], wi = t.wi[s];
       double tr = wr * buf[j0] - wi * buf[j0 + 1];
       double ti = wr * buf[j0 + 1] + wi * buf[j0];
       buf[j0] = buf[i0] - tr;
@@ -43,13 +67,13 @@ __device__ __forceinline__ void four1_stages(double *buf, int n, int lane, const
 }
 
 // split step of the real transform; c2 = -0.5 forward, +0.5 inverse
-__device__ __forceinline__ void split_step(double *buf, int n, int lane, const double *tr, double c2, bool active) {
+__device__ __forceinline__ void split_step(double *buf, int n, int lane, const LaneTw &t, double c2, bool active) {
   const double c1 = 0.5;
   if (active && lane < n / 4 - 1) {
     int i = lane + 2;
     int i1 = i + i - 1, i2 = 1 + i1, i3 = n + 3 - i2, i4 = 1 + i3;
     i1--; i2--; i3--; i4--;
-    double wr = tr[2 * lane], wi = tr[2 * lane + 1];
+    double wr = t.sr, wi = t.si;
     double h1r = c1 * (buf[i1] + buf[i3]);
     double h1i = c1 * (buf[i2] - buf[i4]);
     double h2r = -c2 * (buf[i2] + buf[i4]);
@@ -62,11 +86,9 @@ __device__ __forceinline__ void split_step(double *buf, int n, int lane, const d
   __syncthreads();
 }
 
-__global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgrid, int theta_floor) {
+__global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgrid, int pads) {
   MEMBER;
   extern __shared__ double lds[];
-  // planes below k_sigma-1 are not filtered for theta (uniform per block)
-  if (theta_floor && (int)(k0 + blockIdx.y) < d.ks - 1) return;
   const int n = d.ni, nq = n >> 2, RB = TPB / nq;
   const int rs = threadIdx.x / nq, lane = threadIdx.x - rs * nq;
   const int nrows = vgrid ? d.nrow_v : d.nrow_h;
@@ -78,14 +100,20 @@ __global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgr
   double *buf = lds + (size_t)rs * n;
   const int bits = d.nlogn - 1;
   const double *hl = (vgrid ? d.hl_pv : d.hl_h) + (size_t)J * (n / 2 + 2);
+  LaneTw tf, ti;
+  load_tw(tf, d.tw_f, d.tr_f, n, lane);
+  load_tw(ti, d.tw_i, d.tr_i, n, lane);
+  double gain[4];
   if (active)
-    for (int e = lane; e < n; e += nq) {
+    for (int q = 0, e = lane; q < 4; q++, e += nq) {
+      int gi = (e == 0) ? 1 : (e == 1 ? n / 2 + 1 : (e >> 1) + 1);
+      gain[q] = hl[gi];
       int k = e >> 1, c = e & 1;
       buf[(brev(k, bits) << 1) | c] = a[ix2(d, J, e + 1)];
     }
   __syncthreads();
-  four1_stages(buf, n, lane, d.tw_f, active);
-  split_step(buf, n, lane, d.tr_f, -0.5, active);
+  four1_stages(buf, n, lane, tf, active);
+  split_step(buf, n, lane, tf, -0.5, active);
   if (active && lane == 0) {
     double h1r;
     buf[0] = (h1r = buf[0]) + buf[1];
@@ -93,12 +121,9 @@ __global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgr
   }
   __syncthreads();
   if (active)
-    for (int e = lane; e < n; e += nq) {
-      int gi = (e == 0) ? 1 : (e == 1 ? n / 2 + 1 : (e >> 1) + 1);
-      buf[e] *= hl[gi];
-    }
+    for (int q = 0, e = lane; q < 4; q++, e += nq) buf[e] *= gain[q];
   __syncthreads();
-  split_step(buf, n, lane, d.tr_i, 0.5, active);
+  split_step(buf, n, lane, ti, 0.5, active);
   if (active && lane == 0) {
     double h1r;
     buf[0] = 0.5 * ((h1r = buf[0]) + buf[1]);
@@ -115,39 +140,28 @@ __global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgr
       buf[(brev(k, bits) << 1) | c] = v[q];
     }
   __syncthreads();
-  four1_stages(buf, n, lane, d.tw_i, active);
+  four1_stages(buf, n, lane, ti, active);
   if (active)
-    for (int e = lane; e < n; e += nq) a[ix2(d, J, e + 1)] = buf[e] * (2. / (n));
+    for (int e = lane; e < n; e += nq) {
+      const double y = buf[e] * (2. / (n));
+      a[ix2(d, J, e + 1)] = y;
+      if (pads && e == n - 1) a[ix2(d, J, 0)] = y;
+      if (pads && e == 0) a[ix2(d, J, n + 1)] = y;
+    }
 }
 
-// zonal pads of every row 0..nj+1 of planes k0..k1
-__global__ void k_pads(const MemberDesc *D, int mb, int f, int k0) {
-  MEMBER;
-  int J = blockIdx.x * blockDim.x + threadIdx.x;
-  if (J > d.nj + 1) return;
-  double *a = fld_ptr(d, f) + (size_t)(k0 + blockIdx.y) * d.N2;
-  a[ix2(d, J, 0)] = a[ix2(d, J, d.ni)];
-  a[ix2(d, J, d.ni + 1)] = a[ix2(d, J, 1)];
-}
-
-static int filter_rows(const Bucket &b, int vgrid, const MemberDesc *host_hint) {
-  (void)host_hint;
-  (void)vgrid;
-  return b.nj + 1;  // upper bound on listed rows
-}
-
-static void launch_fft(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int theta_floor) {
+static void launch_fft(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int pads) {
   const int n = b.ni, nq = n / 4, RB = TPB / nq;
-  const int rows = filter_rows(b, vgrid, nullptr);
+  const int rows = b.nj + 1;  // upper bound on the listed rows
   dim3 g((rows + RB - 1) / RB, k1 - k0 + 1, b.nm);
   size_t shm = sizeof(double) * (size_t)RB * n;
-  LAUNCH(k_fft_filter, g, dim3(TPB), shm, s, b.D, b.mb, f, k0, vgrid, theta_floor);
+  LAUNCH(k_fft_filter, g, dim3(TPB), shm, s, b.D, b.mb, f, k0, vgrid, pads);
 }
 
+// mode 0: also refresh the zonal pads of the filtered rows (the producers of
+// every filtered copy write the pads of the rows they fill); mode 1: no pads
 void L_filter(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int mode) {
-  launch_fft(s, b, f, k0, k1, vgrid, 0);
-  if (mode == 0)
-    LAUNCH(k_pads, dim3((b.nj + 2 + 63) / 64, k1 - k0 + 1, b.nm), dim3(64), 0, s, b.D, b.mb, f, k0);
+  launch_fft(s, b, f, k0, k1, vgrid, mode == 0);
 }
 
 // ------------------------------------------------------------- 3-D form
@@ -219,79 +233,164 @@ __device__ __forceinline__ F3 f3_range(const MemberDesc &d, int kind) {
   return r;
 }
 
-// one lane per row: column of lowest bottom GZ (first of equals), the
-// profile table at that column and its PCHIP derivatives
+// one wavefront per row: the column of lowest bottom GZ (the first of
+// equals, as the reference's strict-less scan finds it: each lane keeps the
+// first minimum of its strided subset, ties between lanes go to the lower
+// I), the profile table at that column and its PCHIP derivatives
 __global__ void k_spline_prep(const MemberDesc *D, int mb, int f, int kind) {
   MEMBER;
   F3 r = f3_range(d, kind);
-  const int J = r.jstart + blockIdx.x * blockDim.x + threadIdx.x;
-  if (J > d.nj) return;
-  const int nk = d.nk, KT = nk + 2, n = r.kend - r.kstart + 1;
-  int Imin = 1;
-  double gzmin = 1.7976931348623157e308;
-  for (int I = 1; I <= d.ni; I++) {
+  const int J = r.jstart + blockIdx.x;
+  if (J > d.nj) return;  // uniform per block
+  const int nk = d.nk, KT = nk + 2, n = r.kend - r.kstart + 1, lane = threadIdx.x;
+  __shared__ double sx[KMAX + 2], sy[KMAX + 2], sz[KMAX + 2], mv[64];
+  __shared__ int mi[64];
+  __shared__ int imin_s;
+  double best = 1.7976931348623157e308;
+  int bi = -1;
+  for (int I = 1 + lane; I <= d.ni; I += 64) {
     double gv = d.gz[ix3(d, nk, J, I)];
-    if (gv < gzmin) { Imin = I; gzmin = gv; }
+    if (gv < best) { best = gv; bi = I; }
   }
+  mv[lane] = best;
+  mi[lane] = bi;
+  __syncthreads();
+  if (lane == 0) {
+    int Imin = 1;
+    double gzmin = 1.7976931348623157e308;
+    int found = -1;
+    for (int l = 0; l < 64; l++) {
+      if (mi[l] < 0) continue;
+      if (found < 0 || mv[l] < gzmin || (mv[l] == gzmin && mi[l] < found)) {
+        gzmin = mv[l];
+        found = mi[l];
+      }
+    }
+    if (found >= 0) Imin = found;
+    imin_s = Imin;
+  }
+  __syncthreads();
+  const int Imin = imin_s;
   const double *b = fld_ptr(d, f);
-  double *tx = d.spl + (size_t)J * 3 * KT, *ty = tx + KT, *tz = ty + KT;
-  for (int K = r.kend; K >= r.kstart; K--) {
-    tx[r.kend - K] = geez(d, kind, K, J, Imin);
+  for (int e = lane; e < n; e += 64) {
+    const int K = r.kend - e;
+    sx[e] = geez(d, kind, K, J, Imin);
     double v = b[ix3(d, K, J, Imin)];
-    ty[r.kend - K] = (kind == 2) ? log(v) : v;
+    sy[e] = (kind == 2) ? log(v) : v;
   }
-  if (pchip(n, tx, ty, tz)) atomicOr(&d.dyn->err, 4);
+  __syncthreads();
+  if (lane == 0 && pchip(n, sx, sy, sz)) atomicOr(&d.dyn->err, 4);
+  __syncthreads();
+  double *tx = d.spl + (size_t)J * 3 * KT, *ty = tx + KT, *tz = ty + KT;
+  for (int e = lane; e < n; e += 64) {
+    tx[e] = sx[e];
+    ty[e] = sy[e];
+    tz[e] = sz[e];
+  }
 }
 
-__global__ void k_spline_sub(const MemberDesc *D, int mb, int f, int kind) {
+// The rest of the 3-D filter in one launch: on every row J = jstart..nj of
+// planes kstart..kend the PCHIP profile v(K,J,I) is removed, the rows of the
+// filter band are transformed, gain-multiplied and transformed back in LDS,
+// the profile is added back and the zonal pads are written.  Rows outside the
+// band come out as (a - v) + v, as in the reference, which subtracts and adds
+// the profile on every row.  RB rows per block, ni/4 lanes per row.
+__global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
   MEMBER;
+  extern __shared__ double lds[];
   F3 r = f3_range(d, kind);
-  const int nI = d.ni, nJ = d.nj - r.jstart + 1;
-  int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= nJ * nI) return;
-  const int J = r.jstart + t / nI, I = 1 + t % nI, K = r.kstart + blockIdx.y;
-  if (K > r.kend) return;
-  const int KT = d.nk + 2, n = r.kend - r.kstart + 1;
-  const double *tx = d.spl + (size_t)J * 3 * KT, *ty = tx + KT, *tz = ty + KT;
-  double xx = geez(d, kind, K, J, I), h;
-  int k = table_place(n, tx, xx, &h);
-  double delta = (ty[k + 1] - ty[k]) / h;
-  double e1 = (tz[k] - delta) / h, e2 = (tz[k + 1] - delta) / h;
-  double c3 = (e1 + e2) / h, c2 = -(e1 + e1 + e2);
-  double x = xx - tx[k];
-  double v = ty[k] + x * (tz[k] + x * (c2 + x * c3));
-  if (kind == 2) v = exp(v);
-  size_t q = ix3(d, K, J, I);
-  d.varz[q] = v;
-  double *b = fld_ptr(d, f);
-  b[q] -= v;
-}
-
-__global__ void k_spline_add(const MemberDesc *D, int mb, int f, int kind) {
-  MEMBER;
-  F3 r = f3_range(d, kind);
-  const int nI = d.ni, nJ = d.nj - r.jstart + 1;
-  int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= nJ * nI) return;
-  const int J = r.jstart + t / nI, I = 1 + t % nI, K = r.kstart + blockIdx.y;
-  if (K > r.kend) return;
-  size_t q = ix3(d, K, J, I);
-  double *b = fld_ptr(d, f);
-  b[q] += d.varz[q];
+  const int K = r.kstart + blockIdx.y;
+  if (K > r.kend) return;  // uniform per block
+  const int n = d.ni, nq = n >> 2, RB = blockDim.x / nq;
+  const int rs = threadIdx.x / nq, lane = threadIdx.x - rs * nq;
+  const int J = r.jstart + blockIdx.x * RB + rs;
+  const bool live = J <= d.nj;
+  const int vg = (kind == 1);
+  const bool filt = live && (vg ? d.rflag_v[J] : d.rflag_h[J]);
+  const int KT = d.nk + 2, nt = r.kend - r.kstart + 1;
+  double *buf = lds + (size_t)rs * n;
+  double *tab = lds + (size_t)RB * n + (size_t)rs * 3 * KT;
+  if (live)
+    for (int e = lane; e < 3 * KT; e += nq) tab[e] = d.spl[(size_t)J * 3 * KT + e];
+  __syncthreads();
+  double *a = fld_ptr(d, f) + (size_t)K * d.N2;
+  const double *tx = tab, *ty = tab + KT, *tz = tab + 2 * KT;
+  double v[4], x[4];
+  if (live)
+    for (int q = 0; q < 4; q++) {
+      const int I = lane + q * nq + 1;
+      double xx = geez(d, kind, K, J, I), h;
+      int k = table_place(nt, tx, xx, &h);
+      double delta = (ty[k + 1] - ty[k]) / h;
+      double e1 = (tz[k] - delta) / h, e2 = (tz[k + 1] - delta) / h;
+      double c3 = (e1 + e2) / h, c2 = -(e1 + e1 + e2);
+      double xr = xx - tx[k];
+      double val = ty[k] + xr * (tz[k] + xr * (c2 + xr * c3));
+      if (kind == 2) val = exp(val);
+      v[q] = val;
+      x[q] = a[ix2(d, J, I)] - val;
+    }
+  LaneTw tf, ti;
+  load_tw(tf, d.tw_f, d.tr_f, n, lane);
+  load_tw(ti, d.tw_i, d.tr_i, n, lane);
+  const int bits = d.nlogn - 1;
+  const double *hl = (vg ? d.hl_pv : d.hl_h) + (size_t)(live ? J : 0) * (n / 2 + 2);
+  double gain[4];
+  if (filt)
+    for (int q = 0, e = lane; q < 4; q++, e += nq) {
+      int gi = (e == 0) ? 1 : (e == 1 ? n / 2 + 1 : (e >> 1) + 1);
+      gain[q] = hl[gi];
+      int kq = e >> 1, c = e & 1;
+      buf[(brev(kq, bits) << 1) | c] = x[q];
+    }
+  __syncthreads();
+  four1_stages(buf, n, lane, tf, filt);
+  split_step(buf, n, lane, tf, -0.5, filt);
+  if (filt && lane == 0) {
+    double h1r;
+    buf[0] = (h1r = buf[0]) + buf[1];
+    buf[1] = h1r - buf[1];
+  }
+  __syncthreads();
+  if (filt)
+    for (int q = 0, e = lane; q < 4; q++, e += nq) buf[e] *= gain[q];
+  __syncthreads();
+  split_step(buf, n, lane, ti, 0.5, filt);
+  if (filt && lane == 0) {
+    double h1r;
+    buf[0] = 0.5 * ((h1r = buf[0]) + buf[1]);
+    buf[1] = 0.5 * (h1r - buf[1]);
+  }
+  __syncthreads();
+  double w[4];
+  if (filt)
+    for (int q = 0, e = lane; q < 4; q++, e += nq) w[q] = buf[e];
+  __syncthreads();
+  if (filt)
+    for (int q = 0, e = lane; q < 4; q++, e += nq) {
+      int kq = e >> 1, c = e & 1;
+      buf[(brev(kq, bits) << 1) | c] = w[q];
+    }
+  __syncthreads();
+  four1_stages(buf, n, lane, ti, filt);
+  if (filt)
+    for (int q = 0, e = lane; q < 4; q++, e += nq) x[q] = buf[e] * (2. / (n));
+  if (live)
+    for (int q = 0, e = lane; q < 4; q++, e += nq) {
+      const double out = x[q] + v[q];
+      a[ix2(d, J, e + 1)] = out;
+      if (e == n - 1) a[ix2(d, J, 0)] = out;
+      if (e == 0) a[ix2(d, J, n + 1)] = out;
+    }
 }
 
 void L_filter3(hipStream_t s, const Bucket &b, int f, int kind) {
-  int jstart = (kind == 1) ? 1 : 0;
-  LAUNCH(k_spline_prep, dim3((b.nj + 1 + 63) / 64, 1, b.nm), dim3(64), 0, s, b.D, b.mb, f, kind);
-  int nJ = b.nj - jstart + 1;
-  // kstart depends on k_sigma for theta; cover K = 1..nk and let the kernel trim
-  int nK = b.nk;
-  dim3 g((nJ * b.ni + TPB - 1) / TPB, nK + 1, b.nm);
-  LAUNCH(k_spline_sub, g, dim3(TPB), 0, s, b.D, b.mb, f, kind);
-  // filter planes kstart..nk (theta starts at k_sigma-1, trimmed per member)
-  launch_fft(s, b, f, 1, b.nk, kind == 1, kind == 3);
-  LAUNCH(k_spline_add, g, dim3(TPB), 0, s, b.D, b.mb, f, kind);
-  L_bc3(s, b, f);
+  const int jstart = (kind == 1) ? 1 : 0, nrow = b.nj + 1 - jstart;
+  LAUNCH(k_spline_prep, dim3(nrow, 1, b.nm), dim3(64), 0, s, b.D, b.mb, f, kind);
+  const int nq = b.ni / 4, RB = (256 / nq) < 16 ? (256 / nq) : 16;
+  dim3 g((nrow + RB - 1) / RB, b.nk, b.nm);
+  size_t shm = sizeof(double) * ((size_t)RB * b.ni + (size_t)RB * 3 * (b.nk + 2));
+  LAUNCH(k_filter3, g, dim3(RB * nq), shm, s, b.D, b.mb, f, kind);
 }
 
 }  // namespace epic

@@ -23,7 +23,7 @@ CASES = {
     "jupiter_16x8x6": ea.spec(ea.JUPITER_LIKE, 16, 8, 6, 60, 5, 0.5, 0.0, 0.0, 0.5),
     "saturn_64x24x12": ea.spec(ea.SATURN_LIKE, 64, 24, 12, 60, 13, 0.4, 0.1, 0.0, 0.5),
     "venus_32x16x5": ea.spec(ea.VENUS_LIKE, 32, 16, 5, 600, 14, 0.5, 0.0, 0.2, 0.2),
-    "hs_128x64x20_divdamp": ea.spec(ea.HELD_SUAREZ, 128, 64, 20, 600, 3, 0.5, 0.15, 0.3, 0.4),
+    "hs_128x64x16_divdamp": ea.spec(ea.HELD_SUAREZ, 128, 64, 16, 300, 3, 0.5, 0.1, 0.3, 0.2),
 }
 
 
@@ -115,3 +115,33 @@ def test_periodic_pads_and_poles():
     v = eng.read(s, "V")
     assert not v[1:nk, 1, 1:ni + 1].any() and not v[1:nk, nj + 2, 1:ni + 1].any()
     eng.close()
+
+
+def test_fatal_conditions_flagged_at_the_same_step_as_the_oracle():
+    """Held-Suarez 128x64x20 at dt = 600 s is unstable (the reference blows
+    up within ten steps): the engine must report the fatal condition (a
+    non-monotone GZ3 column, an out-of-range pressure) at the step where the
+    oracle first reports it, and keep reporting it."""
+    sp = ea.spec(ea.HELD_SUAREZ, 128, 64, 20, 600, 3, 0.5, 0.15, 0.3, 0.4)
+    eng = ea.Engine(ea.FP64, 1)
+    eng.add(sp)
+    ref = CpuRef(sp)
+    ref.sync()
+    first_ref = first_gpu = None
+    for n in range(1, 11):
+        ref.step(1)
+        eng.step(1)
+        if first_ref is None and ref.error():
+            first_ref = n
+        try:
+            eng.sync()
+        except ea.EpicError as e:
+            if first_gpu is None:
+                first_gpu = n
+                say("gpu flagged at step", n, str(e))
+    say("oracle flagged at step", first_ref)
+    assert first_ref is not None and first_gpu == first_ref
+    with pytest.raises(ea.EpicError):
+        eng.sync()
+    eng.close()
+    ref.close()
