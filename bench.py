@@ -85,6 +85,24 @@ def cpu_baseline(planet, ni, nj, nk, dt, budget_s):
     return ni * (nj + 1) * nk * n / el, n, el
 
 
+def copy_probe(nbytes=256 << 20, reps=20):
+    """Bandwidth of a plain device-to-device copy (read + write bytes / s)."""
+    import torch
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    b.synchronize()
+    s = a.elapsed_time(b) / 1e3 / reps
+    return {"bytes_per_copy": 2 * nbytes, "GBps": 2 * nbytes / s / 1e9, "copy_us": s * 1e6}
+
+
 def spec_of(ea, planet, ni, nj, nk, dt, seed):
     gas = planet in (1, 2)
     return ea.spec(planet, ni, nj, nk, dt, seed=seed, nu4=0.5,
@@ -100,6 +118,10 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve", type=int, default=16,
+                    help="requests of the serving mix to run after the step benchmark (0: skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also time a plain 256 MiB device copy (the measured stream-copy peak)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -155,6 +177,16 @@ def main():
                           "frac": value * step_bytes / PEAK_HBM, "unit": "GB/s"},
     }
     eng.close()
+    if args.copy_probe:
+        out["copy_probe"] = copy_probe()
+    if args.serve > 0:
+        from epic_amd import serving as sv
+        reqs = sv.poisson_mix(args.serve, rate=4.0, seed=2024)
+        prec_s = ea.FP64 if args.precision == "fp64" else ea.MIXED
+        res, summ = sv.serve(reqs, precision=prec_s, hbm_budget=128 << 30, max_members=32, chunk=50)
+        summ["mix"] = "Poisson arrivals at 4/s of {64x32x10,128x64x16,256x128x20,512x256x32} x " \
+                      "{held_suarez, jupiter, saturn, venus} x U(200,2000) steps, seed 2024"
+        out["serving"] = summ
     if not args.no_cpu:
         v, n, cel = cpu_baseline(planet, ni, nj, nk, dt, args.cpu_seconds)
         out["cpu_baseline"] = {"value": v, "unit": "grid-pt-steps/s", "cores": 1, "kind": "port",

@@ -62,6 +62,8 @@ def lib():
         L.epic_engine_read.argtypes = [vp, ci, ci, vp]
         L.epic_engine_write.argtypes = [vp, ci, ci, vp]
         L.epic_engine_sync.argtypes = [vp]
+        L.epic_engine_wait.argtypes = [vp]
+        L.epic_engine_member_error.argtypes = [vp, ci]
         L.epic_engine_steps_done.argtypes = [vp, ci]
         L.epic_engine_steps_done.restype = ctypes.c_long
         L.epic_engine_bytes_in_use.argtypes = [vp]
@@ -114,6 +116,14 @@ class Engine:
 
     def sync(self):
         self._check(lib().epic_engine_sync(self._e), "sync")
+
+    def wait(self):
+        """Wait for the queued work without checking the members' error bits."""
+        self._check(lib().epic_engine_wait(self._e), "wait")
+
+    def member_error(self, slot):
+        """Sticky fatal-condition bits of one member (0: healthy)."""
+        return self._check(lib().epic_engine_member_error(self._e, slot), "member_error")
 
     def read(self, slot, field):
         sp = self.specs[slot]

@@ -44,7 +44,9 @@ struct MemberDesc {
   // state and diagnostics
   double *U, *V, *DUDT, *DVDT, *HDRY, *THETA, *P3, *P2, *THETA2, *H3, *T2, *T3, *RHO2, *RHO3;
   double *EXNER3, *GZ3, *MONT3, *HEAT3, *PV3, *DIV, *W3, *DZDT3, *USPIN, *GZS;
-  float *DUDTf, *DVDTf;  // tendency planes in the mixed mode
+  // mixed mode: DUDT, DVDT are one FP64 plane (the current step's sum) and
+  // DUDTf, DVDTf the three FP32 Adams-Bashforth history planes
+  float *DUDTf, *DVDTf;
   // scratch
   double *gz, *varz, *sA, *sB, *sC, *sD, *unity, *old, *spl;
   double *wstale;  // hsu column statics carried between calls (per member)
@@ -173,8 +175,25 @@ __device__ __forceinline__ double *fld_ptr(const MemberDesc &d, int f) {
     case F_SD: return d.sD;
     case F_UNITY: return d.unity;
     case F_OLD: return d.old;
-    case F_DUDT_T: return d.DUDT + (size_t)d.dyn->it0 * d.N3;
-    case F_DVDT_T: return d.DVDT + (size_t)d.dyn->it0 * d.N3;
+    // the mixed mode keeps one FP64 plane for the current step's tendency
+    case F_DUDT_T: return d.mixed ? d.DUDT : d.DUDT + (size_t)d.dyn->it0 * d.N3;
+    case F_DVDT_T: return d.mixed ? d.DVDT : d.DVDT + (size_t)d.dyn->it0 * d.N3;
+  }
+  return nullptr;
+}
+
+// FP32 views used by the mixed mode: the tendency planes and the scratch
+// arrays (an FP32 view of a scratch array uses the first half of its bytes)
+template <class T> __device__ __forceinline__ T *fld_ptr_t(const MemberDesc &d, int f);
+template <> __device__ __forceinline__ double *fld_ptr_t<double>(const MemberDesc &d, int f) { return fld_ptr(d, f); }
+template <> __device__ __forceinline__ float *fld_ptr_t<float>(const MemberDesc &d, int f) {
+  switch (f) {
+    case F_DUDT_T: return d.DUDTf + (size_t)d.dyn->it0 * d.N3;
+    case F_DVDT_T: return d.DVDTf + (size_t)d.dyn->it0 * d.N3;
+    case F_SA: return reinterpret_cast<float *>(d.sA);
+    case F_SB: return reinterpret_cast<float *>(d.sB);
+    case F_SC: return reinterpret_cast<float *>(d.sC);
+    case F_SD: return reinterpret_cast<float *>(d.sD);
   }
   return nullptr;
 }

@@ -169,6 +169,7 @@ static int rebuild_buckets(epic_engine *E) {
     b.ni = hd[i].ni;
     b.nj = hd[i].nj;
     b.nk = hd[i].nk;
+    b.mixed = (E->precision == EPIC_MIXED);
     E->buckets.push_back(b);
     i = j;
   }
@@ -298,10 +299,11 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
     snprintf(E->err, sizeof(E->err), "only del^4 hyperviscosity is implemented");
     return -1;
   }
-  if (E->precision != EPIC_FP64) {
-    snprintf(E->err, sizeof(E->err), "precision mode %d not available", E->precision);
+  if (E->precision != EPIC_FP64 && E->precision != EPIC_MIXED) {
+    snprintf(E->err, sizeof(E->err), "unknown precision mode %d", E->precision);
     return -1;
   }
+  const bool mixed = (E->precision == EPIC_MIXED);
   int slot = -1;
   for (int s = 0; s < E->max_members; s++)
     if (!E->m[s].active) { slot = s; break; }
@@ -348,6 +350,13 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   add(&d.tw_i, sizeof(double) * T.n_tw, T.twi);
   add(&d.tr_f, sizeof(double) * T.n_tr, T.trf);
   add(&d.tr_i, sizeof(double) * T.n_tr, T.tri);
+  // FP32 copies of the twiddles for the filters of FP32 arrays (mixed mode)
+  std::vector<float> twf32(T.twf, T.twf + T.n_tw), twi32(T.twi, T.twi + T.n_tw);
+  std::vector<float> trf32(T.trf, T.trf + T.n_tr), tri32(T.tri, T.tri + T.n_tr);
+  add(&d.tw_f32, sizeof(float) * T.n_tw, twf32.data());
+  add(&d.tw_i32, sizeof(float) * T.n_tw, twi32.data());
+  add(&d.tr_f32, sizeof(float) * T.n_tr, trf32.data());
+  add(&d.tr_i32, sizeof(float) * T.n_tr, tri32.data());
   add(&d.rows_h, sizeof(int) * (c->nj + 2), T.rows_h);
   add(&d.rows_v, sizeof(int) * (c->nj + 2), T.rows_v);
   std::vector<int> flag_h(c->nj + 2, 0), flag_v(c->nj + 2, 0);
@@ -365,8 +374,15 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
                        &d.MONT3, &d.HEAT3, &d.PV3, &d.DIV, &d.W3, &d.DZDT3, &d.gz, &d.varz, &d.sA, &d.sB,
                        &d.sC, &d.sD, &d.unity, &d.old};
   for (auto p : fields) add(p, sizeof(double) * N3, nul);
-  add(&d.DUDT, sizeof(double) * 3 * N3, nul);
-  add(&d.DVDT, sizeof(double) * 3 * N3, nul);
+  if (mixed) {
+    add(&d.DUDT, sizeof(double) * N3, nul);
+    add(&d.DVDT, sizeof(double) * N3, nul);
+    add(&d.DUDTf, sizeof(float) * 3 * N3, nul);
+    add(&d.DVDTf, sizeof(float) * 3 * N3, nul);
+  } else {
+    add(&d.DUDT, sizeof(double) * 3 * N3, nul);
+    add(&d.DVDT, sizeof(double) * 3 * N3, nul);
+  }
   add(&d.spl, sizeof(double) * (size_t)(c->nj + 2) * 3 * (c->nk + 2), nul);
   add(&d.wstale, sizeof(double) * 4, nul);
   add(&d.rowbuf, sizeof(double) * 8 * (c->nj + 3), nul);
@@ -412,9 +428,20 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
     for (int J = 0; J <= c->nj; J++)
       for (int I = 1; I <= c->ni; I++) t1[IX(K, J, I)] = 1.7976931348623157e308;
   HIPCHK(hipMemcpyAsync(d.HDRY, h.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
+  std::vector<float> t1f;
+  if (mixed) {
+    t1f.assign(N3, 0.f);
+    for (size_t i = 0; i < N3; i++)
+      if (t1[i] != 0.) t1f[i] = 3.40282347e+38f;
+  }
   for (int p = 1; p <= 2; p++) {
-    HIPCHK(hipMemcpyAsync(d.DUDT + p * N3, t1.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
-    HIPCHK(hipMemcpyAsync(d.DVDT + p * N3, t1.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
+    if (mixed) {
+      HIPCHK(hipMemcpyAsync(d.DUDTf + p * N3, t1f.data(), sizeof(float) * N3, hipMemcpyHostToDevice, E->stream));
+      HIPCHK(hipMemcpyAsync(d.DVDTf + p * N3, t1f.data(), sizeof(float) * N3, hipMemcpyHostToDevice, E->stream));
+    } else {
+      HIPCHK(hipMemcpyAsync(d.DUDT + p * N3, t1.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
+      HIPCHK(hipMemcpyAsync(d.DVDT + p * N3, t1.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
+    }
   }
   MemberDyn dy;
   memset(&dy, 0, sizeof(dy));
@@ -431,7 +458,7 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   int lg = 0;
   while ((1 << lg) < c->ni) lg++;
   d.nlogn = lg;
-  d.mixed = 0;
+  d.mixed = mixed ? 1 : 0;
   d.dt = (double)c->dt; d.kappa = c->kappa; d.cp = c->cp; d.cpr = c->cpr; d.rgas = c->rgas;
   d.press0 = c->press0; d.zeta0 = c->zeta0; d.zeta1 = c->zeta1; d.alpha = c->hybrid_alpha;
   d.sigma_sigma = c->sigma_sigma; d.topo_scale = c->topo_scale; d.prandtl = c->prandtl; d.tau_drag = c->tau_drag;
@@ -550,6 +577,23 @@ extern "C" int epic_engine_sync(epic_engine *E) {
   return check_errors(E);
 }
 
+// Sticky fatal-condition bits of one member (0: healthy); waits for the
+// queued work first
+extern "C" int epic_engine_member_error(epic_engine *E, int slot) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return -1;
+  MemberDyn chk;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  HIPCHK(hipMemcpy(&chk, E->m[slot].dyn, sizeof(chk), hipMemcpyDeviceToHost));
+  return chk.err;
+}
+
+// Wait for the queued work without checking the members' error bits
+extern "C" int epic_engine_wait(epic_engine *E) {
+  if (!E) return -1;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  return 0;
+}
+
 extern "C" long epic_engine_steps_done(epic_engine *E, int slot) {
   if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return -1;
   return E->m[slot].steps;
@@ -590,6 +634,17 @@ static double *field_ptr(MemberDesc &d, int field, size_t &n, int itplanes[3]) {
 
 extern "C" int epic_engine_read(epic_engine *E, int slot, int field, double *out) {
   if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active || !out) return -1;
+  const MemberDesc &hd = E->m[slot].h;
+  if (hd.mixed && field >= EPIC_F_DUDT0 && field <= EPIC_F_DVDT2) {
+    const bool isu = field <= EPIC_F_DUDT2;
+    const int plane = isu ? field - EPIC_F_DUDT0 : field - EPIC_F_DVDT0;
+    const float *src = (isu ? hd.DUDTf : hd.DVDTf) + (size_t)plane * hd.N3;
+    std::vector<float> tmp(hd.N3);
+    HIPCHK(hipStreamSynchronize(E->stream));
+    HIPCHK(hipMemcpy(tmp.data(), src, sizeof(float) * hd.N3, hipMemcpyDeviceToHost));
+    for (int i = 0; i < hd.N3; i++) out[i] = tmp[i];
+    return 0;
+  }
   size_t n;
   double *p = field_ptr(E->m[slot].h, field, n, nullptr);
   if (!p) return -1;

@@ -132,6 +132,7 @@ __global__ void k_dudt_core(const MemberDesc *D, int mb) {
 }
 
 void L_uv_core(hipStream_t s, const Bucket &b) {
+  if (b.mixed) { LM_uv_core(s, b); return; }
   const int nK = b.nk - 1;
   LAUNCH(k_uvh_kin, grid2(b.nj + 2, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
   LAUNCH(k_dvdt_core, grid2(b.nj, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
@@ -224,6 +225,7 @@ __global__ void k_hs_pbl(const MemberDesc *D, int mb) {
 }
 
 void L_uv_drag(hipStream_t s, const Bucket &b) {
+  if (b.mixed) { LM_uv_drag(s, b); return; }
   LAUNCH(k_sponge, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
   int nrow = b.nk * (b.nj + 1);
   LAUNCH(k_rayleigh, dim3((nrow + 63) / 64, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
@@ -399,6 +401,7 @@ __global__ void k_pgrad(const MemberDesc *D, int mb) {
 
 void L_pgrad(hipStream_t s, const Bucket &b) {
   LAUNCH(k_gz_polefill, dim3(2 * (b.nk + 1), 1, b.nm), dim3(64), sizeof(double) * (b.ni + 1), s, b.D, b.mb);
+  if (b.mixed) { LM_pgrad(s, b); return; }
   LAUNCH(k_pgrad, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
@@ -421,6 +424,7 @@ void L_divdamp(hipStream_t s, const Bucket &b) {
   L_copy_planes(s, b, F_U, F_SA, 1, b.nk - 1);
   L_filter(s, b, F_SA, 1, b.nk - 1, 0, 0);
   L_divergence(s, b, F_SA, F_V, F_SB, 1, b.nk - 1);
+  if (b.mixed) { LM_divdamp_apply(s, b); return; }
   LAUNCH(k_divdamp, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
@@ -466,6 +470,7 @@ __global__ void k_rotate(const MemberDesc *D, int mb) {
   y->itime += 1;
 }
 void L_ab(hipStream_t s, const Bucket &b) {
+  if (b.mixed) { LM_ab(s, b); return; }
   LAUNCH(k_ab, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 void L_rotate(hipStream_t s, const Bucket &b) {
@@ -483,6 +488,7 @@ __global__ void k_zero_tend(const MemberDesc *D, int mb) {
   }
 }
 void L_zero_tend(hipStream_t s, const Bucket &b) {
+  if (b.mixed) { LM_zero_tend(s, b); return; }
   LAUNCH(k_zero_tend, dim3(256, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 

@@ -26,27 +26,44 @@ __device__ __forceinline__ int brev(int k, int bits) { return bits ? (int)(__bre
 // transforms start (one round of loads instead of one per stage).
 constexpr int MAX_STAGES = 9;  // n <= 1024: stages with 2 << s < n
 
+template <class T>
 struct LaneTw {
-  double wr[MAX_STAGES], wi[MAX_STAGES];
-  double sr, si;  // split-step twiddle of this lane
+  T wr[MAX_STAGES], wi[MAX_STAGES];
+  T sr, si;  // split-step twiddle of this lane
 };
 
-__device__ __forceinline__ void load_tw(LaneTw &t, const double *tw, const double *tr, int n, int lane) {
+template <class T>
+__device__ __forceinline__ void load_tw(LaneTw<T> &t, const T *tw, const T *tr, int n, int lane) {
 #pragma unroll
   for (int s = 0; s < MAX_STAGES; s++) {
     const int half = 1 << s, mm = lane & (half - 1);
     const bool on = (2 << s) < n;
-    t.wr[s] = on ? tw[2 * (half - 1 + mm)] : 0.;
-    t.wi[s] = on ? tw[2 * (half - 1 + mm) + 1] : 0.;
+    t.wr[s] = on ? tw[2 * (half - 1 + mm)] : T(0);
+    t.wi[s] = on ? tw[2 * (half - 1 + mm) + 1] : T(0);
   }
   const bool on = lane < n / 4 - 1;
-  t.sr = on ? tr[2 * lane] : 0.;
-  t.si = on ? tr[2 * lane + 1] : 0.;
+  t.sr = on ? tr[2 * lane] : T(0);
+  t.si = on ? tr[2 * lane + 1] : T(0);
 }
+
+// twiddle tables and field views of a storage type
+template <class T> __device__ __forceinline__ const T *tw_fwd(const MemberDesc &d);
+template <class T> __device__ __forceinline__ const T *tw_inv(const MemberDesc &d);
+template <class T> __device__ __forceinline__ const T *tr_fwd(const MemberDesc &d);
+template <class T> __device__ __forceinline__ const T *tr_inv(const MemberDesc &d);
+template <> __device__ __forceinline__ const double *tw_fwd<double>(const MemberDesc &d) { return d.tw_f; }
+template <> __device__ __forceinline__ const double *tw_inv<double>(const MemberDesc &d) { return d.tw_i; }
+template <> __device__ __forceinline__ const double *tr_fwd<double>(const MemberDesc &d) { return d.tr_f; }
+template <> __device__ __forceinline__ const double *tr_inv<double>(const MemberDesc &d) { return d.tr_i; }
+template <> __device__ __forceinline__ const float *tw_fwd<float>(const MemberDesc &d) { return d.tw_f32; }
+template <> __device__ __forceinline__ const float *tw_inv<float>(const MemberDesc &d) { return d.tw_i32; }
+template <> __device__ __forceinline__ const float *tr_fwd<float>(const MemberDesc &d) { return d.tr_f32; }
+template <> __device__ __forceinline__ const float *tr_inv<float>(const MemberDesc &d) { return d.tr_i32; }
 
 // complex radix-2 stages on buf[0..n) (n doubles = n/2 complex), one
 // butterfly per lane, in the reference's stage order
-__device__ __forceinline__ void four1_stages(double *buf, int n, int lane, const LaneTw &t, bool active) {
+template <class T>
+__device__ __forceinline__ void four1_stages(T *buf, int n, int lane, const LaneTw<T> &t, bool active) {
 #pragma unroll
   for (int s = 0; s < MAX_STAGES; s++) {
     if ((2 << s) >= n) break;
@@ -54,9 +71,9 @@ __device__ __forceinline__ void four1_stages(double *buf, int n, int lane, const
     if (active) {
       int g = lane >> s, mm = lane & (half - 1);
       int i0 = 2 * mm + g * 2 * mmax, j0 = i0 + mmax;
-      double wr = t.wr[s], wi = t.wi[s];
-      double tr = wr * buf[j0] - wi * buf[j0 + 1];
-      double ti = wr * buf[j0 + 1] + wi * buf[j0];
+      T wr = t.wr[s], wi = t.wi[s];
+      T tr = wr * buf[j0] - wi * buf[j0 + 1];
+      T ti = wr * buf[j0 + 1] + wi * buf[j0];
       buf[j0] = buf[i0] - tr;
       buf[j0 + 1] = buf[i0 + 1] - ti;
       buf[i0] += tr;
@@ -67,17 +84,18 @@ __device__ __forceinline__ void four1_stages(double *buf, int n, int lane, const
 }
 
 // split step of the real transform; c2 = -0.5 forward, +0.5 inverse
-__device__ __forceinline__ void split_step(double *buf, int n, int lane, const LaneTw &t, double c2, bool active) {
-  const double c1 = 0.5;
+template <class T>
+__device__ __forceinline__ void split_step(T *buf, int n, int lane, const LaneTw<T> &t, T c2, bool active) {
+  const T c1 = T(0.5);
   if (active && lane < n / 4 - 1) {
     int i = lane + 2;
     int i1 = i + i - 1, i2 = 1 + i1, i3 = n + 3 - i2, i4 = 1 + i3;
     i1--; i2--; i3--; i4--;
-    double wr = t.sr, wi = t.si;
-    double h1r = c1 * (buf[i1] + buf[i3]);
-    double h1i = c1 * (buf[i2] - buf[i4]);
-    double h2r = -c2 * (buf[i2] + buf[i4]);
-    double h2i = c2 * (buf[i1] - buf[i3]);
+    T wr = t.sr, wi = t.si;
+    T h1r = c1 * (buf[i1] + buf[i3]);
+    T h1i = c1 * (buf[i2] - buf[i4]);
+    T h2r = -c2 * (buf[i2] + buf[i4]);
+    T h2i = c2 * (buf[i1] - buf[i3]);
     buf[i1] = h1r + wr * h2r - wi * h2i;
     buf[i2] = h1i + wr * h2i + wi * h2r;
     buf[i3] = h1r - wr * h2r + wi * h2i;
@@ -86,9 +104,11 @@ __device__ __forceinline__ void split_step(double *buf, int n, int lane, const L
   __syncthreads();
 }
 
+template <class T>
 __global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgrid, int pads) {
   MEMBER;
-  extern __shared__ double lds[];
+  extern __shared__ double lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
   const int n = d.ni, nq = n >> 2, RB = TPB / nq;
   const int rs = threadIdx.x / nq, lane = threadIdx.x - rs * nq;
   const int nrows = vgrid ? d.nrow_v : d.nrow_h;
@@ -96,26 +116,26 @@ __global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgr
   const bool active = r < nrows;
   const int J = active ? (vgrid ? d.rows_v[r] : d.rows_h[r]) : 0;
   const int K = k0 + blockIdx.y;
-  double *a = fld_ptr(d, f) + (size_t)K * d.N2;
-  double *buf = lds + (size_t)rs * n;
+  T *a = fld_ptr_t<T>(d, f) + (size_t)K * d.N2;
+  T *buf = lds + (size_t)rs * n;
   const int bits = d.nlogn - 1;
   const double *hl = (vgrid ? d.hl_pv : d.hl_h) + (size_t)J * (n / 2 + 2);
-  LaneTw tf, ti;
-  load_tw(tf, d.tw_f, d.tr_f, n, lane);
-  load_tw(ti, d.tw_i, d.tr_i, n, lane);
-  double gain[4];
+  LaneTw<T> tf, ti;
+  load_tw(tf, tw_fwd<T>(d), tr_fwd<T>(d), n, lane);
+  load_tw(ti, tw_inv<T>(d), tr_inv<T>(d), n, lane);
+  T gain[4];
   if (active)
     for (int q = 0, e = lane; q < 4; q++, e += nq) {
       int gi = (e == 0) ? 1 : (e == 1 ? n / 2 + 1 : (e >> 1) + 1);
-      gain[q] = hl[gi];
+      gain[q] = T(hl[gi]);
       int k = e >> 1, c = e & 1;
       buf[(brev(k, bits) << 1) | c] = a[ix2(d, J, e + 1)];
     }
   __syncthreads();
   four1_stages(buf, n, lane, tf, active);
-  split_step(buf, n, lane, tf, -0.5, active);
+  split_step(buf, n, lane, tf, T(-0.5), active);
   if (active && lane == 0) {
-    double h1r;
+    T h1r;
     buf[0] = (h1r = buf[0]) + buf[1];
     buf[1] = h1r - buf[1];
   }
@@ -123,14 +143,14 @@ __global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgr
   if (active)
     for (int q = 0, e = lane; q < 4; q++, e += nq) buf[e] *= gain[q];
   __syncthreads();
-  split_step(buf, n, lane, ti, 0.5, active);
+  split_step(buf, n, lane, ti, T(0.5), active);
   if (active && lane == 0) {
-    double h1r;
-    buf[0] = 0.5 * ((h1r = buf[0]) + buf[1]);
-    buf[1] = 0.5 * (h1r - buf[1]);
+    T h1r;
+    buf[0] = T(0.5) * ((h1r = buf[0]) + buf[1]);
+    buf[1] = T(0.5) * (h1r - buf[1]);
   }
   __syncthreads();
-  double v[4];
+  T v[4];
   if (active)
     for (int q = 0, e = lane; q < 4; q++, e += nq) v[q] = buf[e];
   __syncthreads();
@@ -143,25 +163,30 @@ __global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgr
   four1_stages(buf, n, lane, ti, active);
   if (active)
     for (int e = lane; e < n; e += nq) {
-      const double y = buf[e] * (2. / (n));
+      const T y = buf[e] * T(2. / (n));
       a[ix2(d, J, e + 1)] = y;
       if (pads && e == n - 1) a[ix2(d, J, 0)] = y;
       if (pads && e == 0) a[ix2(d, J, n + 1)] = y;
     }
 }
 
+template <class T>
 static void launch_fft(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int pads) {
   const int n = b.ni, nq = n / 4, RB = TPB / nq;
   const int rows = b.nj + 1;  // upper bound on the listed rows
   dim3 g((rows + RB - 1) / RB, k1 - k0 + 1, b.nm);
-  size_t shm = sizeof(double) * (size_t)RB * n;
-  LAUNCH(k_fft_filter, g, dim3(TPB), shm, s, b.D, b.mb, f, k0, vgrid, pads);
+  size_t shm = sizeof(T) * (size_t)RB * n;
+  LAUNCH(k_fft_filter<T>, g, dim3(TPB), shm, s, b.D, b.mb, f, k0, vgrid, pads);
 }
 
 // mode 0: also refresh the zonal pads of the filtered rows (the producers of
 // every filtered copy write the pads of the rows they fill); mode 1: no pads
 void L_filter(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int mode) {
-  launch_fft(s, b, f, k0, k1, vgrid, mode == 0);
+  launch_fft<double>(s, b, f, k0, k1, vgrid, mode == 0);
+}
+// the same on an FP32 array (mixed precision)
+void L_filter_f32(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int mode) {
+  launch_fft<float>(s, b, f, k0, k1, vgrid, mode == 0);
 }
 
 // ------------------------------------------------------------- 3-D form
@@ -237,6 +262,7 @@ __device__ __forceinline__ F3 f3_range(const MemberDesc &d, int kind) {
 // equals, as the reference's strict-less scan finds it: each lane keeps the
 // first minimum of its strided subset, ties between lanes go to the lower
 // I), the profile table at that column and its PCHIP derivatives
+template <class T>
 __global__ void k_spline_prep(const MemberDesc *D, int mb, int f, int kind) {
   MEMBER;
   F3 r = f3_range(d, kind);
@@ -271,11 +297,11 @@ __global__ void k_spline_prep(const MemberDesc *D, int mb, int f, int kind) {
   }
   __syncthreads();
   const int Imin = imin_s;
-  const double *b = fld_ptr(d, f);
+  const T *b = fld_ptr_t<T>(d, f);
   for (int e = lane; e < n; e += 64) {
     const int K = r.kend - e;
     sx[e] = geez(d, kind, K, J, Imin);
-    double v = b[ix3(d, K, J, Imin)];
+    double v = (double)b[ix3(d, K, J, Imin)];
     sy[e] = (kind == 2) ? log(v) : v;
   }
   __syncthreads();
@@ -295,9 +321,11 @@ __global__ void k_spline_prep(const MemberDesc *D, int mb, int f, int kind) {
 // the profile is added back and the zonal pads are written.  Rows outside the
 // band come out as (a - v) + v, as in the reference, which subtracts and adds
 // the profile on every row.  RB rows per block, ni/4 lanes per row.
+template <class T>
 __global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
   MEMBER;
-  extern __shared__ double lds[];
+  extern __shared__ double lds_raw[];
+  double *tabs = lds_raw;
   F3 r = f3_range(d, kind);
   const int K = r.kstart + blockIdx.y;
   if (K > r.kend) return;  // uniform per block
@@ -308,14 +336,15 @@ __global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
   const int vg = (kind == 1);
   const bool filt = live && (vg ? d.rflag_v[J] : d.rflag_h[J]);
   const int KT = d.nk + 2, nt = r.kend - r.kstart + 1;
-  double *buf = lds + (size_t)rs * n;
-  double *tab = lds + (size_t)RB * n + (size_t)rs * 3 * KT;
+  double *tab = tabs + (size_t)rs * 3 * KT;
+  T *buf = reinterpret_cast<T *>(tabs + (size_t)RB * 3 * KT) + (size_t)rs * n;
   if (live)
     for (int e = lane; e < 3 * KT; e += nq) tab[e] = d.spl[(size_t)J * 3 * KT + e];
   __syncthreads();
-  double *a = fld_ptr(d, f) + (size_t)K * d.N2;
+  T *a = fld_ptr_t<T>(d, f) + (size_t)K * d.N2;
   const double *tx = tab, *ty = tab + KT, *tz = tab + 2 * KT;
-  double v[4], x[4];
+  double v[4];
+  T x[4];
   if (live)
     for (int q = 0; q < 4; q++) {
       const int I = lane + q * nq + 1;
@@ -328,26 +357,26 @@ __global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
       double val = ty[k] + xr * (tz[k] + xr * (c2 + xr * c3));
       if (kind == 2) val = exp(val);
       v[q] = val;
-      x[q] = a[ix2(d, J, I)] - val;
+      x[q] = T((double)a[ix2(d, J, I)] - val);
     }
-  LaneTw tf, ti;
-  load_tw(tf, d.tw_f, d.tr_f, n, lane);
-  load_tw(ti, d.tw_i, d.tr_i, n, lane);
+  LaneTw<T> tf, ti;
+  load_tw(tf, tw_fwd<T>(d), tr_fwd<T>(d), n, lane);
+  load_tw(ti, tw_inv<T>(d), tr_inv<T>(d), n, lane);
   const int bits = d.nlogn - 1;
   const double *hl = (vg ? d.hl_pv : d.hl_h) + (size_t)(live ? J : 0) * (n / 2 + 2);
-  double gain[4];
+  T gain[4];
   if (filt)
     for (int q = 0, e = lane; q < 4; q++, e += nq) {
       int gi = (e == 0) ? 1 : (e == 1 ? n / 2 + 1 : (e >> 1) + 1);
-      gain[q] = hl[gi];
+      gain[q] = T(hl[gi]);
       int kq = e >> 1, c = e & 1;
       buf[(brev(kq, bits) << 1) | c] = x[q];
     }
   __syncthreads();
   four1_stages(buf, n, lane, tf, filt);
-  split_step(buf, n, lane, tf, -0.5, filt);
+  split_step(buf, n, lane, tf, T(-0.5), filt);
   if (filt && lane == 0) {
-    double h1r;
+    T h1r;
     buf[0] = (h1r = buf[0]) + buf[1];
     buf[1] = h1r - buf[1];
   }
@@ -355,14 +384,14 @@ __global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
   if (filt)
     for (int q = 0, e = lane; q < 4; q++, e += nq) buf[e] *= gain[q];
   __syncthreads();
-  split_step(buf, n, lane, ti, 0.5, filt);
+  split_step(buf, n, lane, ti, T(0.5), filt);
   if (filt && lane == 0) {
-    double h1r;
-    buf[0] = 0.5 * ((h1r = buf[0]) + buf[1]);
-    buf[1] = 0.5 * (h1r - buf[1]);
+    T h1r;
+    buf[0] = T(0.5) * ((h1r = buf[0]) + buf[1]);
+    buf[1] = T(0.5) * (h1r - buf[1]);
   }
   __syncthreads();
-  double w[4];
+  T w[4];
   if (filt)
     for (int q = 0, e = lane; q < 4; q++, e += nq) w[q] = buf[e];
   __syncthreads();
@@ -374,23 +403,25 @@ __global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
   __syncthreads();
   four1_stages(buf, n, lane, ti, filt);
   if (filt)
-    for (int q = 0, e = lane; q < 4; q++, e += nq) x[q] = buf[e] * (2. / (n));
+    for (int q = 0, e = lane; q < 4; q++, e += nq) x[q] = buf[e] * T(2. / (n));
   if (live)
     for (int q = 0, e = lane; q < 4; q++, e += nq) {
-      const double out = x[q] + v[q];
+      const T out = T((double)x[q] + v[q]);
       a[ix2(d, J, e + 1)] = out;
       if (e == n - 1) a[ix2(d, J, 0)] = out;
       if (e == 0) a[ix2(d, J, n + 1)] = out;
     }
 }
 
-void L_filter3(hipStream_t s, const Bucket &b, int f, int kind) {
+template <class T>
+static void filter3_t(hipStream_t s, const Bucket &b, int f, int kind) {
   const int jstart = (kind == 1) ? 1 : 0, nrow = b.nj + 1 - jstart;
-  LAUNCH(k_spline_prep, dim3(nrow, 1, b.nm), dim3(64), 0, s, b.D, b.mb, f, kind);
+  LAUNCH(k_spline_prep<T>, dim3(nrow, 1, b.nm), dim3(64), 0, s, b.D, b.mb, f, kind);
   const int nq = b.ni / 4, RB = (256 / nq) < 16 ? (256 / nq) : 16;
   dim3 g((nrow + RB - 1) / RB, b.nk, b.nm);
-  size_t shm = sizeof(double) * ((size_t)RB * b.ni + (size_t)RB * 3 * (b.nk + 2));
-  LAUNCH(k_filter3, g, dim3(RB * nq), shm, s, b.D, b.mb, f, kind);
+  size_t shm = sizeof(double) * (size_t)RB * 3 * (b.nk + 2) + sizeof(T) * (size_t)RB * b.ni;
+  LAUNCH(k_filter3<T>, g, dim3(RB * nq), shm, s, b.D, b.mb, f, kind);
 }
+void L_filter3(hipStream_t s, const Bucket &b, int f, int kind) { filter3_t<double>(s, b, f, kind); }
 
 }  // namespace epic

@@ -1,0 +1,454 @@
+// k_mixed.hip -- the momentum kernels of the mixed-precision mode.
+//
+// The state (u, v, h, theta), the hydrostatic diagnostics (P3, P2, GZ3,
+// MONT3, RHO3, ...) and the Adams-Bashforth accumulation stay FP64; the
+// Adams-Bashforth history planes of DUDT, DVDT are FP32 and every tendency
+// term is formed in FP32: Coriolis and kinetic energy (uv_core), the sponge
+// and boundary-layer drag (uv_drag), the finite-volume pressure gradient
+// (uv_pgrad) and divergence damping.  The terms of the current step are
+// summed into one FP64 plane (DUDT, DVDT hold a single plane in this mode)
+// and rounded to FP32 once, when the step stores it in the history: in
+// balanced flow the v tendency is the small residual of the Coriolis and
+// pressure-gradient terms, which an FP32 running sum would lose.  Where FP32 arithmetic would difference two
+// full-magnitude FP64 values, the difference is taken in FP64 first:
+//   * the pressure gradient works on geopotential shifted by the value at the
+//     cell (GZ3 - GZ3(K,J,I), formed in FP64).  Every face integral of the
+//     reference's finite-volume scheme (src/shared/epic_timestep.c:909-1273)
+//     is invariant under a uniform shift of z (the t1 and zt*pb - zb*pt
+//     terms change by cancelling amounts, t2 and b1+b2-b3 not at all), so
+//     the shift changes nothing but the conditioning;
+//   * vertical differences of u and v are formed in FP64.
+// The FP64 kernels (k_dyn.hip) are untouched; this file is selected per
+// bucket when the engine runs in the mixed mode.
+#include "kernels.h"
+
+namespace epic {
+
+#define MEMBER const MemberDesc &d = D[mb + blockIdx.z]
+#define FLAT2(j0, nJ, i0, nI)                          \
+  int t_ = blockIdx.x * TPB + threadIdx.x;             \
+  if (t_ >= (nJ) * (nI)) return;                       \
+  const int J = (j0) + t_ / (nI);                      \
+  const int I = (i0) + t_ % (nI);
+
+static inline dim3 grid2(int nJ, int nI, int nK, int nm) {
+  return dim3((unsigned)((nJ * nI + TPB - 1) / TPB), (unsigned)nK, (unsigned)nm);
+}
+
+// the current step's tendency plane (FP64) and the FP32 history ring
+__device__ __forceinline__ double *curU(const MemberDesc &d) { return d.DUDT; }
+__device__ __forceinline__ double *curV(const MemberDesc &d) { return d.DVDT; }
+__device__ __forceinline__ float *sAf(const MemberDesc &d) { return reinterpret_cast<float *>(d.sA); }
+__device__ __forceinline__ float *sBf(const MemberDesc &d) { return reinterpret_cast<float *>(d.sB); }
+__device__ __forceinline__ float *sCf(const MemberDesc &d) { return reinterpret_cast<float *>(d.sC); }
+
+// ------------------------------------------------------------ tendencies
+__global__ void km_zero_tend(const MemberDesc *D, int mb) {
+  MEMBER;
+  double *du = curU(d), *dv = curV(d);
+  size_t i = (size_t)blockIdx.x * TPB + threadIdx.x, st = (size_t)gridDim.x * TPB;
+  for (; i < (size_t)d.N3; i += st) {
+    du[i] = 0.;
+    dv[i] = 0.;
+  }
+}
+
+// UH, VH, KIN (FP32 views of sA, sB, sC) on planes 1..nk-1 with zonal pads
+__global__ void km_uvh_kin(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 2, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const size_t o = (size_t)K * d.N2;
+  const double *u = d.U + o, *v = d.V + o, *h3 = d.H3 + o;
+  float *uh = sAf(d) + o, *vh = sBf(d) + o, *kin = sCf(d) + o;
+  const int ni = d.ni, nj = d.nj;
+  if (J <= nj) {
+    float havg = .5f * ((float)h3[ix2(d, J, I)] + (float)h3[ix2(d, J, I - 1)]);
+    float x = (float)u[ix2(d, J, I)] * havg * (float)(1. / d.n[2 * J + 1]);
+    uh[ix2(d, J, I)] = x;
+    if (I == ni) uh[ix2(d, J, 0)] = x;
+    if (I == 1) uh[ix2(d, J, ni + 1)] = x;
+    const int jj = 2 * J;
+    const float mn1 = (float)d.mn[jj + 1], mi0 = (float)d.mn_inv[jj], mi2 = (float)d.mn_inv[jj + 2];
+    float u2 = (float)u[ix2(d, J, I)], u4 = (float)u[ix2(d, J, I + 1)];
+    float v2 = (float)v[ix2(d, J, I)], v4 = (float)v[ix2(d, J + 1, I)];
+    float kc = (u2 * u2 + u4 * u4 + mn1 * (v2 * v2 * mi0 + v4 * v4 * mi2)) * .25f, k;
+    if (J == 0 || J == nj) {
+      k = kc;
+    } else {
+      u2 = .5f * ((float)u[ix2(d, J + 1, I)] + (float)u[ix2(d, J - 1, I)]);
+      u4 = .5f * ((float)u[ix2(d, J + 1, I + 1)] + (float)u[ix2(d, J - 1, I + 1)]);
+      v2 = .5f * ((float)v[ix2(d, J, I + 1)] + (float)v[ix2(d, J, I - 1)]);
+      v4 = .5f * ((float)v[ix2(d, J + 1, I + 1)] + (float)v[ix2(d, J + 1, I - 1)]);
+      float ks = (u2 * u2 + u4 * u4 + mn1 * (v2 * v2 * mi0 + v4 * v4 * mi2)) * .25f;
+      k = (5.f / 6.f) * kc + (1.f / 6.f) * ks;
+    }
+    kin[ix2(d, J, I)] = k;
+    if (I == ni) kin[ix2(d, J, 0)] = k;
+    if (I == 1) kin[ix2(d, J, ni + 1)] = k;
+  }
+  float y = 0.f;
+  if (J >= 1 && J <= nj) {
+    float havg = .5f * ((float)h3[ix2(d, J, I)] + (float)h3[ix2(d, J - 1, I)]);
+    y = (float)v[ix2(d, J, I)] * havg * (float)(1. / d.m[2 * J]);
+  }
+  vh[ix2(d, J, I)] = y;
+  if (I == ni) vh[ix2(d, J, 0)] = y;
+  if (I == 1) vh[ix2(d, J, ni + 1)] = y;
+}
+
+__global__ void km_dvdt_core(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(1, d.nj, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const size_t o = (size_t)K * d.N2;
+  const float *uh = sAf(d) + o, *kin = sCf(d) + o;
+  const double *pv = d.PV3 + o;
+  double *dv = curV(d) + o;
+#define PV(j, i) ((float)pv[ix2(d, j, i)])
+  float ga = (PV(J, I) + PV(J, I + 1) + PV(J + 1, I + 1));
+  float de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
+  float al = (PV(J - 1, I) + PV(J, I) + PV(J, I + 1));
+  float be = (PV(J, I) + PV(J, I + 1) + PV(J - 1, I + 1));
+#undef PV
+  float x = ((-ga * uh[ix2(d, J, I + 1)] - de * uh[ix2(d, J, I)] - al * uh[ix2(d, J - 1, I)] -
+              be * uh[ix2(d, J - 1, I + 1)]) * (1.f / 12.f) +
+             (kin[ix2(d, J - 1, I)] - kin[ix2(d, J, I)])) *
+            (float)d.n[2 * J];
+  const int kk = 2 * K + 1;
+  const float ds_inv = (float)(1. / (d.sgth[kk - 2] - d.sgth[kk + 2]));
+  const float dvz = (float)(d.V[ix3(d, K - 1, J, I)] - d.V[ix3(d, K + 1, J, I)]);
+  x -= .5f * (float)(d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K, J - 1, I)]) * dvz * ds_inv;
+  dv[ix2(d, J, I)] += (double)x;
+}
+
+__global__ void km_dudt_core(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const size_t o = (size_t)K * d.N2;
+  const float *vh = sBf(d) + o, *kin = sCf(d) + o;
+  const double *pv = d.PV3 + o;
+  double *du = curU(d) + o;
+#define PV(j, i) ((float)pv[ix2(d, j, i)])
+  float al = (PV(J, I) + PV(J + 1, I) + PV(J + 1, I + 1));
+  float be = (PV(J + 1, I - 1) + PV(J + 1, I) + PV(J, I));
+  float ga = (PV(J, I - 1) + PV(J, I) + PV(J + 1, I));
+  float de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
+#undef PV
+  float x = ((al * vh[ix2(d, J + 1, I)] + be * vh[ix2(d, J + 1, I - 1)] + ga * vh[ix2(d, J, I - 1)] +
+              de * vh[ix2(d, J, I)]) * (1.f / 12.f) +
+             (kin[ix2(d, J, I - 1)] - kin[ix2(d, J, I)])) *
+            (float)d.m[2 * J + 1];
+  const int kk = 2 * K + 1;
+  const float ds_inv = (float)(1. / (d.sgth[kk - 2] - d.sgth[kk + 2]));
+  const float duz = (float)(d.U[ix3(d, K - 1, J, I)] - d.U[ix3(d, K + 1, J, I)]);
+  x -= .5f * (float)(d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K, J, I - 1)]) * duz * ds_inv;
+  du[ix2(d, J, I)] += (double)x;
+}
+
+// ------------------------------------------------------------- uv_drag
+__global__ void km_sponge(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = blockIdx.y;
+  if (K > d.k_sponge) return;
+  double *du = curU(d), *dv = curV(d);
+  const size_t q = ix3(d, K, J, I);
+  const float ts = (float)d.t_sponge[K];
+  du[q] -= (double)(ts * (float)(d.U[q] - d.USPIN[q]));
+  if (J >= 1) dv[q] -= (double)(ts * (float)d.V[q]);
+}
+
+__device__ __forceinline__ float t_rad_hs_f(const MemberDesc &d, int K, int J, int I) {
+  const float ka = (float)(1. / (40. * 60. * 60. * 24.)), ks = (float)(1. / (4. * 60. * 60. * 24.));
+  float p = (float)d.P3[ix3(d, K, J, I)], kt = ka, pbot = (float)d.P3[ix3(d, d.nk, J, I)];
+  float amp = (p / pbot - 0.7f) / (1.f - 0.7f);
+  if (amp > 0.f) {
+    float c = cosf((float)d.lat[2 * J + 1] * (float)(M_PI / 180.));
+    kt += (ks - ka) * (c * c) * (c * c) * amp;
+  }
+  return 1.f / kt;
+}
+
+// Rayleigh drag: one wavefront per (K,J) row, the zonal mean by a wavefront
+// reduction
+__global__ void km_rayleigh(const MemberDesc *D, int mb) {
+  MEMBER;
+  if (!((d.newt_cool_on && d.prandtl > 0.) || (d.tau_drag > 0. && d.tau_drag < 1.e+20))) return;
+  const int K = d.k_sponge + 1 + blockIdx.y, J = blockIdx.x;
+  if (K > d.nk - 1 || J > d.nj) return;
+  double *du = curU(d), *dv = curV(d);
+  const int ni = d.ni, lane = threadIdx.x;
+  const float za = (float)d.drag_zonal_avg, zb = (float)(1 - d.drag_zonal_avg);
+  for (int w = 0; w < 2; w++) {
+    if (w == 1 && !(d.drag_v && J >= 1)) break;
+    const double *X = w ? d.V : d.U;
+    float s = 0.f;
+    for (int I = 1 + lane; I <= ni; I += 64) s += (float)X[ix3(d, K, J, I)];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    const float avg = s / (float)ni;
+    for (int I = 1 + lane; I <= ni; I += 64) {
+      const size_t q = ix3(d, K, J, I);
+      float ray = za * avg + zb * (float)X[q];
+      float nu0;
+      if (d.newt_cool_on) {
+        float t0 = t_rad_hs_f(d, K, J, I), t1 = w ? t_rad_hs_f(d, K, J - 1, I) : t_rad_hs_f(d, K, J, I - 1);
+        nu0 = (float)d.prandtl / (.5f * (t0 + t1));
+      } else {
+        nu0 = (float)(1. / d.tau_drag);
+      }
+      if (w == 0) du[q] -= (double)(nu0 * (ray - (float)d.USPIN[q]));
+      else dv[q] -= (double)(nu0 * ray);
+    }
+  }
+}
+
+__global__ void km_hs_pbl(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  if (!d.is_hs) return;
+  const float nu00 = (float)(1. / (24. * 60. * 60.));
+  const int nk = d.nk;
+  double *du = curU(d), *dv = curV(d);
+  const double *P = d.P3;
+  float pb = 0.5f * (float)(P[ix3(d, nk, J, I)] + P[ix3(d, nk, J, I - 1)]);
+  for (int K = nk - 1; K >= 1; K--) {
+    float p3 = .5f * (float)(P[ix3(d, K, J, I)] + P[ix3(d, K, J, I - 1)]);
+    float amp = (p3 / pb - 0.7f) / (1.f - 0.7f);
+    if (amp < 0.f) break;
+    du[ix3(d, K, J, I)] -= (double)(nu00 * amp * (float)d.U[ix3(d, K, J, I)]);
+  }
+  if (J >= 1) {
+    pb = 0.5f * (float)(P[ix3(d, nk, J, I)] + P[ix3(d, nk, J - 1, I)]);
+    for (int K = nk - 1; K >= 1; K--) {
+      float p3 = .5f * (float)(P[ix3(d, K, J, I)] + P[ix3(d, K, J - 1, I)]);
+      float amp = (p3 / pb - 0.7f) / (1.f - 0.7f);
+      if (amp < 0.f) break;
+      dv[ix3(d, K, J, I)] -= (double)(nu00 * amp * (float)d.V[ix3(d, K, J, I)]);
+    }
+  }
+}
+
+// ------------------------------------------------------------- pgrad
+struct ZS {  // geopotential shifted by the cell's own value, formed in FP64
+  const MemberDesc *d;
+  double ref;
+  __device__ float operator()(int k, int j, int i) const { return (float)(d->GZ3[ix3(*d, k, j, i)] - ref); }
+};
+
+__device__ __forceinline__ float face_lon_f(const MemberDesc &d, const ZS &Z, int K, int J, int I, float &zbse,
+                                            float &zbne, float &ztse, float &ztne, float &zbe, float &zte, float &pbe,
+                                            float &pte) {
+  zbse = .25f * (Z(K, J, I) + Z(K, J - 1, I) + Z(K + 1, J, I) + Z(K + 1, J - 1, I));
+  zbne = .25f * (Z(K, J, I) + Z(K, J + 1, I) + Z(K + 1, J, I) + Z(K + 1, J + 1, I));
+  ztse = .25f * (Z(K, J, I) + Z(K, J - 1, I) + Z(K - 1, J, I) + Z(K - 1, J - 1, I));
+  ztne = .25f * (Z(K, J, I) + Z(K, J + 1, I) + Z(K - 1, J, I) + Z(K - 1, J + 1, I));
+  zbe = .5f * (Z(K, J, I) + Z(K + 1, J, I));
+  zte = .5f * (Z(K, J, I) + Z(K - 1, J, I));
+  pbe = (float)d.P2[ix3(d, K + 1, J, I)];
+  pte = (float)d.P2[ix3(d, K, J, I)];
+  float a1 = ztne + ztse, b1 = ztne * ztse, a2 = zbne + zbse, b2 = zbne * zbse;
+  float t1 = a1 * a1 - b1 - (a2 * a2 - b2), t2 = 3.f * (a1 - a2);
+  return (-t1 * (pbe - pte) + t2 * (zte * pbe - zbe * pte)) / (zte - zbe);
+}
+
+__device__ __forceinline__ float face_lat_f(const MemberDesc &d, const ZS &Z, int K, int J, int I, float rlnn,
+                                            float &zbnw, float &zbne, float &ztnw, float &ztne, float &zbn, float &ztn,
+                                            float &pbn, float &ptn) {
+  zbnw = .25f * (Z(K, J, I) + Z(K, J, I - 1) + Z(K + 1, J, I) + Z(K + 1, J, I - 1));
+  zbne = .25f * (Z(K, J, I) + Z(K, J, I + 1) + Z(K + 1, J, I) + Z(K + 1, J, I + 1));
+  ztnw = .25f * (Z(K, J, I) + Z(K, J, I - 1) + Z(K - 1, J, I) + Z(K - 1, J, I - 1));
+  ztne = .25f * (Z(K, J, I) + Z(K, J, I + 1) + Z(K - 1, J, I) + Z(K - 1, J, I + 1));
+  zbn = .5f * (Z(K, J, I) + Z(K + 1, J, I));
+  ztn = .5f * (Z(K, J, I) + Z(K - 1, J, I));
+  pbn = (float)d.P2[ix3(d, K + 1, J, I)];
+  ptn = (float)d.P2[ix3(d, K, J, I)];
+  float a1 = ztne + ztnw, b1 = ztne * ztnw, a2 = zbne + zbnw, b2 = zbne * zbnw;
+  float t1 = a1 * a1 - b1 - (a2 * a2 - b2), t2 = 3.f * (a1 - a2);
+  return rlnn * (-t1 * (pbn - ptn) + t2 * (ztn * pbn - zbn * ptn)) / (ztn - zbn);
+}
+
+__global__ void km_pgrad(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const float TOL = 1.e-2f;
+  const double DG = M_PI / 180.;
+  const double *P2 = d.P2;
+  double *du = curU(d), *dv = curV(d);
+  const ZS Z{&d, d.GZ3[ix3(d, K, J, I)]};
+#define PZ(k, j, i) ((float)P2[ix3(d, k, j, i)])
+  {
+    float rlns = (float)d.rln[2 * J], rlnn = (float)d.rln[2 * J + 2];
+    float zbsw, zbnw, ztsw, ztnw, zbw, ztw, pbw, ptw;
+    float west = face_lon_f(d, Z, K, J, I - 1, zbsw, zbnw, ztsw, ztnw, zbw, ztw, pbw, ptw);
+    float zbse, zbne, ztse, ztne, zbe, zte, pbe, pte;
+    float east = face_lon_f(d, Z, K, J, I, zbse, zbne, ztse, ztne, zbe, zte, pbe, pte);
+    float dln = (float)((d.lon[2 * I + 1] - d.lon[2 * (I - 1) + 1]) * DG);
+    float rho = .5f * (float)(d.RHO3[ix3(d, K, J, I)] + d.RHO3[ix3(d, K, J, I - 1)]);
+    float top, bottom, a1, a2, b1, b2, t1, t2;
+    a1 = ztne + ztse; a2 = ztnw + ztsw;
+    t2 = 3.f * (a1 - a2);
+    if (fabsf(2.f * (zte - ztw) / (zte + ztw - zbe - zbw)) > TOL) {
+      b1 = ztne * ztse; b2 = ztnw * ztsw;
+      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+      top = (-t1 * (ptw - pte) + t2 * (zte * ptw - ztw * pte)) / (zte - ztw);
+    } else {
+      top = .5f * (ptw + pte) * t2;
+    }
+    a1 = zbne + zbse; a2 = zbnw + zbsw;
+    t2 = 3.f * (a1 - a2);
+    if (fabsf(2.f * (zbe - zbw) / (zte + ztw - zbe - zbw)) > TOL) {
+      b1 = zbne * zbse; b2 = zbnw * zbsw;
+      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+      bottom = (-t1 * (pbw - pbe) + t2 * (zbe * pbw - zbw * pbe)) / (zbe - zbw);
+    } else {
+      bottom = .5f * (pbw + pbe) * t2;
+    }
+    float pgf = top - bottom + west - east;
+    float dzn = ztne - zbne + ztnw - zbnw, dzs = ztse - zbse + ztsw - zbsw;
+    pgf /= dln * rho * (rlns * (dzs + .5f * dzn) + rlnn * (dzn + .5f * dzs));
+    du[ix3(d, K, J, I)] += (double)pgf;
+  }
+  if (J >= 1) {
+    float rlns = (float)d.rln[2 * J - 1];
+    float zbsw, zbse, ztsw, ztse, zbs, zts, pbs, pts;
+    float south = face_lat_f(d, Z, K, J - 1, I, rlns, zbsw, zbse, ztsw, ztse, zbs, zts, pbs, pts);
+    float rlnn = (float)d.rln[2 * J + 1];
+    float zbnw, zbne, ztnw, ztne, zbn, ztn, pbn, ptn;
+    float north = face_lat_f(d, Z, K, J, I, rlnn, zbnw, zbne, ztnw, ztne, zbn, ztn, pbn, ptn);
+    float dlt = (float)((d.lat[2 * J + 1] - d.lat[2 * J - 1]) * DG);
+    float rho = .5f * (float)(d.RHO3[ix3(d, K, J, I)] + d.RHO3[ix3(d, K, J - 1, I)]);
+    float rlt = (float)d.rlt[2 * J];
+    float a1, a2, b1, b2, b3, t1, t2;
+    float zbe = .125f * (Z(K, J, I) + Z(K, J, I + 1) + Z(K, J - 1, I) + Z(K, J - 1, I + 1) + Z(K + 1, J, I) +
+                         Z(K + 1, J, I + 1) + Z(K + 1, J - 1, I) + Z(K + 1, J - 1, I + 1));
+    float zte = .125f * (Z(K - 1, J, I) + Z(K - 1, J, I + 1) + Z(K - 1, J - 1, I) + Z(K - 1, J - 1, I + 1) +
+                         Z(K, J, I) + Z(K, J, I + 1) + Z(K, J - 1, I) + Z(K, J - 1, I + 1));
+    float pbe = .25f * (PZ(K + 1, J, I) + PZ(K + 1, J, I + 1) + PZ(K + 1, J - 1, I) + PZ(K + 1, J - 1, I + 1));
+    float pte = .25f * (PZ(K, J, I) + PZ(K, J, I + 1) + PZ(K, J - 1, I) + PZ(K, J - 1, I + 1));
+    a1 = ztne + ztse; b1 = ztne * ztse; a2 = zbne + zbse; b2 = zbne * zbse;
+    t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+    t2 = 3.f * (a1 - a2);
+    float east = .5f * (rlns - rlnn) * (-t1 * (pbe - pte) + t2 * (zte * pbe - zbe * pte)) / (zte - zbe);
+    float zbw = .125f * (Z(K, J, I) + Z(K, J, I - 1) + Z(K, J - 1, I) + Z(K, J - 1, I - 1) + Z(K + 1, J, I) +
+                         Z(K + 1, J, I - 1) + Z(K + 1, J - 1, I) + Z(K + 1, J - 1, I - 1));
+    float ztw = .125f * (Z(K - 1, J, I) + Z(K - 1, J, I - 1) + Z(K - 1, J - 1, I) + Z(K - 1, J - 1, I - 1) +
+                         Z(K, J, I) + Z(K, J, I - 1) + Z(K, J - 1, I) + Z(K, J - 1, I - 1));
+    float pbw = .25f * (PZ(K + 1, J, I) + PZ(K + 1, J, I - 1) + PZ(K + 1, J - 1, I) + PZ(K + 1, J - 1, I - 1));
+    float ptw = .25f * (PZ(K, J, I) + PZ(K, J, I - 1) + PZ(K, J - 1, I) + PZ(K, J - 1, I - 1));
+    a1 = ztnw + ztsw; b1 = ztnw * ztsw; a2 = zbnw + zbsw; b2 = zbnw * zbsw;
+    t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+    t2 = 3.f * (a1 - a2);
+    float west = .5f * (rlns - rlnn) * (-t1 * (pbw - ptw) + t2 * (ztw * pbw - zbw * ptw)) / (ztw - zbw);
+    float top, bottom;
+    a1 = ztne + ztnw; a2 = ztse + ztsw;
+    t2 = 3.f * (a1 - a2);
+    if (fabsf(2.f * (ztn - zts) / (ztn + zts - zbn - zbs)) > TOL) {
+      b1 = ztne * ztnw; b2 = ztse * ztsw; b3 = ztne * ztse + ztnw * ztsw;
+      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+      top = .5f * ((ptn - pts) * ((rlnn + rlns) * t1 + (rlnn - rlns) * (b1 + b2 - b3)) +
+                   (rlnn + rlns) * t2 * (ztn * pts - zts * ptn)) / (ztn - zts);
+    } else {
+      top = .25f * (rlnn + rlns) * (pts + ptn) * t2;
+    }
+    a1 = zbne + zbnw; a2 = zbse + zbsw;
+    t2 = 3.f * (a1 - a2);
+    if (fabsf(2.f * (zbn - zbs) / (ztn + zts - zbn - zbs)) > TOL) {
+      b1 = zbne * zbnw; b2 = zbse * zbsw; b3 = zbne * zbse + zbnw * zbsw;
+      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
+      bottom = .5f * ((pbn - pbs) * ((rlnn + rlns) * t1 + (rlnn - rlns) * (b1 + b2 - b3)) +
+                      (rlnn + rlns) * t2 * (zbn * pbs - zbs * pbn)) / (zbn - zbs);
+    } else {
+      bottom = .25f * (rlnn + rlns) * (pbs + pbn) * t2;
+    }
+    float pgf = top - bottom + south - (north + west + east);
+    float dzn = ztne - zbne + ztnw - zbnw, dzs = ztse - zbse + ztsw - zbsw;
+    pgf /= dlt * rho * rlt * (rlns * (dzs + .5f * dzn) + rlnn * (dzn + .5f * dzs));
+    dv[ix3(d, K, J, I)] += (double)pgf;
+  }
+#undef PZ
+}
+
+// ------------------------------------------------------ divergence damping
+__global__ void km_divdamp(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  if (d.nudiv == 0.) return;
+  const double *div = d.sB + (size_t)K * d.N2;
+  double *du = curU(d), *dv = curV(d);
+  float coef = (float)((d.lat[2 * J + 1] > 45.) ? d.nudiv * d.m0z : d.nudiv * d.m[2 * J + 1]);
+  du[ix3(d, K, J, I)] += (double)(coef * (float)(div[ix2(d, J, I)] - div[ix2(d, J, I - 1)]));
+  if (J >= 1) {
+    float cv = (float)(d.nudiv * d.n[2 * J]);
+    dv[ix3(d, K, J, I)] += (double)(cv * (float)(div[ix2(d, J, I)] - div[ix2(d, J - 1, I)]));
+  }
+}
+
+// ------------------------------------------------------- Adams-Bashforth
+// FP64 accumulation: the current (FP64) tendency and the FP32 history
+// planes IT_MINUS1, IT_MINUS2 (FLT_MAX flags start AB1/AB2); the current
+// tendency is then stored, rounded to FP32, in the history slot IT_ZERO,
+// which the rotation at the end of the step makes IT_MINUS1.  Lanes cover
+// every cell of planes 1..nk-1 so the stored history is the whole plane.
+__global__ void km_ab(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const MemberDyn dy = *d.dyn;
+  const float FMAX = 3.40282347e+38f;
+  for (int w = 0; w < 2; w++) {
+    const int jlo = w;
+    double *val = w ? d.V : d.U;
+    const double *cur = w ? d.DVDT : d.DUDT;
+    float *hist = w ? d.DVDTf : d.DUDTf;
+    const size_t q = ix3(d, K, J, I);
+    const double t0 = cur[q];
+    if (J >= jlo) {
+      double ab0, ab1, ab2;
+      if (hist[(size_t)dy.itm2 * d.N3 + ix3(d, 1, jlo, 1)] == FMAX) {
+        if (hist[(size_t)dy.itm1 * d.N3 + ix3(d, 1, jlo, 1)] == FMAX) { ab0 = 1.; ab1 = 0.; ab2 = 0.; }
+        else { ab0 = 3. / 2.; ab1 = -1. / 2.; ab2 = 0.; }
+      } else {
+        ab0 = 23. / 12.; ab1 = -16. / 12.; ab2 = 5. / 12.;
+      }
+      const double t1 = (ab1 != 0.) ? hist[(size_t)dy.itm1 * d.N3 + q] : 0.,
+                   t2 = (ab2 != 0.) ? hist[(size_t)dy.itm2 * d.N3 + q] : 0.;
+      const double y = val[q] + d.dt * (ab0 * t0 + ab1 * t1 + ab2 * t2);
+      val[q] = y;
+      if (I == d.ni) val[ix3(d, K, J, 0)] = y;
+      if (I == 1) val[ix3(d, K, J, d.ni + 1)] = y;
+    }
+    hist[(size_t)dy.it0 * d.N3 + q] = (float)t0;
+  }
+}
+
+// ------------------------------------------------------------- launchers
+void LM_zero_tend(hipStream_t s, const Bucket &b) {
+  LAUNCH(km_zero_tend, dim3(256, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+void LM_uv_core(hipStream_t s, const Bucket &b) {
+  const int nK = b.nk - 1;
+  LAUNCH(km_uvh_kin, grid2(b.nj + 2, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  LAUNCH(km_dvdt_core, grid2(b.nj, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  L_filter_f32(s, b, F_SC, 1, b.nk - 1, 0, 0);
+  LAUNCH(km_dudt_core, grid2(b.nj + 1, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+void LM_uv_drag(hipStream_t s, const Bucket &b) {
+  LAUNCH(km_sponge, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  LAUNCH(km_rayleigh, dim3(b.nj + 1, b.nk, b.nm), dim3(64), 0, s, b.D, b.mb);
+  LAUNCH(km_hs_pbl, grid2(b.nj + 1, b.ni, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+void LM_pgrad(hipStream_t s, const Bucket &b) {
+  LAUNCH(km_pgrad, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+void LM_divdamp_apply(hipStream_t s, const Bucket &b) {
+  LAUNCH(km_divdamp, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+void LM_ab(hipStream_t s, const Bucket &b) {
+  LAUNCH(km_ab, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+
+}  // namespace epic

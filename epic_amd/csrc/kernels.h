@@ -24,6 +24,7 @@ struct Bucket {
   const MemberDesc *D;  // device array of descriptors
   int mb, nm;           // first member, member count
   int ni, nj, nk;
+  int mixed;            // FP32 tendencies and horizontal arithmetic
 };
 
 
@@ -47,6 +48,7 @@ void L_zero_dzdt(hipStream_t s, const Bucket &b);
 // 2-D filter of planes k0..k1 of fld; vgrid selects the v/pv rows; mode 0
 // also refreshes the zonal pads of every row, mode 1 leaves pads alone.
 void L_filter(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode);
+void L_filter_f32(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode);
 // the 3-D filter with the PCHIP profile removal (kind: 0 U, 1 V, 2 HDRY, 3 THETA)
 void L_filter3(hipStream_t s, const Bucket &b, int fld, int kind);
 
@@ -63,6 +65,14 @@ void L_restore_mass(hipStream_t s, const Bucket &b);
 // hyperviscosity: kind 0 U, 1 V, 2 HDRY, 3 THETA
 void L_merid_hv(hipStream_t s, const Bucket &b, int fld, int kind);
 void L_zonal_hv(hipStream_t s, const Bucket &b, int fld, int kind);
+
+// ---- mixed-precision momentum kernels (k_mixed.hip)
+void LM_zero_tend(hipStream_t s, const Bucket &b);
+void LM_uv_core(hipStream_t s, const Bucket &b);
+void LM_uv_drag(hipStream_t s, const Bucket &b);
+void LM_pgrad(hipStream_t s, const Bucket &b);
+void LM_divdamp_apply(hipStream_t s, const Bucket &b);
+void LM_ab(hipStream_t s, const Bucket &b);
 
 // ---- advection (k_adv.hip)
 void L_advection(hipStream_t s, const Bucket &b);

@@ -1,0 +1,184 @@
+"""Continuous-batching request loop over the engine.
+
+Requests are whole integrations: (planet, grid, steps, seed) arriving over
+time.  The loop keeps every admitted integration resident on the device as
+one engine member and advances all of them together (members of one grid
+shape share every kernel launch).  A request is admitted when it has
+arrived, a member slot is free and its HBM footprint fits the budget
+(first come, first served; a request that does not fit blocks only itself,
+later smaller requests may pass it).  Between ticks, finished integrations
+leave and waiting ones join, so new work overlaps with work already on the
+GPU.  Completion latency (arrival to result) and throughput are reported.
+
+The scheduler is independent of the device: `engine` is anything with the
+`epic_amd.Engine` methods used here, which is how the CPU tests drive it.
+"""
+import dataclasses
+import math
+import random
+import time
+
+import numpy as np
+
+import epic_amd as ea
+
+# (ni, nj, nk) grids of the mix and a stable timestep per planet and grid
+GRIDS = [(64, 32, 10), (128, 64, 16), (256, 128, 20), (512, 256, 32)]
+PLANETS = [ea.HELD_SUAREZ, ea.JUPITER_LIKE, ea.SATURN_LIKE, ea.VENUS_LIKE]
+_DT = {  # seconds, by planet then ni
+    ea.HELD_SUAREZ: {64: 600, 128: 300, 256: 150, 512: 75},
+    ea.VENUS_LIKE: {64: 300, 128: 150, 256: 75, 512: 40},
+    ea.JUPITER_LIKE: {64: 60, 128: 60, 256: 60, 512: 30},
+    ea.SATURN_LIKE: {64: 60, 128: 60, 256: 60, 512: 30},
+}
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    planet: int
+    ni: int
+    nj: int
+    nk: int
+    steps: int
+    seed: int
+    arrival: float  # seconds after the start of the run
+
+    def spec(self):
+        gas = self.planet in (ea.JUPITER_LIKE, ea.SATURN_LIKE)
+        return ea.spec(self.planet, self.ni, self.nj, self.nk, _DT[self.planet][self.ni], self.seed, 0.5, 0.0,
+                       0.0 if gas else 0.1, 0.5 if gas else 0.0)
+
+    @property
+    def points(self):
+        return self.ni * (self.nj + 1) * self.nk
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    status: str          # "ok" or "failed: <bits>"
+    arrival: float
+    admitted: float
+    finished: float
+    steps: int
+    checksum: float      # sum of the final u field's interior (a cheap fingerprint)
+    fields: dict = None  # final prognostic fields when the server keeps them
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def poisson_mix(n, rate, seed, grids=GRIDS, planets=PLANETS, steps=(200, 2000)):
+    """n requests with exponential inter-arrival times (rate per second),
+    uniform over grids x planets and steps."""
+    rng = random.Random(seed)
+    t, out = 0.0, []
+    for i in range(n):
+        t += rng.expovariate(rate)
+        ni, nj, nk = rng.choice(grids)
+        out.append(Request(i, rng.choice(planets), ni, nj, nk, rng.randint(*steps), rng.randrange(1 << 31), t))
+    return out
+
+
+class Server:
+    """Admission, stepping and retirement of resident integrations."""
+
+    def __init__(self, engine, hbm_budget, max_members, precision=ea.FP64, chunk=50, keep_fields=False,
+                 footprint=None, clock=time.perf_counter, sleep=time.sleep):
+        self.eng = engine
+        self.budget = hbm_budget
+        self.max_members = max_members
+        self.precision = precision
+        self.chunk = chunk
+        self.keep_fields = keep_fields
+        self.footprint = footprint or (lambda r: ea.footprint(r.ni, r.nj, r.nk, precision))
+        self.clock = clock
+        self.sleep = sleep
+        self.active = {}   # slot -> [request, steps_left, admitted_time]
+        self.in_use = 0
+        self.point_steps = 0
+
+    def _admit(self, waiting, now):
+        admitted = []
+        for r in list(waiting):
+            if r.arrival > now or len(self.active) >= self.max_members:
+                continue
+            fp = self.footprint(r)
+            if self.in_use + fp > self.budget:
+                continue
+            slot = self.eng.add(r.spec())
+            self.active[slot] = [r, r.steps, now, fp]
+            self.in_use += fp
+            waiting.remove(r)
+            admitted.append(r)
+        return admitted
+
+    def _retire(self, done_slots, now, results):
+        for slot in done_slots:
+            r, left, t_adm, fp = self.active.pop(slot)
+            err = self.eng.member_error(slot)
+            fields = None
+            u = self.eng.read(slot, "U")
+            checksum = float(u[1:r.nk + 1, 1:r.nj + 2, 1:r.ni + 1].sum())
+            if self.keep_fields:
+                fields = {f: self.eng.read(slot, f) for f in ("U", "V", "HDRY", "THETA", "P3")}
+            self.eng.remove(slot)
+            self.in_use -= fp
+            status = "ok" if err == 0 else f"failed: {err}"
+            results.append(Result(r.rid, status, r.arrival, t_adm, now, r.steps - left, checksum, fields))
+
+    def run(self, requests, time_limit=None):
+        """Serve `requests` (arrival times relative to the call); returns the
+        results in completion order."""
+        waiting = sorted(requests, key=lambda r: r.arrival)
+        results = []
+        # a request larger than the whole budget can never be admitted
+        for r in [r for r in waiting if self.footprint(r) > self.budget]:
+            waiting.remove(r)
+            results.append(Result(r.rid, "rejected: footprint exceeds the HBM budget", r.arrival, r.arrival,
+                                  r.arrival, 0, 0.0))
+        t0 = self.clock()
+        while waiting or self.active:
+            now = self.clock() - t0
+            if time_limit is not None and now > time_limit:
+                break
+            self._admit(waiting, now)
+            if not self.active:
+                # idle until the next arrival
+                self.sleep(max(0.0, waiting[0].arrival - (self.clock() - t0)))
+                continue
+            n = min(self.chunk, min(v[1] for v in self.active.values()))
+            self.eng.step(n)
+            self.eng.wait()
+            for v in self.active.values():
+                v[1] -= n
+                self.point_steps += v[0].points * n
+            now = self.clock() - t0
+            self._retire([s for s, v in self.active.items() if v[1] == 0], now, results)
+        self.elapsed = self.clock() - t0
+        return results
+
+
+def summarize(results, elapsed, point_steps):
+    lat = np.array([r.latency for r in results]) if results else np.zeros(1)
+    return {
+        "requests": len(results),
+        "failed": sum(r.status.startswith("failed") for r in results),
+        "rejected": sum(r.status.startswith("rejected") for r in results),
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(lat.mean()),
+        "throughput_pt_steps_per_s": point_steps / elapsed if elapsed > 0 else 0.0,
+        "elapsed_s": elapsed,
+    }
+
+
+def serve(requests, precision=ea.FP64, hbm_budget=64 << 30, max_members=64, chunk=50, keep_fields=False):
+    """Run a request list on the GPU; returns (results, summary)."""
+    eng = ea.Engine(precision, max_members, hbm_budget)
+    srv = Server(eng, hbm_budget, max_members, precision, chunk, keep_fields)
+    res = srv.run(requests)
+    eng.close()
+    return res, summarize(res, srv.elapsed, srv.point_steps)

@@ -277,8 +277,13 @@ int  epic_engine_read(epic_engine *e, int slot, int field, double *out);
 int  epic_engine_write(epic_engine *e, int slot, int field, const double *in);
 long epic_engine_steps_done(epic_engine *e, int slot);
 size_t epic_engine_bytes_in_use(epic_engine *e);
-/* Wait for all queued device work. */
+/* Wait for all queued device work; -4 if any member has raised a fatal
+ * condition (non-monotone GZ3 = 1, pressure out of range = 2, spline = 4). */
 int  epic_engine_sync(epic_engine *e);
+/* Wait for all queued device work without checking the members. */
+int  epic_engine_wait(epic_engine *e);
+/* The sticky fatal-condition bits of one member (0: healthy). */
+int  epic_engine_member_error(epic_engine *e, int slot);
 /* Record device timing of each kernel over the next `nsteps`; fills
  * names/ms arrays (up to cap entries).  Returns the entry count. */
 int  epic_engine_profile(epic_engine *e, int nsteps, char (*names)[48], double *ms_total,

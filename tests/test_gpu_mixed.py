@@ -1,0 +1,78 @@
+"""The mixed-precision mode (FP32 tendencies and tendency arithmetic, FP64
+state, hydrostatics and Adams-Bashforth accumulation) against the FP64
+path.  Every field's relative L2 and max error is reported; u, v, h, theta
+and p must stay within 1e-5 relative L2 of the FP64 result after 1000 steps
+of the Jupiter-like 256x128x20 case (dt = 60 s), and the global mass and
+total energy must drift no more than the FP64 run's by 1e-6 relative."""
+import numpy as np
+import pytest
+
+import epic_amd as ea
+from tests.gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("U", "V", "HDRY", "THETA", "P3")
+
+
+def interior(a, sp):
+    return a[1:sp.nk + 1, 1:sp.nj + 2, 1:sp.ni + 1]
+
+
+def mass_energy(eng, slot, sp):
+    """Global dry mass sum(h*area) and a kinetic + enthalpy energy proxy."""
+    h = interior(eng.read(slot, "HDRY"), sp)
+    u = interior(eng.read(slot, "U"), sp)
+    t = interior(eng.read(slot, "T3"), sp)
+    mass = float(h.sum())
+    energy = float((h * (0.5 * u * u + t)).sum())
+    return mass, energy
+
+
+def run(sp, precision, steps):
+    e = ea.Engine(precision, 1)
+    s = e.add(sp)
+    m0 = mass_energy(e, s, sp)
+    e.step(steps)
+    e.sync()
+    out = {f: e.read(s, f) for f in FIELDS + ("GZ3", "PV3")}
+    m1 = mass_energy(e, s, sp)
+    e.close()
+    return out, m0, m1
+
+
+@pytest.mark.parametrize("steps", [1000])
+def test_mixed_against_fp64_jupiter_256x128x20(steps):
+    sp = ea.spec(ea.JUPITER_LIKE, 256, 128, 20, 60, 1, 0.5, 0.0, 0.0, 0.5)
+    ref, r0, r1 = run(sp, ea.FP64, steps)
+    mix, x0, x1 = run(sp, ea.MIXED, steps)
+    worst = {}
+    for f in FIELDS + ("GZ3", "PV3"):
+        a, b = interior(mix[f], sp), interior(ref[f], sp)
+        l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        mx = float(np.abs(a - b).max() / np.abs(b).max())
+        worst[f] = l2
+        say(f"mixed vs fp64 after {steps} steps {f}: rel L2 {l2:.2e}  rel max {mx:.2e}")
+    dm_ref, dm_mix = (r1[0] - r0[0]) / r0[0], (x1[0] - x0[0]) / x0[0]
+    de_ref, de_mix = (r1[1] - r0[1]) / r0[1], (x1[1] - x0[1]) / x0[1]
+    say(f"mass drift fp64 {dm_ref:.3e} mixed {dm_mix:.3e}; energy drift fp64 {de_ref:.3e} mixed {de_mix:.3e}")
+    for f in FIELDS:
+        assert worst[f] <= 1e-5, (f, worst[f])
+    assert abs(dm_mix - dm_ref) <= 1e-6 and abs(de_mix - de_ref) <= 1e-6
+
+
+def test_mixed_alone_equals_mixed_in_batch():
+    a = ea.spec(ea.HELD_SUAREZ, 64, 32, 10, 600, 1, 0.5, 0.0, 0.1, 0.0)
+    b = ea.spec(ea.JUPITER_LIKE, 64, 32, 10, 60, 2, 0.5, 0.0, 0.0, 0.5)
+    e1 = ea.Engine(ea.MIXED, 1)
+    s1 = e1.add(a)
+    e1.step(20)
+    alone = {f: e1.read(s1, f) for f in FIELDS}
+    e1.close()
+    e2 = ea.Engine(ea.MIXED, 2)
+    s2 = e2.add(a)
+    e2.add(b)
+    e2.step(20)
+    for f in FIELDS:
+        assert np.array_equal(e2.read(s2, f), alone[f]), f
+    e2.close()

@@ -1,0 +1,107 @@
+"""CPU tests of the request loop's scheduling: admission by HBM footprint
+and free slots, continuous batching (work joins while other work is on
+the device), retirement, latency bookkeeping.  The engine is a host-side
+stand-in with the same methods; the GPU test runs the real one."""
+import numpy as np
+
+import epic_amd as ea
+from epic_amd import serving as sv
+
+
+class FakeEngine:
+    """Advances step counters only; the clock charges each step by the
+    resident grid points."""
+
+    def __init__(self, clock):
+        self.clock = clock
+        self.members = {}
+        self.next = 0
+        self.max_resident = 0
+        self.joined_while_busy = 0
+
+    def add(self, sp):
+        if self.members:
+            self.joined_while_busy += 1
+        s = self.next
+        self.next += 1
+        self.members[s] = [sp, 0]
+        self.max_resident = max(self.max_resident, len(self.members))
+        return s
+
+    def step(self, n):
+        pts = sum(sp.ni * (sp.nj + 1) * sp.nk for sp, _ in self.members.values())
+        for m in self.members.values():
+            m[1] += n
+        self.clock.t += n * pts * 1e-10
+
+    def wait(self):
+        pass
+
+    def member_error(self, slot):
+        return 0
+
+    def read(self, slot, f):
+        sp, n = self.members[slot]
+        return np.full((sp.nk + 2, sp.nj + 3, sp.ni + 2), float(n))
+
+    def remove(self, slot):
+        del self.members[slot]
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += dt
+
+
+def footprint(r):
+    return (r.ni + 2) * (r.nj + 3) * (r.nk + 2) * 8 * 36
+
+
+def test_admission_respects_budget_and_everything_completes():
+    reqs = sv.poisson_mix(40, rate=20.0, seed=3, steps=(20, 200))
+    clk = Clock()
+    eng = FakeEngine(clk)
+    budget = 3 * footprint(sv.Request(0, 0, 512, 256, 32, 1, 0, 0.))
+    srv = sv.Server(eng, budget, max_members=8, chunk=25, footprint=footprint, clock=clk, sleep=clk.sleep)
+    peak = [0]
+    orig_add = eng.add
+
+    def add(sp):
+        slot = orig_add(sp)
+        peak[0] = max(peak[0], srv.in_use + footprint(sv.Request(0, 0, sp.ni, sp.nj, sp.nk, 1, 0, 0.)))
+        return slot
+
+    eng.add = add
+    res = srv.run(reqs)
+    assert len(res) == 40 and all(r.status == "ok" for r in res)
+    assert peak[0] <= budget
+    assert eng.max_resident <= 8
+    assert eng.joined_while_busy > 0  # continuous batching, not a queue
+    for r in res:
+        req = reqs[r.rid]
+        assert r.steps == req.steps and r.checksum == req.steps * req.ni * (req.nj + 1) * req.nk
+        assert r.admitted >= r.arrival and r.finished >= r.admitted
+    summ = sv.summarize(res, srv.elapsed, srv.point_steps)
+    assert summ["p99_latency_s"] >= summ["p50_latency_s"] > 0
+    assert srv.point_steps == sum(r.steps * r.points for r in reqs)
+
+
+def test_oversized_request_is_rejected_not_waited_on():
+    clk = Clock()
+    srv = sv.Server(FakeEngine(clk), 10 * 2 ** 20, 4, footprint=footprint, clock=clk, sleep=clk.sleep)
+    reqs = [sv.Request(0, 0, 512, 256, 32, 10, 1, 0.0), sv.Request(1, 0, 64, 32, 10, 10, 2, 0.1)]
+    res = srv.run(reqs)
+    st = {r.rid: r.status for r in res}
+    assert st[0].startswith("rejected") and st[1] == "ok"
+
+
+def test_poisson_mix_is_seeded():
+    a = sv.poisson_mix(10, 5.0, 7)
+    b = sv.poisson_mix(10, 5.0, 7)
+    assert a == b and all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
