@@ -42,8 +42,9 @@ WORKLOADS = {
 KERNEL_BYTES = {
     "k_set_p2etc": 56, "k_thermo": 72, "k_gz3": 40, "k_calc_w": 56, "k_dzdt3": 72,
     "k_hsu_vert": 24, "k_hsu_mer_ap": 24, "k_hsu_mer_ff": 32, "k_hsu_mer_fin": 32, "k_hsu_zonal": 32,
-    "k_filter3": 24, "k_fft_filter": 16, "k_pgrad": 48, "k_zonal_hv": 16, "k_d2dy2": 16, "k_hv_apply": 24,
+    "k_filter3<T>": 24, "k_fft_filter<T>": 16, "k_pgrad": 48, "k_zonal_hv": 16, "k_d2dy2x2": 16, "k_hv_apply": 24,
     "k_uvh_kin": 48, "k_dvdt_core": 40, "k_dudt_core": 40, "k_ab": 80, "k_divdamp": 40,
+    "km_uvh_kin": 36, "km_dvdt_core": 32, "km_dudt_core": 32, "km_ab": 64, "km_pgrad": 48, "km_divdamp": 40,
     "k_restore_mass": 16, "k_gz_mono": 16, "k_copy_planes": 16, "k_zero_tend": 16, "k_adv_elem": 16,
     "k_heating": 24, "k_sponge": 24, "k_hs_pbl": 32, "k_divergence": 24, "k_vort_interior": 32,
 }
@@ -60,7 +61,7 @@ def kernel_bytes(name, ni, nj, nk, members):
     """Algorithmic bytes of one launch of `name` (None if not tabulated)."""
     if name not in KERNEL_BYTES:
         return None
-    if name == "k_fft_filter":
+    if name == "k_fft_filter<T>":
         cells = ni * filtered_rows(nj) * nk
     else:
         cells = ni * (nj + 1) * nk
@@ -86,20 +87,10 @@ def cpu_baseline(planet, ni, nj, nk, dt, budget_s):
 
 
 def copy_probe(nbytes=256 << 20, reps=20):
-    """Bandwidth of a plain device-to-device copy (read + write bytes / s)."""
-    import torch
-    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
-    y = torch.empty_like(x)
-    for _ in range(3):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(reps):
-        y.copy_(x)
-    b.record()
-    b.synchronize()
-    s = a.elapsed_time(b) / 1e3 / reps
+    """Bandwidth of a plain device-to-device streaming copy (k_copy_probe):
+    (read + write bytes) / s."""
+    import epic_amd as ea
+    s = ea.copy_probe(nbytes, reps)
     return {"bytes_per_copy": 2 * nbytes, "GBps": 2 * nbytes / s / 1e9, "copy_us": s * 1e6}
 
 

@@ -72,6 +72,7 @@ def lib():
         L.epic_engine_last_error.restype = ctypes.c_char_p
         L.epic_engine_time_planes.argtypes = [vp, ci, ctypes.POINTER(ci)]
         L.epic_engine_profile.argtypes = [vp, ci, vp, ctypes.POINTER(cd), ctypes.POINTER(ci), ci]
+        L.epic_copy_probe.argtypes = [ctypes.c_size_t, ci, ctypes.POINTER(cd)]
         L.epic_member_footprint.argtypes = [ci, ci, ci, ci]
         L.epic_member_footprint.restype = ctypes.c_size_t
         _lib = L
@@ -182,6 +183,15 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def copy_probe(nbytes=256 << 20, reps=20):
+    """Seconds per plain device copy of nbytes (nbytes read + nbytes written)."""
+    sec = ctypes.c_double()
+    rc = lib().epic_copy_probe(nbytes, reps, ctypes.byref(sec))
+    if rc:
+        raise EpicError(f"copy probe failed ({rc})")
+    return sec.value
 
 
 def footprint(ni, nj, nk, precision=FP64):

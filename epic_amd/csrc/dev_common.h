@@ -145,6 +145,33 @@ __device__ __forceinline__ int table_place_s(int n, const double *x, int st, dou
   return il;
 }
 
+// Barrier among the lanes of one wavefront (LDS traffic of a single wave is
+// ordered; the fences keep the compiler from moving accesses across it).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// s + a[0] + a[1] + ... + a[n-1] (or s - a[0] - a[1] - ...), accumulated left
+// to right as the reference's loops do; the loads are issued eight at a time
+// ahead of the additions so the chain waits on additions only.
+template <bool SUB = false>
+__device__ __forceinline__ double seq_acc(const double *a, int n, double s = 0.) {
+  int i = 0;
+  for (; i + 8 <= n; i += 8) {
+    const double v0 = a[i], v1 = a[i + 1], v2 = a[i + 2], v3 = a[i + 3];
+    const double v4 = a[i + 4], v5 = a[i + 5], v6 = a[i + 6], v7 = a[i + 7];
+    if (SUB) { s -= v0; s -= v1; s -= v2; s -= v3; s -= v4; s -= v5; s -= v6; s -= v7; }
+    else { s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7; }
+  }
+  for (; i < n; i++) {
+    if (SUB) s -= a[i];
+    else s += a[i];
+  }
+  return s;
+}
+
 // field selector: which 3-D array of the descriptor
 enum Fld : int {
   F_U, F_V, F_HDRY, F_THETA, F_P3, F_P2, F_THETA2, F_H3, F_GZ3, F_HEAT3, F_PV3, F_DIV, F_W3, F_DZDT3,

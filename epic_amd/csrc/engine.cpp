@@ -719,6 +719,45 @@ extern "C" int epic_engine_profile(epic_engine *E, int nsteps, char (*names)[48]
   return k;
 }
 
+// --------------------------------------------------------- copy probe
+// A plain streaming copy (16-byte vector loads and stores, grid-stride),
+// the reference point for bandwidth figures and for scaling the
+// FETCH_SIZE / WRITE_SIZE counters.
+__global__ void __launch_bounds__(256) k_copy_probe(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) dst[i] = src[i];
+}
+
+extern "C" int epic_copy_probe(size_t nbytes, int reps, double *seconds_per_copy) {
+  if (nbytes < 4096 || reps < 1 || !seconds_per_copy) return -1;
+  const size_t n = nbytes / sizeof(double2);
+  double2 *a = nullptr, *b = nullptr;
+  if (hipMalloc((void **)&a, n * sizeof(double2)) != hipSuccess) return -2;
+  if (hipMalloc((void **)&b, n * sizeof(double2)) != hipSuccess) {
+    (void)hipFree(a);
+    return -2;
+  }
+  int rc = 0;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipMemset(a, 0, n * sizeof(double2));
+  const dim3 g(4096), blk(256);
+  for (int r = 0; r < 3; r++) hipLaunchKernelGGL(k_copy_probe, g, blk, 0, 0, a, b, n);
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_probe, g, blk, 0, 0, a, b, n);
+  (void)hipEventRecord(e1, 0);
+  if (hipEventSynchronize(e1) != hipSuccess) rc = -5;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  *seconds_per_copy = ms / 1e3 / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
+}
+
 extern "C" int epic_engine_add_spec(epic_engine *E, const epic_case_spec *sp) {
   epic_case c;
   int rc = epic_case_build(sp, &c);

@@ -46,7 +46,7 @@ __device__ __forceinline__ double tt_iface(const double *wa, int K, int nk, doub
   return tt * tt;
 }
 
-__global__ void k_hsu_vert(const MemberDesc *D, int mb, int which, int C) {
+__global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, int which, int C) {
   MEMBER;
   extern __shared__ double lds[];
   const int c = threadIdx.x % C, kk = threadIdx.x / C, KS = blockDim.x / C;
@@ -238,7 +238,7 @@ struct MerCol {
   c.DT = d.dt;                                                      \
   const size_t o = (size_t)K * d.N2;
 
-__global__ void k_hsu_mer_ap(const MemberDesc *D, int mb, int which) {
+__global__ void __launch_bounds__(256) k_hsu_mer_ap(const MemberDesc *D, int mb, int which) {
   MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= (d.nj + 1) * d.ni) return;
@@ -246,7 +246,7 @@ __global__ void k_hsu_mer_ap(const MemberDesc *D, int mb, int which) {
   MER_SETUP;
   d.sC[o + ix2(d, J, I)] = c.ap(J);
 }
-__global__ void k_hsu_mer_ff(const MemberDesc *D, int mb, int which) {
+__global__ void __launch_bounds__(256) k_hsu_mer_ff(const MemberDesc *D, int mb, int which) {
   MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= d.nj * d.ni) return;
@@ -271,7 +271,7 @@ __global__ void k_hsu_mer_ff(const MemberDesc *D, int mb, int which) {
   double d2J = (J == nj) ? 0. : APj - c.a(J + 1);
   d.sD[o + ix2(d, J, I)] = fc + -al * (VP * bp * d1J - vhp * bhp * d1m + VM * bm * d2m - vhm * bhm * d2J);
 }
-__global__ void k_hsu_mer_fin(const MemberDesc *D, int mb, int which) {
+__global__ void __launch_bounds__(256) k_hsu_mer_fin(const MemberDesc *D, int mb, int which) {
   MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= (d.nj + 1) * d.ni) return;
@@ -293,7 +293,7 @@ __global__ void k_hsu_mer_fin(const MemberDesc *D, int mb, int which) {
 // One block per (K,J) row of the filtered copy sB, staged in LDS with its
 // pads: velocities, first-guess fluxes, predictor, corrected fluxes and the
 // update sA + d(flux) each run on all lanes over the row, periodic in I.
-__global__ void k_hsu_zonal(const MemberDesc *D, int mb, int which) {
+__global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, int which) {
   MEMBER;
   const int J = blockIdx.x, K = 1 + blockIdx.y;
   if (which == AV_THETA && K <= d.ks - 2) return;
@@ -366,7 +366,7 @@ static void hsu(hipStream_t s, const Bucket &b, int which, int fld) {
 
 // unity = 1 on K = 0..nk+1, J = 0..nj, I = 0..ni+1; OLD = THETA and the
 // theta correction on K = k_sigma-1..nk+1 over the same rows
-__global__ void k_adv_elem(const MemberDesc *D, int mb, int op) {
+__global__ void __launch_bounds__(256) k_adv_elem(const MemberDesc *D, int mb, int op) {
   MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
   const int nI = d.ni + 2;

@@ -31,7 +31,7 @@ static inline dim3 grid2(int nJ, int nI, int nK, int nm) {
 }
 
 // ------------------------------------------------------------ utilities
-__global__ void k_fill(const MemberDesc *D, int mb, int f, double v) {
+__global__ void __launch_bounds__(256) k_fill(const MemberDesc *D, int mb, int f, double v) {
   MEMBER;
   double *a = fld_ptr(d, f);
   size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
@@ -43,7 +43,7 @@ void L_fill(hipStream_t s, const Bucket &b, int f, double v) {
 }
 
 // DZDT3 is zeroed whole while grid.itime <= 1 (src/shared/epic_funcs_diag.c:3674-3690)
-__global__ void k_zero_dzdt(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_zero_dzdt(const MemberDesc *D, int mb) {
   MEMBER;
   if (d.dyn->itime > 1) return;
   size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
@@ -56,7 +56,7 @@ void L_zero_dzdt(hipStream_t s, const Bucket &b) {
 
 // zonal periodicity of a whole 3-D array (bc_lateral THREEDIM,
 // src/single/s_epic_functions.c:72-88): K = 0..nk+1, J = 0..nj+1
-__global__ void k_bc3(const MemberDesc *D, int mb, int f) {
+__global__ void __launch_bounds__(256) k_bc3(const MemberDesc *D, int mb, int f) {
   MEMBER;
   double *a = fld_ptr(d, f);
   int t = blockIdx.x * TPB + threadIdx.x;
@@ -72,7 +72,7 @@ void L_bc3(hipStream_t s, const Bucket &b, int f) {
 }
 
 // copy planes k0..k1 (whole planes) src -> dst
-__global__ void k_copy_planes(const MemberDesc *D, int mb, int src, int dst, int k0) {
+__global__ void __launch_bounds__(256) k_copy_planes(const MemberDesc *D, int mb, int src, int dst, int k0) {
   MEMBER;
   const double *a = fld_ptr(d, src);
   double *o = fld_ptr(d, dst);
@@ -118,7 +118,7 @@ static inline dim3 tile_grid(const Bucket &b) {
 // P3 from the top down (prefix of dsgth*g*h), the sigma-region pressures,
 // h = max(h_min, dp/(g dsgth)), and the layer diagnostics P2, H3, THETA2
 // (and the theta of the sigma region when update_theta)
-__global__ void k_set_p2etc(const MemberDesc *D, int mb, int update_theta) {
+__global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, int update_theta) {
   MEMBER;
   extern __shared__ double lds[];
   const Tile T = tile_of(d);
@@ -211,7 +211,7 @@ void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
 // ------------------------------------------------- store_pgrad_vars parts
 // T3, EXNER3, RHO3, T2, RHO2 per cell and the top values; DZDT3 <- GZ3 when
 // itime > 1.  One lane per (K,J,I), K = 0..nk.
-__global__ void k_thermo(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_thermo(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 0, d.ni + 2);
   const int K = blockIdx.y;
@@ -261,7 +261,7 @@ __device__ double kin_uniform(const double *mn, const double *mni, int nj, const
 // wind, in three launches: the zonal means of u, p, rho on the bottom
 // interface (one wavefront per row; lanes 0-2 sum in the reference's
 // sequential order), the meridional integral (one block), the fills.
-__global__ void k_gzu_means(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_gzu_means(const MemberDesc *D, int mb) {
   MEMBER;
   if (!d.is_gas) return;
   extern __shared__ double row[];
@@ -273,14 +273,9 @@ __global__ void k_gzu_means(const MemberDesc *D, int mb) {
     row[2 * ni + i] = d.RHO3[q];
   }
   __syncthreads();
-  if (threadIdx.x < 3) {
-    const double *v = row + threadIdx.x * ni;
-    double sum = 0.;
-    for (int i = 0; i < ni; i++) sum += v[i];
-    d.rowbuf[threadIdx.x * R + J] = sum / ni;
-  }
+  if (threadIdx.x < 3) d.rowbuf[threadIdx.x * R + J] = seq_acc(row + threadIdx.x * ni, ni) / ni;
 }
-__global__ void k_gzu_core(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_gzu_core(const MemberDesc *D, int mb) {
   MEMBER;
   if (!d.is_gas) return;
   extern __shared__ double sh[];
@@ -351,7 +346,7 @@ __global__ void k_gzu_core(const MemberDesc *D, int mb) {
   }
 }
 // PV3 plane nk+1 holds the absolute vorticity of the uniform rows; GZ_SURFACE
-__global__ void k_gzu_fill(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_gzu_fill(const MemberDesc *D, int mb) {
   MEMBER;
   if (!d.is_gas) return;
   FLAT2(0, d.nj + 2, 0, d.ni + 2);
@@ -368,7 +363,7 @@ void L_gz_from_u(hipStream_t s, const Bucket &b) {
 
 // hydrostatic GZ3 and MONT3 from the bottom up (Leslie-Purser 4-point
 // integral in ln p) and the monotonicity screen
-__global__ void k_gz3(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb) {
   MEMBER;
   extern __shared__ double lds[];
   const Tile T = tile_of(d);
@@ -423,7 +418,7 @@ void L_gz3(hipStream_t s, const Bucket &b) {
 }
 
 // finish DZDT3 (diagnostic vertical velocity)
-__global__ void k_dzdt3(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_dzdt3(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -476,7 +471,7 @@ __device__ double hs_newton(const MemberDesc &d, int K, int J, int I) {
 
 // HEAT3 = 0 everywhere, then the Held-Suarez relaxation on K = 1..nk with
 // its zonal pads (the bc_lateral at the end of calc_heating)
-__global__ void k_heating(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_heating(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -487,7 +482,7 @@ __global__ void k_heating(const MemberDesc *D, int mb) {
   if (I == 1) d.HEAT3[ix3(d, K, J, d.ni + 1)] = h;
 }
 // layer-mean removal (K <= k_sponge or newt_cool_adjust), sequential sums
-__global__ void k_heating_adjust(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_heating_adjust(const MemberDesc *D, int mb) {
   MEMBER;
   const int K = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (K > d.nk) return;
@@ -521,7 +516,7 @@ void L_heating(hipStream_t s, const Bucket &b, int adjust) {
 // SUM_DIV in one lane per column.  SUM_DIV(1) and SUM_DIV(nk) are zero (the
 // reference's top/bottom statements write stale-index pad cells), so the
 // dF/dpbot term vanishes and is not formed.
-__global__ void k_calc_w(const MemberDesc *D, int mb, int with_source) {
+__global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int with_source) {
   MEMBER;
   extern __shared__ double lds[];
   const Tile T = tile_of(d);
@@ -593,7 +588,7 @@ void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
 
 // ------------------------------------------------ vorticity / divergence
 // PV3 planes k0..k1 from U (uu) and a filtered copy of V (vv), with H3
-__global__ void k_vort_interior(const MemberDesc *D, int mb, int uf, int vf, int k0) {
+__global__ void __launch_bounds__(256) k_vort_interior(const MemberDesc *D, int mb, int uf, int vf, int k0) {
   MEMBER;
   FLAT2(1, d.nj, 1, d.ni);
   const int K = k0 + blockIdx.y;
@@ -616,7 +611,7 @@ __global__ void k_vort_interior(const MemberDesc *D, int mb, int uf, int vf, int
 // pole PV from the circulation of the adjacent u-row: one wavefront per
 // (pole, K); the row is staged in LDS and lanes 0/1 form the two zonal sums
 // in the reference's sequential order
-__global__ void k_vort_poles(const MemberDesc *D, int mb, int uf, int k0) {
+__global__ void __launch_bounds__(256) k_vort_poles(const MemberDesc *D, int mb, int uf, int k0) {
   MEMBER;
   extern __shared__ double row[];
   const int K = k0 + (blockIdx.x >> 1), north = blockIdx.x & 1;
@@ -631,18 +626,17 @@ __global__ void k_vort_poles(const MemberDesc *D, int mb, int uf, int k0) {
   __syncthreads();
   double *res = row + 2 * ni;
   if (threadIdx.x == 0) {
-    double ze = 0.;
+    double ze;
     if (!north) {
-      for (int i = 0; i < ni; i++) ze -= row[i];
+      ze = seq_acc<true>(row, ni);
       ze *= d.mn[0] / ((d.m[1]) * (double)(ni));
     } else {
-      for (int i = 0; i < ni; i++) ze += row[i];
+      ze = seq_acc(row, ni);
       ze *= d.mn[2 * (nj + 1)] / (d.m[2 * nj + 1] * (double)(ni));
     }
     res[0] = ze;
   } else if (threadIdx.x == 1) {
-    double h = 0.;
-    for (int i = 0; i < ni; i++) h += row[ni + i];
+    double h = seq_acc(row + ni, ni);
     h /= (double)(ni);
     res[1] = h;
   }
@@ -657,7 +651,7 @@ void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1)
   LAUNCH(k_vort_poles, dim3(2 * nK, 1, b.nm), dim3(64), sizeof(double) * (2 * b.ni + 2), s, b.D, b.mb, uu, k0);
 }
 
-__global__ void k_divergence(const MemberDesc *D, int mb, int uf, int vf, int df, int k0) {
+__global__ void __launch_bounds__(256) k_divergence(const MemberDesc *D, int mb, int uf, int vf, int df, int k0) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = k0 + blockIdx.y;

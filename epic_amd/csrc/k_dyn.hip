@@ -35,7 +35,7 @@ __device__ __forceinline__ double *tend(const MemberDesc &d, double *base, int p
 // ------------------------------------------------------------- uv_core
 // UH (sA), VH (sB), KIN (sC) on planes K = 1..nk-1 with zonal pads; VH rows
 // 0 and nj+1 are zero as in the reference's cleared buffers
-__global__ void k_uvh_kin(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_uvh_kin(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 2, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -78,7 +78,7 @@ __global__ void k_uvh_kin(const MemberDesc *D, int mb) {
 }
 
 // DVDT = (DVDT + PV*UH terms + dKIN/dy) - vertical advection
-__global__ void k_dvdt_core(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_dvdt_core(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(1, d.nj, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -105,7 +105,7 @@ __global__ void k_dvdt_core(const MemberDesc *D, int mb) {
 }
 
 // DUDT = (DUDT + PV*VH terms + dKIN/dx with filtered KIN) - vertical advection
-__global__ void k_dudt_core(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_dudt_core(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -150,7 +150,7 @@ __device__ __forceinline__ double t_rad_hs(const MemberDesc &d, int K, int J, in
 }
 
 // top sponge, K = 0..k_sponge (K = 0 runs even when k_sponge = 0)
-__global__ void k_sponge(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_sponge(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = blockIdx.y;
@@ -164,7 +164,7 @@ __global__ void k_sponge(const MemberDesc *D, int mb) {
 
 // Rayleigh drag (tau_drag, or prandtl with Held-Suarez t_rad): one lane per
 // (K,J) row, sequential zonal means as in the reference
-__global__ void k_rayleigh(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_rayleigh(const MemberDesc *D, int mb) {
   MEMBER;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   int nrow = (d.nk - 1 - d.k_sponge) * (d.nj + 1);
@@ -198,7 +198,7 @@ __global__ void k_rayleigh(const MemberDesc *D, int mb) {
 }
 
 // Held-Suarez boundary-layer drag, one lane per u- and v-column
-__global__ void k_hs_pbl(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_hs_pbl(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   if (!d.is_hs) return;
@@ -235,7 +235,7 @@ void L_uv_drag(hipStream_t s, const Bucket &b) {
 // ------------------------------------------------------------- pgrad
 // GZ3 rows -1 and nj+1 take the zonal mean of the adjacent row, K = 0..nk:
 // one wavefront per (K, pole), the row staged in LDS and summed in order
-__global__ void k_gz_polefill(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_gz_polefill(const MemberDesc *D, int mb) {
   MEMBER;
   extern __shared__ double row[];
   const int K = blockIdx.x >> 1, north = blockIdx.x & 1, ni = d.ni;
@@ -243,8 +243,7 @@ __global__ void k_gz_polefill(const MemberDesc *D, int mb) {
   for (int i = threadIdx.x; i < ni; i += blockDim.x) row[i] = d.GZ3[ix3(d, K, Js, i + 1)];
   __syncthreads();
   if (threadIdx.x == 0) {
-    double tmp = 0.;
-    for (int i = 0; i < ni; i++) tmp += row[i];
+    double tmp = seq_acc(row, ni);
     tmp /= ni;
     row[ni] = tmp;
   }
@@ -294,7 +293,7 @@ __device__ __forceinline__ double face_lat(const MemberDesc &d, int K, int J, in
   return rlnn * (-t1 * (pbn - ptn) + t2 * (ztn * pbn - zbn * ptn)) / (ztn - zbn);
 }
 
-__global__ void k_pgrad(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_pgrad(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -406,7 +405,7 @@ void L_pgrad(hipStream_t s, const Bucket &b) {
 }
 
 // ------------------------------------------------------ divergence damping
-__global__ void k_divdamp(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_divdamp(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -431,7 +430,7 @@ void L_divdamp(hipStream_t s, const Bucket &b) {
 // ------------------------------------------------------ Adams-Bashforth
 // Coefficients from the FLAT_MAX start-up flags of the older planes,
 // tested per wind at (KLO, jlo, ILO) as the reference does.
-__global__ void k_ab(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_ab(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -459,7 +458,7 @@ __global__ void k_ab(const MemberDesc *D, int mb) {
   }
 }
 // rotate IT_ZERO <- IT_MINUS2 <- IT_MINUS1 <- IT_ZERO and advance itime
-__global__ void k_rotate(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_rotate(const MemberDesc *D, int mb) {
   MEMBER;
   if (threadIdx.x) return;
   MemberDyn *y = d.dyn;
@@ -478,7 +477,7 @@ void L_rotate(hipStream_t s, const Bucket &b) {
 }
 
 // zero the current tendency planes
-__global__ void k_zero_tend(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_zero_tend(const MemberDesc *D, int mb) {
   MEMBER;
   double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
   size_t i = (size_t)blockIdx.x * TPB + threadIdx.x, st = (size_t)gridDim.x * TPB;
@@ -493,7 +492,7 @@ void L_zero_tend(hipStream_t s, const Bucket &b) {
 }
 
 // monotone filtered geopotential, K = nk-1 down to 0 (src/shared/epic_timestep.c:176-196)
-__global__ void k_gz_mono(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_gz_mono(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 0, d.ni + 2);
   double *gz = d.gz;
@@ -511,7 +510,7 @@ void L_gz_mono(hipStream_t s, const Bucket &b) {
   LAUNCH(k_gz_mono, grid2(b.nj + 1, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
-__global__ void k_restore_mass(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_restore_mass(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 0, d.ni + 2);
   const int K = 1 + blockIdx.y;
@@ -533,8 +532,21 @@ __device__ __forceinline__ HvRange hv_range(const MemberDesc &d, int kind) {
   return r;
 }
 
-// d2dy2 with coefficient dc from src -> dst on planes of the kind's range
-__global__ void k_d2dy2(const MemberDesc *D, int mb, int src, int dst, int kind, int pass) {
+// One d2dy2 application (coefficient dc) at row J, column I of a plane
+__device__ __forceinline__ double d2dy2_at(const double *hh, const MemberDesc &d, int J, int I, int jmax, double dc,
+                                           const double *cx, const double *cy, const double *cz) {
+  int j0, j1, j2;
+  if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
+  else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
+  else { j0 = J - 1; j1 = J; j2 = J + 1; }
+  double a0 = dc * cx[J], a1 = dc * cy[J], a2 = dc * cz[J];
+  return a0 * hh[ix2(d, j0, I)] + a1 * hh[ix2(d, j1, I)] + a2 * hh[ix2(d, j2, I)];
+}
+
+// The two d2dy2 passes of del^4 (coef2, then coef4) in one launch: the
+// first-pass values a row needs (rows j0..j2) are recomputed by the lane
+// with the same expression instead of being staged in a scratch plane.
+__global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, int src, int dst, int kind) {
   MEMBER;
   const int vg = (kind == 1);
   const int jmax = vg ? d.nj + 1 : d.nj;
@@ -544,18 +556,21 @@ __global__ void k_d2dy2(const MemberDesc *D, int mb, int src, int dst, int kind,
   if (K > r.k1) return;
   const double *hh = fld_ptr(d, src) + (size_t)K * d.N2;
   double *out = fld_ptr(d, dst) + (size_t)K * d.N2;
-  const double dc = pass ? d.coef4 : d.coef2;
   const double *cx = vg ? d.vcx : d.hcx, *cy = vg ? d.vcy : d.hcy, *cz = vg ? d.vcz : d.hcz;
   int j0, j1, j2;
   if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
   else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
   else { j0 = J - 1; j1 = J; j2 = J + 1; }
-  double a0 = dc * cx[J], a1 = dc * cy[J], a2 = dc * cz[J];
-  out[ix2(d, J, I)] = a0 * hh[ix2(d, j0, I)] + a1 * hh[ix2(d, j1, I)] + a2 * hh[ix2(d, j2, I)];
+  const double c2 = d.coef2, c4 = d.coef4;
+  const double p0 = d2dy2_at(hh, d, j0, I, jmax, c2, cx, cy, cz);
+  const double p1 = d2dy2_at(hh, d, j1, I, jmax, c2, cx, cy, cz);
+  const double p2 = d2dy2_at(hh, d, j2, I, jmax, c2, cx, cy, cz);
+  double a0 = c4 * cx[J], a1 = c4 * cy[J], a2 = c4 * cz[J];
+  out[ix2(d, J, I)] = a0 * p0 + a1 * p1 + a2 * p2;
 }
 
 // A += DT*(-1)*D2 on the kind's rows, then zonal pads of every row 0..nj+1
-__global__ void k_hv_apply(const MemberDesc *D, int mb, int f, int src, int kind) {
+__global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, int f, int src, int kind) {
   MEMBER;
   FLAT2(0, d.nj + 2, 1, d.ni);
   HvRange r = hv_range(d, kind);
@@ -575,17 +590,17 @@ __global__ void k_hv_apply(const MemberDesc *D, int mb, int f, int src, int kind
 void L_merid_hv(hipStream_t s, const Bucket &b, int f, int kind) {
   const int jrows = (kind == 1) ? b.nj + 2 : b.nj + 1;
   dim3 g = grid2(jrows, b.ni, b.nk, b.nm);
-  LAUNCH(k_d2dy2, g, dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SA, kind, 0);
-  LAUNCH(k_d2dy2, g, dim3(TPB), 0, s, b.D, b.mb, (int)F_SA, (int)F_SB, kind, 1);
+  LAUNCH(k_d2dy2x2, g, dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SB, kind);
   // filter D2 on the kind's planes (planes outside the range are filtered
   // too, harmlessly: they are scratch)
   L_filter(s, b, F_SB, 1, b.nk, kind == 1, 0);
   LAUNCH(k_hv_apply, grid2(b.nj + 2, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SB, kind);
 }
 
-// zonal hyperviscosity: one 64-lane wavefront per (K,J) row held in LDS;
-// the zonal means are sequential sums in lane 0, as the reference sums them
-__global__ void k_zonal_hv(const MemberDesc *D, int mb, int f, int kind) {
+// zonal hyperviscosity: one 64-lane wavefront per (K,J) row held in LDS, the
+// waves of a block independent (wavefront-level barriers only); the zonal
+// means are sequential sums in lane 0, as the reference sums them
+__global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, int f, int kind) {
   MEMBER;
   extern __shared__ double lds[];
   const int ni = d.ni, W = ni + 2;
@@ -594,55 +609,48 @@ __global__ void k_zonal_hv(const MemberDesc *D, int mb, int f, int kind) {
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y;
   const int J = blockIdx.x * rpb + wv;
-  if (K > r.k1 || d.nu4 == 0.) return;  // uniform per block
-  const bool live = (J <= d.nj + 1);
-  double *hh = lds + (size_t)wv * 2 * W, *dd = hh + W;
-  __shared__ double meanv[16];
+  if (K > r.k1 || d.nu4 == 0. || J > d.nj + 1) return;  // whole wavefronts leave together
+  double *hh = lds + (size_t)wv * (2 * W + 1), *dd = hh + W, *meanv = dd + W;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
-  const bool upd = live && J >= r.jbot && J <= d.nj;
-  if (live)
-    for (int I = lane; I < W; I += 64) hh[I] = a[ix2(d, J, I)];
-  __syncthreads();
+  const bool upd = J >= r.jbot && J <= d.nj;
+  for (int I = lane; I < W; I += 64) hh[I] = a[ix2(d, J, I)];
+  wave_sync();
   double dc[2] = {0., 0.};
   if (upd) {
     double m = d.m[2 * J + r.jh];
     bool taper = fabs(d.lat[2 * J + r.jh]) > 45.;
     dc[0] = taper ? d.coef2 * d.m0z * d.m0z : d.coef2 * m * m;
     dc[1] = taper ? d.coef4 * d.m0z * d.m0z : d.coef4 * m * m;
-  }
-  for (int pass = 0; pass < 2; pass++) {
-    double *src = pass ? dd : hh, *dst = pass ? hh : dd;
-    if (upd)
+    for (int pass = 0; pass < 2; pass++) {
+      double *src = pass ? dd : hh, *dst = pass ? hh : dd;
       for (int I = 1 + lane; I <= ni; I += 64) dst[I] = dc[pass] * (src[I - 1] - 2. * src[I] + src[I + 1]);
-    __syncthreads();
-    if (upd && lane == 0) {
-      dst[ni + 1] = dst[1];
-      dst[0] = dst[ni];
-      double fm = 0.;
-      for (int I = 1; I <= ni; I++) fm += dst[I];
-      fm /= ni;
-      meanv[wv] = fm;
+      wave_sync();
+      if (lane == 0) {
+        dst[ni + 1] = dst[1];
+        dst[0] = dst[ni];
+        double fm = seq_acc(dst + 1, ni);
+        fm /= ni;
+        meanv[0] = fm;
+      }
+      wave_sync();
+      const double fm = meanv[0];
+      for (int I = lane; I < W; I += 64) dst[I] -= fm;
+      wave_sync();
     }
-    __syncthreads();
-    if (upd)
-      for (int I = lane; I < W; I += 64) dst[I] -= meanv[wv];
-    __syncthreads();
   }
   // pass 2 result is in hh; apply with sign -1, then pads
-  if (live) {
-    for (int I = 1 + lane; I <= ni; I += 64) {
-      double v = a[ix2(d, J, I)];
-      if (upd) v += d.dt * (double)(-1) * hh[I];
-      a[ix2(d, J, I)] = v;
-      if (I == ni) a[ix2(d, J, 0)] = v;
-      if (I == 1) a[ix2(d, J, ni + 1)] = v;
-    }
+  for (int I = 1 + lane; I <= ni; I += 64) {
+    double v = a[ix2(d, J, I)];
+    if (upd) v += d.dt * (double)(-1) * hh[I];
+    a[ix2(d, J, I)] = v;
+    if (I == ni) a[ix2(d, J, 0)] = v;
+    if (I == 1) a[ix2(d, J, ni + 1)] = v;
   }
 }
 void L_zonal_hv(hipStream_t s, const Bucket &b, int f, int kind) {
-  const int rpb = b.ni >= 512 ? 2 : 4, W = b.ni + 2;
+  const int rpb = 4, W = b.ni + 2;
   dim3 g((b.nj + 2 + rpb - 1) / rpb, b.nk, b.nm);
-  LAUNCH(k_zonal_hv, g, dim3(64 * rpb), sizeof(double) * 2 * W * rpb, s, b.D, b.mb, f, kind);
+  LAUNCH(k_zonal_hv, g, dim3(64 * rpb), sizeof(double) * (2 * W + 1) * rpb, s, b.D, b.mb, f, kind);
 }
 
 }  // namespace epic

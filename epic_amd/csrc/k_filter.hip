@@ -22,6 +22,14 @@ namespace epic {
 
 __device__ __forceinline__ int brev(int k, int bits) { return bits ? (int)(__brev((unsigned)k) >> (32 - bits)) : 0; }
 
+// Barrier between the transform stages of one row.  A row's ni/4 lanes lie
+// in one wavefront when ni <= 256, and rows never exchange data, so a
+// wavefront-level barrier suffices there; larger rows span waves.
+__device__ __forceinline__ void row_sync(int nq) {
+  if (nq <= 64) wave_sync();
+  else __syncthreads();
+}
+
 // The lane's twiddles for every stage are loaded into registers before the
 // transforms start (one round of loads instead of one per stage).
 constexpr int MAX_STAGES = 9;  // n <= 1024: stages with 2 << s < n
@@ -79,7 +87,7 @@ __device__ __forceinline__ void four1_stages(T *buf, int n, int lane, const Lane
       buf[i0] += tr;
       buf[i0 + 1] += ti;
     }
-    __syncthreads();
+    row_sync(n >> 2);
   }
 }
 
@@ -101,11 +109,11 @@ __device__ __forceinline__ void split_step(T *buf, int n, int lane, const LaneTw
     buf[i3] = h1r - wr * h2r + wi * h2i;
     buf[i4] = -h1i + wr * h2i + wi * h2r;
   }
-  __syncthreads();
+  row_sync(n >> 2);
 }
 
 template <class T>
-__global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgrid, int pads) {
+__global__ void __launch_bounds__(256) k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgrid, int pads) {
   MEMBER;
   extern __shared__ double lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
@@ -131,7 +139,7 @@ __global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgr
       int k = e >> 1, c = e & 1;
       buf[(brev(k, bits) << 1) | c] = a[ix2(d, J, e + 1)];
     }
-  __syncthreads();
+  row_sync(nq);
   four1_stages(buf, n, lane, tf, active);
   split_step(buf, n, lane, tf, T(-0.5), active);
   if (active && lane == 0) {
@@ -139,27 +147,27 @@ __global__ void k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgr
     buf[0] = (h1r = buf[0]) + buf[1];
     buf[1] = h1r - buf[1];
   }
-  __syncthreads();
+  row_sync(nq);
   if (active)
     for (int q = 0, e = lane; q < 4; q++, e += nq) buf[e] *= gain[q];
-  __syncthreads();
+  row_sync(nq);
   split_step(buf, n, lane, ti, T(0.5), active);
   if (active && lane == 0) {
     T h1r;
     buf[0] = T(0.5) * ((h1r = buf[0]) + buf[1]);
     buf[1] = T(0.5) * (h1r - buf[1]);
   }
-  __syncthreads();
+  row_sync(nq);
   T v[4];
   if (active)
     for (int q = 0, e = lane; q < 4; q++, e += nq) v[q] = buf[e];
-  __syncthreads();
+  row_sync(nq);
   if (active)
     for (int q = 0, e = lane; q < 4; q++, e += nq) {
       int k = e >> 1, c = e & 1;
       buf[(brev(k, bits) << 1) | c] = v[q];
     }
-  __syncthreads();
+  row_sync(nq);
   four1_stages(buf, n, lane, ti, active);
   if (active)
     for (int e = lane; e < n; e += nq) {
@@ -206,49 +214,6 @@ __device__ __forceinline__ double sgn_prod(double a, double b) {
   return b < 0. ? -1. : (b == 0. ? 0. : 1.);
 }
 
-// PCHIP derivatives (Fritsch-Carlson with Brodlie weights and
-// shape-preserving three-point ends), returns 0 or -1
-__device__ int pchip(int n, const double *x, const double *y, double *dd) {
-  for (int i = 1; i < n; i++)
-    if (x[i] <= x[i - 1]) return -1;
-  double h1 = x[1] - x[0], del1 = (y[1] - y[0]) / h1;
-  if (n == 2) { dd[0] = dd[1] = del1; return 0; }
-  double h2 = x[2] - x[1], del2 = (y[2] - y[1]) / h2;
-  double hsum = h1 + h2, w1 = (h1 + hsum) / hsum, w2 = -h1 / hsum;
-  dd[0] = w1 * del1 + w2 * del2;
-  if (sgn_prod(dd[0], del1) <= 0.0) dd[0] = 0.0;
-  else if (sgn_prod(del1, del2) < 0.0) {
-    double dm = 3.0 * del1;
-    if (fabs(dm) < fabs(dd[0])) dd[0] = dm;
-  }
-  for (int i = 2; i <= n - 1; i++) {
-    if (2 < i) {
-      h1 = h2;
-      h2 = x[i] - x[i - 1];
-      hsum = h1 + h2;
-      del1 = del2;
-      del2 = (y[i] - y[i - 1]) / h2;
-    }
-    dd[i - 1] = 0.0;
-    if (sgn_prod(del1, del2) > 0.0) {
-      double hsumt3 = 3.0 * hsum;
-      double a1 = (hsum + h1) / hsumt3, a2 = (hsum + h2) / hsumt3;
-      double dmx = dmax(fabs(del1), fabs(del2)), dmn = dmin(fabs(del1), fabs(del2));
-      double r1 = del1 / dmx, r2 = del2 / dmx;
-      dd[i - 1] = dmn / (a1 * r1 + a2 * r2);
-    }
-  }
-  w1 = -h2 / hsum;
-  w2 = (h2 + hsum) / hsum;
-  dd[n - 1] = w1 * del1 + w2 * del2;
-  if (sgn_prod(dd[n - 1], del2) <= 0.0) dd[n - 1] = 0.0;
-  else if (sgn_prod(del1, del2) < 0.0) {
-    double dm = 3.0 * del2;
-    if (fabs(dm) < fabs(dd[n - 1])) dd[n - 1] = dm;
-  }
-  return 0;
-}
-
 struct F3 { int kstart, kend, jstart; };
 __device__ __forceinline__ F3 f3_range(const MemberDesc &d, int kind) {
   F3 r;
@@ -258,45 +223,92 @@ __device__ __forceinline__ F3 f3_range(const MemberDesc &d, int kind) {
   return r;
 }
 
+// PCHIP derivatives with one lane per interior point: the serial sweep of
+// the reference carries (h1, h2, del1, del2) from point to point, but each
+// of them is a function of three neighbouring table entries, so every lane
+// forms its own with the same expressions.  Lanes 0 and 1 form the two
+// shape-preserving end derivatives.  Returns nonzero (in every lane) when
+// x is not strictly increasing.
+__device__ int pchip_wave(int n, const double *x, const double *y, double *dd, int lane) {
+  int bad = 0;
+  for (int i = 1 + lane; i < n; i += 64)
+    if (x[i] <= x[i - 1]) bad = 1;
+  if (__any(bad)) return -1;
+  if (n == 2) {
+    if (lane == 0) {
+      double del1 = (y[1] - y[0]) / (x[1] - x[0]);
+      dd[0] = dd[1] = del1;
+    }
+    return 0;
+  }
+  for (int i = 2 + lane; i <= n - 1; i += 64) {
+    double h1 = x[i - 1] - x[i - 2], h2 = x[i] - x[i - 1], hsum = h1 + h2;
+    double del1 = (y[i - 1] - y[i - 2]) / h1, del2 = (y[i] - y[i - 1]) / h2;
+    double v = 0.0;
+    if (sgn_prod(del1, del2) > 0.0) {
+      double hsumt3 = 3.0 * hsum;
+      double a1 = (hsum + h1) / hsumt3, a2 = (hsum + h2) / hsumt3;
+      double dmx = dmax(fabs(del1), fabs(del2)), dmn = dmin(fabs(del1), fabs(del2));
+      double r1 = del1 / dmx, r2 = del2 / dmx;
+      v = dmn / (a1 * r1 + a2 * r2);
+    }
+    dd[i - 1] = v;
+  }
+  if (lane == 0) {
+    double h1 = x[1] - x[0], del1 = (y[1] - y[0]) / h1;
+    double h2 = x[2] - x[1], del2 = (y[2] - y[1]) / h2;
+    double hsum = h1 + h2, w1 = (h1 + hsum) / hsum, w2 = -h1 / hsum;
+    double v = w1 * del1 + w2 * del2;
+    if (sgn_prod(v, del1) <= 0.0) v = 0.0;
+    else if (sgn_prod(del1, del2) < 0.0) {
+      double dm = 3.0 * del1;
+      if (fabs(dm) < fabs(v)) v = dm;
+    }
+    dd[0] = v;
+  } else if (lane == 1) {
+    const int i = n - 1;
+    double h1 = x[i - 1] - x[i - 2], h2 = x[i] - x[i - 1], hsum = h1 + h2;
+    double del1 = (y[i - 1] - y[i - 2]) / h1, del2 = (y[i] - y[i - 1]) / h2;
+    double w1 = -h2 / hsum, w2 = (h2 + hsum) / hsum;
+    double v = w1 * del1 + w2 * del2;
+    if (sgn_prod(v, del2) <= 0.0) v = 0.0;
+    else if (sgn_prod(del1, del2) < 0.0) {
+      double dm = 3.0 * del2;
+      if (fabs(dm) < fabs(v)) v = dm;
+    }
+    dd[n - 1] = v;
+  }
+  return 0;
+}
+
 // one wavefront per row: the column of lowest bottom GZ (the first of
 // equals, as the reference's strict-less scan finds it: each lane keeps the
-// first minimum of its strided subset, ties between lanes go to the lower
-// I), the profile table at that column and its PCHIP derivatives
+// first minimum of its strided subset, the lanes reduce to the lowest value
+// and, among equals, the lowest I), the profile table at that column and its
+// PCHIP derivatives
 template <class T>
-__global__ void k_spline_prep(const MemberDesc *D, int mb, int f, int kind) {
+__global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb, int f, int kind) {
   MEMBER;
   F3 r = f3_range(d, kind);
   const int J = r.jstart + blockIdx.x;
   if (J > d.nj) return;  // uniform per block
   const int nk = d.nk, KT = nk + 2, n = r.kend - r.kstart + 1, lane = threadIdx.x;
-  __shared__ double sx[KMAX + 2], sy[KMAX + 2], sz[KMAX + 2], mv[64];
-  __shared__ int mi[64];
-  __shared__ int imin_s;
+  __shared__ double sx[KMAX + 2], sy[KMAX + 2], sz[KMAX + 2];
   double best = 1.7976931348623157e308;
-  int bi = -1;
+  int bi = 0x7fffffff;
   for (int I = 1 + lane; I <= d.ni; I += 64) {
     double gv = d.gz[ix3(d, nk, J, I)];
     if (gv < best) { best = gv; bi = I; }
   }
-  mv[lane] = best;
-  mi[lane] = bi;
-  __syncthreads();
-  if (lane == 0) {
-    int Imin = 1;
-    double gzmin = 1.7976931348623157e308;
-    int found = -1;
-    for (int l = 0; l < 64; l++) {
-      if (mi[l] < 0) continue;
-      if (found < 0 || mv[l] < gzmin || (mv[l] == gzmin && mi[l] < found)) {
-        gzmin = mv[l];
-        found = mi[l];
-      }
+  for (int off = 32; off > 0; off >>= 1) {
+    double ov = __shfl_xor(best, off, 64);
+    int oi = __shfl_xor(bi, off, 64);
+    if (oi != 0x7fffffff && (bi == 0x7fffffff || ov < best || (ov == best && oi < bi))) {
+      best = ov;
+      bi = oi;
     }
-    if (found >= 0) Imin = found;
-    imin_s = Imin;
   }
-  __syncthreads();
-  const int Imin = imin_s;
+  const int Imin = (bi == 0x7fffffff) ? 1 : bi;
   const T *b = fld_ptr_t<T>(d, f);
   for (int e = lane; e < n; e += 64) {
     const int K = r.kend - e;
@@ -304,9 +316,9 @@ __global__ void k_spline_prep(const MemberDesc *D, int mb, int f, int kind) {
     double v = (double)b[ix3(d, K, J, Imin)];
     sy[e] = (kind == 2) ? log(v) : v;
   }
-  __syncthreads();
-  if (lane == 0 && pchip(n, sx, sy, sz)) atomicOr(&d.dyn->err, 4);
-  __syncthreads();
+  wave_sync();
+  if (pchip_wave(n, sx, sy, sz, lane) && lane == 0) atomicOr(&d.dyn->err, 4);
+  wave_sync();
   double *tx = d.spl + (size_t)J * 3 * KT, *ty = tx + KT, *tz = ty + KT;
   for (int e = lane; e < n; e += 64) {
     tx[e] = sx[e];
@@ -322,7 +334,7 @@ __global__ void k_spline_prep(const MemberDesc *D, int mb, int f, int kind) {
 // band come out as (a - v) + v, as in the reference, which subtracts and adds
 // the profile on every row.  RB rows per block, ni/4 lanes per row.
 template <class T>
-__global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
+__global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, int f, int kind) {
   MEMBER;
   extern __shared__ double lds_raw[];
   double *tabs = lds_raw;
@@ -340,7 +352,7 @@ __global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
   T *buf = reinterpret_cast<T *>(tabs + (size_t)RB * 3 * KT) + (size_t)rs * n;
   if (live)
     for (int e = lane; e < 3 * KT; e += nq) tab[e] = d.spl[(size_t)J * 3 * KT + e];
-  __syncthreads();
+  row_sync(nq);
   T *a = fld_ptr_t<T>(d, f) + (size_t)K * d.N2;
   const double *tx = tab, *ty = tab + KT, *tz = tab + 2 * KT;
   double v[4];
@@ -372,7 +384,7 @@ __global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
       int kq = e >> 1, c = e & 1;
       buf[(brev(kq, bits) << 1) | c] = x[q];
     }
-  __syncthreads();
+  row_sync(nq);
   four1_stages(buf, n, lane, tf, filt);
   split_step(buf, n, lane, tf, T(-0.5), filt);
   if (filt && lane == 0) {
@@ -380,27 +392,27 @@ __global__ void k_filter3(const MemberDesc *D, int mb, int f, int kind) {
     buf[0] = (h1r = buf[0]) + buf[1];
     buf[1] = h1r - buf[1];
   }
-  __syncthreads();
+  row_sync(nq);
   if (filt)
     for (int q = 0, e = lane; q < 4; q++, e += nq) buf[e] *= gain[q];
-  __syncthreads();
+  row_sync(nq);
   split_step(buf, n, lane, ti, T(0.5), filt);
   if (filt && lane == 0) {
     T h1r;
     buf[0] = T(0.5) * ((h1r = buf[0]) + buf[1]);
     buf[1] = T(0.5) * (h1r - buf[1]);
   }
-  __syncthreads();
+  row_sync(nq);
   T w[4];
   if (filt)
     for (int q = 0, e = lane; q < 4; q++, e += nq) w[q] = buf[e];
-  __syncthreads();
+  row_sync(nq);
   if (filt)
     for (int q = 0, e = lane; q < 4; q++, e += nq) {
       int kq = e >> 1, c = e & 1;
       buf[(brev(kq, bits) << 1) | c] = w[q];
     }
-  __syncthreads();
+  row_sync(nq);
   four1_stages(buf, n, lane, ti, filt);
   if (filt)
     for (int q = 0, e = lane; q < 4; q++, e += nq) x[q] = buf[e] * T(2. / (n));

@@ -43,7 +43,7 @@ __device__ __forceinline__ float *sBf(const MemberDesc &d) { return reinterpret_
 __device__ __forceinline__ float *sCf(const MemberDesc &d) { return reinterpret_cast<float *>(d.sC); }
 
 // ------------------------------------------------------------ tendencies
-__global__ void km_zero_tend(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_zero_tend(const MemberDesc *D, int mb) {
   MEMBER;
   double *du = curU(d), *dv = curV(d);
   size_t i = (size_t)blockIdx.x * TPB + threadIdx.x, st = (size_t)gridDim.x * TPB;
@@ -54,7 +54,7 @@ __global__ void km_zero_tend(const MemberDesc *D, int mb) {
 }
 
 // UH, VH, KIN (FP32 views of sA, sB, sC) on planes 1..nk-1 with zonal pads
-__global__ void km_uvh_kin(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_uvh_kin(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 2, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -97,7 +97,7 @@ __global__ void km_uvh_kin(const MemberDesc *D, int mb) {
   if (I == 1) vh[ix2(d, J, ni + 1)] = y;
 }
 
-__global__ void km_dvdt_core(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_dvdt_core(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(1, d.nj, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -122,7 +122,7 @@ __global__ void km_dvdt_core(const MemberDesc *D, int mb) {
   dv[ix2(d, J, I)] += (double)x;
 }
 
-__global__ void km_dudt_core(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_dudt_core(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -148,7 +148,7 @@ __global__ void km_dudt_core(const MemberDesc *D, int mb) {
 }
 
 // ------------------------------------------------------------- uv_drag
-__global__ void km_sponge(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_sponge(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = blockIdx.y;
@@ -173,7 +173,7 @@ __device__ __forceinline__ float t_rad_hs_f(const MemberDesc &d, int K, int J, i
 
 // Rayleigh drag: one wavefront per (K,J) row, the zonal mean by a wavefront
 // reduction
-__global__ void km_rayleigh(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_rayleigh(const MemberDesc *D, int mb) {
   MEMBER;
   if (!((d.newt_cool_on && d.prandtl > 0.) || (d.tau_drag > 0. && d.tau_drag < 1.e+20))) return;
   const int K = d.k_sponge + 1 + blockIdx.y, J = blockIdx.x;
@@ -204,7 +204,7 @@ __global__ void km_rayleigh(const MemberDesc *D, int mb) {
   }
 }
 
-__global__ void km_hs_pbl(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_hs_pbl(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   if (!d.is_hs) return;
@@ -269,7 +269,7 @@ __device__ __forceinline__ float face_lat_f(const MemberDesc &d, const ZS &Z, in
   return rlnn * (-t1 * (pbn - ptn) + t2 * (ztn * pbn - zbn * ptn)) / (ztn - zbn);
 }
 
-__global__ void km_pgrad(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_pgrad(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -372,7 +372,7 @@ __global__ void km_pgrad(const MemberDesc *D, int mb) {
 }
 
 // ------------------------------------------------------ divergence damping
-__global__ void km_divdamp(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_divdamp(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
@@ -393,7 +393,7 @@ __global__ void km_divdamp(const MemberDesc *D, int mb) {
 // tendency is then stored, rounded to FP32, in the history slot IT_ZERO,
 // which the rotation at the end of the step makes IT_MINUS1.  Lanes cover
 // every cell of planes 1..nk-1 so the stored history is the whole plane.
-__global__ void km_ab(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) km_ab(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;

@@ -290,6 +290,11 @@ int  epic_engine_profile(epic_engine *e, int nsteps, char (*names)[48], double *
                          int *launches, int cap);
 const char *epic_engine_last_error(epic_engine *e);
 
+/* Measurement helper with no counterpart in the reference: time a plain
+ * device-to-device streaming copy of nbytes (reads nbytes, writes nbytes),
+ * the measured copy peak beside the 8 TB/s specification. */
+int  epic_copy_probe(size_t nbytes, int reps, double *seconds_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
