@@ -105,6 +105,22 @@ __device__ __forceinline__ double sgth_of(const MemberDesc &d, double theta, dou
   return 0.;
 }
 
+// The same three functions with the column's log(ptop/pbot) formed once by
+// the caller: identical operations on identical operands, so identical
+// results, with one logarithm fewer per call.
+__device__ __forceinline__ double get_sigma_l(double pbot, double p, double lr) { return log(p / pbot) / lr; }
+__device__ __forceinline__ double get_p_sigma_l(double pbot, double s, double lr) { return pbot * exp(s * lr); }
+__device__ __forceinline__ double sgth_of_l(const MemberDesc &d, double theta, double p, double pbot, double ptop,
+                                            double lr, int *bad) {
+  if (p <= ptop) return theta;
+  if (p <= pbot) {
+    double s = get_sigma_l(pbot, p, lr);
+    return f_sig(d, s) + g_sig(d, s) * theta;
+  }
+  *bad = 1;
+  return 0.;
+}
+
 // bisection search of an ascending or descending table (find_place_in_table)
 __device__ __forceinline__ int table_place(int n, const double *x, double xx, double *dx) {
   int il = -1, iu = n;

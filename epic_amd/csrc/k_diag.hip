@@ -135,10 +135,10 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
     for (int K = 1; K <= nk; K++) a[K] = a[K - 1] + a[K];
   __syncthreads();
   if (T.live) {
-    const double ptop = a[0], pbot = a[nk];
+    const double ptop = a[0], pbot = a[nk], lr = log(ptop / pbot);
     for (int K = ks - 1 + T.kk; K < nk; K += T.KS) {
       double sigma = (d.sgth[2 * K + 1] - d.zeta0) / (d.zeta1 - d.zeta0);
-      a[K] = get_p_sigma(pbot, sigma, ptop);
+      a[K] = get_p_sigma_l(pbot, sigma, lr);
     }
   }
   __syncthreads();
@@ -148,14 +148,14 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
     for (int K = T.kk; K <= nk; K += T.KS) pw[K] = pow(a[K], d.kappa + 1.);
   __syncthreads();
   if (T.live) {
-    const double ptop = a[0], pbot = a[nk], kp1 = d.kappa + 1.;
+    const double ptop = a[0], pbot = a[nk], kp1 = d.kappa + 1., lr = log(ptop / pbot);
     const int nkay = nk - (ks - 1) + 1;
     const double *tx = d.sgth + 2 * nk + 1;  // tx[kay*(-2)] = sgth[2*(nk-kay)+1]
     for (int K = T.kk; K <= nk; K += T.KS) {
       const size_t q = ix3(d, K, J, I);
       P3[q] = a[K];
       if (update_theta && K < ks - 1) {
-        double sigma = get_sigma(pbot, a[K], ptop);
+        double sigma = get_sigma_l(pbot, a[K], lr);
         double th = ((double)d.sgth[2 * K + 1] - f_sig(d, sigma)) / g_sig(d, sigma);
         d.THETA[q] = th;
         if (K == ks - 2) aux[0] = th;
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
       if (K == 1) aux[1] = h3;
       if (update_theta) {
         if (K < ks - 1) {
-          double sigma = get_sigma(pbot, pp, ptop);
+          double sigma = get_sigma_l(pbot, pp, lr);
           d.THETA2[q] = ((double)d.sgth[2 * K] - f_sig(d, sigma)) / g_sig(d, sigma);
         } else if (K >= ks) {
           // linear interpolation of theta against sigmatheta, table ordered
@@ -368,19 +368,24 @@ __global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb) {
   extern __shared__ double lds[];
   const Tile T = tile_of(d);
   const int nk = d.nk, W = nk + 1, J = T.J, I = T.I;
-  double *inc = lds + T.c * W, *tt = lds + COLS * W + T.c * W;
+  double *inc = lds + T.c * W, *tt = lds + COLS * W + T.c * W, *lg = lds + 2 * COLS * W + T.c * W;
   const double *P = d.P3, *RH = d.RHO3;
-  if (T.live)
-    for (int K = T.kk; K <= nk; K += T.KS) tt[K] = d.T3[ix3(d, K, J, I)];
 #define PP(k) P[ix3(d, k, J, I)]
 #define RR(k) RH[ix3(d, k, J, I)]
+  // lg[k] = log(P3(k)/P3(k-1)): every ln-p interval the integral uses, once
+  if (T.live)
+    for (int K = T.kk; K <= nk; K += T.KS) {
+      tt[K] = d.T3[ix3(d, K, J, I)];
+      if (K >= 1) lg[K] = log(PP(K) / PP(K - 1));
+    }
+  __syncthreads();
   if (T.live)
     for (int K = 1 + T.kk; K <= nk; K += T.KS) {
       double v;
       if (K == nk || K == 1) {
-        v = ((PP(K) + PP(K - 1)) / (RR(K) + RR(K - 1))) * log(PP(K) / PP(K - 1));
+        v = ((PP(K) + PP(K - 1)) / (RR(K) + RR(K - 1))) * lg[K];
       } else {
-        double dz10 = log(PP(K + 1) / PP(K)), dz21 = log(PP(K) / PP(K - 1)), dz32 = log(PP(K - 1) / PP(K - 2));
+        double dz10 = lg[K + 1], dz21 = lg[K], dz32 = lg[K - 1];
         double dz20 = dz21 + dz10, dz30 = dz32 + dz20, dz31 = dz32 + dz21;
         double df0 = PP(K + 1) / RR(K + 1), df1 = PP(K) / RR(K), df2 = PP(K - 1) / RR(K - 1),
                df3 = PP(K - 2) / RR(K - 2);
@@ -414,7 +419,7 @@ __global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb) {
   }
 }
 void L_gz3(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_gz3, tile_grid(b), dim3(TPB), sizeof(double) * 2 * COLS * (b.nk + 1), s, b.D, b.mb);
+  LAUNCH(k_gz3, tile_grid(b), dim3(TPB), sizeof(double) * 3 * COLS * (b.nk + 1), s, b.D, b.mb);
 }
 
 // finish DZDT3 (diagnostic vertical velocity)
@@ -522,14 +527,19 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
   const Tile T = tile_of(d);
   const int nk = d.nk, ks = d.ks, W = nk + 1, J = T.J, I = T.I;
   double *dsv = lds + T.c * W, *dfv = lds + COLS * W + T.c * W, *wbv = lds + 2 * COLS * W + T.c * W;
+  double *lg = lds + 3 * COLS * W + T.c * W;
   const double *P = d.P3;
   double *TH = d.THETA;
   int bad = 0;
-  if (T.live) {
-    const double ptop = P[ix3(d, 0, J, I)], pbot = P[ix3(d, nk, J, I)], psig = P[ix3(d, ks - 1, J, I)];
-    const double eps = d.eps13;
 #define PP(k) P[ix3(d, k, J, I)]
 #define DV(k) d.DIV[ix3(d, k, J, I)]
+  // lg[k] = log(P3(k)/P3(k-1)), the ln-p intervals of the Leslie-Purser weights
+  if (T.live)
+    for (int K = 1 + T.kk; K <= nk; K += T.KS) lg[K] = log(PP(K) / PP(K - 1));
+  __syncthreads();
+  if (T.live) {
+    const double ptop = P[ix3(d, 0, J, I)], pbot = P[ix3(d, nk, J, I)], psig = P[ix3(d, ks - 1, J, I)];
+    const double eps = d.eps13, lr = log(ptop / pbot);
     for (int K = 1 + T.kk; K <= nk; K += T.KS) {
       const size_t q = ix3(d, K, J, I);
       double th = TH[q];
@@ -540,12 +550,12 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
       if (K == nk) continue;
       double wb = 0.;
       if (K < ks - 1) {
-        double sigma = get_sigma(pbot, P[q], ptop);
+        double sigma = get_sigma_l(pbot, P[q], lr);
         wb = g_sig(d, sigma) * d.HEAT3[q] / d.EXNER3[q];
       }
       double ds = 0.;
       if (K >= 2) {
-        double dz10 = log(PP(K - 1) / PP(K - 2)), dz21 = log(PP(K) / PP(K - 1)), dz32 = log(PP(K + 1) / PP(K));
+        double dz10 = lg[K - 1], dz21 = lg[K], dz32 = lg[K + 1];
         double dz20 = dz21 + dz10, dz30 = dz32 + dz20, dz31 = dz32 + dz21;
         double df0 = PP(K - 2) * DV(K - 2), df1 = PP(K - 1) * DV(K - 1), df2 = PP(K) * DV(K),
                df3 = PP(K + 1) * DV(K + 1);
@@ -560,15 +570,17 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
       double p2 = dmax(p3 - dp * .5, ptop), p4 = dmin(p3 + dp * .5, pbot);
       if (p2 < psig && p4 >= psig) p2 = p3;
       double dFdp = 0.;
-      if (p2 < p4) dFdp = (sgth_of(d, th, p2, pb3, ptop, &bad) - sgth_of(d, th, p4, pb3, ptop, &bad)) / (p2 - p4);
-      else bad = 1;
+      if (p2 < p4)
+        dFdp = (sgth_of_l(d, th, p2, pb3, ptop, lr, &bad) - sgth_of_l(d, th, p4, pb3, ptop, lr, &bad)) / (p2 - p4);
+      else
+        bad = 1;
       dsv[K] = ds;
       dfv[K] = dFdp;
       wbv[K] = wb;
     }
+  }
 #undef PP
 #undef DV
-  }
   __syncthreads();
   if (T.live && T.kk == 0) {
     double *Wf = d.W3;
@@ -583,7 +595,7 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
   if (bad) atomicOr(&d.dyn->err, 2);
 }
 void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
-  LAUNCH(k_calc_w, tile_grid(b), dim3(TPB), sizeof(double) * 3 * COLS * (b.nk + 1), s, b.D, b.mb, with_source);
+  LAUNCH(k_calc_w, tile_grid(b), dim3(TPB), sizeof(double) * 4 * COLS * (b.nk + 1), s, b.D, b.mb, with_source);
 }
 
 // ------------------------------------------------ vorticity / divergence

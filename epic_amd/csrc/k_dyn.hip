@@ -615,10 +615,10 @@ __global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, i
   const bool upd = J >= r.jbot && J <= d.nj;
   for (int I = lane; I < W; I += 64) hh[I] = a[ix2(d, J, I)];
   wave_sync();
-  double dc[2] = {0., 0.};
   if (upd) {
-    double m = d.m[2 * J + r.jh];
-    bool taper = fabs(d.lat[2 * J + r.jh]) > 45.;
+    const double m = d.m[2 * J + r.jh];
+    const bool taper = fabs(d.lat[2 * J + r.jh]) > 45.;
+    double dc[2];
     dc[0] = taper ? d.coef2 * d.m0z * d.m0z : d.coef2 * m * m;
     dc[1] = taper ? d.coef4 * d.m0z * d.m0z : d.coef4 * m * m;
     for (int pass = 0; pass < 2; pass++) {

@@ -26,11 +26,20 @@ import epic_amd as ea
 GRIDS = [(64, 32, 10), (128, 64, 16), (256, 128, 20), (512, 256, 32)]
 PLANETS = [ea.HELD_SUAREZ, ea.JUPITER_LIKE, ea.SATURN_LIKE, ea.VENUS_LIKE]
 _DT = {  # seconds, by planet then ni
-    ea.HELD_SUAREZ: {64: 600, 128: 300, 256: 150, 512: 75},
+    ea.HELD_SUAREZ: {64: 600, 128: 300},
     ea.VENUS_LIKE: {64: 300, 128: 150, 256: 75, 512: 40},
     ea.JUPITER_LIKE: {64: 60, 128: 60, 256: 60, 512: 30},
     ea.SATURN_LIKE: {64: 60, 128: 60, 256: 60, 512: 30},
 }
+
+
+def stable(planet, ni):
+    """Planet/grid pairs whose seeded state integrates 2000 steps without a
+    fatal condition.  The Held-Suarez state of 256x128x20 and 512x256x32 is
+    not among them: at nk >= 20 it blows up within 100-200 steps at any
+    timestep tried (the reference build does the same on 128x64x20), so the
+    mix draws another planet for those grids."""
+    return ni in _DT[planet]
 
 
 @dataclasses.dataclass
@@ -78,7 +87,8 @@ def poisson_mix(n, rate, seed, grids=GRIDS, planets=PLANETS, steps=(200, 2000)):
     for i in range(n):
         t += rng.expovariate(rate)
         ni, nj, nk = rng.choice(grids)
-        out.append(Request(i, rng.choice(planets), ni, nj, nk, rng.randint(*steps), rng.randrange(1 << 31), t))
+        planet = rng.choice([p for p in planets if stable(p, ni)])
+        out.append(Request(i, planet, ni, nj, nk, rng.randint(*steps), rng.randrange(1 << 31), t))
     return out
 
 
