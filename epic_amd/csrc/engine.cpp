@@ -187,14 +187,11 @@ static void rec_pgrad_vars(hipStream_t s, const Bucket &b) {
   L_dzdt3(s, b);
 }
 
-// store_diag (src/shared/epic_funcs_diag.c:3963-4053)
+// store_diag (src/shared/epic_funcs_diag.c:3963-4053); the filtered copies
+// of V and U are formed row by row inside the PV and divergence kernels
 static void rec_store_diag(hipStream_t s, const Bucket &b) {
-  L_copy_planes(s, b, F_V, F_SA, 0, b.nk);
-  L_filter(s, b, F_SA, 0, b.nk, 1, 0);
-  L_vorticity(s, b, F_U, F_SA, 0, b.nk);
-  L_copy_planes(s, b, F_U, F_SB, 0, b.nk);
-  L_filter(s, b, F_SB, 0, b.nk, 0, 0);
-  L_divergence(s, b, F_SB, F_V, F_DIV, 0, b.nk);
+  L_vorticity(s, b, F_U, F_V, 0, b.nk);
+  L_divergence(s, b, F_U, F_V, F_DIV, 0, b.nk);
 }
 
 static void rec_sync(hipStream_t s, const Bucket &b, bool adjust) {
@@ -224,8 +221,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
   L_advection(s, b);
   rec_pgrad_vars(s, b);
   // filtered, monotone copy of GZ3
-  L_copy_planes(s, b, F_GZ3, F_GZ, 0, b.nk + 1);
-  L_filter(s, b, F_GZ, 0, b.nk - 1, 0, 0);
+  L_gz_filtercopy(s, b);
   L_gz_mono(s, b);
   L_filter3(s, b, F_DUDT_T, 0);
   L_filter3(s, b, F_DVDT_T, 1);

@@ -15,6 +15,7 @@
 // w_a[k_sigma-3] as left by the last column of the preceding unity call.
 // Those two values are handed over explicitly (wstale).
 #include "kernels.h"
+#include "filter_row.h"
 
 namespace epic {
 
@@ -290,58 +291,68 @@ __global__ void __launch_bounds__(256) k_hsu_mer_fin(const MemberDesc *D, int mb
 #undef MER_SETUP
 
 // --------------------------------------------------------------- zonal
-// One block per (K,J) row of the filtered copy sB, staged in LDS with its
-// pads: velocities, first-guess fluxes, predictor, corrected fluxes and the
-// update sA + d(flux) each run on all lanes over the row, periodic in I.
+// Rows of the zonally filtered copy sB (256/ni rows per block, or one row
+// per block when ni > 256), staged in LDS with their pads: velocities,
+// first-guess fluxes, predictor, corrected fluxes and the update
+// sA + d(flux) each run on all lanes over the row, periodic in I.
 __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, int which) {
   MEMBER;
-  const int J = blockIdx.x, K = 1 + blockIdx.y;
-  if (which == AV_THETA && K <= d.ks - 2) return;
+  const RowMap m = row_map(d.ni);
+  const int K = 1 + blockIdx.y, Jr = blockIdx.x * m.rpb + m.g;
+  if (which == AV_THETA && K <= d.ks - 2) return;  // uniform per block
+  const bool live = Jr <= d.nj;
+  const int J = live ? Jr : 0;
   extern __shared__ double sh[];
   const int ni = d.ni, W = ni + 2;
-  double *A = sh, *UP = sh + W, *UM = sh + 2 * W, *FP = sh + 3 * W, *AP = sh + 4 * W, *FF = sh + 5 * W;
+  double *A = sh + (size_t)m.g * 6 * W;
+  double *UP = A + W, *UM = A + 2 * W, *FP = A + 3 * W, *AP = A + 4 * W, *FF = A + 5 * W;
   const double *af = d.sB + (size_t)K * d.N2;
   const double ninv = 1. / d.n[2 * J + 1], mn = d.mn[2 * J + 1], DT = d.dt;
   const double amin = (which == AV_HDRY) ? d.h_min[K] : 0.;
   auto wr = [ni](int I) { return I < 1 ? I + ni : (I > ni ? I - ni : I); };
-  for (int I = threadIdx.x; I < W; I += blockDim.x) {
-    A[I] = af[ix2(d, J, I)];
-    double u2;
-    if (which == AV_HDRY) {
-      int Ii = wr(I);
-      u2 = .5 * (d.U[ix3(d, K, J, Ii)] + d.U[ix3(d, K - 1, J, Ii)]);
-    } else {
-      u2 = d.U[ix3(d, K, J, I)];
+  if (live)
+    for (int I = m.lane; I < W; I += m.stride) {
+      A[I] = af[ix2(d, J, I)];
+      double u2;
+      if (which == AV_HDRY) {
+        int Ii = wr(I);
+        u2 = .5 * (d.U[ix3(d, K, J, Ii)] + d.U[ix3(d, K - 1, J, Ii)]);
+      } else {
+        u2 = d.U[ix3(d, K, J, I)];
+      }
+      UM[I] = dmin(0., u2 * ninv);
+      UP[I] = dmax(0., u2 * ninv);
     }
-    UM[I] = dmin(0., u2 * ninv);
-    UP[I] = dmax(0., u2 * ninv);
-  }
   __syncthreads();
-  for (int c = 1 + threadIdx.x; c <= ni; c += blockDim.x) FP[c] = UP[c] * A[c - 1] + UM[c] * A[c];
+  if (live)
+    for (int c = 1 + m.lane; c <= ni; c += m.stride) FP[c] = UP[c] * A[c - 1] + UM[c] * A[c];
   __syncthreads();
-  for (int c = 1 + threadIdx.x; c <= ni; c += blockDim.x) AP[c] = A[c] + (FP[c] - FP[wr(c + 1)]) * mn * DT;
+  if (live)
+    for (int c = 1 + m.lane; c <= ni; c += m.stride) AP[c] = A[c] + (FP[c] - FP[wr(c + 1)]) * mn * DT;
   __syncthreads();
-  for (int c = 1 + threadIdx.x; c <= ni; c += blockDim.x) {
-    const int cm = wr(c - 1);
-    double UPc = UP[c], UMc = UM[c], APc = AP[c], APm = AP[cm], Am = A[c - 1], Ac = A[c];
-    double fc = .5 * (UPc * (APc + Am) + UMc * (APm + Ac));
-    double d1c = APc - Am, d2c = APc - A[c + 1];
-    double d1m = APm - A[cm - 1], d2m = APm - A[cm + 1];
-    double uhp = sqrt(UPc * UP[c - 1]), uhm = -sqrt(UMc * UM[c + 1]);
-    double mu = (UPc - UMc) * mn * DT, al = (1. + mu) / 6.;
-    double t0 = fabs(A[cm - 1] - 2. * A[cm] + A[cm + 1]) + 1.e-10, q0 = t0 * t0;
-    double t1 = fabs(A[c - 1] - 2. * A[c] + A[c + 1]) + 1.e-10, q1 = t1 * t1;
-    double gp = q0 / (q0 + (Am - amin) * (Ac - amin));
-    gp = gp * gp;
-    double gm = q1 / (q1 + (Am - amin) * (Ac - amin));
-    gm = gm * gm;
-    double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
-    double bhp = 1. - gp, bhm = 1. - gm;
-    FF[c] = fc + -al * (UPc * bp * d1c - uhp * bhp * d1m + UMc * bm * d2m - uhm * bhm * d2c);
-  }
+  if (live)
+    for (int c = 1 + m.lane; c <= ni; c += m.stride) {
+      const int cm = wr(c - 1);
+      double UPc = UP[c], UMc = UM[c], APc = AP[c], APm = AP[cm], Am = A[c - 1], Ac = A[c];
+      double fc = .5 * (UPc * (APc + Am) + UMc * (APm + Ac));
+      double d1c = APc - Am, d2c = APc - A[c + 1];
+      double d1m = APm - A[cm - 1], d2m = APm - A[cm + 1];
+      double uhp = sqrt(UPc * UP[c - 1]), uhm = -sqrt(UMc * UM[c + 1]);
+      double mu = (UPc - UMc) * mn * DT, al = (1. + mu) / 6.;
+      double t0 = fabs(A[cm - 1] - 2. * A[cm] + A[cm + 1]) + 1.e-10, q0 = t0 * t0;
+      double t1 = fabs(A[c - 1] - 2. * A[c] + A[c + 1]) + 1.e-10, q1 = t1 * t1;
+      double gp = q0 / (q0 + (Am - amin) * (Ac - amin));
+      gp = gp * gp;
+      double gm = q1 / (q1 + (Am - amin) * (Ac - amin));
+      gm = gm * gm;
+      double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
+      double bhp = 1. - gp, bhm = 1. - gm;
+      FF[c] = fc + -al * (UPc * bp * d1c - uhp * bhp * d1m + UMc * bm * d2m - uhm * bhm * d2c);
+    }
   __syncthreads();
+  if (!live) return;
   double *out = adv_field(d, which);
-  for (int I = 1 + threadIdx.x; I <= ni; I += blockDim.x) {
+  for (int I = 1 + m.lane; I <= ni; I += m.stride) {
     const size_t q = ix3(d, K, J, I);
     const double y = d.sA[q] + (FF[I] - FF[wr(I + 1)]) * mn * DT;
     out[q] = y;
@@ -360,7 +371,9 @@ static void hsu(hipStream_t s, const Bucket &b, int which, int fld) {
   LAUNCH(k_hsu_mer_ff, dim3((b.nj * b.ni + TPB - 1) / TPB, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb, which);
   LAUNCH(k_hsu_mer_fin, g, dim3(TPB), 0, s, b.D, b.mb, which);
   L_filter(s, b, F_SB, 1, b.nk, 0, 0);
-  LAUNCH(k_hsu_zonal, dim3(b.nj + 1, b.nk, b.nm), dim3(256), sizeof(double) * 6 * (b.ni + 2), s, b.D, b.mb, which);
+  const int rpb = rows_per_block(b.ni);
+  LAUNCH(k_hsu_zonal, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, b.nm), dim3(256), sizeof(double) * rpb * 6 * (b.ni + 2),
+         s, b.D, b.mb, which);
   (void)fld;
 }
 

@@ -14,6 +14,7 @@
 // LDS and sum it in one lane in the reference's order.  The arithmetic of
 // each expression follows the reference's operation order.
 #include "kernels.h"
+#include "filter_row.h"
 
 namespace epic {
 
@@ -599,26 +600,41 @@ void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
 }
 
 // ------------------------------------------------ vorticity / divergence
-// PV3 planes k0..k1 from U (uu) and a filtered copy of V (vv), with H3
+// PV3 planes k0..k1 from U (uu) and the zonally filtered V (vv), with H3.
+// One block per (K,J) pv-row: row J of vv is filtered in LDS (rows of the
+// v-grid filter band), then one lane per I.
 __global__ void __launch_bounds__(256) k_vort_interior(const MemberDesc *D, int mb, int uf, int vf, int k0) {
   MEMBER;
-  FLAT2(1, d.nj, 1, d.ni);
-  const int K = k0 + blockIdx.y;
+  extern __shared__ double lds[];
+  const RowMap m = row_map(d.ni);
+  const int Jr = 1 + blockIdx.x * m.rpb + m.g, K = k0 + blockIdx.y, ni = d.ni;
+  const bool live = Jr <= d.nj;
+  const int J = live ? Jr : 1;
   const double *uu = fld_ptr(d, uf) + (size_t)K * d.N2, *vv = fld_ptr(d, vf) + (size_t)K * d.N2;
   const double *hh = d.H3 + (size_t)K * d.N2;
   double *pv = d.PV3 + (size_t)K * d.N2;
+  double *xv = lds + (size_t)m.g * 2 * ni, *work = xv + ni;  // xv[I-1] = filtered vv(J,I)
+  if (live)
+    for (int I = 1 + m.lane; I <= ni; I += m.stride) xv[I - 1] = vv[ix2(d, J, I)];
+  __syncthreads();
+  filter_row_lds(d, xv, work, J, 1, live && d.rflag_v[J] != 0, m.lane);
+  __syncthreads();
+  if (!live) return;
   double f2j = d.f[2 * J], m2j = d.m[2 * J], n2j = d.n[2 * J];
   double mm1i = 1. / d.m[2 * J - 1], mp1i = 1. / d.m[2 * J + 1];
   double mnm1i = 1. / d.mn[2 * J - 1], mnp1i = 1. / d.mn[2 * J + 1];
   double mn_pv = .5 / (mnm1i + mnp1i);
-  double ze = m2j * ((vv[ix2(d, J, I)] - vv[ix2(d, J, I - 1)]) +
-                     n2j * (uu[ix2(d, J - 1, I)] * mm1i - uu[ix2(d, J, I)] * mp1i));
-  double v = (ze + f2j);
-  double h_pv = ((hh[ix2(d, J, I)] + hh[ix2(d, J, I - 1)]) * mnp1i + (hh[ix2(d, J - 1, I)] + hh[ix2(d, J - 1, I - 1)]) * mnm1i) * mn_pv;
-  v /= h_pv;
-  pv[ix2(d, J, I)] = v;
-  if (I == d.ni) pv[ix2(d, J, 0)] = v;
-  if (I == 1) pv[ix2(d, J, d.ni + 1)] = v;
+  for (int I = 1 + m.lane; I <= ni; I += m.stride) {
+    const double vc = xv[I - 1], vw = xv[I == 1 ? ni - 1 : I - 2];
+    double ze = m2j * ((vc - vw) + n2j * (uu[ix2(d, J - 1, I)] * mm1i - uu[ix2(d, J, I)] * mp1i));
+    double v = (ze + f2j);
+    double h_pv = ((hh[ix2(d, J, I)] + hh[ix2(d, J, I - 1)]) * mnp1i +
+                   (hh[ix2(d, J - 1, I)] + hh[ix2(d, J - 1, I - 1)]) * mnm1i) * mn_pv;
+    v /= h_pv;
+    pv[ix2(d, J, I)] = v;
+    if (I == ni) pv[ix2(d, J, 0)] = v;
+    if (I == 1) pv[ix2(d, J, ni + 1)] = v;
+  }
 }
 // pole PV from the circulation of the adjacent u-row: one wavefront per
 // (pole, K); the row is staged in LDS and lanes 0/1 form the two zonal sums
@@ -659,29 +675,46 @@ __global__ void __launch_bounds__(256) k_vort_poles(const MemberDesc *D, int mb,
 }
 void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1) {
   int nK = k1 - k0 + 1;
-  LAUNCH(k_vort_interior, grid2(b.nj, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb, uu, vv, k0);
+  const int rpb = rows_per_block(b.ni);
+  LAUNCH(k_vort_interior, dim3((b.nj + rpb - 1) / rpb, nK, b.nm), dim3(256), sizeof(double) * rpb * 2 * b.ni, s, b.D,
+         b.mb, uu, vv, k0);
   LAUNCH(k_vort_poles, dim3(2 * nK, 1, b.nm), dim3(64), sizeof(double) * (2 * b.ni + 2), s, b.D, b.mb, uu, k0);
 }
 
+// DIV of (uu zonally filtered, vv) on planes k0..: one block per (K,J)
+// h-row, row J of uu filtered in LDS (rows of the h-grid filter band)
 __global__ void __launch_bounds__(256) k_divergence(const MemberDesc *D, int mb, int uf, int vf, int df, int k0) {
   MEMBER;
-  FLAT2(0, d.nj + 1, 1, d.ni);
-  const int K = k0 + blockIdx.y;
+  extern __shared__ double lds[];
+  const RowMap m = row_map(d.ni);
+  const int Jr = blockIdx.x * m.rpb + m.g, K = k0 + blockIdx.y, ni = d.ni;
+  const bool live = Jr <= d.nj;
+  const int J = live ? Jr : 0;
   const double *uu = fld_ptr(d, uf) + (size_t)K * d.N2, *vv = fld_ptr(d, vf) + (size_t)K * d.N2;
   double *di = fld_ptr(d, df) + (size_t)K * d.N2;
+  double *xu = lds + (size_t)m.g * 2 * ni, *work = xu + ni;  // xu[I-1] = filtered uu(J,I)
+  if (live)
+    for (int I = 1 + m.lane; I <= ni; I += m.stride) xu[I - 1] = uu[ix2(d, J, I)];
+  __syncthreads();
+  filter_row_lds(d, xu, work, J, 0, live && d.rflag_h[J] != 0, m.lane);
+  __syncthreads();
+  if (!live) return;
   int jj = 2 * J + 1;
   double n_inv = 1. / d.n[jj], mn = d.mn[jj];
   double m2j_inv = (J == 0) ? 0. : 1. / d.m[jj - 1];
   double m2jp2_inv = (J == d.nj) ? 0. : 1. / d.m[jj + 1];
-  double v = mn * ((uu[ix2(d, J, I + 1)] * n_inv - uu[ix2(d, J, I)] * n_inv) +
-                   (vv[ix2(d, J + 1, I)] * m2jp2_inv - vv[ix2(d, J, I)] * m2j_inv));
-  di[ix2(d, J, I)] = v;
-  if (I == d.ni) di[ix2(d, J, 0)] = v;
-  if (I == 1) di[ix2(d, J, d.ni + 1)] = v;
+  for (int I = 1 + m.lane; I <= ni; I += m.stride) {
+    const double ue = xu[I == ni ? 0 : I], uc = xu[I - 1];
+    double v = mn * ((ue * n_inv - uc * n_inv) + (vv[ix2(d, J + 1, I)] * m2jp2_inv - vv[ix2(d, J, I)] * m2j_inv));
+    di[ix2(d, J, I)] = v;
+    if (I == ni) di[ix2(d, J, 0)] = v;
+    if (I == 1) di[ix2(d, J, ni + 1)] = v;
+  }
 }
 void L_divergence(hipStream_t s, const Bucket &b, int uu, int vv, int dst, int k0, int k1) {
-  LAUNCH(k_divergence, grid2(b.nj + 1, b.ni, k1 - k0 + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb, uu, vv, dst,
-                     k0);
+  const int rpb = rows_per_block(b.ni);
+  LAUNCH(k_divergence, dim3((b.nj + 1 + rpb - 1) / rpb, k1 - k0 + 1, b.nm), dim3(256), sizeof(double) * rpb * 2 * b.ni,
+         s, b.D, b.mb, uu, vv, dst, k0);
 }
 
 }  // namespace epic

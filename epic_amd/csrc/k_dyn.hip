@@ -14,6 +14,7 @@
 // neighbour with the same expression, so the result is the same bits
 // without the carried dependence.
 #include "kernels.h"
+#include "filter_row.h"
 
 namespace epic {
 
@@ -104,31 +105,46 @@ __global__ void __launch_bounds__(256) k_dvdt_core(const MemberDesc *D, int mb) 
   dv[ix2(d, J, I)] = x;
 }
 
-// DUDT = (DUDT + PV*VH terms + dKIN/dx with filtered KIN) - vertical advection
+// DUDT = (DUDT + PV*VH terms + dKIN/dx with filtered KIN) - vertical
+// advection.  One block per (K,J) u-row: the KIN row is zonally filtered in
+// LDS (rows of the filter band) and its pad refreshed, then one lane per I.
 __global__ void __launch_bounds__(256) k_dudt_core(const MemberDesc *D, int mb) {
   MEMBER;
-  FLAT2(0, d.nj + 1, 1, d.ni);
-  const int K = 1 + blockIdx.y;
+  extern __shared__ double lds[];
+  const RowMap m = row_map(d.ni);
+  const int Jr = blockIdx.x * m.rpb + m.g, K = 1 + blockIdx.y, ni = d.ni;
+  const bool live = Jr <= d.nj;
+  const int J = live ? Jr : 0;
   const size_t o = (size_t)K * d.N2;
   const double *uh = d.sA + o, *vh = d.sB + o, *kin = d.sC + o, *pv = d.PV3 + o;
+  double *kr = lds + (size_t)m.g * 2 * ni, *work = kr + ni;  // kr[I-1] = KIN(J,I)
+  if (live)
+    for (int I = 1 + m.lane; I <= ni; I += m.stride) kr[I - 1] = kin[ix2(d, J, I)];
+  __syncthreads();
+  filter_row_lds(d, kr, work, J, 0, live && d.rflag_h[J] != 0, m.lane);
+  __syncthreads();
+  if (!live) return;
   double *du = tend(d, d.DUDT, d.dyn->it0) + o;
+  for (int I = 1 + m.lane; I <= ni; I += m.stride) {
 #define PV(j, i) pv[ix2(d, j, i)]
-  double al = (PV(J, I) + PV(J + 1, I) + PV(J + 1, I + 1));
-  double be = (PV(J + 1, I - 1) + PV(J + 1, I) + PV(J, I));
-  double ga = (PV(J, I - 1) + PV(J, I) + PV(J + 1, I));
-  double de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
+    double al = (PV(J, I) + PV(J + 1, I) + PV(J + 1, I + 1));
+    double be = (PV(J + 1, I - 1) + PV(J + 1, I) + PV(J, I));
+    double ga = (PV(J, I - 1) + PV(J, I) + PV(J + 1, I));
+    double de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
 #undef PV
-  double x = du[ix2(d, J, I)];
-  x += ((al * vh[ix2(d, J + 1, I)] + be * vh[ix2(d, J + 1, I - 1)] + ga * vh[ix2(d, J, I - 1)] + de * vh[ix2(d, J, I)] +
-         0. * uh[ix2(d, J, I - 1)] - 0. * uh[ix2(d, J, I + 1)]) *
-            (1. / 12.) +
-        kin[ix2(d, J, I - 1)] - kin[ix2(d, J, I)]) *
-       d.m[2 * J + 1];
-  int kk = 2 * K + 1;
-  double ds = d.sgth[kk - 2] - d.sgth[kk + 2];
-  x -= .5 * (d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K, J, I - 1)]) *
-       (d.U[ix3(d, K - 1, J, I)] - d.U[ix3(d, K + 1, J, I)]) / ds;
-  du[ix2(d, J, I)] = x;
+    const double kin_w = kr[I == 1 ? ni - 1 : I - 2], kin_c = kr[I - 1];
+    double x = du[ix2(d, J, I)];
+    x += ((al * vh[ix2(d, J + 1, I)] + be * vh[ix2(d, J + 1, I - 1)] + ga * vh[ix2(d, J, I - 1)] +
+           de * vh[ix2(d, J, I)] + 0. * uh[ix2(d, J, I - 1)] - 0. * uh[ix2(d, J, I + 1)]) *
+              (1. / 12.) +
+          kin_w - kin_c) *
+         d.m[2 * J + 1];
+    int kk = 2 * K + 1;
+    double ds = d.sgth[kk - 2] - d.sgth[kk + 2];
+    x -= .5 * (d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K, J, I - 1)]) *
+         (d.U[ix3(d, K - 1, J, I)] - d.U[ix3(d, K + 1, J, I)]) / ds;
+    du[ix2(d, J, I)] = x;
+  }
 }
 
 void L_uv_core(hipStream_t s, const Bucket &b) {
@@ -136,8 +152,9 @@ void L_uv_core(hipStream_t s, const Bucket &b) {
   const int nK = b.nk - 1;
   LAUNCH(k_uvh_kin, grid2(b.nj + 2, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
   LAUNCH(k_dvdt_core, grid2(b.nj, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
-  L_filter(s, b, F_SC, 1, b.nk - 1, 0, 0);
-  LAUNCH(k_dudt_core, grid2(b.nj + 1, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  const int rpb = rows_per_block(b.ni);
+  LAUNCH(k_dudt_core, dim3((b.nj + 1 + rpb - 1) / rpb, nK, b.nm), dim3(256), sizeof(double) * rpb * 2 * b.ni, s,
+         b.D, b.mb);
 }
 
 // ------------------------------------------------------------- uv_drag
@@ -420,9 +437,7 @@ __global__ void __launch_bounds__(256) k_divdamp(const MemberDesc *D, int mb) {
   }
 }
 void L_divdamp(hipStream_t s, const Bucket &b) {
-  L_copy_planes(s, b, F_U, F_SA, 1, b.nk - 1);
-  L_filter(s, b, F_SA, 1, b.nk - 1, 0, 0);
-  L_divergence(s, b, F_SA, F_V, F_SB, 1, b.nk - 1);
+  L_divergence(s, b, F_U, F_V, F_SB, 1, b.nk - 1);  // U filtered row by row inside
   if (b.mixed) { LM_divdamp_apply(s, b); return; }
   LAUNCH(k_divdamp, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
@@ -506,6 +521,38 @@ __global__ void __launch_bounds__(256) k_gz_mono(const MemberDesc *D, int mb) {
     below = v;
   }
 }
+// gz = GZ3 with planes 0..nk-1 zonally filtered (rows of the h-grid band),
+// one block per (K,J) row of the whole plane (J = -1..nj+1), pads included
+__global__ void __launch_bounds__(256) k_gz_filtercopy(const MemberDesc *D, int mb) {
+  MEMBER;
+  extern __shared__ double lds[];
+  const RowMap m = row_map(d.ni);
+  const int Jr = (int)(blockIdx.x * m.rpb + m.g) - 1, K = blockIdx.y, ni = d.ni, W = ni + 2;
+  const bool live = Jr <= d.nj + 1;
+  const int J = live ? Jr : 0;
+  double *x = lds + (size_t)m.g * (W + ni), *work = x + W;
+  const double *src = d.GZ3 + (size_t)K * d.N2;
+  double *dst = d.gz + (size_t)K * d.N2;
+  if (live)
+    for (int I = m.lane; I < W; I += m.stride) x[I] = src[ix2(d, J, I)];
+  __syncthreads();
+  const bool filt = live && K <= d.nk - 1 && J >= 0 && J <= d.nj && d.rflag_h[J] != 0;
+  filter_row_lds(d, x + 1, work, J < 0 ? 0 : J, 0, filt, m.lane);
+  __syncthreads();
+  if (!live) return;
+  for (int I = m.lane; I < W; I += m.stride) {
+    double v = x[I];
+    if (filt && I == 0) v = x[ni];
+    if (filt && I == ni + 1) v = x[1];
+    dst[ix2(d, J, I)] = v;
+  }
+}
+void L_gz_filtercopy(hipStream_t s, const Bucket &b) {
+  const int rpb = rows_per_block(b.ni);
+  LAUNCH(k_gz_filtercopy, dim3((b.nj + 3 + rpb - 1) / rpb, b.nk + 2, b.nm), dim3(256),
+         sizeof(double) * rpb * (2 * b.ni + 2), s, b.D, b.mb);
+}
+
 void L_gz_mono(hipStream_t s, const Bucket &b) {
   LAUNCH(k_gz_mono, grid2(b.nj + 1, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
@@ -593,7 +640,7 @@ void L_merid_hv(hipStream_t s, const Bucket &b, int f, int kind) {
   LAUNCH(k_d2dy2x2, g, dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SB, kind);
   // filter D2 on the kind's planes (planes outside the range are filtered
   // too, harmlessly: they are scratch)
-  L_filter(s, b, F_SB, 1, b.nk, kind == 1, 0);
+  L_filter(s, b, F_SB, 1, b.nk, kind == 1, 1);
   LAUNCH(k_hv_apply, grid2(b.nj + 2, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SB, kind);
 }
 

@@ -15,102 +15,11 @@
 // geopotential before filtering and adds it back afterwards, on every row
 // (src/shared/epic_adjust.c:258-397, 463-474), fused with the transform.
 #include "kernels.h"
+#include "filter_row.h"
 
 namespace epic {
 
 #define MEMBER const MemberDesc &d = D[mb + blockIdx.z]
-
-__device__ __forceinline__ int brev(int k, int bits) { return bits ? (int)(__brev((unsigned)k) >> (32 - bits)) : 0; }
-
-// Barrier between the transform stages of one row.  A row's ni/4 lanes lie
-// in one wavefront when ni <= 256, and rows never exchange data, so a
-// wavefront-level barrier suffices there; larger rows span waves.
-__device__ __forceinline__ void row_sync(int nq) {
-  if (nq <= 64) wave_sync();
-  else __syncthreads();
-}
-
-// The lane's twiddles for every stage are loaded into registers before the
-// transforms start (one round of loads instead of one per stage).
-constexpr int MAX_STAGES = 9;  // n <= 1024: stages with 2 << s < n
-
-template <class T>
-struct LaneTw {
-  T wr[MAX_STAGES], wi[MAX_STAGES];
-  T sr, si;  // split-step twiddle of this lane
-};
-
-template <class T>
-__device__ __forceinline__ void load_tw(LaneTw<T> &t, const T *tw, const T *tr, int n, int lane) {
-#pragma unroll
-  for (int s = 0; s < MAX_STAGES; s++) {
-    const int half = 1 << s, mm = lane & (half - 1);
-    const bool on = (2 << s) < n;
-    t.wr[s] = on ? tw[2 * (half - 1 + mm)] : T(0);
-    t.wi[s] = on ? tw[2 * (half - 1 + mm) + 1] : T(0);
-  }
-  const bool on = lane < n / 4 - 1;
-  t.sr = on ? tr[2 * lane] : T(0);
-  t.si = on ? tr[2 * lane + 1] : T(0);
-}
-
-// twiddle tables and field views of a storage type
-template <class T> __device__ __forceinline__ const T *tw_fwd(const MemberDesc &d);
-template <class T> __device__ __forceinline__ const T *tw_inv(const MemberDesc &d);
-template <class T> __device__ __forceinline__ const T *tr_fwd(const MemberDesc &d);
-template <class T> __device__ __forceinline__ const T *tr_inv(const MemberDesc &d);
-template <> __device__ __forceinline__ const double *tw_fwd<double>(const MemberDesc &d) { return d.tw_f; }
-template <> __device__ __forceinline__ const double *tw_inv<double>(const MemberDesc &d) { return d.tw_i; }
-template <> __device__ __forceinline__ const double *tr_fwd<double>(const MemberDesc &d) { return d.tr_f; }
-template <> __device__ __forceinline__ const double *tr_inv<double>(const MemberDesc &d) { return d.tr_i; }
-template <> __device__ __forceinline__ const float *tw_fwd<float>(const MemberDesc &d) { return d.tw_f32; }
-template <> __device__ __forceinline__ const float *tw_inv<float>(const MemberDesc &d) { return d.tw_i32; }
-template <> __device__ __forceinline__ const float *tr_fwd<float>(const MemberDesc &d) { return d.tr_f32; }
-template <> __device__ __forceinline__ const float *tr_inv<float>(const MemberDesc &d) { return d.tr_i32; }
-
-// complex radix-2 stages on buf[0..n) (n doubles = n/2 complex), one
-// butterfly per lane, in the reference's stage order
-template <class T>
-__device__ __forceinline__ void four1_stages(T *buf, int n, int lane, const LaneTw<T> &t, bool active) {
-#pragma unroll
-  for (int s = 0; s < MAX_STAGES; s++) {
-    if ((2 << s) >= n) break;
-    const int mmax = 2 << s, half = 1 << s;
-    if (active) {
-      int g = lane >> s, mm = lane & (half - 1);
-      int i0 = 2 * mm + g * 2 * mmax, j0 = i0 + mmax;
-      T wr = t.wr[s], wi = t.wi[s];
-      T tr = wr * buf[j0] - wi * buf[j0 + 1];
-      T ti = wr * buf[j0 + 1] + wi * buf[j0];
-      buf[j0] = buf[i0] - tr;
-      buf[j0 + 1] = buf[i0 + 1] - ti;
-      buf[i0] += tr;
-      buf[i0 + 1] += ti;
-    }
-    row_sync(n >> 2);
-  }
-}
-
-// split step of the real transform; c2 = -0.5 forward, +0.5 inverse
-template <class T>
-__device__ __forceinline__ void split_step(T *buf, int n, int lane, const LaneTw<T> &t, T c2, bool active) {
-  const T c1 = T(0.5);
-  if (active && lane < n / 4 - 1) {
-    int i = lane + 2;
-    int i1 = i + i - 1, i2 = 1 + i1, i3 = n + 3 - i2, i4 = 1 + i3;
-    i1--; i2--; i3--; i4--;
-    T wr = t.sr, wi = t.si;
-    T h1r = c1 * (buf[i1] + buf[i3]);
-    T h1i = c1 * (buf[i2] - buf[i4]);
-    T h2r = -c2 * (buf[i2] + buf[i4]);
-    T h2i = c2 * (buf[i1] - buf[i3]);
-    buf[i1] = h1r + wr * h2r - wi * h2i;
-    buf[i2] = h1i + wr * h2i + wi * h2r;
-    buf[i3] = h1r - wr * h2r + wi * h2i;
-    buf[i4] = -h1i + wr * h2i + wi * h2r;
-  }
-  row_sync(n >> 2);
-}
 
 template <class T>
 __global__ void __launch_bounds__(256) k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgrid, int pads) {

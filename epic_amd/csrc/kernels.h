@@ -61,6 +61,7 @@ void L_ab(hipStream_t s, const Bucket &b);
 void L_rotate(hipStream_t s, const Bucket &b);
 void L_zero_tend(hipStream_t s, const Bucket &b);
 void L_gz_mono(hipStream_t s, const Bucket &b);
+void L_gz_filtercopy(hipStream_t s, const Bucket &b);
 void L_restore_mass(hipStream_t s, const Bucket &b);
 // hyperviscosity: kind 0 U, 1 V, 2 HDRY, 3 THETA
 void L_merid_hv(hipStream_t s, const Bucket &b, int fld, int kind);
