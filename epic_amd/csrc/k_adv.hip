@@ -291,10 +291,10 @@ __global__ void __launch_bounds__(256) k_hsu_mer_fin(const MemberDesc *D, int mb
 #undef MER_SETUP
 
 // --------------------------------------------------------------- zonal
-// Rows of the zonally filtered copy sB (256/ni rows per block, or one row
-// per block when ni > 256), staged in LDS with their pads: velocities,
-// first-guess fluxes, predictor, corrected fluxes and the update
-// sA + d(flux) each run on all lanes over the row, periodic in I.
+// Rows of the zonally filtered copy sB (1024/ni rows per block), staged in
+// LDS with their pads: velocities, first-guess fluxes, predictor, corrected
+// fluxes and the update sA + d(flux) each run on all lanes of the row,
+// periodic in I.
 __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, int which) {
   MEMBER;
   const RowMap m = row_map(d.ni);
@@ -370,6 +370,9 @@ static void hsu(hipStream_t s, const Bucket &b, int which, int fld) {
   LAUNCH(k_hsu_mer_ap, g, dim3(TPB), 0, s, b.D, b.mb, which);
   LAUNCH(k_hsu_mer_ff, dim3((b.nj * b.ni + TPB - 1) / TPB, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb, which);
   LAUNCH(k_hsu_mer_fin, g, dim3(TPB), 0, s, b.D, b.mb, which);
+  // the zonal pass reads a zonally filtered copy (the reference filters it
+  // per level before the zonal sweep); a separate launch packs the rows of
+  // the filter band, which is faster here than filtering inside the pass
   L_filter(s, b, F_SB, 1, b.nk, 0, 0);
   const int rpb = rows_per_block(b.ni);
   LAUNCH(k_hsu_zonal, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, b.nm), dim3(256), sizeof(double) * rpb * 6 * (b.ni + 2),

@@ -616,32 +616,46 @@ __global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, in
   out[ix2(d, J, I)] = a0 * p0 + a1 * p1 + a2 * p2;
 }
 
-// A += DT*(-1)*D2 on the kind's rows, then zonal pads of every row 0..nj+1
+// Rows of the del^4 result D (sB, 1024/ni rows per block): D is zonally
+// filtered in LDS (rows of the filter band), then A += DT*(-1)*D on the
+// kind's rows and the zonal pads of every row 0..nj+1 are written
 __global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, int f, int src, int kind) {
   MEMBER;
-  FLAT2(0, d.nj + 2, 1, d.ni);
+  extern __shared__ double lds[];
+  const RowMap m = row_map(d.ni);
   HvRange r = hv_range(d, kind);
-  const int K = r.k0 + blockIdx.y;
-  if (K > r.k1 || d.nu4 == 0.) return;
+  const int K = r.k0 + blockIdx.y, Jr = blockIdx.x * m.rpb + m.g;
+  if (K > r.k1 || d.nu4 == 0.) return;  // uniform per block
+  const bool live = Jr <= d.nj + 1;
+  const int J = live ? Jr : 0, ni = d.ni, vg = (kind == 1);
+  double *x = lds + (size_t)m.g * 2 * ni, *work = x + ni;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
   const double *dd = fld_ptr(d, src) + (size_t)K * d.N2;
-  double v = a[ix2(d, J, I)];
-  if (J >= r.jbot && J <= d.nj) {
-    v += d.dt * (double)(-1) * dd[ix2(d, J, I)];
-    a[ix2(d, J, I)] = v;
+  const bool upd = live && J >= r.jbot && J <= d.nj;
+  if (upd)
+    for (int I = 1 + m.lane; I <= ni; I += m.stride) x[I - 1] = dd[ix2(d, J, I)];
+  __syncthreads();
+  filter_row_lds(d, x, work, J, vg, upd && (vg ? d.rflag_v[J] : d.rflag_h[J]) != 0, m.lane);
+  __syncthreads();
+  if (!live) return;
+  for (int I = 1 + m.lane; I <= ni; I += m.stride) {
+    double v = a[ix2(d, J, I)];
+    if (upd) {
+      v += d.dt * (double)(-1) * x[I - 1];
+      a[ix2(d, J, I)] = v;
+    }
+    if (I == d.ni) a[ix2(d, J, 0)] = v;
+    if (I == 1) a[ix2(d, J, d.ni + 1)] = v;
   }
-  if (I == d.ni) a[ix2(d, J, 0)] = v;
-  if (I == 1) a[ix2(d, J, d.ni + 1)] = v;
 }
 
 void L_merid_hv(hipStream_t s, const Bucket &b, int f, int kind) {
   const int jrows = (kind == 1) ? b.nj + 2 : b.nj + 1;
   dim3 g = grid2(jrows, b.ni, b.nk, b.nm);
   LAUNCH(k_d2dy2x2, g, dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SB, kind);
-  // filter D2 on the kind's planes (planes outside the range are filtered
-  // too, harmlessly: they are scratch)
-  L_filter(s, b, F_SB, 1, b.nk, kind == 1, 1);
-  LAUNCH(k_hv_apply, grid2(b.nj + 2, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SB, kind);
+  const int rpb = rows_per_block(b.ni);
+  LAUNCH(k_hv_apply, dim3((b.nj + 2 + rpb - 1) / rpb, b.nk, b.nm), dim3(256), sizeof(double) * rpb * 2 * b.ni, s,
+         b.D, b.mb, f, (int)F_SB, kind);
 }
 
 // zonal hyperviscosity: one 64-lane wavefront per (K,J) row held in LDS, the
