@@ -48,8 +48,9 @@ struct MemberDesc {
   // DUDTf, DVDTf the three FP32 Adams-Bashforth history planes
   float *DUDTf, *DVDTf;
   // scratch
+  // sA..sD: scratch, three 3-D copies each (the batched advection gives each
+  // advected field its own copy; every other user takes the first)
   double *gz, *varz, *sA, *sB, *sC, *sD, *unity, *old, *spl;
-  double *wstale;  // hsu column statics carried between calls (per member)
   double *rowbuf;  // 8*(nj+3) doubles: zonal means and meridional profiles
   MemberDyn *dyn;
 };

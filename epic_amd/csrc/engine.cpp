@@ -367,9 +367,10 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   add(&d.USPIN, sizeof(double) * N3, c->u_spinup);
   add(&d.GZS, sizeof(double) * N2, c->gz_surface);
   double **fields[] = {&d.HDRY, &d.P2, &d.THETA2, &d.H3, &d.T2, &d.T3, &d.RHO2, &d.RHO3, &d.EXNER3, &d.GZ3,
-                       &d.MONT3, &d.HEAT3, &d.PV3, &d.DIV, &d.W3, &d.DZDT3, &d.gz, &d.varz, &d.sA, &d.sB,
-                       &d.sC, &d.sD, &d.unity, &d.old};
+                       &d.MONT3, &d.HEAT3, &d.PV3, &d.DIV, &d.W3, &d.DZDT3, &d.gz, &d.varz, &d.unity, &d.old};
   for (auto p : fields) add(p, sizeof(double) * N3, nul);
+  // scratch: three copies each, one per field of the batched advection
+  for (auto p : {&d.sA, &d.sB, &d.sC, &d.sD}) add(p, sizeof(double) * 3 * N3, nul);
   if (mixed) {
     add(&d.DUDT, sizeof(double) * N3, nul);
     add(&d.DVDT, sizeof(double) * N3, nul);
@@ -380,7 +381,6 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
     add(&d.DVDT, sizeof(double) * 3 * N3, nul);
   }
   add(&d.spl, sizeof(double) * (size_t)(c->nj + 2) * 3 * (c->nk + 2), nul);
-  add(&d.wstale, sizeof(double) * 4, nul);
   add(&d.rowbuf, sizeof(double) * 8 * (c->nj + 3), nul);
   add(&d.dyn, sizeof(MemberDyn), nul);
   size_t total = 0;

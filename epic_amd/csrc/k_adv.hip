@@ -10,16 +10,28 @@
 // every intermediate value is formed by the same expression as in the
 // reference, so nothing changes but where it is computed.
 //
+// The three advected fields (unity on the pressure grid, theta, hdry) are
+// independent of one another until the theta correction that follows them,
+// so every launch carries all three: blockIdx.z = field * members + member,
+// each field with its own scratch copies (sA..sD + field * N3).
+//
 // The reference keeps its column work arrays in statics across calls: the
 // theta call starts at K = k_sigma-1 and reads wm[k_sigma-2] and
-// w_a[k_sigma-3] as left by the last column of the preceding unity call.
-// Those two values are handed over explicitly (wstale).
+// w_a[k_sigma-3] as left by the last column (J = nj, I = ni+1) of the
+// preceding unity call.  Both are functions of W3 and of unity before its
+// advection (1 everywhere), so the theta blocks form them themselves.
 #include "kernels.h"
 #include "filter_row.h"
 
 namespace epic {
 
 #define MEMBER const MemberDesc &d = D[mb + blockIdx.z]
+// field and member of a block; so: offset of the field's scratch copies
+#define ADV_MEMBER                                          \
+  const int which = (int)blockIdx.z / nm;                   \
+  const MemberDesc &d = D[mb + (int)blockIdx.z - which * nm]; \
+  const size_t so = (size_t)which * d.N3
+constexpr int ADV_FIELDS = 3;
 
 enum { AV_P3 = 0, AV_THETA = 1, AV_HDRY = 2 };
 
@@ -47,8 +59,8 @@ __device__ __forceinline__ double tt_iface(const double *wa, int K, int nk, doub
   return tt * tt;
 }
 
-__global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, int which, int C) {
-  MEMBER;
+__global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, int nm, int C) {
+  ADV_MEMBER;
   extern __shared__ double lds[];
   const int c = threadIdx.x % C, kk = threadIdx.x / C, KS = blockDim.x / C;
   const int nI = d.ni + 2, col = blockIdx.x * C + c;
@@ -72,21 +84,18 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
       } else {
         if (K >= kf - 1 && K <= nk) x = b[ix3(d, K, J, I)];
         if (K >= kf && K <= nk) w = (K == 1 || K == nk) ? 0. : .5 * (d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K - 1, J, I)]);
-        // statics left by the unity pass (wm[kf-1], wa[kf-2])
-        if (which == AV_THETA && K == kf - 1 && K >= 1) w = d.wstale[0];
-        if (which == AV_THETA && K == kf - 2) x = d.wstale[1];
+        // statics left by the unity pass (wm[kf-1], wa[kf-2] of its last column)
+        if (which == AV_THETA && K == kf - 1 && K >= 1) {
+          const int Jl = d.nj, Il = d.ni + 1;
+          w = dmin(0., (K == 1 || K == nk) ? 0. : .5 * (d.W3[ix3(d, K, Jl, Il)] + d.W3[ix3(d, K - 1, Jl, Il)]));
+        }
+        if (which == AV_THETA && K == kf - 2) x = 1.;
       }
       wa[K] = x;
       wv[K] = w;
     }
   }
   __syncthreads();
-  if (which == AV_P3 && live && kk == 0 && J == d.nj && I == d.ni + 1) {
-    // hand the statics the theta call will read to it
-    const int kft = d.ks - 1;
-    if (kft - 1 >= 1) d.wstale[0] = dmin(0., wv[kft - 1]);
-    if (kft - 2 >= 0) d.wstale[1] = wa[kft - 2];
-  }
 #define WM(k) dmin(0., wv[k])
 #define WP(k) dmax(0., wv[k])
   if (layer) {
@@ -223,9 +232,9 @@ struct MerCol {
   }
 };
 
-// Three launches per field, each one lane per cell: the predictor ap(J)
-// (into sC), the corrected flux ff(J) on v-rows 1..nj (into sD), and the
-// update into sA and its copy sB (which the zonal pass filters).
+// Three launches, each one lane per cell: the predictor ap(J) (into sC),
+// the corrected flux ff(J) on v-rows 1..nj (into sD), and the update into
+// sA and its copy sB (which the zonal pass filters).
 #define MER_SETUP                                                   \
   const int K = 1 + blockIdx.y;                                     \
   if (which == AV_THETA && K <= d.ks - 2) return;                   \
@@ -239,21 +248,21 @@ struct MerCol {
   c.DT = d.dt;                                                      \
   const size_t o = (size_t)K * d.N2;
 
-__global__ void __launch_bounds__(256) k_hsu_mer_ap(const MemberDesc *D, int mb, int which) {
-  MEMBER;
+__global__ void __launch_bounds__(256) k_hsu_mer_ap(const MemberDesc *D, int mb, int nm) {
+  ADV_MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= (d.nj + 1) * d.ni) return;
   const int J = t / d.ni, I = 1 + t % d.ni;
   MER_SETUP;
-  d.sC[o + ix2(d, J, I)] = c.ap(J);
+  d.sC[so + o + ix2(d, J, I)] = c.ap(J);
 }
-__global__ void __launch_bounds__(256) k_hsu_mer_ff(const MemberDesc *D, int mb, int which) {
-  MEMBER;
+__global__ void __launch_bounds__(256) k_hsu_mer_ff(const MemberDesc *D, int mb, int nm) {
+  ADV_MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= d.nj * d.ni) return;
   const int J = 1 + t / d.ni, I = 1 + t % d.ni;
   MER_SETUP;
-  const double *AP = d.sC + o;
+  const double *AP = d.sC + so + o;
   const int nj = d.nj;
   double VP = c.vp(J), VM = c.vm(J), APj = AP[ix2(d, J, I)], APm = AP[ix2(d, J - 1, I)], Am = c.a(J - 1), Aj = c.a(J);
   double fc = .5 * (VP * (APj + Am) + VM * (APm + Aj));
@@ -270,23 +279,23 @@ __global__ void __launch_bounds__(256) k_hsu_mer_ff(const MemberDesc *D, int mb,
   double d1m = (J - 1 == 0) ? 0. : APm - c.a(J - 2);
   double d2m = APm - c.a(J);
   double d2J = (J == nj) ? 0. : APj - c.a(J + 1);
-  d.sD[o + ix2(d, J, I)] = fc + -al * (VP * bp * d1J - vhp * bhp * d1m + VM * bm * d2m - vhm * bhm * d2J);
+  d.sD[so + o + ix2(d, J, I)] = fc + -al * (VP * bp * d1J - vhp * bhp * d1m + VM * bm * d2m - vhm * bhm * d2J);
 }
-__global__ void __launch_bounds__(256) k_hsu_mer_fin(const MemberDesc *D, int mb, int which) {
-  MEMBER;
+__global__ void __launch_bounds__(256) k_hsu_mer_fin(const MemberDesc *D, int mb, int nm) {
+  ADV_MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
   if (t >= (d.nj + 1) * d.ni) return;
   const int J = t / d.ni, I = 1 + t % d.ni;
   MER_SETUP;
-  const double *FF = d.sD + o;
+  const double *FF = d.sD + so + o;
   double f0 = (J >= 1) ? FF[ix2(d, J, I)] : 0., f1 = (J + 1 <= d.nj) ? FF[ix2(d, J + 1, I)] : 0.;
   double v = c.a(J) + (f0 - f1) * d.mn[2 * J + 1] * c.DT;
-  const size_t q = o + ix2(d, J, I);
+  const size_t q = so + o + ix2(d, J, I);
   d.sA[q] = v;
   d.sB[q] = v;
   // the copy the zonal pass filters carries its zonal pads
-  if (I == d.ni) d.sB[o + ix2(d, J, 0)] = v;
-  if (I == 1) d.sB[o + ix2(d, J, d.ni + 1)] = v;
+  if (I == d.ni) d.sB[so + o + ix2(d, J, 0)] = v;
+  if (I == 1) d.sB[so + o + ix2(d, J, d.ni + 1)] = v;
 }
 #undef MER_SETUP
 
@@ -295,8 +304,8 @@ __global__ void __launch_bounds__(256) k_hsu_mer_fin(const MemberDesc *D, int mb
 // LDS with their pads: velocities, first-guess fluxes, predictor, corrected
 // fluxes and the update sA + d(flux) each run on all lanes of the row,
 // periodic in I.
-__global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, int which) {
-  MEMBER;
+__global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, int nm) {
+  ADV_MEMBER;
   const RowMap m = row_map(d.ni);
   const int K = 1 + blockIdx.y, Jr = blockIdx.x * m.rpb + m.g;
   if (which == AV_THETA && K <= d.ks - 2) return;  // uniform per block
@@ -306,7 +315,7 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   const int ni = d.ni, W = ni + 2;
   double *A = sh + (size_t)m.g * 6 * W;
   double *UP = A + W, *UM = A + 2 * W, *FP = A + 3 * W, *AP = A + 4 * W, *FF = A + 5 * W;
-  const double *af = d.sB + (size_t)K * d.N2;
+  const double *af = d.sB + so + (size_t)K * d.N2;
   const double ninv = 1. / d.n[2 * J + 1], mn = d.mn[2 * J + 1], DT = d.dt;
   const double amin = (which == AV_HDRY) ? d.h_min[K] : 0.;
   auto wr = [ni](int I) { return I < 1 ? I + ni : (I > ni ? I - ni : I); };
@@ -354,34 +363,35 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   double *out = adv_field(d, which);
   for (int I = 1 + m.lane; I <= ni; I += m.stride) {
     const size_t q = ix3(d, K, J, I);
-    const double y = d.sA[q] + (FF[I] - FF[wr(I + 1)]) * mn * DT;
+    const double y = d.sA[so + q] + (FF[I] - FF[wr(I + 1)]) * mn * DT;
     out[q] = y;
     if (I == ni) out[ix3(d, K, J, 0)] = y;
     if (I == 1) out[ix3(d, K, J, ni + 1)] = y;
   }
 }
 
-static void hsu(hipStream_t s, const Bucket &b, int which, int fld) {
-  int ncol = (b.nj + 1) * (b.ni + 2);
+// one advection pass over all three fields of every member
+static void hsu3(hipStream_t s, const Bucket &b) {
+  const int ncol = (b.nj + 1) * (b.ni + 2), nz = ADV_FIELDS * b.nm;
   const int C = b.nk <= 30 ? 64 : 32;
-  LAUNCH(k_hsu_vert, dim3((ncol + C - 1) / C, 1, b.nm), dim3(256), sizeof(double) * 4 * C * (b.nk + 2), s, b.D, b.mb,
-         which, C);
-  dim3 g(((b.nj + 1) * b.ni + TPB - 1) / TPB, b.nk, b.nm);
-  LAUNCH(k_hsu_mer_ap, g, dim3(TPB), 0, s, b.D, b.mb, which);
-  LAUNCH(k_hsu_mer_ff, dim3((b.nj * b.ni + TPB - 1) / TPB, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb, which);
-  LAUNCH(k_hsu_mer_fin, g, dim3(TPB), 0, s, b.D, b.mb, which);
+  LAUNCH(k_hsu_vert, dim3((ncol + C - 1) / C, 1, nz), dim3(256), sizeof(double) * 4 * C * (b.nk + 2), s, b.D, b.mb,
+         b.nm, C);
+  dim3 g(((b.nj + 1) * b.ni + TPB - 1) / TPB, b.nk, nz);
+  LAUNCH(k_hsu_mer_ap, g, dim3(TPB), 0, s, b.D, b.mb, b.nm);
+  LAUNCH(k_hsu_mer_ff, dim3((b.nj * b.ni + TPB - 1) / TPB, b.nk, nz), dim3(TPB), 0, s, b.D, b.mb, b.nm);
+  LAUNCH(k_hsu_mer_fin, g, dim3(TPB), 0, s, b.D, b.mb, b.nm);
   // the zonal pass reads a zonally filtered copy (the reference filters it
   // per level before the zonal sweep); a separate launch packs the rows of
   // the filter band, which is faster here than filtering inside the pass
-  L_filter(s, b, F_SB, 1, b.nk, 0, 0);
+  L_filter(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
   const int rpb = rows_per_block(b.ni);
-  LAUNCH(k_hsu_zonal, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, b.nm), dim3(256), sizeof(double) * rpb * 6 * (b.ni + 2),
-         s, b.D, b.mb, which);
-  (void)fld;
+  LAUNCH(k_hsu_zonal, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, nz), dim3(256), sizeof(double) * rpb * 6 * (b.ni + 2),
+         s, b.D, b.mb, b.nm);
 }
 
-// unity = 1 on K = 0..nk+1, J = 0..nj, I = 0..ni+1; OLD = THETA and the
-// theta correction on K = k_sigma-1..nk+1 over the same rows
+// op 0: unity = 1 on K = 0..nk+1, J = 0..nj, I = 0..ni+1, and OLD = THETA
+// on K = k_sigma-1..nk+1 over the same rows; op 1: the theta correction
+// there
 __global__ void __launch_bounds__(256) k_adv_elem(const MemberDesc *D, int mb, int op) {
   MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
@@ -392,21 +402,20 @@ __global__ void __launch_bounds__(256) k_adv_elem(const MemberDesc *D, int mb, i
   size_t q = ix3(d, K, J, I);
   if (op == 0) {
     d.unity[q] = 1.;
+    if (K >= d.ks - 1) d.old[q] = d.THETA[q];
   } else if (K >= d.ks - 1) {
-    if (op == 1) d.old[q] = d.THETA[q];
-    else d.THETA[q] -= d.old[q] * (d.unity[q] - 1.);
+    d.THETA[q] -= d.old[q] * (d.unity[q] - 1.);
   }
 }
 
 void L_advection(hipStream_t s, const Bucket &b) {
   int ncol = (b.nj + 1) * (b.ni + 2);
   dim3 ge((ncol + TPB - 1) / TPB, b.nk + 2, b.nm);
+  // unity, theta and hdry advected together; the theta correction after
+  // them (hdry's advection reads neither theta nor unity)
   LAUNCH(k_adv_elem, ge, dim3(TPB), 0, s, b.D, b.mb, 0);
-  hsu(s, b, AV_P3, F_UNITY);
+  hsu3(s, b);
   LAUNCH(k_adv_elem, ge, dim3(TPB), 0, s, b.D, b.mb, 1);
-  hsu(s, b, AV_THETA, F_THETA);
-  LAUNCH(k_adv_elem, ge, dim3(TPB), 0, s, b.D, b.mb, 2);
-  hsu(s, b, AV_HDRY, F_HDRY);
   L_restore_mass(s, b);
   L_set_p2etc(s, b, 0);
 }

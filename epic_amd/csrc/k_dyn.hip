@@ -658,60 +658,85 @@ void L_merid_hv(hipStream_t s, const Bucket &b, int f, int kind) {
          b.D, b.mb, f, (int)F_SB, kind);
 }
 
-// zonal hyperviscosity: one 64-lane wavefront per (K,J) row held in LDS, the
-// waves of a block independent (wavefront-level barriers only); the zonal
-// means are sequential sums in lane 0, as the reference sums them
-__global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, int f, int kind) {
+// zonal hyperviscosity: a block holds RB consecutive (K,J) rows in LDS, each
+// wavefront working on RB/4 of them along I; the zonal means are sequential
+// sums, as the reference sums them, with lane r of the first wavefront
+// summing row r (so one chain of additions serves RB rows).  Rows are
+// strided by an odd number of doubles so the lanes of the serial phase fall
+// in distinct LDS banks.
+__global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, int f, int kind, int RB) {
   MEMBER;
   extern __shared__ double lds[];
-  const int ni = d.ni, W = ni + 2;
+  const int ni = d.ni, W = ni + 2, WS = W | 1;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int rpb = blockDim.x >> 6;
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y;
-  const int J = blockIdx.x * rpb + wv;
-  if (K > r.k1 || d.nu4 == 0. || J > d.nj + 1) return;  // whole wavefronts leave together
-  double *hh = lds + (size_t)wv * (2 * W + 1), *dd = hh + W, *meanv = dd + W;
+  if (K > r.k1 || d.nu4 == 0.) return;  // uniform over the block
+  double *HH = lds, *DD = lds + (size_t)RB * WS, *meanv = DD + (size_t)RB * WS;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
-  const bool upd = J >= r.jbot && J <= d.nj;
-  for (int I = lane; I < W; I += 64) hh[I] = a[ix2(d, J, I)];
-  wave_sync();
-  if (upd) {
-    const double m = d.m[2 * J + r.jh];
-    const bool taper = fabs(d.lat[2 * J + r.jh]) > 45.;
-    double dc[2];
-    dc[0] = taper ? d.coef2 * d.m0z * d.m0z : d.coef2 * m * m;
-    dc[1] = taper ? d.coef4 * d.m0z * d.m0z : d.coef4 * m * m;
-    for (int pass = 0; pass < 2; pass++) {
-      double *src = pass ? dd : hh, *dst = pass ? hh : dd;
-      for (int I = 1 + lane; I <= ni; I += 64) dst[I] = dc[pass] * (src[I - 1] - 2. * src[I] + src[I + 1]);
-      wave_sync();
-      if (lane == 0) {
-        dst[ni + 1] = dst[1];
-        dst[0] = dst[ni];
-        double fm = seq_acc(dst + 1, ni);
-        fm /= ni;
-        meanv[0] = fm;
-      }
-      wave_sync();
-      const double fm = meanv[0];
-      for (int I = lane; I < W; I += 64) dst[I] -= fm;
-      wave_sync();
-    }
+  const int J0 = blockIdx.x * RB, jlast = d.nj + 1;
+  for (int t = wv; t < RB; t += 4) {
+    const int J = J0 + t;
+    double *hh = HH + t * WS;
+    for (int I = lane; I < W; I += 64) hh[I] = (J <= jlast) ? a[ix2(d, J, I)] : 0.;
   }
-  // pass 2 result is in hh; apply with sign -1, then pads
-  for (int I = 1 + lane; I <= ni; I += 64) {
-    double v = a[ix2(d, J, I)];
-    if (upd) v += d.dt * (double)(-1) * hh[I];
-    a[ix2(d, J, I)] = v;
-    if (I == ni) a[ix2(d, J, 0)] = v;
-    if (I == 1) a[ix2(d, J, ni + 1)] = v;
+  double dc[2][4];
+  for (int u = 0; u < RB / 4; u++) {
+    const int Jc = min(J0 + wv + 4 * u, d.nj);  // rows past nj are never applied
+    const double m = d.m[2 * Jc + r.jh];
+    const bool taper = fabs(d.lat[2 * Jc + r.jh]) > 45.;
+    dc[0][u] = taper ? d.coef2 * d.m0z * d.m0z : d.coef2 * m * m;
+    dc[1][u] = taper ? d.coef4 * d.m0z * d.m0z : d.coef4 * m * m;
+  }
+  __syncthreads();
+  for (int pass = 0; pass < 2; pass++) {
+    double *SRC = pass ? DD : HH, *DST = pass ? HH : DD;
+    for (int u = 0; u < RB / 4; u++) {
+      const int t = wv + 4 * u;
+      const double *src = SRC + t * WS;
+      double *dst = DST + t * WS;
+      for (int I = 1 + lane; I <= ni; I += 64) dst[I] = dc[pass][u] * (src[I - 1] - 2. * src[I] + src[I + 1]);
+    }
+    __syncthreads();
+    if (wv == 0 && lane < RB) {
+      double *dst = DST + lane * WS;
+      dst[ni + 1] = dst[1];
+      dst[0] = dst[ni];
+      meanv[lane] = seq_acc(dst + 1, ni) / ni;
+    }
+    __syncthreads();
+    for (int u = 0; u < RB / 4; u++) {
+      const int t = wv + 4 * u;
+      double *dst = DST + t * WS;
+      const double fm = meanv[t];
+      for (int I = lane; I < W; I += 64) dst[I] -= fm;
+    }
+    __syncthreads();
+  }
+  // pass 2 result is in HH; apply with sign -1 on the rows of the range,
+  // then pads
+  for (int u = 0; u < RB / 4; u++) {
+    const int t = wv + 4 * u, J = J0 + t;
+    if (J > jlast) break;
+    const bool upd = J >= r.jbot && J <= d.nj;
+    const double *hh = HH + t * WS;
+    for (int I = 1 + lane; I <= ni; I += 64) {
+      double v = a[ix2(d, J, I)];
+      if (upd) v += d.dt * (double)(-1) * hh[I];
+      a[ix2(d, J, I)] = v;
+      if (I == ni) a[ix2(d, J, 0)] = v;
+      if (I == 1) a[ix2(d, J, ni + 1)] = v;
+    }
   }
 }
 void L_zonal_hv(hipStream_t s, const Bucket &b, int f, int kind) {
-  const int rpb = 4, W = b.ni + 2;
-  dim3 g((b.nj + 2 + rpb - 1) / rpb, b.nk, b.nm);
-  LAUNCH(k_zonal_hv, g, dim3(64 * rpb), sizeof(double) * (2 * W + 1) * rpb, s, b.D, b.mb, f, kind);
+  // 16 (or 8) rows per block when that still gives every CU a few blocks
+  // and two blocks fit in one CU's LDS, else 4
+  const int rows = (b.nj + 2) * b.nk * b.nm;
+  const int RB = (rows / 16 >= 1024 && b.ni <= 128) ? 16 : (rows / 8 >= 1024 && b.ni <= 256) ? 8 : 4;
+  const int WS = (b.ni + 2) | 1;
+  dim3 g((b.nj + 2 + RB - 1) / RB, b.nk, b.nm);
+  LAUNCH(k_zonal_hv, g, dim3(256), sizeof(double) * (2 * RB * WS + RB), s, b.D, b.mb, f, kind, RB);
 }
 
 }  // namespace epic

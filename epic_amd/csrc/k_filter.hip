@@ -22,7 +22,8 @@ namespace epic {
 #define MEMBER const MemberDesc &d = D[mb + blockIdx.z]
 
 template <class T>
-__global__ void __launch_bounds__(256) k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int vgrid, int pads) {
+__global__ void __launch_bounds__(256) k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int kn, int vgrid,
+                                                    int pads) {
   MEMBER;
   extern __shared__ double lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
@@ -32,8 +33,9 @@ __global__ void __launch_bounds__(256) k_fft_filter(const MemberDesc *D, int mb,
   const int r = blockIdx.x * RB + rs;
   const bool active = r < nrows;
   const int J = active ? (vgrid ? d.rows_v[r] : d.rows_h[r]) : 0;
-  const int K = k0 + blockIdx.y;
-  T *a = fld_ptr_t<T>(d, f) + (size_t)K * d.N2;
+  // planes k0..k0+kn-1 of each of gridDim.y/kn contiguous copies of the array
+  const int copy = blockIdx.y / kn, K = k0 + blockIdx.y - copy * kn;
+  T *a = fld_ptr_t<T>(d, f) + (size_t)copy * d.N3 + (size_t)K * d.N2;
   T *buf = lds + (size_t)rs * n;
   const int bits = d.nlogn - 1;
   const double *hl = (vgrid ? d.hl_pv : d.hl_h) + (size_t)J * (n / 2 + 2);
@@ -88,22 +90,22 @@ __global__ void __launch_bounds__(256) k_fft_filter(const MemberDesc *D, int mb,
 }
 
 template <class T>
-static void launch_fft(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int pads) {
+static void launch_fft(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int pads, int copies) {
   const int n = b.ni, nq = n / 4, RB = TPB / nq;
   const int rows = b.nj + 1;  // upper bound on the listed rows
-  dim3 g((rows + RB - 1) / RB, k1 - k0 + 1, b.nm);
+  dim3 g((rows + RB - 1) / RB, (k1 - k0 + 1) * copies, b.nm);
   size_t shm = sizeof(T) * (size_t)RB * n;
-  LAUNCH(k_fft_filter<T>, g, dim3(TPB), shm, s, b.D, b.mb, f, k0, vgrid, pads);
+  LAUNCH(k_fft_filter<T>, g, dim3(TPB), shm, s, b.D, b.mb, f, k0, k1 - k0 + 1, vgrid, pads);
 }
 
 // mode 0: also refresh the zonal pads of the filtered rows (the producers of
 // every filtered copy write the pads of the rows they fill); mode 1: no pads
-void L_filter(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int mode) {
-  launch_fft<double>(s, b, f, k0, k1, vgrid, mode == 0);
+void L_filter(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int mode, int copies) {
+  launch_fft<double>(s, b, f, k0, k1, vgrid, mode == 0, copies);
 }
 // the same on an FP32 array (mixed precision)
 void L_filter_f32(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int mode) {
-  launch_fft<float>(s, b, f, k0, k1, vgrid, mode == 0);
+  launch_fft<float>(s, b, f, k0, k1, vgrid, mode == 0, 1);
 }
 
 // ------------------------------------------------------------- 3-D form

@@ -46,8 +46,9 @@ void L_zero_dzdt(hipStream_t s, const Bucket &b);
 
 // ---- zonal filter (k_filter.hip)
 // 2-D filter of planes k0..k1 of fld; vgrid selects the v/pv rows; mode 0
-// also refreshes the zonal pads of every row, mode 1 leaves pads alone.
-void L_filter(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode);
+// also refreshes the zonal pads of every row, mode 1 leaves pads alone;
+// copies > 1 filters the same planes of that many consecutive 3-D copies.
+void L_filter(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode, int copies = 1);
 void L_filter_f32(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode);
 // the 3-D filter with the PCHIP profile removal (kind: 0 U, 1 V, 2 HDRY, 3 THETA)
 void L_filter3(hipStream_t s, const Bucket &b, int fld, int kind);
