@@ -179,8 +179,9 @@ static int rebuild_buckets(epic_engine *E) {
 }
 
 // store_pgrad_vars (src/shared/epic_funcs_diag.c:3595-3950)
-static void rec_pgrad_vars(hipStream_t s, const Bucket &b) {
-  L_zero_dzdt(s, b);
+// set_p2etc then store_pgrad_vars
+static void rec_p2etc_pgrad_vars(hipStream_t s, const Bucket &b, int update_theta) {
+  L_set_p2etc(s, b, update_theta);
   L_thermo(s, b);
   L_gz_from_u(s, b);
   L_gz3(s, b);
@@ -195,8 +196,7 @@ static void rec_store_diag(hipStream_t s, const Bucket &b) {
 }
 
 static void rec_sync(hipStream_t s, const Bucket &b, bool adjust) {
-  L_set_p2etc(s, b, 1);
-  rec_pgrad_vars(s, b);
+  rec_p2etc_pgrad_vars(s, b, 1);
   rec_store_diag(s, b);
   L_heating(s, b, adjust);
   L_calc_w(s, b, 0);
@@ -209,35 +209,32 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
   L_calc_w(s, b, 1);  // source_sink + calc_w
   L_uv_core(s, b);
   L_uv_drag(s, b);
+  // hdry and theta take their hyperviscosity together (neither reads the
+  // other, and restore_mass touches hdry only)
+  const FieldSet hth = {2, {F_HDRY, F_THETA}, {2, 3}};
   if (hv) {
-    L_merid_hv(s, b, F_HDRY, 2);
-    L_zonal_hv(s, b, F_HDRY, 2);
+    L_merid_hv(s, b, hth);
+    L_zonal_hv(s, b, hth);
   }
   L_restore_mass(s, b);
-  if (hv) {
-    L_merid_hv(s, b, F_THETA, 3);
-    L_zonal_hv(s, b, F_THETA, 3);
-  }
   L_advection(s, b);
-  rec_pgrad_vars(s, b);
+  rec_p2etc_pgrad_vars(s, b, 0);
   // filtered, monotone copy of GZ3
   L_gz_filtercopy(s, b);
   L_gz_mono(s, b);
-  L_filter3(s, b, F_DUDT_T, 0);
-  L_filter3(s, b, F_DVDT_T, 1);
-  L_filter3(s, b, F_HDRY, 2);
+  // the four 3-D filters in one pass (each reads only its own field and the
+  // filtered GZ3 copy; restore_mass after them touches hdry only)
+  const FieldSet f4 = {4, {F_DUDT_T, F_DVDT_T, F_HDRY, F_THETA}, {0, 1, 2, 3}};
+  L_filter3(s, b, f4);
   L_restore_mass(s, b);
-  L_filter3(s, b, F_THETA, 3);
-  L_set_p2etc(s, b, 1);
-  rec_pgrad_vars(s, b);
+  rec_p2etc_pgrad_vars(s, b, 1);
   L_pgrad(s, b);
   if (divd) L_divdamp(s, b);
   L_ab(s, b);
   if (hv) {
-    L_merid_hv(s, b, F_U, 0);
-    L_merid_hv(s, b, F_V, 1);
-    L_zonal_hv(s, b, F_U, 0);
-    L_zonal_hv(s, b, F_V, 1);
+    const FieldSet uv = {2, {F_U, F_V}, {0, 1}};
+    L_merid_hv(s, b, uv);
+    L_zonal_hv(s, b, uv);
   }
   rec_store_diag(s, b);
   L_heating(s, b, adjust);
@@ -380,7 +377,8 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
     add(&d.DUDT, sizeof(double) * 3 * N3, nul);
     add(&d.DVDT, sizeof(double) * 3 * N3, nul);
   }
-  add(&d.spl, sizeof(double) * (size_t)(c->nj + 2) * 3 * (c->nk + 2), nul);
+  // PCHIP tables of the batched 3-D filter, one set per field
+  add(&d.spl, sizeof(double) * 4 * (size_t)(c->nj + 2) * 3 * (c->nk + 2), nul);
   add(&d.rowbuf, sizeof(double) * 8 * (c->nj + 3), nul);
   add(&d.dyn, sizeof(MemberDyn), nul);
   size_t total = 0;

@@ -417,7 +417,7 @@ void L_advection(hipStream_t s, const Bucket &b) {
   hsu3(s, b);
   LAUNCH(k_adv_elem, ge, dim3(TPB), 0, s, b.D, b.mb, 1);
   L_restore_mass(s, b);
-  L_set_p2etc(s, b, 0);
+  // set_p2etc(DONT_UPDATE_THETA) follows, with store_pgrad_vars
 }
 
 }  // namespace epic

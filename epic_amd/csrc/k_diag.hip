@@ -43,18 +43,6 @@ void L_fill(hipStream_t s, const Bucket &b, int f, double v) {
   LAUNCH(k_fill, dim3(256, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb, f, v);
 }
 
-// DZDT3 is zeroed whole while grid.itime <= 1 (src/shared/epic_funcs_diag.c:3674-3690)
-__global__ void __launch_bounds__(256) k_zero_dzdt(const MemberDesc *D, int mb) {
-  MEMBER;
-  if (d.dyn->itime > 1) return;
-  size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * TPB;
-  for (; i < (size_t)d.N3; i += stride) d.DZDT3[i] = 0.;
-}
-void L_zero_dzdt(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_zero_dzdt, dim3(256, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
-}
-
 // zonal periodicity of a whole 3-D array (bc_lateral THREEDIM,
 // src/single/s_epic_functions.c:72-88): K = 0..nk+1, J = 0..nj+1
 __global__ void __launch_bounds__(256) k_bc3(const MemberDesc *D, int mb, int f) {
@@ -211,9 +199,17 @@ void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
 
 // ------------------------------------------------- store_pgrad_vars parts
 // T3, EXNER3, RHO3, T2, RHO2 per cell and the top values; DZDT3 <- GZ3 when
-// itime > 1.  One lane per (K,J,I), K = 0..nk.
+// itime > 1, DZDT3 zeroed whole while itime <= 1
+// (src/shared/epic_funcs_diag.c:3674-3690).  One lane per (K,J,I), K = 0..nk.
 __global__ void __launch_bounds__(256) k_thermo(const MemberDesc *D, int mb) {
   MEMBER;
+  if (d.dyn->itime <= 1) {
+    // no lane of this kernel writes DZDT3 while itime <= 1
+    const size_t nthr = (size_t)gridDim.x * gridDim.y * blockDim.x;
+    for (size_t i = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; i < (size_t)d.N3;
+         i += nthr)
+      d.DZDT3[i] = 0.;
+  }
   FLAT2(0, d.nj + 1, 0, d.ni + 2);
   const int K = blockIdx.y;
   const size_t q = ix3(d, K, J, I);

@@ -593,16 +593,22 @@ __device__ __forceinline__ double d2dy2_at(const double *hh, const MemberDesc &d
 // The two d2dy2 passes of del^4 (coef2, then coef4) in one launch: the
 // first-pass values a row needs (rows j0..j2) are recomputed by the lane
 // with the same expression instead of being staged in a scratch plane.
-__global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, int src, int dst, int kind) {
-  MEMBER;
+// field i of a set and its member
+#define SET_MEMBER                                              \
+  const int fi = (int)blockIdx.z / nm;                          \
+  const MemberDesc &d = D[mb + (int)blockIdx.z - fi * nm];      \
+  const int f = fs.f[fi], kind = fs.kind[fi]
+
+__global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+  SET_MEMBER;
   const int vg = (kind == 1);
   const int jmax = vg ? d.nj + 1 : d.nj;
   FLAT2(0, jmax + 1, 1, d.ni);
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y;
   if (K > r.k1) return;
-  const double *hh = fld_ptr(d, src) + (size_t)K * d.N2;
-  double *out = fld_ptr(d, dst) + (size_t)K * d.N2;
+  const double *hh = fld_ptr(d, f) + (size_t)K * d.N2;
+  double *out = d.sB + (size_t)fi * d.N3 + (size_t)K * d.N2;
   const double *cx = vg ? d.vcx : d.hcx, *cy = vg ? d.vcy : d.hcy, *cz = vg ? d.vcz : d.hcz;
   int j0, j1, j2;
   if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
@@ -616,11 +622,11 @@ __global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, in
   out[ix2(d, J, I)] = a0 * p0 + a1 * p1 + a2 * p2;
 }
 
-// Rows of the del^4 result D (sB, 1024/ni rows per block): D is zonally
-// filtered in LDS (rows of the filter band), then A += DT*(-1)*D on the
-// kind's rows and the zonal pads of every row 0..nj+1 are written
-__global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, int f, int src, int kind) {
-  MEMBER;
+// Rows of the del^4 result D (copy i of sB, 1024/ni rows per block): D is
+// zonally filtered in LDS (rows of the filter band), then A += DT*(-1)*D on
+// the kind's rows and the zonal pads of every row 0..nj+1 are written
+__global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+  SET_MEMBER;
   extern __shared__ double lds[];
   const RowMap m = row_map(d.ni);
   HvRange r = hv_range(d, kind);
@@ -630,7 +636,7 @@ __global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, i
   const int J = live ? Jr : 0, ni = d.ni, vg = (kind == 1);
   double *x = lds + (size_t)m.g * 2 * ni, *work = x + ni;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
-  const double *dd = fld_ptr(d, src) + (size_t)K * d.N2;
+  const double *dd = d.sB + (size_t)fi * d.N3 + (size_t)K * d.N2;
   const bool upd = live && J >= r.jbot && J <= d.nj;
   if (upd)
     for (int I = 1 + m.lane; I <= ni; I += m.stride) x[I - 1] = dd[ix2(d, J, I)];
@@ -649,13 +655,16 @@ __global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, i
   }
 }
 
-void L_merid_hv(hipStream_t s, const Bucket &b, int f, int kind) {
-  const int jrows = (kind == 1) ? b.nj + 2 : b.nj + 1;
-  dim3 g = grid2(jrows, b.ni, b.nk, b.nm);
-  LAUNCH(k_d2dy2x2, g, dim3(TPB), 0, s, b.D, b.mb, f, (int)F_SB, kind);
+void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
+  bool any_v = false;
+  for (int i = 0; i < fs.n; i++) any_v |= (fs.kind[i] == 1);
+  // rows 0..nj (+ nj+1 on the v grid); the kernel drops rows past its kind's
+  const int jrows = any_v ? b.nj + 2 : b.nj + 1;
+  dim3 g = grid2(jrows, b.ni, b.nk, fs.n * b.nm);
+  LAUNCH(k_d2dy2x2, g, dim3(TPB), 0, s, b.D, b.mb, b.nm, fs);
   const int rpb = rows_per_block(b.ni);
-  LAUNCH(k_hv_apply, dim3((b.nj + 2 + rpb - 1) / rpb, b.nk, b.nm), dim3(256), sizeof(double) * rpb * 2 * b.ni, s,
-         b.D, b.mb, f, (int)F_SB, kind);
+  LAUNCH(k_hv_apply, dim3((b.nj + 2 + rpb - 1) / rpb, b.nk, fs.n * b.nm), dim3(256), sizeof(double) * rpb * 2 * b.ni,
+         s, b.D, b.mb, b.nm, fs);
 }
 
 // zonal hyperviscosity: a block holds RB consecutive (K,J) rows in LDS, each
@@ -664,8 +673,8 @@ void L_merid_hv(hipStream_t s, const Bucket &b, int f, int kind) {
 // summing row r (so one chain of additions serves RB rows).  Rows are
 // strided by an odd number of doubles so the lanes of the serial phase fall
 // in distinct LDS banks.
-__global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, int f, int kind, int RB) {
-  MEMBER;
+__global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, int nm, FieldSet fs, int RB) {
+  SET_MEMBER;
   extern __shared__ double lds[];
   const int ni = d.ni, W = ni + 2, WS = W | 1;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -729,14 +738,14 @@ __global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, i
     }
   }
 }
-void L_zonal_hv(hipStream_t s, const Bucket &b, int f, int kind) {
+void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   // 16 (or 8) rows per block when that still gives every CU a few blocks
   // and two blocks fit in one CU's LDS, else 4
-  const int rows = (b.nj + 2) * b.nk * b.nm;
+  const int rows = (b.nj + 2) * b.nk * b.nm * fs.n;
   const int RB = (rows / 16 >= 1024 && b.ni <= 128) ? 16 : (rows / 8 >= 1024 && b.ni <= 256) ? 8 : 4;
   const int WS = (b.ni + 2) | 1;
-  dim3 g((b.nj + 2 + RB - 1) / RB, b.nk, b.nm);
-  LAUNCH(k_zonal_hv, g, dim3(256), sizeof(double) * (2 * RB * WS + RB), s, b.D, b.mb, f, kind, RB);
+  dim3 g((b.nj + 2 + RB - 1) / RB, b.nk, fs.n * b.nm);
+  LAUNCH(k_zonal_hv, g, dim3(256), sizeof(double) * (2 * RB * WS + RB), s, b.D, b.mb, b.nm, fs, RB);
 }
 
 }  // namespace epic

@@ -197,9 +197,15 @@ __device__ int pchip_wave(int n, const double *x, const double *y, double *dd, i
 // first minimum of its strided subset, the lanes reduce to the lowest value
 // and, among equals, the lowest I), the profile table at that column and its
 // PCHIP derivatives
+// field i of a set and its member
+#define SET_MEMBER                                              \
+  const int fi = (int)blockIdx.z / nm;                          \
+  const MemberDesc &d = D[mb + (int)blockIdx.z - fi * nm];      \
+  const int f = fs.f[fi], kind = fs.kind[fi]
+
 template <class T>
-__global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb, int f, int kind) {
-  MEMBER;
+__global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+  SET_MEMBER;
   F3 r = f3_range(d, kind);
   const int J = r.jstart + blockIdx.x;
   if (J > d.nj) return;  // uniform per block
@@ -230,7 +236,7 @@ __global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb
   wave_sync();
   if (pchip_wave(n, sx, sy, sz, lane) && lane == 0) atomicOr(&d.dyn->err, 4);
   wave_sync();
-  double *tx = d.spl + (size_t)J * 3 * KT, *ty = tx + KT, *tz = ty + KT;
+  double *tx = d.spl + (size_t)(fi * (d.nj + 2) + J) * 3 * KT, *ty = tx + KT, *tz = ty + KT;
   for (int e = lane; e < n; e += 64) {
     tx[e] = sx[e];
     ty[e] = sy[e];
@@ -245,8 +251,8 @@ __global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb
 // band come out as (a - v) + v, as in the reference, which subtracts and adds
 // the profile on every row.  RB rows per block, ni/4 lanes per row.
 template <class T>
-__global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, int f, int kind) {
-  MEMBER;
+__global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+  SET_MEMBER;
   extern __shared__ double lds_raw[];
   double *tabs = lds_raw;
   F3 r = f3_range(d, kind);
@@ -262,7 +268,7 @@ __global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, in
   double *tab = tabs + (size_t)rs * 3 * KT;
   T *buf = reinterpret_cast<T *>(tabs + (size_t)RB * 3 * KT) + (size_t)rs * n;
   if (live)
-    for (int e = lane; e < 3 * KT; e += nq) tab[e] = d.spl[(size_t)J * 3 * KT + e];
+    for (int e = lane; e < 3 * KT; e += nq) tab[e] = d.spl[(size_t)(fi * (d.nj + 2) + J) * 3 * KT + e];
   row_sync(nq);
   T *a = fld_ptr_t<T>(d, f) + (size_t)K * d.N2;
   const double *tx = tab, *ty = tab + KT, *tz = tab + 2 * KT;
@@ -336,15 +342,15 @@ __global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, in
     }
 }
 
-template <class T>
-static void filter3_t(hipStream_t s, const Bucket &b, int f, int kind) {
-  const int jstart = (kind == 1) ? 1 : 0, nrow = b.nj + 1 - jstart;
-  LAUNCH(k_spline_prep<T>, dim3(nrow, 1, b.nm), dim3(64), 0, s, b.D, b.mb, f, kind);
+// the rows of a kind: v rows 1..nj for V, 0..nj otherwise (the launches
+// cover 0..nj and the kernels drop the rows outside the kind's range)
+void L_filter3(hipStream_t s, const Bucket &b, const FieldSet &fs) {
+  const int nrow = b.nj + 1, nz = fs.n * b.nm;
+  LAUNCH(k_spline_prep<double>, dim3(nrow, 1, nz), dim3(64), 0, s, b.D, b.mb, b.nm, fs);
   const int nq = b.ni / 4, RB = (256 / nq) < 16 ? (256 / nq) : 16;
-  dim3 g((nrow + RB - 1) / RB, b.nk, b.nm);
-  size_t shm = sizeof(double) * (size_t)RB * 3 * (b.nk + 2) + sizeof(T) * (size_t)RB * b.ni;
-  LAUNCH(k_filter3<T>, g, dim3(RB * nq), shm, s, b.D, b.mb, f, kind);
+  dim3 g((nrow + RB - 1) / RB, b.nk, nz);
+  size_t shm = sizeof(double) * (size_t)RB * 3 * (b.nk + 2) + sizeof(double) * (size_t)RB * b.ni;
+  LAUNCH(k_filter3<double>, g, dim3(RB * nq), shm, s, b.D, b.mb, b.nm, fs);
 }
-void L_filter3(hipStream_t s, const Bucket &b, int f, int kind) { filter3_t<double>(s, b, f, kind); }
 
 }  // namespace epic

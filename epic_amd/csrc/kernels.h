@@ -20,6 +20,14 @@ extern bool g_prof_on;
     if (::epic::g_prof_on) ::epic::prof_end(s);          \
   } while (0)
 
+// Up to four fields processed by one launch of a batched kernel, each with
+// its kind (0 U, 1 V, 2 HDRY, 3 THETA); blockIdx.z = field index * members
+// + member, and field i uses scratch copy i.
+struct FieldSet {
+  int n;
+  int f[4], kind[4];
+};
+
 struct Bucket {
   const MemberDesc *D;  // device array of descriptors
   int mb, nm;           // first member, member count
@@ -42,7 +50,6 @@ void L_calc_w(hipStream_t s, const Bucket &b, int with_source);
 void L_copy_planes(hipStream_t s, const Bucket &b, int src, int dst, int k0, int k1);
 void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1);
 void L_divergence(hipStream_t s, const Bucket &b, int uu, int vv, int dst, int k0, int k1);
-void L_zero_dzdt(hipStream_t s, const Bucket &b);
 
 // ---- zonal filter (k_filter.hip)
 // 2-D filter of planes k0..k1 of fld; vgrid selects the v/pv rows; mode 0
@@ -50,8 +57,8 @@ void L_zero_dzdt(hipStream_t s, const Bucket &b);
 // copies > 1 filters the same planes of that many consecutive 3-D copies.
 void L_filter(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode, int copies = 1);
 void L_filter_f32(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode);
-// the 3-D filter with the PCHIP profile removal (kind: 0 U, 1 V, 2 HDRY, 3 THETA)
-void L_filter3(hipStream_t s, const Bucket &b, int fld, int kind);
+// the 3-D filter with the PCHIP profile removal, on every field of the set
+void L_filter3(hipStream_t s, const Bucket &b, const FieldSet &fs);
 
 // ---- momentum (k_dyn.hip)
 void L_uv_core(hipStream_t s, const Bucket &b);
@@ -64,9 +71,10 @@ void L_zero_tend(hipStream_t s, const Bucket &b);
 void L_gz_mono(hipStream_t s, const Bucket &b);
 void L_gz_filtercopy(hipStream_t s, const Bucket &b);
 void L_restore_mass(hipStream_t s, const Bucket &b);
-// hyperviscosity: kind 0 U, 1 V, 2 HDRY, 3 THETA
-void L_merid_hv(hipStream_t s, const Bucket &b, int fld, int kind);
-void L_zonal_hv(hipStream_t s, const Bucket &b, int fld, int kind);
+// hyperviscosity on every field of the set (at most two: the meridional
+// part keeps its del^4 result in scratch copy i of sB)
+void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs);
+void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs);
 
 // ---- mixed-precision momentum kernels (k_mixed.hip)
 void LM_zero_tend(hipStream_t s, const Bucket &b);
