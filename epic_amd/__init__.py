@@ -19,7 +19,7 @@ FP64, MIXED = 0, 1
 
 FIELDS = ["U", "V", "HDRY", "THETA", "P3", "P2", "H3", "THETA2", "T2", "T3", "RHO2", "RHO3",
           "EXNER3", "GZ3", "MONT3", "PV3", "DIV_UV3", "W3", "HEAT3", "DZDT3", "DUDT0", "DUDT1",
-          "DUDT2", "DVDT0", "DVDT1", "DVDT2", "GZ_SURFACE"]
+          "DUDT2", "DVDT0", "DVDT1", "DVDT2", "GZ_SURFACE", "RI2"]
 FIELD = {n: i for i, n in enumerate(FIELDS)}
 
 

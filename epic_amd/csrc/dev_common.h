@@ -44,10 +44,10 @@ struct MemberDesc {
   // state and diagnostics
   double *U, *V, *DUDT, *DVDT, *HDRY, *THETA, *P3, *P2, *THETA2, *H3, *T2, *T3, *RHO2, *RHO3;
   double *EXNER3, *GZ3, *MONT3, *HEAT3, *PV3, *DIV, *W3, *DZDT3, *USPIN, *GZS;
+  double *RI2;  // formed on demand (epic_engine_read, the drop-in's write-back)
   // mixed mode: DUDT, DVDT are one FP64 plane (the current step's sum) and
   // DUDTf, DVDTf the three FP32 Adams-Bashforth history planes
   float *DUDTf, *DVDTf;
-  // scratch
   // sA..sD: scratch, three 3-D copies each (the batched advection gives each
   // advected field its own copy; every other user takes the first)
   double *gz, *varz, *sA, *sB, *sC, *sD, *unity, *old, *spl;

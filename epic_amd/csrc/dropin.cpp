@@ -69,7 +69,8 @@ struct Xfer {
   double *host;
   double *dev;
   size_t n;
-  bool down;  // written by the step: copied back to the caller
+  bool down;     // written by the step: copied back to the caller
+  bool up = true;  // read by the step: copied to the device
 };
 
 // Every array of var.* that the step reads or writes, paired with the
@@ -91,6 +92,7 @@ std::vector<Xfer> transfers(const MemberDesc &d) {
       {var.pv3.value, d.PV3, N3, true},       {var.div_uv3.value, d.DIV, N3, true},
       {var.w3.value, d.W3, N3, true},         {var.dzdt3.value, d.DZDT3, N3, true},
       {var.u_spinup.value, d.USPIN, N3, false}, {var.gz_surface.value, d.GZS, N2, false},
+      {var.ri2.value, d.RI2, N3, true, false},
   };
   return x;
 }
@@ -98,7 +100,7 @@ std::vector<Xfer> transfers(const MemberDesc &d) {
 void upload(const MemberDesc &d) {
   hipStream_t s = engine_stream(g_di.E);
   for (auto &t : transfers(d)) {
-    if (!t.host) continue;
+    if (!t.host || !t.up) continue;
     need(hipMemcpyAsync(t.dev, t.host, sizeof(double) * t.n, hipMemcpyHostToDevice, s) == hipSuccess,
          "host-to-device copy failed");
   }
@@ -107,6 +109,8 @@ void upload(const MemberDesc &d) {
 
 void download(const MemberDesc &d) {
   hipStream_t s = engine_stream(g_di.E);
+  // RI2 is formed from the synced state when it is copied back
+  if (var.ri2.value) need(engine_richardson(g_di.E, g_di.slot) == 0, "RI2 launch failed");
   for (auto &t : transfers(d)) {
     if (!t.host || !t.down) continue;
     need(hipMemcpyAsync(t.host, t.dev, sizeof(double) * t.n, hipMemcpyDeviceToHost, s) == hipSuccess,
@@ -140,6 +144,7 @@ void check_config(const struct epic_planet_abi *planet) {
        "planet->type must be gas-giant or terrestrial");
   need(var.u.value && var.v.value && var.u.tendency && var.v.tendency && var.hdry.value && var.theta.value,
        "prognostic arrays are not allocated");
+  need(planet->x_h2 == 0., "the ortho/para hydrogen thermodynamics (planet->x_h2 > 0) are not implemented");
 }
 
 void create_member(const struct epic_planet_abi *planet) {

@@ -364,7 +364,8 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   add(&d.USPIN, sizeof(double) * N3, c->u_spinup);
   add(&d.GZS, sizeof(double) * N2, c->gz_surface);
   double **fields[] = {&d.HDRY, &d.P2, &d.THETA2, &d.H3, &d.T2, &d.T3, &d.RHO2, &d.RHO3, &d.EXNER3, &d.GZ3,
-                       &d.MONT3, &d.HEAT3, &d.PV3, &d.DIV, &d.W3, &d.DZDT3, &d.gz, &d.varz, &d.unity, &d.old};
+                       &d.MONT3, &d.HEAT3, &d.PV3, &d.DIV, &d.W3, &d.DZDT3, &d.RI2, &d.gz, &d.varz, &d.unity,
+                       &d.old};
   for (auto p : fields) add(p, sizeof(double) * N3, nul);
   // scratch: three copies each, one per field of the batched advection
   for (auto p : {&d.sA, &d.sB, &d.sC, &d.sD}) add(p, sizeof(double) * 3 * N3, nul);
@@ -499,6 +500,27 @@ int engine_sync_member(epic_engine *E, int slot) {
   return 0;
 }
 
+// RI2 of one member, queued on the engine's stream.  store_diag forms it
+// from u, v, P2, T2, RHO2, P3 and RHO3 as the step leaves them, and nothing
+// after store_diag in the step changes those, so forming it when it is read
+// gives store_diag's values (unless u or v were written since the step)
+// and keeps it out of the step itself.
+int engine_richardson(epic_engine *E, int slot) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return -1;
+  if (E->dirty && rebuild_buckets(E)) return -5;
+  for (const Bucket &b : E->buckets)
+    for (int i = 0; i < b.nm; i++)
+      if (E->order[b.mb + i] == slot) {
+        Bucket one = b;
+        one.mb = b.mb + i;
+        one.nm = 1;
+        L_richardson(E->stream, one);
+        HIPCHK(hipGetLastError());
+        return 0;
+      }
+  return -1;
+}
+
 const MemberDesc *engine_member_desc(epic_engine *E, int slot) {
   if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active) return nullptr;
   return &E->m[slot].h;
@@ -621,6 +643,7 @@ static double *field_ptr(MemberDesc &d, int field, size_t &n, int itplanes[3]) {
     case EPIC_F_DVDT0: case EPIC_F_DVDT1: case EPIC_F_DVDT2:
       return d.DVDT + (size_t)(field - EPIC_F_DVDT0) * d.N3;
     case EPIC_F_GZ_SURFACE: n = (size_t)d.N2; return d.GZS;
+    case EPIC_F_RI2: return d.RI2;
   }
   (void)itplanes;
   return nullptr;
@@ -642,6 +665,7 @@ extern "C" int epic_engine_read(epic_engine *E, int slot, int field, double *out
   size_t n;
   double *p = field_ptr(E->m[slot].h, field, n, nullptr);
   if (!p) return -1;
+  if (field == EPIC_F_RI2 && engine_richardson(E, slot) != 0) return -1;
   HIPCHK(hipStreamSynchronize(E->stream));
   HIPCHK(hipMemcpy(out, p, sizeof(double) * n, hipMemcpyDeviceToHost));
   return 0;

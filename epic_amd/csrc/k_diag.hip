@@ -240,6 +240,62 @@ void L_thermo(hipStream_t s, const Bucket &b) {
   LAUNCH(k_thermo, grid2(b.nj + 1, b.ni + 2, b.nk + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
+// ------------------------------------------------------------ RI2
+// Richardson number in the layers, N^2 / ((du)^2 + (dv)^2) (get_richardson
+// with kk = 2K, src/shared/epic_funcs_diag.c:3066-3156; the reference forms
+// dz_inv there and leaves it unused, and its v difference carries a + on
+// V(K,J+1)), with get_brunt2 of a layer (:2885-3064, no smoothing:
+// Drho/Dp from the interfaces above and below, the partial derivatives of
+// the ideal-gas density by centred differences of 0.1 %), return_cp
+// (:3311-3337: 1004 for Held-Suarez, else the centred derivative of the
+// enthalpy cpr*T*rgas of x_h2 = 0) and the layer molar mass mu2.
+// Store_diag computes it for J = 0..nj, I = 1..ni, K = 1..nk and applies
+// bc_lateral; every other cell stays zero.
+__global__ void __launch_bounds__(256) k_richardson(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + blockIdx.y;
+  const double *U = d.U, *V = d.V;
+  double dudz2 = .5 * (U[ix3(d, K - 1, J, I)] + U[ix3(d, K - 1, J, I + 1)] - U[ix3(d, K, J, I)] -
+                       U[ix3(d, K, J, I + 1)]);
+  double dvdz2 = .5 * (V[ix3(d, K - 1, J, I)] + V[ix3(d, K - 1, J + 1, I)] - V[ix3(d, K, J, I)] +
+                       V[ix3(d, K, J + 1, I)]);
+  dudz2 *= dudz2;
+  dvdz2 *= dvdz2;
+  double ri = 1.e+6;
+  if (fcmp_d(dudz2, 0.) != 0) {
+    const double R_GAS = 8.314472e+3;
+    const size_t q = ix3(d, K, J, I);
+    const double pressure = d.P2[q], temperature = d.T2[q], density = d.RHO2[q];
+    double cp;
+    if (d.is_hs) {
+      cp = 1004.;
+    } else {
+      const double dT = temperature * 1.e-6;
+      double h2 = d.cpr * (temperature + dT), h1 = d.cpr * (temperature - dT);
+      h2 *= d.rgas;
+      h1 *= d.rgas;
+      cp = (h2 - h1) / (2. * dT);
+    }
+    const double mu = d.mu2, deltap = 0.001 * pressure, deltaT = 0.001 * temperature;
+    const double drho_dp_T = ((pressure + deltap) * mu / (R_GAS * temperature) -
+                              (pressure - deltap) * mu / (R_GAS * temperature)) / (2. * deltap);
+    const double drho_dT_p = (pressure * mu / (R_GAS * (temperature + deltaT)) -
+                              pressure * mu / (R_GAS * (temperature - deltaT))) / (2. * deltaT);
+    const double Drho_Dp = (d.RHO3[q] - d.RHO3[ix3(d, K - 1, J, I)]) / (d.P3[q] - d.P3[ix3(d, K - 1, J, I)]);
+    const double g = d.g[2 * J + 1];
+    const double brunt2 =
+        g * g * (Drho_Dp - drho_dp_T + temperature / (cp * density * density) * drho_dT_p * drho_dT_p);
+    ri = brunt2 / (dudz2 + dvdz2);
+  }
+  d.RI2[ix3(d, K, J, I)] = ri;
+  if (I == d.ni) d.RI2[ix3(d, K, J, 0)] = ri;
+  if (I == 1) d.RI2[ix3(d, K, J, d.ni + 1)] = ri;
+}
+void L_richardson(hipStream_t s, const Bucket &b) {
+  LAUNCH(k_richardson, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+}
+
 // kinetic energy of a zonally uniform u row (get_kin with v = 0)
 __device__ double kin_uniform(const double *mn, const double *mni, int nj, const double *u1, int J) {
   int jj = 2 * J;

@@ -247,6 +247,7 @@ enum epic_field {
   EPIC_F_EXNER3, EPIC_F_GZ3, EPIC_F_MONT3, EPIC_F_PV3, EPIC_F_DIV_UV3,
   EPIC_F_W3, EPIC_F_HEAT3, EPIC_F_DZDT3, EPIC_F_DUDT0, EPIC_F_DUDT1,
   EPIC_F_DUDT2, EPIC_F_DVDT0, EPIC_F_DVDT1, EPIC_F_DVDT2, EPIC_F_GZ_SURFACE,
+  EPIC_F_RI2,
   EPIC_F_COUNT
 };
 

@@ -42,7 +42,7 @@ struct cpuref {
   int is_hs, is_gas;
   /* fields */
   double *U, *V, *DUDT[3], *DVDT[3], *HDRY, *THETA, *P3, *P2, *THETA2, *H3;
-  double *T2, *T3, *RHO2, *RHO3, *EXNER3, *GZ3, *MONT3, *HEAT3, *PV3, *DIV, *W3, *DZDT3;
+  double *T2, *T3, *RHO2, *RHO3, *EXNER3, *GZ3, *MONT3, *HEAT3, *PV3, *DIV, *W3, *DZDT3, *RI2;
   double *USPIN, *GZS;
   /* scratch and persistent statics of the reference */
   double *gz, *sum_div, *unity, *old, *varz;
@@ -687,8 +687,49 @@ static void store_pgrad_vars(cpuref *R) {
   bc3(R, R->DZDT3);
 }
 
-/* store_diag, src/shared/epic_funcs_diag.c:3963-4053 (RI2 is not restated:
- * no other field reads it while turbulence is off) */
+/* get_brunt2 in a layer (kk = 2K; np = 1, no smoothing),
+ * src/shared/epic_funcs_diag.c:2885-3064, with return_cp :3311-3337 (the
+ * enthalpy of x_h2 = 0 is cpr*T*rgas, :6652-6760), avg_molar_mass
+ * :5360-5398 (nq = 0) and return_density :6353-6395 (ideal gas) */
+static double brunt2_layer(cpuref *R, int K, int J, int I) {
+  epic_case *c = &R->c;
+  const double pressure = A3(R->P2, K, J, I), temperature = A3(R->T2, K, J, I), density = A3(R->RHO2, K, J, I);
+  double cp;
+  if (R->is_hs) {
+    cp = 1004.;
+  } else {
+    double deltaT = temperature * 1.e-6, h2, h1;
+    h2 = c->cpr * (temperature + deltaT);
+    h2 *= c->rgas;
+    h1 = c->cpr * (temperature - deltaT);
+    h1 *= c->rgas;
+    cp = (h2 - h1) / (2. * deltaT);
+  }
+  const double mu3 = 1. / (c->rgas / R_GAS), mu = .5 * (mu3 + mu3);
+  const double deltap = 0.001 * pressure, deltaT = 0.001 * temperature;
+  const double drho_dp_T = ((pressure + deltap) * mu / (R_GAS * temperature) -
+                            (pressure - deltap) * mu / (R_GAS * temperature)) / (2. * deltap);
+  const double drho_dT_p = (pressure * mu / (R_GAS * (temperature + deltaT)) -
+                            pressure * mu / (R_GAS * (temperature - deltaT))) / (2. * deltaT);
+  const double Drho_Dp = (A3(R->RHO3, K, J, I) - A3(R->RHO3, K - 1, J, I)) /
+                         (A3(R->P3, K, J, I) - A3(R->P3, K - 1, J, I));
+  const double g = c->g[2 * J + 1];
+  return g * g * (Drho_Dp - drho_dp_T + temperature / (cp * density * density) * drho_dT_p * drho_dT_p);
+}
+
+/* get_richardson for a layer (kk = 2K), src/shared/epic_funcs_diag.c:3066-3156:
+ * the squared shears are plain differences (the reference forms dz_inv and
+ * leaves it unused), and the v shear carries the reference's sign */
+static double richardson_layer(cpuref *R, int K, int J, int I) {
+  double dudz2 = .5 * (A3(R->U, K - 1, J, I) + A3(R->U, K - 1, J, I + 1) - A3(R->U, K, J, I) - A3(R->U, K, J, I + 1));
+  double dvdz2 = .5 * (A3(R->V, K - 1, J, I) + A3(R->V, K - 1, J + 1, I) - A3(R->V, K, J, I) + A3(R->V, K, J + 1, I));
+  dudz2 *= dudz2;
+  dvdz2 *= dvdz2;
+  if (cpuref_fcmp(dudz2, 0.) != 0) return brunt2_layer(R, K, J, I) / (dudz2 + dvdz2);
+  return 1.e+6;
+}
+
+/* store_diag, src/shared/epic_funcs_diag.c:3963-4053 */
 static void store_diag(cpuref *R) {
   double *buf = R->b2[7];
   for (int K = 0; K <= R->nk; K++) {
@@ -701,6 +742,10 @@ static void store_diag(cpuref *R) {
     filter2(R, buf, FK_U);
     divergence(R, buf, R->V + (size_t)K * R->N2, R->DIV + (size_t)K * R->N2);
   }
+  for (int J = 0; J <= R->nj; J++)
+    for (int I = 1; I <= R->ni; I++)
+      for (int K = 1; K <= R->nk; K++) A3(R->RI2, K, J, I) = richardson_layer(R, K, J, I);
+  bc3(R, R->RI2);
 }
 
 /* calc_heating -> newtonian_cooling with the Held-Suarez T_eq and k_T,
@@ -1672,7 +1717,7 @@ cpuref *cpuref_create(const epic_case_spec *spec) {
   ALLOC3(R->U); ALLOC3(R->V); ALLOC3(R->HDRY); ALLOC3(R->THETA); ALLOC3(R->P3); ALLOC3(R->P2);
   ALLOC3(R->THETA2); ALLOC3(R->H3); ALLOC3(R->T2); ALLOC3(R->T3); ALLOC3(R->RHO2); ALLOC3(R->RHO3);
   ALLOC3(R->EXNER3); ALLOC3(R->GZ3); ALLOC3(R->MONT3); ALLOC3(R->HEAT3); ALLOC3(R->PV3); ALLOC3(R->DIV);
-  ALLOC3(R->W3); ALLOC3(R->DZDT3); ALLOC3(R->USPIN); ALLOC3(R->gz); ALLOC3(R->sum_div);
+  ALLOC3(R->W3); ALLOC3(R->DZDT3); ALLOC3(R->RI2); ALLOC3(R->USPIN); ALLOC3(R->gz); ALLOC3(R->sum_div);
   ALLOC3(R->unity); ALLOC3(R->old); ALLOC3(R->varz);
   for (int p = 0; p < 3; p++) { ALLOC3(R->DUDT[p]); ALLOC3(R->DVDT[p]); }
 #undef ALLOC3
@@ -1745,7 +1790,7 @@ cpuref *cpuref_create(const epic_case_spec *spec) {
 void cpuref_destroy(cpuref *R) {
   if (!R) return;
   double *all[] = {R->U, R->V, R->HDRY, R->THETA, R->P3, R->P2, R->THETA2, R->H3, R->T2, R->T3, R->RHO2,
-                   R->RHO3, R->EXNER3, R->GZ3, R->MONT3, R->HEAT3, R->PV3, R->DIV, R->W3, R->DZDT3, R->USPIN,
+                   R->RHO3, R->EXNER3, R->GZ3, R->MONT3, R->HEAT3, R->PV3, R->DIV, R->W3, R->DZDT3, R->RI2, R->USPIN,
                    R->gz, R->sum_div, R->unity, R->old, R->varz, R->GZS, R->gzu2d, R->gzv2d, R->gz2d, R->hl_h,
                    R->hl_pv, R->w_wm, R->w_wp, R->w_a, R->w_pred, R->w_min, R->w_ff, R->w_d1, R->w_d2, R->w_aaa,
                    R->tx, R->ty, R->tz, R->mn_inv, R->hcx, R->hcy, R->hcz, R->vcx, R->vcy, R->vcz, R->fdata};
@@ -1783,10 +1828,56 @@ int cpuref_read(cpuref *R, int field, double *out) {
     case EPIC_F_DUDT0: case EPIC_F_DUDT1: case EPIC_F_DUDT2: src = R->DUDT[field - EPIC_F_DUDT0]; break;
     case EPIC_F_DVDT0: case EPIC_F_DVDT1: case EPIC_F_DVDT2: src = R->DVDT[field - EPIC_F_DVDT0]; break;
     case EPIC_F_GZ_SURFACE: src = R->GZS; n = R->N2; break;
+    case EPIC_F_RI2: src = R->RI2; break;
     default: return -1;
   }
   memcpy(out, src, sizeof(double) * n);
   return 0;
+}
+
+int cpuref_write(cpuref *R, int field, const double *in) {
+  if (field == EPIC_F_GZ_SURFACE || (field >= EPIC_F_DUDT0 && field <= EPIC_F_DVDT2)) return -1;
+  double *tmp = (double *)malloc(sizeof(double) * R->N3);
+  if (!tmp || cpuref_read(R, field, tmp) != 0) {
+    free(tmp);
+    return -1;
+  }
+  free(tmp);
+  double *dst = NULL;
+  switch (field) {
+    case EPIC_F_U: dst = R->U; break;
+    case EPIC_F_V: dst = R->V; break;
+    case EPIC_F_HDRY: dst = R->HDRY; break;
+    case EPIC_F_THETA: dst = R->THETA; break;
+    case EPIC_F_P3: dst = R->P3; break;
+    case EPIC_F_P2: dst = R->P2; break;
+    case EPIC_F_H3: dst = R->H3; break;
+    case EPIC_F_THETA2: dst = R->THETA2; break;
+    case EPIC_F_T2: dst = R->T2; break;
+    case EPIC_F_T3: dst = R->T3; break;
+    case EPIC_F_RHO2: dst = R->RHO2; break;
+    case EPIC_F_RHO3: dst = R->RHO3; break;
+    case EPIC_F_EXNER3: dst = R->EXNER3; break;
+    case EPIC_F_GZ3: dst = R->GZ3; break;
+    case EPIC_F_MONT3: dst = R->MONT3; break;
+    case EPIC_F_PV3: dst = R->PV3; break;
+    case EPIC_F_DIV_UV3: dst = R->DIV; break;
+    case EPIC_F_W3: dst = R->W3; break;
+    case EPIC_F_HEAT3: dst = R->HEAT3; break;
+    case EPIC_F_DZDT3: dst = R->DZDT3; break;
+    case EPIC_F_RI2: dst = R->RI2; break;
+    default: return -1;
+  }
+  memcpy(dst, in, sizeof(double) * R->N3);
+  return 0;
+}
+
+void cpuref_richardson(cpuref *R) {
+  memset(R->RI2, 0, sizeof(double) * R->N3);
+  for (int J = 0; J <= R->nj; J++)
+    for (int I = 1; I <= R->ni; I++)
+      for (int K = 1; K <= R->nk; K++) A3(R->RI2, K, J, I) = richardson_layer(R, K, J, I);
+  bc3(R, R->RI2);
 }
 
 void cpuref_time_planes(cpuref *R, int *it3, long *steps) {

@@ -19,6 +19,11 @@ int     cpuref_sync(cpuref *r);
 int     cpuref_step(cpuref *r);
 /* Field copy-out in epic_field order; tendencies by time plane. */
 int     cpuref_read(cpuref *r, int field, double *out);
+/* Overwrite a 3-D field (not GZ_SURFACE, not a tendency plane) and form
+ * RI2 again from the current U, V, P2, T2, RHO2, P3, RHO3: pins a
+ * diagnostic of the GPU to the oracle's function of the same inputs. */
+int     cpuref_write(cpuref *r, int field, const double *in);
+void    cpuref_richardson(cpuref *r);
 /* Current IT_ZERO, IT_MINUS1, IT_MINUS2 and the number of steps done. */
 void    cpuref_time_planes(cpuref *r, int *it3, long *steps);
 /* Non-zero once a fatal condition of the reference was met

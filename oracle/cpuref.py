@@ -28,6 +28,8 @@ def lib(nudged=False):
             getattr(L, f).argtypes = [vp]
         L.cpuref_destroy.argtypes = [vp]
         L.cpuref_read.argtypes = [vp, ctypes.c_int, vp]
+        L.cpuref_write.argtypes = [vp, ctypes.c_int, vp]
+        L.cpuref_richardson.argtypes = [vp]
         L.cpuref_time_planes.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)]
         L.cpuref_realft.argtypes = [vp, ctypes.c_ulong, ctypes.c_int]
         L.cpuref_pchip.argtypes = [ctypes.c_int, vp, vp, vp]
@@ -65,6 +67,21 @@ class CpuRef:
         out = np.empty(shape, dtype=np.float64)
         self._L.cpuref_read(self._r, q, out.ctypes.data)
         return out
+
+    def write(self, f, a):
+        from epic_amd import FIELD
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape3
+        if self._L.cpuref_write(self._r, FIELD[f], a.ctypes.data) != 0:
+            raise ValueError(f"cpuref_write({f}) refused")
+
+    def richardson_of(self, fields):
+        """RI2 as store_diag forms it from the given U, V, P2, T2, RHO2, P3,
+        RHO3 (written into this oracle instance, which is left holding them)."""
+        for f in ("U", "V", "P2", "T2", "RHO2", "P3", "RHO3"):
+            self.write(f, fields[f])
+        self._L.cpuref_richardson(self._r)
+        return self.read("RI2")
 
     def time_planes(self):
         it = (ctypes.c_int * 3)()

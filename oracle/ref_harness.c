@@ -136,6 +136,7 @@ static void write_fields(FILE *fo, int step) {
   for (int p = 0; p < 3; p++) fwrite(var.u.tendency + (size_t)p * Nelem3d, sizeof(double), Nelem3d, fo);
   for (int p = 0; p < 3; p++) fwrite(var.v.tendency + (size_t)p * Nelem3d, sizeof(double), Nelem3d, fo);
   fwrite(var.gz_surface.value, sizeof(double), Nelem2d, fo);
+  fwrite(var.ri2.value, sizeof(double), Nelem3d, fo);
 }
 
 int main(int argc, char **argv) {

@@ -48,6 +48,8 @@ def read_dump(fn):
             off += 8 * n3
         f["GZ_SURFACE"] = np.frombuffer(b[off:off + 8 * n2], dtype=np.float64).reshape(nj + 3, ni + 2).copy()
         off += 8 * n2
+        f["RI2"] = np.frombuffer(b[off:off + 8 * n3], dtype=np.float64).reshape(nk + 2, nj + 3, ni + 2).copy()
+        off += 8 * n3
         out.append((hdr, f))
     return out
 

@@ -55,6 +55,7 @@ class Pair:
         self.ref.sync()
         self.nud = CpuRef(sp, nudged=True)
         self.nud.sync()
+        self.fn = None  # oracle instance used as a function of the GPU's fields
         self.steps = 0
 
     def step(self, n):
@@ -87,9 +88,26 @@ class Pair:
         rep = self.report(tag)
         bad = {k: v for k, v in rep.items() if v[0] > v[2]}
         assert not bad, (tag, bad)
+        self.check_ri2(tag)
         return rep
+
+    def check_ri2(self, tag):
+        """RI2 (a ratio of a near-cancelling buoyancy term to the squared
+        shear, with no libm call) is pinned bit for bit as a function: the
+        GPU's RI2 must equal the oracle's store_diag formula applied to the
+        GPU's own U, V, P2, T2, RHO2, P3, RHO3.  Its distance from the
+        oracle's RI2 is reported (one-ulp differences in P2, T2, RHO2 move it
+        by up to about 1e-9 in relative L2 through the cancellation)."""
+        if self.fn is None:
+            self.fn = CpuRef(self.sp)
+        g = {f: self.eng.read(self.slot, f) for f in ("U", "V", "P2", "T2", "RHO2", "P3", "RHO3", "RI2")}
+        want = self.fn.richardson_of(g)
+        say(tag, f"RI2 vs oracle {rel_l2(g['RI2'], self.ref.read('RI2')):.1e}")
+        assert g["RI2"].tobytes() == want.tobytes(), (tag, "RI2", int((g["RI2"] != want).sum()))
 
     def close(self):
         self.eng.close()
         self.ref.close()
         self.nud.close()
+        if self.fn is not None:
+            self.fn.close()

@@ -152,6 +152,7 @@ int main(int argc, char **argv) {
   fwrite(var.u.tendency, sizeof(double), 3 * N3, fo);
   fwrite(var.v.tendency, sizeof(double), 3 * N3, fo);
   fwrite(var.gz_surface.value, sizeof(double), N2, fo);
+  fwrite(var.ri2.value, sizeof(double), N3, fo);
   fclose(fo);
   printf("model_time %ld it_uv_tend %d\n", (long)var.model_time, grid.it_uv_tend);
   return 0;

@@ -60,6 +60,15 @@ def compare(f, ref, tag):
         assert e <= tol, (tag, q, e, fl)
 
 
+def ri2_of(f):
+    """The oracle's store_diag RI2 of the U, V, P2, T2, RHO2, P3, RHO3 in f."""
+    fn = CpuRef(ea.spec(*ARGS))
+    try:
+        return fn.richardson_of(f)
+    finally:
+        fn.close()
+
+
 @pytest.mark.skipif(not os.path.exists(HOST), reason="tests/native/_bin/dropin_host not built")
 def test_dropin_eager_and_lazy_match_oracle(tmp_path):
     h_e, f_e = run(HOST, [0], tmp_path, "eager.bin")
@@ -69,6 +78,8 @@ def test_dropin_eager_and_lazy_match_oracle(tmp_path):
     tp, ref = oracle_state()
     assert [int(x) for x in h_e[1:4]] == tp
     compare(f_e, ref, "dropin_host")
+    # var.ri2 comes back formed from the written-back state
+    assert f_e["RI2"].tobytes() == ri2_of(f_e).tobytes()
 
 
 @pytest.mark.skipif(not (os.path.exists(REF_DROPIN) and os.path.exists(REF)),
@@ -78,3 +89,7 @@ def test_reference_program_with_dropin_matches_reference(tmp_path):
     h_c, f_c = run(REF, [NSTEPS], tmp_path, "cpu.bin")
     assert [int(x) for x in h_g[1:4]] == [int(x) for x in h_c[1:4]]
     compare(f_g, f_c, "epic_ref_dropin")
+    # RI2: the drop-in's from its own state, the reference's from its own,
+    # each equal to the oracle's formula of that state
+    assert f_g["RI2"].tobytes() == ri2_of(f_g).tobytes()
+    assert f_c["RI2"].tobytes() == ri2_of(f_c).tobytes()

@@ -92,6 +92,22 @@ def test_oracle_bitwise_vs_reference_build(args, tmp_path):
         assert r.time_planes()[:3] == [int(x) for x in hdr[1:4]]
 
 
+def test_richardson_of_given_fields_reproduces_store_diag():
+    """The function form the GPU tests use for RI2 (oracle fields written in,
+    RI2 formed again) gives store_diag's RI2 bit for bit, and the golden
+    hashes above pin store_diag's RI2 to the reference build's."""
+    sp = spec_of("jup_tiny")
+    r = cr.CpuRef(sp)
+    r.sync()
+    r.step(3)
+    f = {k: r.read(k) for k in ("U", "V", "P2", "T2", "RHO2", "P3", "RHO3", "RI2")}
+    fn = cr.CpuRef(sp)
+    assert same_bits(fn.richardson_of(f), f["RI2"])
+    # a changed shear changes RI2 where it enters
+    f["U"][2, 4, 5] += 1.0
+    assert not same_bits(fn.richardson_of(f), f["RI2"])
+
+
 # ---------------------------------------------------- numerical pins
 def test_realft_is_the_real_dft_in_nr_packing():
     L = cr.lib()
