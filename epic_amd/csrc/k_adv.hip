@@ -373,7 +373,9 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
 // one advection pass over all three fields of every member
 static void hsu3(hipStream_t s, const Bucket &b) {
   const int ncol = (b.nj + 1) * (b.ni + 2), nz = ADV_FIELDS * b.nm;
-  const int C = b.nk <= 30 ? 64 : 32;
+  // columns per block: as many as keep the four LDS columns within 40 KB
+  int C = 64;
+  while (C > 16 && (size_t)4 * C * (b.nk + 2) * sizeof(double) > 40 * 1024) C /= 2;
   LAUNCH(k_hsu_vert, dim3((ncol + C - 1) / C, 1, nz), dim3(256), sizeof(double) * 4 * C * (b.nk + 2), s, b.D, b.mb,
          b.nm, C);
   dim3 g(((b.nj + 1) * b.ni + TPB - 1) / TPB, b.nk, nz);

@@ -77,43 +77,51 @@ void L_copy_planes(hipStream_t s, const Bucket &b, int src, int dst, int k0, int
 }
 
 // ------------------------------------------------------------ column tiles
-// A block holds COLS consecutive (J,I) columns (J = 0..nj, I = 0..ni+1) and
+// A block holds C consecutive (J,I) columns (J = 0..nj, I = 0..ni+1) and
 // all their levels.  The per-level work (transcendentals, Leslie-Purser
 // weights) runs on all lanes, (column, level) pairs strided over the block;
 // the recurrences in K (pressure from the top, geopotential from the
 // bottom, the running SUM_DIV) run in one lane per column over values staged
-// in LDS, in the reference's order.
-constexpr int COLS = 64;
+// in LDS, in the reference's order.  C is 64 where the block's LDS stays
+// small; deep grids take 32 or 16 columns so that several blocks still fit
+// in a CU's LDS.
 struct Tile {
-  int c, kk, KS, J, I;
+  int C, c, kk, KS, J, I;
   bool live;
 };
-__device__ __forceinline__ Tile tile_of(const MemberDesc &d) {
+__device__ __forceinline__ Tile tile_of(const MemberDesc &d, int C) {
   Tile t;
-  t.c = threadIdx.x % COLS;
-  t.kk = threadIdx.x / COLS;
-  t.KS = blockDim.x / COLS;
-  const int nI = d.ni + 2, col = blockIdx.x * COLS + t.c;
+  t.C = C;
+  t.c = threadIdx.x % C;
+  t.kk = threadIdx.x / C;
+  t.KS = blockDim.x / C;
+  const int nI = d.ni + 2, col = blockIdx.x * C + t.c;
   t.live = col < (d.nj + 1) * nI;
   t.J = t.live ? col / nI : 0;
   t.I = t.live ? col % nI : 0;
   return t;
 }
-static inline dim3 tile_grid(const Bucket &b) {
-  return dim3((unsigned)(((b.nj + 1) * (b.ni + 2) + COLS - 1) / COLS), 1, (unsigned)b.nm);
+// columns per block for a kernel keeping `arrays` (nk+1)-long columns in LDS
+static inline int tile_cols(const Bucket &b, int arrays) {
+  int C = 64;
+  while (C > 16 && (size_t)arrays * C * (b.nk + 1) * sizeof(double) > 40 * 1024) C /= 2;
+  return C;
+}
+static inline dim3 tile_grid(const Bucket &b, int C) {
+  return dim3((unsigned)(((b.nj + 1) * (b.ni + 2) + C - 1) / C), 1, (unsigned)b.nm);
 }
 
 // ------------------------------------------------------------ set_p2etc
 // P3 from the top down (prefix of dsgth*g*h), the sigma-region pressures,
 // h = max(h_min, dp/(g dsgth)), and the layer diagnostics P2, H3, THETA2
 // (and the theta of the sigma region when update_theta)
-__global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, int update_theta) {
+__global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, int update_theta, int C) {
   MEMBER;
   extern __shared__ double lds[];
-  const Tile T = tile_of(d);
+  const Tile T = tile_of(d, C);
   const int nk = d.nk, ks = d.ks, W = nk + 1, J = T.J, I = T.I;
-  double *a = lds + T.c * W, *p2 = lds + COLS * W + T.c * W, *pw = lds + 2 * COLS * W + T.c * W;
-  double *aux = lds + 3 * COLS * W + 2 * T.c;
+  double *a = lds + T.c * W, *pw = lds + C * W + T.c * W;
+  double *aux = lds + 2 * C * W + 3 * T.c;  // theta(ks-2), H3(1), P2(1)
   double *P3 = d.P3, *H = d.HDRY;
   const double g = d.g[2 * J + 1];
   if (T.live)
@@ -156,7 +164,7 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
       if (fcmp_d(bot, top) == 0) pp = .5 * (bot + top);
       else if (top < 0. || bot < 0.) { atomicOr(&d.dyn->err, 2); pp = 0.; }
       else pp = pow((pw[K] - pw[K - 1]) / (kp1 * (bot - top)), 1. / d.kappa);
-      p2[K] = pp;
+      if (K == 1) aux[2] = pp;
       d.P2[q] = pp;
       double h3;
       if (K < nk) {
@@ -187,14 +195,15 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
   __syncthreads();
   if (T.live && T.kk == 0) {
     const size_t q0 = ix3(d, 0, J, I);
-    d.P2[q0] = a[0] * a[0] / p2[1];
+    d.P2[q0] = a[0] * a[0] / aux[2];
     d.H3[q0] = aux[1];
     if (update_theta) d.THETA2[ix3(d, ks - 1, J, I)] = .5 * (d.THETA[ix3(d, ks - 1, J, I)] + aux[0]);
   }
 }
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
-  size_t shm = sizeof(double) * (3 * COLS * (b.nk + 1) + 2 * COLS);
-  LAUNCH(k_set_p2etc, tile_grid(b), dim3(TPB), shm, s, b.D, b.mb, update_theta);
+  const int C = tile_cols(b, 2);
+  size_t shm = sizeof(double) * (2 * C * (b.nk + 1) + 3 * C);
+  LAUNCH(k_set_p2etc, tile_grid(b, C), dim3(TPB), shm, s, b.D, b.mb, update_theta, C);
 }
 
 // ------------------------------------------------- store_pgrad_vars parts
@@ -416,12 +425,12 @@ void L_gz_from_u(hipStream_t s, const Bucket &b) {
 
 // hydrostatic GZ3 and MONT3 from the bottom up (Leslie-Purser 4-point
 // integral in ln p) and the monotonicity screen
-__global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb, int C) {
   MEMBER;
   extern __shared__ double lds[];
-  const Tile T = tile_of(d);
+  const Tile T = tile_of(d, C);
   const int nk = d.nk, W = nk + 1, J = T.J, I = T.I;
-  double *inc = lds + T.c * W, *tt = lds + COLS * W + T.c * W, *lg = lds + 2 * COLS * W + T.c * W;
+  double *inc = lds + T.c * W, *tt = lds + C * W + T.c * W, *lg = lds + 2 * C * W + T.c * W;
   const double *P = d.P3, *RH = d.RHO3;
 #define PP(k) P[ix3(d, k, J, I)]
 #define RR(k) RH[ix3(d, k, J, I)]
@@ -472,7 +481,8 @@ __global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb) {
   }
 }
 void L_gz3(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_gz3, tile_grid(b), dim3(TPB), sizeof(double) * 3 * COLS * (b.nk + 1), s, b.D, b.mb);
+  const int C = tile_cols(b, 3);
+  LAUNCH(k_gz3, tile_grid(b, C), dim3(TPB), sizeof(double) * 3 * C * (b.nk + 1), s, b.D, b.mb, C);
 }
 
 // finish DZDT3 (diagnostic vertical velocity)
@@ -574,13 +584,13 @@ void L_heating(hipStream_t s, const Bucket &b, int adjust) {
 // SUM_DIV in one lane per column.  SUM_DIV(1) and SUM_DIV(nk) are zero (the
 // reference's top/bottom statements write stale-index pad cells), so the
 // dF/dpbot term vanishes and is not formed.
-__global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int with_source) {
+__global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int with_source, int C) {
   MEMBER;
   extern __shared__ double lds[];
-  const Tile T = tile_of(d);
+  const Tile T = tile_of(d, C);
   const int nk = d.nk, ks = d.ks, W = nk + 1, J = T.J, I = T.I;
-  double *dsv = lds + T.c * W, *dfv = lds + COLS * W + T.c * W, *wbv = lds + 2 * COLS * W + T.c * W;
-  double *lg = lds + 3 * COLS * W + T.c * W;
+  double *dsv = lds + T.c * W, *dfv = lds + C * W + T.c * W, *wbv = lds + 2 * C * W + T.c * W;
+  double *lg = lds + 3 * C * W + T.c * W;
   const double *P = d.P3;
   double *TH = d.THETA;
   int bad = 0;
@@ -648,7 +658,8 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
   if (bad) atomicOr(&d.dyn->err, 2);
 }
 void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
-  LAUNCH(k_calc_w, tile_grid(b), dim3(TPB), sizeof(double) * 4 * COLS * (b.nk + 1), s, b.D, b.mb, with_source);
+  const int C = tile_cols(b, 4);
+  LAUNCH(k_calc_w, tile_grid(b, C), dim3(TPB), sizeof(double) * 4 * C * (b.nk + 1), s, b.D, b.mb, with_source, C);
 }
 
 // ------------------------------------------------ vorticity / divergence

@@ -16,6 +16,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "../../include/epic_amd.h"
 
@@ -131,11 +132,19 @@ int main(int argc, char **argv) {
   if (lazy) epic_amd_set_sync_mode(1);
   grid.itime = 0;
   timestep(&pl, EPIC_SYNC_DIAGS_ONLY, NULL);
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
   for (int s = 1; s <= nsteps; s++) {
     grid.itime = (unsigned long)(s - 1);
     timestep(&pl, EPIC_STEP_PROGS_AND_SYNC_DIAGS, NULL);
   }
   if (lazy) epic_amd_flush();
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  const double sec = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+  /* wall time of the stepping calls (and the final flush in lazy mode),
+   * host-to-device and back included */
+  fprintf(stderr, "timestep calls %d mean_ms %.4f pt_steps_per_s %.4e\n", nsteps, 1e3 * sec / nsteps,
+          (double)c.ni * (c.nj + 1) * c.nk * nsteps / sec);
 
   FILE *fo = fopen(out, "wb");
   if (!fo) return 4;
