@@ -55,6 +55,21 @@ KERNEL_BYTES = {
 }
 
 
+def measured_traffic(workload, precision, name):
+    """HBM bytes per launch of `name` from the committed counter summary of
+    this workload (profiles/traffic_<workload>_<precision>.json: FETCH_SIZE
+    and WRITE_SIZE per dispatch, each scaled by what the counter reads for a
+    plain copy of known size), or (None, None) if none is committed."""
+    fn = os.path.join(HERE, "profiles", f"traffic_{workload}_{precision}.json")
+    if not os.path.exists(fn):
+        return None, None
+    k = json.load(open(fn))["kernels"]
+    for cand in (name, name.replace("<T>", "<double>")):
+        if cand in k:
+            return k[cand]["traffic_bytes"], os.path.relpath(fn, HERE)
+    return None, None
+
+
 def filtered_rows(nj):
     """h-rows with |lat| >= 45 deg on EPIC's globe grid."""
     dlt = 180.0 / ((nj + 1) + 2 * math.sqrt(0.5))
@@ -149,6 +164,7 @@ def main():
     nbytes = kernel_bytes(name, ni, nj, nk, members)
     achieved = nbytes / avg_s
     step_bytes = 112 if prec == ea.FP64 else 88
+    traffic, traffic_src = measured_traffic(args.workload, args.precision, name)
 
     out = {
         "metric": "grid-pt-steps/s",
@@ -166,7 +182,7 @@ def main():
         "config": {"workload": args.workload, "ni": ni, "nj": nj, "nk": nk, "members": members, "dt": dt,
                    "precision": args.precision, "n_pts": npts},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic, "traffic_source": traffic_src,
                      "launch_us": avg_s * 1e6, "bytes_per_launch": nbytes,
                      "share_of_step": ms / tot if tot else None,
                      "top_kernels": [[n, round(m / 3 * 1e3, 1), c // 3] for n, m, c in ranked[:8]]},
