@@ -94,6 +94,17 @@ __device__ __forceinline__ double f_sig(const MemberDesc &d, double s) {
   if (s < 1.) return (1. - g_sig(d, s)) * (d.zeta0 + s * (d.zeta1 - d.zeta0));
   return 0.;
 }
+// (sigmatheta - f_sigma(s)) / g_sigma(s), the theta of a hybrid level at
+// sigma s, with g_sigma formed once for both terms (the same operations as
+// calling f_sig and g_sig, so the same result)
+__device__ __forceinline__ double theta_of_sigma(const MemberDesc &d, double sgth, double s) {
+  const bool mid = (s > d.sigma_sigma && s < 1.);
+  const double gm = mid ? d.g_coeff * (1. - exp(-d.alpha * (s - d.sigma_sigma))) : 0.;
+  const double lin = d.zeta0 + s * (d.zeta1 - d.zeta0);
+  const double g = (s <= d.sigma_sigma) ? 0. : (s >= 1. ? 1. : gm);
+  const double f = (s < 0.) ? d.zeta0 : (s <= d.sigma_sigma ? lin : (s < 1. ? (1. - gm) * lin : 0.));
+  return (sgth - f) / g;
+}
 // return_sigmatheta; sets *bad when p is outside [ptop, pbot]
 __device__ __forceinline__ double sgth_of(const MemberDesc &d, double theta, double p, double pbot, double ptop,
                                           int *bad) {

@@ -216,17 +216,16 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
     L_merid_hv(s, b, hth);
     L_zonal_hv(s, b, hth);
   }
-  L_restore_mass(s, b);
-  L_advection(s, b);
+  L_advection(s, b);  // restore_mass first, inside
   rec_p2etc_pgrad_vars(s, b, 0);
   // filtered, monotone copy of GZ3
   L_gz_filtercopy(s, b);
   L_gz_mono(s, b);
   // the four 3-D filters in one pass (each reads only its own field and the
-  // filtered GZ3 copy; restore_mass after them touches hdry only)
+  // filtered GZ3 copy); restore_mass, which touches hdry only and follows
+  // them, is applied to the filtered hdry as it is stored
   const FieldSet f4 = {4, {F_DUDT_T, F_DVDT_T, F_HDRY, F_THETA}, {0, 1, 2, 3}};
   L_filter3(s, b, f4);
-  L_restore_mass(s, b);
   rec_p2etc_pgrad_vars(s, b, 1);
   L_pgrad(s, b);
   if (divd) L_divdamp(s, b);

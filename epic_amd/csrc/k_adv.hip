@@ -393,7 +393,10 @@ static void hsu3(hipStream_t s, const Bucket &b) {
 
 // op 0: unity = 1 on K = 0..nk+1, J = 0..nj, I = 0..ni+1, and OLD = THETA
 // on K = k_sigma-1..nk+1 over the same rows; op 1: the theta correction
-// there
+// there.  Both also apply restore_mass (HDRY = max(h_min, HDRY) on K = 1..nk
+// over the same rows), which the reference runs right before advection
+// (src/shared/epic_timestep.c) and right after it (src/shared/epic_flux.c)
+// and which touches HDRY only.
 __global__ void __launch_bounds__(256) k_adv_elem(const MemberDesc *D, int mb, int op) {
   MEMBER;
   int t = blockIdx.x * TPB + threadIdx.x;
@@ -408,8 +411,11 @@ __global__ void __launch_bounds__(256) k_adv_elem(const MemberDesc *D, int mb, i
   } else if (K >= d.ks - 1) {
     d.THETA[q] -= d.old[q] * (d.unity[q] - 1.);
   }
+  if (K >= 1 && K <= d.nk) d.HDRY[q] = dmax(d.h_min[K], d.HDRY[q]);
 }
 
+// restore_mass, advection, restore_mass (the two restore_mass calls run
+// inside the element-wise launches around the advection passes)
 void L_advection(hipStream_t s, const Bucket &b) {
   int ncol = (b.nj + 1) * (b.ni + 2);
   dim3 ge((ncol + TPB - 1) / TPB, b.nk + 2, b.nm);
@@ -418,7 +424,6 @@ void L_advection(hipStream_t s, const Bucket &b) {
   LAUNCH(k_adv_elem, ge, dim3(TPB), 0, s, b.D, b.mb, 0);
   hsu3(s, b);
   LAUNCH(k_adv_elem, ge, dim3(TPB), 0, s, b.D, b.mb, 1);
-  L_restore_mass(s, b);
   // set_p2etc(DONT_UPDATE_THETA) follows, with store_pgrad_vars
 }
 

@@ -115,7 +115,8 @@ static inline dim3 tile_grid(const Bucket &b, int C) {
 // P3 from the top down (prefix of dsgth*g*h), the sigma-region pressures,
 // h = max(h_min, dp/(g dsgth)), and the layer diagnostics P2, H3, THETA2
 // (and the theta of the sigma region when update_theta)
-__global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, int update_theta, int C) {
+template <int update_theta>
+__global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, int C) {
   MEMBER;
   extern __shared__ double lds[];
   const Tile T = tile_of(d, C);
@@ -153,7 +154,7 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
       P3[q] = a[K];
       if (update_theta && K < ks - 1) {
         double sigma = get_sigma_l(pbot, a[K], lr);
-        double th = ((double)d.sgth[2 * K + 1] - f_sig(d, sigma)) / g_sig(d, sigma);
+        double th = theta_of_sigma(d, d.sgth[2 * K + 1], sigma);
         d.THETA[q] = th;
         if (K == ks - 2) aux[0] = th;
       }
@@ -180,7 +181,7 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
       if (update_theta) {
         if (K < ks - 1) {
           double sigma = get_sigma_l(pbot, pp, lr);
-          d.THETA2[q] = ((double)d.sgth[2 * K] - f_sig(d, sigma)) / g_sig(d, sigma);
+          d.THETA2[q] = theta_of_sigma(d, d.sgth[2 * K], sigma);
         } else if (K >= ks) {
           // linear interpolation of theta against sigmatheta, table ordered
           // from the bottom interface up (src/shared/epic_funcs_diag.c:3522-3562)
@@ -203,7 +204,8 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
   const int C = tile_cols(b, 2);
   size_t shm = sizeof(double) * (2 * C * (b.nk + 1) + 3 * C);
-  LAUNCH(k_set_p2etc, tile_grid(b, C), dim3(TPB), shm, s, b.D, b.mb, update_theta, C);
+  if (update_theta) LAUNCH(k_set_p2etc<1>, tile_grid(b, C), dim3(TPB), shm, s, b.D, b.mb, C);
+  else LAUNCH(k_set_p2etc<0>, tile_grid(b, C), dim3(TPB), shm, s, b.D, b.mb, C);
 }
 
 // ------------------------------------------------- store_pgrad_vars parts

@@ -7,7 +7,10 @@
 //   divergence_damping      src/shared/epic_subgrid.c:1792-1885
 //   adams_bashforth_step    src/shared/epic_timestep.c:406-498
 //   zonal/meridional hyperviscosity, d2dy2  src/shared/epic_subgrid.c:590-1232
-//   restore_mass            src/shared/epic_adjust.c:45-96
+//
+// (restore_mass, src/shared/epic_adjust.c:45-96, runs inside the launches
+// next to each of its calls: k_adv_elem in k_adv.hip, k_filter3 in
+// k_filter.hip.)
 //
 // The reference carries the finite-volume faces west->east and
 // south->north; here every lane recomputes the face shared with its
@@ -555,17 +558,6 @@ void L_gz_filtercopy(hipStream_t s, const Bucket &b) {
 
 void L_gz_mono(hipStream_t s, const Bucket &b) {
   LAUNCH(k_gz_mono, grid2(b.nj + 1, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
-}
-
-__global__ void __launch_bounds__(256) k_restore_mass(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(0, d.nj + 1, 0, d.ni + 2);
-  const int K = 1 + blockIdx.y;
-  size_t q = ix3(d, K, J, I);
-  d.HDRY[q] = dmax(d.h_min[K], d.HDRY[q]);
-}
-void L_restore_mass(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_restore_mass, grid2(b.nj + 1, b.ni + 2, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
 // ------------------------------------------------------ hyperviscosity

@@ -249,7 +249,8 @@ __global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb
 // filter band are transformed, gain-multiplied and transformed back in LDS,
 // the profile is added back and the zonal pads are written.  Rows outside the
 // band come out as (a - v) + v, as in the reference, which subtracts and adds
-// the profile on every row.  RB rows per block, ni/4 lanes per row.
+// the profile on every row.  HDRY is stored with the restore_mass that
+// follows the filters in the step.  RB rows per block, ni/4 lanes per row.
 template <class T>
 __global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
@@ -333,9 +334,13 @@ __global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, in
   four1_stages(buf, n, lane, ti, filt);
   if (filt)
     for (int q = 0, e = lane; q < 4; q++, e += nq) x[q] = buf[e] * T(2. / (n));
+  // hdry leaves with restore_mass applied (max(h_min, h), every cell of
+  // K = 1..nk, J = 0..nj, I = 0..ni+1 is written here)
+  const double hmin = (kind == 2) ? d.h_min[K] : 0.;
   if (live)
     for (int q = 0, e = lane; q < 4; q++, e += nq) {
-      const T out = T((double)x[q] + v[q]);
+      T out = T((double)x[q] + v[q]);
+      if (kind == 2) out = T(dmax(hmin, (double)out));
       a[ix2(d, J, e + 1)] = out;
       if (e == n - 1) a[ix2(d, J, 0)] = out;
       if (e == 0) a[ix2(d, J, n + 1)] = out;

@@ -71,7 +71,6 @@ void L_rotate(hipStream_t s, const Bucket &b);
 void L_zero_tend(hipStream_t s, const Bucket &b);
 void L_gz_mono(hipStream_t s, const Bucket &b);
 void L_gz_filtercopy(hipStream_t s, const Bucket &b);
-void L_restore_mass(hipStream_t s, const Bucket &b);
 // hyperviscosity on every field of the set (at most two: the meridional
 // part keeps its del^4 result in scratch copy i of sB)
 void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs);
