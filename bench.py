@@ -33,7 +33,7 @@ WORKLOADS = {
     "jupiter_256x128x20": (1, 256, 128, 20, 60, 1),
     "held_suarez_64x32x10": (0, 64, 32, 10, 600, 1),
     "hs_ensemble_64x128x64x16": (0, 128, 64, 16, 300, 64),
-    "jupiter_1024x512x64": (1, 1024, 512, 64, 30, 1),
+    "hot_jupiter_1024x512x64": (4, 1024, 512, 64, 30, 1),
 }
 
 # Algorithmic bytes per cell of each kernel: the distinct arrays it must
@@ -117,7 +117,12 @@ def copy_probe(nbytes=256 << 20, reps=20):
 
 
 def spec_of(ea, planet, ni, nj, nk, dt, seed):
-    gas = planet in (1, 2)
+    """The seeded case of a workload: jets plus 0.5 m/s u, v noise on the
+    Jupiter/Saturn-like planets, 1 K theta noise on the hot Jupiter (at rest,
+    day-night forcing), 0.1 K on Held-Suarez."""
+    if planet == ea.HOT_JUPITER:
+        return ea.spec(planet, ni, nj, nk, dt, seed=seed, nu4=0.5, noise_theta=1.0, noise_uv=0.0)
+    gas = planet in (ea.JUPITER_LIKE, ea.SATURN_LIKE)
     return ea.spec(planet, ni, nj, nk, dt, seed=seed, nu4=0.5,
                    noise_theta=0.0 if gas else 0.1, noise_uv=0.5 if gas else 0.0)
 

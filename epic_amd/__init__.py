@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libepic_amd.so")
 
 # planets (enum epic_planet_kind)
-HELD_SUAREZ, JUPITER_LIKE, SATURN_LIKE, VENUS_LIKE = 0, 1, 2, 3
+HELD_SUAREZ, JUPITER_LIKE, SATURN_LIKE, VENUS_LIKE, HOT_JUPITER = 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
 
 FIELDS = ["U", "V", "HDRY", "THETA", "P3", "P2", "H3", "THETA2", "T2", "T3", "RHO2", "RHO3",

@@ -34,6 +34,9 @@ struct MemberDesc {
   double dt, kappa, cp, cpr, rgas, press0, zeta0, zeta1, alpha, sigma_sigma;
   double topo_scale, prandtl, tau_drag;
   double coef2, coef4, nu4, m0z, nudiv, eps13, mu3, mu2, g_coeff;
+  // day-night relaxation of the hot-Jupiter case (is_hj)
+  int is_hj;
+  double pbot, teq_contrast, teq_p_contrast, tau_rad, ref_theta_b, ref_pbk, ref_a_over_kappa;
   // 1-D metrics and tables
   const double *lat, *lon, *m, *n, *mn, *f, *g, *rln, *rlt, *sgth, *dsgth, *dsgth_inv, *h_min, *t_sponge;
   const double *mn_inv, *hl_h, *hl_pv, *hcx, *hcy, *hcz, *vcx, *vcy, *vcz;

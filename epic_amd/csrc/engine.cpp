@@ -446,6 +446,15 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   d.ks = c->k_sigma; d.k_sponge = c->k_sponge;
   d.is_hs = (strcmp(c->planet_name, "held_suarez") == 0);
   d.is_gas = (strcmp(c->planet_type, "gas-giant") == 0);
+  d.is_hj = (strcmp(c->planet_name, "hot_jupiter") == 0);
+  d.pbot = c->pbot;
+  d.teq_contrast = c->teq_contrast;
+  d.teq_p_contrast = c->teq_p_contrast;
+  d.tau_rad = c->tau_rad;
+  d.ref_theta_b = c->ref_theta_b;
+  d.ref_a_over_kappa = c->ref_a_over_kappa;
+  // (ref_pb/press0)^kappa, with the host libm like the oracle's
+  d.ref_pbk = d.is_hj ? pow(c->ref_pb / c->press0, c->kappa) : 0.;
   d.newt_cool_on = c->newt_cool_on; d.newt_cool_adjust = c->newt_cool_adjust;
   d.drag_v = c->drag_v; d.drag_zonal_avg = c->drag_zonal_avg; d.jtp = c->jtp;
   d.nrow_h = T.nrow_h; d.nrow_v = T.nrow_v;

@@ -322,6 +322,22 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
       theta_b = 400.;
       c->kinvisc = 14.60e-6;
       break;
+    case EPIC_PLANET_HOT_JUPITER:
+      strcpy(c->planet_name, "hot_jupiter");
+      c->planet_index = 20;
+      c->re = c->rp = 9.44e7;
+      c->omega = 2.06e-5;
+      c->cp = 14300.;
+      c->rgas = 3523.;
+      gsurf = 9.42;
+      ptop = 100.;
+      pbot = 1.e6;
+      n2 = 1.5e-6;
+      theta_b = 1500.;
+      gas = 1;
+      c->newt_cool_on = 1;
+      c->kinvisc = 101.00e-6;
+      break;
     default:
       return -1;
   }
@@ -394,6 +410,14 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
     pr.theta_b = theta_b;
     pr.a_over_kappa = n2 * c->rgas / (gsurf * gsurf) / c->kappa;
   }
+  if (s->planet == EPIC_PLANET_HOT_JUPITER) {
+    c->teq_contrast = 1000.;
+    c->teq_p_contrast = 1.e5;
+    c->tau_rad = 1.e5;
+    c->ref_theta_b = pr.theta_b;
+    c->ref_pb = pr.pb;
+    c->ref_a_over_kappa = pr.a_over_kappa;
+  }
   /* theta_ref(p): cos-weighted meridional mean of the profile */
   double *pref = c->p_ref, *tref = c->theta_ref;
   pref[1] = ptop;
@@ -418,6 +442,10 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
    * knee at the top of layer k_sigma, as the generic branch of `initial`
    * does, src/shared/epic_initial.c:1695-1740), sigmatheta[kk] ---- */
   int k_sigma = nk - nk / 3;
+  /* The hot Jupiter's day-night heating would move whole isentropic layers
+   * across one another within ~100 steps at nk = 64; its levels are sigma
+   * levels below the top layer, where heating changes theta in place. */
+  if (s->planet == EPIC_PLANET_HOT_JUPITER) k_sigma = 2;
   if (k_sigma < 2) k_sigma = 2;
   c->k_sigma = k_sigma;
   hyb h;
@@ -490,7 +518,7 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   }
   free(col);
   /* zonal jets of the gas giants: u = sum_n A_n cos(n pi lat / 90), A_n ~ U(-60,60) */
-  if (gas) {
+  if (gas && s->planet != EPIC_PLANET_HOT_JUPITER) {
     double A[9];
     for (int q = 1; q <= 8; q++) A[q] = -60. + 120. * uniform01(s->seed, 7, (uint64_t)q);
     for (int J = 0; J <= nj; J++) {

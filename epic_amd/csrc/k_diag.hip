@@ -539,14 +539,30 @@ __device__ double hs_newton(const MemberDesc &d, int K, int J, int I) {
   return -cpt * (t - teq);
 }
 
-// HEAT3 = 0 everywhere, then the Held-Suarez relaxation on K = 1..nk with
-// its zonal pads (the bc_lateral at the end of calc_heating)
+// Day-night relaxation of the hot-Jupiter case (no reference counterpart):
+// HEAT3 = cp (T_eq - T)/tau_rad, T_eq = T_ref(p) + dT(p) (max(0, cos(lat)
+// cos(lon)) - 1/4), T_ref the initial profile's temperature, dT the full
+// contrast above teq_p_contrast and linear in ln p to 0 at pbot.
+__device__ double hj_newton(const MemberDesc &d, int K, int J, int I) {
+  const double p = d.P3[ix3(d, K, J, I)], t = d.T3[ix3(d, K, J, I)];
+  const double pk = pow(p / d.press0, d.kappa);
+  const double tref = (1. / (1. / d.ref_theta_b + d.ref_a_over_kappa * (pk - d.ref_pbk))) * pk;
+  double dT = d.teq_contrast;
+  if (p > d.teq_p_contrast) dT *= dmax(0., log(d.pbot / p) / log(d.pbot / d.teq_p_contrast));
+  const double cl = cos(d.lat[2 * J + 1] * (M_PI / 180.));
+  const double day = dmax(0., cl * cos(d.lon[2 * I + 1] * (M_PI / 180.)));
+  const double teq = tref + dT * (day - .25);
+  return d.cp * (teq - t) / d.tau_rad;
+}
+
+// HEAT3 = 0 everywhere, then the Held-Suarez (or day-night) relaxation on
+// K = 1..nk with its zonal pads (the bc_lateral at the end of calc_heating)
 __global__ void __launch_bounds__(256) k_heating(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
-  if (!(d.newt_cool_on && d.is_hs)) return;
-  double h = 0. + hs_newton(d, K, J, I);
+  if (!d.newt_cool_on || !(d.is_hs || d.is_hj)) return;
+  double h = d.is_hj ? hj_newton(d, K, J, I) : 0. + hs_newton(d, K, J, I);
   d.HEAT3[ix3(d, K, J, I)] = h;
   if (I == d.ni) d.HEAT3[ix3(d, K, J, 0)] = h;
   if (I == 1) d.HEAT3[ix3(d, K, J, d.ni + 1)] = h;

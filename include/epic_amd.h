@@ -191,7 +191,9 @@ enum epic_planet_kind {
   EPIC_PLANET_HELD_SUAREZ = 0, /* dry Earth, Held & Suarez (1994) forcing   */
   EPIC_PLANET_JUPITER_LIKE = 1,/* ideal-gas gas giant, jets, no forcing      */
   EPIC_PLANET_SATURN_LIKE = 2, /* ideal-gas gas giant, jets, no forcing      */
-  EPIC_PLANET_VENUS_LIKE = 3   /* slow rotator, Held-Suarez-type relaxation  */
+  EPIC_PLANET_VENUS_LIKE = 3,  /* slow rotator, Held-Suarez-type relaxation  */
+  EPIC_PLANET_HOT_JUPITER = 4  /* tidally locked gas giant: Newtonian
+                                  relaxation to a day-night T_eq            */
 };
 
 typedef struct {
@@ -218,6 +220,13 @@ typedef struct {
   double press0, pbot, ptop, zeta0, zeta1, hybrid_alpha, sigma_sigma;
   double topo_scale, prandtl, tau_drag, nudiv_nondim;
   double nu[EPIC_MAX_NU_ORDER + 1], nu_nondim[EPIC_MAX_NU_ORDER + 1];
+  /* day-night relaxation (EPIC_PLANET_HOT_JUPITER; unused otherwise):
+   * HEAT3 = cp (T_eq - T)/tau_rad with T_eq = T_ref(p) + dT(p) (max(0,
+   * cos(lat) cos(lon)) - 1/4), T_ref the initial profile's temperature
+   * 1/theta = 1/ref_theta_b + ref_a_over_kappa ((p/press0)^kappa -
+   * (ref_pb/press0)^kappa) times (p/press0)^kappa, dT = teq_contrast for
+   * p <= teq_p_contrast, falling linearly in ln p to 0 at pbot. */
+  double teq_contrast, teq_p_contrast, tau_rad, ref_theta_b, ref_pb, ref_a_over_kappa;
   /* padded extents */
   int    Iadim, Jadim, Kadim, Nelem2d, Nelem3d;
   /* 1-D arrays (length 2*(ni+1)+1 for lon, 2*(nj+1)+1 for the J-arrays,

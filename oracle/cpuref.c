@@ -39,7 +39,7 @@ struct cpuref {
   epic_case c;
   int ni, nj, nk, Ia, N2, N3, k_sigma, k_sponge;
   double dt;
-  int is_hs, is_gas;
+  int is_hs, is_gas, is_hj;
   /* fields */
   double *U, *V, *DUDT[3], *DVDT[3], *HDRY, *THETA, *P3, *P2, *THETA2, *H3;
   double *T2, *T3, *RHO2, *RHO3, *EXNER3, *GZ3, *MONT3, *HEAT3, *PV3, *DIV, *W3, *DZDT3, *RI2;
@@ -794,6 +794,27 @@ static void calc_heating(cpuref *R) {
       for (int J = 0; J <= R->nj; J++)
         for (int I = 1; I <= R->ni; I++) A3(R->HEAT3, K, J, I) += A2(buf, J, I);
     }
+  }
+  if (c->newt_cool_on && R->is_hj) {
+    /* Day-night Newtonian relaxation of the hot-Jupiter case (no reference
+     * counterpart: its temp_eq has no such branch,
+     * src/shared/epic_sensible_heating.c:394-460), on the interfaces
+     * K = 1..nk like the Held-Suarez relaxation: HEAT3 = cp (T_eq - T)/tau. */
+    const double pbk = pow(c->ref_pb / c->press0, c->kappa);
+    for (int K = 1; K <= R->nk; K++)
+      for (int J = 0; J <= R->nj; J++) {
+        const double cl = cos(c->lat[2 * J + 1] * DEGR);
+        for (int I = 1; I <= R->ni; I++) {
+          const double p = A3(R->P3, K, J, I), t = A3(R->T3, K, J, I);
+          const double pk = pow(p / c->press0, c->kappa);
+          const double tref = (1. / (1. / c->ref_theta_b + c->ref_a_over_kappa * (pk - pbk))) * pk;
+          double dT = c->teq_contrast;
+          if (p > c->teq_p_contrast) dT *= mx(0., log(c->pbot / p) / log(c->pbot / c->teq_p_contrast));
+          const double day = mx(0., cl * cos(c->lon[2 * I + 1] * DEGR));
+          const double teq = tref + dT * (day - .25);
+          A3(R->HEAT3, K, J, I) = c->cp * (teq - t) / c->tau_rad;
+        }
+      }
   }
   bc3(R, R->HEAT3);
 }
@@ -1711,6 +1732,7 @@ cpuref *cpuref_create(const epic_case_spec *spec) {
   R->k_sigma = c->k_sigma; R->k_sponge = c->k_sponge;
   R->dt = (double)c->dt;
   R->is_hs = (strcmp(c->planet_name, "held_suarez") == 0);
+  R->is_hj = (strcmp(c->planet_name, "hot_jupiter") == 0);
   R->is_gas = (strcmp(c->planet_type, "gas-giant") == 0);
   size_t N3 = R->N3, N2 = R->N2;
 #define ALLOC3(p) p = (double *)calloc(N3, sizeof(double))

@@ -23,6 +23,7 @@ CASES = {
     "jupiter_16x8x6": ea.spec(ea.JUPITER_LIKE, 16, 8, 6, 60, 5, 0.5, 0.0, 0.0, 0.5),
     "saturn_64x24x12": ea.spec(ea.SATURN_LIKE, 64, 24, 12, 60, 13, 0.4, 0.1, 0.0, 0.5),
     "venus_32x16x5": ea.spec(ea.VENUS_LIKE, 32, 16, 5, 600, 14, 0.5, 0.0, 0.2, 0.2),
+    "hot_jupiter_32x16x8": ea.spec(ea.HOT_JUPITER, 32, 16, 8, 30, 3, 0.5, 0.0, 1.0, 0.0),
     "hs_128x64x16_divdamp": ea.spec(ea.HELD_SUAREZ, 128, 64, 16, 300, 3, 0.5, 0.1, 0.3, 0.2),
 }
 

@@ -108,6 +108,35 @@ def test_richardson_of_given_fields_reproduces_store_diag():
     assert not same_bits(fn.richardson_of(f), f["RI2"])
 
 
+def test_hot_jupiter_relaxation_is_the_stated_closed_form():
+    """The day-night relaxation has no reference counterpart, so it is pinned
+    to its definition, evaluated here independently (lat/lon of the h-points
+    from the grid spacing, the reference temperature through exp/log):
+    HEAT3 = cp (T_eq - T)/tau, T_eq = T_ref(p) + dT(p)(max(0, cos lat cos lon)
+    - 1/4), dT = 1000 K for p <= 1 bar, linear in ln p to 0 at 10 bar."""
+    ni, nj, nk = 32, 16, 8
+    r = cr.CpuRef(ea.spec(ea.HOT_JUPITER, ni, nj, nk, 30, 3, 0.5, 0.0, 1.0, 0.0))
+    r.sync()
+    heat, t3, p3 = r.read("HEAT3"), r.read("T3"), r.read("P3")
+    cp, rgas, g, n2, thb, p0, pb = 14300.0, 3523.0, 9.42, 1.5e-6, 1500.0, 1.0e5, 1.0e6
+    kap = rgas / cp
+    aok = n2 * rgas / g ** 2 / kap
+    dlt = 180.0 / ((nj + 1) + 2 * np.sqrt(0.5))
+    lat = np.radians(-90.0 + dlt * np.sqrt(0.5) + (np.arange(nj + 1) + 0.5) * dlt)
+    lon = np.radians(-180.0 + (np.arange(1, ni + 1) - 0.5) * 360.0 / ni)
+    p, t = p3[1:nk + 1, 1:nj + 2, 1:ni + 1], t3[1:nk + 1, 1:nj + 2, 1:ni + 1]
+    lnx = kap * np.log(p / p0)
+    tref = np.exp(lnx) / (1.0 / thb + aok * (np.exp(lnx) - np.exp(kap * np.log(pb / p0))))
+    dT = 1000.0 * np.clip(np.log(pb / p) / np.log(pb / 1.0e5), 0.0, 1.0)
+    day = np.maximum(0.0, np.cos(lat)[None, :, None] * np.cos(lon)[None, None, :])
+    want = cp * (tref + dT * (day - 0.25) - t) / 1.0e5
+    got = heat[1:nk + 1, 1:nj + 2, 1:ni + 1]
+    assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()
+    # the pads are the zonal periodic copies, the top interface is not heated
+    assert np.array_equal(heat[:, :, 0], heat[:, :, ni]) and np.array_equal(heat[:, :, ni + 1], heat[:, :, 1])
+    assert not heat[0].any()
+
+
 # ---------------------------------------------------- numerical pins
 def test_realft_is_the_real_dft_in_nr_packing():
     L = cr.lib()
