@@ -138,6 +138,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", type=int, default=16,
                     help="requests of the serving mix to run after the step benchmark (0: skip)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch shapes instead of timing the candidates first")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a plain 256 MiB device copy (the measured stream-copy peak)")
     args = ap.parse_args()
@@ -154,6 +156,9 @@ def main():
     for m in range(members):
         eng.add(spec_of(ea, planet, ni, nj, nk, dt, 1 + m))
     eng.step(args.warmup)
+    eng.sync()
+    tuned = None if args.no_autotune else eng.autotune(2).strip().splitlines()
+    eng.step(1)  # captures the step's graph with the chosen shapes
     eng.sync()
     t0 = time.perf_counter()
     eng.step(args.steps)
@@ -186,7 +191,7 @@ def main():
         "dtype": "f64" if prec == ea.FP64 else "f32+f64",
         "data": "synthetic (seeded initial state of the workload's planet)",
         "config": {"workload": args.workload, "ni": ni, "nj": nj, "nk": nk, "members": members, "dt": dt,
-                   "precision": args.precision, "n_pts": npts},
+                   "precision": args.precision, "n_pts": npts, "launch_shapes": tuned},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic, "traffic_source": traffic_src,
                      "launch_us": avg_s * 1e6, "bytes_per_launch": nbytes,

@@ -72,6 +72,7 @@ def lib():
         L.epic_engine_last_error.restype = ctypes.c_char_p
         L.epic_engine_time_planes.argtypes = [vp, ci, ctypes.POINTER(ci)]
         L.epic_engine_profile.argtypes = [vp, ci, vp, ctypes.POINTER(cd), ctypes.POINTER(ci), ci]
+        L.epic_engine_autotune.argtypes = [vp, ci, vp, ci]
         L.epic_copy_probe.argtypes = [ctypes.c_size_t, ci, ctypes.POINTER(cd)]
         L.epic_member_footprint.argtypes = [ci, ci, ci, ci]
         L.epic_member_footprint.restype = ctypes.c_size_t
@@ -172,6 +173,13 @@ class Engine:
             nm = raw[48 * i:48 * (i + 1)].split(b"\0", 1)[0].decode()
             out.append((nm, ms[i], cnt[i]))
         return out
+
+    def autotune(self, steps=2):
+        """Pick the launch shapes of the tunable kernels for every bucket
+        (state and results unchanged); returns the choices as text."""
+        buf = ctypes.create_string_buffer(4096)
+        self._check(lib().epic_engine_autotune(self._e, steps, buf, len(buf)), "autotune")
+        return buf.value.decode()
 
     def close(self):
         if self._e:

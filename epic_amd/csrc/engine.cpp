@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -88,6 +89,8 @@ struct epic_engine {
   MemberDesc *dD;       // device descriptor array (bucket order)
   std::vector<int> order;  // bucket order -> slot
   std::vector<Bucket> buckets;
+  // launch shapes chosen by epic_engine_autotune, by (ni, nj, nk, members)
+  std::map<std::array<int, 4>, std::array<int, KN_COUNT>> tuned;
   bool dirty;
   hipStream_t stream;
   hipGraph_t graph;
@@ -170,6 +173,8 @@ static int rebuild_buckets(epic_engine *E) {
     b.nj = hd[i].nj;
     b.nk = hd[i].nk;
     b.mixed = (E->precision == EPIC_MIXED);
+    auto t = E->tuned.find({b.ni, b.nj, b.nk, b.nm});
+    for (int k = 0; k < KN_COUNT; k++) b.knob[k] = (t == E->tuned.end()) ? 0 : t->second[k];
     E->buckets.push_back(b);
     i = j;
   }
@@ -743,6 +748,116 @@ extern "C" int epic_engine_profile(epic_engine *E, int nsteps, char (*names)[48]
     k++;
   }
   return k;
+}
+
+// --------------------------------------------------------------- autotune
+// Picks, per bucket, the launch shape of each tunable kernel (columns per
+// column tile, rows per zonal-hyperviscosity block) by timing every
+// candidate over a few steps of the bucket's own members.  The members'
+// blobs are copied aside first and put back after every candidate and at
+// the end, so tuning leaves the state (and the step counts) as it found
+// them; the shapes never change a result.  The graph is re-captured with
+// the chosen shapes at the next step.
+static const char *const knob_name[KN_COUNT] = {"cols_p2etc", "cols_gz3", "cols_calcw", "cols_hsuv", "rows_zhv"};
+
+static bool knob_fits(const Bucket &b, int k, int v) {
+  const size_t W = (size_t)b.nk + 1, L = (size_t)b.nk + 2, lim = 128 * 1024;
+  switch (k) {
+    case KN_COLS_P2ETC: return sizeof(double) * (2 * v * W + 3 * v) <= lim;
+    case KN_COLS_GZ3: return sizeof(double) * 3 * v * W <= lim;
+    case KN_COLS_CALCW: return sizeof(double) * 4 * v * W <= lim;
+    case KN_COLS_HSUV: return sizeof(double) * 4 * v * L <= lim;
+    default: return sizeof(double) * (2 * (size_t)v * ((b.ni + 2) | 1) + v) <= lim;
+  }
+}
+
+static bool knob_times(const char *kernel, int k) {
+  static const char *const pfx[KN_COUNT] = {"k_set_p2etc", "k_gz3", "k_calc_w", "k_hsu_vert", "k_zonal_hv"};
+  return strncmp(kernel, pfx[k], strlen(pfx[k])) == 0;
+}
+
+extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int cap) {
+  if (!E) return -1;
+  if (E->dirty && rebuild_buckets(E)) return -5;
+  if (steps < 1) steps = 1;
+  if (report && cap > 0) report[0] = 0;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  // put the state aside
+  std::vector<void *> snap(E->order.size(), nullptr);
+  std::vector<long> done(E->order.size());
+  auto release = [&]() {
+    for (void *p : snap)
+      if (p) (void)hipFree(p);
+  };
+  for (size_t i = 0; i < E->order.size(); i++) {
+    Member &M = E->m[E->order[i]];
+    if (hipMalloc(&snap[i], M.bytes) != hipSuccess) {
+      release();
+      snprintf(E->err, sizeof(E->err), "autotune: no memory for the state copy (%zu B)", M.bytes);
+      return -2;
+    }
+    HIPCHK(hipMemcpyAsync(snap[i], M.blob, M.bytes, hipMemcpyDeviceToDevice, E->stream));
+    done[i] = M.steps;
+  }
+  auto restore = [&]() {
+    for (size_t i = 0; i < E->order.size(); i++) {
+      Member &M = E->m[E->order[i]];
+      (void)hipMemcpyAsync(M.blob, snap[i], M.bytes, hipMemcpyDeviceToDevice, E->stream);
+      M.steps = done[i];
+    }
+  };
+  static const int cand[KN_COUNT][3] = {{16, 32, 64}, {16, 32, 64}, {16, 32, 64}, {16, 32, 64}, {4, 8, 16}};
+  size_t used = 0;
+  for (Bucket &b : E->buckets) {
+    bool hv, divd, adj;
+    bucket_flags(E, b, hv, divd, adj);
+    for (int k = 0; k < KN_COUNT; k++) {
+      double best = 1e300;
+      int bestv = b.knob[k];
+      for (int v : cand[k]) {
+        if (!knob_fits(b, k, v)) continue;
+        b.knob[k] = v;
+        restore();
+        rec_step(E->stream, b, hv, divd, adj);  // warm-up, not timed
+        HIPCHK(hipStreamSynchronize(E->stream));
+        g_prof.clear();
+        g_evnext = 0;
+        g_prof_on = true;
+        for (int n = 0; n < steps; n++) rec_step(E->stream, b, hv, divd, adj);
+        g_prof_on = false;
+        HIPCHK(hipStreamSynchronize(E->stream));
+        double t = 0.;
+        for (auto &r : g_prof) {
+          if (!knob_times(r.name, k)) continue;
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, r.a, r.b);
+          t += ms;
+        }
+        if (t < best) {
+          best = t;
+          bestv = v;
+        }
+      }
+      b.knob[k] = bestv;
+      if (report && used < (size_t)cap) {
+        if (best < 1e300)
+          used += snprintf(report + used, cap - used, "%dx%dx%d x%d %s=%d (%.1f us/step)\n", b.ni, b.nj, b.nk,
+                           b.nm, knob_name[k], bestv, best / steps * 1e3);
+        else
+          used += snprintf(report + used, cap - used, "%dx%dx%d x%d %s=built-in (no candidate fits)\n", b.ni,
+                           b.nj, b.nk, b.nm, knob_name[k]);
+      }
+    }
+    std::array<int, KN_COUNT> kv;
+    for (int k = 0; k < KN_COUNT; k++) kv[k] = b.knob[k];
+    E->tuned[{b.ni, b.nj, b.nk, b.nm}] = kv;
+  }
+  restore();
+  HIPCHK(hipStreamSynchronize(E->stream));
+  release();
+  HIPCHK(hipGetLastError());
+  drop_graph(E);
+  return 0;
 }
 
 // --------------------------------------------------------- copy probe

@@ -376,6 +376,7 @@ static void hsu3(hipStream_t s, const Bucket &b) {
   // columns per block: as many as keep the four LDS columns within 40 KB
   int C = 64;
   while (C > 16 && (size_t)4 * C * (b.nk + 2) * sizeof(double) > 40 * 1024) C /= 2;
+  if (b.knob[KN_COLS_HSUV]) C = b.knob[KN_COLS_HSUV];
   LAUNCH(k_hsu_vert, dim3((ncol + C - 1) / C, 1, nz), dim3(256), sizeof(double) * 4 * C * (b.nk + 2), s, b.D, b.mb,
          b.nm, C);
   dim3 g(((b.nj + 1) * b.ni + TPB - 1) / TPB, b.nk, nz);

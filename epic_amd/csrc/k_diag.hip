@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
   }
 }
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
-  const int C = tile_cols(b, 2);
+  const int C = b.knob[KN_COLS_P2ETC] ? b.knob[KN_COLS_P2ETC] : tile_cols(b, 2);
   size_t shm = sizeof(double) * (2 * C * (b.nk + 1) + 3 * C);
   if (update_theta) LAUNCH(k_set_p2etc<1>, tile_grid(b, C), dim3(TPB), shm, s, b.D, b.mb, C);
   else LAUNCH(k_set_p2etc<0>, tile_grid(b, C), dim3(TPB), shm, s, b.D, b.mb, C);
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb, int C)
   }
 }
 void L_gz3(hipStream_t s, const Bucket &b) {
-  const int C = tile_cols(b, 3);
+  const int C = b.knob[KN_COLS_GZ3] ? b.knob[KN_COLS_GZ3] : tile_cols(b, 3);
   LAUNCH(k_gz3, tile_grid(b, C), dim3(TPB), sizeof(double) * 3 * C * (b.nk + 1), s, b.D, b.mb, C);
 }
 
@@ -676,7 +676,7 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
   if (bad) atomicOr(&d.dyn->err, 2);
 }
 void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
-  const int C = tile_cols(b, 4);
+  const int C = b.knob[KN_COLS_CALCW] ? b.knob[KN_COLS_CALCW] : tile_cols(b, 4);
   LAUNCH(k_calc_w, tile_grid(b, C), dim3(TPB), sizeof(double) * 4 * C * (b.nk + 1), s, b.D, b.mb, with_source, C);
 }
 

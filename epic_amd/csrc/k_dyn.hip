@@ -734,7 +734,10 @@ void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   // 16 (or 8) rows per block when that still gives every CU a few blocks
   // and two blocks fit in one CU's LDS, else 4
   const int rows = (b.nj + 2) * b.nk * b.nm * fs.n;
-  const int RB = (rows / 16 >= 1024 && b.ni <= 128) ? 16 : (rows / 8 >= 1024 && b.ni <= 256) ? 8 : 4;
+  const int RB = b.knob[KN_ROWS_ZHV]                          ? b.knob[KN_ROWS_ZHV]
+                 : (rows / 16 >= 1024 && b.ni <= 128) ? 16
+                 : (rows / 8 >= 1024 && b.ni <= 256)  ? 8
+                                                      : 4;
   const int WS = (b.ni + 2) | 1;
   dim3 g((b.nj + 2 + RB - 1) / RB, b.nk, fs.n * b.nm);
   LAUNCH(k_zonal_hv, g, dim3(256), sizeof(double) * (2 * RB * WS + RB), s, b.D, b.mb, b.nm, fs, RB);

@@ -28,11 +28,17 @@ struct FieldSet {
   int f[4], kind[4];
 };
 
+// Launch shapes a bucket can be tuned over (epic_engine_autotune); 0 takes
+// the built-in choice.  None of them changes a result: every cell is formed
+// by the same operations whatever the tile.
+enum Knob { KN_COLS_P2ETC, KN_COLS_GZ3, KN_COLS_CALCW, KN_COLS_HSUV, KN_ROWS_ZHV, KN_COUNT };
+
 struct Bucket {
   const MemberDesc *D;  // device array of descriptors
   int mb, nm;           // first member, member count
   int ni, nj, nk;
   int mixed;            // FP32 tendencies and horizontal arithmetic
+  int knob[KN_COUNT];   // columns per column tile / rows per zonal-mean block
 };
 
 

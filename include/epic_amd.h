@@ -298,6 +298,15 @@ int  epic_engine_member_error(epic_engine *e, int slot);
  * names/ms arrays (up to cap entries).  Returns the entry count. */
 int  epic_engine_profile(epic_engine *e, int nsteps, char (*names)[48], double *ms_total,
                          int *launches, int cap);
+/* Autotuner (no counterpart in the reference, whose loops have no launch
+ * shapes): for each bucket, time every candidate launch shape of the tunable
+ * kernels (columns per column tile of set_p2etc, gz3, calc_w and the
+ * vertical advection pass; rows per zonal-hyperviscosity block) over
+ * `steps` steps of the bucket's own members and keep the fastest.  The
+ * members' state is copied aside and put back, so the call changes no
+ * result and no state; it needs room for one copy of every member.  Writes
+ * one line per choice into report (up to cap bytes).  0 or a negative code. */
+int  epic_engine_autotune(epic_engine *e, int steps, char *report, int cap);
 const char *epic_engine_last_error(epic_engine *e);
 
 /* Measurement helper with no counterpart in the reference: time a plain

@@ -89,3 +89,29 @@ def test_rejects_unsupported_shapes():
     with pytest.raises(ea.EpicError):
         e.add(ea.spec(ea.HELD_SUAREZ, 64, 32, 80))
     e.close()
+
+
+def test_autotune_changes_no_state_and_no_result():
+    """Tuning the launch shapes leaves every member's state and step count
+    as it was, and the tuned step gives the same bits as the built-in shapes
+    (a 40-level member starts from the narrow column tiles)."""
+    say("autotune")
+    D = ea.spec(ea.JUPITER_LIKE, 64, 32, 40, 60, 4, 0.5, 0.0, 0.0, 0.5)
+    ref = {k: run_alone(sp, 6) for k, sp in (("A", A), ("B", B), ("D", D))}
+    e = ea.Engine(ea.FP64, 4)
+    slots = {"A": e.add(A), "B": e.add(B), "D": e.add(D)}
+    e.step(2)
+    before = {k: {f: e.read(s, f) for f in PROG} for k, s in slots.items()}
+    rep = e.autotune(1)
+    say(rep)
+    assert rep.count("\n") == 3 * 5  # three buckets, five shapes each
+    for k, s in slots.items():
+        assert e.steps_done(s) == 2
+        for f in PROG:
+            assert e.read(s, f).tobytes() == before[k][f].tobytes(), (k, f)
+    e.step(4)
+    for k, s in slots.items():
+        for f, a in ref[k].items():
+            assert np.isfinite(a).all(), (k, f)
+            assert e.read(s, f).tobytes() == a.tobytes(), (k, f)
+    e.close()
