@@ -56,6 +56,9 @@ def lib():
         L.epic_engine_destroy.argtypes = [vp]
         L.epic_engine_add_spec.argtypes = [vp, ctypes.POINTER(CaseSpec)]
         L.epic_engine_add_spec.restype = ci
+        L.epic_engine_checkpoint.argtypes = [vp, ci, ctypes.c_char_p]
+        L.epic_engine_restore_spec.argtypes = [vp, ctypes.POINTER(CaseSpec), ctypes.c_char_p]
+        L.epic_engine_restore_spec.restype = ci
         L.epic_engine_remove.argtypes = [vp, ci]
         L.epic_engine_step.argtypes = [vp, ci]
         L.epic_engine_step_members.argtypes = [vp, ctypes.POINTER(ci), ci, ci]
@@ -173,6 +176,18 @@ class Engine:
             nm = raw[48 * i:48 * (i + 1)].split(b"\0", 1)[0].decode()
             out.append((nm, ms[i], cnt[i]))
         return out
+
+    def checkpoint(self, slot, path):
+        """Write the member's state for an exact restart to `path`."""
+        self._check(lib().epic_engine_checkpoint(self._e, slot, str(path).encode()), "checkpoint")
+
+    def restore(self, sp, path):
+        """A new member from its case spec and a checkpoint of it; returns
+        the slot.  It continues bit for bit as the checkpointed member."""
+        slot = self._check(lib().epic_engine_restore_spec(self._e, ctypes.byref(sp), str(path).encode()),
+                           "restore")
+        self.specs[slot] = sp
+        return slot
 
     def autotune(self, steps=2):
         """Pick the launch shapes of the tunable kernels for every bucket

@@ -53,7 +53,7 @@ struct MemberDesc {
   float *DUDTf, *DVDTf;
   // sA..sD: scratch, three 3-D copies each (the batched advection gives each
   // advected field its own copy; every other user takes the first)
-  double *gz, *varz, *sA, *sB, *sC, *sD, *unity, *old, *spl;
+  double *gz, *sA, *sB, *sC, *sD, *unity, *old, *spl;
   double *rowbuf;  // 8*(nj+3) doubles: zonal means and meridional profiles
   MemberDyn *dyn;
 };
@@ -206,7 +206,7 @@ __device__ __forceinline__ double seq_acc(const double *a, int n, double s = 0.)
 // field selector: which 3-D array of the descriptor
 enum Fld : int {
   F_U, F_V, F_HDRY, F_THETA, F_P3, F_P2, F_THETA2, F_H3, F_GZ3, F_HEAT3, F_PV3, F_DIV, F_W3, F_DZDT3,
-  F_GZ, F_VARZ, F_SA, F_SB, F_SC, F_SD, F_UNITY, F_OLD, F_DUDT_T, F_DVDT_T
+  F_GZ, F_SA, F_SB, F_SC, F_SD, F_UNITY, F_OLD, F_DUDT_T, F_DVDT_T
 };
 
 __device__ __forceinline__ double *fld_ptr(const MemberDesc &d, int f) {
@@ -226,7 +226,6 @@ __device__ __forceinline__ double *fld_ptr(const MemberDesc &d, int f) {
     case F_W3: return d.W3;
     case F_DZDT3: return d.DZDT3;
     case F_GZ: return d.gz;
-    case F_VARZ: return d.varz;
     case F_SA: return d.sA;
     case F_SB: return d.sB;
     case F_SC: return d.sC;

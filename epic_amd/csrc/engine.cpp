@@ -368,7 +368,7 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   add(&d.USPIN, sizeof(double) * N3, c->u_spinup);
   add(&d.GZS, sizeof(double) * N2, c->gz_surface);
   double **fields[] = {&d.HDRY, &d.P2, &d.THETA2, &d.H3, &d.T2, &d.T3, &d.RHO2, &d.RHO3, &d.EXNER3, &d.GZ3,
-                       &d.MONT3, &d.HEAT3, &d.PV3, &d.DIV, &d.W3, &d.DZDT3, &d.RI2, &d.gz, &d.varz, &d.unity,
+                       &d.MONT3, &d.HEAT3, &d.PV3, &d.DIV, &d.W3, &d.DZDT3, &d.RI2, &d.gz, &d.unity,
                        &d.old};
   for (auto p : fields) add(p, sizeof(double) * N3, nul);
   // scratch: three copies each, one per field of the batched advection
@@ -750,6 +750,148 @@ extern "C" int epic_engine_profile(epic_engine *E, int nsteps, char (*names)[48]
   return k;
 }
 
+// ------------------------------------------------------ checkpoint/restart
+// A member's state for an exact restart: every array the next step reads
+// before writing it (the prognostic fields, the Adams-Bashforth history in
+// its own precision, the diagnostics of the last step: pressure,
+// thermodynamics, geopotential, PV, divergence, heating, W3, DZDT3,
+// GZ_SURFACE), and the time-plane state.  Scratch, the filtered copies and
+// RI2 are formed again by the step.  File layout (little-endian):
+//   "EPICAMD1", int32 {version 1, ni, nj, nk, mixed, it0, itm1, itm2, itime,
+//   err}, int64 steps, int32 array count, then per array int32 id, int64
+//   bytes and the raw device bytes.
+namespace {
+struct Ckpt {
+  int id;
+  void *dev;
+  size_t bytes;
+};
+std::vector<Ckpt> ckpt_arrays(const MemberDesc &d) {
+  const size_t N3 = (size_t)d.N3 * sizeof(double), N2 = (size_t)d.N2 * sizeof(double);
+  std::vector<Ckpt> a = {
+      {EPIC_F_U, d.U, N3},           {EPIC_F_V, d.V, N3},         {EPIC_F_HDRY, d.HDRY, N3},
+      {EPIC_F_THETA, d.THETA, N3},   {EPIC_F_P3, d.P3, N3},       {EPIC_F_P2, d.P2, N3},
+      {EPIC_F_H3, d.H3, N3},         {EPIC_F_THETA2, d.THETA2, N3}, {EPIC_F_T2, d.T2, N3},
+      {EPIC_F_T3, d.T3, N3},         {EPIC_F_RHO2, d.RHO2, N3},   {EPIC_F_RHO3, d.RHO3, N3},
+      {EPIC_F_EXNER3, d.EXNER3, N3}, {EPIC_F_GZ3, d.GZ3, N3},     {EPIC_F_MONT3, d.MONT3, N3},
+      {EPIC_F_PV3, d.PV3, N3},       {EPIC_F_DIV_UV3, d.DIV, N3}, {EPIC_F_W3, d.W3, N3},
+      {EPIC_F_HEAT3, d.HEAT3, N3},   {EPIC_F_DZDT3, d.DZDT3, N3}, {EPIC_F_GZ_SURFACE, d.GZS, N2},
+      {100, d.USPIN, N3},
+  };
+  if (d.mixed) {
+    a.push_back({101, d.DUDT, N3});
+    a.push_back({102, d.DVDT, N3});
+    a.push_back({103, d.DUDTf, 3 * (size_t)d.N3 * sizeof(float)});
+    a.push_back({104, d.DVDTf, 3 * (size_t)d.N3 * sizeof(float)});
+  } else {
+    a.push_back({101, d.DUDT, 3 * N3});
+    a.push_back({102, d.DVDT, 3 * N3});
+  }
+  return a;
+}
+const char CKPT_MAGIC[8] = {'E', 'P', 'I', 'C', 'A', 'M', 'D', '1'};
+}  // namespace
+
+extern "C" int epic_engine_checkpoint(epic_engine *E, int slot, const char *path) {
+  if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active || !path) return -1;
+  const MemberDesc &d = E->m[slot].h;
+  MemberDyn dy;
+  HIPCHK(hipStreamSynchronize(E->stream));
+  HIPCHK(hipMemcpy(&dy, d.dyn, sizeof(dy), hipMemcpyDeviceToHost));
+  FILE *f = fopen(path, "wb");
+  if (!f) {
+    snprintf(E->err, sizeof(E->err), "checkpoint: cannot open %s", path);
+    return -3;
+  }
+  auto arr = ckpt_arrays(d);
+  int hdr[10] = {1, d.ni, d.nj, d.nk, d.mixed, dy.it0, dy.itm1, dy.itm2, dy.itime, dy.err};
+  long long steps = E->m[slot].steps;
+  int n = (int)arr.size();
+  bool ok = fwrite(CKPT_MAGIC, 1, 8, f) == 8 && fwrite(hdr, sizeof(int), 10, f) == 10 &&
+            fwrite(&steps, sizeof(steps), 1, f) == 1 && fwrite(&n, sizeof(int), 1, f) == 1;
+  std::vector<char> buf;
+  for (auto &a : arr) {
+    if (!ok) break;
+    buf.resize(a.bytes);
+    if (hipMemcpy(buf.data(), a.dev, a.bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+      ok = false;
+      break;
+    }
+    long long nb = (long long)a.bytes;
+    ok = fwrite(&a.id, sizeof(int), 1, f) == 1 && fwrite(&nb, sizeof(nb), 1, f) == 1 &&
+         fwrite(buf.data(), 1, a.bytes, f) == a.bytes;
+  }
+  if (fclose(f) != 0) ok = false;
+  if (!ok) {
+    snprintf(E->err, sizeof(E->err), "checkpoint: write of %s failed", path);
+    return -3;
+  }
+  return 0;
+}
+
+// A new member from the case it was built from and a checkpoint of it:
+// the case supplies the metrics and tables (built deterministically), the
+// file the state and time planes.  Returns the slot.
+extern "C" int epic_engine_restore(epic_engine *E, const epic_case *c, const char *path) {
+  if (!E || !c || !path) return -1;
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    snprintf(E->err, sizeof(E->err), "restore: cannot open %s", path);
+    return -3;
+  }
+  char magic[8];
+  int hdr[10], n = 0;
+  long long steps = 0;
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, CKPT_MAGIC, 8) == 0 && fread(hdr, sizeof(int), 10, f) == 10 &&
+            fread(&steps, sizeof(steps), 1, f) == 1 && fread(&n, sizeof(int), 1, f) == 1 && hdr[0] == 1;
+  const int mixed = (E->precision == EPIC_MIXED);
+  if (!ok || hdr[1] != c->ni || hdr[2] != c->nj || hdr[3] != c->nk || hdr[4] != mixed) {
+    fclose(f);
+    snprintf(E->err, sizeof(E->err), "restore: %s is not a checkpoint of a %dx%dx%d member in this precision", path,
+             c->ni, c->nj, c->nk);
+    return -3;
+  }
+  int slot = engine_add_member(E, c, nullptr, false);
+  if (slot < 0) {
+    fclose(f);
+    return slot;
+  }
+  const MemberDesc &d = E->m[slot].h;
+  auto arr = ckpt_arrays(d);
+  std::vector<char> buf;
+  HIPCHK(hipStreamSynchronize(E->stream));  // the member's set-up uploads first
+  for (int q = 0; q < n && ok; q++) {
+    int id;
+    long long nb;
+    ok = fread(&id, sizeof(int), 1, f) == 1 && fread(&nb, sizeof(nb), 1, f) == 1;
+    if (!ok) break;
+    auto it = std::find_if(arr.begin(), arr.end(), [&](const Ckpt &a) { return a.id == id; });
+    if (it == arr.end() || (size_t)nb != it->bytes) {
+      ok = false;
+      break;
+    }
+    buf.resize(it->bytes);
+    ok = fread(buf.data(), 1, it->bytes, f) == it->bytes &&
+         hipMemcpy(it->dev, buf.data(), it->bytes, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  fclose(f);
+  if (!ok || n != (int)arr.size()) {
+    epic_engine_remove(E, slot);
+    snprintf(E->err, sizeof(E->err), "restore: %s is truncated or holds other arrays", path);
+    return -3;
+  }
+  MemberDyn dy;
+  memset(&dy, 0, sizeof(dy));
+  dy.it0 = hdr[5];
+  dy.itm1 = hdr[6];
+  dy.itm2 = hdr[7];
+  dy.itime = hdr[8];
+  dy.err = hdr[9];
+  HIPCHK(hipMemcpy(d.dyn, &dy, sizeof(dy), hipMemcpyHostToDevice));
+  E->m[slot].steps = (long)steps;
+  return slot;
+}
+
 // --------------------------------------------------------------- autotune
 // Picks, per bucket, the launch shape of each tunable kernel (columns per
 // column tile, rows per zonal-hyperviscosity block) by timing every
@@ -896,6 +1038,19 @@ extern "C" int epic_copy_probe(size_t nbytes, int reps, double *seconds_per_copy
   (void)hipEventDestroy(e1);
   (void)hipFree(a);
   (void)hipFree(b);
+  return rc;
+}
+
+extern "C" int epic_engine_restore_spec(epic_engine *E, const epic_case_spec *sp, const char *path) {
+  epic_case c;
+  int rc = epic_case_build(sp, &c);
+  if (rc) {
+    if (E) snprintf(E->err, sizeof(E->err), "case build failed (%d)", rc);
+    epic_case_free(&c);
+    return -1;
+  }
+  rc = epic_engine_restore(E, &c, path);
+  epic_case_free(&c);
   return rc;
 }
 

@@ -104,10 +104,10 @@ static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(doub
 
 size_t epic_member_footprint(int ni, int nj, int nk, int precision_mode) {
   size_t n3 = (size_t)(ni + 2) * (nj + 3) * (nk + 2);
-  /* 38 FP64 3-D arrays of state, diagnostics and scratch, six tendency
+  /* 37 FP64 3-D arrays of state, diagnostics and scratch, six tendency
    * planes (FP32 in the mixed mode, which adds two FP64 current planes)
    * and small tables. */
-  size_t b = n3 * 8 * 38 + (precision_mode == EPIC_MIXED ? n3 * 8 * 2 + n3 * 4 * 6 : n3 * 8 * 6);
+  size_t b = n3 * 8 * 37 + (precision_mode == EPIC_MIXED ? n3 * 8 * 2 + n3 * 4 * 6 : n3 * 8 * 6);
   return b + (size_t)(nj + 3) * (ni + 2) * 8 * 6 + 65536;
 }
 

@@ -298,6 +298,16 @@ int  epic_engine_member_error(epic_engine *e, int slot);
  * names/ms arrays (up to cap entries).  Returns the entry count. */
 int  epic_engine_profile(epic_engine *e, int nsteps, char (*names)[48], double *ms_total,
                          int *launches, int cap);
+/* Exact restart (the engine's counterpart of the reference's restart file,
+ * src/shared/epic_funcs_io.c:730-2257, in a binary layout of its own):
+ * write every array of a member that its next step reads, with its
+ * Adams-Bashforth history and time planes, to `path`; a member restored
+ * from that file with the same case continues bit for bit as the original
+ * would.  restore returns the new slot. */
+int  epic_engine_checkpoint(epic_engine *e, int slot, const char *path);
+int  epic_engine_restore(epic_engine *e, const epic_case *c, const char *path);
+/* epic_case_build() + epic_engine_restore() + epic_case_free() in one call. */
+int  epic_engine_restore_spec(epic_engine *e, const epic_case_spec *spec, const char *path);
 /* Autotuner (no counterpart in the reference, whose loops have no launch
  * shapes): for each bucket, time every candidate launch shape of the tunable
  * kernels (columns per column tile of set_p2etc, gz3, calc_w and the

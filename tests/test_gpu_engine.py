@@ -115,3 +115,31 @@ def test_autotune_changes_no_state_and_no_result():
             assert np.isfinite(a).all(), (k, f)
             assert e.read(s, f).tobytes() == a.tobytes(), (k, f)
     e.close()
+
+
+@pytest.mark.parametrize("precision", [ea.FP64, ea.MIXED])
+def test_checkpoint_restart_is_exact(tmp_path, precision):
+    """A member restored from a checkpoint (state, Adams-Bashforth history,
+    time planes) continues bit for bit as the original, also when it lands
+    in a bucket beside another member."""
+    say("checkpoint", precision)
+    e = ea.Engine(precision, 2)
+    s = e.add(B)
+    e.step(9)
+    fn = tmp_path / "member.ckpt"
+    e.checkpoint(s, fn)
+    e.step(6)
+    want = {f: e.read(s, f) for f in PROG + ("PV3", "GZ3", "DUDT0", "DUDT1", "DUDT2")}
+    tp = e.time_planes(s)
+    e.close()
+    r = ea.Engine(precision, 3)
+    r.add(C)
+    t = r.restore(B, fn)
+    assert r.steps_done(t) == 9
+    r.step(6)
+    assert r.time_planes(t) == tp
+    for f, a in want.items():
+        assert r.read(t, f).tobytes() == a.tobytes(), f
+    with pytest.raises(ea.EpicError):
+        r.restore(A, fn)  # another grid
+    r.close()
