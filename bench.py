@@ -44,7 +44,7 @@ WORKLOADS = {
 # hyperviscosity kernels two fields).
 KERNEL_BYTES = {
     "k_set_p2etc": (56, 1), "k_thermo": (88, 1), "k_gz3": (40, 1), "k_calc_w": (56, 1), "k_dzdt3": (72, 1),
-    "k_hsu_vert": (24, 3), "k_hsu_mer_ap": (24, 3), "k_hsu_mer_ff": (32, 3), "k_hsu_mer_fin": (32, 3),
+    "k_hsu_vert": (24, 3), "k_hsu_mer": (40, 3),
     "k_hsu_zonal": (32, 3), "k_fft_filter": (16, 3),
     "k_filter3": (24, 4), "k_zonal_hv": (16, 2), "k_d2dy2x2": (16, 2), "k_hv_apply": (24, 2),
     "k_pgrad": (48, 1), "k_uvh_kin": (48, 1), "k_dvdt_core": (40, 1), "k_dudt_core": (40, 1), "k_ab": (80, 1),

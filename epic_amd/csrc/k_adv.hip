@@ -177,49 +177,74 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
 }
 
 // ---------------------------------------------------------- meridional
-struct MerCol {
-  const MemberDesc *d;
-  const double *A;  // plane K of the field
-  int K, I, which;
-  double amin, DT;
-  __device__ double a(int J) const { return A[ix2(*d, J, I)]; }
-  __device__ double v2(int J) const {
-    const MemberDesc &m = *d;
-    if (which == AV_HDRY) {
-      if (J < 1 || J > m.nj) return 0.;
-      return .5 * (m.V[ix3(m, K, J, I)] + m.V[ix3(m, K - 1, J, I)]);
-    }
-    return m.V[ix3(m, K, J, I)];
+// One launch: a block holds a tile of MER_TJ rows x CW columns of one plane
+// of one field in LDS, with halos: the field on rows J0-2..J0+MER_TJ+1, the
+// meridional velocities vp, vm (v times 1/m, split by sign) on v-rows
+// J0-1..J0+MER_TJ+1, the smoothness terms aq and the predictor ap on rows
+// J0-1..J0+MER_TJ and the corrected flux on v-rows J0..J0+MER_TJ, each formed
+// once per cell by the reference's expression (the halo rows are formed again
+// by the neighbouring tile: the same values).  The update goes to sA and to
+// its copy sB (which the zonal pass filters), with sB's zonal pads.
+constexpr int MER_TJ = 8;
+__global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, int nm, int CW) {
+  ADV_MEMBER;
+  const int K = 1 + blockIdx.y;
+  if (which == AV_THETA && K <= d.ks - 2) return;  // uniform per block
+  extern __shared__ double sh[];
+  const int nj = d.nj, nchunk = d.ni / CW;
+  const int tile = blockIdx.x / nchunk, chunk = blockIdx.x - tile * nchunk;
+  const int J0 = tile * MER_TJ, c = threadIdx.x % CW, r0 = threadIdx.x / CW, RS = blockDim.x / CW;
+  const int I = 1 + chunk * CW + c;
+  constexpr int RA = MER_TJ + 4, RV = MER_TJ + 3, RP = MER_TJ + 2;
+  double *ta = sh, *tvp = ta + RA * CW, *tvm = tvp + RV * CW, *taq = tvm + RV * CW, *tap = taq + RP * CW;
+  double *tff = tap + RP * CW;
+  const size_t o = (size_t)K * d.N2;
+  const double *A = adv_field(d, which) + o;
+  const double amin = (which == AV_HDRY) ? d.h_min[K] : 0., DT = d.dt, ep = 1.e-10;
+  // row index helpers: field rows from J0-2, v-rows and predictor rows from J0-1
+  auto a = [&](int J) { return ta[(J - J0 + 2) * CW + c]; };
+  auto vp = [&](int J) { return tvp[(J - J0 + 1) * CW + c]; };
+  auto vm = [&](int J) { return tvm[(J - J0 + 1) * CW + c]; };
+  auto aq = [&](int J) { return taq[(J - J0 + 1) * CW + c]; };
+  auto ap = [&](int J) { return tap[(J - J0 + 1) * CW + c]; };
+  // the field (rows outside -1..nj+1 are never read) and the velocities
+  for (int r = r0; r < RA; r += RS) {
+    const int J = J0 - 2 + r;
+    ta[r * CW + c] = (J >= -1 && J <= nj + 1) ? A[ix2(d, J, I)] : 0.;
   }
-  __device__ double vm(int J) const {
-    double mi = (fabs(d->lat[2 * J]) == 90.) ? 0. : 1. / d->m[2 * J];
-    return dmin(0., v2(J) * mi);
+  for (int r = r0; r < RV; r += RS) {
+    const int J = J0 - 1 + r;
+    if (J < 0 || J > nj + 1) continue;
+    double v2;
+    if (which == AV_HDRY) v2 = (J < 1 || J > nj) ? 0. : .5 * (d.V[ix3(d, K, J, I)] + d.V[ix3(d, K - 1, J, I)]);
+    else v2 = d.V[ix3(d, K, J, I)];
+    const double mi = (fabs(d.lat[2 * J]) == 90.) ? 0. : 1. / d.m[2 * J];
+    tvm[r * CW + c] = dmin(0., v2 * mi);
+    tvp[r * CW + c] = dmax(0., v2 * mi);
   }
-  __device__ double vp(int J) const {
-    double mi = (fabs(d->lat[2 * J]) == 90.) ? 0. : 1. / d->m[2 * J];
-    return dmax(0., v2(J) * mi);
-  }
-  __device__ double ffp(int J) const {
-    if (J < 1 || J > d->nj) return 0.;
-    return vp(J) * a(J - 1) + vm(J) * a(J);
-  }
-  __device__ double ap(int J) const { return a(J) + (ffp(J) - ffp(J + 1)) * d->mn[2 * J + 1] * DT; }
-  __device__ double dif1(int J) const { return J == 0 ? 0. : ap(J) - a(J - 1); }
-  __device__ double dif2(int J) const { return J == d->nj ? 0. : ap(J) - a(J + 1); }
-  __device__ double aq(int J) const {
-    const double ep = 1.e-10;
+  __syncthreads();
+  // smoothness terms and predictor on rows J0-1 .. J0+MER_TJ within 0..nj
+  for (int r = r0; r < RP; r += RS) {
+    const int J = J0 - 1 + r;
+    if (J < 0 || J > nj) continue;
     double t;
     if (J == 0) t = fabs(-a(0) + a(1)) + ep;
-    else if (J == d->nj) t = fabs(a(J - 1) - a(J)) + ep;
+    else if (J == nj) t = fabs(a(J - 1) - a(J)) + ep;
     else t = fabs(a(J - 1) - 2. * a(J) + a(J + 1)) + ep;
-    return t * t;
+    taq[r * CW + c] = t * t;
+    const double f0 = (J < 1) ? 0. : vp(J) * a(J - 1) + vm(J) * a(J);
+    const double f1 = (J + 1 > nj) ? 0. : vp(J + 1) * a(J) + vm(J + 1) * a(J + 1);
+    tap[r * CW + c] = a(J) + (f0 - f1) * d.mn[2 * J + 1] * DT;
   }
-  __device__ double ff(int J) const {
-    if (J < 1 || J > d->nj) return 0.;
+  __syncthreads();
+  // corrected flux on v-rows J0 .. J0+MER_TJ within 1..nj
+  for (int r = r0; r <= MER_TJ; r += RS) {
+    const int J = J0 + r;
+    if (J < 1 || J > nj) continue;
     double VP = vp(J), VM = vm(J), APj = ap(J), APm = ap(J - 1), Am = a(J - 1), Aj = a(J);
     double fc = .5 * (VP * (APj + Am) + VM * (APm + Aj));
     double vhp = sqrt(VP * vp(J - 1)), vhm = -sqrt(VM * vm(J + 1));
-    double mu = (VP - VM) * d->mn[2 * J] * DT, al = (1. + mu) / 6.;
+    double mu = (VP - VM) * d.mn[2 * J] * DT, al = (1. + mu) / 6.;
     double q0 = aq(J - 1), q1 = aq(J);
     double gp = q0 / (q0 + (Am - amin) * (Aj - amin));
     gp = gp * gp;
@@ -227,77 +252,26 @@ struct MerCol {
     gm = gm * gm;
     double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
     double bhp = 1. - gp, bhm = 1. - gm;
-    double d1J = (APj - Am), d1m = dif1(J - 1), d2m = dif2(J - 1), d2J = dif2(J);
-    return fc + -al * (VP * bp * d1J - vhp * bhp * d1m + VM * bm * d2m - vhm * bhm * d2J);
+    double d1J = (APj - Am);
+    double d1m = (J - 1 == 0) ? 0. : APm - a(J - 2);
+    double d2m = APm - a(J);
+    double d2J = (J == nj) ? 0. : APj - a(J + 1);
+    tff[r * CW + c] = fc + -al * (VP * bp * d1J - vhp * bhp * d1m + VM * bm * d2m - vhm * bhm * d2J);
   }
-};
-
-// Three launches, each one lane per cell: the predictor ap(J) (into sC),
-// the corrected flux ff(J) on v-rows 1..nj (into sD), and the update into
-// sA and its copy sB (which the zonal pass filters).
-#define MER_SETUP                                                   \
-  const int K = 1 + blockIdx.y;                                     \
-  if (which == AV_THETA && K <= d.ks - 2) return;                   \
-  MerCol c;                                                         \
-  c.d = &d;                                                         \
-  c.A = adv_field(d, which) + (size_t)K * d.N2;                     \
-  c.K = K;                                                          \
-  c.I = I;                                                          \
-  c.which = which;                                                  \
-  c.amin = (which == AV_HDRY) ? d.h_min[K] : 0.;                    \
-  c.DT = d.dt;                                                      \
-  const size_t o = (size_t)K * d.N2;
-
-__global__ void __launch_bounds__(256) k_hsu_mer_ap(const MemberDesc *D, int mb, int nm) {
-  ADV_MEMBER;
-  int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= (d.nj + 1) * d.ni) return;
-  const int J = t / d.ni, I = 1 + t % d.ni;
-  MER_SETUP;
-  d.sC[so + o + ix2(d, J, I)] = c.ap(J);
+  __syncthreads();
+  // update on the tile's rows
+  double *sA = d.sA + so + o, *sB = d.sB + so + o;
+  for (int r = r0; r < MER_TJ; r += RS) {
+    const int J = J0 + r;
+    if (J > nj) break;
+    const double f0 = (J >= 1) ? tff[r * CW + c] : 0., f1 = (J + 1 <= nj) ? tff[(r + 1) * CW + c] : 0.;
+    const double v = a(J) + (f0 - f1) * d.mn[2 * J + 1] * DT;
+    sA[ix2(d, J, I)] = v;
+    sB[ix2(d, J, I)] = v;
+    if (I == d.ni) sB[ix2(d, J, 0)] = v;
+    if (I == 1) sB[ix2(d, J, d.ni + 1)] = v;
+  }
 }
-__global__ void __launch_bounds__(256) k_hsu_mer_ff(const MemberDesc *D, int mb, int nm) {
-  ADV_MEMBER;
-  int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= d.nj * d.ni) return;
-  const int J = 1 + t / d.ni, I = 1 + t % d.ni;
-  MER_SETUP;
-  const double *AP = d.sC + so + o;
-  const int nj = d.nj;
-  double VP = c.vp(J), VM = c.vm(J), APj = AP[ix2(d, J, I)], APm = AP[ix2(d, J - 1, I)], Am = c.a(J - 1), Aj = c.a(J);
-  double fc = .5 * (VP * (APj + Am) + VM * (APm + Aj));
-  double vhp = sqrt(VP * c.vp(J - 1)), vhm = -sqrt(VM * c.vm(J + 1));
-  double mu = (VP - VM) * d.mn[2 * J] * c.DT, al = (1. + mu) / 6.;
-  double q0 = c.aq(J - 1), q1 = c.aq(J);
-  double gp = q0 / (q0 + (Am - c.amin) * (Aj - c.amin));
-  gp = gp * gp;
-  double gm = q1 / (q1 + (Am - c.amin) * (Aj - c.amin));
-  gm = gm * gm;
-  double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
-  double bhp = 1. - gp, bhm = 1. - gm;
-  double d1J = (APj - Am);
-  double d1m = (J - 1 == 0) ? 0. : APm - c.a(J - 2);
-  double d2m = APm - c.a(J);
-  double d2J = (J == nj) ? 0. : APj - c.a(J + 1);
-  d.sD[so + o + ix2(d, J, I)] = fc + -al * (VP * bp * d1J - vhp * bhp * d1m + VM * bm * d2m - vhm * bhm * d2J);
-}
-__global__ void __launch_bounds__(256) k_hsu_mer_fin(const MemberDesc *D, int mb, int nm) {
-  ADV_MEMBER;
-  int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= (d.nj + 1) * d.ni) return;
-  const int J = t / d.ni, I = 1 + t % d.ni;
-  MER_SETUP;
-  const double *FF = d.sD + so + o;
-  double f0 = (J >= 1) ? FF[ix2(d, J, I)] : 0., f1 = (J + 1 <= d.nj) ? FF[ix2(d, J + 1, I)] : 0.;
-  double v = c.a(J) + (f0 - f1) * d.mn[2 * J + 1] * c.DT;
-  const size_t q = so + o + ix2(d, J, I);
-  d.sA[q] = v;
-  d.sB[q] = v;
-  // the copy the zonal pass filters carries its zonal pads
-  if (I == d.ni) d.sB[so + o + ix2(d, J, 0)] = v;
-  if (I == 1) d.sB[so + o + ix2(d, J, d.ni + 1)] = v;
-}
-#undef MER_SETUP
 
 // --------------------------------------------------------------- zonal
 // Rows of the zonally filtered copy sB (1024/ni rows per block), staged in
@@ -379,10 +353,10 @@ static void hsu3(hipStream_t s, const Bucket &b) {
   if (b.knob[KN_COLS_HSUV]) C = b.knob[KN_COLS_HSUV];
   LAUNCH(k_hsu_vert, dim3((ncol + C - 1) / C, 1, nz), dim3(256), sizeof(double) * 4 * C * (b.nk + 2), s, b.D, b.mb,
          b.nm, C);
-  dim3 g(((b.nj + 1) * b.ni + TPB - 1) / TPB, b.nk, nz);
-  LAUNCH(k_hsu_mer_ap, g, dim3(TPB), 0, s, b.D, b.mb, b.nm);
-  LAUNCH(k_hsu_mer_ff, dim3((b.nj * b.ni + TPB - 1) / TPB, b.nk, nz), dim3(TPB), 0, s, b.D, b.mb, b.nm);
-  LAUNCH(k_hsu_mer_fin, g, dim3(TPB), 0, s, b.D, b.mb, b.nm);
+  const int CW = b.ni < 64 ? b.ni : 64, tiles = (b.nj + 1 + MER_TJ - 1) / MER_TJ;
+  LAUNCH(k_hsu_mer, dim3(tiles * (b.ni / CW), b.nk, nz), dim3(256),
+         sizeof(double) * CW * ((MER_TJ + 4) + 2 * (MER_TJ + 3) + 2 * (MER_TJ + 2) + (MER_TJ + 1)), s, b.D, b.mb,
+         b.nm, CW);
   // the zonal pass reads a zonally filtered copy (the reference filters it
   // per level before the zonal sweep); a separate launch packs the rows of
   // the filter band, which is faster here than filtering inside the pass
