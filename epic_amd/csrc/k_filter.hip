@@ -266,13 +266,26 @@ __global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, in
   const int vg = (kind == 1);
   const bool filt = live && (vg ? d.rflag_v[J] : d.rflag_h[J]);
   const int KT = d.nk + 2, nt = r.kend - r.kstart + 1;
-  double *tab = tabs + (size_t)rs * 3 * KT;
-  T *buf = reinterpret_cast<T *>(tabs + (size_t)RB * 3 * KT) + (size_t)rs * n;
+  double *tab = tabs + (size_t)rs * 5 * KT;
+  T *buf = reinterpret_cast<T *>(tabs + (size_t)RB * 5 * KT) + (size_t)rs * n;
   if (live)
     for (int e = lane; e < 3 * KT; e += nq) tab[e] = d.spl[(size_t)(fi * (d.nj + 2) + J) * 3 * KT + e];
   row_sync(nq);
   T *a = fld_ptr_t<T>(d, f) + (size_t)K * d.N2;
   const double *tx = tab, *ty = tab + KT, *tz = tab + 2 * KT;
+  // the cubic of each interval k depends on the table alone (splint_pchip,
+  // src/shared/epic_funcs_util.c:926-999): its coefficients are formed once
+  // per row, by the same expressions, instead of once per cell
+  double *tc2 = tab + 3 * KT, *tc3 = tab + 4 * KT;
+  if (live)
+    for (int k = lane; k < nt - 1; k += nq) {
+      const double h = tx[k + 1] - tx[k];
+      const double delta = (ty[k + 1] - ty[k]) / h;
+      const double e1 = (tz[k] - delta) / h, e2 = (tz[k + 1] - delta) / h;
+      tc3[k] = (e1 + e2) / h;
+      tc2[k] = -(e1 + e1 + e2);
+    }
+  row_sync(nq);
   double v[4];
   T x[4];
   if (live)
@@ -280,11 +293,8 @@ __global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, in
       const int I = lane + q * nq + 1;
       double xx = geez(d, kind, K, J, I), h;
       int k = table_place(nt, tx, xx, &h);
-      double delta = (ty[k + 1] - ty[k]) / h;
-      double e1 = (tz[k] - delta) / h, e2 = (tz[k + 1] - delta) / h;
-      double c3 = (e1 + e2) / h, c2 = -(e1 + e1 + e2);
       double xr = xx - tx[k];
-      double val = ty[k] + xr * (tz[k] + xr * (c2 + xr * c3));
+      double val = ty[k] + xr * (tz[k] + xr * (tc2[k] + xr * tc3[k]));
       if (kind == 2) val = exp(val);
       v[q] = val;
       x[q] = T((double)a[ix2(d, J, I)] - val);
@@ -354,7 +364,7 @@ void L_filter3(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   LAUNCH(k_spline_prep<double>, dim3(nrow, 1, nz), dim3(64), 0, s, b.D, b.mb, b.nm, fs);
   const int nq = b.ni / 4, RB = (256 / nq) < 16 ? (256 / nq) : 16;
   dim3 g((nrow + RB - 1) / RB, b.nk, nz);
-  size_t shm = sizeof(double) * (size_t)RB * 3 * (b.nk + 2) + sizeof(double) * (size_t)RB * b.ni;
+  size_t shm = sizeof(double) * (size_t)RB * 5 * (b.nk + 2) + sizeof(double) * (size_t)RB * b.ni;
   LAUNCH(k_filter3<double>, g, dim3(RB * nq), shm, s, b.D, b.mb, b.nm, fs);
 }
 
