@@ -187,12 +187,12 @@ __device__ __forceinline__ void wave_sync() {
 // s + a[0] + a[1] + ... + a[n-1] (or s - a[0] - a[1] - ...), accumulated left
 // to right as the reference's loops do; the loads are issued eight at a time
 // ahead of the additions so the chain waits on additions only.
-template <bool SUB = false>
-__device__ __forceinline__ double seq_acc(const double *a, int n, double s = 0.) {
+template <bool SUB = false, class T = double>
+__device__ __forceinline__ T seq_acc(const T *a, int n, T s = T(0)) {
   int i = 0;
   for (; i + 8 <= n; i += 8) {
-    const double v0 = a[i], v1 = a[i + 1], v2 = a[i + 2], v3 = a[i + 3];
-    const double v4 = a[i + 4], v5 = a[i + 5], v6 = a[i + 6], v7 = a[i + 7];
+    const T v0 = a[i], v1 = a[i + 1], v2 = a[i + 2], v3 = a[i + 3];
+    const T v4 = a[i + 4], v5 = a[i + 5], v6 = a[i + 6], v7 = a[i + 7];
     if (SUB) { s -= v0; s -= v1; s -= v2; s -= v3; s -= v4; s -= v5; s -= v6; s -= v7; }
     else { s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7; }
   }

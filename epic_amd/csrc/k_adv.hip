@@ -44,24 +44,36 @@ __device__ __forceinline__ double *adv_field(const MemberDesc &d, int which) {
 // The pass is a chain of K-stencils (first-guess flux, predictor, centred
 // flux, limiter correction, update) with no recurrence, so each stage runs
 // on all lanes over (column, level) pairs, separated by barriers.
-__device__ __forceinline__ double tt_layer(const double *wa, int K, int nk, double ep) {
-  double tt;
+//
+// The advection passes are templated on the arithmetic type T of the fluxes:
+// double reproduces the reference operation for operation; float is the
+// mixed mode's (the fields stay FP64: each update adds the FP32 flux
+// divergence to the FP64 value).
+template <class T> __device__ __forceinline__ T tmin(T x, T y) { return x < y ? x : y; }
+template <class T> __device__ __forceinline__ T tmax(T x, T y) { return x > y ? x : y; }
+
+template <class T>
+__device__ __forceinline__ T tt_layer(const T *wa, int K, int nk, T ep) {
+  T tt;
   if (K == 1) tt = fabs(wa[K + 1] - wa[K]) + ep;
   else if (K == nk) tt = fabs(-wa[K] + wa[K - 1]) + ep;
-  else tt = fabs(wa[K + 1] - 2. * wa[K] + wa[K - 1]) + ep;
+  else tt = fabs(wa[K + 1] - T(2.) * wa[K] + wa[K - 1]) + ep;
   return tt * tt;
 }
-__device__ __forceinline__ double tt_iface(const double *wa, int K, int nk, double ep) {
-  double tt;
+template <class T>
+__device__ __forceinline__ T tt_iface(const T *wa, int K, int nk, T ep) {
+  T tt;
   if (K == 0) tt = fabs(wa[K + 1] - wa[K]) + ep;
   else if (K == nk) tt = fabs(-wa[K] + wa[K - 1]) + ep;
-  else tt = fabs(wa[K + 1] - 2. * wa[K] + wa[K - 1]) + ep;
+  else tt = fabs(wa[K + 1] - T(2.) * wa[K] + wa[K - 1]) + ep;
   return tt * tt;
 }
 
+template <class T>
 __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, int nm, int C) {
   ADV_MEMBER;
-  extern __shared__ double lds[];
+  extern __shared__ double lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
   const int c = threadIdx.x % C, kk = threadIdx.x / C, KS = blockDim.x / C;
   const int nI = d.ni + 2, col = blockIdx.x * C + c;
   const bool live = col < (d.nj + 1) * nI;
@@ -69,10 +81,10 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
   const int nk = d.nk, L = nk + 2;
   const int layer = (which == AV_HDRY);
   const int kf = (which == AV_THETA) ? d.ks - 1 : 1;
-  const double ep = 1.e-10, DT = d.dt;
+  const T ep = T(1.e-10), DT = T(d.dt);
   const double *dsi = d.dsgth_inv;
   double *b = adv_field(d, which);
-  double *wa = lds + c * L, *wv = lds + (C + c) * L, *pr = lds + (2 * C + c) * L, *ff = lds + (3 * C + c) * L;
+  T *wa = lds + c * L, *wv = lds + (C + c) * L, *pr = lds + (2 * C + c) * L, *ff = lds + (3 * C + c) * L;
   // stage 1: field and vertical velocity (wv holds w on the points the
   // field lives on: W3 itself for layers, the interface average otherwise)
   if (live) {
@@ -91,77 +103,79 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
         }
         if (which == AV_THETA && K == kf - 2) x = 1.;
       }
-      wa[K] = x;
-      wv[K] = w;
+      wa[K] = T(x);
+      wv[K] = T(w);
     }
   }
   __syncthreads();
-#define WM(k) dmin(0., wv[k])
-#define WP(k) dmax(0., wv[k])
+#define WM(k) tmin(T(0.), wv[k])
+#define WP(k) tmax(T(0.), wv[k])
   if (layer) {
     // first-guess flux on interfaces kf..nk-1 (zero at kf-1 and nk)
     if (live)
       for (int K = kf - 1 + kk; K <= nk; K += KS)
-        ff[K] = (K == kf - 1 || K == nk) ? 0. : WP(K) * wa[K + 1] + WM(K) * wa[K];
+        ff[K] = (K == kf - 1 || K == nk) ? T(0.) : WP(K) * wa[K + 1] + WM(K) * wa[K];
     __syncthreads();
     if (live)
-      for (int K = kf + kk; K <= nk; K += KS) pr[K] = wa[K] + (ff[K] - ff[K - 1]) * dsi[2 * K] * DT;
+      for (int K = kf + kk; K <= nk; K += KS) pr[K] = wa[K] + (ff[K] - ff[K - 1]) * T(dsi[2 * K]) * DT;
     __syncthreads();
     if (live)
-      for (int K = kf + kk; K < nk; K += KS) ff[K] = .5 * (WP(K) * (pr[K] + wa[K + 1]) + WM(K) * (pr[K + 1] + wa[K]));
+      for (int K = kf + kk; K < nk; K += KS)
+        ff[K] = T(.5) * (WP(K) * (pr[K] + wa[K + 1]) + WM(K) * (pr[K + 1] + wa[K]));
     __syncthreads();
     if (live)
       for (int K = kf + kk; K < nk; K += KS) {
-        const double d1K = pr[K] - wa[K + 1];
-        const double d1P = (K + 1 == nk) ? 0. : pr[K + 1] - wa[K + 2];
-        const double d2K = (K == 1) ? 0. : pr[K] - wa[K - 1];
-        const double d2P = pr[K + 1] - wa[K];
-        const double aaK = tt_layer(wa, K, nk, ep), aaP = tt_layer(wa, K + 1, nk, ep);
-        const double aminK = (K >= kf) ? d.h_min[K] : 0., aminP = d.h_min[K + 1];
-        const double wpK = WP(K), wmK = WM(K);
-        double whp = sqrt(wpK * WP(K + 1)), whm = -sqrt(wmK * WM(K - 1));
-        double mu = (wpK - wmK) * dsi[2 * K + 1] * DT, al = (1. + mu) / 6.;
-        double gp = aaP / (aaP + (wa[K + 1] - aminP) * (wa[K] - aminK));
+        const T d1K = pr[K] - wa[K + 1];
+        const T d1P = (K + 1 == nk) ? T(0.) : pr[K + 1] - wa[K + 2];
+        const T d2K = (K == 1) ? T(0.) : pr[K] - wa[K - 1];
+        const T d2P = pr[K + 1] - wa[K];
+        const T aaK = tt_layer(wa, K, nk, ep), aaP = tt_layer(wa, K + 1, nk, ep);
+        const T aminK = (K >= kf) ? T(d.h_min[K]) : T(0.), aminP = T(d.h_min[K + 1]);
+        const T wpK = WP(K), wmK = WM(K);
+        T whp = sqrt(wpK * WP(K + 1)), whm = -sqrt(wmK * WM(K - 1));
+        T mu = (wpK - wmK) * T(dsi[2 * K + 1]) * DT, al = (T(1.) + mu) / T(6.);
+        T gp = aaP / (aaP + (wa[K + 1] - aminP) * (wa[K] - aminK));
         gp = gp * gp;
-        double gm = aaK / (aaK + (wa[K + 1] - aminP) * (wa[K] - aminK));
+        T gm = aaK / (aaK + (wa[K + 1] - aminP) * (wa[K] - aminK));
         gm = gm * gm;
-        double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
-        double bhp = 1. - gp, bhm = 1. - gm;
+        T bp = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gp, bm = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gm;
+        T bhp = T(1.) - gp, bhm = T(1.) - gm;
         ff[K] += -al * (wpK * bp * d1K - whp * bhp * d1P + wmK * bm * d2P - whm * bhm * d2K);
       }
     __syncthreads();
     if (live)
-      for (int K = kf + kk; K <= nk; K += KS) b[ix3(d, K, J, I)] += (ff[K] - ff[K - 1]) * dsi[2 * K] * DT;
+      for (int K = kf + kk; K <= nk; K += KS) b[ix3(d, K, J, I)] += (double)((ff[K] - ff[K - 1]) * T(dsi[2 * K]) * DT);
   } else {
     // first-guess flux on layers kf..nk (zero below the bottom)
     if (live)
-      for (int K = kf + kk; K <= nk + 1; K += KS) ff[K] = (K == nk + 1) ? 0. : WP(K) * wa[K] + WM(K) * wa[K - 1];
+      for (int K = kf + kk; K <= nk + 1; K += KS) ff[K] = (K == nk + 1) ? T(0.) : WP(K) * wa[K] + WM(K) * wa[K - 1];
     __syncthreads();
     if (live)
       for (int K = kf - 1 + kk; K <= nk; K += KS)
-        pr[K] = (K == kf - 1) ? wa[K] : wa[K] + (ff[K + 1] - ff[K]) * dsi[2 * K + 1] * DT;
+        pr[K] = (K == kf - 1) ? wa[K] : wa[K] + (ff[K + 1] - ff[K]) * T(dsi[2 * K + 1]) * DT;
     __syncthreads();
     if (live)
-      for (int K = kf + kk; K <= nk; K += KS) ff[K] = .5 * (WP(K) * (pr[K - 1] + wa[K]) + WM(K) * (pr[K] + wa[K - 1]));
+      for (int K = kf + kk; K <= nk; K += KS)
+        ff[K] = T(.5) * (WP(K) * (pr[K - 1] + wa[K]) + WM(K) * (pr[K] + wa[K - 1]));
     __syncthreads();
     if (live)
       for (int K = kf + kk; K <= nk; K += KS) {
-        const double d1K = pr[K - 1] - wa[K];
-        const double d2K = pr[K] - wa[K - 1];
-        const double d2M = (K == kf) ? 0. : pr[K - 1] - wa[K - 2];
-        const double aaK = tt_iface(wa, K, nk, ep), aaM = tt_iface(wa, K - 1, nk, ep);
-        const double wpK = WP(K), wmK = WM(K);
-        double whm = (K == 1) ? 0. : -sqrt(wmK * WM(K - 1));
-        double whp = (K == nk) ? 0. : sqrt(wpK * WP(K + 1));
-        double mu = (wpK - wmK) * dsi[2 * K] * DT, al = (1. + mu) / 6.;
-        double gp = aaK / (aaK + (wa[K] - 0.) * (wa[K - 1] - 0.));
+        const T d1K = pr[K - 1] - wa[K];
+        const T d2K = pr[K] - wa[K - 1];
+        const T d2M = (K == kf) ? T(0.) : pr[K - 1] - wa[K - 2];
+        const T aaK = tt_iface(wa, K, nk, ep), aaM = tt_iface(wa, K - 1, nk, ep);
+        const T wpK = WP(K), wmK = WM(K);
+        T whm = (K == 1) ? T(0.) : -sqrt(wmK * WM(K - 1));
+        T whp = (K == nk) ? T(0.) : sqrt(wpK * WP(K + 1));
+        T mu = (wpK - wmK) * T(dsi[2 * K]) * DT, al = (T(1.) + mu) / T(6.);
+        T gp = aaK / (aaK + (wa[K] - T(0.)) * (wa[K - 1] - T(0.)));
         gp = gp * gp;
-        double gm = aaM / (aaM + (wa[K] - 0.) * (wa[K - 1] - 0.));
+        T gm = aaM / (aaM + (wa[K] - T(0.)) * (wa[K - 1] - T(0.)));
         gm = gm * gm;
-        double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
-        double bhp = 1. - gp, bhm = 1. - gm, g;
+        T bp = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gp, bm = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gm;
+        T bhp = T(1.) - gp, bhm = T(1.) - gm, g;
         if (K < nk) {
-          const double d1P = pr[K] - wa[K + 1];
+          const T d1P = pr[K] - wa[K + 1];
           g = -al * (wpK * bp * d1K - whp * bhp * d1P + wmK * bm * d2K - whm * bhm * d2M);
         } else {
           g = -al * (wpK * bp * d1K + wmK * bm * d2K - whm * bhm * d2M);
@@ -170,7 +184,8 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
       }
     __syncthreads();
     if (live)
-      for (int K = kf + kk; K <= nk; K += KS) b[ix3(d, K, J, I)] += (ff[K + 1] - ff[K]) * dsi[2 * K + 1] * DT;
+      for (int K = kf + kk; K <= nk; K += KS)
+        b[ix3(d, K, J, I)] += (double)((ff[K + 1] - ff[K]) * T(dsi[2 * K + 1]) * DT);
   }
 #undef WM
 #undef WP
@@ -186,21 +201,23 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
 // by the neighbouring tile: the same values).  The update goes to sA and to
 // its copy sB (which the zonal pass filters), with sB's zonal pads.
 constexpr int MER_TJ = 8;
+template <class T>
 __global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, int nm, int CW) {
   ADV_MEMBER;
   const int K = 1 + blockIdx.y;
   if (which == AV_THETA && K <= d.ks - 2) return;  // uniform per block
-  extern __shared__ double sh[];
+  extern __shared__ double sh_raw[];
+  T *sh = reinterpret_cast<T *>(sh_raw);
   const int nj = d.nj, nchunk = d.ni / CW;
   const int tile = blockIdx.x / nchunk, chunk = blockIdx.x - tile * nchunk;
   const int J0 = tile * MER_TJ, c = threadIdx.x % CW, r0 = threadIdx.x / CW, RS = blockDim.x / CW;
   const int I = 1 + chunk * CW + c;
   constexpr int RA = MER_TJ + 4, RV = MER_TJ + 3, RP = MER_TJ + 2;
-  double *ta = sh, *tvp = ta + RA * CW, *tvm = tvp + RV * CW, *taq = tvm + RV * CW, *tap = taq + RP * CW;
-  double *tff = tap + RP * CW;
+  T *ta = sh, *tvp = ta + RA * CW, *tvm = tvp + RV * CW, *taq = tvm + RV * CW, *tap = taq + RP * CW;
+  T *tff = tap + RP * CW;
   const size_t o = (size_t)K * d.N2;
   const double *A = adv_field(d, which) + o;
-  const double amin = (which == AV_HDRY) ? d.h_min[K] : 0., DT = d.dt, ep = 1.e-10;
+  const T amin = (which == AV_HDRY) ? T(d.h_min[K]) : T(0.), DT = T(d.dt), ep = T(1.e-10);
   // row index helpers: field rows from J0-2, v-rows and predictor rows from J0-1
   auto a = [&](int J) { return ta[(J - J0 + 2) * CW + c]; };
   auto vp = [&](int J) { return tvp[(J - J0 + 1) * CW + c]; };
@@ -210,7 +227,7 @@ __global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, in
   // the field (rows outside -1..nj+1 are never read) and the velocities
   for (int r = r0; r < RA; r += RS) {
     const int J = J0 - 2 + r;
-    ta[r * CW + c] = (J >= -1 && J <= nj + 1) ? A[ix2(d, J, I)] : 0.;
+    ta[r * CW + c] = (J >= -1 && J <= nj + 1) ? T(A[ix2(d, J, I)]) : T(0.);
   }
   for (int r = r0; r < RV; r += RS) {
     const int J = J0 - 1 + r;
@@ -219,57 +236,59 @@ __global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, in
     if (which == AV_HDRY) v2 = (J < 1 || J > nj) ? 0. : .5 * (d.V[ix3(d, K, J, I)] + d.V[ix3(d, K - 1, J, I)]);
     else v2 = d.V[ix3(d, K, J, I)];
     const double mi = (fabs(d.lat[2 * J]) == 90.) ? 0. : 1. / d.m[2 * J];
-    tvm[r * CW + c] = dmin(0., v2 * mi);
-    tvp[r * CW + c] = dmax(0., v2 * mi);
+    tvm[r * CW + c] = T(dmin(0., v2 * mi));
+    tvp[r * CW + c] = T(dmax(0., v2 * mi));
   }
   __syncthreads();
   // smoothness terms and predictor on rows J0-1 .. J0+MER_TJ within 0..nj
   for (int r = r0; r < RP; r += RS) {
     const int J = J0 - 1 + r;
     if (J < 0 || J > nj) continue;
-    double t;
+    T t;
     if (J == 0) t = fabs(-a(0) + a(1)) + ep;
     else if (J == nj) t = fabs(a(J - 1) - a(J)) + ep;
-    else t = fabs(a(J - 1) - 2. * a(J) + a(J + 1)) + ep;
+    else t = fabs(a(J - 1) - T(2.) * a(J) + a(J + 1)) + ep;
     taq[r * CW + c] = t * t;
-    const double f0 = (J < 1) ? 0. : vp(J) * a(J - 1) + vm(J) * a(J);
-    const double f1 = (J + 1 > nj) ? 0. : vp(J + 1) * a(J) + vm(J + 1) * a(J + 1);
-    tap[r * CW + c] = a(J) + (f0 - f1) * d.mn[2 * J + 1] * DT;
+    const T f0 = (J < 1) ? T(0.) : vp(J) * a(J - 1) + vm(J) * a(J);
+    const T f1 = (J + 1 > nj) ? T(0.) : vp(J + 1) * a(J) + vm(J + 1) * a(J + 1);
+    tap[r * CW + c] = a(J) + (f0 - f1) * T(d.mn[2 * J + 1]) * DT;
   }
   __syncthreads();
   // corrected flux on v-rows J0 .. J0+MER_TJ within 1..nj
   for (int r = r0; r <= MER_TJ; r += RS) {
     const int J = J0 + r;
     if (J < 1 || J > nj) continue;
-    double VP = vp(J), VM = vm(J), APj = ap(J), APm = ap(J - 1), Am = a(J - 1), Aj = a(J);
-    double fc = .5 * (VP * (APj + Am) + VM * (APm + Aj));
-    double vhp = sqrt(VP * vp(J - 1)), vhm = -sqrt(VM * vm(J + 1));
-    double mu = (VP - VM) * d.mn[2 * J] * DT, al = (1. + mu) / 6.;
-    double q0 = aq(J - 1), q1 = aq(J);
-    double gp = q0 / (q0 + (Am - amin) * (Aj - amin));
+    T VP = vp(J), VM = vm(J), APj = ap(J), APm = ap(J - 1), Am = a(J - 1), Aj = a(J);
+    T fc = T(.5) * (VP * (APj + Am) + VM * (APm + Aj));
+    T vhp = sqrt(VP * vp(J - 1)), vhm = -sqrt(VM * vm(J + 1));
+    T mu = (VP - VM) * T(d.mn[2 * J]) * DT, al = (T(1.) + mu) / T(6.);
+    T q0 = aq(J - 1), q1 = aq(J);
+    T gp = q0 / (q0 + (Am - amin) * (Aj - amin));
     gp = gp * gp;
-    double gm = q1 / (q1 + (Am - amin) * (Aj - amin));
+    T gm = q1 / (q1 + (Am - amin) * (Aj - amin));
     gm = gm * gm;
-    double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
-    double bhp = 1. - gp, bhm = 1. - gm;
-    double d1J = (APj - Am);
-    double d1m = (J - 1 == 0) ? 0. : APm - a(J - 2);
-    double d2m = APm - a(J);
-    double d2J = (J == nj) ? 0. : APj - a(J + 1);
+    T bp = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gp, bm = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gm;
+    T bhp = T(1.) - gp, bhm = T(1.) - gm;
+    T d1J = (APj - Am);
+    T d1m = (J - 1 == 0) ? T(0.) : APm - a(J - 2);
+    T d2m = APm - a(J);
+    T d2J = (J == nj) ? T(0.) : APj - a(J + 1);
     tff[r * CW + c] = fc + -al * (VP * bp * d1J - vhp * bhp * d1m + VM * bm * d2m - vhm * bhm * d2J);
   }
   __syncthreads();
-  // update on the tile's rows
-  double *sA = d.sA + so + o, *sB = d.sB + so + o;
+  // update on the tile's rows: the FP64 field plus the flux divergence; sB
+  // (the copy the zonal pass filters) in the flux type
+  double *sA = d.sA + so + o;
+  T *sB = reinterpret_cast<T *>(d.sB) + so + o;
   for (int r = r0; r < MER_TJ; r += RS) {
     const int J = J0 + r;
     if (J > nj) break;
-    const double f0 = (J >= 1) ? tff[r * CW + c] : 0., f1 = (J + 1 <= nj) ? tff[(r + 1) * CW + c] : 0.;
-    const double v = a(J) + (f0 - f1) * d.mn[2 * J + 1] * DT;
+    const T f0 = (J >= 1) ? tff[r * CW + c] : T(0.), f1 = (J + 1 <= nj) ? tff[(r + 1) * CW + c] : T(0.);
+    const double v = A[ix2(d, J, I)] + (double)((f0 - f1) * T(d.mn[2 * J + 1]) * DT);
     sA[ix2(d, J, I)] = v;
-    sB[ix2(d, J, I)] = v;
-    if (I == d.ni) sB[ix2(d, J, 0)] = v;
-    if (I == 1) sB[ix2(d, J, d.ni + 1)] = v;
+    sB[ix2(d, J, I)] = T(v);
+    if (I == d.ni) sB[ix2(d, J, 0)] = T(v);
+    if (I == 1) sB[ix2(d, J, d.ni + 1)] = T(v);
   }
 }
 
@@ -278,6 +297,7 @@ __global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, in
 // LDS with their pads: velocities, first-guess fluxes, predictor, corrected
 // fluxes and the update sA + d(flux) each run on all lanes of the row,
 // periodic in I.
+template <class T>
 __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, int nm) {
   ADV_MEMBER;
   const RowMap m = row_map(d.ni);
@@ -285,13 +305,15 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   if (which == AV_THETA && K <= d.ks - 2) return;  // uniform per block
   const bool live = Jr <= d.nj;
   const int J = live ? Jr : 0;
-  extern __shared__ double sh[];
+  extern __shared__ double sh_raw[];
+  T *sh = reinterpret_cast<T *>(sh_raw);
   const int ni = d.ni, W = ni + 2;
-  double *A = sh + (size_t)m.g * 6 * W;
-  double *UP = A + W, *UM = A + 2 * W, *FP = A + 3 * W, *AP = A + 4 * W, *FF = A + 5 * W;
-  const double *af = d.sB + so + (size_t)K * d.N2;
-  const double ninv = 1. / d.n[2 * J + 1], mn = d.mn[2 * J + 1], DT = d.dt;
-  const double amin = (which == AV_HDRY) ? d.h_min[K] : 0.;
+  T *A = sh + (size_t)m.g * 6 * W;
+  T *UP = A + W, *UM = A + 2 * W, *FP = A + 3 * W, *AP = A + 4 * W, *FF = A + 5 * W;
+  const T *af = reinterpret_cast<const T *>(d.sB) + so + (size_t)K * d.N2;
+  const double ninv = 1. / d.n[2 * J + 1];
+  const T mn = T(d.mn[2 * J + 1]), DT = T(d.dt);
+  const T amin = (which == AV_HDRY) ? T(d.h_min[K]) : T(0.);
   auto wr = [ni](int I) { return I < 1 ? I + ni : (I > ni ? I - ni : I); };
   if (live)
     for (int I = m.lane; I < W; I += m.stride) {
@@ -303,8 +325,8 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
       } else {
         u2 = d.U[ix3(d, K, J, I)];
       }
-      UM[I] = dmin(0., u2 * ninv);
-      UP[I] = dmax(0., u2 * ninv);
+      UM[I] = T(dmin(0., u2 * ninv));
+      UP[I] = T(dmax(0., u2 * ninv));
     }
   __syncthreads();
   if (live)
@@ -316,20 +338,20 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   if (live)
     for (int c = 1 + m.lane; c <= ni; c += m.stride) {
       const int cm = wr(c - 1);
-      double UPc = UP[c], UMc = UM[c], APc = AP[c], APm = AP[cm], Am = A[c - 1], Ac = A[c];
-      double fc = .5 * (UPc * (APc + Am) + UMc * (APm + Ac));
-      double d1c = APc - Am, d2c = APc - A[c + 1];
-      double d1m = APm - A[cm - 1], d2m = APm - A[cm + 1];
-      double uhp = sqrt(UPc * UP[c - 1]), uhm = -sqrt(UMc * UM[c + 1]);
-      double mu = (UPc - UMc) * mn * DT, al = (1. + mu) / 6.;
-      double t0 = fabs(A[cm - 1] - 2. * A[cm] + A[cm + 1]) + 1.e-10, q0 = t0 * t0;
-      double t1 = fabs(A[c - 1] - 2. * A[c] + A[c + 1]) + 1.e-10, q1 = t1 * t1;
-      double gp = q0 / (q0 + (Am - amin) * (Ac - amin));
+      T UPc = UP[c], UMc = UM[c], APc = AP[c], APm = AP[cm], Am = A[c - 1], Ac = A[c];
+      T fc = T(.5) * (UPc * (APc + Am) + UMc * (APm + Ac));
+      T d1c = APc - Am, d2c = APc - A[c + 1];
+      T d1m = APm - A[cm - 1], d2m = APm - A[cm + 1];
+      T uhp = sqrt(UPc * UP[c - 1]), uhm = -sqrt(UMc * UM[c + 1]);
+      T mu = (UPc - UMc) * mn * DT, al = (T(1.) + mu) / T(6.);
+      T t0 = fabs(A[cm - 1] - T(2.) * A[cm] + A[cm + 1]) + T(1.e-10), q0 = t0 * t0;
+      T t1 = fabs(A[c - 1] - T(2.) * A[c] + A[c + 1]) + T(1.e-10), q1 = t1 * t1;
+      T gp = q0 / (q0 + (Am - amin) * (Ac - amin));
       gp = gp * gp;
-      double gm = q1 / (q1 + (Am - amin) * (Ac - amin));
+      T gm = q1 / (q1 + (Am - amin) * (Ac - amin));
       gm = gm * gm;
-      double bp = 1. + (1. / (2. * al) - 1.) * gp, bm = 1. + (1. / (2. * al) - 1.) * gm;
-      double bhp = 1. - gp, bhm = 1. - gm;
+      T bp = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gp, bm = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gm;
+      T bhp = T(1.) - gp, bhm = T(1.) - gm;
       FF[c] = fc + -al * (UPc * bp * d1c - uhp * bhp * d1m + UMc * bm * d2m - uhm * bhm * d2c);
     }
   __syncthreads();
@@ -337,33 +359,40 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   double *out = adv_field(d, which);
   for (int I = 1 + m.lane; I <= ni; I += m.stride) {
     const size_t q = ix3(d, K, J, I);
-    const double y = d.sA[so + q] + (FF[I] - FF[wr(I + 1)]) * mn * DT;
+    const double y = d.sA[so + q] + (double)((FF[I] - FF[wr(I + 1)]) * mn * DT);
     out[q] = y;
     if (I == ni) out[ix3(d, K, J, 0)] = y;
     if (I == 1) out[ix3(d, K, J, ni + 1)] = y;
   }
 }
 
-// one advection pass over all three fields of every member
-static void hsu3(hipStream_t s, const Bucket &b) {
+// one advection pass over all three fields of every member; the mixed mode
+// forms the fluxes in FP32 (sB, the filtered copy, FP32 too)
+template <class T>
+static void hsu3_t(hipStream_t s, const Bucket &b) {
   const int ncol = (b.nj + 1) * (b.ni + 2), nz = ADV_FIELDS * b.nm;
   // columns per block: as many as keep the four LDS columns within 40 KB
   int C = 64;
   while (C > 16 && (size_t)4 * C * (b.nk + 2) * sizeof(double) > 40 * 1024) C /= 2;
   if (b.knob[KN_COLS_HSUV]) C = b.knob[KN_COLS_HSUV];
-  LAUNCH(k_hsu_vert, dim3((ncol + C - 1) / C, 1, nz), dim3(256), sizeof(double) * 4 * C * (b.nk + 2), s, b.D, b.mb,
+  LAUNCH(k_hsu_vert<T>, dim3((ncol + C - 1) / C, 1, nz), dim3(256), sizeof(T) * 4 * C * (b.nk + 2), s, b.D, b.mb,
          b.nm, C);
   const int CW = b.ni < 64 ? b.ni : 64, tiles = (b.nj + 1 + MER_TJ - 1) / MER_TJ;
-  LAUNCH(k_hsu_mer, dim3(tiles * (b.ni / CW), b.nk, nz), dim3(256),
-         sizeof(double) * CW * ((MER_TJ + 4) + 2 * (MER_TJ + 3) + 2 * (MER_TJ + 2) + (MER_TJ + 1)), s, b.D, b.mb,
-         b.nm, CW);
+  LAUNCH(k_hsu_mer<T>, dim3(tiles * (b.ni / CW), b.nk, nz), dim3(256),
+         sizeof(T) * CW * ((MER_TJ + 4) + 2 * (MER_TJ + 3) + 2 * (MER_TJ + 2) + (MER_TJ + 1)), s, b.D, b.mb, b.nm,
+         CW);
   // the zonal pass reads a zonally filtered copy (the reference filters it
   // per level before the zonal sweep); a separate launch packs the rows of
   // the filter band, which is faster here than filtering inside the pass
-  L_filter(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
+  if (sizeof(T) == sizeof(double)) L_filter(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
+  else L_filter_f32(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
   const int rpb = rows_per_block(b.ni);
-  LAUNCH(k_hsu_zonal, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, nz), dim3(256), sizeof(double) * rpb * 6 * (b.ni + 2),
-         s, b.D, b.mb, b.nm);
+  LAUNCH(k_hsu_zonal<T>, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, nz), dim3(256), sizeof(T) * rpb * 6 * (b.ni + 2), s,
+         b.D, b.mb, b.nm);
+}
+static void hsu3(hipStream_t s, const Bucket &b) {
+  if (b.mixed) hsu3_t<float>(s, b);
+  else hsu3_t<double>(s, b);
 }
 
 // op 0: unity = 1 on K = 0..nk+1, J = 0..nj, I = 0..ni+1, and OLD = THETA

@@ -571,15 +571,17 @@ __device__ __forceinline__ HvRange hv_range(const MemberDesc &d, int kind) {
   return r;
 }
 
-// One d2dy2 application (coefficient dc) at row J, column I of a plane
-__device__ __forceinline__ double d2dy2_at(const double *hh, const MemberDesc &d, int J, int I, int jmax, double dc,
-                                           const double *cx, const double *cy, const double *cz) {
+// One d2dy2 application (coefficient dc) at row J, column I of a plane, in
+// the arithmetic type T (double: the reference's; float: the mixed mode's)
+template <class T>
+__device__ __forceinline__ T d2dy2_at(const double *hh, const MemberDesc &d, int J, int I, int jmax, T dc,
+                                      const double *cx, const double *cy, const double *cz) {
   int j0, j1, j2;
   if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
   else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
   else { j0 = J - 1; j1 = J; j2 = J + 1; }
-  double a0 = dc * cx[J], a1 = dc * cy[J], a2 = dc * cz[J];
-  return a0 * hh[ix2(d, j0, I)] + a1 * hh[ix2(d, j1, I)] + a2 * hh[ix2(d, j2, I)];
+  T a0 = dc * T(cx[J]), a1 = dc * T(cy[J]), a2 = dc * T(cz[J]);
+  return a0 * T(hh[ix2(d, j0, I)]) + a1 * T(hh[ix2(d, j1, I)]) + a2 * T(hh[ix2(d, j2, I)]);
 }
 
 // The two d2dy2 passes of del^4 (coef2, then coef4) in one launch: the
@@ -591,6 +593,7 @@ __device__ __forceinline__ double d2dy2_at(const double *hh, const MemberDesc &d
   const MemberDesc &d = D[mb + (int)blockIdx.z - fi * nm];      \
   const int f = fs.f[fi], kind = fs.kind[fi]
 
+template <class T>
 __global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
   const int vg = (kind == 1);
@@ -600,35 +603,37 @@ __global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, in
   const int K = r.k0 + blockIdx.y;
   if (K > r.k1) return;
   const double *hh = fld_ptr(d, f) + (size_t)K * d.N2;
-  double *out = d.sB + (size_t)fi * d.N3 + (size_t)K * d.N2;
+  T *out = reinterpret_cast<T *>(d.sB) + (size_t)fi * d.N3 + (size_t)K * d.N2;
   const double *cx = vg ? d.vcx : d.hcx, *cy = vg ? d.vcy : d.hcy, *cz = vg ? d.vcz : d.hcz;
   int j0, j1, j2;
   if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
   else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
   else { j0 = J - 1; j1 = J; j2 = J + 1; }
-  const double c2 = d.coef2, c4 = d.coef4;
-  const double p0 = d2dy2_at(hh, d, j0, I, jmax, c2, cx, cy, cz);
-  const double p1 = d2dy2_at(hh, d, j1, I, jmax, c2, cx, cy, cz);
-  const double p2 = d2dy2_at(hh, d, j2, I, jmax, c2, cx, cy, cz);
-  double a0 = c4 * cx[J], a1 = c4 * cy[J], a2 = c4 * cz[J];
+  const T c2 = T(d.coef2), c4 = T(d.coef4);
+  const T p0 = d2dy2_at(hh, d, j0, I, jmax, c2, cx, cy, cz);
+  const T p1 = d2dy2_at(hh, d, j1, I, jmax, c2, cx, cy, cz);
+  const T p2 = d2dy2_at(hh, d, j2, I, jmax, c2, cx, cy, cz);
+  T a0 = c4 * T(cx[J]), a1 = c4 * T(cy[J]), a2 = c4 * T(cz[J]);
   out[ix2(d, J, I)] = a0 * p0 + a1 * p1 + a2 * p2;
 }
 
 // Rows of the del^4 result D (copy i of sB, 1024/ni rows per block): D is
 // zonally filtered in LDS (rows of the filter band), then A += DT*(-1)*D on
 // the kind's rows and the zonal pads of every row 0..nj+1 are written
+template <class T>
 __global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
-  extern __shared__ double lds[];
+  extern __shared__ double lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
   const RowMap m = row_map(d.ni);
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y, Jr = blockIdx.x * m.rpb + m.g;
   if (K > r.k1 || d.nu4 == 0.) return;  // uniform per block
   const bool live = Jr <= d.nj + 1;
   const int J = live ? Jr : 0, ni = d.ni, vg = (kind == 1);
-  double *x = lds + (size_t)m.g * 2 * ni, *work = x + ni;
+  T *x = lds + (size_t)m.g * 2 * ni, *work = x + ni;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
-  const double *dd = d.sB + (size_t)fi * d.N3 + (size_t)K * d.N2;
+  const T *dd = reinterpret_cast<const T *>(d.sB) + (size_t)fi * d.N3 + (size_t)K * d.N2;
   const bool upd = live && J >= r.jbot && J <= d.nj;
   if (upd)
     for (int I = 1 + m.lane; I <= ni; I += m.stride) x[I - 1] = dd[ix2(d, J, I)];
@@ -639,7 +644,7 @@ __global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, i
   for (int I = 1 + m.lane; I <= ni; I += m.stride) {
     double v = a[ix2(d, J, I)];
     if (upd) {
-      v += d.dt * (double)(-1) * x[I - 1];
+      v += d.dt * (double)(-1) * (double)x[I - 1];
       a[ix2(d, J, I)] = v;
     }
     if (I == d.ni) a[ix2(d, J, 0)] = v;
@@ -647,16 +652,22 @@ __global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, i
   }
 }
 
-void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
+// the mixed mode forms del^4 in FP32 (the field and its update stay FP64)
+template <class T>
+static void merid_hv_t(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   bool any_v = false;
   for (int i = 0; i < fs.n; i++) any_v |= (fs.kind[i] == 1);
   // rows 0..nj (+ nj+1 on the v grid); the kernel drops rows past its kind's
   const int jrows = any_v ? b.nj + 2 : b.nj + 1;
   dim3 g = grid2(jrows, b.ni, b.nk, fs.n * b.nm);
-  LAUNCH(k_d2dy2x2, g, dim3(TPB), 0, s, b.D, b.mb, b.nm, fs);
+  LAUNCH(k_d2dy2x2<T>, g, dim3(TPB), 0, s, b.D, b.mb, b.nm, fs);
   const int rpb = rows_per_block(b.ni);
-  LAUNCH(k_hv_apply, dim3((b.nj + 2 + rpb - 1) / rpb, b.nk, fs.n * b.nm), dim3(256), sizeof(double) * rpb * 2 * b.ni,
-         s, b.D, b.mb, b.nm, fs);
+  LAUNCH(k_hv_apply<T>, dim3((b.nj + 2 + rpb - 1) / rpb, b.nk, fs.n * b.nm), dim3(256), sizeof(T) * rpb * 2 * b.ni, s,
+         b.D, b.mb, b.nm, fs);
+}
+void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
+  if (b.mixed) merid_hv_t<float>(s, b, fs);
+  else merid_hv_t<double>(s, b, fs);
 }
 
 // zonal hyperviscosity: a block holds RB consecutive (K,J) rows in LDS, each
@@ -665,51 +676,53 @@ void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
 // summing row r (so one chain of additions serves RB rows).  Rows are
 // strided by an odd number of doubles so the lanes of the serial phase fall
 // in distinct LDS banks.
+template <class T>
 __global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, int nm, FieldSet fs, int RB) {
   SET_MEMBER;
-  extern __shared__ double lds[];
+  extern __shared__ double lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
   const int ni = d.ni, W = ni + 2, WS = W | 1;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y;
   if (K > r.k1 || d.nu4 == 0.) return;  // uniform over the block
-  double *HH = lds, *DD = lds + (size_t)RB * WS, *meanv = DD + (size_t)RB * WS;
+  T *HH = lds, *DD = lds + (size_t)RB * WS, *meanv = DD + (size_t)RB * WS;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
   const int J0 = blockIdx.x * RB, jlast = d.nj + 1;
   for (int t = wv; t < RB; t += 4) {
     const int J = J0 + t;
-    double *hh = HH + t * WS;
-    for (int I = lane; I < W; I += 64) hh[I] = (J <= jlast) ? a[ix2(d, J, I)] : 0.;
+    T *hh = HH + t * WS;
+    for (int I = lane; I < W; I += 64) hh[I] = (J <= jlast) ? T(a[ix2(d, J, I)]) : T(0.);
   }
-  double dc[2][4];
+  T dc[2][4];
   for (int u = 0; u < RB / 4; u++) {
     const int Jc = min(J0 + wv + 4 * u, d.nj);  // rows past nj are never applied
     const double m = d.m[2 * Jc + r.jh];
     const bool taper = fabs(d.lat[2 * Jc + r.jh]) > 45.;
-    dc[0][u] = taper ? d.coef2 * d.m0z * d.m0z : d.coef2 * m * m;
-    dc[1][u] = taper ? d.coef4 * d.m0z * d.m0z : d.coef4 * m * m;
+    dc[0][u] = T(taper ? d.coef2 * d.m0z * d.m0z : d.coef2 * m * m);
+    dc[1][u] = T(taper ? d.coef4 * d.m0z * d.m0z : d.coef4 * m * m);
   }
   __syncthreads();
   for (int pass = 0; pass < 2; pass++) {
-    double *SRC = pass ? DD : HH, *DST = pass ? HH : DD;
+    T *SRC = pass ? DD : HH, *DST = pass ? HH : DD;
     for (int u = 0; u < RB / 4; u++) {
       const int t = wv + 4 * u;
-      const double *src = SRC + t * WS;
-      double *dst = DST + t * WS;
-      for (int I = 1 + lane; I <= ni; I += 64) dst[I] = dc[pass][u] * (src[I - 1] - 2. * src[I] + src[I + 1]);
+      const T *src = SRC + t * WS;
+      T *dst = DST + t * WS;
+      for (int I = 1 + lane; I <= ni; I += 64) dst[I] = dc[pass][u] * (src[I - 1] - T(2.) * src[I] + src[I + 1]);
     }
     __syncthreads();
     if (wv == 0 && lane < RB) {
-      double *dst = DST + lane * WS;
+      T *dst = DST + lane * WS;
       dst[ni + 1] = dst[1];
       dst[0] = dst[ni];
-      meanv[lane] = seq_acc(dst + 1, ni) / ni;
+      meanv[lane] = seq_acc(dst + 1, ni) / T(ni);
     }
     __syncthreads();
     for (int u = 0; u < RB / 4; u++) {
       const int t = wv + 4 * u;
-      double *dst = DST + t * WS;
-      const double fm = meanv[t];
+      T *dst = DST + t * WS;
+      const T fm = meanv[t];
       for (int I = lane; I < W; I += 64) dst[I] -= fm;
     }
     __syncthreads();
@@ -720,10 +733,10 @@ __global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, i
     const int t = wv + 4 * u, J = J0 + t;
     if (J > jlast) break;
     const bool upd = J >= r.jbot && J <= d.nj;
-    const double *hh = HH + t * WS;
+    const T *hh = HH + t * WS;
     for (int I = 1 + lane; I <= ni; I += 64) {
       double v = a[ix2(d, J, I)];
-      if (upd) v += d.dt * (double)(-1) * hh[I];
+      if (upd) v += d.dt * (double)(-1) * (double)hh[I];
       a[ix2(d, J, I)] = v;
       if (I == ni) a[ix2(d, J, 0)] = v;
       if (I == 1) a[ix2(d, J, ni + 1)] = v;
@@ -740,7 +753,10 @@ void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
                                                       : 4;
   const int WS = (b.ni + 2) | 1;
   dim3 g((b.nj + 2 + RB - 1) / RB, b.nk, fs.n * b.nm);
-  LAUNCH(k_zonal_hv, g, dim3(256), sizeof(double) * (2 * RB * WS + RB), s, b.D, b.mb, b.nm, fs, RB);
+  if (b.mixed)
+    LAUNCH(k_zonal_hv<float>, g, dim3(256), sizeof(float) * (2 * RB * WS + RB), s, b.D, b.mb, b.nm, fs, RB);
+  else
+    LAUNCH(k_zonal_hv<double>, g, dim3(256), sizeof(double) * (2 * RB * WS + RB), s, b.D, b.mb, b.nm, fs, RB);
 }
 
 }  // namespace epic

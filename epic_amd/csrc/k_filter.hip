@@ -104,8 +104,8 @@ void L_filter(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, 
   launch_fft<double>(s, b, f, k0, k1, vgrid, mode == 0, copies);
 }
 // the same on an FP32 array (mixed precision)
-void L_filter_f32(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int mode) {
-  launch_fft<float>(s, b, f, k0, k1, vgrid, mode == 0, 1);
+void L_filter_f32(hipStream_t s, const Bucket &b, int f, int k0, int k1, int vgrid, int mode, int copies) {
+  launch_fft<float>(s, b, f, k0, k1, vgrid, mode == 0, copies);
 }
 
 // ------------------------------------------------------------- 3-D form

@@ -63,7 +63,7 @@ void L_divergence(hipStream_t s, const Bucket &b, int uu, int vv, int dst, int k
 // also refreshes the zonal pads of every row, mode 1 leaves pads alone;
 // copies > 1 filters the same planes of that many consecutive 3-D copies.
 void L_filter(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode, int copies = 1);
-void L_filter_f32(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode);
+void L_filter_f32(hipStream_t s, const Bucket &b, int fld, int k0, int k1, int vgrid, int mode, int copies = 1);
 // the 3-D filter with the PCHIP profile removal, on every field of the set
 void L_filter3(hipStream_t s, const Bucket &b, const FieldSet &fs);
 
