@@ -19,28 +19,35 @@ __device__ __forceinline__ void row_sync(int nq) {
   else __syncthreads();
 }
 
-// The lane's twiddles for every stage are loaded into registers before the
-// transforms start (one round of loads instead of one per stage).
+// The stage twiddles stay in their table (global, cache-resident: the
+// stage-s twiddle m is complex entry 2^s - 1 + m); the lane's split-step
+// twiddle is held in registers.
 constexpr int MAX_STAGES = 9;  // n <= 1024: stages with 2 << s < n
 
 template <class T>
 struct LaneTw {
-  T wr[MAX_STAGES], wi[MAX_STAGES];
-  T sr, si;  // split-step twiddle of this lane
+  const T *tw;  // stage twiddles
+  T sr, si;     // split-step twiddle of this lane
 };
 
 template <class T>
 __device__ __forceinline__ void load_tw(LaneTw<T> &t, const T *tw, const T *tr, int n, int lane) {
-#pragma unroll
-  for (int s = 0; s < MAX_STAGES; s++) {
-    const int half = 1 << s, mm = lane & (half - 1);
-    const bool on = (2 << s) < n;
-    t.wr[s] = on ? tw[2 * (half - 1 + mm)] : T(0);
-    t.wi[s] = on ? tw[2 * (half - 1 + mm) + 1] : T(0);
-  }
+  t.tw = tw;
   const bool on = lane < n / 4 - 1;
   t.sr = on ? tr[2 * lane] : T(0);
   t.si = on ? tr[2 * lane + 1] : T(0);
+}
+
+// one radix-2 butterfly of the reference's four1 on (xr, xi) and (yr, yi)
+// with twiddle (wr, wi), in its operation order
+template <class T>
+__device__ __forceinline__ void bfly(T &xr, T &xi, T &yr, T &yi, T wr, T wi) {
+  const T tr = wr * yr - wi * yi;
+  const T ti = wr * yi + wi * yr;
+  yr = xr - tr;
+  yi = xi - ti;
+  xr += tr;
+  xi += ti;
 }
 
 // twiddle tables and field views of a storage type
@@ -57,24 +64,56 @@ template <> __device__ __forceinline__ const float *tw_inv<float>(const MemberDe
 template <> __device__ __forceinline__ const float *tr_fwd<float>(const MemberDesc &d) { return d.tr_f32; }
 template <> __device__ __forceinline__ const float *tr_inv<float>(const MemberDesc &d) { return d.tr_i32; }
 
-// complex radix-2 stages on buf[0..n) (n doubles = n/2 complex), one
-// butterfly per lane, in the reference's stage order
+// complex radix-2 stages on buf[0..n) (n doubles = n/2 complex), in the
+// reference's stage order.  Stages go two at a time: a lane loads the four
+// points p, p+h, p+2h, p+3h (h = 2^s) into registers, applies stage s to
+// (p, p+h) and (p+2h, p+3h) and stage s+1 to (p, p+2h) and (p+h, p+3h), and
+// stores them: every point sees the same butterflies in the same order as
+// with one stage per pass, with half the LDS round trips and barriers.  n/8
+// lanes take part in a pair of stages; an odd last stage runs one butterfly
+// per lane on n/4 lanes.
 template <class T>
 __device__ __forceinline__ void four1_stages(T *buf, int n, int lane, const LaneTw<T> &t, bool active) {
-#pragma unroll
-  for (int s = 0; s < MAX_STAGES; s++) {
-    if ((2 << s) >= n) break;
+  const T *tw = t.tw;
+  int S = 0;
+  while ((2 << S) < n) S++;  // stages s with 2 << s < n
+  int s = 0;
+  for (; s + 1 < S; s += 2) {
+    const int half = 1 << s;
+    if (active && lane < (n >> 3)) {
+      const int mm = lane & (half - 1), g = lane >> s;
+      const int i0 = 2 * (mm + g * 4 * half), i1 = i0 + 2 * half, i2 = i1 + 2 * half, i3 = i2 + 2 * half;
+      T x0r = buf[i0], x0i = buf[i0 + 1], x1r = buf[i1], x1i = buf[i1 + 1];
+      T x2r = buf[i2], x2i = buf[i2 + 1], x3r = buf[i3], x3i = buf[i3 + 1];
+      const int ka = half - 1 + mm, kb = 2 * half - 1 + mm;
+      const T ar = tw[2 * ka], ai = tw[2 * ka + 1];
+      const T br = tw[2 * kb], bi = tw[2 * kb + 1], cr = tw[2 * (kb + half)], ci = tw[2 * (kb + half) + 1];
+      bfly(x0r, x0i, x1r, x1i, ar, ai);
+      bfly(x2r, x2i, x3r, x3i, ar, ai);
+      bfly(x0r, x0i, x2r, x2i, br, bi);
+      bfly(x1r, x1i, x3r, x3i, cr, ci);
+      buf[i0] = x0r;
+      buf[i0 + 1] = x0i;
+      buf[i1] = x1r;
+      buf[i1 + 1] = x1i;
+      buf[i2] = x2r;
+      buf[i2 + 1] = x2i;
+      buf[i3] = x3r;
+      buf[i3 + 1] = x3i;
+    }
+    row_sync(n >> 2);
+  }
+  if (s < S) {
     const int mmax = 2 << s, half = 1 << s;
     if (active) {
-      int g = lane >> s, mm = lane & (half - 1);
-      int i0 = 2 * mm + g * 2 * mmax, j0 = i0 + mmax;
-      T wr = t.wr[s], wi = t.wi[s];
-      T tr = wr * buf[j0] - wi * buf[j0 + 1];
-      T ti = wr * buf[j0 + 1] + wi * buf[j0];
-      buf[j0] = buf[i0] - tr;
-      buf[j0 + 1] = buf[i0 + 1] - ti;
-      buf[i0] += tr;
-      buf[i0 + 1] += ti;
+      const int g = lane >> s, mm = lane & (half - 1);
+      const int i0 = 2 * mm + g * 2 * mmax, j0 = i0 + mmax, k = half - 1 + mm;
+      T xr = buf[i0], xi = buf[i0 + 1], yr = buf[j0], yi = buf[j0 + 1];
+      bfly(xr, xi, yr, yi, tw[2 * k], tw[2 * k + 1]);
+      buf[i0] = xr;
+      buf[i0 + 1] = xi;
+      buf[j0] = yr;
+      buf[j0 + 1] = yi;
     }
     row_sync(n >> 2);
   }
