@@ -184,13 +184,18 @@ static int rebuild_buckets(epic_engine *E) {
 }
 
 // store_pgrad_vars (src/shared/epic_funcs_diag.c:3595-3950)
-// set_p2etc then store_pgrad_vars
-static void rec_p2etc_pgrad_vars(hipStream_t s, const Bucket &b, int update_theta) {
-  L_set_p2etc(s, b, update_theta);
-  L_thermo(s, b);
+// set_p2etc then store_pgrad_vars.  full = 1: everything.  full = 0 (the
+// pair that follows the advection inside the step, with DONT_UPDATE_THETA):
+// of what it writes, the step reads only P3, h, T3, RHO3, GZ_SURFACE and GZ3
+// (through the filtered GZ3 copy of the 3-D filter) before the second pair
+// writes every output again, so the first forms those only and leaves out
+// P2, H3, EXNER3, T2, RHO2 and DZDT3 -- the step's results are the same.
+static void rec_p2etc_pgrad_vars(hipStream_t s, const Bucket &b, int full) {
+  L_set_p2etc(s, b, full);
+  L_thermo(s, b, full);
   L_gz_from_u(s, b);
   L_gz3(s, b);
-  L_dzdt3(s, b);
+  if (full) L_dzdt3(s, b);
 }
 
 // store_diag (src/shared/epic_funcs_diag.c:3963-4053); the filtered copies

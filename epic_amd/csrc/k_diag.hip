@@ -115,8 +115,14 @@ static inline dim3 tile_grid(const Bucket &b, int C) {
 // P3 from the top down (prefix of dsgth*g*h), the sigma-region pressures,
 // h = max(h_min, dp/(g dsgth)), and the layer diagnostics P2, H3, THETA2
 // (and the theta of the sigma region when update_theta)
-template <int update_theta>
+// FULL = 1: set_p2etc(UPDATE_THETA) with P2, H3, THETA2.  FULL = 0: the
+// call that follows the advection inside the step forms P3 and h (and the
+// pressure-range check) only: the P2 and H3 it would form are written again
+// by the step's second set_p2etc before anything reads them, so the values
+// the step leaves are the same.
+template <int FULL>
 __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, int C) {
+  constexpr bool update_theta = FULL;
   MEMBER;
   extern __shared__ double lds[];
   const Tile T = tile_of(d, C);
@@ -142,7 +148,7 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
   __syncthreads();
   // p^(kappa+1) once per interface (each is the top of one layer and the
   // bottom of the next)
-  if (T.live)
+  if (FULL && T.live)
     for (int K = T.kk; K <= nk; K += T.KS) pw[K] = pow(a[K], d.kappa + 1.);
   __syncthreads();
   if (T.live) {
@@ -161,6 +167,10 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
       if (K == 0) continue;
       double hl = (a[K] - a[K - 1]) / (g * d.dsgth[2 * K]);
       H[q] = dmax(d.h_min[K], hl);
+      if (!FULL) {
+        if (fcmp_d(a[K], a[K - 1]) != 0 && (a[K - 1] < 0. || a[K] < 0.)) atomicOr(&d.dyn->err, 2);
+        continue;
+      }
       double top = a[K - 1], bot = a[K], pp;
       if (fcmp_d(bot, top) == 0) pp = .5 * (bot + top);
       else if (top < 0. || bot < 0.) { atomicOr(&d.dyn->err, 2); pp = 0.; }
@@ -194,7 +204,7 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
     }
   }
   __syncthreads();
-  if (T.live && T.kk == 0) {
+  if (FULL && T.live && T.kk == 0) {
     const size_t q0 = ix3(d, 0, J, I);
     d.P2[q0] = a[0] * a[0] / aux[2];
     d.H3[q0] = aux[1];
@@ -212,9 +222,12 @@ void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
 // T3, EXNER3, RHO3, T2, RHO2 per cell and the top values; DZDT3 <- GZ3 when
 // itime > 1, DZDT3 zeroed whole while itime <= 1
 // (src/shared/epic_funcs_diag.c:3674-3690).  One lane per (K,J,I), K = 0..nk.
-__global__ void __launch_bounds__(256) k_thermo(const MemberDesc *D, int mb) {
+// full = 0 (the step's first store_pgrad_vars): T3 and RHO3 only, all that
+// the geopotential integration and the filtered GZ3 copy read before the
+// second call writes every output again.
+__global__ void __launch_bounds__(256) k_thermo(const MemberDesc *D, int mb, int full) {
   MEMBER;
-  if (d.dyn->itime <= 1) {
+  if (full && d.dyn->itime <= 1) {
     // no lane of this kernel writes DZDT3 while itime <= 1
     const size_t nthr = (size_t)gridDim.x * gridDim.y * blockDim.x;
     for (size_t i = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; i < (size_t)d.N3;
@@ -224,14 +237,15 @@ __global__ void __launch_bounds__(256) k_thermo(const MemberDesc *D, int mb) {
   FLAT2(0, d.nj + 1, 0, d.ni + 2);
   const int K = blockIdx.y;
   const size_t q = ix3(d, K, J, I);
-  if (K >= 1 && d.dyn->itime > 1 && I >= 1 && I <= d.ni) d.DZDT3[q] = d.GZ3[q];
+  if (full && K >= 1 && d.dyn->itime > 1 && I >= 1 && I <= d.ni) d.DZDT3[q] = d.GZ3[q];
   const double R_GAS = 8.314472e+3;
   double th = d.THETA[q], p = d.P3[q];
   double t = th * pow(p / d.press0, d.kappa);
   double rho3 = p * d.mu3 / (R_GAS * t);
   d.T3[q] = t;
-  d.EXNER3[q] = d.cp * t / th;
   d.RHO3[q] = rho3;
+  if (!full) return;
+  d.EXNER3[q] = d.cp * t / th;
   if (K >= 1) {
     double p2 = d.P2[q], th2 = d.THETA2[q];
     double t2 = th2 * pow(p2 / d.press0, d.kappa);
@@ -247,8 +261,8 @@ __global__ void __launch_bounds__(256) k_thermo(const MemberDesc *D, int mb) {
     d.T2[q] = t;
   }
 }
-void L_thermo(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_thermo, grid2(b.nj + 1, b.ni + 2, b.nk + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+void L_thermo(hipStream_t s, const Bucket &b, int full) {
+  LAUNCH(k_thermo, grid2(b.nj + 1, b.ni + 2, b.nk + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb, full);
 }
 
 // ------------------------------------------------------------ RI2

@@ -46,7 +46,7 @@ struct Bucket {
 // ---- diagnostics (k_diag.hip)
 void L_fill(hipStream_t s, const Bucket &b, int fld, double v);
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta);
-void L_thermo(hipStream_t s, const Bucket &b);
+void L_thermo(hipStream_t s, const Bucket &b, int full);
 void L_richardson(hipStream_t s, const Bucket &b);
 void L_gz_from_u(hipStream_t s, const Bucket &b);
 void L_gz3(hipStream_t s, const Bucket &b);
