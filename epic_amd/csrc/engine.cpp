@@ -905,7 +905,8 @@ extern "C" int epic_engine_restore(epic_engine *E, const epic_case *c, const cha
 // the end, so tuning leaves the state (and the step counts) as it found
 // them; the shapes never change a result.  The graph is re-captured with
 // the chosen shapes at the next step.
-static const char *const knob_name[KN_COUNT] = {"cols_p2etc", "cols_gz3", "cols_calcw", "cols_hsuv", "rows_zhv"};
+static const char *const knob_name[KN_COUNT] = {"cols_p2etc", "cols_gz3", "cols_calcw",
+                                                "cols_hsuv",  "rows_zhv", "rows_mer"};
 
 static bool knob_fits(const Bucket &b, int k, int v) {
   const size_t W = (size_t)b.nk + 1, L = (size_t)b.nk + 2, lim = 128 * 1024;
@@ -914,12 +915,14 @@ static bool knob_fits(const Bucket &b, int k, int v) {
     case KN_COLS_GZ3: return sizeof(double) * 3 * v * W <= lim;
     case KN_COLS_CALCW: return sizeof(double) * 4 * v * W <= lim;
     case KN_COLS_HSUV: return sizeof(double) * 4 * v * L <= lim;
+    case KN_ROWS_MER: return sizeof(double) * (size_t)(b.ni < 64 ? b.ni : 64) * (6 * v + 15) <= lim;
     default: return sizeof(double) * (2 * (size_t)v * ((b.ni + 2) | 1) + v) <= lim;
   }
 }
 
 static bool knob_times(const char *kernel, int k) {
-  static const char *const pfx[KN_COUNT] = {"k_set_p2etc", "k_gz3", "k_calc_w", "k_hsu_vert", "k_zonal_hv"};
+  static const char *const pfx[KN_COUNT] = {"k_set_p2etc", "k_gz3", "k_calc_w", "k_hsu_vert", "k_zonal_hv",
+                                            "k_hsu_mer"};
   return strncmp(kernel, pfx[k], strlen(pfx[k])) == 0;
 }
 
@@ -953,7 +956,8 @@ extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int
       M.steps = done[i];
     }
   };
-  static const int cand[KN_COUNT][3] = {{16, 32, 64}, {16, 32, 64}, {16, 32, 64}, {16, 32, 64}, {4, 8, 16}};
+  static const int cand[KN_COUNT][3] = {{16, 32, 64}, {16, 32, 64}, {16, 32, 64},
+                                        {16, 32, 64}, {4, 8, 16},   {2, 4, 8}};
   size_t used = 0;
   for (Bucket &b : E->buckets) {
     bool hv, divd, adj;

@@ -192,17 +192,16 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
 }
 
 // ---------------------------------------------------------- meridional
-// One launch: a block holds a tile of MER_TJ rows x CW columns of one plane
-// of one field in LDS, with halos: the field on rows J0-2..J0+MER_TJ+1, the
+// One launch: a block holds a tile of TJ rows x CW columns of one plane of
+// one field in LDS, with halos: the field on rows J0-2..J0+TJ+1, the
 // meridional velocities vp, vm (v times 1/m, split by sign) on v-rows
-// J0-1..J0+MER_TJ+1, the smoothness terms aq and the predictor ap on rows
-// J0-1..J0+MER_TJ and the corrected flux on v-rows J0..J0+MER_TJ, each formed
-// once per cell by the reference's expression (the halo rows are formed again
-// by the neighbouring tile: the same values).  The update goes to sA and to
-// its copy sB (which the zonal pass filters), with sB's zonal pads.
-constexpr int MER_TJ = 8;
+// J0-1..J0+TJ+1, the smoothness terms aq and the predictor ap on rows
+// J0-1..J0+TJ and the corrected flux on v-rows J0..J0+TJ, each formed once per
+// cell by the reference's expression (the halo rows are formed again by the
+// neighbouring tile: the same values).  The update goes to sA and to its copy
+// sB (which the zonal pass filters), with sB's zonal pads.
 template <class T>
-__global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, int nm, int CW) {
+__global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, int nm, int CW, int MER_TJ) {
   ADV_MEMBER;
   const int K = 1 + blockIdx.y;
   if (which == AV_THETA && K <= d.ks - 2) return;  // uniform per block
@@ -212,7 +211,7 @@ __global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, in
   const int tile = blockIdx.x / nchunk, chunk = blockIdx.x - tile * nchunk;
   const int J0 = tile * MER_TJ, c = threadIdx.x % CW, r0 = threadIdx.x / CW, RS = blockDim.x / CW;
   const int I = 1 + chunk * CW + c;
-  constexpr int RA = MER_TJ + 4, RV = MER_TJ + 3, RP = MER_TJ + 2;
+  const int RA = MER_TJ + 4, RV = MER_TJ + 3, RP = MER_TJ + 2;
   T *ta = sh, *tvp = ta + RA * CW, *tvm = tvp + RV * CW, *taq = tvm + RV * CW, *tap = taq + RP * CW;
   T *tff = tap + RP * CW;
   const size_t o = (size_t)K * d.N2;
@@ -377,10 +376,10 @@ static void hsu3_t(hipStream_t s, const Bucket &b) {
   if (b.knob[KN_COLS_HSUV]) C = b.knob[KN_COLS_HSUV];
   LAUNCH(k_hsu_vert<T>, dim3((ncol + C - 1) / C, 1, nz), dim3(256), sizeof(T) * 4 * C * (b.nk + 2), s, b.D, b.mb,
          b.nm, C);
-  const int CW = b.ni < 64 ? b.ni : 64, tiles = (b.nj + 1 + MER_TJ - 1) / MER_TJ;
+  const int TJ = b.knob[KN_ROWS_MER] ? b.knob[KN_ROWS_MER] : 4;
+  const int CW = b.ni < 64 ? b.ni : 64, tiles = (b.nj + 1 + TJ - 1) / TJ;
   LAUNCH(k_hsu_mer<T>, dim3(tiles * (b.ni / CW), b.nk, nz), dim3(256),
-         sizeof(T) * CW * ((MER_TJ + 4) + 2 * (MER_TJ + 3) + 2 * (MER_TJ + 2) + (MER_TJ + 1)), s, b.D, b.mb, b.nm,
-         CW);
+         sizeof(T) * CW * ((TJ + 4) + 2 * (TJ + 3) + 2 * (TJ + 2) + (TJ + 1)), s, b.D, b.mb, b.nm, CW, TJ);
   // the zonal pass reads a zonally filtered copy (the reference filters it
   // per level before the zonal sweep); a separate launch packs the rows of
   // the filter band, which is faster here than filtering inside the pass

@@ -31,7 +31,7 @@ struct FieldSet {
 // Launch shapes a bucket can be tuned over (epic_engine_autotune); 0 takes
 // the built-in choice.  None of them changes a result: every cell is formed
 // by the same operations whatever the tile.
-enum Knob { KN_COLS_P2ETC, KN_COLS_GZ3, KN_COLS_CALCW, KN_COLS_HSUV, KN_ROWS_ZHV, KN_COUNT };
+enum Knob { KN_COLS_P2ETC, KN_COLS_GZ3, KN_COLS_CALCW, KN_COLS_HSUV, KN_ROWS_ZHV, KN_ROWS_MER, KN_COUNT };
 
 struct Bucket {
   const MemberDesc *D;  // device array of descriptors

@@ -311,7 +311,8 @@ int  epic_engine_restore_spec(epic_engine *e, const epic_case_spec *spec, const 
 /* Autotuner (no counterpart in the reference, whose loops have no launch
  * shapes): for each bucket, time every candidate launch shape of the tunable
  * kernels (columns per column tile of set_p2etc, gz3, calc_w and the
- * vertical advection pass; rows per zonal-hyperviscosity block) over
+ * vertical advection pass; rows per zonal-hyperviscosity block and per
+ * meridional-advection tile) over
  * `steps` steps of the bucket's own members and keep the fastest.  The
  * members' state is copied aside and put back, so the call changes no
  * result and no state; it needs room for one copy of every member.  Writes

@@ -104,7 +104,7 @@ def test_autotune_changes_no_state_and_no_result():
     before = {k: {f: e.read(s, f) for f in PROG} for k, s in slots.items()}
     rep = e.autotune(1)
     say(rep)
-    assert rep.count("\n") == 3 * 5  # three buckets, five shapes each
+    assert rep.count("\n") == 3 * 6  # three buckets, six shapes each
     for k, s in slots.items():
         assert e.steps_done(s) == 2
         for f in PROG:
