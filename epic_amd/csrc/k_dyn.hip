@@ -571,50 +571,65 @@ __device__ __forceinline__ HvRange hv_range(const MemberDesc &d, int kind) {
   return r;
 }
 
-// One d2dy2 application (coefficient dc) at row J, column I of a plane, in
-// the arithmetic type T (double: the reference's; float: the mixed mode's)
-template <class T>
-__device__ __forceinline__ T d2dy2_at(const double *hh, const MemberDesc &d, int J, int I, int jmax, T dc,
-                                      const double *cx, const double *cy, const double *cz) {
-  int j0, j1, j2;
-  if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
-  else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
-  else { j0 = J - 1; j1 = J; j2 = J + 1; }
-  T a0 = dc * T(cx[J]), a1 = dc * T(cy[J]), a2 = dc * T(cz[J]);
-  return a0 * T(hh[ix2(d, j0, I)]) + a1 * T(hh[ix2(d, j1, I)]) + a2 * T(hh[ix2(d, j2, I)]);
-}
-
-// The two d2dy2 passes of del^4 (coef2, then coef4) in one launch: the
-// first-pass values a row needs (rows j0..j2) are recomputed by the lane
-// with the same expression instead of being staged in a scratch plane.
 // field i of a set and its member
 #define SET_MEMBER                                              \
   const int fi = (int)blockIdx.z / nm;                          \
   const MemberDesc &d = D[mb + (int)blockIdx.z - fi * nm];      \
   const int f = fs.f[fi], kind = fs.kind[fi]
 
+// The two d2dy2 passes of del^4 (coef2, then coef4) in one launch: a block
+// holds a tile of D2_TJ rows x CW columns of one plane; the field on rows
+// J0-3..J0+D2_TJ+2 and the first pass on rows J0-2..J0+D2_TJ+1 (each row
+// clipped to 0..jmax) are staged in LDS, so each first-pass value is formed
+// once per tile (the halo rows again by the neighbouring tile: the same
+// values) by the same expression.
+constexpr int D2_TJ = 8;
 template <class T>
-__global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+__global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, int nm, FieldSet fs, int CW) {
   SET_MEMBER;
+  extern __shared__ double lds_raw[];
+  T *th = reinterpret_cast<T *>(lds_raw), *tp = th + (D2_TJ + 6) * CW;
   const int vg = (kind == 1);
   const int jmax = vg ? d.nj + 1 : d.nj;
-  FLAT2(0, jmax + 1, 1, d.ni);
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y;
-  if (K > r.k1) return;
+  if (K > r.k1) return;  // uniform per block
+  const int nchunk = d.ni / CW, tile = blockIdx.x / nchunk, chunk = blockIdx.x - tile * nchunk;
+  const int J0 = tile * D2_TJ;
+  if (J0 > jmax) return;  // uniform per block (the grid covers the v rows)
+  const int c = threadIdx.x % CW, r0 = threadIdx.x / CW, RS = blockDim.x / CW, I = 1 + chunk * CW + c;
   const double *hh = fld_ptr(d, f) + (size_t)K * d.N2;
   T *out = reinterpret_cast<T *>(d.sB) + (size_t)fi * d.N3 + (size_t)K * d.N2;
   const double *cx = vg ? d.vcx : d.hcx, *cy = vg ? d.vcy : d.hcy, *cz = vg ? d.vcz : d.hcz;
-  int j0, j1, j2;
-  if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
-  else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
-  else { j0 = J - 1; j1 = J; j2 = J + 1; }
+  auto rows = [jmax](int J, int &j0, int &j1, int &j2) {
+    if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
+    else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
+    else { j0 = J - 1; j1 = J; j2 = J + 1; }
+  };
+  for (int q = r0; q < D2_TJ + 6; q += RS) {
+    const int J = J0 - 3 + q;
+    th[q * CW + c] = (J >= 0 && J <= jmax) ? T(hh[ix2(d, J, I)]) : T(0.);
+  }
+  __syncthreads();
   const T c2 = T(d.coef2), c4 = T(d.coef4);
-  const T p0 = d2dy2_at(hh, d, j0, I, jmax, c2, cx, cy, cz);
-  const T p1 = d2dy2_at(hh, d, j1, I, jmax, c2, cx, cy, cz);
-  const T p2 = d2dy2_at(hh, d, j2, I, jmax, c2, cx, cy, cz);
-  T a0 = c4 * T(cx[J]), a1 = c4 * T(cy[J]), a2 = c4 * T(cz[J]);
-  out[ix2(d, J, I)] = a0 * p0 + a1 * p1 + a2 * p2;
+  for (int q = r0; q < D2_TJ + 4; q += RS) {
+    const int J = J0 - 2 + q;
+    if (J < 0 || J > jmax) continue;
+    int j0, j1, j2;
+    rows(J, j0, j1, j2);
+    T a0 = c2 * T(cx[J]), a1 = c2 * T(cy[J]), a2 = c2 * T(cz[J]);
+    tp[q * CW + c] = a0 * th[(j0 - J0 + 3) * CW + c] + a1 * th[(j1 - J0 + 3) * CW + c] + a2 * th[(j2 - J0 + 3) * CW + c];
+  }
+  __syncthreads();
+  for (int q = r0; q < D2_TJ; q += RS) {
+    const int J = J0 + q;
+    if (J > jmax) break;
+    int j0, j1, j2;
+    rows(J, j0, j1, j2);
+    const T p0 = tp[(j0 - J0 + 2) * CW + c], p1 = tp[(j1 - J0 + 2) * CW + c], p2 = tp[(j2 - J0 + 2) * CW + c];
+    T a0 = c4 * T(cx[J]), a1 = c4 * T(cy[J]), a2 = c4 * T(cz[J]);
+    out[ix2(d, J, I)] = a0 * p0 + a1 * p1 + a2 * p2;
+  }
 }
 
 // Rows of the del^4 result D (copy i of sB, 1024/ni rows per block): D is
@@ -659,8 +674,10 @@ static void merid_hv_t(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   for (int i = 0; i < fs.n; i++) any_v |= (fs.kind[i] == 1);
   // rows 0..nj (+ nj+1 on the v grid); the kernel drops rows past its kind's
   const int jrows = any_v ? b.nj + 2 : b.nj + 1;
-  dim3 g = grid2(jrows, b.ni, b.nk, fs.n * b.nm);
-  LAUNCH(k_d2dy2x2<T>, g, dim3(TPB), 0, s, b.D, b.mb, b.nm, fs);
+  int CW = 64;
+  while (b.ni % CW) CW >>= 1;
+  dim3 g((jrows + D2_TJ - 1) / D2_TJ * (b.ni / CW), b.nk, fs.n * b.nm);
+  LAUNCH(k_d2dy2x2<T>, g, dim3(256), sizeof(T) * (2 * D2_TJ + 10) * CW, s, b.D, b.mb, b.nm, fs, CW);
   const int rpb = rows_per_block(b.ni);
   LAUNCH(k_hv_apply<T>, dim3((b.nj + 2 + rpb - 1) / rpb, b.nk, fs.n * b.nm), dim3(256), sizeof(T) * rpb * 2 * b.ni, s,
          b.D, b.mb, b.nm, fs);
