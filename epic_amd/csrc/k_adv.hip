@@ -69,6 +69,11 @@ __device__ __forceinline__ T tt_iface(const T *wa, int K, int nk, T ep) {
   return tt * tt;
 }
 
+// The FP64 value a pass updates: the staged copy itself when the pass
+// computes in FP64 (the same bits), else a re-read of the FP64 field
+__device__ __forceinline__ double old_value(double staged, const double *) { return staged; }
+__device__ __forceinline__ double old_value(float, const double *p) { return *p; }
+
 template <class T>
 __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, int nm, int C) {
   ADV_MEMBER;
@@ -144,7 +149,8 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
       }
     __syncthreads();
     if (live)
-      for (int K = kf + kk; K <= nk; K += KS) b[ix3(d, K, J, I)] += (double)((ff[K] - ff[K - 1]) * T(dsi[2 * K]) * DT);
+      for (int K = kf + kk; K <= nk; K += KS)
+        b[ix3(d, K, J, I)] = old_value(wa[K], b + ix3(d, K, J, I)) + (double)((ff[K] - ff[K - 1]) * T(dsi[2 * K]) * DT);
   } else {
     // first-guess flux on layers kf..nk (zero below the bottom)
     if (live)
@@ -185,7 +191,7 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
     __syncthreads();
     if (live)
       for (int K = kf + kk; K <= nk; K += KS)
-        b[ix3(d, K, J, I)] += (double)((ff[K + 1] - ff[K]) * T(dsi[2 * K + 1]) * DT);
+        b[ix3(d, K, J, I)] = old_value(wa[K], b + ix3(d, K, J, I)) + (double)((ff[K + 1] - ff[K]) * T(dsi[2 * K + 1]) * DT);
   }
 #undef WM
 #undef WP
@@ -283,7 +289,7 @@ __global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, in
     const int J = J0 + r;
     if (J > nj) break;
     const T f0 = (J >= 1) ? tff[r * CW + c] : T(0.), f1 = (J + 1 <= nj) ? tff[(r + 1) * CW + c] : T(0.);
-    const double v = A[ix2(d, J, I)] + (double)((f0 - f1) * T(d.mn[2 * J + 1]) * DT);
+    const double v = old_value(a(J), A + ix2(d, J, I)) + (double)((f0 - f1) * T(d.mn[2 * J + 1]) * DT);
     sA[ix2(d, J, I)] = v;
     sB[ix2(d, J, I)] = T(v);
     if (I == d.ni) sB[ix2(d, J, 0)] = T(v);
