@@ -916,7 +916,8 @@ static bool knob_fits(const Bucket &b, int k, int v) {
     case KN_COLS_CALCW: return sizeof(double) * 4 * v * W <= lim;
     case KN_COLS_HSUV: return sizeof(double) * 4 * v * L <= lim;
     case KN_ROWS_MER: return sizeof(double) * (size_t)(b.ni < 64 ? b.ni : 64) * (6 * v + 15) <= lim;
-    default: return sizeof(double) * (2 * (size_t)v * ((b.ni + 2) | 1) + v) <= lim;
+    default:  // KN_ROWS_ZHV: one row array per row, and at most 16 points per lane
+      return sizeof(double) * ((size_t)v * ((b.ni + 2) | 1) + v) <= lim && v / 4 * (b.ni >= 64 ? b.ni / 64 : 1) <= 16;
   }
 }
 

@@ -687,32 +687,37 @@ void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   else merid_hv_t<double>(s, b, fs);
 }
 
-// zonal hyperviscosity: a block holds RB consecutive (K,J) rows in LDS, each
-// wavefront working on RB/4 of them along I; the zonal means are sequential
-// sums, as the reference sums them, with lane r of the first wavefront
-// summing row r (so one chain of additions serves RB rows).  Rows are
-// strided by an odd number of doubles so the lanes of the serial phase fall
-// in distinct LDS banks.
-template <class T>
-__global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, int nm, FieldSet fs, int RB) {
+// zonal hyperviscosity: a block holds RB = 4*RW consecutive (K,J) rows in
+// LDS, wavefront w owning rows w + 4u (u < RW), each lane the points
+// I = 1 + lane + 64q (q < NQ) of them, whose second differences it keeps in
+// registers; the row array is rewritten with them for the zonal means, which
+// are sequential sums, as the reference sums them, lane r of the first
+// wavefront summing row r (one chain of additions serves RB rows).  One row
+// array per row (not two) lets a CU hold twice the rows, i.e. run twice the
+// serial chains side by side.  Rows are strided by an odd number of doubles
+// so the lanes of the serial phase fall in distinct LDS banks.
+template <class T, int RW, int NQ>
+__global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
+  constexpr int RB = 4 * RW;
   extern __shared__ double lds_raw[];
-  T *lds = reinterpret_cast<T *>(lds_raw);
+  T *X = reinterpret_cast<T *>(lds_raw);
   const int ni = d.ni, W = ni + 2, WS = W | 1;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y;
   if (K > r.k1 || d.nu4 == 0.) return;  // uniform over the block
-  T *HH = lds, *DD = lds + (size_t)RB * WS, *meanv = DD + (size_t)RB * WS;
+  T *meanv = X + (size_t)RB * WS;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
   const int J0 = blockIdx.x * RB, jlast = d.nj + 1;
-  for (int t = wv; t < RB; t += 4) {
-    const int J = J0 + t;
-    T *hh = HH + t * WS;
-    for (int I = lane; I < W; I += 64) hh[I] = (J <= jlast) ? T(a[ix2(d, J, I)]) : T(0.);
+  for (int u = 0; u < RW; u++) {
+    const int t = wv + 4 * u, J = J0 + t;
+    T *x = X + t * WS;
+    for (int I = lane; I < W; I += 64) x[I] = (J <= jlast) ? T(a[ix2(d, J, I)]) : T(0.);
   }
-  T dc[2][4];
-  for (int u = 0; u < RB / 4; u++) {
+  T dc[2][RW];
+#pragma unroll
+  for (int u = 0; u < RW; u++) {
     const int Jc = min(J0 + wv + 4 * u, d.nj);  // rows past nj are never applied
     const double m = d.m[2 * Jc + r.jh];
     const bool taper = fabs(d.lat[2 * Jc + r.jh]) > 45.;
@@ -720,60 +725,106 @@ __global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, i
     dc[1][u] = T(taper ? d.coef4 * d.m0z * d.m0z : d.coef4 * m * m);
   }
   __syncthreads();
+  T y[RW][NQ];
   for (int pass = 0; pass < 2; pass++) {
-    T *SRC = pass ? DD : HH, *DST = pass ? HH : DD;
-    for (int u = 0; u < RB / 4; u++) {
-      const int t = wv + 4 * u;
-      const T *src = SRC + t * WS;
-      T *dst = DST + t * WS;
-      for (int I = 1 + lane; I <= ni; I += 64) dst[I] = dc[pass][u] * (src[I - 1] - T(2.) * src[I] + src[I + 1]);
+#pragma unroll
+    for (int u = 0; u < RW; u++) {
+      const T *x = X + (wv + 4 * u) * WS;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const int I = 1 + lane + 64 * q;
+        if (I <= ni) y[u][q] = dc[pass][u] * (x[I - 1] - T(2.) * x[I] + x[I + 1]);
+      }
     }
     __syncthreads();
-    if (wv == 0 && lane < RB) {
-      T *dst = DST + lane * WS;
-      dst[ni + 1] = dst[1];
-      dst[0] = dst[ni];
-      meanv[lane] = seq_acc(dst + 1, ni) / T(ni);
+#pragma unroll
+    for (int u = 0; u < RW; u++) {
+      T *x = X + (wv + 4 * u) * WS;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const int I = 1 + lane + 64 * q;
+        if (I <= ni) x[I] = y[u][q];
+      }
     }
     __syncthreads();
-    for (int u = 0; u < RB / 4; u++) {
+    if (wv == 0 && lane < RB) meanv[lane] = seq_acc(X + lane * WS + 1, ni) / T(ni);
+    __syncthreads();
+    // every point (and the zonal pads, copies of points ni and 1) less the
+    // mean; the second pass's result stays in registers
+#pragma unroll
+    for (int u = 0; u < RW; u++) {
       const int t = wv + 4 * u;
-      T *dst = DST + t * WS;
+      T *x = X + t * WS;
       const T fm = meanv[t];
-      for (int I = lane; I < W; I += 64) dst[I] -= fm;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const int I = 1 + lane + 64 * q;
+        if (I <= ni) {
+          y[u][q] -= fm;
+          if (pass == 0) {
+            x[I] = y[u][q];
+            if (I == ni) x[0] = y[u][q];
+            if (I == 1) x[ni + 1] = y[u][q];
+          }
+        }
+      }
     }
     __syncthreads();
   }
-  // pass 2 result is in HH; apply with sign -1 on the rows of the range,
-  // then pads
-  for (int u = 0; u < RB / 4; u++) {
-    const int t = wv + 4 * u, J = J0 + t;
+  // apply with sign -1 on the rows of the range, then pads
+#pragma unroll
+  for (int u = 0; u < RW; u++) {
+    const int J = J0 + wv + 4 * u;
     if (J > jlast) break;
     const bool upd = J >= r.jbot && J <= d.nj;
-    const T *hh = HH + t * WS;
-    for (int I = 1 + lane; I <= ni; I += 64) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int I = 1 + lane + 64 * q;
+      if (I > ni) break;
       double v = a[ix2(d, J, I)];
-      if (upd) v += d.dt * (double)(-1) * (double)hh[I];
+      if (upd) v += d.dt * (double)(-1) * (double)y[u][q];
       a[ix2(d, J, I)] = v;
       if (I == ni) a[ix2(d, J, 0)] = v;
       if (I == 1) a[ix2(d, J, ni + 1)] = v;
     }
   }
 }
-void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
-  // 16 (or 8) rows per block when that still gives every CU a few blocks
-  // and two blocks fit in one CU's LDS, else 4
-  const int rows = (b.nj + 2) * b.nk * b.nm * fs.n;
-  const int RB = b.knob[KN_ROWS_ZHV]                          ? b.knob[KN_ROWS_ZHV]
-                 : (rows / 16 >= 1024 && b.ni <= 128) ? 16
-                 : (rows / 8 >= 1024 && b.ni <= 256)  ? 8
-                                                      : 4;
-  const int WS = (b.ni + 2) | 1;
+
+// rows per block RB (4, 8 or 16) such that a lane's registers hold at most
+// 16 points (RB/4 rows x ni/64 points)
+static int zhv_rows(const Bucket &b) {
+  const int nq = b.ni >= 64 ? b.ni / 64 : 1;
+  const int rows = (b.nj + 2) * b.nk * b.nm;
+  int RB = b.knob[KN_ROWS_ZHV] ? b.knob[KN_ROWS_ZHV] : (rows / 16 >= 2048) ? 16 : (rows / 8 >= 2048) ? 8 : 4;
+  while (RB > 4 && RB / 4 * nq > 16) RB /= 2;
+  return RB;
+}
+template <class T, int RW>
+static void zonal_hv_rw(hipStream_t s, const Bucket &b, const FieldSet &fs, dim3 g, size_t shm) {
+  switch (b.ni >= 64 ? b.ni / 64 : 1) {
+    case 1: LAUNCH_AS("k_zonal_hv<T>", (k_zonal_hv<T, RW, 1>), g, dim3(256), shm, s, b.D, b.mb, b.nm, fs); break;
+    case 2: LAUNCH_AS("k_zonal_hv<T>", (k_zonal_hv<T, RW, 2>), g, dim3(256), shm, s, b.D, b.mb, b.nm, fs); break;
+    case 4: LAUNCH_AS("k_zonal_hv<T>", (k_zonal_hv<T, RW, 4>), g, dim3(256), shm, s, b.D, b.mb, b.nm, fs); break;
+    case 8:
+      if constexpr (RW <= 2) LAUNCH_AS("k_zonal_hv<T>", (k_zonal_hv<T, RW, 8>), g, dim3(256), shm, s, b.D, b.mb, b.nm, fs);
+      break;
+    default:
+      if constexpr (RW == 1) LAUNCH_AS("k_zonal_hv<T>", (k_zonal_hv<T, 1, 16>), g, dim3(256), shm, s, b.D, b.mb, b.nm, fs);
+      break;
+  }
+}
+template <class T>
+static void zonal_hv_t(hipStream_t s, const Bucket &b, const FieldSet &fs) {
+  const int RB = zhv_rows(b), WS = (b.ni + 2) | 1;
   dim3 g((b.nj + 2 + RB - 1) / RB, b.nk, fs.n * b.nm);
-  if (b.mixed)
-    LAUNCH(k_zonal_hv<float>, g, dim3(256), sizeof(float) * (2 * RB * WS + RB), s, b.D, b.mb, b.nm, fs, RB);
-  else
-    LAUNCH(k_zonal_hv<double>, g, dim3(256), sizeof(double) * (2 * RB * WS + RB), s, b.D, b.mb, b.nm, fs, RB);
+  const size_t shm = sizeof(T) * ((size_t)RB * WS + RB);
+  if (RB == 4) zonal_hv_rw<T, 1>(s, b, fs, g, shm);
+  else if (RB == 8) zonal_hv_rw<T, 2>(s, b, fs, g, shm);
+  else zonal_hv_rw<T, 4>(s, b, fs, g, shm);
+}
+void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
+  if (b.mixed) zonal_hv_t<float>(s, b, fs);
+  else zonal_hv_t<double>(s, b, fs);
 }
 
 }  // namespace epic

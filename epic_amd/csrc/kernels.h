@@ -13,9 +13,12 @@ namespace epic {
 void prof_begin(hipStream_t s, const char *name);
 void prof_end(hipStream_t s);
 extern bool g_prof_on;
-#define LAUNCH(k, g, b, shm, s, ...)                     \
+#define LAUNCH(k, g, b, shm, s, ...) LAUNCH_AS(#k, k, g, b, shm, s, __VA_ARGS__)
+// the same under a given profile name (for a kernel whose template arguments
+// hold commas, which is passed in parentheses)
+#define LAUNCH_AS(name, k, g, b, shm, s, ...)            \
   do {                                                   \
-    if (::epic::g_prof_on) ::epic::prof_begin(s, #k);    \
+    if (::epic::g_prof_on) ::epic::prof_begin(s, name);  \
     hipLaunchKernelGGL(k, g, b, shm, s, __VA_ARGS__);    \
     if (::epic::g_prof_on) ::epic::prof_end(s);          \
   } while (0)
