@@ -697,7 +697,7 @@ void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
 // serial chains side by side.  Rows are strided by an odd number of doubles
 // so the lanes of the serial phase fall in distinct LDS banks.
 template <class T, int RW, int NQ>
-__global__ void __launch_bounds__(256) k_zonal_hv(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_zonal_hv(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
   constexpr int RB = 4 * RW;
   extern __shared__ double lds_raw[];
