@@ -67,6 +67,11 @@ def measured_traffic(workload, precision, name):
     for cand in (name, name.replace("<T>", "<double>")):
         if cand in k:
             return k[cand]["traffic_bytes"], os.path.relpath(fn, HERE)
+    # a kernel with more template arguments than its launch name shows
+    base = name.split("<")[0]
+    same = [c for c in k if c.split("<")[0] == base]
+    if len(same) == 1:
+        return k[same[0]]["traffic_bytes"], os.path.relpath(fn, HERE)
     return None, None
 
 
