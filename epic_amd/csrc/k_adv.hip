@@ -313,8 +313,11 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   extern __shared__ double sh_raw[];
   T *sh = reinterpret_cast<T *>(sh_raw);
   const int ni = d.ni, W = ni + 2;
-  T *A = sh + (size_t)m.g * 6 * W;
-  T *UP = A + W, *UM = A + 2 * W, *FP = A + 3 * W, *AP = A + 4 * W, *FF = A + 5 * W;
+  // four row arrays: the field, the velocity u/n (split by sign where it is
+  // used), the predictor, and the first-guess flux, which the corrected flux
+  // overwrites once the predictor is formed
+  T *A = sh + (size_t)m.g * 4 * W;
+  T *UV = A + W, *AP = A + 2 * W, *FP = A + 3 * W, *FF = FP;
   const T *af = reinterpret_cast<const T *>(d.sB) + so + (size_t)K * d.N2;
   const double ninv = 1. / d.n[2 * J + 1];
   const T mn = T(d.mn[2 * J + 1]), DT = T(d.dt);
@@ -330,12 +333,14 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
       } else {
         u2 = d.U[ix3(d, K, J, I)];
       }
-      UM[I] = T(dmin(0., u2 * ninv));
-      UP[I] = T(dmax(0., u2 * ninv));
+      UV[I] = T(u2 * ninv);
     }
   __syncthreads();
+  // T(min(0, x)) = min(0, T(x)): rounding is monotone and keeps zero
+  auto UP = [UV](int I) { return tmax(T(0.), UV[I]); };
+  auto UM = [UV](int I) { return tmin(T(0.), UV[I]); };
   if (live)
-    for (int c = 1 + m.lane; c <= ni; c += m.stride) FP[c] = UP[c] * A[c - 1] + UM[c] * A[c];
+    for (int c = 1 + m.lane; c <= ni; c += m.stride) FP[c] = UP(c) * A[c - 1] + UM(c) * A[c];
   __syncthreads();
   if (live)
     for (int c = 1 + m.lane; c <= ni; c += m.stride) AP[c] = A[c] + (FP[c] - FP[wr(c + 1)]) * mn * DT;
@@ -343,11 +348,11 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   if (live)
     for (int c = 1 + m.lane; c <= ni; c += m.stride) {
       const int cm = wr(c - 1);
-      T UPc = UP[c], UMc = UM[c], APc = AP[c], APm = AP[cm], Am = A[c - 1], Ac = A[c];
+      T UPc = UP(c), UMc = UM(c), APc = AP[c], APm = AP[cm], Am = A[c - 1], Ac = A[c];
       T fc = T(.5) * (UPc * (APc + Am) + UMc * (APm + Ac));
       T d1c = APc - Am, d2c = APc - A[c + 1];
       T d1m = APm - A[cm - 1], d2m = APm - A[cm + 1];
-      T uhp = sqrt(UPc * UP[c - 1]), uhm = -sqrt(UMc * UM[c + 1]);
+      T uhp = sqrt(UPc * UP(c - 1)), uhm = -sqrt(UMc * UM(c + 1));
       T mu = (UPc - UMc) * mn * DT, al = (T(1.) + mu) / T(6.);
       T t0 = fabs(A[cm - 1] - T(2.) * A[cm] + A[cm + 1]) + T(1.e-10), q0 = t0 * t0;
       T t1 = fabs(A[c - 1] - T(2.) * A[c] + A[c + 1]) + T(1.e-10), q1 = t1 * t1;
@@ -392,7 +397,7 @@ static void hsu3_t(hipStream_t s, const Bucket &b) {
   if (sizeof(T) == sizeof(double)) L_filter(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
   else L_filter_f32(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
   const int rpb = rows_per_block(b.ni);
-  LAUNCH(k_hsu_zonal<T>, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, nz), dim3(256), sizeof(T) * rpb * 6 * (b.ni + 2), s,
+  LAUNCH(k_hsu_zonal<T>, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, nz), dim3(256), sizeof(T) * rpb * 4 * (b.ni + 2), s,
          b.D, b.mb, b.nm);
 }
 static void hsu3(hipStream_t s, const Bucket &b) {
