@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb
 // the profile on every row.  HDRY is stored with the restore_mass that
 // follows the filters in the step.  RB rows per block, ni/4 lanes per row.
 template <class T>
-__global__ void __launch_bounds__(256) k_filter3(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_filter3(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
   extern __shared__ double lds_raw[];
   double *tabs = lds_raw;
