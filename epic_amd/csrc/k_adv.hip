@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, i
 // neighbouring tile: the same values).  The update goes to sA and to its copy
 // sB (which the zonal pass filters), with sB's zonal pads.
 template <class T>
-__global__ void __launch_bounds__(256) k_hsu_mer(const MemberDesc *D, int mb, int nm, int CW, int MER_TJ) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_hsu_mer(const MemberDesc *D, int mb, int nm, int CW, int MER_TJ) {
   ADV_MEMBER;
   const int K = 1 + blockIdx.y;
   if (which == AV_THETA && K <= d.ks - 2) return;  // uniform per block

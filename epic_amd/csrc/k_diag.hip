@@ -698,7 +698,7 @@ void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
 // PV3 planes k0..k1 from U (uu) and the zonally filtered V (vv), with H3.
 // One block per (K,J) pv-row: row J of vv is filtered in LDS (rows of the
 // v-grid filter band), then one lane per I.
-__global__ void __launch_bounds__(256) k_vort_interior(const MemberDesc *D, int mb, int uf, int vf, int k0) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_vort_interior(const MemberDesc *D, int mb, int uf, int vf, int k0) {
   MEMBER;
   extern __shared__ double lds[];
   const RowMap m = row_map(d.ni);
@@ -778,7 +778,7 @@ void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1)
 
 // DIV of (uu zonally filtered, vv) on planes k0..: one block per (K,J)
 // h-row, row J of uu filtered in LDS (rows of the h-grid filter band)
-__global__ void __launch_bounds__(256) k_divergence(const MemberDesc *D, int mb, int uf, int vf, int df, int k0) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_divergence(const MemberDesc *D, int mb, int uf, int vf, int df, int k0) {
   MEMBER;
   extern __shared__ double lds[];
   const RowMap m = row_map(d.ni);

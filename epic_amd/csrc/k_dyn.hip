@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) k_dvdt_core(const MemberDesc *D, int mb) 
 // DUDT = (DUDT + PV*VH terms + dKIN/dx with filtered KIN) - vertical
 // advection.  One block per (K,J) u-row: the KIN row is zonally filtered in
 // LDS (rows of the filter band) and its pad refreshed, then one lane per I.
-__global__ void __launch_bounds__(256) k_dudt_core(const MemberDesc *D, int mb) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_dudt_core(const MemberDesc *D, int mb) {
   MEMBER;
   extern __shared__ double lds[];
   const RowMap m = row_map(d.ni);
@@ -636,7 +636,7 @@ __global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, in
 // zonally filtered in LDS (rows of the filter band), then A += DT*(-1)*D on
 // the kind's rows and the zonal pads of every row 0..nj+1 are written
 template <class T>
-__global__ void __launch_bounds__(256) k_hv_apply(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_hv_apply(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
   extern __shared__ double lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);

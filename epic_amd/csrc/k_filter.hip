@@ -22,7 +22,7 @@ namespace epic {
 #define MEMBER const MemberDesc &d = D[mb + blockIdx.z]
 
 template <class T>
-__global__ void __launch_bounds__(256) k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int kn, int vgrid,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_fft_filter(const MemberDesc *D, int mb, int f, int k0, int kn, int vgrid,
                                                     int pads) {
   MEMBER;
   extern __shared__ double lds_raw[];
