@@ -899,7 +899,8 @@ extern "C" int epic_engine_restore(epic_engine *E, const epic_case *c, const cha
 
 // --------------------------------------------------------------- autotune
 // Picks, per bucket, the launch shape of each tunable kernel (columns per
-// column tile, rows per zonal-hyperviscosity block) by timing every
+// column tile, rows per zonal-hyperviscosity block or meridional-advection
+// tile) by timing every
 // candidate over a few steps of the bucket's own members.  The members'
 // blobs are copied aside first and put back after every candidate and at
 // the end, so tuning leaves the state (and the step counts) as it found
@@ -957,8 +958,8 @@ extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int
       M.steps = done[i];
     }
   };
-  static const int cand[KN_COUNT][3] = {{16, 32, 64}, {16, 32, 64}, {16, 32, 64},
-                                        {16, 32, 64}, {4, 8, 16},   {2, 4, 8}};
+  static const int cand[KN_COUNT][4] = {{8, 16, 32, 64}, {8, 16, 32, 64}, {8, 16, 32, 64},
+                                        {8, 16, 32, 64}, {4, 8, 16},      {2, 4, 8}};
   size_t used = 0;
   for (Bucket &b : E->buckets) {
     bool hv, divd, adj;
@@ -967,7 +968,7 @@ extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int
       double best = 1e300;
       int bestv = b.knob[k];
       for (int v : cand[k]) {
-        if (!knob_fits(b, k, v)) continue;
+        if (!v || !knob_fits(b, k, v)) continue;
         b.knob[k] = v;
         restore();
         rec_step(E->stream, b, hv, divd, adj);  // warm-up, not timed
