@@ -313,11 +313,12 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   extern __shared__ double sh_raw[];
   T *sh = reinterpret_cast<T *>(sh_raw);
   const int ni = d.ni, W = ni + 2;
-  // four row arrays: the field, the velocity u/n (split by sign where it is
-  // used), the predictor, and the first-guess flux, which the corrected flux
-  // overwrites once the predictor is formed
-  T *A = sh + (size_t)m.g * 4 * W;
-  T *UV = A + W, *AP = A + 2 * W, *FP = A + 3 * W, *FF = FP;
+  // three row arrays: the field, the velocity u/n (split by sign where it is
+  // used; the corrected flux overwrites it once every lane has read it) and
+  // the predictor; the first-guess flux is formed where the predictor needs
+  // it (twice per cell, by the same expression)
+  T *A = sh + (size_t)m.g * 3 * W;
+  T *UV = A + W, *AP = A + 2 * W, *FF = UV;
   const T *af = reinterpret_cast<const T *>(d.sB) + so + (size_t)K * d.N2;
   const double ninv = 1. / d.n[2 * J + 1];
   const T mn = T(d.mn[2 * J + 1]), DT = T(d.dt);
@@ -339,14 +340,16 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   // T(min(0, x)) = min(0, T(x)): rounding is monotone and keeps zero
   auto UP = [UV](int I) { return tmax(T(0.), UV[I]); };
   auto UM = [UV](int I) { return tmin(T(0.), UV[I]); };
+  auto FP = [&](int c) { return UP(c) * A[c - 1] + UM(c) * A[c]; };
   if (live)
-    for (int c = 1 + m.lane; c <= ni; c += m.stride) FP[c] = UP(c) * A[c - 1] + UM(c) * A[c];
+    for (int c = 1 + m.lane; c <= ni; c += m.stride) AP[c] = A[c] + (FP(c) - FP(wr(c + 1))) * mn * DT;
   __syncthreads();
+  // a lane's four corrected fluxes (c = 1 + lane + q*stride) in registers
+  T ffr[4];
   if (live)
-    for (int c = 1 + m.lane; c <= ni; c += m.stride) AP[c] = A[c] + (FP[c] - FP[wr(c + 1)]) * mn * DT;
-  __syncthreads();
-  if (live)
-    for (int c = 1 + m.lane; c <= ni; c += m.stride) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int c = 1 + m.lane + q * m.stride;
       const int cm = wr(c - 1);
       T UPc = UP(c), UMc = UM(c), APc = AP[c], APm = AP[cm], Am = A[c - 1], Ac = A[c];
       T fc = T(.5) * (UPc * (APc + Am) + UMc * (APm + Ac));
@@ -362,8 +365,12 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
       gm = gm * gm;
       T bp = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gp, bm = T(1.) + (T(1.) / (T(2.) * al) - T(1.)) * gm;
       T bhp = T(1.) - gp, bhm = T(1.) - gm;
-      FF[c] = fc + -al * (UPc * bp * d1c - uhp * bhp * d1m + UMc * bm * d2m - uhm * bhm * d2c);
+      ffr[q] = fc + -al * (UPc * bp * d1c - uhp * bhp * d1m + UMc * bm * d2m - uhm * bhm * d2c);
     }
+  __syncthreads();
+  if (live)
+#pragma unroll
+    for (int q = 0; q < 4; q++) FF[1 + m.lane + q * m.stride] = ffr[q];
   __syncthreads();
   if (!live) return;
   double *out = adv_field(d, which);
@@ -397,7 +404,7 @@ static void hsu3_t(hipStream_t s, const Bucket &b) {
   if (sizeof(T) == sizeof(double)) L_filter(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
   else L_filter_f32(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
   const int rpb = rows_per_block(b.ni);
-  LAUNCH(k_hsu_zonal<T>, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, nz), dim3(256), sizeof(T) * rpb * 4 * (b.ni + 2), s,
+  LAUNCH(k_hsu_zonal<T>, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, nz), dim3(256), sizeof(T) * rpb * 3 * (b.ni + 2), s,
          b.D, b.mb, b.nm);
 }
 static void hsu3(hipStream_t s, const Bucket &b) {
