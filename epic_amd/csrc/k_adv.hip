@@ -75,7 +75,7 @@ __device__ __forceinline__ double old_value(double staged, const double *) { ret
 __device__ __forceinline__ double old_value(float, const double *p) { return *p; }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_hsu_vert(const MemberDesc *D, int mb, int nm, int C) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_hsu_vert(const MemberDesc *D, int mb, int nm, int C) {
   ADV_MEMBER;
   extern __shared__ double lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
