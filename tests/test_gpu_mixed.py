@@ -80,3 +80,21 @@ def test_mixed_alone_equals_mixed_in_batch():
     for f in FIELDS:
         assert np.array_equal(e2.read(s2, f), alone[f]), f
     e2.close()
+
+
+@pytest.mark.parametrize("ni,nj,nk", [(256, 128, 16), (1024, 512, 64)])
+def test_mixed_against_fp64_hot_jupiter(ni, nj, nk):
+    """The hot Jupiter in the mixed mode (at 1024x512x64: rows of 1024 points,
+    the FP32 row filters with one row per block, the widest FP32 register
+    layout of the zonal hyperviscosity), 100 steps against FP64."""
+    sp = ea.spec(ea.HOT_JUPITER, ni, nj, nk, 30, seed=1, nu4=0.5, noise_theta=1.0, noise_uv=0.0)
+    ref, _, _ = run(sp, ea.FP64, 100)
+    mix, _, _ = run(sp, ea.MIXED, 100)
+    worst = {}
+    for f in FIELDS + ("GZ3", "PV3"):
+        a, b = interior(mix[f], sp), interior(ref[f], sp)
+        assert np.isfinite(a).all(), f
+        worst[f] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        say(f"mixed vs fp64 hot Jupiter {ni}x{nj}x{nk}, 100 steps {f}: rel L2 {worst[f]:.2e}")
+    for f in ("HDRY", "THETA", "P3"):
+        assert worst[f] <= 1e-5, (f, worst[f])

@@ -73,6 +73,20 @@ def test_jupiter_256x128x20_full_size():
         p.close()
 
 
+@pytest.mark.parametrize("ni,nj,nk,dt", [(512, 256, 5, 60), (1024, 256, 4, 30)])
+def test_wide_rows(ni, nj, nk, dt):
+    """Rows of 512 and 1024 points (Jupiter-like): one row per block in the
+    row filters, the widest register layouts of the zonal hyperviscosity,
+    64-column tiles in the meridional passes; 3 steps against the oracle."""
+    sp = ea.spec(ea.JUPITER_LIKE, ni, nj, nk, dt, 11, 0.5, 0.0, 0.5, 0.5)
+    p = Pair(sp)
+    try:
+        p.step(3)
+        p.check(f"wide {ni}x{nj}x{nk} step3")
+    finally:
+        p.close()
+
+
 @pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny"])
 def test_against_reference_fixtures(name):
     """The GPU path against arrays written by the reference's own build."""
