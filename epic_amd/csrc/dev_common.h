@@ -37,6 +37,8 @@ struct MemberDesc {
   // day-night relaxation of the hot-Jupiter case (is_hj)
   int is_hj;
   double pbot, teq_contrast, teq_p_contrast, tau_rad, ref_theta_b, ref_pbk, ref_a_over_kappa;
+  double hj_lnref;                 // log(pbot / teq_p_contrast)
+  const double *hj_cl, *hj_clon;   // cos(lat) per row J, cos(lon) per column I
   // 1-D metrics and tables
   const double *lat, *lon, *m, *n, *mn, *f, *g, *rln, *rlt, *sgth, *dsgth, *dsgth_inv, *h_min, *t_sponge;
   const double *mn_inv, *hl_h, *hl_pv, *hcx, *hcy, *hcz, *vcx, *vcy, *vcz;

@@ -338,6 +338,14 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   add(&d.dsgth, sizeof(double) * nK, c->dsgth);
   add(&d.dsgth_inv, sizeof(double) * nK, c->dsgth_inv);
   add(&d.h_min, sizeof(double) * (c->nk + 2), c->h_min);
+  // the day-night forcing's cos(lat) per row and cos(lon) per column, with
+  // the host libm like the oracle's (every case carries them; small)
+  // (rows 0..nj and columns 0..ni are defined; the last entries stay 0)
+  std::vector<double> hj_cl(c->nj + 2, 0.), hj_clon(c->ni + 2, 0.);
+  for (int J = 0; J <= c->nj; J++) hj_cl[J] = cos(c->lat[2 * J + 1] * (M_PI / 180.));
+  for (int I = 0; I <= c->ni; I++) hj_clon[I] = cos(c->lon[2 * I + 1] * (M_PI / 180.));
+  add(&d.hj_cl, sizeof(double) * (c->nj + 2), hj_cl.data());
+  add(&d.hj_clon, sizeof(double) * (c->ni + 2), hj_clon.data());
   add(&d.t_sponge, sizeof(double) * (c->k_sponge + 1), c->t_sponge_inv);
   add(&d.mn_inv, sizeof(double) * nJ, T.mn_inv);
   add(&d.hl_h, sizeof(double) * T.n_hl, T.hl_h);
@@ -465,6 +473,7 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   d.ref_a_over_kappa = c->ref_a_over_kappa;
   // (ref_pb/press0)^kappa, with the host libm like the oracle's
   d.ref_pbk = d.is_hj ? pow(c->ref_pb / c->press0, c->kappa) : 0.;
+  d.hj_lnref = d.is_hj ? log(c->pbot / c->teq_p_contrast) : 0.;
   d.newt_cool_on = c->newt_cool_on; d.newt_cool_adjust = c->newt_cool_adjust;
   d.drag_v = c->drag_v; d.drag_zonal_avg = c->drag_zonal_avg; d.jtp = c->jtp;
   d.nrow_h = T.nrow_h; d.nrow_v = T.nrow_v;

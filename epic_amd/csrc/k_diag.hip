@@ -562,9 +562,11 @@ __device__ double hj_newton(const MemberDesc &d, int K, int J, int I) {
   const double pk = pow(p / d.press0, d.kappa);
   const double tref = (1. / (1. / d.ref_theta_b + d.ref_a_over_kappa * (pk - d.ref_pbk))) * pk;
   double dT = d.teq_contrast;
-  if (p > d.teq_p_contrast) dT *= dmax(0., log(d.pbot / p) / log(d.pbot / d.teq_p_contrast));
-  const double cl = cos(d.lat[2 * J + 1] * (M_PI / 180.));
-  const double day = dmax(0., cl * cos(d.lon[2 * I + 1] * (M_PI / 180.)));
+  // the per-member log and the per-row/column cosines come from host tables
+  // formed with the host libm, as the oracle forms them
+  if (p > d.teq_p_contrast) dT *= dmax(0., log(d.pbot / p) / d.hj_lnref);
+  const double cl = d.hj_cl[J];
+  const double day = dmax(0., cl * d.hj_clon[I]);
   const double teq = tref + dT * (day - .25);
   return d.cp * (teq - t) / d.tau_rad;
 }
