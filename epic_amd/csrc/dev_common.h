@@ -37,8 +37,9 @@ struct MemberDesc {
   // day-night relaxation of the hot-Jupiter case (is_hj)
   int is_hj;
   double pbot, teq_contrast, teq_p_contrast, tau_rad, ref_theta_b, ref_pbk, ref_a_over_kappa;
-  double hj_lnref;                 // log(pbot / teq_p_contrast)
-  const double *hj_cl, *hj_clon;   // cos(lat) per row J, cos(lon) per column I
+  double hj_lnref;  // log(pbot / teq_p_contrast)
+  // host-libm tables: cos(lat), cos(lat)^4 per h-row J, cos(lon) per column I
+  const double *coslat_h, *coslat4_h, *coslon_h;
   // 1-D metrics and tables
   const double *lat, *lon, *m, *n, *mn, *f, *g, *rln, *rlt, *sgth, *dsgth, *dsgth_inv, *h_min, *t_sponge;
   const double *mn_inv, *hl_h, *hl_pv, *hcx, *hcy, *hcz, *vcx, *vcy, *vcz;

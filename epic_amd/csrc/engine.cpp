@@ -338,14 +338,19 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   add(&d.dsgth, sizeof(double) * nK, c->dsgth);
   add(&d.dsgth_inv, sizeof(double) * nK, c->dsgth_inv);
   add(&d.h_min, sizeof(double) * (c->nk + 2), c->h_min);
-  // the day-night forcing's cos(lat) per row and cos(lon) per column, with
-  // the host libm like the oracle's (every case carries them; small)
-  // (rows 0..nj and columns 0..ni are defined; the last entries stay 0)
-  std::vector<double> hj_cl(c->nj + 2, 0.), hj_clon(c->ni + 2, 0.);
-  for (int J = 0; J <= c->nj; J++) hj_cl[J] = cos(c->lat[2 * J + 1] * (M_PI / 180.));
-  for (int I = 0; I <= c->ni; I++) hj_clon[I] = cos(c->lon[2 * I + 1] * (M_PI / 180.));
-  add(&d.hj_cl, sizeof(double) * (c->nj + 2), hj_cl.data());
-  add(&d.hj_clon, sizeof(double) * (c->ni + 2), hj_clon.data());
+  // cos(lat) and cos(lat)^4 per h-row and cos(lon) per column for the
+  // Newtonian relaxations, with the host libm like the oracle's (every case
+  // carries them; small).  Rows 0..nj and columns 0..ni are defined; the
+  // last entries stay 0.
+  std::vector<double> cl_h(c->nj + 2, 0.), cl4_h(c->nj + 2, 0.), clon_h(c->ni + 2, 0.);
+  for (int J = 0; J <= c->nj; J++) {
+    cl_h[J] = cos(c->lat[2 * J + 1] * (M_PI / 180.));
+    cl4_h[J] = pow(cl_h[J], 4.);
+  }
+  for (int I = 0; I <= c->ni; I++) clon_h[I] = cos(c->lon[2 * I + 1] * (M_PI / 180.));
+  add(&d.coslat_h, sizeof(double) * (c->nj + 2), cl_h.data());
+  add(&d.coslat4_h, sizeof(double) * (c->nj + 2), cl4_h.data());
+  add(&d.coslon_h, sizeof(double) * (c->ni + 2), clon_h.data());
   add(&d.t_sponge, sizeof(double) * (c->k_sponge + 1), c->t_sponge_inv);
   add(&d.mn_inv, sizeof(double) * nJ, T.mn_inv);
   add(&d.hl_h, sizeof(double) * T.n_hl, T.hl_h);

@@ -535,17 +535,15 @@ void L_dzdt3(hipStream_t s, const Bucket &b) {
 __device__ double hs_newton(const MemberDesc &d, int K, int J, int I) {
   const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
   const double dt_y = 60., dth_z = 10., p0 = 1000. * 100.;
-  double lat = d.lat[2 * J + 1];
   double p = d.P3[ix3(d, K, J, I)], t = d.T3[ix3(d, K, J, I)];
   double kt = ka, pbot = d.P3[ix3(d, d.nk, J, I)];
   double amp = (p / pbot - 0.7) / (1. - 0.7);
-  if (amp > 0.) {
-    double cl = cos(d.lat[2 * J + 1] * (M_PI / 180.));
-    kt += (ks - ka) * pow(cl, 4.) * amp;
-  }
+  // cos(lat) and its fourth power from the host tables (the host libm, as
+  // the oracle forms them)
+  if (amp > 0.) kt += (ks - ka) * d.coslat4_h[J] * amp;
   double t_cool = 1. / kt;
   double cpt = 1004. / t_cool;
-  double c2 = cos(lat * (M_PI / 180.));
+  double c2 = d.coslat_h[J];
   c2 *= c2;
   double s2 = 1. - c2, pp0 = p / p0;
   double teq = (315. - dt_y * s2 - dth_z * log(pp0) * c2) * pow(pp0, d.kappa);
@@ -565,8 +563,8 @@ __device__ double hj_newton(const MemberDesc &d, int K, int J, int I) {
   // the per-member log and the per-row/column cosines come from host tables
   // formed with the host libm, as the oracle forms them
   if (p > d.teq_p_contrast) dT *= dmax(0., log(d.pbot / p) / d.hj_lnref);
-  const double cl = d.hj_cl[J];
-  const double day = dmax(0., cl * d.hj_clon[I]);
+  const double cl = d.coslat_h[J];
+  const double day = dmax(0., cl * d.coslon_h[I]);
   const double teq = tref + dT * (day - .25);
   return d.cp * (teq - t) / d.tau_rad;
 }
