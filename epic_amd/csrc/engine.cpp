@@ -208,7 +208,7 @@ static void rec_store_diag(hipStream_t s, const Bucket &b) {
 static void rec_sync(hipStream_t s, const Bucket &b, bool adjust) {
   rec_p2etc_pgrad_vars(s, b, 1);
   rec_store_diag(s, b);
-  L_heating(s, b, adjust);
+  L_heating(s, b, adjust, false);
   L_calc_w(s, b, 0);
 }
 
@@ -246,7 +246,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
     L_zonal_hv(s, b, uv);
   }
   rec_store_diag(s, b);
-  L_heating(s, b, adjust);
+  L_heating(s, b, adjust, true);  // the relaxation itself was formed in k_thermo
   L_rotate(s, b);
 }
 

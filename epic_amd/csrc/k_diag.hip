@@ -225,7 +225,14 @@ void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
 // full = 0 (the step's first store_pgrad_vars): T3 and RHO3 only, all that
 // the geopotential integration and the filtered GZ3 copy read before the
 // second call writes every output again.
-__global__ void __launch_bounds__(256) k_thermo(const MemberDesc *D, int mb, int full) {
+__device__ __forceinline__ double hs_relax(const MemberDesc &d, int J, double p, double t, double pbot, double pk0);
+__device__ __forceinline__ double hj_relax(const MemberDesc &d, int J, int I, double p, double t, double pk);
+// full = 1 also forms the Newtonian relaxation HEAT3 of calc_heating on
+// K = 1..nk, J = 0..nj and the zonal pads: it reads only P3 and T3, which
+// nothing changes between here and the end of the step, where the reference
+// forms it, and it takes the p^kappa formed here for T3 (a pad takes the
+// value of the column it copies)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_thermo(const MemberDesc *D, int mb, int full) {
   MEMBER;
   if (full && d.dyn->itime <= 1) {
     // no lane of this kernel writes DZDT3 while itime <= 1
@@ -240,11 +247,22 @@ __global__ void __launch_bounds__(256) k_thermo(const MemberDesc *D, int mb, int
   if (full && K >= 1 && d.dyn->itime > 1 && I >= 1 && I <= d.ni) d.DZDT3[q] = d.GZ3[q];
   const double R_GAS = 8.314472e+3;
   double th = d.THETA[q], p = d.P3[q];
-  double t = th * pow(p / d.press0, d.kappa);
+  const double pk = pow(p / d.press0, d.kappa);
+  double t = th * pk;
   double rho3 = p * d.mu3 / (R_GAS * t);
   d.T3[q] = t;
   d.RHO3[q] = rho3;
   if (!full) return;
+  if (K >= 1 && d.newt_cool_on && (d.is_hs || d.is_hj)) {
+    double h;
+    if (d.is_hj) {
+      h = hj_relax(d, J, (I == 0) ? d.ni : (I == d.ni + 1 ? 1 : I), p, t, pk);
+    } else {
+      const double pk0 = (d.press0 == 1000. * 100.) ? pk : pow(p / (1000. * 100.), d.kappa);
+      h = 0. + hs_relax(d, J, p, t, d.P3[ix3(d, d.nk, J, I)], pk0);
+    }
+    d.HEAT3[q] = h;
+  }
   d.EXNER3[q] = d.cp * t / th;
   if (K >= 1) {
     double p2 = d.P2[q], th2 = d.THETA2[q];
@@ -532,11 +550,12 @@ void L_dzdt3(hipStream_t s, const Bucket &b) {
 }
 
 // ------------------------------------------------------------ heating
-__device__ double hs_newton(const MemberDesc &d, int K, int J, int I) {
+// the Held-Suarez relaxation of a cell from its p, T, the column's bottom
+// pressure and (p/p0)^kappa, p0 = 1000 hPa
+__device__ __forceinline__ double hs_relax(const MemberDesc &d, int J, double p, double t, double pbot, double pk0) {
   const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
   const double dt_y = 60., dth_z = 10., p0 = 1000. * 100.;
-  double p = d.P3[ix3(d, K, J, I)], t = d.T3[ix3(d, K, J, I)];
-  double kt = ka, pbot = d.P3[ix3(d, d.nk, J, I)];
+  double kt = ka;
   double amp = (p / pbot - 0.7) / (1. - 0.7);
   // cos(lat) and its fourth power from the host tables (the host libm, as
   // the oracle forms them)
@@ -546,18 +565,22 @@ __device__ double hs_newton(const MemberDesc &d, int K, int J, int I) {
   double c2 = d.coslat_h[J];
   c2 *= c2;
   double s2 = 1. - c2, pp0 = p / p0;
-  double teq = (315. - dt_y * s2 - dth_z * log(pp0) * c2) * pow(pp0, d.kappa);
+  double teq = (315. - dt_y * s2 - dth_z * log(pp0) * c2) * pk0;
   teq = dmax(200., teq);
   return -cpt * (t - teq);
+}
+__device__ double hs_newton(const MemberDesc &d, int K, int J, int I) {
+  const double p = d.P3[ix3(d, K, J, I)];
+  return hs_relax(d, J, p, d.T3[ix3(d, K, J, I)], d.P3[ix3(d, d.nk, J, I)], pow(p / (1000. * 100.), d.kappa));
 }
 
 // Day-night relaxation of the hot-Jupiter case (no reference counterpart):
 // HEAT3 = cp (T_eq - T)/tau_rad, T_eq = T_ref(p) + dT(p) (max(0, cos(lat)
 // cos(lon)) - 1/4), T_ref the initial profile's temperature, dT the full
 // contrast above teq_p_contrast and linear in ln p to 0 at pbot.
-__device__ double hj_newton(const MemberDesc &d, int K, int J, int I) {
-  const double p = d.P3[ix3(d, K, J, I)], t = d.T3[ix3(d, K, J, I)];
-  const double pk = pow(p / d.press0, d.kappa);
+// the day-night relaxation of a cell from its p, T, (p/press0)^kappa and the
+// column I whose longitude it takes
+__device__ __forceinline__ double hj_relax(const MemberDesc &d, int J, int I, double p, double t, double pk) {
   const double tref = (1. / (1. / d.ref_theta_b + d.ref_a_over_kappa * (pk - d.ref_pbk))) * pk;
   double dT = d.teq_contrast;
   // the per-member log and the per-row/column cosines come from host tables
@@ -567,6 +590,10 @@ __device__ double hj_newton(const MemberDesc &d, int K, int J, int I) {
   const double day = dmax(0., cl * d.coslon_h[I]);
   const double teq = tref + dT * (day - .25);
   return d.cp * (teq - t) / d.tau_rad;
+}
+__device__ double hj_newton(const MemberDesc &d, int K, int J, int I) {
+  const double p = d.P3[ix3(d, K, J, I)];
+  return hj_relax(d, J, I, p, d.T3[ix3(d, K, J, I)], pow(p / d.press0, d.kappa));
 }
 
 // HEAT3 = 0 everywhere, then the Held-Suarez (or day-night) relaxation on
@@ -603,9 +630,15 @@ __global__ void __launch_bounds__(256) k_heating_adjust(const MemberDesc *D, int
       d.HEAT3[ix3(d, K, J, I)] = 0. + (hs_newton(d, K, J, Ii) - havg);
     }
 }
-void L_heating(hipStream_t s, const Bucket &b, int adjust) {
-  L_fill(s, b, F_HEAT3, 0.);
-  LAUNCH(k_heating, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+// in_thermo: the relaxation was formed by this step's k_thermo(full = 1),
+// and every other cell of HEAT3 still holds the zero of the priming call's
+// fill (no kernel writes outside K = 1..nk, J = 0..nj), so only the
+// layer-mean adjustment is left
+void L_heating(hipStream_t s, const Bucket &b, int adjust, bool in_thermo) {
+  if (!in_thermo) {
+    L_fill(s, b, F_HEAT3, 0.);
+    LAUNCH(k_heating, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  }
   if (adjust) LAUNCH(k_heating_adjust, dim3(1, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
 }
 

@@ -55,7 +55,7 @@ void L_gz_from_u(hipStream_t s, const Bucket &b);
 void L_gz3(hipStream_t s, const Bucket &b);
 void L_dzdt3(hipStream_t s, const Bucket &b);
 void L_bc3(hipStream_t s, const Bucket &b, int fld);
-void L_heating(hipStream_t s, const Bucket &b, int adjust);
+void L_heating(hipStream_t s, const Bucket &b, int adjust, bool in_thermo);
 void L_calc_w(hipStream_t s, const Bucket &b, int with_source);
 void L_copy_planes(hipStream_t s, const Bucket &b, int src, int dst, int k0, int k1);
 void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1);
