@@ -173,6 +173,13 @@ static int rebuild_buckets(epic_engine *E) {
     b.nj = hd[i].nj;
     b.nk = hd[i].nk;
     b.mixed = (E->precision == EPIC_MIXED);
+    b.any_gas = b.any_hs = b.any_ray = 0;
+    for (size_t k = i; k < j; k++) {
+      const MemberDesc &h = hd[k];
+      b.any_gas |= (h.is_gas != 0);
+      b.any_hs |= (h.is_hs != 0);
+      b.any_ray |= ((h.newt_cool_on && h.prandtl > 0.) || (h.tau_drag > 0. && h.tau_drag < 1.e+20));
+    }
     auto t = E->tuned.find({b.ni, b.nj, b.nk, b.nm});
     for (int k = 0; k < KN_COUNT; k++) b.knob[k] = (t == E->tuned.end()) ? 0 : t->second[k];
     E->buckets.push_back(b);

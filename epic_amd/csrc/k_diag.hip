@@ -451,6 +451,7 @@ __global__ void __launch_bounds__(256) k_gzu_fill(const MemberDesc *D, int mb) {
   if (J <= d.nj) d.GZS[ix2(d, J, I)] = d.rowbuf[4 * R + J];
 }
 void L_gz_from_u(hipStream_t s, const Bucket &b) {
+  if (!b.any_gas) return;
   LAUNCH(k_gzu_means, dim3(b.nj + 1, 1, b.nm), dim3(64), sizeof(double) * 3 * b.ni, s, b.D, b.mb);
   LAUNCH(k_gzu_core, dim3(1, 1, b.nm), dim3(256), sizeof(double) * (7 * (b.nj + 3) + 5 * (2 * b.nj + 3)), s, b.D,
          b.mb);

@@ -248,8 +248,8 @@ void L_uv_drag(hipStream_t s, const Bucket &b) {
   if (b.mixed) { LM_uv_drag(s, b); return; }
   LAUNCH(k_sponge, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
   int nrow = b.nk * (b.nj + 1);
-  LAUNCH(k_rayleigh, dim3((nrow + 63) / 64, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
-  LAUNCH(k_hs_pbl, grid2(b.nj + 1, b.ni, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  if (b.any_ray) LAUNCH(k_rayleigh, dim3((nrow + 63) / 64, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
+  if (b.any_hs) LAUNCH(k_hs_pbl, grid2(b.nj + 1, b.ni, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
 // ------------------------------------------------------------- pgrad

@@ -438,8 +438,8 @@ void LM_uv_core(hipStream_t s, const Bucket &b) {
 }
 void LM_uv_drag(hipStream_t s, const Bucket &b) {
   LAUNCH(km_sponge, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
-  LAUNCH(km_rayleigh, dim3(b.nj + 1, b.nk, b.nm), dim3(64), 0, s, b.D, b.mb);
-  LAUNCH(km_hs_pbl, grid2(b.nj + 1, b.ni, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  if (b.any_ray) LAUNCH(km_rayleigh, dim3(b.nj + 1, b.nk, b.nm), dim3(64), 0, s, b.D, b.mb);
+  if (b.any_hs) LAUNCH(km_hs_pbl, grid2(b.nj + 1, b.ni, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 void LM_pgrad(hipStream_t s, const Bucket &b) {
   LAUNCH(km_pgrad, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);

@@ -42,6 +42,10 @@ struct Bucket {
   int ni, nj, nk;
   int mixed;            // FP32 tendencies and horizontal arithmetic
   int knob[KN_COUNT];   // columns per column tile / rows per zonal-mean block
+  // some member is a gas giant (gz_from_u), Held-Suarez (boundary-layer
+  // drag), or has Rayleigh drag: launches none of the members needs are
+  // left out (their kernels would return at once for every member)
+  int any_gas, any_hs, any_ray;
 };
 
 
