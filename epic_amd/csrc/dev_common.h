@@ -40,6 +40,9 @@ struct MemberDesc {
   double hj_lnref;  // log(pbot / teq_p_contrast)
   // host-libm tables: cos(lat), cos(lat)^4 per h-row J, cos(lon) per column I
   const double *coslat_h, *coslat4_h, *coslon_h;
+  // per layer K >= k_sigma: THETA2's interval in the sigmatheta table and its width
+  const int *t2_kay;
+  const double *t2_h;
   // 1-D metrics and tables
   const double *lat, *lon, *m, *n, *mn, *f, *g, *rln, *rlt, *sgth, *dsgth, *dsgth_inv, *h_min, *t_sponge;
   const double *mn_inv, *hl_h, *hl_pv, *hcx, *hcy, *hcz, *vcx, *vcy, *vcz;

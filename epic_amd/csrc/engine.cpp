@@ -294,6 +294,36 @@ extern "C" size_t epic_engine_bytes_in_use(epic_engine *E) { return E ? E->in_us
 // Upload a case into a free slot.  hdry_src: the prognostic h of the caller
 // (nullptr: reconstitute it from p3 as epic_main does).  prime: run the
 // SYNC_DIAGS_ONLY sequence for the new member.
+// Host copies of the device's fcmp_d / table_place_s (dev_common.h), for
+// searches whose table and argument are the same in every column: the
+// index (and interval width) is formed once here, by the same operations.
+static int fcmp_host(double x1, double x2) {
+  int e;
+  frexp(fabs(x1) > fabs(x2) ? x1 : x2, &e);
+  double delta = ldexp(2.220446049250313e-16, e), d = x1 - x2;
+  if (d > delta) return 1;
+  if (d < -delta) return -1;
+  return 0;
+}
+static int table_place_s_host(int n, const double *x, int st, double xx, double *dx) {
+  int il = -1, iu = n;
+  int ascend = (x[(n - 1) * st] >= x[0]);
+  int lo = fcmp_host(xx, x[0]), hi = fcmp_host(xx, x[(n - 1) * st]);
+  if (ascend) {
+    if (lo <= 0) { *dx = x[st] - x[0]; return 0; }
+    if (hi >= 0) { *dx = x[(n - 1) * st] - x[(n - 2) * st]; return n - 2; }
+  } else {
+    if (lo >= 0) { *dx = x[st] - x[0]; return 0; }
+    if (hi <= 0) { *dx = x[(n - 1) * st] - x[(n - 2) * st]; return n - 2; }
+  }
+  while (iu - il > 1) {
+    int im = (iu + il) >> 1;
+    if ((fcmp_host(xx, x[im * st]) >= 0) == ascend) il = im; else iu = im;
+  }
+  *dx = x[(il + 1) * st] - x[il * st];
+  return il;
+}
+
 int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src, bool prime) {
   if (!E || !c) return -1;
   if (c->ni > 1024 || c->ni < 8 || (c->ni & (c->ni - 1)) || c->nk > 64 || c->nk < 4) {
@@ -355,6 +385,18 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
     cl4_h[J] = pow(cl_h[J], 4.);
   }
   for (int I = 0; I <= c->ni; I++) clon_h[I] = cos(c->lon[2 * I + 1] * (M_PI / 180.));
+  // THETA2 above the sigma region interpolates theta against sigmatheta:
+  // the interval of layer K in the interface table (set_p2etc,
+  // src/shared/epic_funcs_diag.c:3522-3562) is the same in every column
+  std::vector<int> t2_kay(c->nk + 2, 0);
+  std::vector<double> t2_h(c->nk + 2, 1.);
+  {
+    const int nk = c->nk, nkay = nk - (c->k_sigma - 1) + 1;
+    const double *tx = c->sigmatheta + 2 * nk + 1;
+    for (int K = c->k_sigma; K <= nk; K++) t2_kay[K] = table_place_s_host(nkay, tx, -2, c->sigmatheta[2 * K], &t2_h[K]);
+  }
+  add(&d.t2_kay, sizeof(int) * (c->nk + 2), t2_kay.data());
+  add(&d.t2_h, sizeof(double) * (c->nk + 2), t2_h.data());
   add(&d.coslat_h, sizeof(double) * (c->nj + 2), cl_h.data());
   add(&d.coslat4_h, sizeof(double) * (c->nj + 2), cl4_h.data());
   add(&d.coslon_h, sizeof(double) * (c->ni + 2), clon_h.data());

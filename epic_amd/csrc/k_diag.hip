@@ -153,7 +153,6 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
   __syncthreads();
   if (T.live) {
     const double ptop = a[0], pbot = a[nk], kp1 = d.kappa + 1., lr = log(ptop / pbot);
-    const int nkay = nk - (ks - 1) + 1;
     const double *tx = d.sgth + 2 * nk + 1;  // tx[kay*(-2)] = sgth[2*(nk-kay)+1]
     for (int K = T.kk; K <= nk; K += T.KS) {
       const size_t q = ix3(d, K, J, I);
@@ -194,9 +193,11 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
           d.THETA2[q] = theta_of_sigma(d, d.sgth[2 * K], sigma);
         } else if (K >= ks) {
           // linear interpolation of theta against sigmatheta, table ordered
-          // from the bottom interface up (src/shared/epic_funcs_diag.c:3522-3562)
-          double sg = d.sgth[2 * K], h;
-          int kay = table_place_s(nkay, tx, -2, sg, &h);
+          // from the bottom interface up (src/shared/epic_funcs_diag.c:3522-3562);
+          // the interval is the same in every column, found once on the host
+          // by the same search
+          const double sg = d.sgth[2 * K], h = d.t2_h[K];
+          const int kay = d.t2_kay[K];
           double y0 = d.THETA[ix3(d, nk - kay, J, I)], y1 = d.THETA[ix3(d, nk - kay - 1, J, I)];
           d.THETA2[q] = y0 + (y1 - y0) * (sg - tx[-2 * kay]) / h;
         }
