@@ -162,6 +162,46 @@ __device__ __forceinline__ int table_place(int n, const double *x, double xx, do
   return il;
 }
 
+// table_place for a table whose neighbouring entries differ by more than two
+// of its largest ulps, in one direction: every comparison the search makes
+// (within one ulp of the larger magnitude, and the argument lies inside the
+// table's range once the end checks pass) is then monotone along the table,
+// so the binary search's interval is the unique i with x[i] "<=" xx and
+// x[i+1] ">" xx.  It is found here by walking from the guess g.
+__device__ __forceinline__ int table_place_near(int n, const double *x, double xx, int g, double *dx) {
+  int ascend = (x[n - 1] >= x[0]);
+  int lo = fcmp_d(xx, x[0]), hi = fcmp_d(xx, x[n - 1]);
+  if (ascend) {
+    if (lo <= 0) { *dx = x[1] - x[0]; return 0; }
+    if (hi >= 0) { *dx = x[n - 1] - x[n - 2]; return n - 2; }
+  } else {
+    if (lo >= 0) { *dx = x[1] - x[0]; return 0; }
+    if (hi <= 0) { *dx = x[n - 1] - x[n - 2]; return n - 2; }
+  }
+  g = g < 0 ? 0 : (g > n - 2 ? n - 2 : g);
+  if ((fcmp_d(xx, x[g]) >= 0) == ascend) {
+    while (g + 1 <= n - 2 && (fcmp_d(xx, x[g + 1]) >= 0) == ascend) g++;
+  } else {
+    while (g > 0 && !((fcmp_d(xx, x[g]) >= 0) == ascend)) g--;
+  }
+  *dx = x[g + 1] - x[g];
+  return g;
+}
+// whether x[0..n) is such a table (then table_place_near may replace table_place)
+__device__ __forceinline__ bool table_well_separated(int n, const double *x) {
+  double mx = 0.;
+  for (int e = 0; e < n; e++) mx = dmax(mx, fabs(x[e]));
+  int ex;
+  frexp(mx, &ex);
+  const double gap = 2. * ldexp(2.220446049250313e-16, ex + 1);
+  const bool ascend = (x[n - 1] >= x[0]);
+  for (int e = 0; e + 1 < n; e++) {
+    const double dd = ascend ? x[e + 1] - x[e] : x[e] - x[e + 1];
+    if (!(dd > gap)) return false;
+  }
+  return n >= 2;
+}
+
 // strided variant: table entries at x[k*stride]
 __device__ __forceinline__ int table_place_s(int n, const double *x, int st, double xx, double *dx) {
   int il = -1, iu = n;

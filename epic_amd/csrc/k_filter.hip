@@ -242,6 +242,9 @@ __global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb
     ty[e] = sy[e];
     tz[e] = sz[e];
   }
+  // the table's last slot (n <= nk < KT - 1) says whether the filter may
+  // place its points by walking from a guess
+  if (lane == 0) tx[KT - 1] = table_well_separated(n, sx) ? 1. : 0.;
 }
 
 // The rest of the 3-D filter in one launch: on every row J = jstart..nj of
@@ -277,6 +280,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
   // src/shared/epic_funcs_util.c:926-999): its coefficients are formed once
   // per row, by the same expressions, instead of once per cell
   double *tc2 = tab + 3 * KT, *tc3 = tab + 4 * KT;
+  const bool near = live && tab[KT - 1] != 0.;
   if (live)
     for (int k = lane; k < nt - 1; k += nq) {
       const double h = tx[k + 1] - tx[k];
@@ -292,7 +296,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
     for (int q = 0; q < 4; q++) {
       const int I = lane + q * nq + 1;
       double xx = geez(d, kind, K, J, I), h;
-      int k = table_place(nt, tx, xx, &h);
+      // entry e of the table is level kend - e of the column at Imin
+      int k = near ? table_place_near(nt, tx, xx, r.kend - K, &h) : table_place(nt, tx, xx, &h);
       double xr = xx - tx[k];
       double val = ty[k] + xr * (tz[k] + xr * (tc2[k] + xr * tc3[k]));
       if (kind == 2) val = exp(val);
