@@ -633,9 +633,10 @@ __global__ void __launch_bounds__(256) k_heating_adjust(const MemberDesc *D, int
     }
 }
 // in_thermo: the relaxation was formed by this step's k_thermo(full = 1),
-// and every other cell of HEAT3 still holds the zero of the priming call's
-// fill (no kernel writes outside K = 1..nk, J = 0..nj), so only the
-// layer-mean adjustment is left
+// and every other cell of HEAT3 is zero (a member's blob is cleared when it
+// is created, a drop-in upload brings the caller's array, which the
+// reference zeroes there, and no kernel writes outside K = 1..nk, J = 0..nj),
+// so only the layer-mean adjustment is left
 void L_heating(hipStream_t s, const Bucket &b, int adjust, bool in_thermo) {
   if (!in_thermo) {
     L_fill(s, b, F_HEAT3, 0.);
