@@ -135,8 +135,14 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
     for (int K = T.kk; K <= nk; K += T.KS)
       a[K] = (K == 0) ? P3[ix3(d, 0, J, I)] : d.dsgth[2 * K] * g * H[ix3(d, K, J, I)];
   __syncthreads();
-  if (T.live && T.kk == 0)
-    for (int K = 1; K <= nk; K++) a[K] = a[K - 1] + a[K];
+  // running sum in a register (a[K-1] + a[K], in order), not through LDS
+  if (T.live && T.kk == 0) {
+    double acc = a[0];
+    for (int K = 1; K <= nk; K++) {
+      acc = acc + a[K];
+      a[K] = acc;
+    }
+  }
   __syncthreads();
   if (T.live) {
     const double ptop = a[0], pbot = a[nk], lr = log(ptop / pbot);
@@ -431,10 +437,20 @@ __global__ void __launch_bounds__(256) k_gzu_core(const MemberDesc *D, int mb) {
   __syncthreads();
   if (threadIdx.x == 0) {
     // Bernoulli function integrated from the tie-point row both ways
+    // (running values in a register, in the same order)
     const int j0 = d.jtp;
-    bern[j0] = 0. + kn[j0];
-    for (int J = j0 + 1; J <= nj; J++) bern[J] = bern[J - 1] - pvh[J];
-    for (int J = j0 - 1; J >= 0; J--) bern[J] = bern[J + 1] + pvh[J + 1];
+    const double b0 = 0. + kn[j0];
+    bern[j0] = b0;
+    double b = b0;
+    for (int J = j0 + 1; J <= nj; J++) {
+      b = b - pvh[J];
+      bern[J] = b;
+    }
+    b = b0;
+    for (int J = j0 - 1; J >= 0; J--) {
+      b = b + pvh[J + 1];
+      bern[J] = b;
+    }
   }
   __syncthreads();
   for (int J = threadIdx.x; J <= nj + 1; J += blockDim.x) {

@@ -515,7 +515,24 @@ __global__ void __launch_bounds__(256) k_gz_mono(const MemberDesc *D, int mb) {
   FLAT2(0, d.nj + 1, 0, d.ni + 2);
   double *gz = d.gz;
   double below = gz[ix3(d, d.nk, J, I)];
-  for (int K = d.nk - 1; K >= 0; K--) {
+  // levels go eight at a time: their loads are issued together ahead of the
+  // chain (each level is read before it is written; the order of the chain
+  // is unchanged)
+  int K = d.nk - 1;
+  for (; K >= 7; K -= 8) {
+    double g[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) g[u] = gz[ix3(d, K - u, J, I)];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      double v;
+      if (fcmp_d(below, 0.) == 0) v = 1.e-3;
+      else v = dmax(g[u], below + fabs(below) * 1.e-5);
+      gz[ix3(d, K - u, J, I)] = v;
+      below = v;
+    }
+  }
+  for (; K >= 0; K--) {
     size_t q = ix3(d, K, J, I);
     double v;
     if (fcmp_d(below, 0.) == 0) v = 1.e-3;
