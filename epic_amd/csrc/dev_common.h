@@ -135,8 +135,15 @@ __device__ __forceinline__ double sgth_of_l(const MemberDesc &d, double theta, d
                                             double lr, int *bad) {
   if (p <= ptop) return theta;
   if (p <= pbot) {
-    double s = get_sigma_l(pbot, p, lr);
-    return f_sig(d, s) + g_sig(d, s) * theta;
+    const double s = get_sigma_l(pbot, p, lr);
+    // f_sig(s) + g_sig(s) * theta with g_sigma's exponential formed once (the
+    // same operations as the two calls, so the same result)
+    const bool mid = (s > d.sigma_sigma && s < 1.);
+    const double gm = mid ? d.g_coeff * (1. - exp(-d.alpha * (s - d.sigma_sigma))) : 0.;
+    const double g = (s <= d.sigma_sigma) ? 0. : (s >= 1. ? 1. : gm);
+    const double f = (s < 0.) ? d.zeta0 : (s <= d.sigma_sigma ? d.zeta0 + s * (d.zeta1 - d.zeta0)
+                                                              : (s < 1. ? (1. - gm) * (d.zeta0 + s * (d.zeta1 - d.zeta0)) : 0.));
+    return f + g * theta;
   }
   *bad = 1;
   return 0.;
