@@ -437,17 +437,40 @@ __global__ void __launch_bounds__(256) k_gzu_core(const MemberDesc *D, int mb) {
   __syncthreads();
   if (threadIdx.x == 0) {
     // Bernoulli function integrated from the tie-point row both ways
-    // (running values in a register, in the same order)
+    // (running values in a register, in the same order; the increments are
+    // read eight at a time ahead of the chain)
     const int j0 = d.jtp;
     const double b0 = 0. + kn[j0];
     bern[j0] = b0;
     double b = b0;
-    for (int J = j0 + 1; J <= nj; J++) {
+    int J = j0 + 1;
+    for (; J + 7 <= nj; J += 8) {
+      double q[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) q[u] = pvh[J + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        b = b - q[u];
+        bern[J + u] = b;
+      }
+    }
+    for (; J <= nj; J++) {
       b = b - pvh[J];
       bern[J] = b;
     }
     b = b0;
-    for (int J = j0 - 1; J >= 0; J--) {
+    J = j0 - 1;
+    for (; J - 7 >= 0; J -= 8) {
+      double q[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) q[u] = pvh[J - u + 1];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        b = b + q[u];
+        bern[J - u] = b;
+      }
+    }
+    for (; J >= 0; J--) {
       b = b + pvh[J + 1];
       bern[J] = b;
     }
