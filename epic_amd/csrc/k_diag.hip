@@ -135,10 +135,22 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
     for (int K = T.kk; K <= nk; K += T.KS)
       a[K] = (K == 0) ? P3[ix3(d, 0, J, I)] : d.dsgth[2 * K] * g * H[ix3(d, K, J, I)];
   __syncthreads();
-  // running sum in a register (a[K-1] + a[K], in order), not through LDS
+  // running sum in a register (a[K-1] + a[K], in order), not through LDS,
+  // its terms read eight at a time ahead of the chain
   if (T.live && T.kk == 0) {
     double acc = a[0];
-    for (int K = 1; K <= nk; K++) {
+    int K = 1;
+    for (; K + 7 <= nk; K += 8) {
+      double t8[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) t8[u] = a[K + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        acc = acc + t8[u];
+        a[K + u] = acc;
+      }
+    }
+    for (; K <= nk; K++) {
       acc = acc + a[K];
       a[K] = acc;
     }
@@ -545,7 +557,26 @@ __global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb, int C)
     M[ix3(d, nk, J, I)] = gprev + cpr * tt[nk] * rgas;
     const bool inner = (I >= 1 && I <= d.ni);
     int bad = 0;
-    for (int K = nk; K >= 1; K--) {
+    // the increments and temperatures are read eight levels at a time ahead
+    // of the chain
+    int K = nk;
+    for (; K - 7 >= 1; K -= 8) {
+      double i8[8], t8[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        i8[u] = inc[K - u];
+        t8[u] = tt[K - u - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        double gn = gprev + i8[u];
+        G[ix3(d, K - u - 1, J, I)] = gn;
+        M[ix3(d, K - u - 1, J, I)] = gn + cpr * t8[u] * rgas;
+        if (inner && gprev >= gn) bad = 1;
+        gprev = gn;
+      }
+    }
+    for (; K >= 1; K--) {
       double gn = gprev + inc[K];
       G[ix3(d, K - 1, J, I)] = gn;
       M[ix3(d, K - 1, J, I)] = gn + cpr * tt[K - 1] * rgas;
@@ -756,8 +787,24 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
     double *Wf = d.W3;
     Wf[ix3(d, 0, J, I)] = 0.;
     Wf[ix3(d, nk, J, I)] = 0.;
+    // (the three columns read eight levels at a time ahead of the chain)
     double S = 0.;
-    for (int K = 1; K < nk; K++) {
+    int K = 1;
+    for (; K + 7 < nk; K += 8) {
+      double s8[8], w8[8], f8[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        s8[u] = dsv[K + u];
+        w8[u] = wbv[K + u];
+        f8[u] = dfv[K + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (K + u >= 2) S = S + s8[u];
+        Wf[ix3(d, K + u, J, I)] = w8[u] - S * f8[u];
+      }
+    }
+    for (; K < nk; K++) {
       if (K >= 2) S = S + dsv[K];
       Wf[ix3(d, K, J, I)] = wbv[K] - S * dfv[K];
     }
