@@ -48,6 +48,11 @@ struct DropIn {
   bool resident = false;  // the device copy is newer than or equal to var.*
   bool dirty_host = false;  // var.* lags the device copy (lazy mode)
   int ni = 0, nj = 0, nk = 0;
+  // the member's time planes and itime as the device holds them (after a
+  // step the device rotates them as the host does and counts itime on), so a
+  // call whose IT_* and itime match sends nothing
+  bool planes_known = false;
+  int planes[4] = {0, 0, 0, 0};
 };
 DropIn g_di;
 
@@ -188,6 +193,7 @@ void create_member(const struct epic_planet_abi *planet) {
   int slot = engine_add_member(g_di.E, &c, var.hdry.value, false);
   if (slot < 0) fail(engine_errbuf(g_di.E));
   g_di.slot = slot;
+  g_di.planes_known = false;
   g_di.ni = grid.ni; g_di.nj = grid.nj; g_di.nk = grid.nk;
   g_di.resident = false;
 }
@@ -205,6 +211,7 @@ extern "C" void epic_amd_flush(void) {
 extern "C" void epic_amd_invalidate(void) {
   if (g_di.E && g_di.slot >= 0) epic_engine_remove(g_di.E, g_di.slot);
   g_di.slot = -1;
+  g_di.planes_known = false;
   g_di.resident = false;
   g_di.dirty_host = false;
 }
@@ -227,13 +234,20 @@ extern "C" void timestep(struct epic_planet_abi *planet, int action, double **Bu
     g_di.resident = true;
   }
   int planes[4] = {IT_ZERO, IT_MINUS1, IT_MINUS2, (int)grid.itime};
-  if (epic_engine_set_time_planes(g_di.E, g_di.slot, planes) != 0) fail(engine_errbuf(g_di.E));
+  if (!g_di.planes_known || memcmp(planes, g_di.planes, sizeof(planes)) != 0) {
+    if (epic_engine_set_time_planes(g_di.E, g_di.slot, planes) != 0) fail(engine_errbuf(g_di.E));
+  }
 
   if (action == EPIC_SYNC_DIAGS_ONLY) {
     if (engine_sync_member(g_di.E, g_di.slot) != 0) fail(engine_errbuf(g_di.E));
+    memcpy(g_di.planes, planes, sizeof(planes));
   } else {
     if (epic_engine_step(g_di.E, 1) != 0) fail(engine_errbuf(g_di.E));
+    // the step's k_rotate: (it0, itm1, itm2) <- (itm2, it0, itm1), itime + 1
+    const int rot[4] = {planes[2], planes[0], planes[1], planes[3] + 1};
+    memcpy(g_di.planes, rot, sizeof(rot));
   }
+  g_di.planes_known = true;
   if (epic_engine_sync(g_di.E) != 0) fail(engine_errbuf(g_di.E));
 
   if (action == EPIC_STEP_PROGS_AND_SYNC_DIAGS) {
