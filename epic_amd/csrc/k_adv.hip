@@ -4,11 +4,11 @@
 //
 // Directions are split as in the reference: vertical column pass, then a
 // meridional pass on every level, then a zonal pass on a zonally filtered
-// copy.  The column pass runs one lane per (J,I) column.  The horizontal
-// passes run one lane per cell and recompute the few neighbouring fluxes
-// they need (stencil radius 2-3) instead of staging intermediate planes;
-// every intermediate value is formed by the same expression as in the
-// reference, so nothing changes but where it is computed.
+// copy.  The column pass works on LDS column tiles, the meridional pass on
+// LDS tiles of a few rows by up to 64 columns (each intermediate of the
+// tile formed once), the zonal pass on LDS rows; every intermediate value
+// is formed by the same expression as in the reference, so nothing changes
+// but where it is computed.
 //
 // The three advected fields (unity on the pressure grid, theta, hdry) are
 // independent of one another until the theta correction that follows them,
