@@ -4,7 +4,9 @@
 //   store_pgrad_vars  src/shared/epic_funcs_diag.c:3595-3950  (column scans)
 //   gz_from_u         src/shared/epic_funcs_diag.c:4460-4610
 //   calc_heating      src/shared/epic_funcs_diag.c:4793-4869 +
-//                     src/shared/epic_sensible_heating.c:51-463 (Held-Suarez)
+//                     src/shared/epic_sensible_heating.c:51-463 (Held-Suarez,
+//                     and the hot Jupiter's day-night relaxation; inside the
+//                     step it is formed by the second k_thermo)
 //   calc_w            src/shared/epic_flux.c:123-335           (column scan)
 //   vorticity         src/shared/epic_funcs_diag.c:4131-4445
 //   divergence        src/shared/epic_funcs_diag.c:4064-4117
