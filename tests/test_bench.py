@@ -1,0 +1,42 @@
+"""bench.py's host-side pieces, without a GPU: the one-GPU refusal, the
+algorithmic bytes of a launch and the lookup of the committed counter
+summaries (including kernels whose launch name hides template arguments)."""
+import os
+import subprocess
+import sys
+
+import bench
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+def test_more_than_one_gpu_is_refused_before_any_gpu_work():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one MI355X" in (r.stdout + r.stderr)
+
+
+def test_kernel_bytes_is_bytes_per_cell_times_fields_times_cells():
+    ni, nj, nk = 256, 128, 20
+    per_cell, fields = bench.KERNEL_BYTES["k_hv_apply"]
+    assert bench.kernel_bytes("k_hv_apply<T>", ni, nj, nk, 1) == per_cell * fields * ni * (nj + 1) * nk
+    # template arguments do not change the bytes; unknown kernels have none
+    assert bench.kernel_bytes("k_hv_apply<float>", ni, nj, nk, 1) == bench.kernel_bytes("k_hv_apply<T>", ni, nj, nk, 1)
+    assert bench.kernel_bytes("k_no_such_kernel", ni, nj, nk, 1) is None
+
+
+def test_measured_traffic_lookup():
+    t, src = bench.measured_traffic("jupiter_256x128x20", "fp64", "k_hv_apply<T>")
+    assert t and t > 0 and src.endswith("traffic_jupiter_256x128x20_fp64.json")
+    # the zonal hyperviscosity's profile name carries its register layout
+    t, _ = bench.measured_traffic("hot_jupiter_1024x512x64", "fp64", "k_zonal_hv<T>")
+    assert t and t > 0
+    assert bench.measured_traffic("jupiter_256x128x20", "mixed", "k_hv_apply<T>") == (None, None)
+
+
+def test_filter_band_rows():
+    # rows with |lat| >= 45 deg: half of the h-rows on these globes
+    assert bench.filtered_rows(128) == 64
+    assert bench.filtered_rows(64) == 32
