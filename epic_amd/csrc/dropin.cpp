@@ -150,6 +150,13 @@ void check_config(const struct epic_planet_abi *planet) {
   need(var.u.value && var.v.value && var.u.tendency && var.v.tendency && var.hdry.value && var.theta.value,
        "prognostic arrays are not allocated");
   need(planet->x_h2 == 0., "the ortho/para hydrogen thermodynamics (planet->x_h2 > 0) are not implemented");
+  // calc_heating: Held-Suarez's analytic relaxation and the hot Jupiter's
+  // day-night one are built; every other planet relaxes toward its t_vs_p
+  // sounding with t_cool_table's times (src/shared/epic_sensible_heating.c:
+  // 114-130, 294-311), which the GPU step does not have
+  need(!grid.newt_cool_on || strcmp(planet->name, "held_suarez") == 0 || strcmp(planet->name, "hot_jupiter") == 0,
+       "grid.newt_cool_on: Newtonian cooling from t_cool_table and the t_vs_p sounding is not implemented "
+       "(only held_suarez and hot_jupiter)");
 }
 
 void create_member(const struct epic_planet_abi *planet) {

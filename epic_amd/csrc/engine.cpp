@@ -334,6 +334,11 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
     snprintf(E->err, sizeof(E->err), "tie-point row jtp=%d outside 0..%d", c->jtp, c->nj);
     return -1;
   }
+  if (c->newt_cool_on && strcmp(c->planet_name, "held_suarez") != 0 && strcmp(c->planet_name, "hot_jupiter") != 0) {
+    snprintf(E->err, sizeof(E->err), "Newtonian cooling from t_cool_table and the t_vs_p sounding (planet %s) "
+             "is not implemented", c->planet_name);
+    return -1;
+  }
   if (c->nu[6] != 0. || c->nu[8] != 0.) {
     snprintf(E->err, sizeof(E->err), "only del^4 hyperviscosity is implemented");
     return -1;

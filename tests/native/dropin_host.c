@@ -7,7 +7,7 @@
  * timestep(STEP_PROGS_AND_SYNC_DIAGS) NSTEPS times, advancing grid.itime as
  * epic_main does.
  *
- * Usage: dropin_host PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT LAZY
+ * Usage: dropin_host PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT LAZY [NAME]
  * OUT holds the dump format of the oracle's reference harness: an int32
  * header {magic, ni, nj, nk, 1, nfields, nelem3d, nelem2d, 0} and one dump
  * of the final state.  LAZY=1 selects epic_amd_set_sync_mode(1) and a
@@ -32,7 +32,8 @@ static double *arr(size_t n) {
 
 int main(int argc, char **argv) {
   if (argc < 14) {
-    fprintf(stderr, "usage: %s PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT LAZY\n", argv[0]);
+    fprintf(stderr, "usage: %s PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT LAZY [NAME]\n",
+            argv[0]);
     return 2;
   }
   epic_case_spec sp;
@@ -57,7 +58,9 @@ int main(int argc, char **argv) {
 
   static struct epic_planet_abi pl;
   pl.index = c.planet_index;
-  snprintf(pl.name, sizeof(pl.name), "%s", c.planet_name);
+  /* NAME renames the planet while keeping the case's settings: a caller's
+   * configuration that timestep() must refuse before it touches the GPU */
+  snprintf(pl.name, sizeof(pl.name), "%s", argc > 14 ? argv[14] : c.planet_name);
   snprintf(pl.type, sizeof(pl.type), "%s", c.planet_type);
   pl.re = c.re; pl.rp = c.rp; pl.omega_sidereal = pl.omega_synodic = c.omega;
   pl.cp = c.cp; pl.cpr = c.cpr; pl.rgas = c.rgas; pl.kappa = c.kappa;

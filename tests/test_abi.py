@@ -72,3 +72,20 @@ def test_library_leaves_the_callers_globals_weak():
         if len(p) >= 8 and p[7] in ("grid", "var", "IT_ZERO", "IT_MINUS1", "IT_MINUS2", "solar_longitude",
                                     "epic_error"):
             assert p[4] == "WEAK" and p[6] == "UND", line
+
+
+@pytest.mark.parametrize("name", ["jupiter", "venus"])
+def test_dropin_refuses_newtonian_cooling_it_does_not_have(tmp_path, name):
+    """With grid.newt_cool_on, a planet other than held_suarez or hot_jupiter
+    relaxes toward its t_vs_p sounding with t_cool_table's times in the
+    reference; timestep() stops with an EPIC-style error before any GPU work
+    instead of stepping without that forcing."""
+    exe = os.path.join(ROOT, "tests", "native", "_bin", "dropin_host")
+    if not os.path.exists(exe):
+        pytest.skip("dropin_host not built")
+    # the Held-Suarez case (newt_cool_on = 1) under another planet's name
+    args = [exe, "0", "32", "16", "8", "600", "1", "0.5", "0", "0.1", "0", "1", str(tmp_path / "out"), "0", name]
+    r = subprocess.run(args, cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert "Newtonian cooling from t_cool_table" in r.stderr
+    assert not os.path.exists(tmp_path / "out")
