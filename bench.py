@@ -12,8 +12,9 @@ device time in a profiled pass (HIP events around every launch, on the
 engine's stream): achieved = its algorithmic bytes per launch / its mean
 launch duration, with the per-cell byte counts of KERNEL_BYTES.
 `step_roofline` prices the whole step: N_pts*steps/s * 112 B (FP64; 88 B in
-the mixed mode) against 8 TB/s.  `cpu_baseline` is the FP64 CPU
-restatement of the step (oracle/) timed on this host, one thread, on a
+the mixed mode) against 8 TB/s.  `cpu_baseline` is the reference's own
+timestep() built for one CPU (oracle/_ref/epic_ref, where it was built) or
+else the oracle's FP64 CPU restatement, timed on this host, one thread, on a
 bounded number of steps of the same workload.
 """
 import argparse
@@ -93,6 +94,37 @@ def kernel_bytes(name, ni, nj, nk, members):
     else:
         cells = ni * (nj + 1) * nk
     return per_cell * cells * fields * members
+
+
+REF_EXE = os.path.join(HERE, "oracle", "_ref", "epic_ref")
+
+
+def cpu_baseline_reference(planet, ni, nj, nk, dt, budget_s):
+    """The reference's own timestep() on one core: its C sources built
+    single-processor in FP64 (EPIC_PRECISION=8, gcc -O2) by oracle/Makefile
+    and driven by oracle/ref_harness.c from the same seeded case, which
+    times its step loop itself.  A 2-step run sizes the sample to about
+    budget_s of stepping."""
+    import re
+    import subprocess
+    import tempfile
+    import epic_amd as ea
+    sp = spec_of(ea, planet, ni, nj, nk, dt, 1)
+
+    def run(n):
+        with tempfile.TemporaryDirectory() as td:
+            args = [REF_EXE] + [str(x) for x in (sp.planet, sp.ni, sp.nj, sp.nk, sp.dt, sp.seed, sp.nu4_nondim,
+                                                   sp.nudiv_nondim, sp.noise_theta, sp.noise_uv, n)]
+            r = subprocess.run(args + [os.path.join(td, "out")], cwd=td, capture_output=True, text=True,
+                               timeout=4 * budget_s + 300)
+        m = re.search(r"timed (\d+) steps in ([0-9.eE+-]+) s", r.stdout)
+        if r.returncode != 0 or not m:
+            raise RuntimeError(f"{REF_EXE} failed ({r.returncode}): {r.stderr[-400:]}")
+        return int(m.group(1)), float(m.group(2))
+
+    n, el = run(2)
+    n, el = run(max(2, min(200, int(budget_s / (el / n)))))
+    return ni * (nj + 1) * nk * n / el, n, el
 
 
 def cpu_baseline(planet, ni, nj, nk, dt, budget_s):
@@ -217,9 +249,14 @@ def main():
                       "{held_suarez, jupiter, saturn, venus} x U(200,2000) steps, seed 2024"
         out["serving"] = summ
     if not args.no_cpu:
-        v, n, cel = cpu_baseline(planet, ni, nj, nk, dt, args.cpu_seconds)
-        out["cpu_baseline"] = {"value": v, "unit": "grid-pt-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{n} FP64 steps of one {ni}x{nj}x{nk} member in {cel:.1f} s"}
+        if os.path.exists(REF_EXE):
+            v, n, cel = cpu_baseline_reference(planet, ni, nj, nk, dt, args.cpu_seconds)
+            kind, what = "reference", "the reference's own timestep() (single-processor FP64 build)"
+        else:
+            v, n, cel = cpu_baseline(planet, ni, nj, nk, dt, args.cpu_seconds)
+            kind, what = "port", "the oracle's CPU restatement"
+        out["cpu_baseline"] = {"value": v, "unit": "grid-pt-steps/s", "cores": 1, "kind": kind,
+                               "sample": f"{n} FP64 steps of one {ni}x{nj}x{nk} member by {what} in {cel:.1f} s"}
     print(json.dumps(out))
 
 

@@ -5,6 +5,8 @@ import os
 import subprocess
 import sys
 
+import pytest
+
 import bench
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -40,3 +42,14 @@ def test_filter_band_rows():
     # rows with |lat| >= 45 deg: half of the h-rows on these globes
     assert bench.filtered_rows(128) == 64
     assert bench.filtered_rows(64) == 32
+
+
+def test_reference_cpu_leg_times_the_reference_build():
+    """bench.py's CPU baseline runs the reference's own timestep() where
+    oracle/_ref/epic_ref was built, on the same seeded case."""
+    if not os.path.exists(bench.REF_EXE):
+        pytest.skip("oracle/_ref/epic_ref not built")
+    import epic_amd as ea
+    v, n, el = bench.cpu_baseline_reference(ea.HELD_SUAREZ, 32, 16, 8, 600, 0.5)
+    assert n >= 2 and el > 0
+    assert abs(v - 32 * 17 * 8 * n / el) < 1e-6 * v
