@@ -1,6 +1,7 @@
 """bench.py's host-side pieces, without a GPU: the one-GPU refusal, the
-algorithmic bytes of a launch and the lookup of the committed counter
-summaries (including kernels whose launch name hides template arguments)."""
+algorithmic bytes of a launch, the lookup of the committed counter
+summaries (including kernels whose launch name hides template arguments)
+and the CPU baseline's run of the reference build."""
 import os
 import subprocess
 import sys
