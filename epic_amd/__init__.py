@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "libepic_amd.so")
 # planets (enum epic_planet_kind)
 HELD_SUAREZ, JUPITER_LIKE, SATURN_LIKE, VENUS_LIKE, HOT_JUPITER = 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
+# timestep() actions (include/epic.h:294-295)
+STEP_PROGS_AND_SYNC_DIAGS, SYNC_DIAGS_ONLY = 0, 1
 
 FIELDS = ["U", "V", "HDRY", "THETA", "P3", "P2", "H3", "THETA2", "T2", "T3", "RHO2", "RHO3",
           "EXNER3", "GZ3", "MONT3", "PV3", "DIV_UV3", "W3", "HEAT3", "DZDT3", "DUDT0", "DUDT1",
@@ -23,17 +25,89 @@ FIELDS = ["U", "V", "HDRY", "THETA", "P3", "P2", "H3", "THETA2", "T2", "T3", "RH
 FIELD = {n: i for i, n in enumerate(FIELDS)}
 
 
+K_SPONGE_DEFAULT = -1  # the reference `initial` program's choice of k_sponge
+
+
 class CaseSpec(ctypes.Structure):
     """epic_case_spec of include/epic_amd.h."""
     _fields_ = [("planet", ctypes.c_int), ("ni", ctypes.c_int), ("nj", ctypes.c_int),
                 ("nk", ctypes.c_int), ("dt", ctypes.c_int), ("seed", ctypes.c_uint64),
                 ("nu4_nondim", ctypes.c_double), ("nudiv_nondim", ctypes.c_double),
-                ("noise_theta", ctypes.c_double), ("noise_uv", ctypes.c_double)]
+                ("noise_theta", ctypes.c_double), ("noise_uv", ctypes.c_double),
+                ("k_sponge", ctypes.c_int), ("newt_cool", ctypes.c_int),
+                ("newt_cool_adjust", ctypes.c_int), ("drag_v", ctypes.c_int),
+                ("drag_zonal_avg", ctypes.c_int), ("tau_drag", ctypes.c_double),
+                ("prandtl", ctypes.c_double), ("nu6_nondim", ctypes.c_double),
+                ("nu8_nondim", ctypes.c_double)]
+
+    # the reference's switches as keyword arguments of spec(), in the order
+    # of the command-line options of oracle/ref_harness.c
+    OPTIONS = ("k_sponge", "newt_cool", "newt_cool_adjust", "drag_v", "drag_zonal_avg", "tau_drag",
+               "prandtl", "nu6_nondim", "nu8_nondim")
+
+    def options(self):
+        """{option: value} of the switches that differ from a zeroed spec."""
+        return {k: getattr(self, k) for k in self.OPTIONS if getattr(self, k)}
+
+    def args(self):
+        """The ten positional values (planet .. noise_uv)."""
+        return (self.planet, self.ni, self.nj, self.nk, self.dt, self.seed, self.nu4_nondim,
+                self.nudiv_nondim, self.noise_theta, self.noise_uv)
+
+    def __repr__(self):
+        extra = "".join(f", {k}={v}" for k, v in self.options().items())
+        return f"spec{self.args()}{extra}"
 
 
 def spec(planet=HELD_SUAREZ, ni=64, nj=32, nk=10, dt=600, seed=1, nu4=0.5, nudiv=0.0,
-         noise_theta=0.1, noise_uv=0.0):
-    return CaseSpec(planet, ni, nj, nk, dt, seed, nu4, nudiv, noise_theta, noise_uv)
+         noise_theta=0.1, noise_uv=0.0, k_sponge=K_SPONGE_DEFAULT, newt_cool=0, newt_cool_adjust=0,
+         drag_v=0, drag_zonal_avg=0, tau_drag=0.0, prandtl=0.0, nu6=0.0, nu8=0.0):
+    """A case: planet, grid, dt and seed, the del^4 and divergence-damping
+    fractions, the initial noise, and the reference's sponge, drag and
+    Newtonian-cooling switches (k_sponge defaults to the reference `initial`
+    program's choice: none for Held-Suarez, 1..4 layers otherwise)."""
+    return CaseSpec(planet, ni, nj, nk, dt, seed, nu4, nudiv, noise_theta, noise_uv, k_sponge, newt_cool,
+                    newt_cool_adjust, drag_v, drag_zonal_avg, tau_drag, prandtl, nu6, nu8)
+
+
+_D, _I, _P = ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+
+
+class Case(ctypes.Structure):
+    """epic_case of include/epic_amd.h (built by epic_case_build)."""
+    _fields_ = ([("planet_name", ctypes.c_char * 32), ("planet_type", ctypes.c_char * 16), ("planet_index", _I)] +
+                [(n, _D) for n in ("re", "rp", "omega", "cp", "rgas", "cpr", "kappa", "GM", "J2", "x_h2",
+                                   "kinvisc")] +
+                [(n, _I) for n in ("ni", "nj", "nk", "dt", "jtp", "k_sigma", "k_sponge", "newt_cool_on",
+                                   "newt_cool_adjust", "drag_v", "drag_zonal_avg")] +
+                [(n, _D) for n in ("dln", "dlt", "dy0", "globe_lonbot", "globe_lontop", "globe_latbot",
+                                   "globe_lattop", "press0", "pbot", "ptop", "zeta0", "zeta1", "hybrid_alpha",
+                                   "sigma_sigma", "topo_scale", "prandtl", "tau_drag", "nudiv_nondim")] +
+                [("nu", _D * 9), ("nu_nondim", _D * 9)] +
+                [(n, _D) for n in ("teq_contrast", "teq_p_contrast", "tau_rad", "ref_theta_b", "ref_pb",
+                                   "ref_a_over_kappa")] +
+                [(n, _I) for n in ("Iadim", "Jadim", "Kadim", "Nelem2d", "Nelem3d")] +
+                [(n, _P) for n in ("lon", "lat", "rln", "rlt", "f", "m", "n", "mn", "g", "sigmatheta", "dsgth",
+                                   "dsgth_inv", "p_ref", "theta_ref", "h_min", "t_sponge_inv", "u", "v", "hdry",
+                                   "theta", "p3", "u_spinup", "gz_surface")])
+
+    def array(self, name, n):
+        """A numpy copy of the first n values of one of the case's arrays."""
+        return np.ctypeslib.as_array(getattr(self, name), shape=(n,)).copy()
+
+
+def build_case(sp):
+    """epic_case_build of a spec; free it with free_case."""
+    c = Case()
+    rc = lib().epic_case_build(ctypes.byref(sp), ctypes.byref(c))
+    if rc:
+        lib().epic_case_free(ctypes.byref(c))
+        raise EpicError(f"epic_case_build failed ({rc})")
+    return c
+
+
+def free_case(c):
+    lib().epic_case_free(ctypes.byref(c))
 
 
 def dims(sp):
@@ -79,6 +153,9 @@ def lib():
         L.epic_copy_probe.argtypes = [ctypes.c_size_t, ci, ctypes.POINTER(cd)]
         L.epic_member_footprint.argtypes = [ci, ci, ci, ci]
         L.epic_member_footprint.restype = ctypes.c_size_t
+        L.epic_case_build.argtypes = [ctypes.POINTER(CaseSpec), ctypes.POINTER(Case)]
+        L.epic_case_build.restype = ci
+        L.epic_case_free.argtypes = [ctypes.POINTER(Case)]
         _lib = L
     return _lib
 

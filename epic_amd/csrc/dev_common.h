@@ -38,6 +38,9 @@ struct MemberDesc {
   int is_hj;
   double pbot, teq_contrast, teq_p_contrast, tau_rad, ref_theta_b, ref_pbk, ref_a_over_kappa;
   double hj_lnref;  // log(pbot / teq_p_contrast)
+  // sum of the cell areas 1/mn over J = 0..nj, I = 1..ni in that order (the
+  // layer-mean removal of the Newtonian relaxation)
+  double heat_area;
   // host-libm tables: cos(lat), cos(lat)^4 per h-row J, cos(lon) per column I
   const double *coslat_h, *coslat4_h, *coslon_h;
   // per layer K >= k_sigma: THETA2's interval in the sigmatheta table and its width

@@ -96,7 +96,7 @@ std::vector<Xfer> transfers(const MemberDesc &d) {
       {var.mont3.value, d.MONT3, N3, true},   {var.heat3.value, d.HEAT3, N3, true},
       {var.pv3.value, d.PV3, N3, true},       {var.div_uv3.value, d.DIV, N3, true},
       {var.w3.value, d.W3, N3, true},         {var.dzdt3.value, d.DZDT3, N3, true},
-      {var.u_spinup.value, d.USPIN, N3, false}, {var.gz_surface.value, d.GZS, N2, false},
+      {var.u_spinup.value, d.USPIN, N3, false}, {var.gz_surface.value, d.GZS, N2, true},
       {var.ri2.value, d.RI2, N3, true, false},
   };
   return x;
@@ -150,13 +150,14 @@ void check_config(const struct epic_planet_abi *planet) {
   need(var.u.value && var.v.value && var.u.tendency && var.v.tendency && var.hdry.value && var.theta.value,
        "prognostic arrays are not allocated");
   need(planet->x_h2 == 0., "the ortho/para hydrogen thermodynamics (planet->x_h2 > 0) are not implemented");
-  // calc_heating: Held-Suarez's analytic relaxation and the hot Jupiter's
-  // day-night one are built; every other planet relaxes toward its t_vs_p
-  // sounding with t_cool_table's times (src/shared/epic_sensible_heating.c:
-  // 114-130, 294-311), which the GPU step does not have
-  need(!grid.newt_cool_on || strcmp(planet->name, "held_suarez") == 0 || strcmp(planet->name, "hot_jupiter") == 0,
+  // calc_heating: Held-Suarez's analytic relaxation is the only one of the
+  // reference's that is built here.  Every other planet, a hot Jupiter
+  // included, relaxes toward its t_vs_p sounding with t_cool_table's times
+  // (src/shared/epic_sensible_heating.c:114-130, 294-311); the engine's own
+  // day-night relaxation is not the reference's and is not offered here.
+  need(!grid.newt_cool_on || strcmp(planet->name, "held_suarez") == 0,
        "grid.newt_cool_on: Newtonian cooling from t_cool_table and the t_vs_p sounding is not implemented "
-       "(only held_suarez and hot_jupiter)");
+       "(only held_suarez)");
 }
 
 void create_member(const struct epic_planet_abi *planet) {

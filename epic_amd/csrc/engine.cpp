@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/epic_amd.h"
+#include "engine_internal.h"
 #include "kernels.h"
 #include "tables.h"
 
@@ -257,6 +258,11 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
   L_rotate(s, b);
 }
 
+void member_record(hipStream_t s, const Bucket &b, bool sync_only, bool hv, bool divd, bool adjust) {
+  if (sync_only) rec_sync(s, b, adjust);
+  else rec_step(s, b, hv, divd, adjust);
+}
+
 static void bucket_flags(epic_engine *E, const Bucket &b, bool &hv, bool &divd, bool &adj) {
   hv = divd = adj = false;
   for (int i = 0; i < b.nm; i++) {
@@ -324,48 +330,53 @@ static int table_place_s_host(int n, const double *x, int st, double xx, double 
   return il;
 }
 
-int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src, bool prime) {
-  if (!E || !c) return -1;
+// Whether the engine implements a case; writes the reason into err (cap
+// bytes) and returns -1 when it does not.
+int member_check(const epic_case *c, int precision, char *err, size_t cap) {
   if (c->ni > 1024 || c->ni < 8 || (c->ni & (c->ni - 1)) || c->nk > 64 || c->nk < 4) {
-    snprintf(E->err, sizeof(E->err), "unsupported grid %dx%dx%d", c->ni, c->nj, c->nk);
+    snprintf(err, cap, "unsupported grid %dx%dx%d", c->ni, c->nj, c->nk);
     return -1;
   }
   if (strcmp(c->planet_type, "gas-giant") == 0 && (c->jtp < 0 || c->jtp > c->nj)) {
-    snprintf(E->err, sizeof(E->err), "tie-point row jtp=%d outside 0..%d", c->jtp, c->nj);
+    snprintf(err, cap, "tie-point row jtp=%d outside 0..%d", c->jtp, c->nj);
     return -1;
   }
   if (c->newt_cool_on && strcmp(c->planet_name, "held_suarez") != 0 && strcmp(c->planet_name, "hot_jupiter") != 0) {
-    snprintf(E->err, sizeof(E->err), "Newtonian cooling from t_cool_table and the t_vs_p sounding (planet %s) "
+    snprintf(err, cap, "Newtonian cooling from t_cool_table and the t_vs_p sounding (planet %s) "
              "is not implemented", c->planet_name);
     return -1;
   }
   if (c->nu[6] != 0. || c->nu[8] != 0.) {
-    snprintf(E->err, sizeof(E->err), "only del^4 hyperviscosity is implemented");
+    snprintf(err, cap, "only del^4 hyperviscosity is implemented");
     return -1;
   }
-  if (E->precision != EPIC_FP64 && E->precision != EPIC_MIXED) {
-    snprintf(E->err, sizeof(E->err), "unknown precision mode %d", E->precision);
+  if (c->k_sponge < 0 || c->k_sponge >= c->nk) {
+    snprintf(err, cap, "k_sponge=%d outside 0..nk-1", c->k_sponge);
     return -1;
   }
-  const bool mixed = (E->precision == EPIC_MIXED);
-  int slot = -1;
-  for (int s = 0; s < E->max_members; s++)
-    if (!E->m[s].active) { slot = s; break; }
-  if (slot < 0) return -3;
-  epic_tables T;
-  epic_tables_build(c, &T);
-  struct TablesGuard { epic_tables *t; ~TablesGuard() { epic_tables_free(t); } } tg{&T};
+  if (precision != EPIC_FP64 && precision != EPIC_MIXED) {
+    snprintf(err, cap, "unknown precision mode %d", precision);
+    return -1;
+  }
+  return 0;
+}
+
+// The arrays of one member's descriptor in their fixed order, with the
+// scalars of the descriptor filled in.  Each part names the descriptor's
+// pointer, its size, its initial contents (null: zero) and, for the arrays a
+// caller of the engine reads and writes, the field id (EPIC_F_*; the tendency
+// planes as EPIC_F_DUDT0 / EPIC_F_DVDT0, three planes each); working memory
+// and tables have field -1.  The host arrays that parts point into live in
+// `hold` until the copies are done.
+void member_plan(const epic_case *c, bool mixed, const epic_tables &T, MemberDesc &d, std::vector<MemberPart> &parts,
+                 MemberHold &hold) {
+  memset(&d, 0, sizeof(d));
+  parts.clear();
   const size_t N3 = (size_t)c->Nelem3d, N2 = (size_t)c->Nelem2d;
   const int nJ = 2 * (c->nj + 1) + 1, nK = 2 * (c->nk + 1) + 1;
-  // layout of the member blob
-  struct Part { void **dst; size_t bytes; const void *src; };
-  Member &M = E->m[slot];
-  memset(&M.h, 0, sizeof(M.h));
-  MemberDesc &d = M.h;
-  std::vector<Part> parts;
-  double *dummy = nullptr;
-  (void)dummy;
-  auto add = [&](void *pp, size_t bytes, const void *src) { parts.push_back(Part{(void **)pp, bytes, src}); };
+  auto add = [&](void *pp, size_t bytes, const void *src, int field = -1) {
+    parts.push_back(MemberPart{(void **)pp, bytes, src, field});
+  };
   const double *nul = nullptr;
   add(&d.lat, sizeof(double) * nJ, c->lat);
   add(&d.lon, sizeof(double) * (2 * (c->ni + 1) + 1), c->lon);
@@ -384,27 +395,30 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   // Newtonian relaxations, with the host libm like the oracle's (every case
   // carries them; small).  Rows 0..nj and columns 0..ni are defined; the
   // last entries stay 0.
-  std::vector<double> cl_h(c->nj + 2, 0.), cl4_h(c->nj + 2, 0.), clon_h(c->ni + 2, 0.);
+  hold.cl_h.assign(c->nj + 2, 0.);
+  hold.cl4_h.assign(c->nj + 2, 0.);
+  hold.clon_h.assign(c->ni + 2, 0.);
   for (int J = 0; J <= c->nj; J++) {
-    cl_h[J] = cos(c->lat[2 * J + 1] * (M_PI / 180.));
-    cl4_h[J] = pow(cl_h[J], 4.);
+    hold.cl_h[J] = cos(c->lat[2 * J + 1] * (M_PI / 180.));
+    hold.cl4_h[J] = pow(hold.cl_h[J], 4.);
   }
-  for (int I = 0; I <= c->ni; I++) clon_h[I] = cos(c->lon[2 * I + 1] * (M_PI / 180.));
+  for (int I = 0; I <= c->ni; I++) hold.clon_h[I] = cos(c->lon[2 * I + 1] * (M_PI / 180.));
   // THETA2 above the sigma region interpolates theta against sigmatheta:
   // the interval of layer K in the interface table (set_p2etc,
   // src/shared/epic_funcs_diag.c:3522-3562) is the same in every column
-  std::vector<int> t2_kay(c->nk + 2, 0);
-  std::vector<double> t2_h(c->nk + 2, 1.);
+  hold.t2_kay.assign(c->nk + 2, 0);
+  hold.t2_h.assign(c->nk + 2, 1.);
   {
     const int nk = c->nk, nkay = nk - (c->k_sigma - 1) + 1;
     const double *tx = c->sigmatheta + 2 * nk + 1;
-    for (int K = c->k_sigma; K <= nk; K++) t2_kay[K] = table_place_s_host(nkay, tx, -2, c->sigmatheta[2 * K], &t2_h[K]);
+    for (int K = c->k_sigma; K <= nk; K++)
+      hold.t2_kay[K] = table_place_s_host(nkay, tx, -2, c->sigmatheta[2 * K], &hold.t2_h[K]);
   }
-  add(&d.t2_kay, sizeof(int) * (c->nk + 2), t2_kay.data());
-  add(&d.t2_h, sizeof(double) * (c->nk + 2), t2_h.data());
-  add(&d.coslat_h, sizeof(double) * (c->nj + 2), cl_h.data());
-  add(&d.coslat4_h, sizeof(double) * (c->nj + 2), cl4_h.data());
-  add(&d.coslon_h, sizeof(double) * (c->ni + 2), clon_h.data());
+  add(&d.t2_kay, sizeof(int) * (c->nk + 2), hold.t2_kay.data());
+  add(&d.t2_h, sizeof(double) * (c->nk + 2), hold.t2_h.data());
+  add(&d.coslat_h, sizeof(double) * (c->nj + 2), hold.cl_h.data());
+  add(&d.coslat4_h, sizeof(double) * (c->nj + 2), hold.cl4_h.data());
+  add(&d.coslon_h, sizeof(double) * (c->ni + 2), hold.clon_h.data());
   add(&d.t_sponge, sizeof(double) * (c->k_sponge + 1), c->t_sponge_inv);
   add(&d.mn_inv, sizeof(double) * nJ, T.mn_inv);
   add(&d.hl_h, sizeof(double) * T.n_hl, T.hl_h);
@@ -420,29 +434,35 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   add(&d.tr_f, sizeof(double) * T.n_tr, T.trf);
   add(&d.tr_i, sizeof(double) * T.n_tr, T.tri);
   // FP32 copies of the twiddles for the filters of FP32 arrays (mixed mode)
-  std::vector<float> twf32(T.twf, T.twf + T.n_tw), twi32(T.twi, T.twi + T.n_tw);
-  std::vector<float> trf32(T.trf, T.trf + T.n_tr), tri32(T.tri, T.tri + T.n_tr);
-  add(&d.tw_f32, sizeof(float) * T.n_tw, twf32.data());
-  add(&d.tw_i32, sizeof(float) * T.n_tw, twi32.data());
-  add(&d.tr_f32, sizeof(float) * T.n_tr, trf32.data());
-  add(&d.tr_i32, sizeof(float) * T.n_tr, tri32.data());
+  hold.twf32.assign(T.twf, T.twf + T.n_tw);
+  hold.twi32.assign(T.twi, T.twi + T.n_tw);
+  hold.trf32.assign(T.trf, T.trf + T.n_tr);
+  hold.tri32.assign(T.tri, T.tri + T.n_tr);
+  add(&d.tw_f32, sizeof(float) * T.n_tw, hold.twf32.data());
+  add(&d.tw_i32, sizeof(float) * T.n_tw, hold.twi32.data());
+  add(&d.tr_f32, sizeof(float) * T.n_tr, hold.trf32.data());
+  add(&d.tr_i32, sizeof(float) * T.n_tr, hold.tri32.data());
   add(&d.rows_h, sizeof(int) * (c->nj + 2), T.rows_h);
   add(&d.rows_v, sizeof(int) * (c->nj + 2), T.rows_v);
-  std::vector<int> flag_h(c->nj + 2, 0), flag_v(c->nj + 2, 0);
-  for (int r = 0; r < T.nrow_h; r++) flag_h[T.rows_h[r]] = 1;
-  for (int r = 0; r < T.nrow_v; r++) flag_v[T.rows_v[r]] = 1;
-  add(&d.rflag_h, sizeof(int) * (c->nj + 2), flag_h.data());
-  add(&d.rflag_v, sizeof(int) * (c->nj + 2), flag_v.data());
-  add(&d.U, sizeof(double) * N3, c->u);
-  add(&d.V, sizeof(double) * N3, c->v);
-  add(&d.THETA, sizeof(double) * N3, c->theta);
-  add(&d.P3, sizeof(double) * N3, c->p3);
+  hold.flag_h.assign(c->nj + 2, 0);
+  hold.flag_v.assign(c->nj + 2, 0);
+  for (int r = 0; r < T.nrow_h; r++) hold.flag_h[T.rows_h[r]] = 1;
+  for (int r = 0; r < T.nrow_v; r++) hold.flag_v[T.rows_v[r]] = 1;
+  add(&d.rflag_h, sizeof(int) * (c->nj + 2), hold.flag_h.data());
+  add(&d.rflag_v, sizeof(int) * (c->nj + 2), hold.flag_v.data());
+  add(&d.U, sizeof(double) * N3, c->u, EPIC_F_U);
+  add(&d.V, sizeof(double) * N3, c->v, EPIC_F_V);
+  add(&d.THETA, sizeof(double) * N3, c->theta, EPIC_F_THETA);
+  add(&d.P3, sizeof(double) * N3, c->p3, EPIC_F_P3);
   add(&d.USPIN, sizeof(double) * N3, c->u_spinup);
-  add(&d.GZS, sizeof(double) * N2, c->gz_surface);
-  double **fields[] = {&d.HDRY, &d.P2, &d.THETA2, &d.H3, &d.T2, &d.T3, &d.RHO2, &d.RHO3, &d.EXNER3, &d.GZ3,
-                       &d.MONT3, &d.HEAT3, &d.PV3, &d.DIV, &d.W3, &d.DZDT3, &d.RI2, &d.gz, &d.unity,
-                       &d.old};
-  for (auto p : fields) add(p, sizeof(double) * N3, nul);
+  add(&d.GZS, sizeof(double) * N2, c->gz_surface, EPIC_F_GZ_SURFACE);
+  struct { double **p; int f; } fields[] = {
+      {&d.HDRY, EPIC_F_HDRY},     {&d.P2, EPIC_F_P2},       {&d.THETA2, EPIC_F_THETA2}, {&d.H3, EPIC_F_H3},
+      {&d.T2, EPIC_F_T2},         {&d.T3, EPIC_F_T3},       {&d.RHO2, EPIC_F_RHO2},     {&d.RHO3, EPIC_F_RHO3},
+      {&d.EXNER3, EPIC_F_EXNER3}, {&d.GZ3, EPIC_F_GZ3},     {&d.MONT3, EPIC_F_MONT3},   {&d.HEAT3, EPIC_F_HEAT3},
+      {&d.PV3, EPIC_F_PV3},       {&d.DIV, EPIC_F_DIV_UV3}, {&d.W3, EPIC_F_W3},         {&d.DZDT3, EPIC_F_DZDT3},
+      {&d.RI2, EPIC_F_RI2},       {&d.gz, -1},              {&d.unity, -1},             {&d.old, -1}};
+  for (auto &p : fields) add(p.p, sizeof(double) * N3, nul, p.f);
   // scratch: three copies each, one per field of the batched advection
   for (auto p : {&d.sA, &d.sB, &d.sC, &d.sD}) add(p, sizeof(double) * 3 * N3, nul);
   if (mixed) {
@@ -451,13 +471,65 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
     add(&d.DUDTf, sizeof(float) * 3 * N3, nul);
     add(&d.DVDTf, sizeof(float) * 3 * N3, nul);
   } else {
-    add(&d.DUDT, sizeof(double) * 3 * N3, nul);
-    add(&d.DVDT, sizeof(double) * 3 * N3, nul);
+    add(&d.DUDT, sizeof(double) * 3 * N3, nul, EPIC_F_DUDT0);
+    add(&d.DVDT, sizeof(double) * 3 * N3, nul, EPIC_F_DVDT0);
   }
   // PCHIP tables of the batched 3-D filter, one set per field
   add(&d.spl, sizeof(double) * 4 * (size_t)(c->nj + 2) * 3 * (c->nk + 2), nul);
   add(&d.rowbuf, sizeof(double) * 8 * (c->nj + 3), nul);
   add(&d.dyn, sizeof(MemberDyn), nul);
+
+  // scalars
+  d.ni = c->ni; d.nj = c->nj; d.nk = c->nk; d.Ia = c->Iadim; d.N2 = c->Nelem2d; d.N3 = c->Nelem3d;
+  d.ks = c->k_sigma; d.k_sponge = c->k_sponge;
+  d.is_hs = (strcmp(c->planet_name, "held_suarez") == 0);
+  d.is_gas = (strcmp(c->planet_type, "gas-giant") == 0);
+  d.is_hj = (strcmp(c->planet_name, "hot_jupiter") == 0);
+  d.pbot = c->pbot;
+  d.teq_contrast = c->teq_contrast;
+  d.teq_p_contrast = c->teq_p_contrast;
+  d.tau_rad = c->tau_rad;
+  d.ref_theta_b = c->ref_theta_b;
+  d.ref_a_over_kappa = c->ref_a_over_kappa;
+  // (ref_pb/press0)^kappa, with the host libm like the oracle's
+  d.ref_pbk = d.is_hj ? pow(c->ref_pb / c->press0, c->kappa) : 0.;
+  d.hj_lnref = d.is_hj ? log(c->pbot / c->teq_p_contrast) : 0.;
+  d.heat_area = 0.;
+  for (int J = 0; J <= c->nj; J++) {
+    const double da = 1. / c->mn[2 * J + 1];
+    for (int I = 1; I <= c->ni; I++) d.heat_area += da;
+  }
+  d.newt_cool_on = c->newt_cool_on; d.newt_cool_adjust = c->newt_cool_adjust;
+  d.drag_v = c->drag_v; d.drag_zonal_avg = c->drag_zonal_avg; d.jtp = c->jtp;
+  d.nrow_h = T.nrow_h; d.nrow_v = T.nrow_v;
+  int lg = 0;
+  while ((1 << lg) < c->ni) lg++;
+  d.nlogn = lg;
+  d.mixed = mixed ? 1 : 0;
+  d.dt = (double)c->dt; d.kappa = c->kappa; d.cp = c->cp; d.cpr = c->cpr; d.rgas = c->rgas;
+  d.press0 = c->press0; d.zeta0 = c->zeta0; d.zeta1 = c->zeta1; d.alpha = c->hybrid_alpha;
+  d.sigma_sigma = c->sigma_sigma; d.topo_scale = c->topo_scale; d.prandtl = c->prandtl; d.tau_drag = c->tau_drag;
+  d.coef2 = T.coef2; d.coef4 = T.coef4; d.nu4 = c->nu[4]; d.m0z = T.m0z; d.nudiv = T.nudiv; d.eps13 = T.eps13;
+  d.mu3 = T.mu3; d.mu2 = T.mu2; d.g_coeff = T.gcoef;
+}
+
+int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src, bool prime) {
+  if (!E || !c) return -1;
+  if (member_check(c, E->precision, E->err, sizeof(E->err))) return -1;
+  const bool mixed = (E->precision == EPIC_MIXED);
+  int slot = -1;
+  for (int s = 0; s < E->max_members; s++)
+    if (!E->m[s].active) { slot = s; break; }
+  if (slot < 0) return -3;
+  epic_tables T;
+  epic_tables_build(c, &T);
+  struct TablesGuard { epic_tables *t; ~TablesGuard() { epic_tables_free(t); } } tg{&T};
+  const size_t N3 = (size_t)c->Nelem3d;
+  Member &M = E->m[slot];
+  MemberDesc &d = M.h;
+  std::vector<MemberPart> parts;
+  MemberHold hold;
+  member_plan(c, mixed, T, d, parts, hold);
   size_t total = 0;
   for (auto &p : parts) total += align256(p.bytes);
   if (E->budget && E->in_use + total > E->budget) {
@@ -518,33 +590,8 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
   memset(&dy, 0, sizeof(dy));
   dy.it0 = 0; dy.itm1 = 1; dy.itm2 = 2; dy.itime = 0;
   HIPCHK(hipMemcpyAsync(d.dyn, &dy, sizeof(dy), hipMemcpyHostToDevice, E->stream));
-  // scalars
-  d.ni = c->ni; d.nj = c->nj; d.nk = c->nk; d.Ia = c->Iadim; d.N2 = c->Nelem2d; d.N3 = c->Nelem3d;
-  d.ks = c->k_sigma; d.k_sponge = c->k_sponge;
-  d.is_hs = (strcmp(c->planet_name, "held_suarez") == 0);
-  d.is_gas = (strcmp(c->planet_type, "gas-giant") == 0);
-  d.is_hj = (strcmp(c->planet_name, "hot_jupiter") == 0);
-  d.pbot = c->pbot;
-  d.teq_contrast = c->teq_contrast;
-  d.teq_p_contrast = c->teq_p_contrast;
-  d.tau_rad = c->tau_rad;
-  d.ref_theta_b = c->ref_theta_b;
-  d.ref_a_over_kappa = c->ref_a_over_kappa;
-  // (ref_pb/press0)^kappa, with the host libm like the oracle's
-  d.ref_pbk = d.is_hj ? pow(c->ref_pb / c->press0, c->kappa) : 0.;
-  d.hj_lnref = d.is_hj ? log(c->pbot / c->teq_p_contrast) : 0.;
-  d.newt_cool_on = c->newt_cool_on; d.newt_cool_adjust = c->newt_cool_adjust;
-  d.drag_v = c->drag_v; d.drag_zonal_avg = c->drag_zonal_avg; d.jtp = c->jtp;
-  d.nrow_h = T.nrow_h; d.nrow_v = T.nrow_v;
-  int lg = 0;
-  while ((1 << lg) < c->ni) lg++;
-  d.nlogn = lg;
-  d.mixed = mixed ? 1 : 0;
-  d.dt = (double)c->dt; d.kappa = c->kappa; d.cp = c->cp; d.cpr = c->cpr; d.rgas = c->rgas;
-  d.press0 = c->press0; d.zeta0 = c->zeta0; d.zeta1 = c->zeta1; d.alpha = c->hybrid_alpha;
-  d.sigma_sigma = c->sigma_sigma; d.topo_scale = c->topo_scale; d.prandtl = c->prandtl; d.tau_drag = c->tau_drag;
-  d.coef2 = T.coef2; d.coef4 = T.coef4; d.nu4 = c->nu[4]; d.m0z = T.m0z; d.nudiv = T.nudiv; d.eps13 = T.eps13;
-  d.mu3 = T.mu3; d.mu2 = T.mu2; d.g_coeff = T.gcoef;
+  // the host arrays above are read by the copies: wait for them here
+  HIPCHK(hipStreamSynchronize(E->stream));
   M.blob = blob;
   M.bytes = total;
   M.dyn = d.dyn;

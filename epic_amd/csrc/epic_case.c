@@ -363,10 +363,34 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   c->pbot = pbot;
   c->ptop = ptop;
   c->topo_scale = 1.;
-  c->prandtl = 0.;
-  c->tau_drag = 1.e20;
+  c->prandtl = s->prandtl;
+  c->tau_drag = (s->tau_drag > 0.) ? s->tau_drag : 1.e20;
+  c->drag_v = s->drag_v ? 1 : 0;
+  c->drag_zonal_avg = s->drag_zonal_avg ? 1 : 0;
+  c->newt_cool_adjust = s->newt_cool_adjust ? 1 : 0;
   c->nudiv_nondim = s->nudiv_nondim;
-  c->k_sponge = 0;
+  if (s->newt_cool > 0) {
+    /* Newtonian cooling of a planet that has no forcing here would need its
+     * t_vs_p sounding and t_cool_table */
+    if (!c->newt_cool_on) return -5;
+  } else if (s->newt_cool < 0) {
+    c->newt_cool_on = 0;
+  }
+  /* the top sponge: the `initial` program's choice
+   * (src/shared/epic_initial.c:1249-1266) or the spec's */
+  if (s->k_sponge == EPIC_K_SPONGE_DEFAULT) {
+    if (s->planet == EPIC_PLANET_HELD_SUAREZ || nk < 5) {
+      c->k_sponge = 0;
+    } else {
+      const double x = (double)nk * .2;
+      int k = x > 0. ? (int)(x + .5) : (int)(x - .5);
+      c->k_sponge = k < 1 ? 1 : (k > 4 ? 4 : k);
+    }
+  } else if (s->k_sponge >= 0 && s->k_sponge < nk) {
+    c->k_sponge = s->k_sponge;
+  } else {
+    return -1;
+  }
 
   /* ---- arrays ---- */
   int nJ = 2 * (nj + 1) + 1, nK = 2 * (nk + 1) + 1;
@@ -386,6 +410,15 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   c->theta_ref = zalloc(nK);
   c->h_min = zalloc(nk + 2);
   c->t_sponge_inv = zalloc(c->k_sponge + 1);
+  /* set_sponge (src/shared/epic_funcs_diag.c:1696-1731): relax u to its
+   * initial value and v to zero, strongest (1/(30 dt)) in the top layer */
+  if (c->k_sponge > 0) {
+    const double t0_inv = 1. / (30. * (double)c->dt);
+    for (int K = 0; K <= c->k_sponge; K++) {
+      const double tmp = (double)(c->k_sponge - K) / (c->k_sponge);
+      c->t_sponge_inv[K] = t0_inv * .5 * (1. - cos(M_PI * tmp));
+    }
+  }
   size_t N3 = (size_t)c->Nelem3d;
   c->u = zalloc(N3);
   c->v = zalloc(N3);
@@ -479,9 +512,15 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   {
     double dt = (double)c->dt, dy2 = c->dy0 * c->dy0;
     double max_nu4 = (dy2 / 240.) * (dy2 / dt);
+    double max_nu6 = (dy2 / 800.) * (dy2 / dt) * dy2;
+    double max_nu8 = (dy2 / 2400.) * (dy2 / dt) * dy2 * dy2;
     for (int q = 0; q <= EPIC_MAX_NU_ORDER; q++) c->nu[q] = c->nu_nondim[q] = 0.;
     c->nu_nondim[4] = s->nu4_nondim;
     c->nu[4] = s->nu4_nondim * max_nu4;
+    c->nu_nondim[6] = s->nu6_nondim;
+    c->nu[6] = s->nu6_nondim * max_nu6;
+    c->nu_nondim[8] = s->nu8_nondim;
+    c->nu[8] = s->nu8_nondim * max_nu8;
     c->nu[2] = 0.; /* the hyperviscosity initialisers force nu[2] = 0 */
   }
 

@@ -682,27 +682,42 @@ __global__ void __launch_bounds__(256) k_heating(const MemberDesc *D, int mb) {
   if (I == d.ni) d.HEAT3[ix3(d, K, J, 0)] = h;
   if (I == 1) d.HEAT3[ix3(d, K, J, d.ni + 1)] = h;
 }
-// layer-mean removal (K <= k_sponge or newt_cool_adjust), sequential sums
-__global__ void __launch_bounds__(256) k_heating_adjust(const MemberDesc *D, int mb) {
+// Layer-mean removal of the relaxation (K <= k_sponge or newt_cool_adjust,
+// src/shared/epic_sensible_heating.c:139-167), from the relaxation HEAT3
+// holds: one block per layer stages rows of HEAT3 times the cell area
+// 1/mn in LDS, and one lane adds them in the reference's order (J, then I);
+// the total area, which depends on the metric only, is formed on the host
+// by the same loop.  The layer mean goes to sD[K].
+__global__ void __launch_bounds__(256) k_heat_layer_mean(const MemberDesc *D, int mb) {
   MEMBER;
-  const int K = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (K > d.nk) return;
-  if (!(d.newt_cool_on && d.is_hs)) return;
+  extern __shared__ double lds[];
+  const int K = 1 + blockIdx.x;
+  if (!(d.newt_cool_on && (d.is_hs || d.is_hj))) return;
   if (!(K <= d.k_sponge || d.newt_cool_adjust)) return;
-  double area = 0., ha = 0.;
-  for (int J = 0; J <= d.nj; J++) {
-    double da = 1. / d.mn[2 * J + 1];
-    for (int I = 1; I <= d.ni; I++) {
-      area += da;
-      ha += hs_newton(d, K, J, I) * da;
+  const int ni = d.ni, nrow = d.nj + 1;
+  const int rpc = (2048 / ni) > 0 ? 2048 / ni : 1;  // rows per LDS chunk
+  double ha = 0.;
+  for (int j0 = 0; j0 < nrow; j0 += rpc) {
+    const int nr = (nrow - j0) < rpc ? (nrow - j0) : rpc;
+    for (int t = threadIdx.x; t < nr * ni; t += blockDim.x) {
+      const int J = j0 + t / ni, I = 1 + t % ni;
+      const double da = 1. / d.mn[2 * J + 1];
+      lds[t] = d.HEAT3[ix3(d, K, J, I)] * da;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) ha = seq_acc(lds, nr * ni, ha);
+    __syncthreads();
   }
-  double havg = ha / area;
-  for (int J = 0; J <= d.nj; J++)
-    for (int I = 0; I <= d.ni + 1; I++) {
-      int Ii = (I == 0) ? d.ni : (I == d.ni + 1 ? 1 : I);
-      d.HEAT3[ix3(d, K, J, I)] = 0. + (hs_newton(d, K, J, Ii) - havg);
-    }
+  if (threadIdx.x == 0) d.sD[K] = ha / d.heat_area;
+}
+__global__ void __launch_bounds__(256) k_heat_adjust_apply(const MemberDesc *D, int mb) {
+  MEMBER;
+  FLAT2(0, d.nj + 1, 0, d.ni + 2);
+  const int K = 1 + blockIdx.y;
+  if (!(d.newt_cool_on && (d.is_hs || d.is_hj))) return;
+  if (!(K <= d.k_sponge || d.newt_cool_adjust)) return;
+  const size_t q = ix3(d, K, J, I);
+  d.HEAT3[q] = 0. + (d.HEAT3[q] - d.sD[K]);
 }
 // in_thermo: the relaxation was formed by this step's k_thermo(full = 1),
 // and every other cell of HEAT3 is zero (a member's blob is cleared when it
@@ -714,7 +729,10 @@ void L_heating(hipStream_t s, const Bucket &b, int adjust, bool in_thermo) {
     L_fill(s, b, F_HEAT3, 0.);
     LAUNCH(k_heating, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
   }
-  if (adjust) LAUNCH(k_heating_adjust, dim3(1, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
+  if (adjust) {
+    LAUNCH(k_heat_layer_mean, dim3(b.nk, 1, b.nm), dim3(256), sizeof(double) * 2048, s, b.D, b.mb);
+    LAUNCH(k_heat_adjust_apply, grid2(b.nj + 1, b.ni + 2, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  }
 }
 
 // ------------------------------------------------------------ calc_w

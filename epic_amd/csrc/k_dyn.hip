@@ -161,7 +161,10 @@ void L_uv_core(hipStream_t s, const Bucket &b) {
 }
 
 // ------------------------------------------------------------- uv_drag
-__device__ __forceinline__ double t_rad_hs(const MemberDesc &d, int K, int J, int I) {
+// t_rad (src/shared/epic_sensible_heating.c:217-314): the Held-Suarez
+// cooling time; the hot Jupiter's relaxation time is its constant tau_rad
+__device__ __forceinline__ double t_rad(const MemberDesc &d, int K, int J, int I) {
+  if (d.is_hj) return d.tau_rad;
   const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
   double p = d.P3[ix3(d, K, J, I)], kt = ka, pbot = d.P3[ix3(d, d.nk, J, I)];
   double amp = (p / pbot - 0.7) / (1. - 0.7);
@@ -178,11 +181,23 @@ __global__ void __launch_bounds__(256) k_sponge(const MemberDesc *D, int mb) {
   double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
   size_t q = ix3(d, K, J, I);
   double ts = d.t_sponge[K];
-  du[q] -= (ts) * (d.U[q] - d.USPIN[q]);
-  if (J >= 1) dv[q] -= (ts) * (d.V[q] - 0.);
+  const double x = du[q] - (ts) * (d.U[q] - d.USPIN[q]);
+  du[q] = x;
+  // zonal pads: nothing after uv_drag writes the interior of plane K = 0,
+  // whose pads the reference fills in the bc_lateral that ends the 3-D
+  // filter of the tendency (the filter itself starts at K = 1); the pads
+  // of the other sponge planes are written again by the filter
+  if (I == d.ni) du[ix3(d, K, J, 0)] = x;
+  if (I == 1) du[ix3(d, K, J, d.ni + 1)] = x;
+  if (J >= 1) {
+    const double y = dv[q] - (ts) * (d.V[q] - 0.);
+    dv[q] = y;
+    if (I == d.ni) dv[ix3(d, K, J, 0)] = y;
+    if (I == 1) dv[ix3(d, K, J, d.ni + 1)] = y;
+  }
 }
 
-// Rayleigh drag (tau_drag, or prandtl with Held-Suarez t_rad): one lane per
+// Rayleigh drag (tau_drag, or prandtl / t_rad): one lane per
 // (K,J) row, sequential zonal means as in the reference
 __global__ void __launch_bounds__(256) k_rayleigh(const MemberDesc *D, int mb) {
   MEMBER;
@@ -200,7 +215,7 @@ __global__ void __launch_bounds__(256) k_rayleigh(const MemberDesc *D, int mb) {
   for (int I = 1; I <= ni; I++) {
     size_t q = ix3(d, K, J, I);
     double uray = za * uavg + zb * d.U[q];
-    double nu0 = d.newt_cool_on ? d.prandtl / (.5 * (t_rad_hs(d, K, J, I) + t_rad_hs(d, K, J, I - 1))) : 1. / d.tau_drag;
+    double nu0 = d.newt_cool_on ? d.prandtl / (.5 * (t_rad(d, K, J, I) + t_rad(d, K, J, I - 1))) : 1. / d.tau_drag;
     du[q] -= nu0 * (uray - d.USPIN[q]);
   }
   if (d.drag_v && J >= 1) {
@@ -211,7 +226,7 @@ __global__ void __launch_bounds__(256) k_rayleigh(const MemberDesc *D, int mb) {
       size_t q = ix3(d, K, J, I);
       double vray = za * vavg + zb * d.V[q];
       double nu0 =
-          d.newt_cool_on ? d.prandtl / (.5 * (t_rad_hs(d, K, J, I) + t_rad_hs(d, K, J - 1, I))) : 1. / d.tau_drag;
+          d.newt_cool_on ? d.prandtl / (.5 * (t_rad(d, K, J, I) + t_rad(d, K, J - 1, I))) : 1. / d.tau_drag;
       dv[q] -= nu0 * vray;
     }
   }

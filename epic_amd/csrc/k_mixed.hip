@@ -156,11 +156,22 @@ __global__ void __launch_bounds__(256) km_sponge(const MemberDesc *D, int mb) {
   double *du = curU(d), *dv = curV(d);
   const size_t q = ix3(d, K, J, I);
   const float ts = (float)d.t_sponge[K];
-  du[q] -= (double)(ts * (float)(d.U[q] - d.USPIN[q]));
-  if (J >= 1) dv[q] -= (double)(ts * (float)d.V[q]);
+  const double x = du[q] - (double)(ts * (float)(d.U[q] - d.USPIN[q]));
+  du[q] = x;
+  // zonal pads of plane K = 0, which no later kernel writes (the 3-D
+  // filter's bc_lateral does in the reference)
+  if (I == d.ni) du[ix3(d, K, J, 0)] = x;
+  if (I == 1) du[ix3(d, K, J, d.ni + 1)] = x;
+  if (J >= 1) {
+    const double y = dv[q] - (double)(ts * (float)d.V[q]);
+    dv[q] = y;
+    if (I == d.ni) dv[ix3(d, K, J, 0)] = y;
+    if (I == 1) dv[ix3(d, K, J, d.ni + 1)] = y;
+  }
 }
 
-__device__ __forceinline__ float t_rad_hs_f(const MemberDesc &d, int K, int J, int I) {
+__device__ __forceinline__ float t_rad_f(const MemberDesc &d, int K, int J, int I) {
+  if (d.is_hj) return (float)d.tau_rad;
   const float ka = (float)(1. / (40. * 60. * 60. * 24.)), ks = (float)(1. / (4. * 60. * 60. * 24.));
   float p = (float)d.P3[ix3(d, K, J, I)], kt = ka, pbot = (float)d.P3[ix3(d, d.nk, J, I)];
   float amp = (p / pbot - 0.7f) / (1.f - 0.7f);
@@ -193,7 +204,7 @@ __global__ void __launch_bounds__(256) km_rayleigh(const MemberDesc *D, int mb) 
       float ray = za * avg + zb * (float)X[q];
       float nu0;
       if (d.newt_cool_on) {
-        float t0 = t_rad_hs_f(d, K, J, I), t1 = w ? t_rad_hs_f(d, K, J - 1, I) : t_rad_hs_f(d, K, J, I - 1);
+        float t0 = t_rad_f(d, K, J, I), t1 = w ? t_rad_f(d, K, J - 1, I) : t_rad_f(d, K, J, I - 1);
         nu0 = (float)d.prandtl / (.5f * (t0 + t1));
       } else {
         nu0 = (float)(1. / d.tau_drag);

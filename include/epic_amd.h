@@ -196,6 +196,11 @@ enum epic_planet_kind {
                                   relaxation to a day-night T_eq            */
 };
 
+/* k_sponge value that asks for the reference `initial` program's choice:
+ * 0 for Held-Suarez, NINT(0.2 nk) clamped to 1..4 for any other planet with
+ * nk >= 5, 0 for nk <= 4 (src/shared/epic_initial.c:1249-1266). */
+#define EPIC_K_SPONGE_DEFAULT (-1)
+
 typedef struct {
   int    planet;          /* enum epic_planet_kind                         */
   int    ni, nj, nk;      /* ni a power of two, nj >= 4, nk >= 4           */
@@ -205,6 +210,22 @@ typedef struct {
   double nudiv_nondim;    /* divergence damping, fraction of maximum       */
   double noise_theta;     /* K, standard deviation of theta noise          */
   double noise_uv;        /* m/s, standard deviation of u,v noise          */
+  /* The reference's drag and forcing switches (gridspec members of the same
+   * names, src/shared/epic_timestep.c:1303-1509,
+   * src/shared/epic_sensible_heating.c:51-180).  A zeroed tail gives no
+   * sponge, no Rayleigh drag and the planet's own Newtonian-cooling
+   * default (on for Held-Suarez and the hot Jupiter). */
+  int    k_sponge;        /* top sponge layers (t_sponge_inv as set_sponge
+                             forms it), or EPIC_K_SPONGE_DEFAULT            */
+  int    newt_cool;       /* 0: planet default, 1: on, -1: off              */
+  int    newt_cool_adjust;/* remove the layer mean of the heating           */
+  int    drag_v;          /* Rayleigh drag on v as well as u                */
+  int    drag_zonal_avg;  /* Rayleigh drag acts on the zonal mean           */
+  double tau_drag;        /* s, Rayleigh drag time; 0 or >= 1e20: none      */
+  double prandtl;         /* Rayleigh drag rate prandtl/t_rad (with
+                             Newtonian cooling on); 0: none                 */
+  double nu6_nondim;      /* del^6 hyperviscosity, fraction of maximum     */
+  double nu8_nondim;      /* del^8 hyperviscosity, fraction of maximum     */
 } epic_case_spec;
 
 typedef struct {

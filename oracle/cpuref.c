@@ -749,19 +749,27 @@ static void store_diag(cpuref *R) {
 }
 
 /* calc_heating -> newtonian_cooling with the Held-Suarez T_eq and k_T,
- * src/shared/epic_funcs_diag.c:4793-4869, src/shared/epic_sensible_heating.c:51-180,217-314,325-463 */
+ * src/shared/epic_funcs_diag.c:4793-4869, src/shared/epic_sensible_heating.c:51-180,217-314,325-463.
+ * Layer by layer: the relaxation into a buffer, its area-weighted layer mean
+ * removed for K <= k_sponge or with newt_cool_adjust (:139-167), then added
+ * to the cleared HEAT3. */
 static void calc_heating(cpuref *R) {
   epic_case *c = &R->c;
   memset(R->HEAT3, 0, sizeof(double) * R->N3);
-  if (c->newt_cool_on && R->is_hs) {
-    const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
-    const double dt_y = 60., dth_z = 10., p0 = 1000. * 100.;
-    double *buf = R->b2[6];
-    for (int K = 1; K <= R->nk; K++) {
-      for (int J = 0; J <= R->nj; J++) {
-        double lat = c->lat[2 * J + 1];
-        for (int I = 1; I <= R->ni; I++) {
-          double p = A3(R->P3, K, J, I), t = A3(R->T3, K, J, I);
+  if (!c->newt_cool_on || !(R->is_hs || R->is_hj)) {
+    bc3(R, R->HEAT3);
+    return;
+  }
+  const double pbk = R->is_hj ? pow(c->ref_pb / c->press0, c->kappa) : 0.;
+  double *buf = R->b2[6];
+  for (int K = 1; K <= R->nk; K++) {
+    for (int J = 0; J <= R->nj; J++) {
+      double lat = c->lat[2 * J + 1];
+      for (int I = 1; I <= R->ni; I++) {
+        double p = A3(R->P3, K, J, I), t = A3(R->T3, K, J, I);
+        if (R->is_hs) {
+          const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
+          const double dt_y = 60., dth_z = 10., p0 = 1000. * 100.;
           double kt = ka, pbot = A3(R->P3, R->nk, J, I);
           double amp = (p / pbot - 0.7) / (1. - 0.7);
           if (amp > 0.) {
@@ -776,45 +784,37 @@ static void calc_heating(cpuref *R) {
           double teq = (315. - dt_y * s2 - dth_z * log(pp0) * c2) * pow(pp0, c->kappa);
           teq = mx(200., teq);
           A2(buf, J, I) = -cpt * (t - teq);
-        }
-      }
-      if (K <= R->k_sponge || c->newt_cool_adjust) {
-        double area = 0., ha = 0.;
-        for (int J = 0; J <= R->nj; J++) {
-          double da = 1. / c->mn[2 * J + 1];
-          for (int I = 1; I <= R->ni; I++) {
-            area += da;
-            ha += A2(buf, J, I) * da;
-          }
-        }
-        double havg = ha / area;
-        for (int J = 0; J <= R->nj; J++)
-          for (int I = 1; I <= R->ni; I++) A2(buf, J, I) -= havg;
-      }
-      for (int J = 0; J <= R->nj; J++)
-        for (int I = 1; I <= R->ni; I++) A3(R->HEAT3, K, J, I) += A2(buf, J, I);
-    }
-  }
-  if (c->newt_cool_on && R->is_hj) {
-    /* Day-night Newtonian relaxation of the hot-Jupiter case (no reference
-     * counterpart: its temp_eq has no such branch,
-     * src/shared/epic_sensible_heating.c:394-460), on the interfaces
-     * K = 1..nk like the Held-Suarez relaxation: HEAT3 = cp (T_eq - T)/tau. */
-    const double pbk = pow(c->ref_pb / c->press0, c->kappa);
-    for (int K = 1; K <= R->nk; K++)
-      for (int J = 0; J <= R->nj; J++) {
-        const double cl = cos(c->lat[2 * J + 1] * DEGR);
-        for (int I = 1; I <= R->ni; I++) {
-          const double p = A3(R->P3, K, J, I), t = A3(R->T3, K, J, I);
+        } else {
+          /* Day-night Newtonian relaxation of the hot-Jupiter case (no
+           * reference counterpart: its temp_eq has no such branch,
+           * src/shared/epic_sensible_heating.c:394-460), on the interfaces
+           * K = 1..nk like the Held-Suarez relaxation: cp (T_eq - T)/tau. */
+          const double cl = cos(c->lat[2 * J + 1] * DEGR);
           const double pk = pow(p / c->press0, c->kappa);
           const double tref = (1. / (1. / c->ref_theta_b + c->ref_a_over_kappa * (pk - pbk))) * pk;
           double dT = c->teq_contrast;
           if (p > c->teq_p_contrast) dT *= mx(0., log(c->pbot / p) / log(c->pbot / c->teq_p_contrast));
           const double day = mx(0., cl * cos(c->lon[2 * I + 1] * DEGR));
           const double teq = tref + dT * (day - .25);
-          A3(R->HEAT3, K, J, I) = c->cp * (teq - t) / c->tau_rad;
+          A2(buf, J, I) = c->cp * (teq - t) / c->tau_rad;
         }
       }
+    }
+    if (K <= R->k_sponge || c->newt_cool_adjust) {
+      double area = 0., ha = 0.;
+      for (int J = 0; J <= R->nj; J++) {
+        double da = 1. / c->mn[2 * J + 1];
+        for (int I = 1; I <= R->ni; I++) {
+          area += da;
+          ha += A2(buf, J, I) * da;
+        }
+      }
+      double havg = ha / area;
+      for (int J = 0; J <= R->nj; J++)
+        for (int I = 1; I <= R->ni; I++) A2(buf, J, I) -= havg;
+    }
+    for (int J = 0; J <= R->nj; J++)
+      for (int I = 1; I <= R->ni; I++) A3(R->HEAT3, K, J, I) += A2(buf, J, I);
   }
   bc3(R, R->HEAT3);
 }
@@ -967,7 +967,11 @@ static void uv_core(cpuref *R, double *dudt, double *dvdt) {
 }
 
 /* uv_drag, src/shared/epic_timestep.c:1303-1509 */
-static double t_rad_hs(cpuref *R, int K, int J, int I) {
+/* t_rad, src/shared/epic_sensible_heating.c:217-314: the Held-Suarez
+ * cooling time from k_a, k_s; the hot Jupiter's relaxation time is its
+ * constant tau_rad */
+static double t_rad(cpuref *R, int K, int J, int I) {
+  if (R->is_hj) return R->c.tau_rad;
   const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
   double p = A3(R->P3, K, J, I), kt = ka, pbot = A3(R->P3, R->nk, J, I);
   double amp = (p / pbot - 0.7) / (1. - 0.7);
@@ -994,7 +998,7 @@ static void uv_drag(cpuref *R, double *dudt, double *dvdt) {
         for (int I = 1; I <= ni; I++) {
           double uray = (c->drag_zonal_avg) * uavg + (1 - c->drag_zonal_avg) * A3(R->U, K, J, I);
           double nu0;
-          if (c->newt_cool_on) nu0 = c->prandtl / (.5 * (t_rad_hs(R, K, J, I) + t_rad_hs(R, K, J, I - 1)));
+          if (c->newt_cool_on) nu0 = c->prandtl / (.5 * (t_rad(R, K, J, I) + t_rad(R, K, J, I - 1)));
           else nu0 = 1. / c->tau_drag;
           A3(dudt, K, J, I) -= nu0 * (uray - A3(R->USPIN, K, J, I));
         }
@@ -1007,7 +1011,7 @@ static void uv_drag(cpuref *R, double *dudt, double *dvdt) {
           for (int I = 1; I <= ni; I++) {
             double vray = (c->drag_zonal_avg) * vavg + (1 - c->drag_zonal_avg) * A3(R->V, K, J, I);
             double nu0;
-            if (c->newt_cool_on) nu0 = c->prandtl / (.5 * (t_rad_hs(R, K, J, I) + t_rad_hs(R, K, J - 1, I)));
+            if (c->newt_cool_on) nu0 = c->prandtl / (.5 * (t_rad(R, K, J, I) + t_rad(R, K, J - 1, I)));
             else nu0 = 1. / c->tau_drag;
             A3(dvdt, K, J, I) -= nu0 * vray;
           }

@@ -19,7 +19,11 @@
  * t_sponge_inv[0] (src/shared/epic_timestep.c:1363-1374), a zeroed
  * one-element array (zero sponge).
  *
- * Usage: epic_ref PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT [DUMP_STEP ...]
+ * Usage: epic_ref [--OPTION=VALUE ...] PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT [DUMP_STEP ...]
+ * OPTION is one of the epic_case_spec switches k_sponge, newt_cool,
+ * newt_cool_adjust, drag_v, drag_zonal_avg, tau_drag, prandtl, nu6_nondim,
+ * nu8_nondim.  With k_sponge > 0 the reference's own set_sponge() fills
+ * t_sponge_inv, and the case builder's copy must equal it bit for bit.
  * A DUMP_STEP of 0 dumps the state after the SYNC call; s > 0 after s steps.
  * A negative DUMP_STEP -s dumps at s and prints the metric comparison only.
  * Output: int32 header {magic, ni, nj, nk, ndumps, nfields, nelem3d, nelem2d,
@@ -142,14 +146,37 @@ static void write_fields(FILE *fo, int step) {
   fwrite(var.ri2.value, sizeof(double), Nelem3d, fo);
 }
 
+/* --name=value switches of epic_case_spec; returns 0 or -1 */
+static int spec_option(epic_case_spec *sp, const char *arg) {
+  const char *eq = strchr(arg, '=');
+  if (strncmp(arg, "--", 2) != 0 || !eq) return -1;
+  size_t n = (size_t)(eq - arg - 2);
+  const char *v = eq + 1, *k = arg + 2;
+#define OPT_I(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atoi(v); return 0; }
+#define OPT_D(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atof(v); return 0; }
+  OPT_I(k_sponge) OPT_I(newt_cool) OPT_I(newt_cool_adjust) OPT_I(drag_v) OPT_I(drag_zonal_avg)
+  OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim)
+#undef OPT_I
+#undef OPT_D
+  return -1;
+}
+
 int main(int argc, char **argv) {
+  epic_case_spec sp;
+  memset(&sp, 0, sizeof(sp));
+  while (argc > 1 && strncmp(argv[1], "--", 2) == 0) {
+    if (spec_option(&sp, argv[1])) {
+      fprintf(stderr, "unknown option %s\n", argv[1]);
+      return 2;
+    }
+    argv++;
+    argc--;
+  }
   if (argc < 13) {
-    fprintf(stderr, "usage: %s PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT [DUMP ...]\n",
+    fprintf(stderr, "usage: %s [--OPTION=VALUE ...] PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT [DUMP ...]\n",
             argv[0]);
     return 2;
   }
-  epic_case_spec sp;
-  memset(&sp, 0, sizeof(sp));
   sp.planet = atoi(argv[1]);
   sp.ni = atoi(argv[2]);
   sp.nj = atoi(argv[3]);
@@ -248,7 +275,9 @@ int main(int argc, char **argv) {
   var.time_fp_bar = 3.e8 * 1000. * 100.;
 
   make_arrays(planet);
-  grid.t_sponge_inv = fvector(0, grid.k_sponge > 0 ? grid.k_sponge : 0, "harness");
+  /* make_arrays allocates t_sponge_inv only for k_sponge > 0 (and
+   * set_sponge fills it then); uv_drag reads t_sponge_inv[0] in any case */
+  if (grid.k_sponge <= 0) grid.t_sponge_inv = fvector(0, 0, "harness");
 
   set_lonlat();
   set_fmn(planet);
@@ -260,6 +289,7 @@ int main(int argc, char **argv) {
   }
   for (int K = 0; K <= grid.nk + 1; K++) grid.h_min[K] = c.h_min[K];
   set_dsgth();
+  set_sponge();
 
   int nJ = 2 * (grid.nj + 1) + 1, nK = 2 * (grid.nk + 1) + 1;
   int mism = 0;
@@ -274,6 +304,7 @@ int main(int argc, char **argv) {
   mism += cmp_arr("g", grid.g, c.g, nJ);
   mism += cmp_arr("dsgth", grid.dsgth, c.dsgth, nK);
   mism += cmp_arr("dsgth_inv", grid.dsgth_inv, c.dsgth_inv, nK);
+  mism += cmp_arr("t_sponge_inv", grid.t_sponge_inv, c.t_sponge_inv, grid.k_sponge + 1);
   if (memcmp(&grid.dy0, &c.dy0, sizeof(double))) {
     fprintf(stderr, "metric dy0 differs\n");
     mism++;

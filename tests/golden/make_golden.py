@@ -26,12 +26,32 @@ sys.path.insert(0, ROOT)
 from epic_amd import FIELDS  # noqa: E402
 
 CASES = {
-    # name: (planet, ni, nj, nk, dt, seed, nu4, nudiv, noise_theta, noise_uv, dumps)
-    "hs_tiny": (0, 16, 8, 6, 600, 7, 0.3, 0.2, 0.5, 0.5, [0, 1, 10]),
-    "jup_tiny": (1, 16, 8, 6, 60, 5, 0.5, 0.0, 0.0, 0.5, [0, 1, 10]),
-    "venus_tiny": (3, 16, 8, 6, 600, 9, 0.5, 0.1, 0.1, 0.5, [0, 1, 10]),
-    "hs_c1": (0, 64, 32, 10, 600, 1, 0.5, 0.0, 0.1, 0.0, [1, 10, 100]),
+    # name: (planet, ni, nj, nk, dt, seed, nu4, nudiv, noise_theta, noise_uv, dumps[, options])
+    # options: the reference's switches as --name=value of the harness
+    # (k_sponge -1 is the reference `initial` program's choice)
+    "hs_tiny": (0, 16, 8, 6, 600, 7, 0.3, 0.2, 0.5, 0.5, [0, 1, 10], {"k_sponge": 0}),
+    "jup_tiny": (1, 16, 8, 6, 60, 5, 0.5, 0.0, 0.0, 0.5, [0, 1, 10], {"k_sponge": 0}),
+    "venus_tiny": (3, 16, 8, 6, 600, 9, 0.5, 0.1, 0.1, 0.5, [0, 1, 10], {"k_sponge": 0}),
+    "hs_c1": (0, 64, 32, 10, 600, 1, 0.5, 0.0, 0.1, 0.0, [1, 10, 100], {"k_sponge": 0}),
+    # the top sponge as `initial` sets it, Rayleigh drag on u and v about
+    # the zonal mean
+    "jup_drag_tiny": (1, 16, 8, 6, 60, 6, 0.5, 0.0, 0.0, 0.5, [0, 1, 10],
+                      {"k_sponge": -1, "tau_drag": 2.0e4, "drag_v": 1, "drag_zonal_avg": 1}),
+    # Held-Suarez with the layer-mean heating removal and prandtl/t_rad drag
+    "hs_adjust_tiny": (0, 16, 8, 6, 600, 8, 0.3, 0.2, 0.5, 0.5, [0, 1, 10],
+                       {"newt_cool_adjust": 1, "prandtl": 0.3, "drag_v": 1}),
 }
+
+
+def spec_of(entry):
+    """The case spec of a golden.json case entry: its positional values and
+    the reference switches it set (k_sponge 0 where none is given)."""
+    from epic_amd import spec
+    opts = dict(entry.get("options", {}))
+    sp = spec(*entry["spec"], k_sponge=opts.pop("k_sponge", 0))
+    for k, v in opts.items():
+        setattr(sp, k, v)
+    return sp
 
 
 def read_dump(fn):
@@ -59,11 +79,12 @@ def main():
         sys.exit("reference build missing: make -C oracle ref")
     hashes, meta = {}, {}
     for name, c in CASES.items():
-        *args, dumps = c
+        args, dumps, opts = list(c[:10]), c[10], (c[11] if len(c) > 11 else {})
         fn = f"/tmp/golden_{name}.bin"
-        cmd = [EXE] + [str(x) for x in args] + [str(max(dumps)), fn] + [str(d) for d in dumps]
+        cmd = ([EXE] + [f"--{k}={v}" for k, v in opts.items()] + [str(x) for x in args] +
+               [str(max(dumps)), fn] + [str(d) for d in dumps])
         subprocess.run(cmd, check=True, cwd="/tmp")
-        meta[name] = {"spec": args, "dumps": dumps}
+        meta[name] = {"spec": args, "dumps": dumps, "options": opts}
         arrays = {}
         for hdr, f in read_dump(fn):
             step = int(hdr[0])

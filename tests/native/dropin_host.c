@@ -7,7 +7,8 @@
  * timestep(STEP_PROGS_AND_SYNC_DIAGS) NSTEPS times, advancing grid.itime as
  * epic_main does.
  *
- * Usage: dropin_host PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT LAZY [NAME]
+ * Usage: dropin_host [--OPTION=VALUE ...] PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT LAZY [NAME]
+ * (OPTION: an epic_case_spec switch, as for the oracle's reference harness)
  * OUT holds the dump format of the oracle's reference harness: an int32
  * header {magic, ni, nj, nk, 1, nfields, nelem3d, nelem2d, 0} and one dump
  * of the final state.  LAZY=1 selects epic_amd_set_sync_mode(1) and a
@@ -30,14 +31,38 @@ static double *arr(size_t n) {
   return p;
 }
 
+/* --name=value switches of epic_case_spec (the options of the oracle's
+ * reference harness); returns 0 or -1 */
+static int spec_option(epic_case_spec *sp, const char *arg) {
+  const char *eq = strchr(arg, '=');
+  if (strncmp(arg, "--", 2) != 0 || !eq) return -1;
+  size_t n = (size_t)(eq - arg - 2);
+  const char *v = eq + 1, *k = arg + 2;
+#define OPT_I(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atoi(v); return 0; }
+#define OPT_D(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atof(v); return 0; }
+  OPT_I(k_sponge) OPT_I(newt_cool) OPT_I(newt_cool_adjust) OPT_I(drag_v) OPT_I(drag_zonal_avg)
+  OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim)
+#undef OPT_I
+#undef OPT_D
+  return -1;
+}
+
 int main(int argc, char **argv) {
+  epic_case_spec sp;
+  memset(&sp, 0, sizeof(sp));
+  while (argc > 1 && strncmp(argv[1], "--", 2) == 0) {
+    if (spec_option(&sp, argv[1])) {
+      fprintf(stderr, "unknown option %s\n", argv[1]);
+      return 2;
+    }
+    argv++;
+    argc--;
+  }
   if (argc < 14) {
-    fprintf(stderr, "usage: %s PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT LAZY [NAME]\n",
+    fprintf(stderr, "usage: %s [--OPTION=VALUE ...] PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT LAZY [NAME]\n",
             argv[0]);
     return 2;
   }
-  epic_case_spec sp;
-  memset(&sp, 0, sizeof(sp));
   sp.planet = atoi(argv[1]);
   sp.ni = atoi(argv[2]);
   sp.nj = atoi(argv[3]);

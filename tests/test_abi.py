@@ -74,12 +74,13 @@ def test_library_leaves_the_callers_globals_weak():
             assert p[4] == "WEAK" and p[6] == "UND", line
 
 
-@pytest.mark.parametrize("name", ["jupiter", "venus"])
+@pytest.mark.parametrize("name", ["jupiter", "venus", "hot_jupiter"])
 def test_dropin_refuses_newtonian_cooling_it_does_not_have(tmp_path, name):
-    """With grid.newt_cool_on, a planet other than held_suarez or hot_jupiter
-    relaxes toward its t_vs_p sounding with t_cool_table's times in the
-    reference; timestep() stops with an EPIC-style error before any GPU work
-    instead of stepping without that forcing."""
+    """With grid.newt_cool_on, a planet other than held_suarez relaxes toward
+    its t_vs_p sounding with t_cool_table's times in the reference (a planet
+    named hot_jupiter too: the engine's day-night relaxation is its own case,
+    not the reference's); timestep() stops with an EPIC-style error before
+    any GPU work instead of stepping with another forcing."""
     exe = os.path.join(ROOT, "tests", "native", "_bin", "dropin_host")
     if not os.path.exists(exe):
         pytest.skip("dropin_host not built")
@@ -89,3 +90,37 @@ def test_dropin_refuses_newtonian_cooling_it_does_not_have(tmp_path, name):
     assert r.returncode == 1, r.stdout + r.stderr
     assert "Newtonian cooling from t_cool_table" in r.stderr
     assert not os.path.exists(tmp_path / "out")
+
+
+@pytest.mark.parametrize("cname,pyname", [("epic_case_spec", "CaseSpec"), ("epic_case", "Case")])
+def test_python_binding_layouts_match_the_header(cname, pyname, tmp_path):
+    """The ctypes restatements of epic_case_spec and epic_case have the C
+    header's size and member offsets (gcc, this machine's ABI)."""
+    st = getattr(ea, pyname)
+    src = tmp_path / "lay.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HDR}"', "int main(void) {",
+             f'  printf("size %zu\\n", sizeof({cname}));']
+    for name, _ in st._fields_:
+        lines.append(f'  printf("{name} %zu\\n", offsetof({cname}, {name}));')
+    lines.append("  return 0;\n}")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "lay"
+    r = subprocess.run(["gcc", "-std=gnu11", "-o", str(exe), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(st)
+    for name, _ in st._fields_:
+        assert int(out[name]) == getattr(st, name).offset, name
+
+
+def test_case_builder_through_the_binding():
+    """epic_case_build through ctypes: the padded extents and the sponge of
+    the reference `initial` program's choice (2 layers at nk = 10)."""
+    c = ea.build_case(ea.spec(ea.JUPITER_LIKE, 32, 16, 10, 60, 3))
+    try:
+        assert (c.ni, c.nj, c.nk, c.Iadim, c.Jadim, c.Kadim) == (32, 16, 10, 34, 19, 12)
+        assert c.Nelem3d == 34 * 19 * 12 and c.k_sponge == 2
+        ts = c.array("t_sponge_inv", 3)
+        assert ts[0] == 1.0 / (30.0 * 60) and ts[2] == 0.0 and 0 < ts[1] < ts[0]
+    finally:
+        ea.free_case(c)

@@ -25,6 +25,16 @@ CASES = {
     "venus_32x16x5": ea.spec(ea.VENUS_LIKE, 32, 16, 5, 600, 14, 0.5, 0.0, 0.2, 0.2),
     "hot_jupiter_32x16x8": ea.spec(ea.HOT_JUPITER, 32, 16, 8, 30, 3, 0.5, 0.0, 1.0, 0.0),
     "hs_128x64x16_divdamp": ea.spec(ea.HELD_SUAREZ, 128, 64, 16, 300, 3, 0.5, 0.1, 0.3, 0.2),
+    # the reference's sponge (as `initial` sets it), Rayleigh drag and
+    # layer-mean heating removal
+    "jupiter_sponge_drag_32x16x10": ea.spec(ea.JUPITER_LIKE, 32, 16, 10, 60, 15, 0.5, 0.0, 0.0, 1.0,
+                                            tau_drag=2.0e4, drag_v=1, drag_zonal_avg=1),
+    "saturn_sponge_ray_32x12x8": ea.spec(ea.SATURN_LIKE, 32, 12, 8, 60, 19, 0.4, 0.0, 0.0, 0.5,
+                                         k_sponge=2, tau_drag=5.0e4, drag_v=1),
+    "hs_adjust_prandtl_32x16x8": ea.spec(ea.HELD_SUAREZ, 32, 16, 8, 600, 21, 0.5, 0.1, 0.3, 0.4,
+                                         newt_cool_adjust=1, prandtl=0.5, drag_v=1),
+    "hot_jupiter_sponge_prandtl_32x16x8": ea.spec(ea.HOT_JUPITER, 32, 16, 8, 30, 4, 0.5, 0.0, 1.0, 0.0,
+                                                  prandtl=0.3, drag_zonal_avg=1),
 }
 
 
@@ -87,13 +97,14 @@ def test_wide_rows(ni, nj, nk, dt):
         p.close()
 
 
-@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny"])
+@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny", "jup_drag_tiny", "hs_adjust_tiny"])
 def test_against_reference_fixtures(name):
     """The GPU path against arrays written by the reference's own build."""
     import json
+    from tests.golden.make_golden import spec_of
     gj = json.load(open(os.path.join(GOLD, "golden.json")))
     z = np.load(os.path.join(GOLD, f"{name}.npz"))
-    sp = ea.spec(*gj["cases"][name]["spec"])
+    sp = spec_of(gj["cases"][name])
     eng = ea.Engine(ea.FP64, 1)
     s = eng.add(sp)
     nud = CpuRef(sp, nudged=True)

@@ -22,15 +22,17 @@ GJ = json.load(open(os.path.join(GOLD, "golden.json")))
 
 
 def spec_of(name):
-    p, ni, nj, nk, dt, seed, nu4, nudiv, nth, nuv = GJ["cases"][name]["spec"]
-    return ea.spec(p, ni, nj, nk, dt, seed, nu4, nudiv, nth, nuv)
+    """The case spec a golden fixture was written from (its positional
+    values and the reference switches it set)."""
+    from tests.golden.make_golden import spec_of as golden_spec
+    return golden_spec(GJ["cases"][name])
 
 
 def same_bits(a, b):
     return a.tobytes() == b.tobytes()
 
 
-@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny"])
+@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny", "jup_drag_tiny", "hs_adjust_tiny"])
 def test_oracle_matches_golden_fields(name):
     z = np.load(os.path.join(GOLD, f"{name}.npz"))
     r = cr.CpuRef(spec_of(name))
@@ -61,26 +63,53 @@ def test_oracle_matches_golden_c1_100_steps():
             assert hashlib.sha256(r.read(nm).tobytes()).hexdigest() == h[nm], (step, nm)
 
 
-def _ref_run(args, dumps, tmp_path):
+def _ref_run(args, dumps, tmp_path, opts=None):
     fn = str(tmp_path / "ref.bin")
-    cmd = [REF_EXE] + [str(a) for a in args] + [str(max(dumps)), fn] + [str(d) for d in dumps]
+    cmd = ([REF_EXE] + [f"--{k}={v}" for k, v in (opts or {}).items()] + [str(a) for a in args] +
+           [str(max(dumps)), fn] + [str(d) for d in dumps])
     rc = subprocess.run(cmd, cwd=str(tmp_path)).returncode
-    assert rc == 0, "reference run failed or its metric arrays differ from the case builder's"
+    assert rc == 0, "reference run failed or its metric arrays (t_sponge_inv included) differ from the case builder's"
     from tests.golden.make_golden import read_dump
     return read_dump(fn)
 
 
+# (positional spec values, reference switches).  k_sponge -1 is the
+# reference `initial` program's choice (src/shared/epic_initial.c:1249-1266):
+# the reference's own set_sponge() then fills t_sponge_inv in the harness.
+REF_CASES = [
+    ((0, 32, 20, 8, 900, 11, 0.5, 0.15, 0.3, 0.4), {}),           # HS with divergence damping
+    ((1, 32, 16, 10, 60, 12, 0.5, 0.0, 0.0, 1.0), {"k_sponge": 0}),  # gas giant: gz_from_u every call
+    ((2, 64, 24, 12, 60, 13, 0.4, 0.1, 0.0, 0.5), {"k_sponge": 0}),  # Saturn-like
+    ((3, 16, 12, 5, 600, 14, 0.5, 0.0, 0.2, 0.2), {"k_sponge": 0}),  # slow rotator, nk = 5
+    # the top sponge as `initial` sets it (2 layers at nk = 10)
+    ((1, 32, 16, 10, 60, 15, 0.5, 0.0, 0.0, 1.0), {"k_sponge": -1}),
+    ((2, 32, 12, 20, 60, 16, 0.4, 0.0, 0.0, 0.5), {"k_sponge": -1}),   # 4 layers
+    # Rayleigh drag: u only, pointwise; u and v about the zonal mean
+    ((1, 32, 16, 10, 60, 17, 0.5, 0.0, 0.0, 1.0), {"k_sponge": 1, "tau_drag": 1.0e5}),
+    ((3, 16, 12, 6, 600, 18, 0.5, 0.0, 0.2, 0.5),
+     {"k_sponge": -1, "tau_drag": 3.0e4, "drag_v": 1, "drag_zonal_avg": 1}),
+    ((2, 32, 12, 8, 60, 19, 0.4, 0.0, 0.0, 0.5), {"k_sponge": 2, "tau_drag": 5.0e4, "drag_v": 1}),
+    # Held-Suarez: layer-mean removal of the heating; prandtl/t_rad drag
+    ((0, 32, 16, 8, 600, 20, 0.5, 0.0, 0.3, 0.4), {"newt_cool_adjust": 1}),
+    ((0, 32, 16, 8, 600, 21, 0.5, 0.1, 0.3, 0.4), {"prandtl": 0.5, "drag_v": 1}),
+    ((0, 32, 16, 8, 600, 22, 0.5, 0.0, 0.3, 0.4), {"prandtl": 0.2, "drag_zonal_avg": 1, "newt_cool_adjust": 1}),
+]
+
+
+def case_spec(args, opts):
+    sp = ea.spec(*args, k_sponge=opts.get("k_sponge", 0))
+    for k, v in opts.items():
+        if k != "k_sponge":
+            setattr(sp, k, v)
+    return sp
+
+
 @pytest.mark.skipif(not os.path.exists(REF_EXE), reason="reference build (oracle/_ref) not present")
-@pytest.mark.parametrize("args", [
-    (0, 32, 20, 8, 900, 11, 0.5, 0.15, 0.3, 0.4),    # HS with divergence damping
-    (1, 32, 16, 10, 60, 12, 0.5, 0.0, 0.0, 1.0),     # gas giant: gz_from_u every call
-    (2, 64, 24, 12, 60, 13, 0.4, 0.1, 0.0, 0.5),     # Saturn-like
-    (3, 16, 12, 5, 600, 14, 0.5, 0.0, 0.2, 0.2),     # slow rotator, nk = 5
-])
-def test_oracle_bitwise_vs_reference_build(args, tmp_path):
+@pytest.mark.parametrize("args,opts", REF_CASES, ids=[f"{a[:4]}{o}" for a, o in REF_CASES])
+def test_oracle_bitwise_vs_reference_build(args, opts, tmp_path):
     dumps = [0, 1, 2, 3, 7]
-    ref = _ref_run(args, dumps, tmp_path)
-    r = cr.CpuRef(ea.spec(*args))
+    ref = _ref_run(args, dumps, tmp_path, opts)
+    r = cr.CpuRef(case_spec(args, opts))
     r.sync()
     done = 0
     for hdr, f in ref:
@@ -113,11 +142,17 @@ def test_hot_jupiter_relaxation_is_the_stated_closed_form():
     to its definition, evaluated here independently (lat/lon of the h-points
     from the grid spacing, the reference temperature through exp/log):
     HEAT3 = cp (T_eq - T)/tau, T_eq = T_ref(p) + dT(p)(max(0, cos lat cos lon)
-    - 1/4), dT = 1000 K for p <= 1 bar, linear in ln p to 0 at 10 bar."""
+    - 1/4), dT = 1000 K for p <= 1 bar, linear in ln p to 0 at 10 bar.
+    With a top sponge of k layers the layers K <= k lose their area-weighted
+    mean (the cell area 1/mn), as the reference's newtonian_cooling does
+    (src/shared/epic_sensible_heating.c:139-167)."""
     ni, nj, nk = 32, 16, 8
-    r = cr.CpuRef(ea.spec(ea.HOT_JUPITER, ni, nj, nk, 30, 3, 0.5, 0.0, 1.0, 0.0))
+    r = cr.CpuRef(ea.spec(ea.HOT_JUPITER, ni, nj, nk, 30, 3, 0.5, 0.0, 1.0, 0.0, k_sponge=0))
     r.sync()
     heat, t3, p3 = r.read("HEAT3"), r.read("T3"), r.read("P3")
+    rs = cr.CpuRef(ea.spec(ea.HOT_JUPITER, ni, nj, nk, 30, 3, 0.5, 0.0, 1.0, 0.0, k_sponge=3))
+    rs.sync()
+    heat_s = rs.read("HEAT3")
     cp, rgas, g, n2, thb, p0, pb = 14300.0, 3523.0, 9.42, 1.5e-6, 1500.0, 1.0e5, 1.0e6
     kap = rgas / cp
     aok = n2 * rgas / g ** 2 / kap
@@ -132,6 +167,21 @@ def test_hot_jupiter_relaxation_is_the_stated_closed_form():
     want = cp * (tref + dT * (day - 0.25) - t) / 1.0e5
     got = heat[1:nk + 1, 1:nj + 2, 1:ni + 1]
     assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()
+    # the sponge layers 1..3 lose their area-weighted mean, the rest do not.
+    # Cell area 1/mn (set_fmn on a sphere): re^2 cos(lat) dlon dlat on the
+    # interior rows; on the two pole rows n is divided by 1 + sqrt(1/2) and
+    # mn = n^2 / (dlon/2), i.e. area = re^2 dlon dlat^2 (1 + sqrt(1/2))^2 / 2
+    dlnr, dltr = np.radians(360.0 / ni), np.radians(dlt)
+    w = np.cos(lat) * dlnr * dltr
+    w[0] = w[-1] = 0.5 * dlnr * dltr ** 2 * (1.0 + np.sqrt(0.5)) ** 2
+    diff = heat[1:nk + 1, 1:nj + 2, 1:ni + 1] - heat_s[1:nk + 1, 1:nj + 2, 1:ni + 1]
+    for k in range(nk):
+        if k + 1 <= 3:
+            havg = (want[k] * w[:, None]).sum() / (w.sum() * ni)
+            assert np.abs(diff[k] - havg).max() <= 1e-9 * np.abs(want[k]).max(), k
+            assert abs(havg) > 1e-6 * np.abs(want[k]).max()
+        else:
+            assert not diff[k].any()
     # the pads are the zonal periodic copies, the top interface is not heated
     assert np.array_equal(heat[:, :, 0], heat[:, :, ni]) and np.array_equal(heat[:, :, ni + 1], heat[:, :, 1])
     assert not heat[0].any()
