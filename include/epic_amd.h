@@ -341,6 +341,32 @@ int  epic_engine_restore_spec(epic_engine *e, const epic_case_spec *spec, const 
 int  epic_engine_autotune(epic_engine *e, int steps, char *report, int cap);
 const char *epic_engine_last_error(epic_engine *e);
 
+/* ------------------------------------------------------------------------
+ * Allocation-free step.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It does what
+ * epic_engine_add (action EPIC_SYNC_DIAGS_ONLY: prime the diagnostics) or
+ * epic_engine_step (action EPIC_STEP_PROGS_AND_SYNC_DIAGS: nsteps steps)
+ * does for one FP64 member, with the same kernels and the same results, on
+ * fields the caller holds in device memory: fields[f] is the device array of
+ * epic_field f (Nelem3d doubles; Nelem2d for EPIC_F_GZ_SURFACE; the three
+ * u and the three v tendency planes consecutive, fields[EPIC_F_DUDT0 + p] ==
+ * fields[EPIC_F_DUDT0] + p * Nelem3d).  RI2 is formed at the end of the call.
+ * The case supplies the metrics, tables and u_spinup (its state arrays are
+ * not read).  it4: {IT_ZERO, IT_MINUS1, IT_MINUS2, itime} in and out.
+ * The library's working memory is a fixed static device array of
+ * EPIC_NOALLOC_SCRATCH_BYTES per GPU (room for Nelem3d up to about 780,000,
+ * e.g. 256x128x20), cleared at the start of every call; larger members are
+ * refused (-2).  The call queues its work on `stream` (a hipStream_t; NULL:
+ * the default stream) and waits for it.  Calls that overlap on one GPU share
+ * the array and race.  Returns 0, or a negative code with the reason in
+ * err (-1 bad arguments or configuration, -2 too large, -4 a fatal
+ * condition of the step, -5 a HIP error).
+ */
+#define EPIC_NOALLOC_SCRATCH_BYTES ((size_t)96 << 20)
+int  epic_step_noalloc(const epic_case *c, double *const *fields, int *it4, int action, int nsteps,
+                       void *stream, char *err, size_t errcap);
+
 /* Measurement helper with no counterpart in the reference: time a plain
  * device-to-device streaming copy of nbytes (reads nbytes, writes nbytes),
  * the measured copy peak beside the 8 TB/s specification. */
