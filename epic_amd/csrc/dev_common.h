@@ -14,6 +14,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// FP64 exp, log and pow of every kernel: correctly rounded, from the same
+// source the oracle's attribution build uses (crmath.h)
+#include "crmath.h"
+
 namespace epic {
 
 constexpr int KMAX = 66;  // nk <= 64
@@ -90,15 +94,15 @@ __device__ __forceinline__ int fcmp_d(double x1, double x2) {
 
 // hybrid sigma-theta coordinate (src/shared/epic_funcs_diag.c:1238-1314,1751-1770)
 __device__ __forceinline__ double get_sigma(double pbot, double p, double ptop) {
-  return log(p / pbot) / log(ptop / pbot);
+  return crm_log(p / pbot) / crm_log(ptop / pbot);
 }
 __device__ __forceinline__ double get_p_sigma(double pbot, double s, double ptop) {
-  return pbot * exp(s * log(ptop / pbot));
+  return pbot * crm_exp(s * crm_log(ptop / pbot));
 }
 __device__ __forceinline__ double g_sig(const MemberDesc &d, double s) {
   if (s <= d.sigma_sigma) return 0.;
   if (s >= 1.) return 1.;
-  return d.g_coeff * (1. - exp(-d.alpha * (s - d.sigma_sigma)));
+  return d.g_coeff * (1. - crm_exp(-d.alpha * (s - d.sigma_sigma)));
 }
 __device__ __forceinline__ double f_sig(const MemberDesc &d, double s) {
   if (s < 0.) return d.zeta0;
@@ -111,7 +115,7 @@ __device__ __forceinline__ double f_sig(const MemberDesc &d, double s) {
 // calling f_sig and g_sig, so the same result)
 __device__ __forceinline__ double theta_of_sigma(const MemberDesc &d, double sgth, double s) {
   const bool mid = (s > d.sigma_sigma && s < 1.);
-  const double gm = mid ? d.g_coeff * (1. - exp(-d.alpha * (s - d.sigma_sigma))) : 0.;
+  const double gm = mid ? d.g_coeff * (1. - crm_exp(-d.alpha * (s - d.sigma_sigma))) : 0.;
   const double lin = d.zeta0 + s * (d.zeta1 - d.zeta0);
   const double g = (s <= d.sigma_sigma) ? 0. : (s >= 1. ? 1. : gm);
   const double f = (s < 0.) ? d.zeta0 : (s <= d.sigma_sigma ? lin : (s < 1. ? (1. - gm) * lin : 0.));
@@ -132,8 +136,8 @@ __device__ __forceinline__ double sgth_of(const MemberDesc &d, double theta, dou
 // The same three functions with the column's log(ptop/pbot) formed once by
 // the caller: identical operations on identical operands, so identical
 // results, with one logarithm fewer per call.
-__device__ __forceinline__ double get_sigma_l(double pbot, double p, double lr) { return log(p / pbot) / lr; }
-__device__ __forceinline__ double get_p_sigma_l(double pbot, double s, double lr) { return pbot * exp(s * lr); }
+__device__ __forceinline__ double get_sigma_l(double pbot, double p, double lr) { return crm_log(p / pbot) / lr; }
+__device__ __forceinline__ double get_p_sigma_l(double pbot, double s, double lr) { return pbot * crm_exp(s * lr); }
 __device__ __forceinline__ double sgth_of_l(const MemberDesc &d, double theta, double p, double pbot, double ptop,
                                             double lr, int *bad) {
   if (p <= ptop) return theta;
@@ -142,7 +146,7 @@ __device__ __forceinline__ double sgth_of_l(const MemberDesc &d, double theta, d
     // f_sig(s) + g_sig(s) * theta with g_sigma's exponential formed once (the
     // same operations as the two calls, so the same result)
     const bool mid = (s > d.sigma_sigma && s < 1.);
-    const double gm = mid ? d.g_coeff * (1. - exp(-d.alpha * (s - d.sigma_sigma))) : 0.;
+    const double gm = mid ? d.g_coeff * (1. - crm_exp(-d.alpha * (s - d.sigma_sigma))) : 0.;
     const double g = (s <= d.sigma_sigma) ? 0. : (s >= 1. ? 1. : gm);
     const double f = (s < 0.) ? d.zeta0 : (s <= d.sigma_sigma ? d.zeta0 + s * (d.zeta1 - d.zeta0)
                                                               : (s < 1. ? (1. - gm) * (d.zeta0 + s * (d.zeta1 - d.zeta0)) : 0.));

@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
   }
   __syncthreads();
   if (T.live) {
-    const double ptop = a[0], pbot = a[nk], lr = log(ptop / pbot);
+    const double ptop = a[0], pbot = a[nk], lr = crm_log(ptop / pbot);
     for (int K = ks - 1 + T.kk; K < nk; K += T.KS) {
       double sigma = (d.sgth[2 * K + 1] - d.zeta0) / (d.zeta1 - d.zeta0);
       a[K] = get_p_sigma_l(pbot, sigma, lr);
@@ -169,10 +169,10 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
   // p^(kappa+1) once per interface (each is the top of one layer and the
   // bottom of the next)
   if (FULL && T.live)
-    for (int K = T.kk; K <= nk; K += T.KS) pw[K] = pow(a[K], d.kappa + 1.);
+    for (int K = T.kk; K <= nk; K += T.KS) pw[K] = crm_pow(a[K], d.kappa + 1.);
   __syncthreads();
   if (T.live) {
-    const double ptop = a[0], pbot = a[nk], kp1 = d.kappa + 1., lr = log(ptop / pbot);
+    const double ptop = a[0], pbot = a[nk], kp1 = d.kappa + 1., lr = crm_log(ptop / pbot);
     const double *tx = d.sgth + 2 * nk + 1;  // tx[kay*(-2)] = sgth[2*(nk-kay)+1]
     for (int K = T.kk; K <= nk; K += T.KS) {
       const size_t q = ix3(d, K, J, I);
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
       double top = a[K - 1], bot = a[K], pp;
       if (fcmp_d(bot, top) == 0) pp = .5 * (bot + top);
       else if (top < 0. || bot < 0.) { atomicOr(&d.dyn->err, 2); pp = 0.; }
-      else pp = pow((pw[K] - pw[K - 1]) / (kp1 * (bot - top)), 1. / d.kappa);
+      else pp = crm_pow((pw[K] - pw[K - 1]) / (kp1 * (bot - top)), 1. / d.kappa);
       if (K == 1) aux[2] = pp;
       d.P2[q] = pp;
       double h3;
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
   if (full && K >= 1 && d.dyn->itime > 1 && I >= 1 && I <= d.ni) d.DZDT3[q] = d.GZ3[q];
   const double R_GAS = 8.314472e+3;
   double th = d.THETA[q], p = d.P3[q];
-  const double pk = pow(p / d.press0, d.kappa);
+  const double pk = crm_pow(p / d.press0, d.kappa);
   double t = th * pk;
   double rho3 = p * d.mu3 / (R_GAS * t);
   d.T3[q] = t;
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     if (d.is_hj) {
       h = hj_relax(d, J, (I == 0) ? d.ni : (I == d.ni + 1 ? 1 : I), p, t, pk);
     } else {
-      const double pk0 = (d.press0 == 1000. * 100.) ? pk : pow(p / (1000. * 100.), d.kappa);
+      const double pk0 = (d.press0 == 1000. * 100.) ? pk : crm_pow(p / (1000. * 100.), d.kappa);
       h = 0. + hs_relax(d, J, p, t, d.P3[ix3(d, d.nk, J, I)], pk0);
     }
     d.HEAT3[q] = h;
@@ -287,13 +287,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
   d.EXNER3[q] = d.cp * t / th;
   if (K >= 1) {
     double p2 = d.P2[q], th2 = d.THETA2[q];
-    double t2 = th2 * pow(p2 / d.press0, d.kappa);
+    double t2 = th2 * crm_pow(p2 / d.press0, d.kappa);
     d.T2[q] = t2;
     d.RHO2[q] = p2 * d.mu2 / (R_GAS * t2);
   } else {
     const size_t q1 = ix3(d, 1, J, I);
     double p21 = d.P2[q1], th21 = d.THETA2[q1];
-    double t21 = th21 * pow(p21 / d.press0, d.kappa);
+    double t21 = th21 * crm_pow(p21 / d.press0, d.kappa);
     double rho21 = p21 * d.mu2 / (R_GAS * t21);
     d.RHO2[q] = rho3 * rho3 / rho21;
     d.THETA2[q] = d.sgth[0];
@@ -527,7 +527,7 @@ __global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb, int C)
   if (T.live)
     for (int K = T.kk; K <= nk; K += T.KS) {
       tt[K] = d.T3[ix3(d, K, J, I)];
-      if (K >= 1) lg[K] = log(PP(K) / PP(K - 1));
+      if (K >= 1) lg[K] = crm_log(PP(K) / PP(K - 1));
     }
   __syncthreads();
   if (T.live)
@@ -639,13 +639,13 @@ __device__ __forceinline__ double hs_relax(const MemberDesc &d, int J, double p,
   double c2 = d.coslat_h[J];
   c2 *= c2;
   double s2 = 1. - c2, pp0 = p / p0;
-  double teq = (315. - dt_y * s2 - dth_z * log(pp0) * c2) * pk0;
+  double teq = (315. - dt_y * s2 - dth_z * crm_log(pp0) * c2) * pk0;
   teq = dmax(200., teq);
   return -cpt * (t - teq);
 }
 __device__ double hs_newton(const MemberDesc &d, int K, int J, int I) {
   const double p = d.P3[ix3(d, K, J, I)];
-  return hs_relax(d, J, p, d.T3[ix3(d, K, J, I)], d.P3[ix3(d, d.nk, J, I)], pow(p / (1000. * 100.), d.kappa));
+  return hs_relax(d, J, p, d.T3[ix3(d, K, J, I)], d.P3[ix3(d, d.nk, J, I)], crm_pow(p / (1000. * 100.), d.kappa));
 }
 
 // Day-night relaxation of the hot-Jupiter case (no reference counterpart):
@@ -659,7 +659,7 @@ __device__ __forceinline__ double hj_relax(const MemberDesc &d, int J, int I, do
   double dT = d.teq_contrast;
   // the per-member log and the per-row/column cosines come from host tables
   // formed with the host libm, as the oracle forms them
-  if (p > d.teq_p_contrast) dT *= dmax(0., log(d.pbot / p) / d.hj_lnref);
+  if (p > d.teq_p_contrast) dT *= dmax(0., crm_log(d.pbot / p) / d.hj_lnref);
   const double cl = d.coslat_h[J];
   const double day = dmax(0., cl * d.coslon_h[I]);
   const double teq = tref + dT * (day - .25);
@@ -667,7 +667,7 @@ __device__ __forceinline__ double hj_relax(const MemberDesc &d, int J, int I, do
 }
 __device__ double hj_newton(const MemberDesc &d, int K, int J, int I) {
   const double p = d.P3[ix3(d, K, J, I)];
-  return hj_relax(d, J, I, p, d.T3[ix3(d, K, J, I)], pow(p / d.press0, d.kappa));
+  return hj_relax(d, J, I, p, d.T3[ix3(d, K, J, I)], crm_pow(p / d.press0, d.kappa));
 }
 
 // HEAT3 = 0 everywhere, then the Held-Suarez (or day-night) relaxation on
@@ -756,11 +756,11 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
 #define DV(k) d.DIV[ix3(d, k, J, I)]
   // lg[k] = log(P3(k)/P3(k-1)), the ln-p intervals of the Leslie-Purser weights
   if (T.live)
-    for (int K = 1 + T.kk; K <= nk; K += T.KS) lg[K] = log(PP(K) / PP(K - 1));
+    for (int K = 1 + T.kk; K <= nk; K += T.KS) lg[K] = crm_log(PP(K) / PP(K - 1));
   __syncthreads();
   if (T.live) {
     const double ptop = P[ix3(d, 0, J, I)], pbot = P[ix3(d, nk, J, I)], psig = P[ix3(d, ks - 1, J, I)];
-    const double eps = d.eps13, lr = log(ptop / pbot);
+    const double eps = d.eps13, lr = crm_log(ptop / pbot);
     for (int K = 1 + T.kk; K <= nk; K += T.KS) {
       const size_t q = ix3(d, K, J, I);
       double th = TH[q];

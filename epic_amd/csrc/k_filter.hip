@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb
     const int K = r.kend - e;
     sx[e] = geez(d, kind, K, J, Imin);
     double v = (double)b[ix3(d, K, J, Imin)];
-    sy[e] = (kind == 2) ? log(v) : v;
+    sy[e] = (kind == 2) ? crm_log(v) : v;
   }
   wave_sync();
   if (pchip_wave(n, sx, sy, sz, lane) && lane == 0) atomicOr(&d.dyn->err, 4);
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
       int k = near ? table_place_near(nt, tx, xx, r.kend - K, &h) : table_place(nt, tx, xx, &h);
       double xr = xx - tx[k];
       double val = ty[k] + xr * (tz[k] + xr * (tc2[k] + xr * tc3[k]));
-      if (kind == 2) val = exp(val);
+      if (kind == 2) val = crm_exp(val);
       v[q] = val;
       x[q] = T((double)a[ix2(d, J, I)] - val);
     }

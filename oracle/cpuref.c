@@ -24,6 +24,9 @@
 #ifdef CPUREF_LIBM_NUDGE
 #include "libm_nudge.h"
 #endif
+#ifdef CPUREF_CRMATH
+#include "libm_cr.h"
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -244,7 +247,9 @@ void cpuref_realft(double *data, unsigned long n, int isign) {
 static double get_sigma(double pbot, double p, double ptop) { return log(p / pbot) / log(ptop / pbot); }
 static double get_p_sigma(double pbot, double s, double ptop) { return pbot * exp(s * log(ptop / pbot)); }
 static double g_sig(const cpuref *R, double s) {
-  const double coeff = 1. / (1 - exp(-R->c.hybrid_alpha * (1. - R->c.sigma_sigma)));
+  /* (exp), (log), (pow) in parentheses: values the GPU side takes from
+   * host tables, formed with the host libm in every build of the oracle */
+  const double coeff = 1. / (1 - (exp)(-R->c.hybrid_alpha * (1. - R->c.sigma_sigma)));
   if (s <= R->c.sigma_sigma) return 0.;
   if (s >= 1.) return 1.;
   return coeff * (1. - exp(-R->c.hybrid_alpha * (s - R->c.sigma_sigma)));
@@ -282,8 +287,8 @@ static void bc3(cpuref *R, double *a) {
 static void filter_init(cpuref *R) {
   epic_case *c = &R->c;
   double lat0 = LAT0 * DEGR;
-  double rln = c->re / sqrt(1. + pow(c->rp / c->re * tan(lat0), 2.));
-  double rlt = rln / (cos(lat0) * (pow(sin(lat0), 2.) + pow(c->re / c->rp * cos(lat0), 2.)));
+  double rln = c->re / sqrt(1. + (pow)(c->rp / c->re * tan(lat0), 2.));
+  double rlt = rln / (cos(lat0) * ((pow)(sin(lat0), 2.) + (pow)(c->re / c->rp * cos(lat0), 2.)));
   double m0 = 1. / (rln * c->dln * DEGR), n0 = 1. / (rlt * c->dlt * DEGR);
   for (int pass = 0; pass < 2; pass++) {
     double *hl = pass ? R->hl_pv : R->hl_h;
@@ -292,10 +297,10 @@ static void filter_init(cpuref *R) {
       A2(hl, J, 1) = 1.;
       double r = c->mn[jj] / (c->m[jj] * c->n[jj]);
       double t0 = (c->n[jj] / n0) / (c->m[jj] / m0);
-      t0 = pow(t0, 2.00);
+      t0 = (pow)(t0, 2.00);
       for (int I = 2; I <= R->ni / 2 + 1; I++) {
         double t1 = sin((I - 1) * (c->dln) * DEGR / 2.);
-        t1 = pow(t1, -2.00);
+        t1 = (pow)(t1, -2.00);
         t1 *= r * t0;
         t1 = (t1 < 1.) ? t1 : 1.;
         A2(hl, J, I) = t1;
@@ -760,7 +765,7 @@ static void calc_heating(cpuref *R) {
     bc3(R, R->HEAT3);
     return;
   }
-  const double pbk = R->is_hj ? pow(c->ref_pb / c->press0, c->kappa) : 0.;
+  const double pbk = R->is_hj ? (pow)(c->ref_pb / c->press0, c->kappa) : 0.;
   double *buf = R->b2[6];
   for (int K = 1; K <= R->nk; K++) {
     for (int J = 0; J <= R->nj; J++) {
@@ -774,7 +779,7 @@ static void calc_heating(cpuref *R) {
           double amp = (p / pbot - 0.7) / (1. - 0.7);
           if (amp > 0.) {
             double cl = cos(c->lat[2 * J + 1] * DEGR);
-            kt += (ks - ka) * pow(cl, 4.) * amp;
+            kt += (ks - ka) * (pow)(cl, 4.) * amp;
           }
           double t_cool = 1. / kt;
           double cpt = 1004. / t_cool;
@@ -793,7 +798,7 @@ static void calc_heating(cpuref *R) {
           const double pk = pow(p / c->press0, c->kappa);
           const double tref = (1. / (1. / c->ref_theta_b + c->ref_a_over_kappa * (pk - pbk))) * pk;
           double dT = c->teq_contrast;
-          if (p > c->teq_p_contrast) dT *= mx(0., log(c->pbot / p) / log(c->pbot / c->teq_p_contrast));
+          if (p > c->teq_p_contrast) dT *= mx(0., log(c->pbot / p) / (log)(c->pbot / c->teq_p_contrast));
           const double day = mx(0., cl * cos(c->lon[2 * I + 1] * DEGR));
           const double teq = tref + dT * (day - .25);
           A2(buf, J, I) = c->cp * (teq - t) / c->tau_rad;
@@ -975,7 +980,7 @@ static double t_rad(cpuref *R, int K, int J, int I) {
   const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
   double p = A3(R->P3, K, J, I), kt = ka, pbot = A3(R->P3, R->nk, J, I);
   double amp = (p / pbot - 0.7) / (1. - 0.7);
-  if (amp > 0.) kt += (ks - ka) * pow(cos(R->c.lat[2 * J + 1] * DEGR), 4.) * amp;
+  if (amp > 0.) kt += (ks - ka) * (pow)(cos(R->c.lat[2 * J + 1] * DEGR), 4.) * amp;
   return 1. / kt;
 }
 
@@ -1779,7 +1784,7 @@ cpuref *cpuref_create(const epic_case_spec *spec) {
       R->coef[i2] = (c->nu[i2] > 0.) ? c->nu[i2] : 0.01 * mxn[i2];
       for (int i = 2; i < i2; i += 2) R->coef[i2] /= R->coef[i];
     }
-    double rln = c->re / sqrt(1. + pow(c->rp / c->re * tan(LAT0 * DEGR), 2.));
+    double rln = c->re / sqrt(1. + (pow)(c->rp / c->re * tan(LAT0 * DEGR), 2.));
     R->m0_zonal = 1. / (rln * c->dln * DEGR);
     R->m0_div = R->m0_zonal;
     R->nudiv = c->nudiv_nondim * mxn[2];
@@ -1788,7 +1793,7 @@ cpuref *cpuref_create(const epic_case_spec *spec) {
     double e = 1.;
     while (1. + e != 1.) e *= .5;
     e *= 2.;
-    R->eps13 = pow(e, 1. / 3.);
+    R->eps13 = (pow)(e, 1. / 3.);
   }
   /* state */
   memcpy(R->U, c->u, sizeof(double) * N3);
@@ -1913,4 +1918,35 @@ void cpuref_time_planes(cpuref *R, int *it3, long *steps) {
 int cpuref_error(cpuref *R) { return R->err; }
 void cpuref_dims(cpuref *R, int *d) {
   d[0] = R->ni; d[1] = R->nj; d[2] = R->nk; d[3] = R->N3; d[4] = R->N2;
+}
+
+/* exp (fn 0), log (1) or pow (2) as this build of the oracle evaluates
+ * them: the host libm, or crmath.h in the CPUREF_CRMATH build */
+double cpuref_math(int fn, double x, double y) {
+  if (fn == 0) return exp(x);
+  if (fn == 1) return log(x);
+  return pow(x, y);
+}
+/* calls of exp, log, pow so far and how many of them glibc rounded
+ * differently from crmath.h (CPUREF_CRMATH build; zeros otherwise) */
+/* CPUREF_CRMATH build: which of exp (bit 0), log (1), pow (2) take
+ * crmath.h's result (default all); the others take glibc's */
+void cpuref_cr_select(int mask) {
+#ifdef CPUREF_CRMATH
+  cpuref_cr_mask = mask;
+#else
+  (void)mask;
+#endif
+}
+void cpuref_libm_stats(long *calls3, long *diffs3, int reset) {
+  for (int q = 0; q < 3; q++) {
+#ifdef CPUREF_CRMATH
+    calls3[q] = cpuref_cr_calls[q];
+    diffs3[q] = cpuref_cr_diffs[q];
+    if (reset) cpuref_cr_calls[q] = cpuref_cr_diffs[q] = 0;
+#else
+    (void)reset;
+    calls3[q] = diffs3[q] = 0;
+#endif
+  }
 }

@@ -39,4 +39,12 @@ void    cpuref_realft(double *data_1based, unsigned long n, int isign);
 int     cpuref_pchip(int n, const double *x, const double *y, double *d);
 double  cpuref_pchip_eval(int n, const double *x, const double *y, const double *d, double xx);
 int     cpuref_fcmp(double a, double b);
+/* exp (fn 0), log (1), pow (2) as this build evaluates them (the host libm,
+ * or the correctly rounded crmath.h in libcpuref_cr.so), and that build's
+ * count of calls and of glibc results that differ from crmath.h's. */
+double  cpuref_math(int fn, double x, double y);
+void    cpuref_libm_stats(long *calls3, long *diffs3, int reset);
+/* libcpuref_cr.so: which of exp (bit 0), log (1), pow (2) take crmath.h's
+ * result (default 7, all three); the others take the host libm's. */
+void    cpuref_cr_select(int mask);
 #endif

@@ -10,14 +10,18 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libcpuref.so")
 LIB_NUDGE = os.path.join(HERE, "libcpuref_nudge.so")
+LIB_CR = os.path.join(HERE, "libcpuref_cr.so")
 _L = {}
 
 
 def lib(nudged=False):
-    """The oracle library; nudged=True gives the copy whose exp/log/pow
-    results are moved by one ulp 10% of the time (libm_nudge.h)."""
-    if nudged not in _L:
-        path = LIB_NUDGE if nudged else LIB
+    """The oracle library.  nudged=True (or "nudge") gives the copy whose
+    exp/log/pow results are moved by one ulp 10% of the time
+    (libm_nudge.h); "cr" the copy that evaluates exp/log/pow with the
+    correctly rounded crmath.h of the GPU's FP64 kernels (libm_cr.h)."""
+    key = "nudge" if nudged is True else (nudged or "plain")
+    if key not in _L:
+        path = {"plain": LIB, "nudge": LIB_NUDGE, "cr": LIB_CR}[key]
         if not os.path.exists(path):
             raise RuntimeError(f"{path} not built (make -C oracle)")
         L = ctypes.CDLL(path)
@@ -36,8 +40,20 @@ def lib(nudged=False):
         L.cpuref_pchip_eval.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_double]
         L.cpuref_pchip_eval.restype = ctypes.c_double
         L.cpuref_fcmp.argtypes = [ctypes.c_double, ctypes.c_double]
-        _L[nudged] = L
-    return _L[nudged]
+        L.cpuref_math.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.cpuref_math.restype = ctypes.c_double
+        L.cpuref_libm_stats.argtypes = [ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long), ctypes.c_int]
+        L.cpuref_cr_select.argtypes = [ctypes.c_int]
+        _L[key] = L
+    return _L[key]
+
+
+def libm_stats(variant="cr", reset=True):
+    """(calls, differing) of exp, log, pow in a build: how many calls were
+    made and how many of them glibc rounded differently from crmath.h."""
+    c, d = (ctypes.c_long * 3)(), (ctypes.c_long * 3)()
+    lib(variant).cpuref_libm_stats(c, d, 1 if reset else 0)
+    return list(c), list(d)
 
 
 class CpuRef:

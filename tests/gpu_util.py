@@ -1,20 +1,23 @@
 """Helpers shared by the GPU parity tests: run a case on the HIP engine and
 on the oracle side by side, and measure their difference.
 
-Tolerance.  The GPU evaluates exp, log and pow with the device math library,
-whose results may differ from the host libm's in the last bit; every other
-operation of the step is performed in the reference's order with no
-contraction, so those last-bit differences are the only source of
-disagreement, and the reference algorithm carries them forward (its
-finite-volume pressure gradient amplifies them by 1e4-1e5 in near-balanced
-states).  The oracle is therefore run twice: as is, and with 10% of its own
-exp/log/pow results moved by one ulp (oracle/libm_nudge.h).  The second
-run's distance from the first is the sensitivity floor of the case.  A
-field passes when its GPU-vs-oracle relative L2 error is at most
-    max(base, FLOOR_FACTOR * floor)
-with base = 1e-12 for u, v, h, theta, p and 1e-10 for the diagnostics and
-tendencies.  Any index, sign or missing-term mistake gives errors many
-orders of magnitude above this bound."""
+Two oracles.  The GPU's FP64 kernels evaluate exp, log and pow with the
+correctly rounded functions of epic_amd/csrc/crmath.h and perform every
+other operation in the reference's order with no contraction.  The oracle
+built with the same crmath.h (oracle/libcpuref_cr.so) must therefore equal
+the GPU bit for bit in every field, and that is the parity check proper:
+any index, sign, order or missing-term mistake breaks it.
+
+Against the plain oracle (host libm, bit-exact with the reference's own
+build) the GPU differs only through glibc's own last-bit roundings of exp,
+log and pow (it rounds about 5e-4 of them differently from the correctly
+rounded value), which the dynamics carry forward: one ulp of theta at a
+single cell grows to ~3e-12 (u) and ~5e-12 (v) relative after 100 steps of
+Held-Suarez 64x32x10 (tests/test_crmath.py).  That distance is reported
+beside the distance of a second copy of the oracle with 10% of its
+exp/log/pow results moved by one ulp (the floor); h, theta and p are held
+to 1e-12 (diagnostics 1e-10) or ten times the floor, u and v to ten times
+the floor."""
 import sys
 
 import numpy as np
@@ -43,23 +46,36 @@ def say(*a):
     sys.stdout.flush()
 
 
+def bitwise_diff(get_a, get_b, fields=None):
+    """[(field, cells differing)] of two sources of fields."""
+    out = []
+    for f in (fields or ea.FIELDS):
+        a, b = get_a(f), get_b(f)
+        if a.tobytes() != b.tobytes():
+            out.append((f, int((a != b).sum())))
+    return out
+
+
 class Pair:
-    """One case advanced on the GPU engine, on the oracle and on the
+    """One case advanced on the GPU engine, on the oracle with the GPU's
+    correctly rounded exp/log/pow, on the plain oracle and on the
     libm-nudged oracle."""
 
     def __init__(self, sp, precision=ea.FP64):
         self.sp = sp
         self.eng = ea.Engine(precision, 1)
         self.slot = self.eng.add(sp)
+        self.cr = CpuRef(sp, nudged="cr")
+        self.cr.sync()
         self.ref = CpuRef(sp)
         self.ref.sync()
         self.nud = CpuRef(sp, nudged=True)
         self.nud.sync()
-        self.fn = None  # oracle instance used as a function of the GPU's fields
         self.steps = 0
 
     def step(self, n):
         self.eng.step(n)
+        self.cr.step(n)
         self.ref.step(n)
         self.nud.step(n)
         self.steps += n
@@ -73,9 +89,10 @@ class Pair:
         return self.field(f"D{wind}DT{it[1]}")
 
     def report(self, tag):
-        """{name: (gpu error, floor, tolerance)} for every checked field."""
+        """{name: (gpu error vs the plain oracle, floor, tolerance)}."""
         out = {}
-        items = [(f, self.field(f), PROG_TOL if f in PROG else DIAG_TOL) for f in PROG + DIAG]
+        items = [(f, self.field(f), PROG_TOL if f in ("HDRY", "THETA", "P3") else (0.0 if f in PROG else DIAG_TOL))
+                 for f in PROG + DIAG]
         if self.steps:
             items += [(f"D{w}DT", self.tendency_field(w), DIAG_TOL) for w in ("U", "V")]
         for name, (g, r, n), base in items:
@@ -85,29 +102,20 @@ class Pair:
         return out
 
     def check(self, tag):
+        """Bit for bit against the oracle with the GPU's exp/log/pow (every
+        field, the tendency planes, GZ_SURFACE and RI2 included, and the
+        time planes), then within tolerance of the plain oracle."""
+        bad = bitwise_diff(lambda f: self.eng.read(self.slot, f), self.cr.read)
+        say(tag, "fields differing from the correctly rounded oracle:", bad)
+        assert not bad, (tag, bad)
+        assert self.eng.time_planes(self.slot)[:3] == self.cr.time_planes()[:3], tag
         rep = self.report(tag)
         bad = {k: v for k, v in rep.items() if v[0] > v[2]}
         assert not bad, (tag, bad)
-        self.check_ri2(tag)
         return rep
-
-    def check_ri2(self, tag):
-        """RI2 (a ratio of a near-cancelling buoyancy term to the squared
-        shear, with no libm call) is pinned bit for bit as a function: the
-        GPU's RI2 must equal the oracle's store_diag formula applied to the
-        GPU's own U, V, P2, T2, RHO2, P3, RHO3.  Its distance from the
-        oracle's RI2 is reported (one-ulp differences in P2, T2, RHO2 move it
-        by up to about 1e-9 in relative L2 through the cancellation)."""
-        if self.fn is None:
-            self.fn = CpuRef(self.sp)
-        g = {f: self.eng.read(self.slot, f) for f in ("U", "V", "P2", "T2", "RHO2", "P3", "RHO3", "RI2")}
-        want = self.fn.richardson_of(g)
-        say(tag, f"RI2 vs oracle {rel_l2(g['RI2'], self.ref.read('RI2')):.1e}")
-        assert g["RI2"].tobytes() == want.tobytes(), (tag, "RI2", int((g["RI2"] != want).sum()))
 
     def close(self):
         self.eng.close()
+        self.cr.close()
         self.ref.close()
         self.nud.close()
-        if self.fn is not None:
-            self.fn.close()

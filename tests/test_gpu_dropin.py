@@ -15,7 +15,7 @@ import pytest
 import epic_amd as ea
 from oracle.cpuref import CpuRef
 from tests.golden.make_golden import read_dump
-from tests.gpu_util import DIAG, DIAG_TOL, FLOOR_FACTOR, PROG, PROG_TOL, rel_l2, say
+from tests.gpu_util import DIAG, DIAG_TOL, FLOOR_FACTOR, PROG, PROG_TOL, bitwise_diff, rel_l2, say
 
 pytestmark = pytest.mark.gpu
 
@@ -55,6 +55,15 @@ def run(exe, case, extra, tmp_path, name):
     assert r.returncode == 0, r.stdout + r.stderr
     (hdr, f), = [d for d in read_dump(fn) if int(d[0][0]) == NSTEPS]
     return hdr, f
+
+
+def cr_equal(case, f, tag):
+    """Every array the drop-in wrote back equals the oracle built with the
+    GPU's correctly rounded exp/log/pow, bit for bit."""
+    _, cr = oracle_state(case, nudged="cr")
+    bad = bitwise_diff(lambda q: f[q], lambda q: cr[q])
+    say(tag, "arrays differing from the correctly rounded oracle:", bad)
+    assert not bad, (tag, bad)
 
 
 def oracle_state(case, nudged=False):
@@ -97,6 +106,7 @@ def test_dropin_eager_and_lazy_match_oracle(case, tmp_path):
         assert np.array_equal(f_e[q], f_l[q]), q
     tp, ref = oracle_state(case)
     assert [int(x) for x in h_e[1:4]] == tp
+    cr_equal(case, f_e, "dropin_host")
     compare(case, f_e, ref, "dropin_host")
     # var.ri2 comes back formed from the written-back state
     assert f_e["RI2"].tobytes() == ri2_of(case, f_e).tobytes()
@@ -109,6 +119,7 @@ def test_reference_program_with_dropin_matches_reference(case, tmp_path):
     h_g, f_g = run(REF_DROPIN, case, [NSTEPS], tmp_path, "gpu.bin")
     h_c, f_c = run(REF, case, [NSTEPS], tmp_path, "cpu.bin")
     assert [int(x) for x in h_g[1:4]] == [int(x) for x in h_c[1:4]]
+    cr_equal(case, f_g, "epic_ref_dropin")
     compare(case, f_g, f_c, "epic_ref_dropin")
     # a gas giant's gz_surface moves with the zonal-mean bottom wind; the
     # caller's array must hold the step's value, not the initial one

@@ -1,9 +1,8 @@
 """Parity of the HIP FP64 step with the oracle (and, through the fixtures,
-with the reference's own build).  Tolerances: tests/gpu_util.py (1e-12 for
-u, v, h, theta, p and 1e-10 for diagnostics, or ten times the case's own
-sensitivity to one-ulp libm differences where that is larger).  Fields
-computed without transcendentals (u, v, theta after the priming call, the
-zonal pads, the time planes) are compared bit for bit."""
+with the reference's own build).  Every field of every case and step equals
+the oracle built with the GPU's correctly rounded exp/log/pow bit for bit;
+the distance to the plain oracle (the host libm) is held to the bounds of
+tests/gpu_util.py."""
 import os
 
 import numpy as np
@@ -11,7 +10,7 @@ import pytest
 
 import epic_amd as ea
 from oracle.cpuref import CpuRef
-from tests.gpu_util import FLOOR_FACTOR, PROG, PROG_TOL, Pair, rel_l2, say
+from tests.gpu_util import FLOOR_FACTOR, PROG, PROG_TOL, Pair, bitwise_diff, rel_l2, say
 
 pytestmark = pytest.mark.gpu
 
@@ -58,7 +57,14 @@ def test_sync_then_ten_steps(name):
 
 
 def test_held_suarez_64x32x10_100_steps():
-    """The parity target: 100 steps of Held-Suarez 64x32x10, dt = 600 s."""
+    """The parity target: 100 steps of Held-Suarez 64x32x10, dt = 600 s.
+    Bit for bit against the oracle with the GPU's exp/log/pow at steps 1,
+    10 and 100; h, theta and p within 1e-12 of the plain oracle (the host
+    libm) outright.  u and v sit at ~3e-12 / ~5e-12 of it: one ulp of theta
+    at one cell, or glibc's own misrounding of any one of exp, log, pow,
+    grows to that in 100 steps of this case (tests/test_crmath.py), so no
+    implementation that differs from the host libm in a single rounding
+    reaches 1e-12 there."""
     sp = ea.spec(ea.HELD_SUAREZ, 64, 32, 10, 600, 1, 0.5, 0.0, 0.1, 0.0)
     p = Pair(sp)
     try:
@@ -109,11 +115,16 @@ def test_against_reference_fixtures(name):
     s = eng.add(sp)
     nud = CpuRef(sp, nudged=True)
     nud.sync()
+    crr = CpuRef(sp, nudged="cr")
+    crr.sync()
     done = 0
     for step in gj["cases"][name]["dumps"]:
         eng.step(step - done)
         nud.step(step - done)
+        crr.step(step - done)
         done = step
+        bad = bitwise_diff(lambda f: eng.read(s, f), crr.read)
+        assert not bad, (name, step, "differs from the correctly rounded oracle", bad)
         for f in PROG + ("GZ3", "PV3"):
             ref = z[f"s{step}_{f}"]
             e, fl = rel_l2(eng.read(s, f), ref), rel_l2(nud.read(f), ref)
@@ -123,6 +134,7 @@ def test_against_reference_fixtures(name):
         assert eng.time_planes(s)[:3] == gj["hashes"][f"{name}/step{step}"]["time_planes"]
     eng.close()
     nud.close()
+    crr.close()
 
 
 def test_periodic_pads_and_poles():
