@@ -35,6 +35,11 @@ static inline dim3 grid2(int nJ, int nI, int nK, int nm) {
 __device__ __forceinline__ double *tend(const MemberDesc &d, double *base, int plane) {
   return base + (size_t)plane * d.N3;
 }
+// the plane the current step's tendency accumulates in: the time plane
+// IT_ZERO, or the mixed mode's single FP64 plane
+__device__ __forceinline__ double *cur_tend(const MemberDesc &d, double *base) {
+  return d.mixed ? base : base + (size_t)d.dyn->it0 * d.N3;
+}
 
 // ------------------------------------------------------------- uv_core
 // UH (sA), VH (sB), KIN (sC) on planes K = 1..nk-1 with zonal pads; VH rows
@@ -88,7 +93,7 @@ __global__ void __launch_bounds__(256) k_dvdt_core(const MemberDesc *D, int mb) 
   const int K = 1 + blockIdx.y;
   const size_t o = (size_t)K * d.N2;
   const double *uh = d.sA + o, *vh = d.sB + o, *kin = d.sC + o, *pv = d.PV3 + o;
-  double *dv = tend(d, d.DVDT, d.dyn->it0) + o;
+  double *dv = cur_tend(d, d.DVDT) + o;
 #define PV(j, i) pv[ix2(d, j, i)]
   double ga = (PV(J, I) + PV(J, I + 1) + PV(J + 1, I + 1));
   double de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
@@ -127,7 +132,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
   filter_row_lds(d, kr, work, J, 0, live && d.rflag_h[J] != 0, m.lane);
   __syncthreads();
   if (!live) return;
-  double *du = tend(d, d.DUDT, d.dyn->it0) + o;
+  double *du = cur_tend(d, d.DUDT) + o;
   for (int I = 1 + m.lane; I <= ni; I += m.stride) {
 #define PV(j, i) pv[ix2(d, j, i)]
     double al = (PV(J, I) + PV(J + 1, I) + PV(J + 1, I + 1));
@@ -178,7 +183,7 @@ __global__ void __launch_bounds__(256) k_sponge(const MemberDesc *D, int mb) {
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = blockIdx.y;
   if (K > d.k_sponge) return;
-  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  double *du = cur_tend(d, d.DUDT), *dv = cur_tend(d, d.DVDT);
   size_t q = ix3(d, K, J, I);
   double ts = d.t_sponge[K];
   const double x = du[q] - (ts) * (d.U[q] - d.USPIN[q]);
@@ -206,7 +211,7 @@ __global__ void __launch_bounds__(256) k_rayleigh(const MemberDesc *D, int mb) {
   if (t >= nrow || nrow <= 0) return;
   if (!((d.newt_cool_on && d.prandtl > 0.) || (d.tau_drag > 0. && d.tau_drag < 1.e+20))) return;
   const int K = d.k_sponge + 1 + t / (d.nj + 1), J = t % (d.nj + 1);
-  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  double *du = cur_tend(d, d.DUDT), *dv = cur_tend(d, d.DVDT);
   const int ni = d.ni;
   const double za = (double)d.drag_zonal_avg, zb = (double)(1 - d.drag_zonal_avg);
   double uavg = 0.;
@@ -239,7 +244,7 @@ __global__ void __launch_bounds__(256) k_hs_pbl(const MemberDesc *D, int mb) {
   if (!d.is_hs) return;
   const double nu00 = (1. / (24. * 60. * 60.));
   const int nk = d.nk;
-  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  double *du = cur_tend(d, d.DUDT), *dv = cur_tend(d, d.DVDT);
   const double *P = d.P3;
   double pb = 0.5 * (P[ix3(d, nk, J, I)] + P[ix3(d, nk, J, I - 1)]);
   for (int K = nk - 1; K >= 1; K--) {
@@ -334,7 +339,7 @@ __global__ void __launch_bounds__(256) k_pgrad(const MemberDesc *D, int mb) {
   const int K = 1 + blockIdx.y;
   const double TOL = 1.e-2, DG = M_PI / 180.;
   const double *G = d.GZ3, *P2 = d.P2;
-  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  double *du = cur_tend(d, d.DUDT), *dv = cur_tend(d, d.DVDT);
 #define Z(k, j, i) G[ix3(d, k, j, i)]
 #define PZ(k, j, i) P2[ix3(d, k, j, i)]
   {
@@ -435,7 +440,7 @@ __global__ void __launch_bounds__(256) k_pgrad(const MemberDesc *D, int mb) {
 
 void L_pgrad(hipStream_t s, const Bucket &b) {
   LAUNCH(k_gz_polefill, dim3(2 * (b.nk + 1), 1, b.nm), dim3(64), sizeof(double) * (b.ni + 1), s, b.D, b.mb);
-  if (b.mixed) { LM_pgrad(s, b); return; }
+  // FP64 in both precision modes (k_mixed.hip says why)
   LAUNCH(k_pgrad, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
@@ -446,7 +451,7 @@ __global__ void __launch_bounds__(256) k_divdamp(const MemberDesc *D, int mb) {
   const int K = 1 + blockIdx.y;
   if (d.nudiv == 0.) return;
   const double *div = d.sB + (size_t)K * d.N2;
-  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
+  double *du = cur_tend(d, d.DUDT), *dv = cur_tend(d, d.DVDT);
   double coef = (d.lat[2 * J + 1] > 45.) ? d.nudiv * d.m0z : d.nudiv * d.m[2 * J + 1];
   du[ix3(d, K, J, I)] += coef * (div[ix2(d, J, I)] - div[ix2(d, J, I - 1)]);
   if (J >= 1) {

@@ -2,24 +2,22 @@
 //
 // The state (u, v, h, theta), the hydrostatic diagnostics (P3, P2, GZ3,
 // MONT3, RHO3, ...) and the Adams-Bashforth accumulation stay FP64; the
-// Adams-Bashforth history planes of DUDT, DVDT are FP32 and every tendency
-// term is formed in FP32: Coriolis and kinetic energy (uv_core), the sponge
-// and boundary-layer drag (uv_drag), the finite-volume pressure gradient
-// (uv_pgrad) and divergence damping.  The terms of the current step are
-// summed into one FP64 plane (DUDT, DVDT hold a single plane in this mode)
-// and rounded to FP32 once, when the step stores it in the history: in
-// balanced flow the v tendency is the small residual of the Coriolis and
-// pressure-gradient terms, which an FP32 running sum would lose.  Where FP32 arithmetic would difference two
-// full-magnitude FP64 values, the difference is taken in FP64 first:
-//   * the pressure gradient works on geopotential shifted by the value at the
-//     cell (GZ3 - GZ3(K,J,I), formed in FP64).  Every face integral of the
-//     reference's finite-volume scheme (src/shared/epic_timestep.c:909-1273)
-//     is invariant under a uniform shift of z (the t1 and zt*pb - zb*pt
-//     terms change by cancelling amounts, t2 and b1+b2-b3 not at all), so
-//     the shift changes nothing but the conditioning;
-//   * vertical differences of u and v are formed in FP64.
-// The FP64 kernels (k_dyn.hip) are untouched; this file is selected per
-// bucket when the engine runs in the mixed mode.
+// Adams-Bashforth history planes of DUDT, DVDT are FP32, and these tendency
+// terms are formed in FP32: Coriolis and kinetic energy (uv_core), the
+// sponge and drag (uv_drag) and divergence damping.  The finite-volume
+// pressure gradient is the FP64 kernel of k_dyn.hip in both modes: its face
+// integrals are products of geopotential-sized numbers whose difference is
+// the small force that balances Coriolis, and formed in FP32 (even on
+// geopotential shifted by the cell's own value) they left u of the hot
+// Jupiter at 1024x512x64 8.4e-4 (relative L2) from the FP64 run after 100
+// steps against 1.1e-5 in FP64, which costs no time (both kernels wait on
+// the same loads).  The terms of the current step are summed into one FP64
+// plane (DUDT, DVDT hold a single plane in this mode) and rounded to FP32
+// once, when the step stores it in the history: in balanced flow the v
+// tendency is the small residual of the Coriolis and pressure-gradient
+// terms, which an FP32 running sum would lose.  Vertical differences of u
+// and v are formed in FP64 before they are rounded.  This file is selected
+// per bucket when the engine runs in the mixed mode.
 #include "kernels.h"
 
 namespace epic {
@@ -241,147 +239,6 @@ __global__ void __launch_bounds__(256) km_hs_pbl(const MemberDesc *D, int mb) {
   }
 }
 
-// ------------------------------------------------------------- pgrad
-struct ZS {  // geopotential shifted by the cell's own value, formed in FP64
-  const MemberDesc *d;
-  double ref;
-  __device__ float operator()(int k, int j, int i) const { return (float)(d->GZ3[ix3(*d, k, j, i)] - ref); }
-};
-
-__device__ __forceinline__ float face_lon_f(const MemberDesc &d, const ZS &Z, int K, int J, int I, float &zbse,
-                                            float &zbne, float &ztse, float &ztne, float &zbe, float &zte, float &pbe,
-                                            float &pte) {
-  zbse = .25f * (Z(K, J, I) + Z(K, J - 1, I) + Z(K + 1, J, I) + Z(K + 1, J - 1, I));
-  zbne = .25f * (Z(K, J, I) + Z(K, J + 1, I) + Z(K + 1, J, I) + Z(K + 1, J + 1, I));
-  ztse = .25f * (Z(K, J, I) + Z(K, J - 1, I) + Z(K - 1, J, I) + Z(K - 1, J - 1, I));
-  ztne = .25f * (Z(K, J, I) + Z(K, J + 1, I) + Z(K - 1, J, I) + Z(K - 1, J + 1, I));
-  zbe = .5f * (Z(K, J, I) + Z(K + 1, J, I));
-  zte = .5f * (Z(K, J, I) + Z(K - 1, J, I));
-  pbe = (float)d.P2[ix3(d, K + 1, J, I)];
-  pte = (float)d.P2[ix3(d, K, J, I)];
-  float a1 = ztne + ztse, b1 = ztne * ztse, a2 = zbne + zbse, b2 = zbne * zbse;
-  float t1 = a1 * a1 - b1 - (a2 * a2 - b2), t2 = 3.f * (a1 - a2);
-  return (-t1 * (pbe - pte) + t2 * (zte * pbe - zbe * pte)) / (zte - zbe);
-}
-
-__device__ __forceinline__ float face_lat_f(const MemberDesc &d, const ZS &Z, int K, int J, int I, float rlnn,
-                                            float &zbnw, float &zbne, float &ztnw, float &ztne, float &zbn, float &ztn,
-                                            float &pbn, float &ptn) {
-  zbnw = .25f * (Z(K, J, I) + Z(K, J, I - 1) + Z(K + 1, J, I) + Z(K + 1, J, I - 1));
-  zbne = .25f * (Z(K, J, I) + Z(K, J, I + 1) + Z(K + 1, J, I) + Z(K + 1, J, I + 1));
-  ztnw = .25f * (Z(K, J, I) + Z(K, J, I - 1) + Z(K - 1, J, I) + Z(K - 1, J, I - 1));
-  ztne = .25f * (Z(K, J, I) + Z(K, J, I + 1) + Z(K - 1, J, I) + Z(K - 1, J, I + 1));
-  zbn = .5f * (Z(K, J, I) + Z(K + 1, J, I));
-  ztn = .5f * (Z(K, J, I) + Z(K - 1, J, I));
-  pbn = (float)d.P2[ix3(d, K + 1, J, I)];
-  ptn = (float)d.P2[ix3(d, K, J, I)];
-  float a1 = ztne + ztnw, b1 = ztne * ztnw, a2 = zbne + zbnw, b2 = zbne * zbnw;
-  float t1 = a1 * a1 - b1 - (a2 * a2 - b2), t2 = 3.f * (a1 - a2);
-  return rlnn * (-t1 * (pbn - ptn) + t2 * (ztn * pbn - zbn * ptn)) / (ztn - zbn);
-}
-
-__global__ void __launch_bounds__(256) km_pgrad(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(0, d.nj + 1, 1, d.ni);
-  const int K = 1 + blockIdx.y;
-  const float TOL = 1.e-2f;
-  const double DG = M_PI / 180.;
-  const double *P2 = d.P2;
-  double *du = curU(d), *dv = curV(d);
-  const ZS Z{&d, d.GZ3[ix3(d, K, J, I)]};
-#define PZ(k, j, i) ((float)P2[ix3(d, k, j, i)])
-  {
-    float rlns = (float)d.rln[2 * J], rlnn = (float)d.rln[2 * J + 2];
-    float zbsw, zbnw, ztsw, ztnw, zbw, ztw, pbw, ptw;
-    float west = face_lon_f(d, Z, K, J, I - 1, zbsw, zbnw, ztsw, ztnw, zbw, ztw, pbw, ptw);
-    float zbse, zbne, ztse, ztne, zbe, zte, pbe, pte;
-    float east = face_lon_f(d, Z, K, J, I, zbse, zbne, ztse, ztne, zbe, zte, pbe, pte);
-    float dln = (float)((d.lon[2 * I + 1] - d.lon[2 * (I - 1) + 1]) * DG);
-    float rho = .5f * (float)(d.RHO3[ix3(d, K, J, I)] + d.RHO3[ix3(d, K, J, I - 1)]);
-    float top, bottom, a1, a2, b1, b2, t1, t2;
-    a1 = ztne + ztse; a2 = ztnw + ztsw;
-    t2 = 3.f * (a1 - a2);
-    if (fabsf(2.f * (zte - ztw) / (zte + ztw - zbe - zbw)) > TOL) {
-      b1 = ztne * ztse; b2 = ztnw * ztsw;
-      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
-      top = (-t1 * (ptw - pte) + t2 * (zte * ptw - ztw * pte)) / (zte - ztw);
-    } else {
-      top = .5f * (ptw + pte) * t2;
-    }
-    a1 = zbne + zbse; a2 = zbnw + zbsw;
-    t2 = 3.f * (a1 - a2);
-    if (fabsf(2.f * (zbe - zbw) / (zte + ztw - zbe - zbw)) > TOL) {
-      b1 = zbne * zbse; b2 = zbnw * zbsw;
-      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
-      bottom = (-t1 * (pbw - pbe) + t2 * (zbe * pbw - zbw * pbe)) / (zbe - zbw);
-    } else {
-      bottom = .5f * (pbw + pbe) * t2;
-    }
-    float pgf = top - bottom + west - east;
-    float dzn = ztne - zbne + ztnw - zbnw, dzs = ztse - zbse + ztsw - zbsw;
-    pgf /= dln * rho * (rlns * (dzs + .5f * dzn) + rlnn * (dzn + .5f * dzs));
-    du[ix3(d, K, J, I)] += (double)pgf;
-  }
-  if (J >= 1) {
-    float rlns = (float)d.rln[2 * J - 1];
-    float zbsw, zbse, ztsw, ztse, zbs, zts, pbs, pts;
-    float south = face_lat_f(d, Z, K, J - 1, I, rlns, zbsw, zbse, ztsw, ztse, zbs, zts, pbs, pts);
-    float rlnn = (float)d.rln[2 * J + 1];
-    float zbnw, zbne, ztnw, ztne, zbn, ztn, pbn, ptn;
-    float north = face_lat_f(d, Z, K, J, I, rlnn, zbnw, zbne, ztnw, ztne, zbn, ztn, pbn, ptn);
-    float dlt = (float)((d.lat[2 * J + 1] - d.lat[2 * J - 1]) * DG);
-    float rho = .5f * (float)(d.RHO3[ix3(d, K, J, I)] + d.RHO3[ix3(d, K, J - 1, I)]);
-    float rlt = (float)d.rlt[2 * J];
-    float a1, a2, b1, b2, b3, t1, t2;
-    float zbe = .125f * (Z(K, J, I) + Z(K, J, I + 1) + Z(K, J - 1, I) + Z(K, J - 1, I + 1) + Z(K + 1, J, I) +
-                         Z(K + 1, J, I + 1) + Z(K + 1, J - 1, I) + Z(K + 1, J - 1, I + 1));
-    float zte = .125f * (Z(K - 1, J, I) + Z(K - 1, J, I + 1) + Z(K - 1, J - 1, I) + Z(K - 1, J - 1, I + 1) +
-                         Z(K, J, I) + Z(K, J, I + 1) + Z(K, J - 1, I) + Z(K, J - 1, I + 1));
-    float pbe = .25f * (PZ(K + 1, J, I) + PZ(K + 1, J, I + 1) + PZ(K + 1, J - 1, I) + PZ(K + 1, J - 1, I + 1));
-    float pte = .25f * (PZ(K, J, I) + PZ(K, J, I + 1) + PZ(K, J - 1, I) + PZ(K, J - 1, I + 1));
-    a1 = ztne + ztse; b1 = ztne * ztse; a2 = zbne + zbse; b2 = zbne * zbse;
-    t1 = a1 * a1 - b1 - (a2 * a2 - b2);
-    t2 = 3.f * (a1 - a2);
-    float east = .5f * (rlns - rlnn) * (-t1 * (pbe - pte) + t2 * (zte * pbe - zbe * pte)) / (zte - zbe);
-    float zbw = .125f * (Z(K, J, I) + Z(K, J, I - 1) + Z(K, J - 1, I) + Z(K, J - 1, I - 1) + Z(K + 1, J, I) +
-                         Z(K + 1, J, I - 1) + Z(K + 1, J - 1, I) + Z(K + 1, J - 1, I - 1));
-    float ztw = .125f * (Z(K - 1, J, I) + Z(K - 1, J, I - 1) + Z(K - 1, J - 1, I) + Z(K - 1, J - 1, I - 1) +
-                         Z(K, J, I) + Z(K, J, I - 1) + Z(K, J - 1, I) + Z(K, J - 1, I - 1));
-    float pbw = .25f * (PZ(K + 1, J, I) + PZ(K + 1, J, I - 1) + PZ(K + 1, J - 1, I) + PZ(K + 1, J - 1, I - 1));
-    float ptw = .25f * (PZ(K, J, I) + PZ(K, J, I - 1) + PZ(K, J - 1, I) + PZ(K, J - 1, I - 1));
-    a1 = ztnw + ztsw; b1 = ztnw * ztsw; a2 = zbnw + zbsw; b2 = zbnw * zbsw;
-    t1 = a1 * a1 - b1 - (a2 * a2 - b2);
-    t2 = 3.f * (a1 - a2);
-    float west = .5f * (rlns - rlnn) * (-t1 * (pbw - ptw) + t2 * (ztw * pbw - zbw * ptw)) / (ztw - zbw);
-    float top, bottom;
-    a1 = ztne + ztnw; a2 = ztse + ztsw;
-    t2 = 3.f * (a1 - a2);
-    if (fabsf(2.f * (ztn - zts) / (ztn + zts - zbn - zbs)) > TOL) {
-      b1 = ztne * ztnw; b2 = ztse * ztsw; b3 = ztne * ztse + ztnw * ztsw;
-      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
-      top = .5f * ((ptn - pts) * ((rlnn + rlns) * t1 + (rlnn - rlns) * (b1 + b2 - b3)) +
-                   (rlnn + rlns) * t2 * (ztn * pts - zts * ptn)) / (ztn - zts);
-    } else {
-      top = .25f * (rlnn + rlns) * (pts + ptn) * t2;
-    }
-    a1 = zbne + zbnw; a2 = zbse + zbsw;
-    t2 = 3.f * (a1 - a2);
-    if (fabsf(2.f * (zbn - zbs) / (ztn + zts - zbn - zbs)) > TOL) {
-      b1 = zbne * zbnw; b2 = zbse * zbsw; b3 = zbne * zbse + zbnw * zbsw;
-      t1 = a1 * a1 - b1 - (a2 * a2 - b2);
-      bottom = .5f * ((pbn - pbs) * ((rlnn + rlns) * t1 + (rlnn - rlns) * (b1 + b2 - b3)) +
-                      (rlnn + rlns) * t2 * (zbn * pbs - zbs * pbn)) / (zbn - zbs);
-    } else {
-      bottom = .25f * (rlnn + rlns) * (pbs + pbn) * t2;
-    }
-    float pgf = top - bottom + south - (north + west + east);
-    float dzn = ztne - zbne + ztnw - zbnw, dzs = ztse - zbse + ztsw - zbsw;
-    pgf /= dlt * rho * rlt * (rlns * (dzs + .5f * dzn) + rlnn * (dzn + .5f * dzs));
-    dv[ix3(d, K, J, I)] += (double)pgf;
-  }
-#undef PZ
-}
-
 // ------------------------------------------------------ divergence damping
 __global__ void __launch_bounds__(256) km_divdamp(const MemberDesc *D, int mb) {
   MEMBER;
@@ -451,9 +308,6 @@ void LM_uv_drag(hipStream_t s, const Bucket &b) {
   LAUNCH(km_sponge, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
   if (b.any_ray) LAUNCH(km_rayleigh, dim3(b.nj + 1, b.nk, b.nm), dim3(64), 0, s, b.D, b.mb);
   if (b.any_hs) LAUNCH(km_hs_pbl, grid2(b.nj + 1, b.ni, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
-}
-void LM_pgrad(hipStream_t s, const Bucket &b) {
-  LAUNCH(km_pgrad, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 void LM_divdamp_apply(hipStream_t s, const Bucket &b) {
   LAUNCH(km_divdamp, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);

@@ -93,7 +93,6 @@ void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs);
 void LM_zero_tend(hipStream_t s, const Bucket &b);
 void LM_uv_core(hipStream_t s, const Bucket &b);
 void LM_uv_drag(hipStream_t s, const Bucket &b);
-void LM_pgrad(hipStream_t s, const Bucket &b);
 void LM_divdamp_apply(hipStream_t s, const Bucket &b);
 void LM_ab(hipStream_t s, const Bucket &b);
 

@@ -1,13 +1,15 @@
-"""The mixed-precision mode (FP32 tendency arithmetic and history, FP64
-state, hydrostatics and Adams-Bashforth accumulation) against the FP64
-path, 1000 steps of the Jupiter-like 256x128x20 case (dt = 60 s).  Every
-field's relative L2 and max error is reported.  u, h, theta and p must stay
-within 1e-5 relative L2 of the FP64 result.  v is held to 1e-3: in these
-geostrophically balanced jets the v tendency is the ~1e-5 m/s^2 residual of
-Coriolis and pressure-gradient terms of ~1e-2 m/s^2 each, and forming
-those two terms in FP32 (relative error ~6e-8) leaves ~1e-4 of that
-residual, which v integrates.  The global mass and total-energy drifts must
-match the FP64 run's to 1e-6 relative."""
+"""The mixed-precision mode (FP32 Coriolis, kinetic energy, drag,
+divergence damping, advection fluxes, hyperviscosity and tendency history;
+FP64 state, pressure gradient, hydrostatics and Adams-Bashforth
+accumulation) against the FP64 path.  Every field's relative L2 and max
+error is reported.  After 1000 steps of the Jupiter-like 256x128x20 case
+(dt = 60 s, the reference's sponge) every field, u and v included, stays
+within 1e-5 relative L2 of the FP64 result, and the global mass and
+total-energy drifts match the FP64 run's to 1e-6.  The hot Jupiter starts
+at rest, so its u and v are nothing but the residual of a nearly
+hydrostatic pressure gradient and relative errors of h and theta of ~1e-8
+(the FP32 advection fluxes) reach them amplified: after 100 steps h, theta
+and p stay within 1e-5 and u, v within 3e-5."""
 import numpy as np
 import pytest
 
@@ -60,8 +62,8 @@ def test_mixed_against_fp64_jupiter_256x128x20(steps):
     dm_ref, dm_mix = (r1[0] - r0[0]) / r0[0], (x1[0] - x0[0]) / x0[0]
     de_ref, de_mix = (r1[1] - r0[1]) / r0[1], (x1[1] - x0[1]) / x0[1]
     say(f"mass drift fp64 {dm_ref:.3e} mixed {dm_mix:.3e}; energy drift fp64 {de_ref:.3e} mixed {de_mix:.3e}")
-    for f in FIELDS:
-        assert worst[f] <= (1e-3 if f == "V" else 1e-5), (f, worst[f])
+    for f in FIELDS + ("GZ3", "PV3"):
+        assert worst[f] <= 1e-5, (f, worst[f])
     assert abs(dm_mix - dm_ref) <= 1e-6 and abs(de_mix - de_ref) <= 1e-6
 
 
@@ -87,7 +89,7 @@ def test_mixed_against_fp64_hot_jupiter(ni, nj, nk):
     """The hot Jupiter in the mixed mode (at 1024x512x64: rows of 1024 points,
     the FP32 row filters with one row per block, the widest FP32 register
     layout of the zonal hyperviscosity), 100 steps against FP64."""
-    sp = ea.spec(ea.HOT_JUPITER, ni, nj, nk, 30, seed=1, nu4=0.5, noise_theta=1.0, noise_uv=0.0)
+    sp = ea.spec(ea.HOT_JUPITER, ni, nj, nk, 30, seed=1, nu4=0.5, noise_theta=1.0, noise_uv=0.0, k_sponge=0)
     ref, _, _ = run(sp, ea.FP64, 100)
     mix, _, _ = run(sp, ea.MIXED, 100)
     worst = {}
@@ -98,3 +100,5 @@ def test_mixed_against_fp64_hot_jupiter(ni, nj, nk):
         say(f"mixed vs fp64 hot Jupiter {ni}x{nj}x{nk}, 100 steps {f}: rel L2 {worst[f]:.2e}")
     for f in ("HDRY", "THETA", "P3"):
         assert worst[f] <= 1e-5, (f, worst[f])
+    if ni == 1024:
+        assert worst["U"] <= 3e-5 and worst["V"] <= 3e-5, (worst["U"], worst["V"])
