@@ -38,6 +38,8 @@ struct MemberDesc {
   double dt, kappa, cp, cpr, rgas, press0, zeta0, zeta1, alpha, sigma_sigma;
   double topo_scale, prandtl, tau_drag;
   double coef2, coef4, nu4, m0z, nudiv, eps13, mu3, mu2, g_coeff;
+  // del^6 and del^8 hyperviscosity: factored coefficients and nu
+  double coef6, coef8, nu6, nu8;
   // day-night relaxation of the hot-Jupiter case (is_hj)
   int is_hj;
   double pbot, teq_contrast, teq_p_contrast, tau_rad, ref_theta_b, ref_pbk, ref_a_over_kappa;
@@ -79,6 +81,20 @@ __device__ __forceinline__ size_t ix2(const MemberDesc &d, int J, int I) {
 }
 
 __device__ __forceinline__ double dmax(double x, double y) { return x > y ? x : y; }
+
+// Hyperviscosity orders (src/shared/epic_subgrid.c:590-1006): pass p forms
+// the order 2(p+1) from the order before it with the factored coefficient
+// coef[2(p+1)], and is applied when nu[2(p+1)] > 0; the passes run up to
+// the highest order with nu > 0 (0 passes: no hyperviscosity).
+__host__ __device__ __forceinline__ double hv_coef(const MemberDesc &d, int pass) {
+  return pass == 0 ? d.coef2 : pass == 1 ? d.coef4 : pass == 2 ? d.coef6 : d.coef8;
+}
+__host__ __device__ __forceinline__ double hv_nu(const MemberDesc &d, int pass) {
+  return pass == 1 ? d.nu4 : pass == 2 ? d.nu6 : pass == 3 ? d.nu8 : 0.;
+}
+__host__ __device__ __forceinline__ int hv_passes(const MemberDesc &d) {
+  return d.nu8 > 0. ? 4 : d.nu6 > 0. ? 3 : d.nu4 > 0. ? 2 : 0;
+}
 __device__ __forceinline__ double dmin(double x, double y) { return x < y ? x : y; }
 
 // fcmp (src/shared/epic_funcs_util.c:4275-4346): compare to within one ulp of

@@ -144,7 +144,6 @@ void check_config(const struct epic_planet_abi *planet) {
        "only the Hsu-Konor-Arakawa advection scheme is implemented");
   need(var.p3.value == var.pdry3.value && var.h3.value == var.hdry3.value,
        "dry runs alias p3/pdry3 and h3/hdry3");
-  need(grid.nu[6] == 0. && grid.nu[8] == 0., "only del^4 hyperviscosity (nu[4]) is implemented");
   need(strcmp(planet->type, "gas-giant") == 0 || strcmp(planet->type, "terrestrial") == 0,
        "planet->type must be gas-giant or terrestrial");
   need(var.u.value && var.v.value && var.u.tendency && var.v.tendency && var.hdry.value && var.theta.value,

@@ -175,8 +175,10 @@ static int rebuild_buckets(epic_engine *E) {
     b.nk = hd[i].nk;
     b.mixed = (E->precision == EPIC_MIXED);
     b.any_gas = b.any_hs = b.any_ray = 0;
+    b.hv_passes = 0;
     for (size_t k = i; k < j; k++) {
       const MemberDesc &h = hd[k];
+      b.hv_passes = std::max(b.hv_passes, hv_passes(h));
       b.any_gas |= (h.is_gas != 0);
       b.any_hs |= (h.is_hs != 0);
       b.any_ray |= ((h.newt_cool_on && h.prandtl > 0.) || (h.tau_drag > 0. && h.tau_drag < 1.e+20));
@@ -267,7 +269,7 @@ static void bucket_flags(epic_engine *E, const Bucket &b, bool &hv, bool &divd, 
   hv = divd = adj = false;
   for (int i = 0; i < b.nm; i++) {
     const Member &x = E->m[E->order[b.mb + i]];
-    hv |= x.nu4 > 0.;
+    hv |= hv_passes(x.h) > 0;
     divd |= x.nudiv != 0.;
     adj |= (x.adjust != 0);
   }
@@ -346,8 +348,8 @@ int member_check(const epic_case *c, int precision, char *err, size_t cap) {
              "is not implemented", c->planet_name);
     return -1;
   }
-  if (c->nu[6] != 0. || c->nu[8] != 0.) {
-    snprintf(err, cap, "only del^4 hyperviscosity is implemented");
+  if (c->nu[4] < 0. || c->nu[6] < 0. || c->nu[8] < 0.) {
+    snprintf(err, cap, "negative hyperviscosity coefficient");
     return -1;
   }
   if (c->k_sponge < 0 || c->k_sponge >= c->nk) {
@@ -510,6 +512,7 @@ void member_plan(const epic_case *c, bool mixed, const epic_tables &T, MemberDes
   d.press0 = c->press0; d.zeta0 = c->zeta0; d.zeta1 = c->zeta1; d.alpha = c->hybrid_alpha;
   d.sigma_sigma = c->sigma_sigma; d.topo_scale = c->topo_scale; d.prandtl = c->prandtl; d.tau_drag = c->tau_drag;
   d.coef2 = T.coef2; d.coef4 = T.coef4; d.nu4 = c->nu[4]; d.m0z = T.m0z; d.nudiv = T.nudiv; d.eps13 = T.eps13;
+  d.coef6 = T.coef6; d.coef8 = T.coef8; d.nu6 = c->nu[6]; d.nu8 = c->nu[8];
   d.mu3 = T.mu3; d.mu2 = T.mu2; d.g_coeff = T.gcoef;
 }
 

@@ -148,9 +148,17 @@ int epic_tables_build(const epic_case *c, epic_tables *T) {
   /* viscosity coefficients (nu[2] is forced to zero by the initialisers) */
   double dt = (double)c->dt, dy2 = c->dy0 * c->dy0;
   double mx2 = (dy2 / 30.) / dt, mx4 = (dy2 / 240.) * (dy2 / dt);
+  double mx6 = (dy2 / 800.) * (dy2 / dt) * dy2, mx8 = (dy2 / 2400.) * (dy2 / dt) * dy2 * dy2;
   T->coef2 = 0.01 * mx2;
   T->coef4 = (c->nu[4] > 0.) ? c->nu[4] : 0.01 * mx4;
   T->coef4 /= T->coef2;
+  T->coef6 = (c->nu[6] > 0.) ? c->nu[6] : 0.01 * mx6;
+  T->coef6 /= T->coef2;
+  T->coef6 /= T->coef4;
+  T->coef8 = (c->nu[8] > 0.) ? c->nu[8] : 0.01 * mx8;
+  T->coef8 /= T->coef2;
+  T->coef8 /= T->coef4;
+  T->coef8 /= T->coef6;
   double rlnz = c->re / sqrt(1. + pow(c->rp / c->re * tan(LAT0 * DEGR), 2.));
   T->m0z = 1. / (rlnz * c->dln * DEGR);
   T->nudiv = c->nudiv_nondim * mx2;

@@ -669,23 +669,29 @@ __global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, in
   }
 }
 
-// Rows of the del^4 result D (copy i of sB, 1024/ni rows per block): D is
-// zonally filtered in LDS (rows of the filter band), then A += DT*(-1)*D on
-// the kind's rows and the zonal pads of every row 0..nj+1 are written
+// Rows of the order-2(pass+1) result D (copy i of sB, or of sC when src is
+// 1; 1024/ni rows per block), for a member that applies that order
+// (nu > 0): D is zonally filtered in LDS (rows of the filter band), then
+// A += DT*sign*D on the kind's rows (sign -1 for del^4 and del^8, +1 for
+// del^6) and the zonal pads of every row 0..nj+1 are written; when a higher
+// order follows, the filtered D goes back to its copy, which the next
+// order's meridional second difference reads (the reference filters it in
+// place, src/shared/epic_subgrid.c:960-975).  Members that do not apply
+// the order leave D and A as they are.
 template <class T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_hv_apply(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_hv_apply(const MemberDesc *D, int mb, int nm, FieldSet fs, int pass, int src) {
   SET_MEMBER;
   extern __shared__ double lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
   const RowMap m = row_map(d.ni);
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y, Jr = blockIdx.x * m.rpb + m.g;
-  if (K > r.k1 || d.nu4 == 0.) return;  // uniform per block
+  if (K > r.k1 || hv_nu(d, pass) == 0. || hv_passes(d) <= pass) return;  // uniform per block
   const bool live = Jr <= d.nj + 1;
   const int J = live ? Jr : 0, ni = d.ni, vg = (kind == 1);
   T *x = lds + (size_t)m.g * 2 * ni, *work = x + ni;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
-  const T *dd = reinterpret_cast<const T *>(d.sB) + (size_t)fi * d.N3 + (size_t)K * d.N2;
+  T *dd = reinterpret_cast<T *>(src ? d.sC : d.sB) + (size_t)fi * d.N3 + (size_t)K * d.N2;
   const bool upd = live && J >= r.jbot && J <= d.nj;
   if (upd)
     for (int I = 1 + m.lane; I <= ni; I += m.stride) x[I - 1] = dd[ix2(d, J, I)];
@@ -693,15 +699,45 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
   filter_row_lds(d, x, work, J, vg, upd && (vg ? d.rflag_v[J] : d.rflag_h[J]) != 0, m.lane);
   __syncthreads();
   if (!live) return;
+  const bool back = upd && hv_passes(d) > pass + 1;
+  const double sign = (pass & 1) ? -1. : 1.;
   for (int I = 1 + m.lane; I <= ni; I += m.stride) {
     double v = a[ix2(d, J, I)];
     if (upd) {
-      v += d.dt * (double)(-1) * (double)x[I - 1];
+      v += d.dt * sign * (double)x[I - 1];
       a[ix2(d, J, I)] = v;
     }
+    if (back) dd[ix2(d, J, I)] = x[I - 1];
     if (I == d.ni) a[ix2(d, J, 0)] = v;
     if (I == 1) a[ix2(d, J, d.ni + 1)] = v;
   }
+}
+
+// One meridional second difference of the hyperviscosity chain past del^4:
+// order 2(pass+1) = coef * d2dy2(order 2 pass) from copy i of sB (src 0) or
+// sC (src 1) into the other, for members that run that pass
+template <class T>
+__global__ void __launch_bounds__(256) k_d2dy2_pass(const MemberDesc *D, int mb, int nm, FieldSet fs, int pass, int src) {
+  SET_MEMBER;
+  const int vg = (kind == 1);
+  const int jmax = vg ? d.nj + 1 : d.nj;
+  HvRange r = hv_range(d, kind);
+  const int K = r.k0 + blockIdx.y;
+  if (K > r.k1 || hv_passes(d) <= pass) return;  // uniform per block
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (jmax + 1) * d.ni) return;
+  const int J = t / d.ni, I = 1 + t - J * d.ni;
+  const size_t o = (size_t)fi * d.N3 + (size_t)K * d.N2;
+  const T *hh = reinterpret_cast<const T *>(src ? d.sC : d.sB) + o;
+  T *out = reinterpret_cast<T *>(src ? d.sB : d.sC) + o;
+  const double *cx = vg ? d.vcx : d.hcx, *cy = vg ? d.vcy : d.hcy, *cz = vg ? d.vcz : d.hcz;
+  int j0, j1, j2;
+  if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
+  else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
+  else { j0 = J - 1; j1 = J; j2 = J + 1; }
+  const T c = T(hv_coef(d, pass));
+  const T a0 = c * T(cx[J]), a1 = c * T(cy[J]), a2 = c * T(cz[J]);
+  out[ix2(d, J, I)] = a0 * hh[ix2(d, j0, I)] + a1 * hh[ix2(d, j1, I)] + a2 * hh[ix2(d, j2, I)];
 }
 
 // the mixed mode forms del^4 in FP32 (the field and its update stay FP64)
@@ -716,8 +752,15 @@ static void merid_hv_t(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   dim3 g((jrows + D2_TJ - 1) / D2_TJ * (b.ni / CW), b.nk, fs.n * b.nm);
   LAUNCH(k_d2dy2x2<T>, g, dim3(256), sizeof(T) * (2 * D2_TJ + 10) * CW, s, b.D, b.mb, b.nm, fs, CW);
   const int rpb = rows_per_block(b.ni);
-  LAUNCH(k_hv_apply<T>, dim3((b.nj + 2 + rpb - 1) / rpb, b.nk, fs.n * b.nm), dim3(256), sizeof(T) * rpb * 2 * b.ni, s,
-         b.D, b.mb, b.nm, fs);
+  const dim3 ga((b.nj + 2 + rpb - 1) / rpb, b.nk, fs.n * b.nm);
+  LAUNCH(k_hv_apply<T>, ga, dim3(256), sizeof(T) * rpb * 2 * b.ni, s, b.D, b.mb, b.nm, fs, 1, 0);
+  // del^6 (pass 2, into sC) and del^8 (pass 3, back into sB)
+  const dim3 gp((jrows * b.ni + TPB - 1) / TPB, b.nk, fs.n * b.nm);
+  for (int pass = 2; pass < b.hv_passes; pass++) {
+    const int src = (pass == 2) ? 0 : 1;
+    LAUNCH(k_d2dy2_pass<T>, gp, dim3(TPB), 0, s, b.D, b.mb, b.nm, fs, pass, src);
+    LAUNCH(k_hv_apply<T>, ga, dim3(256), sizeof(T) * rpb * 2 * b.ni, s, b.D, b.mb, b.nm, fs, pass, 1 - src);
+  }
 }
 void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   if (b.mixed) merid_hv_t<float>(s, b, fs);
@@ -743,7 +786,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y;
-  if (K > r.k1 || d.nu4 == 0.) return;  // uniform over the block
+  const int np = hv_passes(d);
+  if (K > r.k1 || np == 0) return;  // uniform over the block
   T *meanv = X + (size_t)RB * WS;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
   const int J0 = blockIdx.x * RB, jlast = d.nj + 1;
@@ -752,25 +796,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     T *x = X + t * WS;
     for (int I = lane; I < W; I += 64) x[I] = (J <= jlast) ? T(a[ix2(d, J, I)]) : T(0.);
   }
-  T dc[2][RW];
+  // m (or m0 poleward of LAT0) of each row; pass p's coefficient of the
+  // row is (coef[2(p+1)] mm) mm, as the reference forms it
+  double mm[RW];
 #pragma unroll
   for (int u = 0; u < RW; u++) {
     const int Jc = min(J0 + wv + 4 * u, d.nj);  // rows past nj are never applied
-    const double m = d.m[2 * Jc + r.jh];
     const bool taper = fabs(d.lat[2 * Jc + r.jh]) > 45.;
-    dc[0][u] = T(taper ? d.coef2 * d.m0z * d.m0z : d.coef2 * m * m);
-    dc[1][u] = T(taper ? d.coef4 * d.m0z * d.m0z : d.coef4 * m * m);
+    mm[u] = taper ? d.m0z : d.m[2 * Jc + r.jh];
   }
   __syncthreads();
   T y[RW][NQ];
-  for (int pass = 0; pass < 2; pass++) {
+  for (int pass = 0; pass < np; pass++) {
+    const double cp = hv_coef(d, pass);
 #pragma unroll
     for (int u = 0; u < RW; u++) {
       const T *x = X + (wv + 4 * u) * WS;
+      const T dcu = T(cp * mm[u] * mm[u]);
 #pragma unroll
       for (int q = 0; q < NQ; q++) {
         const int I = 1 + lane + 64 * q;
-        if (I <= ni) y[u][q] = dc[pass][u] * (x[I - 1] - T(2.) * x[I] + x[I + 1]);
+        if (I <= ni) y[u][q] = dcu * (x[I - 1] - T(2.) * x[I] + x[I + 1]);
       }
     }
     __syncthreads();
@@ -787,7 +833,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     if (wv == 0 && lane < RB) meanv[lane] = seq_acc(X + lane * WS + 1, ni) / T(ni);
     __syncthreads();
     // every point (and the zonal pads, copies of points ni and 1) less the
-    // mean; the second pass's result stays in registers
+    // mean; the last pass's result stays in registers
 #pragma unroll
     for (int u = 0; u < RW; u++) {
       const int t = wv + 4 * u;
@@ -798,7 +844,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const int I = 1 + lane + 64 * q;
         if (I <= ni) {
           y[u][q] -= fm;
-          if (pass == 0) {
+          if (pass < np - 1) {
             x[I] = y[u][q];
             if (I == ni) x[0] = y[u][q];
             if (I == 1) x[ni + 1] = y[u][q];
@@ -807,22 +853,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       }
     }
     __syncthreads();
-  }
-  // apply with sign -1 on the rows of the range, then pads
+    if (hv_nu(d, pass) == 0.) continue;
+    // an order with nu > 0: A += DT*sign*D on the rows of the range (sign
+    // -1 for del^4 and del^8, +1 for del^6), then pads
+    const double sign = (pass & 1) ? -1. : 1.;
 #pragma unroll
-  for (int u = 0; u < RW; u++) {
-    const int J = J0 + wv + 4 * u;
-    if (J > jlast) break;
-    const bool upd = J >= r.jbot && J <= d.nj;
+    for (int u = 0; u < RW; u++) {
+      const int J = J0 + wv + 4 * u;
+      if (J > jlast) break;
+      const bool upd = J >= r.jbot && J <= d.nj;
 #pragma unroll
-    for (int q = 0; q < NQ; q++) {
-      const int I = 1 + lane + 64 * q;
-      if (I > ni) break;
-      double v = a[ix2(d, J, I)];
-      if (upd) v += d.dt * (double)(-1) * (double)y[u][q];
-      a[ix2(d, J, I)] = v;
-      if (I == ni) a[ix2(d, J, 0)] = v;
-      if (I == 1) a[ix2(d, J, ni + 1)] = v;
+      for (int q = 0; q < NQ; q++) {
+        const int I = 1 + lane + 64 * q;
+        if (I > ni) break;
+        double v = a[ix2(d, J, I)];
+        if (upd) v += d.dt * sign * (double)y[u][q];
+        a[ix2(d, J, I)] = v;
+        if (I == ni) a[ix2(d, J, 0)] = v;
+        if (I == 1) a[ix2(d, J, ni + 1)] = v;
+      }
     }
   }
 }

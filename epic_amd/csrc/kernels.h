@@ -46,6 +46,8 @@ struct Bucket {
   // drag), or has Rayleigh drag: launches none of the members needs are
   // left out (their kernels would return at once for every member)
   int any_gas, any_hs, any_ray;
+  // hyperviscosity passes of the member that needs the most (hv_passes)
+  int hv_passes;
 };
 
 

@@ -40,6 +40,9 @@ CASES = {
     # Held-Suarez with the layer-mean heating removal and prandtl/t_rad drag
     "hs_adjust_tiny": (0, 16, 8, 6, 600, 8, 0.3, 0.2, 0.5, 0.5, [0, 1, 10],
                        {"newt_cool_adjust": 1, "prandtl": 0.3, "drag_v": 1}),
+    # the reference `initial` program's hyperviscosity default: del^8 alone
+    "jup_del8_tiny": (1, 16, 8, 6, 60, 9, 0.0, 0.0, 0.0, 0.5, [0, 1, 10],
+                      {"k_sponge": -1, "nu8_nondim": 0.5}),
 }
 
 

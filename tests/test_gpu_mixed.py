@@ -102,3 +102,15 @@ def test_mixed_against_fp64_hot_jupiter(ni, nj, nk):
         assert worst[f] <= 1e-5, (f, worst[f])
     if ni == 1024:
         assert worst["U"] <= 3e-5 and worst["V"] <= 3e-5, (worst["U"], worst["V"])
+
+
+def test_mixed_with_del8_hyperviscosity():
+    """The reference `initial` program's hyperviscosity (del^8 alone) in the
+    mixed mode: the FP32 passes of the chain stay within 1e-5 of FP64."""
+    sp = ea.spec(ea.JUPITER_LIKE, 64, 32, 10, 60, 4, 0.0, 0.0, 0.0, 0.5, nu8=0.5)
+    ref, _, _ = run(sp, ea.FP64, 200)
+    mix, _, _ = run(sp, ea.MIXED, 200)
+    for f in FIELDS:
+        e = float(np.linalg.norm(interior(mix[f], sp) - interior(ref[f], sp)) / np.linalg.norm(interior(ref[f], sp)))
+        say(f"mixed vs fp64, del^8, 200 steps {f}: rel L2 {e:.2e}")
+        assert e <= 1e-5, (f, e)

@@ -34,6 +34,10 @@ CASES = {
                                          newt_cool_adjust=1, prandtl=0.5, drag_v=1),
     "hot_jupiter_sponge_prandtl_32x16x8": ea.spec(ea.HOT_JUPITER, 32, 16, 8, 30, 4, 0.5, 0.0, 1.0, 0.0,
                                                   prandtl=0.3, drag_zonal_avg=1),
+    # higher-order hyperviscosity: del^8 alone (the reference `initial`
+    # program's default), and del^4 + del^6 + del^8 on both grids
+    "jupiter_del8_32x16x10": ea.spec(ea.JUPITER_LIKE, 32, 16, 10, 60, 23, 0.0, 0.0, 0.0, 1.0, nu8=0.5),
+    "hs_del468_32x16x8": ea.spec(ea.HELD_SUAREZ, 32, 16, 8, 600, 25, 0.3, 0.1, 0.3, 0.4, nu6=0.2, nu8=0.5),
 }
 
 
@@ -89,12 +93,16 @@ def test_jupiter_256x128x20_full_size():
         p.close()
 
 
-@pytest.mark.parametrize("ni,nj,nk,dt", [(512, 256, 5, 60), (1024, 256, 4, 30)])
-def test_wide_rows(ni, nj, nk, dt):
+@pytest.mark.parametrize("ni,nj,nk,dt,nu4,nu8", [(512, 256, 5, 60, 0.5, 0.0), (1024, 256, 4, 30, 0.5, 0.0),
+                                                 (1024, 512, 4, 30, 0.0, 0.5)])
+def test_wide_rows(ni, nj, nk, dt, nu4, nu8):
     """Rows of 512 and 1024 points (Jupiter-like): one row per block in the
-    row filters, the widest register layouts of the zonal hyperviscosity,
-    64-column tiles in the meridional passes; 3 steps against the oracle."""
-    sp = ea.spec(ea.JUPITER_LIKE, ni, nj, nk, dt, 11, 0.5, 0.0, 0.5, 0.5)
+    row filters, the widest register layouts of the zonal hyperviscosity
+    (there with all four passes of del^8 as well, on a grid with dx ~ dy:
+    the reference's del^8 limit assumes it, and 1024 x 256 blows up in the
+    oracle too), 64-column tiles in the meridional passes; 3 steps against
+    the oracle."""
+    sp = ea.spec(ea.JUPITER_LIKE, ni, nj, nk, dt, 11, nu4, 0.0, 0.5, 0.5, nu8=nu8)
     p = Pair(sp)
     try:
         p.step(3)
@@ -103,7 +111,7 @@ def test_wide_rows(ni, nj, nk, dt):
         p.close()
 
 
-@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny", "jup_drag_tiny", "hs_adjust_tiny"])
+@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny", "jup_drag_tiny", "hs_adjust_tiny", "jup_del8_tiny"])
 def test_against_reference_fixtures(name):
     """The GPU path against arrays written by the reference's own build."""
     import json

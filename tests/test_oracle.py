@@ -32,7 +32,7 @@ def same_bits(a, b):
     return a.tobytes() == b.tobytes()
 
 
-@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny", "jup_drag_tiny", "hs_adjust_tiny"])
+@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny", "jup_drag_tiny", "hs_adjust_tiny", "jup_del8_tiny"])
 def test_oracle_matches_golden_fields(name):
     z = np.load(os.path.join(GOLD, f"{name}.npz"))
     r = cr.CpuRef(spec_of(name))
@@ -93,6 +93,12 @@ REF_CASES = [
     ((0, 32, 16, 8, 600, 20, 0.5, 0.0, 0.3, 0.4), {"newt_cool_adjust": 1}),
     ((0, 32, 16, 8, 600, 21, 0.5, 0.1, 0.3, 0.4), {"prandtl": 0.5, "drag_v": 1}),
     ((0, 32, 16, 8, 600, 22, 0.5, 0.0, 0.3, 0.4), {"prandtl": 0.2, "drag_zonal_avg": 1, "newt_cool_adjust": 1}),
+    # higher-order hyperviscosity: del^8 alone (the reference `initial`
+    # program's default, nu_nondim[8] = 0.5), del^6 alone, and del^4, del^6
+    # and del^8 together (each order filtered and applied in turn)
+    ((1, 32, 16, 10, 60, 23, 0.0, 0.0, 0.0, 1.0), {"k_sponge": -1, "nu8_nondim": 0.5}),
+    ((3, 16, 12, 6, 600, 24, 0.0, 0.1, 0.2, 0.5), {"k_sponge": -1, "nu6_nondim": 0.4}),
+    ((0, 32, 16, 8, 600, 25, 0.3, 0.0, 0.3, 0.4), {"nu6_nondim": 0.2, "nu8_nondim": 0.5}),
 ]
 
 
