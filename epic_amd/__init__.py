@@ -89,7 +89,9 @@ class Case(ctypes.Structure):
                 [(n, _I) for n in ("Iadim", "Jadim", "Kadim", "Nelem2d", "Nelem3d")] +
                 [(n, _P) for n in ("lon", "lat", "rln", "rlt", "f", "m", "n", "mn", "g", "sigmatheta", "dsgth",
                                    "dsgth_inv", "p_ref", "theta_ref", "h_min", "t_sponge_inv", "u", "v", "hdry",
-                                   "theta", "p3", "u_spinup", "gz_surface")])
+                                   "theta", "p3", "u_spinup", "gz_surface")] +
+                [("ntp", _I), ("n_t_cool", _I)] +
+                [(n, _P) for n in ("pdat", "tdat", "t_cool_x", "t_cool_y")])
 
     def array(self, name, n):
         """A numpy copy of the first n values of one of the case's arrays."""

@@ -47,6 +47,10 @@ struct MemberDesc {
   // sum of the cell areas 1/mn over J = 0..nj, I = 1..ni in that order (the
   // layer-mean removal of the Newtonian relaxation)
   double heat_area;
+  // Newtonian cooling toward the sounding (is_tab): PCHIP tables against
+  // -log p of the ironed sounding temperature and of the cooling time
+  int is_tab, n_snd, n_cool;
+  const double *snd_x, *snd_y, *snd_d, *cool_x, *cool_y, *cool_d;
   // host-libm tables: cos(lat), cos(lat)^4 per h-row J, cos(lon) per column I
   const double *coslat_h, *coslat4_h, *coslon_h;
   // per layer K >= k_sigma: THETA2's interval in the sigmatheta table and its width
@@ -190,6 +194,52 @@ __device__ __forceinline__ int table_place(int n, const double *x, double xx, do
   }
   *dx = x[il + 1] - x[il];
   return il;
+}
+
+// the cubic of a PCHIP table on the interval k that table_place found, h its
+// width (splint_pchip, src/shared/epic_funcs_util.c:926-999)
+__device__ __forceinline__ double pchip_at(const double *x, const double *y, const double *dd, int k, double h,
+                                           double xx) {
+  const double delta = (y[k + 1] - y[k]) / h;
+  const double e1 = (dd[k] - delta) / h, e2 = (dd[k + 1] - delta) / h;
+  const double c3 = (e1 + e2) / h, c2 = -(e1 + e1 + e2);
+  const double t = xx - x[k];
+  return y[k] + t * (dd[k] + t * (c2 + t * c3));
+}
+
+// Newtonian cooling time at -log p = x from the table (t_rad,
+// src/shared/epic_sensible_heating.c:294-311): the end values past the ends
+__device__ __forceinline__ double tab_cool_time(const MemberDesc &d, double x) {
+  const int n = d.n_cool;
+  if (x >= d.cool_x[n - 1]) return d.cool_y[n - 1];
+  if (x <= d.cool_x[0]) return d.cool_y[0];
+  double h;
+  const int k = table_place(n, d.cool_x, x, &h);
+  return pchip_at(d.cool_x, d.cool_y, d.cool_d, k, h, x);
+}
+// the sounding's temperature at -log p = x (get_sounding,
+// src/shared/epic_funcs_diag.c:3258-3290): its bottom value below the table
+__device__ __forceinline__ double tab_sounding_t(const MemberDesc &d, double x) {
+  if (x < d.snd_x[0]) return d.snd_y[0];
+  double h;
+  const int k = table_place(d.n_snd, d.snd_x, x, &h);
+  return pchip_at(d.snd_x, d.snd_y, d.snd_d, k, h, x);
+}
+// the relaxation toward the sounding, -(cp / t_cool)(T - T_sounding), cp the
+// centred derivative of the ideal-gas enthalpy cpr T rgas (return_cp,
+// src/shared/epic_funcs_diag.c:3311-3337;
+// src/shared/epic_sensible_heating.c:103-130)
+__device__ __forceinline__ double tab_relax(const MemberDesc &d, double p, double t) {
+  const double x = -crm_log(p), dT = t * 1.e-6;
+  double h2 = d.cpr * (t + dT), h1 = d.cpr * (t - dT);
+  h2 *= d.rgas;
+  h1 *= d.rgas;
+  const double cp_over_time = ((h2 - h1) / (2. * dT)) / tab_cool_time(d, x);
+  return -cp_over_time * (t - tab_sounding_t(d, x));
+}
+// whether some Newtonian relaxation forms HEAT3
+__device__ __forceinline__ bool newt_relax_on(const MemberDesc &d) {
+  return d.newt_cool_on && (d.is_hs || d.is_hj || d.is_tab);
 }
 
 // table_place for a table whose neighbouring entries differ by more than two

@@ -149,14 +149,18 @@ void check_config(const struct epic_planet_abi *planet) {
   need(var.u.value && var.v.value && var.u.tendency && var.v.tendency && var.hdry.value && var.theta.value,
        "prognostic arrays are not allocated");
   need(planet->x_h2 == 0., "the ortho/para hydrogen thermodynamics (planet->x_h2 > 0) are not implemented");
-  // calc_heating: Held-Suarez's analytic relaxation is the only one of the
-  // reference's that is built here.  Every other planet, a hot Jupiter
-  // included, relaxes toward its t_vs_p sounding with t_cool_table's times
-  // (src/shared/epic_sensible_heating.c:114-130, 294-311); the engine's own
-  // day-night relaxation is not the reference's and is not offered here.
-  need(!grid.newt_cool_on || strcmp(planet->name, "held_suarez") == 0,
-       "grid.newt_cool_on: Newtonian cooling from t_cool_table and the t_vs_p sounding is not implemented "
-       "(only held_suarez)");
+  // calc_heating: Held-Suarez's analytic relaxation, or, on every other
+  // planet (a hot Jupiter included), the relaxation toward the t_vs_p
+  // sounding with t_cool_table's times (src/shared/epic_sensible_heating.c:
+  // 103-130, 294-311); Venus (LLR05) and Titan take their temp_eq instead,
+  // which is not built here.  The engine's own day-night relaxation is not
+  // the reference's and is not offered here.
+  if (grid.newt_cool_on && strcmp(planet->name, "held_suarez") != 0) {
+    need(planet->index != 20 && planet->index != 10,
+         "grid.newt_cool_on: the temp_eq relaxations of venus_llr05 and titan are not implemented");
+    need(var.ntp >= 2 && var.pdat && var.tdat && var.n_t_cool >= 2 && var.t_cool_table,
+         "grid.newt_cool_on needs the t_vs_p sounding (var.ntp, var.pdat, var.tdat) and var.t_cool_table");
+  }
 }
 
 void create_member(const struct epic_planet_abi *planet) {
@@ -192,6 +196,17 @@ void create_member(const struct epic_planet_abi *planet) {
   c.u = var.u.value; c.v = var.v.value; c.hdry = var.hdry.value; c.theta = var.theta.value;
   c.p3 = var.p3.value; c.u_spinup = var.u_spinup.value; c.gz_surface = var.gz_surface.value;
   need(c.u_spinup && c.gz_surface && c.p3, "u_spinup, gz_surface and p3 must be allocated");
+  // the sounding and the cooling-time table ({x, y, z} triplets, x = -log p)
+  std::vector<double> tcx, tcy;
+  if (c.newt_cool_on && strcmp(planet->name, "held_suarez") != 0) {
+    c.ntp = var.ntp; c.pdat = var.pdat; c.tdat = var.tdat;
+    const double *trip = (const double *)var.t_cool_table;
+    for (int q = 0; q < var.n_t_cool; q++) {
+      tcx.push_back(trip[3 * q]);
+      tcy.push_back(trip[3 * q + 1]);
+    }
+    c.n_t_cool = var.n_t_cool; c.t_cool_x = tcx.data(); c.t_cool_y = tcy.data();
+  }
 
   if (!g_di.E) {
     g_di.E = epic_engine_create(EPIC_FP64, 1, 0);

@@ -343,10 +343,20 @@ int member_check(const epic_case *c, int precision, char *err, size_t cap) {
     snprintf(err, cap, "tie-point row jtp=%d outside 0..%d", c->jtp, c->nj);
     return -1;
   }
-  if (c->newt_cool_on && strcmp(c->planet_name, "held_suarez") != 0 && strcmp(c->planet_name, "hot_jupiter") != 0) {
-    snprintf(err, cap, "Newtonian cooling from t_cool_table and the t_vs_p sounding (planet %s) "
-             "is not implemented", c->planet_name);
-    return -1;
+  // Newtonian cooling on any planet but Held-Suarez relaxes toward the
+  // sounding with the tabulated cooling time; the hot Jupiter without a
+  // sounding has its own day-night relaxation
+  if (c->newt_cool_on && strcmp(c->planet_name, "held_suarez") != 0 &&
+      (c->ntp > 0 || strcmp(c->planet_name, "hot_jupiter") != 0)) {
+    bool ok = c->ntp >= 2 && c->n_t_cool >= 2 && c->pdat && c->tdat && c->t_cool_x && c->t_cool_y;
+    for (int k = 0; ok && k < c->ntp; k++) ok = c->pdat[k] > 0. && (k == 0 || -log(c->pdat[k]) > -log(c->pdat[k - 1]));
+    for (int k = 1; ok && k < c->n_t_cool; k++) ok = c->t_cool_x[k] > c->t_cool_x[k - 1];
+    if (!ok) {
+      snprintf(err, cap, "newt_cool_on (planet %s) needs the sounding (ntp >= 2 pressures pdat, falling, and "
+               "temperatures tdat) and the cooling-time table (n_t_cool >= 2 points, t_cool_x = -log p rising)",
+               c->planet_name);
+      return -1;
+    }
   }
   if (c->nu[4] < 0. || c->nu[6] < 0. || c->nu[8] < 0.) {
     snprintf(err, cap, "negative hyperviscosity coefficient");
@@ -422,6 +432,14 @@ void member_plan(const epic_case *c, bool mixed, const epic_tables &T, MemberDes
   add(&d.coslat4_h, sizeof(double) * (c->nj + 2), hold.cl4_h.data());
   add(&d.coslon_h, sizeof(double) * (c->ni + 2), hold.clon_h.data());
   add(&d.t_sponge, sizeof(double) * (c->k_sponge + 1), c->t_sponge_inv);
+  // the sounding and cooling-time tables (one entry when there are none)
+  const size_t ns = T.n_snd ? T.n_snd : 1, nc = T.n_cool ? T.n_cool : 1;
+  add(&d.snd_x, sizeof(double) * ns, T.snd[0]);
+  add(&d.snd_y, sizeof(double) * ns, T.snd[1]);
+  add(&d.snd_d, sizeof(double) * ns, T.snd[2]);
+  add(&d.cool_x, sizeof(double) * nc, T.cool[0]);
+  add(&d.cool_y, sizeof(double) * nc, T.cool[1]);
+  add(&d.cool_d, sizeof(double) * nc, T.cool[2]);
   add(&d.mn_inv, sizeof(double) * nJ, T.mn_inv);
   add(&d.hl_h, sizeof(double) * T.n_hl, T.hl_h);
   add(&d.hl_pv, sizeof(double) * T.n_hl, T.hl_pv);
@@ -486,7 +504,8 @@ void member_plan(const epic_case *c, bool mixed, const epic_tables &T, MemberDes
   d.ks = c->k_sigma; d.k_sponge = c->k_sponge;
   d.is_hs = (strcmp(c->planet_name, "held_suarez") == 0);
   d.is_gas = (strcmp(c->planet_type, "gas-giant") == 0);
-  d.is_hj = (strcmp(c->planet_name, "hot_jupiter") == 0);
+  d.is_tab = epic_case_table_cooling(c);
+  d.is_hj = (strcmp(c->planet_name, "hot_jupiter") == 0) && !d.is_tab;
   d.pbot = c->pbot;
   d.teq_contrast = c->teq_contrast;
   d.teq_p_contrast = c->teq_p_contrast;
@@ -496,6 +515,8 @@ void member_plan(const epic_case *c, bool mixed, const epic_tables &T, MemberDes
   // (ref_pb/press0)^kappa, with the host libm like the oracle's
   d.ref_pbk = d.is_hj ? pow(c->ref_pb / c->press0, c->kappa) : 0.;
   d.hj_lnref = d.is_hj ? log(c->pbot / c->teq_p_contrast) : 0.;
+  d.n_snd = T.n_snd;
+  d.n_cool = T.n_cool;
   d.heat_area = 0.;
   for (int J = 0; J <= c->nj; J++) {
     const double da = 1. / c->mn[2 * J + 1];

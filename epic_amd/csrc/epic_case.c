@@ -115,7 +115,8 @@ void epic_case_free(epic_case *c) {
   double **arrs[] = {&c->lon, &c->lat, &c->rln, &c->rlt, &c->f, &c->m, &c->n, &c->mn, &c->g,
                      &c->sigmatheta, &c->dsgth, &c->dsgth_inv, &c->p_ref, &c->theta_ref,
                      &c->h_min, &c->t_sponge_inv, &c->u, &c->v, &c->hdry, &c->theta,
-                     &c->p3, &c->u_spinup, &c->gz_surface};
+                     &c->p3, &c->u_spinup, &c->gz_surface, &c->pdat, &c->tdat, &c->t_cool_x,
+                     &c->t_cool_y};
   for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); i++) {
     free(*arrs[i]);
     *arrs[i] = NULL;
@@ -324,7 +325,7 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
       break;
     case EPIC_PLANET_HOT_JUPITER:
       strcpy(c->planet_name, "hot_jupiter");
-      c->planet_index = 20;
+      c->planet_index = 18; /* HOT_JUPITER_INDEX, include/epic_datatypes.h:89 */
       c->re = c->rp = 9.44e7;
       c->omega = 2.06e-5;
       c->cp = 14300.;
@@ -369,10 +370,13 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   c->drag_zonal_avg = s->drag_zonal_avg ? 1 : 0;
   c->newt_cool_adjust = s->newt_cool_adjust ? 1 : 0;
   c->nudiv_nondim = s->nudiv_nondim;
+  /* Newtonian cooling switched on for a planet with no relaxation of its
+   * own relaxes toward a sounding with a cooling-time table, as the
+   * reference's other planets do (formed below, once the profile is known) */
+  int want_sounding = 0;
   if (s->newt_cool > 0) {
-    /* Newtonian cooling of a planet that has no forcing here would need its
-     * t_vs_p sounding and t_cool_table */
-    if (!c->newt_cool_on) return -5;
+    if (!c->newt_cool_on) want_sounding = 1;
+    c->newt_cool_on = 1;
   } else if (s->newt_cool < 0) {
     c->newt_cool_on = 0;
   }
@@ -450,6 +454,32 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
     c->ref_theta_b = pr.theta_b;
     c->ref_pb = pr.pb;
     c->ref_a_over_kappa = pr.a_over_kappa;
+  }
+  if (want_sounding) {
+    /* The sounding: 48 pressures log-spaced from 2 pbot up to ptop/2 (so
+     * pdat falls and -log(p) rises along the table), at the temperature of
+     * the initial profile (theta_profile at the equator times
+     * (p/press0)^kappa) shifted linearly in ln p from +2 K at the bottom to
+     * -2 K at the top: the relaxation warms the deep layers and cools the
+     * upper ones.  The cooling time: 16 pressures over the same range,
+     * 2e6 s (p/pbot)^(1/2), i.e. 2e6 s at pbot and faster aloft. */
+    const double p_hi = 2. * pbot, p_lo = .5 * ptop, lhi = log(p_hi), llo = log(p_lo);
+    c->ntp = 48;
+    c->pdat = zalloc(c->ntp);
+    c->tdat = zalloc(c->ntp);
+    for (int i = 0; i < c->ntp; i++) {
+      const double lp = lhi + (double)i / (double)(c->ntp - 1) * (llo - lhi), p = exp(lp);
+      c->pdat[i] = p;
+      c->tdat[i] = theta_profile(&pr, 0., p) * pow(p / pr.p0, pr.kappa) + (4. * (lp - llo) / (lhi - llo) - 2.);
+    }
+    c->n_t_cool = 16;
+    c->t_cool_x = zalloc(c->n_t_cool);
+    c->t_cool_y = zalloc(c->n_t_cool);
+    for (int i = 0; i < c->n_t_cool; i++) {
+      const double lp = lhi + (double)i / (double)(c->n_t_cool - 1) * (llo - lhi), p = exp(lp);
+      c->t_cool_x[i] = -log(p);
+      c->t_cool_y[i] = 2.e6 * sqrt(p / pbot);
+    }
   }
   /* theta_ref(p): cos-weighted meridional mean of the profile */
   double *pref = c->p_ref, *tref = c->theta_ref;

@@ -11,6 +11,9 @@
  *   get_kin mn_inv           src/shared/epic_funcs_diag.c:2718-2732
  *   g_sigma coefficient      src/shared/epic_funcs_diag.c:1297-1299
  *   machine_epsilon          src/shared/epic_funcs_util.c:1247-1264
+ *   sounding and cooling-time PCHIP tables of the Newtonian cooling
+ *                            src/shared/epic_funcs_diag.c:3200-3256,
+ *                            src/shared/epic_sensible_heating.c:264-271
  */
 #include "tables.h"
 
@@ -20,6 +23,99 @@
 
 #define DEGR (M_PI / 180.)
 #define LAT0 45.
+
+/* sign of a*b without forming it (the Fritsch-Carlson sign test,
+ * src/shared/epic_funcs_util.c:1000-1054) */
+static double sign_of_product(double a, double b) {
+  if (a == 0. || b == 0.) return 0.;
+  return ((a < 0.) == (b < 0.)) ? 1. : -1.;
+}
+
+/* Monotone piecewise-cubic (PCHIP) derivatives at the points of an ascending
+ * table, each formed from its own three neighbours (Fritsch & Carlson with
+ * Brodlie's weights inside, the three-point shape-preserving formula at the
+ * ends), as spline_pchip (src/shared/epic_funcs_util.c:721-925) forms them
+ * along its sweep.  Returns -1 unless x rises strictly and n >= 2. */
+static int pchip_slopes(int n, const double *x, const double *y, double *d) {
+  if (n < 2) return -1;
+  for (int i = 1; i < n; i++)
+    if (!(x[i] > x[i - 1])) return -1;
+  if (n == 2) {
+    d[0] = d[1] = (y[1] - y[0]) / (x[1] - x[0]);
+    return 0;
+  }
+  for (int i = 1; i + 1 < n; i++) {
+    const double h1 = x[i] - x[i - 1], h2 = x[i + 1] - x[i], hsum = h1 + h2;
+    const double del1 = (y[i] - y[i - 1]) / h1, del2 = (y[i + 1] - y[i]) / h2;
+    d[i] = 0.;
+    if (sign_of_product(del1, del2) > 0.) {
+      const double h3 = 3. * hsum, a1 = (hsum + h1) / h3, a2 = (hsum + h2) / h3;
+      const double big = fmax(fabs(del1), fabs(del2)), small = fmin(fabs(del1), fabs(del2));
+      d[i] = small / (a1 * (del1 / big) + a2 * (del2 / big));
+    }
+  }
+  for (int end = 0; end < 2; end++) {
+    /* the end point e, its neighbour and the next: the slope of the nearer
+     * interval weighted against the farther one */
+    const int e = end ? n - 1 : 0;
+    const double hn = end ? x[e] - x[e - 1] : x[1] - x[0];
+    const double hf = end ? x[e - 1] - x[e - 2] : x[2] - x[1];
+    const double dn = end ? (y[e] - y[e - 1]) / hn : (y[1] - y[0]) / hn;
+    const double df = end ? (y[e - 1] - y[e - 2]) / hf : (y[2] - y[1]) / hf;
+    const double hsum = hn + hf;
+    const double v = ((hn + hsum) / hsum) * dn + (-hn / hsum) * df;
+    double r = v;
+    if (sign_of_product(v, dn) <= 0.) {
+      r = 0.;
+    } else if (sign_of_product(df, dn) < 0.) {
+      const double lim = 3. * dn;
+      if (fabs(lim) < fabs(v)) r = lim;
+    }
+    d[e] = r;
+  }
+  return 0;
+}
+
+int epic_case_table_cooling(const epic_case *c) {
+  return c->newt_cool_on && strcmp(c->planet_name, "held_suarez") != 0 && c->ntp > 0;
+}
+
+/* The sounding table of get_sounding's first call: T against -log p, its
+ * PCHIP slopes from those temperatures, then theta = T (press0/p)^kappa
+ * ironed from the top down (a level warmer in theta than the one above it
+ * takes that one's theta) and each ironed level's T formed again from its
+ * theta; the slopes are not formed again.  And the cooling-time table with
+ * its slopes (t_rad's first call). */
+static int cooling_tables(const epic_case *c, epic_tables *T) {
+  const int n = c->ntp, m = c->n_t_cool;
+  if (n < 2 || m < 2) return -1;
+  for (int q = 0; q < 3; q++) {
+    T->snd[q] = (double *)calloc((size_t)n, sizeof(double));
+    T->cool[q] = (double *)calloc((size_t)m, sizeof(double));
+  }
+  double *th = (double *)calloc((size_t)n, sizeof(double));
+  for (int k = 0; k < n; k++) {
+    if (!(c->pdat[k] > 0.)) { free(th); return -1; }
+    T->snd[0][k] = -log(c->pdat[k]);
+    T->snd[1][k] = c->tdat[k];
+    th[k] = c->tdat[k] * pow(c->press0 / c->pdat[k], c->kappa);
+  }
+  int rc = pchip_slopes(n, T->snd[0], T->snd[1], T->snd[2]);
+  for (int k = n - 2; k >= 0; k--)
+    if (th[k] > th[k + 1]) {
+      th[k] = th[k + 1];
+      T->snd[1][k] = th[k] * pow(c->pdat[k] / c->press0, c->kappa);
+    }
+  free(th);
+  for (int k = 0; k < m; k++) {
+    T->cool[0][k] = c->t_cool_x[k];
+    T->cool[1][k] = c->t_cool_y[k];
+  }
+  if (pchip_slopes(m, T->cool[0], T->cool[1], T->cool[2])) rc = -1;
+  T->n_snd = n;
+  T->n_cool = m;
+  return rc;
+}
 
 static int four1_tw(int nn, int isign, double *tw) {
   int n = nn << 1, mmax = 2, k = 0;
@@ -56,7 +152,7 @@ static int realft_tw(int n, int isign, double *tw) {
 
 void epic_tables_free(epic_tables *T) {
   free(T->mn_inv); free(T->hl_h); free(T->hl_pv);
-  for (int q = 0; q < 3; q++) { free(T->hc[q]); free(T->vc[q]); }
+  for (int q = 0; q < 3; q++) { free(T->hc[q]); free(T->vc[q]); free(T->snd[q]); free(T->cool[q]); }
   free(T->twf); free(T->twi); free(T->trf); free(T->tri);
   free(T->rows_h); free(T->rows_v);
   memset(T, 0, sizeof(*T));
@@ -169,5 +265,6 @@ int epic_tables_build(const epic_case *c, epic_tables *T) {
   T->gcoef = 1. / (1 - exp(-c->hybrid_alpha * (1. - c->sigma_sigma)));
   T->mu3 = 1. / (c->rgas / 8.314472e+3);
   T->mu2 = .5 * (T->mu3 + T->mu3);
+  if (epic_case_table_cooling(c) && cooling_tables(c, T)) return -1;
   return 0;
 }

@@ -274,9 +274,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
   d.T3[q] = t;
   d.RHO3[q] = rho3;
   if (!full) return;
-  if (K >= 1 && d.newt_cool_on && (d.is_hs || d.is_hj)) {
+  if (K >= 1 && newt_relax_on(d)) {
     double h;
-    if (d.is_hj) {
+    if (d.is_tab) {
+      h = 0. + tab_relax(d, p, t);
+    } else if (d.is_hj) {
       h = hj_relax(d, J, (I == 0) ? d.ni : (I == d.ni + 1 ? 1 : I), p, t, pk);
     } else {
       const double pk0 = (d.press0 == 1000. * 100.) ? pk : crm_pow(p / (1000. * 100.), d.kappa);
@@ -670,14 +672,21 @@ __device__ double hj_newton(const MemberDesc &d, int K, int J, int I) {
   return hj_relax(d, J, I, p, d.T3[ix3(d, K, J, I)], crm_pow(p / d.press0, d.kappa));
 }
 
-// HEAT3 = 0 everywhere, then the Held-Suarez (or day-night) relaxation on
-// K = 1..nk with its zonal pads (the bc_lateral at the end of calc_heating)
+// HEAT3 = 0 everywhere, then the Held-Suarez, day-night or sounding
+// relaxation on K = 1..nk with its zonal pads (the bc_lateral at the end of
+// calc_heating)
 __global__ void __launch_bounds__(256) k_heating(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
   const int K = 1 + blockIdx.y;
-  if (!d.newt_cool_on || !(d.is_hs || d.is_hj)) return;
-  double h = d.is_hj ? hj_newton(d, K, J, I) : 0. + hs_newton(d, K, J, I);
+  if (!newt_relax_on(d)) return;
+  double h;
+  if (d.is_tab) {
+    const size_t q = ix3(d, K, J, I);
+    h = 0. + tab_relax(d, d.P3[q], d.T3[q]);
+  } else {
+    h = d.is_hj ? hj_newton(d, K, J, I) : 0. + hs_newton(d, K, J, I);
+  }
   d.HEAT3[ix3(d, K, J, I)] = h;
   if (I == d.ni) d.HEAT3[ix3(d, K, J, 0)] = h;
   if (I == 1) d.HEAT3[ix3(d, K, J, d.ni + 1)] = h;
@@ -692,7 +701,7 @@ __global__ void __launch_bounds__(256) k_heat_layer_mean(const MemberDesc *D, in
   MEMBER;
   extern __shared__ double lds[];
   const int K = 1 + blockIdx.x;
-  if (!(d.newt_cool_on && (d.is_hs || d.is_hj))) return;
+  if (!newt_relax_on(d)) return;
   if (!(K <= d.k_sponge || d.newt_cool_adjust)) return;
   const int ni = d.ni, nrow = d.nj + 1;
   const int rpc = (2048 / ni) > 0 ? 2048 / ni : 1;  // rows per LDS chunk
@@ -714,7 +723,7 @@ __global__ void __launch_bounds__(256) k_heat_adjust_apply(const MemberDesc *D, 
   MEMBER;
   FLAT2(0, d.nj + 1, 0, d.ni + 2);
   const int K = 1 + blockIdx.y;
-  if (!(d.newt_cool_on && (d.is_hs || d.is_hj))) return;
+  if (!newt_relax_on(d)) return;
   if (!(K <= d.k_sponge || d.newt_cool_adjust)) return;
   const size_t q = ix3(d, K, J, I);
   d.HEAT3[q] = 0. + (d.HEAT3[q] - d.sD[K]);

@@ -167,9 +167,11 @@ void L_uv_core(hipStream_t s, const Bucket &b) {
 
 // ------------------------------------------------------------- uv_drag
 // t_rad (src/shared/epic_sensible_heating.c:217-314): the Held-Suarez
-// cooling time; the hot Jupiter's relaxation time is its constant tau_rad
+// cooling time, or the tabulated one at the interface pressure; the hot
+// Jupiter's relaxation time is its constant tau_rad
 __device__ __forceinline__ double t_rad(const MemberDesc &d, int K, int J, int I) {
   if (d.is_hj) return d.tau_rad;
+  if (d.is_tab) return tab_cool_time(d, -crm_log(d.P3[ix3(d, K, J, I)]));
   const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
   double p = d.P3[ix3(d, K, J, I)], kt = ka, pbot = d.P3[ix3(d, d.nk, J, I)];
   double amp = (p / pbot - 0.7) / (1. - 0.7);

@@ -170,6 +170,7 @@ __global__ void __launch_bounds__(256) km_sponge(const MemberDesc *D, int mb) {
 
 __device__ __forceinline__ float t_rad_f(const MemberDesc &d, int K, int J, int I) {
   if (d.is_hj) return (float)d.tau_rad;
+  if (d.is_tab) return (float)tab_cool_time(d, -crm_log(d.P3[ix3(d, K, J, I)]));
   const float ka = (float)(1. / (40. * 60. * 60. * 24.)), ks = (float)(1. / (4. * 60. * 60. * 24.));
   float p = (float)d.P3[ix3(d, K, J, I)], kt = ka, pbot = (float)d.P3[ix3(d, d.nk, J, I)];
   float amp = (p / pbot - 0.7f) / (1.f - 0.7f);

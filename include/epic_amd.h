@@ -217,7 +217,10 @@ typedef struct {
    * default (on for Held-Suarez and the hot Jupiter). */
   int    k_sponge;        /* top sponge layers (t_sponge_inv as set_sponge
                              forms it), or EPIC_K_SPONGE_DEFAULT            */
-  int    newt_cool;       /* 0: planet default, 1: on, -1: off              */
+  int    newt_cool;       /* 0: planet default, 1: on, -1: off; on for a
+                             planet without its own relaxation it relaxes
+                             toward a sounding with a cooling-time table
+                             the case builder forms (epic_case: ntp ...)    */
   int    newt_cool_adjust;/* remove the layer mean of the heating           */
   int    drag_v;          /* Rayleigh drag on v as well as u                */
   int    drag_zonal_avg;  /* Rayleigh drag acts on the zonal mean           */
@@ -256,6 +259,16 @@ typedef struct {
   double *sigmatheta, *dsgth, *dsgth_inv, *p_ref, *theta_ref, *h_min, *t_sponge_inv;
   /* initial state, Nelem3d each (gz_surface: Nelem2d) */
   double *u, *v, *hdry, *theta, *p3, *u_spinup, *gz_surface;
+  /* Newtonian cooling toward a sounding, for newt_cool_on on any planet but
+   * Held-Suarez (a hot Jupiter without a sounding takes the day-night
+   * relaxation above instead): the
+   * reference's var.ntp, var.pdat [Pa] and var.tdat [K] (get_sounding,
+   * src/shared/epic_funcs_diag.c:3160-3294) and var.n_t_cool,
+   * var.t_cool_table (x = -log(p [Pa]) ascending, y = cooling time [s];
+   * t_rad, src/shared/epic_sensible_heating.c:217-314).  0 and null when
+   * the case has no such cooling. */
+  int    ntp, n_t_cool;
+  double *pdat, *tdat, *t_cool_x, *t_cool_y;
 } epic_case;
 
 /* Build a case from a spec: metric arrays (set_lonlat/set_fmn/set_gravity/

@@ -43,6 +43,10 @@ struct cpuref {
   int ni, nj, nk, Ia, N2, N3, k_sigma, k_sponge;
   double dt;
   int is_hs, is_gas, is_hj;
+  /* Newtonian cooling toward the case's sounding: the sounding (sx = -log p,
+   * sy = T, sz its PCHIP derivatives) and the cooling-time table (cx, cy, cz) */
+  int is_tab, ns, nc;
+  double *sx, *sy, *sz, *cx, *cy, *cz;
   /* fields */
   double *U, *V, *DUDT[3], *DVDT[3], *HDRY, *THETA, *P3, *P2, *THETA2, *H3;
   double *T2, *T3, *RHO2, *RHO3, *EXNER3, *GZ3, *MONT3, *HEAT3, *PV3, *DIV, *W3, *DZDT3, *RI2;
@@ -758,10 +762,35 @@ static void store_diag(cpuref *R) {
  * Layer by layer: the relaxation into a buffer, its area-weighted layer mean
  * removed for K <= k_sponge or with newt_cool_adjust (:139-167), then added
  * to the cleared HEAT3. */
+/* t_rad's table branch (src/shared/epic_sensible_heating.c:291-311): the
+ * cooling time at pressure p, held at the table's ends */
+static double tab_t_rad(const cpuref *R, double p) {
+  double neglogp = -log(p), dx;
+  if (neglogp >= R->cx[R->nc - 1]) return R->cy[R->nc - 1];
+  if (neglogp <= R->cx[0]) return R->cy[0];
+  int ki = table_place(R->nc, R->cx, neglogp, &dx);
+  return hermite(R->cx + ki, R->cy + ki, R->cz + ki, dx, neglogp);
+}
+/* get_sounding(..., "temperature") (src/shared/epic_funcs_diag.c:3259-3291):
+ * held at the table's bottom below it */
+static double tab_t_eq(const cpuref *R, double p) {
+  double x = -log(p), xd;
+  if (x < R->sx[0]) return R->sy[0];
+  int ki = table_place(R->ns, R->sx, x, &xd);
+  return hermite(R->sx + ki, R->sy + ki, R->sz + ki, xd, x);
+}
+/* return_cp for a dry ideal gas (x_h2 = 0, src/shared/epic_funcs_diag.c:
+ * 3311-3337, 6652-6760): the centred difference of h = cpr T rgas */
+static double cp_ideal(const epic_case *c, double t) {
+  double deltaT = t * 1.e-6;
+  double h2 = (c->cpr * (t + deltaT)) * c->rgas, h1 = (c->cpr * (t - deltaT)) * c->rgas;
+  return (h2 - h1) / (2. * deltaT);
+}
+
 static void calc_heating(cpuref *R) {
   epic_case *c = &R->c;
   memset(R->HEAT3, 0, sizeof(double) * R->N3);
-  if (!c->newt_cool_on || !(R->is_hs || R->is_hj)) {
+  if (!c->newt_cool_on || !(R->is_hs || R->is_hj || R->is_tab)) {
     bc3(R, R->HEAT3);
     return;
   }
@@ -772,7 +801,12 @@ static void calc_heating(cpuref *R) {
       double lat = c->lat[2 * J + 1];
       for (int I = 1; I <= R->ni; I++) {
         double p = A3(R->P3, K, J, I), t = A3(R->T3, K, J, I);
-        if (R->is_hs) {
+        if (R->is_tab) {
+          /* toward the sounding with the tabulated cooling time
+           * (src/shared/epic_sensible_heating.c:103-130) */
+          const double cp_over_time = cp_ideal(c, t) / tab_t_rad(R, p);
+          A2(buf, J, I) = -cp_over_time * (t - tab_t_eq(R, p));
+        } else if (R->is_hs) {
           const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
           const double dt_y = 60., dth_z = 10., p0 = 1000. * 100.;
           double kt = ka, pbot = A3(R->P3, R->nk, J, I);
@@ -977,6 +1011,7 @@ static void uv_core(cpuref *R, double *dudt, double *dvdt) {
  * constant tau_rad */
 static double t_rad(cpuref *R, int K, int J, int I) {
   if (R->is_hj) return R->c.tau_rad;
+  if (R->is_tab) return tab_t_rad(R, A3(R->P3, K, J, I));
   const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
   double p = A3(R->P3, K, J, I), kt = ka, pbot = A3(R->P3, R->nk, J, I);
   double amp = (p / pbot - 0.7) / (1. - 0.7);
@@ -1741,8 +1776,45 @@ cpuref *cpuref_create(const epic_case_spec *spec) {
   R->k_sigma = c->k_sigma; R->k_sponge = c->k_sponge;
   R->dt = (double)c->dt;
   R->is_hs = (strcmp(c->planet_name, "held_suarez") == 0);
-  R->is_hj = (strcmp(c->planet_name, "hot_jupiter") == 0);
+  R->is_hj = (strcmp(c->planet_name, "hot_jupiter") == 0) && c->ntp == 0;
   R->is_gas = (strcmp(c->planet_type, "gas-giant") == 0);
+  R->is_tab = c->newt_cool_on && !R->is_hs && c->ntp > 0;
+  if (R->is_tab) {
+    /* get_sounding's first call (src/shared/epic_funcs_diag.c:3200-3256):
+     * the temperature table in -log p and its PCHIP derivatives, then the
+     * negative-slope regions of theta ironed out from the top down, each
+     * ironed level's temperature formed again from its theta (the table's
+     * derivatives stay those of the original temperatures, as there); and
+     * t_rad's first call (src/shared/epic_sensible_heating.c:264-271) */
+    int n = R->ns = c->ntp;
+    R->sx = (double *)calloc(n, sizeof(double));
+    R->sy = (double *)calloc(n, sizeof(double));
+    R->sz = (double *)calloc(n, sizeof(double));
+    double *th = (double *)calloc(n, sizeof(double));
+    for (int k = 0; k < n; k++) {
+      R->sx[k] = -(log)(c->pdat[k]);
+      R->sy[k] = c->tdat[k];
+      th[k] = c->tdat[k] * (pow)(c->press0 / c->pdat[k], c->kappa);
+    }
+    if (cpuref_pchip(n, R->sx, R->sy, R->sz)) R->err |= 4;
+    for (int k = n - 2; k >= 0; k--) {
+      double tmp = th[k + 1];
+      if (th[k] > tmp) {
+        th[k] = tmp;
+        R->sy[k] = th[k] * (pow)(c->pdat[k] / c->press0, c->kappa);
+      }
+    }
+    free(th);
+    n = R->nc = c->n_t_cool;
+    R->cx = (double *)calloc(n, sizeof(double));
+    R->cy = (double *)calloc(n, sizeof(double));
+    R->cz = (double *)calloc(n, sizeof(double));
+    for (int k = 0; k < n; k++) {
+      R->cx[k] = c->t_cool_x[k];
+      R->cy[k] = c->t_cool_y[k];
+    }
+    if (cpuref_pchip(n, R->cx, R->cy, R->cz)) R->err |= 4;
+  }
   size_t N3 = R->N3, N2 = R->N2;
 #define ALLOC3(p) p = (double *)calloc(N3, sizeof(double))
   ALLOC3(R->U); ALLOC3(R->V); ALLOC3(R->HDRY); ALLOC3(R->THETA); ALLOC3(R->P3); ALLOC3(R->P2);
@@ -1824,7 +1896,8 @@ void cpuref_destroy(cpuref *R) {
                    R->RHO3, R->EXNER3, R->GZ3, R->MONT3, R->HEAT3, R->PV3, R->DIV, R->W3, R->DZDT3, R->RI2, R->USPIN,
                    R->gz, R->sum_div, R->unity, R->old, R->varz, R->GZS, R->gzu2d, R->gzv2d, R->gz2d, R->hl_h,
                    R->hl_pv, R->w_wm, R->w_wp, R->w_a, R->w_pred, R->w_min, R->w_ff, R->w_d1, R->w_d2, R->w_aaa,
-                   R->tx, R->ty, R->tz, R->mn_inv, R->hcx, R->hcy, R->hcz, R->vcx, R->vcy, R->vcz, R->fdata};
+                   R->tx, R->ty, R->tz, R->mn_inv, R->hcx, R->hcy, R->hcz, R->vcx, R->vcy, R->vcz, R->fdata,
+                   R->sx, R->sy, R->sz, R->cx, R->cy, R->cz};
   for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); q++) free(all[q]);
   for (int p = 0; p < 3; p++) { free(R->DUDT[p]); free(R->DVDT[p]); }
   for (int q = 0; q < 8; q++) free(R->b2[q]);

@@ -273,8 +273,19 @@ int main(int argc, char **argv) {
   strcpy(var.hdry.advection_scheme, "Predictor-corrector (Hsu, Konor, and Arakawa)");
   strcpy(var.theta.advection_scheme, "Predictor-corrector (Hsu, Konor, and Arakawa)");
   var.time_fp_bar = 3.e8 * 1000. * 100.;
+  /* the sounding and the cooling-time table (make_arrays sizes them) */
+  var.ntp = c.ntp;
+  var.n_t_cool = c.n_t_cool;
 
   make_arrays(planet);
+  for (int q = 0; q < c.ntp; q++) {
+    var.pdat[q] = c.pdat[q];
+    var.tdat[q] = c.tdat[q];
+  }
+  for (int q = 0; q < c.n_t_cool; q++) {
+    var.t_cool_table[q].x = c.t_cool_x[q];
+    var.t_cool_table[q].y = c.t_cool_y[q];
+  }
   /* make_arrays allocates t_sponge_inv only for k_sponge > 0 (and
    * set_sponge fills it then); uv_drag reads t_sponge_inv[0] in any case */
   if (grid.k_sponge <= 0) grid.t_sponge_inv = fvector(0, 0, "harness");

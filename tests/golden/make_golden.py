@@ -43,6 +43,10 @@ CASES = {
     # the reference `initial` program's hyperviscosity default: del^8 alone
     "jup_del8_tiny": (1, 16, 8, 6, 60, 9, 0.0, 0.0, 0.0, 0.5, [0, 1, 10],
                       {"k_sponge": -1, "nu8_nondim": 0.5}),
+    # Newtonian cooling toward the sounding with the tabulated cooling time,
+    # its layer-mean removal and the prandtl drag that reads that time
+    "jup_newt_tiny": (1, 16, 8, 6, 60, 10, 0.5, 0.0, 0.0, 0.5, [0, 1, 10],
+                      {"k_sponge": -1, "newt_cool": 1, "newt_cool_adjust": 1, "prandtl": 0.3, "drag_v": 1}),
 }
 
 

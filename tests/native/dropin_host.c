@@ -86,6 +86,10 @@ int main(int argc, char **argv) {
   /* NAME renames the planet while keeping the case's settings: a caller's
    * configuration that timestep() must refuse before it touches the GPU */
   snprintf(pl.name, sizeof(pl.name), "%s", argc > 14 ? argv[14] : c.planet_name);
+  /* the reference's index of a renamed planet (include/epic_datatypes.h) */
+  if (!strcmp(pl.name, "titan")) pl.index = 10;
+  if (!strcmp(pl.name, "hot_jupiter")) pl.index = 18;
+  if (!strcmp(pl.name, "venus_llr05")) pl.index = 20;
   snprintf(pl.type, sizeof(pl.type), "%s", c.planet_type);
   pl.re = c.re; pl.rp = c.rp; pl.omega_sidereal = pl.omega_synodic = c.omega;
   pl.cp = c.cp; pl.cpr = c.cpr; pl.rgas = c.rgas; pl.kappa = c.kappa;
@@ -128,6 +132,19 @@ int main(int argc, char **argv) {
                                    &var.dzdt3, &var.u_spinup, &var.ri2};
   for (unsigned q = 0; q < sizeof(diags) / sizeof(diags[0]); q++) diags[q]->value = arr(N3);
   var.gz_surface.value = arr(N2);
+  /* the sounding and the cooling-time table ({x, y, z} triplets) */
+  if (c.ntp > 0) {
+    var.ntp = c.ntp;
+    var.pdat = c.pdat;
+    var.tdat = c.tdat;
+    var.n_t_cool = c.n_t_cool;
+    double *trip = arr(3 * (size_t)c.n_t_cool);
+    for (int q = 0; q < c.n_t_cool; q++) {
+      trip[3 * q] = c.t_cool_x[q];
+      trip[3 * q + 1] = c.t_cool_y[q];
+    }
+    var.t_cool_table = trip;
+  }
   memcpy(var.u.value, c.u, sizeof(double) * N3);
   memcpy(var.v.value, c.v, sizeof(double) * N3);
   memcpy(var.theta.value, c.theta, sizeof(double) * N3);

@@ -74,21 +74,32 @@ def test_library_leaves_the_callers_globals_weak():
             assert p[4] == "WEAK" and p[6] == "UND", line
 
 
-@pytest.mark.parametrize("name", ["jupiter", "venus", "hot_jupiter"])
-def test_dropin_refuses_newtonian_cooling_it_does_not_have(tmp_path, name):
+@pytest.mark.parametrize("name,opts,msg", [
+    ("jupiter", [], "needs the t_vs_p sounding"),
+    ("venus", [], "needs the t_vs_p sounding"),
+    ("hot_jupiter", [], "needs the t_vs_p sounding"),
+    # Venus (LLR05) takes its temp_eq in the reference, not the sounding
+    ("venus_llr05", ["--newt_cool=1"], "venus_llr05 and titan"),
+])
+def test_dropin_refuses_newtonian_cooling_it_does_not_have(tmp_path, name, opts, msg):
     """With grid.newt_cool_on, a planet other than held_suarez relaxes toward
     its t_vs_p sounding with t_cool_table's times in the reference (a planet
     named hot_jupiter too: the engine's day-night relaxation is its own case,
-    not the reference's); timestep() stops with an EPIC-style error before
-    any GPU work instead of stepping with another forcing."""
+    not the reference's), so a caller without the sounding is refused, and
+    so are the two planets whose relaxation is their temp_eq; timestep()
+    stops with an EPIC-style error before any GPU work instead of stepping
+    with another forcing."""
     exe = os.path.join(ROOT, "tests", "native", "_bin", "dropin_host")
     if not os.path.exists(exe):
         pytest.skip("dropin_host not built")
-    # the Held-Suarez case (newt_cool_on = 1) under another planet's name
-    args = [exe, "0", "32", "16", "8", "600", "1", "0.5", "0", "0.1", "0", "1", str(tmp_path / "out"), "0", name]
+    # the Held-Suarez case (newt_cool_on = 1, no sounding) under another
+    # planet's name; with --newt_cool=1 a Jupiter case with its sounding
+    planet = "1" if opts else "0"
+    args = ([exe] + opts + [planet, "32", "16", "8", "600", "1", "0.5", "0", "0.1", "0", "1",
+                            str(tmp_path / "out"), "0", name])
     r = subprocess.run(args, cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
     assert r.returncode == 1, r.stdout + r.stderr
-    assert "Newtonian cooling from t_cool_table" in r.stderr
+    assert msg in r.stderr
     assert not os.path.exists(tmp_path / "out")
 
 

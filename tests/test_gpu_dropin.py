@@ -30,6 +30,9 @@ HS = ((0, 32, 16, 8, 600, 21, 0.5, 0.1, 0.3, 0.3), {})  # Held-Suarez with diver
 # `initial` sets it and Rayleigh drag about the zonal mean
 JUP = ((1, 32, 16, 10, 60, 23, 0.5, 0.0, 0.0, 1.0),
        {"k_sponge": -1, "tau_drag": 2.0e4, "drag_v": 1, "drag_zonal_avg": 1})
+# Newtonian cooling toward the t_vs_p sounding with t_cool_table's times,
+# and the prandtl drag that reads those times
+SAT_NEWT = ((2, 32, 12, 8, 60, 28, 0.4, 0.0, 0.0, 0.5), {"k_sponge": 2, "newt_cool": 1, "prandtl": 0.5, "drag_v": 1})
 NSTEPS = 12
 
 
@@ -98,7 +101,7 @@ def ri2_of(case, f):
 
 
 @pytest.mark.skipif(not os.path.exists(HOST), reason="tests/native/_bin/dropin_host not built")
-@pytest.mark.parametrize("case", [HS, JUP], ids=["held_suarez", "jupiter_sponge_drag"])
+@pytest.mark.parametrize("case", [HS, JUP, SAT_NEWT], ids=["held_suarez", "jupiter_sponge_drag", "saturn_sounding"])
 def test_dropin_eager_and_lazy_match_oracle(case, tmp_path):
     h_e, f_e = run(HOST, case, [0], tmp_path, "eager.bin")
     h_l, f_l = run(HOST, case, [1], tmp_path, "lazy.bin")
@@ -114,7 +117,7 @@ def test_dropin_eager_and_lazy_match_oracle(case, tmp_path):
 
 @pytest.mark.skipif(not (os.path.exists(REF_DROPIN) and os.path.exists(REF)),
                     reason="reference programs (oracle/_ref) not built")
-@pytest.mark.parametrize("case", [HS, JUP], ids=["held_suarez", "jupiter_sponge_drag"])
+@pytest.mark.parametrize("case", [HS, JUP, SAT_NEWT], ids=["held_suarez", "jupiter_sponge_drag", "saturn_sounding"])
 def test_reference_program_with_dropin_matches_reference(case, tmp_path):
     h_g, f_g = run(REF_DROPIN, case, [NSTEPS], tmp_path, "gpu.bin")
     h_c, f_c = run(REF, case, [NSTEPS], tmp_path, "cpu.bin")

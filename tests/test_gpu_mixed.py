@@ -114,3 +114,17 @@ def test_mixed_with_del8_hyperviscosity():
         e = float(np.linalg.norm(interior(mix[f], sp) - interior(ref[f], sp)) / np.linalg.norm(interior(ref[f], sp)))
         say(f"mixed vs fp64, del^8, 200 steps {f}: rel L2 {e:.2e}")
         assert e <= 1e-5, (f, e)
+
+
+def test_mixed_with_sounding_relaxation():
+    """Newtonian cooling toward the sounding with the tabulated cooling time
+    (FP64 in both modes) and the prandtl drag that reads that time (FP32 in
+    the mixed mode): within 1e-5 of FP64 after 200 steps."""
+    sp = ea.spec(ea.SATURN_LIKE, 64, 32, 10, 60, 5, 0.5, 0.0, 0.0, 0.5, newt_cool=1, newt_cool_adjust=1,
+                 prandtl=0.5, drag_v=1)
+    ref, _, _ = run(sp, ea.FP64, 200)
+    mix, _, _ = run(sp, ea.MIXED, 200)
+    for f in FIELDS:
+        e = float(np.linalg.norm(interior(mix[f], sp) - interior(ref[f], sp)) / np.linalg.norm(interior(ref[f], sp)))
+        say(f"mixed vs fp64, sounding relaxation, 200 steps {f}: rel L2 {e:.2e}")
+        assert e <= 1e-5, (f, e)

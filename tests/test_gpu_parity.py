@@ -38,6 +38,13 @@ CASES = {
     # program's default), and del^4 + del^6 + del^8 on both grids
     "jupiter_del8_32x16x10": ea.spec(ea.JUPITER_LIKE, 32, 16, 10, 60, 23, 0.0, 0.0, 0.0, 1.0, nu8=0.5),
     "hs_del468_32x16x8": ea.spec(ea.HELD_SUAREZ, 32, 16, 8, 600, 25, 0.3, 0.1, 0.3, 0.4, nu6=0.2, nu8=0.5),
+    # Newtonian cooling toward the sounding with the tabulated cooling time:
+    # alone under the default sponge, and with the layer-mean removal and
+    # the prandtl drag about the zonal mean
+    "jupiter_newt_32x16x10": ea.spec(ea.JUPITER_LIKE, 32, 16, 10, 60, 26, 0.5, 0.0, 0.0, 1.0, newt_cool=1),
+    "venus_newt_adjust_prandtl_32x16x6": ea.spec(ea.VENUS_LIKE, 32, 16, 6, 600, 27, 0.5, 0.1, 0.2, 0.5,
+                                                 newt_cool=1, newt_cool_adjust=1, prandtl=0.4,
+                                                 drag_zonal_avg=1),
 }
 
 
@@ -111,7 +118,8 @@ def test_wide_rows(ni, nj, nk, dt, nu4, nu8):
         p.close()
 
 
-@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny", "jup_drag_tiny", "hs_adjust_tiny", "jup_del8_tiny"])
+@pytest.mark.parametrize("name", ["hs_tiny", "jup_tiny", "venus_tiny", "jup_drag_tiny", "hs_adjust_tiny", "jup_del8_tiny",
+                                  "jup_newt_tiny"])
 def test_against_reference_fixtures(name):
     """The GPU path against arrays written by the reference's own build."""
     import json

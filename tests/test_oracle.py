@@ -99,6 +99,13 @@ REF_CASES = [
     ((1, 32, 16, 10, 60, 23, 0.0, 0.0, 0.0, 1.0), {"k_sponge": -1, "nu8_nondim": 0.5}),
     ((3, 16, 12, 6, 600, 24, 0.0, 0.1, 0.2, 0.5), {"k_sponge": -1, "nu6_nondim": 0.4}),
     ((0, 32, 16, 8, 600, 25, 0.3, 0.0, 0.3, 0.4), {"nu6_nondim": 0.2, "nu8_nondim": 0.5}),
+    # Newtonian cooling toward a sounding with a tabulated cooling time
+    # (get_sounding, t_rad's table branch), alone, with the layer-mean removal
+    # and with the prandtl drag that reads the tabulated time
+    ((1, 32, 16, 10, 60, 26, 0.5, 0.0, 0.0, 1.0), {"k_sponge": -1, "newt_cool": 1}),
+    ((3, 16, 12, 6, 600, 27, 0.5, 0.0, 0.2, 0.5), {"newt_cool": 1, "newt_cool_adjust": 1}),
+    ((2, 32, 12, 8, 60, 28, 0.4, 0.0, 0.0, 0.5),
+     {"k_sponge": 2, "newt_cool": 1, "prandtl": 0.5, "drag_v": 1}),
 ]
 
 
