@@ -91,7 +91,9 @@ class Case(ctypes.Structure):
                                    "dsgth_inv", "p_ref", "theta_ref", "h_min", "t_sponge_inv", "u", "v", "hdry",
                                    "theta", "p3", "u_spinup", "gz_surface")] +
                 [("ntp", _I), ("n_t_cool", _I)] +
-                [(n, _P) for n in ("pdat", "tdat", "t_cool_x", "t_cool_y")])
+                [(n, _P) for n in ("pdat", "tdat", "t_cool_x", "t_cool_y")] +
+                [("dudt_hist", _P), ("dvdt_hist", _P), ("model_time", _D), ("start_time", _D),
+                 ("nc_header", ctypes.c_void_p)])
 
     def array(self, name, n):
         """A numpy copy of the first n values of one of the case's arrays."""
@@ -110,6 +112,40 @@ def build_case(sp):
 
 def free_case(c):
     lib().epic_case_free(ctypes.byref(c))
+
+
+class NcState(ctypes.Structure):
+    """epic_nc_state of include/epic_amd.h."""
+    _fields_ = ([(n, ctypes.c_void_p) for n in ("u", "v", "p3", "theta", "u_spinup", "gz_surface",
+                                                "dudt_m1", "dudt_m2", "dvdt_m1", "dvdt_m2")] +
+                [("model_time", _D), ("l_s_deg", _D)])
+
+
+def nc_read(path):
+    """A case from an epic.nc file (epic_nc_read); free it with free_case."""
+    c = Case()
+    rc = lib().epic_nc_read(str(path).encode(), ctypes.byref(c))
+    if rc:
+        raise EpicError(f"epic_nc_read failed ({rc}): {lib().epic_nc_error().decode()}")
+    return c
+
+
+def nc_write(path, case, fields, model_time=None, l_s_deg=0.0):
+    """Write an epic.nc file from padded arrays: fields maps U, V, P3,
+    THETA, U_SPINUP, GZ_SURFACE and optionally DUDT_M1, DUDT_M2, DVDT_M1,
+    DVDT_M2 to numpy arrays (epic_nc_write)."""
+    keep = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in fields.items()}
+    st = NcState()
+    for k, nm in (("U", "u"), ("V", "v"), ("P3", "p3"), ("THETA", "theta"), ("U_SPINUP", "u_spinup"),
+                  ("GZ_SURFACE", "gz_surface"), ("DUDT_M1", "dudt_m1"), ("DUDT_M2", "dudt_m2"),
+                  ("DVDT_M1", "dvdt_m1"), ("DVDT_M2", "dvdt_m2")):
+        if k in keep:
+            setattr(st, nm, keep[k].ctypes.data)
+    st.model_time = case.model_time if model_time is None else model_time
+    st.l_s_deg = l_s_deg
+    rc = lib().epic_nc_write(str(path).encode(), ctypes.byref(case), ctypes.byref(st))
+    if rc:
+        raise EpicError(f"epic_nc_write failed ({rc}): {lib().epic_nc_error().decode()}")
 
 
 def dims(sp):
@@ -158,6 +194,15 @@ def lib():
         L.epic_case_build.argtypes = [ctypes.POINTER(CaseSpec), ctypes.POINTER(Case)]
         L.epic_case_build.restype = ci
         L.epic_case_free.argtypes = [ctypes.POINTER(Case)]
+        L.epic_engine_add.argtypes = [vp, ctypes.POINTER(Case)]
+        L.epic_engine_add.restype = ci
+        L.epic_nc_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(Case)]
+        L.epic_nc_read.restype = ci
+        L.epic_nc_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(Case), ctypes.POINTER(NcState)]
+        L.epic_nc_write.restype = ci
+        L.epic_nc_error.restype = ctypes.c_char_p
+        L.epic_engine_write_nc.argtypes = [vp, ci, ctypes.POINTER(Case), ctypes.c_char_p]
+        L.epic_engine_write_nc.restype = ci
         _lib = L
     return _lib
 
@@ -186,6 +231,29 @@ class Engine:
         slot = self._check(lib().epic_engine_add_spec(self._e, ctypes.byref(sp)), "add")
         self.specs[slot] = sp
         return slot
+
+    def add_case(self, c):
+        """A member from a built or read case (epic_engine_add); the engine
+        copies what it needs, the caller keeps and frees the case."""
+        slot = self._check(lib().epic_engine_add(self._e, ctypes.byref(c)), "add")
+        self.specs[slot] = c
+        return slot
+
+    def write_nc(self, slot, path, case=None):
+        """The member's state as an epic.nc file; the header from `case`
+        (default: the case the member was added from, built again from its
+        spec)."""
+        c, built = case, False
+        if c is None:
+            c = self.specs[slot]
+            if isinstance(c, CaseSpec):
+                c, built = build_case(c), True
+        try:
+            self._check(lib().epic_engine_write_nc(self._e, slot, ctypes.byref(c), str(path).encode()),
+                        "write_nc")
+        finally:
+            if built:
+                free_case(c)
 
     def remove(self, slot):
         self._check(lib().epic_engine_remove(self._e, slot), "remove")

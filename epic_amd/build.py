@@ -21,7 +21,7 @@ C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off"]
 
 HIP_SOURCES = ["k_diag.hip", "k_dyn.hip", "k_filter.hip", "k_adv.hip", "k_mixed.hip", "engine.cpp", "dropin.cpp",
                "noalloc.cpp"]
-C_SOURCES = ["epic_case.c", "epic_tables.c"]
+C_SOURCES = ["epic_case.c", "epic_tables.c", "ncio.c", "epic_nc.c"]
 
 
 def _run(cmd):

@@ -1,0 +1,15 @@
+/* case_internal.h -- the parts of the case builder (epic_case.c) that the
+ * epic.nc reader (epic_nc.c) shares. */
+#ifndef EPIC_CASE_INTERNAL_H
+#define EPIC_CASE_INTERNAL_H
+#include "../../include/epic_amd.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+void epic_case_alloc(epic_case *c);
+void epic_case_metrics(epic_case *c);
+int  epic_case_dsgth(epic_case *c);
+#ifdef __cplusplus
+}
+#endif
+#endif

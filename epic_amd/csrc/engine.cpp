@@ -595,21 +595,26 @@ int engine_add_member(epic_engine *E, const epic_case *c, const double *hdry_src
     for (int J = 0; J <= c->nj; J++)
       for (int I = 1; I <= c->ni; I++) t1[IX(K, J, I)] = 1.7976931348623157e308;
   HIPCHK(hipMemcpyAsync(d.HDRY, h.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
-  std::vector<float> t1f;
-  if (mixed) {
-    t1f.assign(N3, 0.f);
-    for (size_t i = 0; i < N3; i++)
-      if (t1[i] != 0.) t1f[i] = 3.40282347e+38f;
-  }
-  for (int p = 1; p <= 2; p++) {
-    if (mixed) {
-      HIPCHK(hipMemcpyAsync(d.DUDTf + p * N3, t1f.data(), sizeof(float) * N3, hipMemcpyHostToDevice, E->stream));
-      HIPCHK(hipMemcpyAsync(d.DVDTf + p * N3, t1f.data(), sizeof(float) * N3, hipMemcpyHostToDevice, E->stream));
-    } else {
-      HIPCHK(hipMemcpyAsync(d.DUDT + p * N3, t1.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
-      HIPCHK(hipMemcpyAsync(d.DVDT + p * N3, t1.data(), sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
+  // Adams-Bashforth planes 1 and 2 (IT_MINUS1, IT_MINUS2): a restart case's
+  // history, else the FLOAT_MAX start-up flags; FP32 copies in the mixed
+  // mode, the flag as FP32's largest value
+  std::vector<float> hf[4];
+  for (int p = 1; p <= 2; p++)
+    for (int w = 0; w < 2; w++) {
+      const double *hist = w ? c->dvdt_hist : c->dudt_hist;
+      const double *src = hist ? hist + (size_t)(p - 1) * N3 : t1.data();
+      double *dst = (w ? d.DVDT : d.DUDT) + p * N3;
+      if (mixed) {
+        std::vector<float> &b = hf[2 * (p - 1) + w];
+        b.assign(N3, 0.f);
+        for (size_t i = 0; i < N3; i++)
+          b[i] = (src[i] == 1.7976931348623157e308) ? 3.40282347e+38f : (float)src[i];
+        HIPCHK(hipMemcpyAsync((w ? d.DVDTf : d.DUDTf) + p * N3, b.data(), sizeof(float) * N3, hipMemcpyHostToDevice,
+                              E->stream));
+      } else {
+        HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * N3, hipMemcpyHostToDevice, E->stream));
+      }
     }
-  }
   MemberDyn dy;
   memset(&dy, 0, sizeof(dy));
   dy.it0 = 0; dy.itm1 = 1; dy.itm2 = 2; dy.itime = 0;
@@ -935,6 +940,47 @@ std::vector<Ckpt> ckpt_arrays(const MemberDesc &d) {
 }
 const char CKPT_MAGIC[8] = {'E', 'P', 'I', 'C', 'A', 'M', 'D', '1'};
 }  // namespace
+
+// epic.nc of one member: its state from the device, the header from its case
+extern "C" int epic_engine_write_nc(epic_engine *E, int slot, const epic_case *c, const char *path) {
+  if (!E || !c || !path || slot < 0 || slot >= E->max_members || !E->m[slot].active) return -1;
+  const MemberDesc &hd = E->m[slot].h;
+  if (hd.ni != c->ni || hd.nj != c->nj || hd.nk != c->nk) {
+    snprintf(E->err, sizeof(E->err), "the case's grid is not the member's");
+    return -1;
+  }
+  if (epic_engine_sync(E) != 0) return -1;
+  const size_t N3 = (size_t)hd.N3;
+  std::vector<double> u(N3), v(N3), p3(N3), th(N3), gzs((size_t)hd.N2), tend(4 * N3);
+  int planes[4];
+  if (epic_engine_read(E, slot, EPIC_F_U, u.data()) || epic_engine_read(E, slot, EPIC_F_V, v.data()) ||
+      epic_engine_read(E, slot, EPIC_F_P3, p3.data()) || epic_engine_read(E, slot, EPIC_F_THETA, th.data()) ||
+      epic_engine_read(E, slot, EPIC_F_GZ_SURFACE, gzs.data()) || epic_engine_time_planes(E, slot, planes))
+    return -1;
+  // the planes IT_MINUS1 and IT_MINUS2 of u and v; FP32's largest value
+  // (the mixed mode's start-up flag) back to FLOAT_MAX
+  for (int w = 0; w < 2; w++)
+    for (int q = 0; q < 2; q++) {
+      double *dst = tend.data() + (size_t)(2 * w + q) * N3;
+      if (epic_engine_read(E, slot, (w ? EPIC_F_DVDT0 : EPIC_F_DUDT0) + planes[1 + q], dst)) return -1;
+      if (hd.mixed)
+        for (size_t i = 0; i < N3; i++)
+          if (dst[i] == (double)3.40282347e+38f) dst[i] = 1.7976931348623157e308;
+    }
+  epic_nc_state st;
+  memset(&st, 0, sizeof(st));
+  st.u = u.data(); st.v = v.data(); st.p3 = p3.data(); st.theta = th.data();
+  st.u_spinup = c->u_spinup; st.gz_surface = gzs.data();
+  st.dudt_m1 = tend.data(); st.dudt_m2 = tend.data() + N3;
+  st.dvdt_m1 = tend.data() + 2 * N3; st.dvdt_m2 = tend.data() + 3 * N3;
+  st.model_time = c->model_time + (double)E->m[slot].steps * (double)c->dt;
+  st.l_s_deg = 0.;
+  if (epic_nc_write(path, c, &st)) {
+    snprintf(E->err, sizeof(E->err), "%s", epic_nc_error());
+    return -1;
+  }
+  return 0;
+}
 
 extern "C" int epic_engine_checkpoint(epic_engine *E, int slot, const char *path) {
   if (!E || slot < 0 || slot >= E->max_members || !E->m[slot].active || !path) return -1;

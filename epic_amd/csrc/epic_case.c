@@ -26,6 +26,7 @@
 #include <string.h>
 
 #include "../../include/epic_amd.h"
+#include "case_internal.h"
 
 #define DEGR (M_PI / 180.)
 #define LAT0_DEG 45.
@@ -116,11 +117,13 @@ void epic_case_free(epic_case *c) {
                      &c->sigmatheta, &c->dsgth, &c->dsgth_inv, &c->p_ref, &c->theta_ref,
                      &c->h_min, &c->t_sponge_inv, &c->u, &c->v, &c->hdry, &c->theta,
                      &c->p3, &c->u_spinup, &c->gz_surface, &c->pdat, &c->tdat, &c->t_cool_x,
-                     &c->t_cool_y};
+                     &c->t_cool_y, &c->dudt_hist, &c->dvdt_hist};
   for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); i++) {
     free(*arrs[i]);
     *arrs[i] = NULL;
   }
+  epic_nc_header_free(c->nc_header);
+  c->nc_header = NULL;
 }
 
 /* set_lonlat (src/shared/epic_funcs_diag.c:1324-1415), globe branch */
@@ -248,6 +251,62 @@ static double solve_level(const hyb *h, const profile *pr, double latdeg, double
   return exp(.5 * (a + b));
 }
 
+/* The arrays of a case whose extents, k_sponge and dt are set (zeroed), and
+ * the sponge rates of set_sponge (src/shared/epic_funcs_diag.c:1696-1731):
+ * relax u to its initial value and v to zero, strongest (1/(30 dt)) in the
+ * top layer. */
+void epic_case_alloc(epic_case *c) {
+  const int ni = c->ni, nj = c->nj, nk = c->nk;
+  const int nJ = 2 * (nj + 1) + 1, nK = 2 * (nk + 1) + 1;
+  c->Iadim = ni + 2;
+  c->Jadim = nj + 3;
+  c->Kadim = nk + 2;
+  c->Nelem2d = c->Iadim * c->Jadim;
+  c->Nelem3d = c->Nelem2d * c->Kadim;
+  c->lon = zalloc(2 * (ni + 1) + 1);
+  c->lat = zalloc(nJ);
+  c->rln = zalloc(nJ);
+  c->rlt = zalloc(nJ);
+  c->f = zalloc(nJ);
+  c->m = zalloc(nJ);
+  c->n = zalloc(nJ);
+  c->mn = zalloc(nJ);
+  c->g = zalloc(nJ);
+  c->sigmatheta = zalloc(nK);
+  c->dsgth = zalloc(nK);
+  c->dsgth_inv = zalloc(nK);
+  c->p_ref = zalloc(nK);
+  c->theta_ref = zalloc(nK);
+  c->h_min = zalloc(nk + 2);
+  c->t_sponge_inv = zalloc(c->k_sponge + 1);
+  if (c->k_sponge > 0) {
+    const double t0_inv = 1. / (30. * (double)c->dt);
+    for (int K = 0; K <= c->k_sponge; K++) {
+      const double tmp = (double)(c->k_sponge - K) / (c->k_sponge);
+      c->t_sponge_inv[K] = t0_inv * .5 * (1. - cos(M_PI * tmp));
+    }
+  }
+  const size_t N3 = (size_t)c->Nelem3d;
+  c->u = zalloc(N3);
+  c->v = zalloc(N3);
+  c->hdry = zalloc(N3);
+  c->theta = zalloc(N3);
+  c->p3 = zalloc(N3);
+  c->u_spinup = zalloc(N3);
+  c->gz_surface = zalloc(c->Nelem2d);
+}
+
+/* lon/lat, the map factors and Coriolis parameter, gravity (set_lonlat,
+ * set_fmn, set_gravity) from the grid and planet scalars */
+void epic_case_metrics(epic_case *c) {
+  build_lonlat(c);
+  build_fmn(c);
+  build_gravity(c);
+}
+
+/* dsgth and its inverse from sigmatheta (set_dsgth); 0 or -3 */
+int epic_case_dsgth(epic_case *c) { return build_dsgth(c) ? -3 : 0; }
+
 int epic_case_build(const epic_case_spec *s, epic_case *c) {
   memset(c, 0, sizeof(*c));
   if (s->ni < 8 || (s->ni & (s->ni - 1)) || s->nj < 4 || s->nk < 4 || s->dt <= 0) return -1;
@@ -256,11 +315,6 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   c->nj = nj;
   c->nk = nk;
   c->dt = s->dt;
-  c->Iadim = ni + 2;
-  c->Jadim = nj + 3;
-  c->Kadim = nk + 2;
-  c->Nelem2d = c->Iadim * c->Jadim;
-  c->Nelem3d = c->Nelem2d * c->Kadim;
 
   /* ---- planet constants ---- */
   double gsurf, ptop, pbot, n2 = 0., theta_b = 0.;
@@ -397,44 +451,8 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   }
 
   /* ---- arrays ---- */
-  int nJ = 2 * (nj + 1) + 1, nK = 2 * (nk + 1) + 1;
-  c->lon = zalloc(2 * (ni + 1) + 1);
-  c->lat = zalloc(nJ);
-  c->rln = zalloc(nJ);
-  c->rlt = zalloc(nJ);
-  c->f = zalloc(nJ);
-  c->m = zalloc(nJ);
-  c->n = zalloc(nJ);
-  c->mn = zalloc(nJ);
-  c->g = zalloc(nJ);
-  c->sigmatheta = zalloc(nK);
-  c->dsgth = zalloc(nK);
-  c->dsgth_inv = zalloc(nK);
-  c->p_ref = zalloc(nK);
-  c->theta_ref = zalloc(nK);
-  c->h_min = zalloc(nk + 2);
-  c->t_sponge_inv = zalloc(c->k_sponge + 1);
-  /* set_sponge (src/shared/epic_funcs_diag.c:1696-1731): relax u to its
-   * initial value and v to zero, strongest (1/(30 dt)) in the top layer */
-  if (c->k_sponge > 0) {
-    const double t0_inv = 1. / (30. * (double)c->dt);
-    for (int K = 0; K <= c->k_sponge; K++) {
-      const double tmp = (double)(c->k_sponge - K) / (c->k_sponge);
-      c->t_sponge_inv[K] = t0_inv * .5 * (1. - cos(M_PI * tmp));
-    }
-  }
-  size_t N3 = (size_t)c->Nelem3d;
-  c->u = zalloc(N3);
-  c->v = zalloc(N3);
-  c->hdry = zalloc(N3);
-  c->theta = zalloc(N3);
-  c->p3 = zalloc(N3);
-  c->u_spinup = zalloc(N3);
-  c->gz_surface = zalloc(c->Nelem2d);
-
-  build_lonlat(c);
-  build_fmn(c);
-  build_gravity(c);
+  epic_case_alloc(c);
+  epic_case_metrics(c);
 
   /* ---- reference profile ---- */
   profile pr;

@@ -269,6 +269,15 @@ typedef struct {
    * the case has no such cooling. */
   int    ntp, n_t_cool;
   double *pdat, *tdat, *t_cool_x, *t_cool_y;
+  /* restart state (a case read from an epic.nc file; null and zero for a
+   * built case): the u and v Adams-Bashforth tendency planes IT_MINUS1 and
+   * IT_MINUS2 (two Nelem3d planes each, in that order; null: the start-up
+   * FLOAT_MAX flags), the model and start times [s] (var.model_time,
+   * var.start_time), and the file's global attributes, which a case
+   * written back carries unchanged where it has no value of its own */
+  double *dudt_hist, *dvdt_hist;
+  double model_time, start_time;
+  void   *nc_header;
 } epic_case;
 
 /* Build a case from a spec: metric arrays (set_lonlat/set_fmn/set_gravity/
@@ -342,6 +351,41 @@ int  epic_engine_checkpoint(epic_engine *e, int slot, const char *path);
 int  epic_engine_restore(epic_engine *e, const epic_case *c, const char *path);
 /* epic_case_build() + epic_engine_restore() + epic_case_free() in one call. */
 int  epic_engine_restore_spec(epic_engine *e, const epic_case_spec *spec, const char *path);
+
+/* ---- epic.nc: the reference's model-state file (netCDF-3) ----------------
+ * The layout var_write writes and var_read reads for ALL_DATA
+ * (src/shared/epic_funcs_io.c:730-2257): the planet and grid scalars as
+ * global attributes; u, v, p3, theta, u_spinup at one time record; the
+ * Adams-Bashforth tendency planes du_IT_MINUS1dt, du_IT_MINUS2dt,
+ * dv_IT_MINUS1dt, dv_IT_MINUS2dt; gz_surface on terrestrial planets.
+ *
+ * epic_nc_read: a case from a file, its metric arrays formed from the
+ * scalars as the reference's set_lonlat/set_fmn/set_gravity/set_dsgth/
+ * set_sponge form them, the tendency planes in dudt_hist/dvdt_hist and
+ * model_time/start_time set; epic_engine_add() then continues the run
+ * (HDRY formed from P3 as epic_main forms it).  Returns 0, -1 unreadable,
+ * -2 not a netCDF classic file, -3 an attribute or variable missing or
+ * malformed, -4 a configuration the GPU step does not implement
+ * (epic_nc_error() says which).  Free the case with epic_case_free().
+ *
+ * epic_nc_write: the state of one member (padded Nelem3d arrays; null
+ * tendency planes are written as the start-up FLOAT_MAX flags) with the
+ * header of its case; the global attributes of the file a case was read
+ * from pass through where the case has no value of its own.
+ * epic_engine_write_nc: the same for an engine member, its state read from
+ * the device after a sync; model_time is the case's plus the member's steps
+ * times dt. */
+typedef struct {
+  const double *u, *v, *p3, *theta, *u_spinup, *gz_surface;
+  const double *dudt_m1, *dudt_m2, *dvdt_m1, *dvdt_m2;
+  double model_time;  /* s, var.model_time */
+  double l_s_deg;     /* solar longitude L_s, degrees */
+} epic_nc_state;
+int  epic_nc_read(const char *path, epic_case *out);
+int  epic_nc_write(const char *path, const epic_case *c, const epic_nc_state *state);
+int  epic_engine_write_nc(epic_engine *e, int slot, const epic_case *c, const char *path);
+const char *epic_nc_error(void);
+void epic_nc_header_free(void *header);
 /* Autotuner (no counterpart in the reference, whose loops have no launch
  * shapes): for each bucket, time every candidate launch shape of the tunable
  * kernels (columns per column tile of set_p2etc, gz3, calc_w and the

@@ -1764,12 +1764,29 @@ int cpuref_step(cpuref *R) {
 }
 
 /* ------------------------------------------------------------ lifetime */
+static cpuref *finish_create(cpuref *R);
+
 cpuref *cpuref_create(const epic_case_spec *spec) {
   cpuref *R = (cpuref *)calloc(1, sizeof(cpuref));
   if (epic_case_build(spec, &R->c)) {
     free(R);
     return NULL;
   }
+  return finish_create(R);
+}
+
+/* the state of an epic.nc file, its Adams-Bashforth history included, as
+ * the reference's epic_main takes it up (var_read, then HDRY from P3) */
+cpuref *cpuref_create_nc(const char *path) {
+  cpuref *R = (cpuref *)calloc(1, sizeof(cpuref));
+  if (epic_nc_read(path, &R->c)) {
+    free(R);
+    return NULL;
+  }
+  return finish_create(R);
+}
+
+static cpuref *finish_create(cpuref *R) {
   epic_case *c = &R->c;
   R->ni = c->ni; R->nj = c->nj; R->nk = c->nk;
   R->Ia = c->Iadim; R->N2 = c->Nelem2d; R->N3 = c->Nelem3d;
@@ -1880,12 +1897,20 @@ cpuref *cpuref_create(const epic_case_spec *spec) {
       for (int I = 1; I <= R->ni; I++) A3(R->HDRY, K, J, I) = h_layer(R, K, J, I);
   bc3(R, R->HDRY);
   R->it0 = 0; R->itm1 = 1; R->itm2 = 2;
-  for (int K = 1; K <= R->nk; K++)
-    for (int J = 0; J <= R->nj; J++)
-      for (int I = 1; I <= R->ni; I++) {
-        A3(R->DUDT[2], K, J, I) = A3(R->DUDT[1], K, J, I) = DBL_MAX;
-        A3(R->DVDT[2], K, J, I) = A3(R->DVDT[1], K, J, I) = DBL_MAX;
-      }
+  if (c->dudt_hist && c->dvdt_hist) {
+    /* IT_MINUS1 and IT_MINUS2 from the file (var_read) */
+    for (int p = 1; p <= 2; p++) {
+      memcpy(R->DUDT[p], c->dudt_hist + (size_t)(p - 1) * N3, sizeof(double) * N3);
+      memcpy(R->DVDT[p], c->dvdt_hist + (size_t)(p - 1) * N3, sizeof(double) * N3);
+    }
+  } else {
+    for (int K = 1; K <= R->nk; K++)
+      for (int J = 0; J <= R->nj; J++)
+        for (int I = 1; I <= R->ni; I++) {
+          A3(R->DUDT[2], K, J, I) = A3(R->DUDT[1], K, J, I) = DBL_MAX;
+          A3(R->DVDT[2], K, J, I) = A3(R->DVDT[1], K, J, I) = DBL_MAX;
+        }
+  }
   R->itime = 0;
   return R;
 }

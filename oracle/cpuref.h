@@ -13,6 +13,8 @@ typedef struct cpuref cpuref;
 
 /* Build the case from the spec (epic_case_build) and take ownership. */
 cpuref *cpuref_create(const epic_case_spec *spec);
+/* the state of an epic.nc file (epic_nc_read), Adams-Bashforth history included */
+cpuref *cpuref_create_nc(const char *path);
 void    cpuref_destroy(cpuref *r);
 /* timestep(SYNC_DIAGS_ONLY) and timestep(STEP_PROGS_AND_SYNC_DIAGS) */
 int     cpuref_sync(cpuref *r);

@@ -28,6 +28,8 @@ def lib(nudged=False):
         vp = ctypes.c_void_p
         L.cpuref_create.restype = vp
         L.cpuref_create.argtypes = [vp]
+        L.cpuref_create_nc.restype = vp
+        L.cpuref_create_nc.argtypes = [ctypes.c_char_p]
         for f in ("cpuref_sync", "cpuref_step", "cpuref_error"):
             getattr(L, f).argtypes = [vp]
         L.cpuref_destroy.argtypes = [vp]
@@ -59,10 +61,15 @@ def libm_stats(variant="cr", reset=True):
 class CpuRef:
     """One integration on the CPU oracle, same field order as epic_amd.FIELDS."""
 
-    def __init__(self, sp, nudged=False):
+    def __init__(self, sp, nudged=False, nc_path=None):
+        """sp: a case spec; or, with nc_path, the state of that epic.nc file
+        (sp then only needs ni, nj, nk)."""
         self.sp = sp
         self._L = lib(nudged)
-        self._r = self._L.cpuref_create(ctypes.addressof(sp))
+        if nc_path is not None:
+            self._r = self._L.cpuref_create_nc(str(nc_path).encode())
+        else:
+            self._r = self._L.cpuref_create(ctypes.addressof(sp))
         if not self._r:
             raise RuntimeError("cpuref_create failed")
         self.shape3 = (sp.nk + 2, sp.nj + 3, sp.ni + 2)
