@@ -59,7 +59,12 @@ WORKLOADS = {
 # unity, theta and hdry; the 3-D filter U, V, hdry and theta; the
 # hyperviscosity kernels two fields).
 KERNEL_BYTES = {
-    "k_set_p2etc": (56, 1), "k_thermo": (88, 1), "k_gz3": (40, 1), "k_calc_w": (56, 1), "k_dzdt3": (72, 1),
+    # set_p2etc<1> reads h, theta and writes p3, h, p2, h3, theta2, theta;
+    # <0> also forms T3 and RHO3: reads h, theta, writes p3, h, t3, rho3;
+    # k_thermo reads theta, p3, p2, theta2 and writes t3, rho3, exner3, t2,
+    # rho2, heat3, dzdt3
+    "k_set_p2etc<1>": (56, 1), "k_set_p2etc<0>": (48, 1), "k_thermo": (88, 1),
+    "k_gz3": (40, 1), "k_calc_w": (56, 1), "k_dzdt3": (72, 1),
     "k_hsu_vert": (24, 3), "k_hsu_mer": (40, 3),
     "k_hsu_zonal": (32, 3), "k_fft_filter": (16, 3),
     "k_filter3": (24, 4), "k_zonal_hv": (16, 2), "k_d2dy2x2": (16, 2), "k_hv_apply": (24, 2),
@@ -67,7 +72,7 @@ KERNEL_BYTES = {
     "k_divdamp": (40, 1), "km_uvh_kin": (36, 1), "km_dvdt_core": (32, 1), "km_dudt_core": (32, 1), "km_ab": (64, 1),
     "km_divdamp": (40, 1), "k_gz_mono": (16, 1),
     "k_zero_tend": (16, 1), "k_adv_elem": (16, 1), "k_heating": (24, 1), "k_sponge": (24, 1), "k_hs_pbl": (32, 1),
-    "k_divergence": (24, 1), "k_vort_interior": (32, 1),
+    "k_divergence": (24, 1), "k_store_diag": (40, 1),
 }
 
 
@@ -101,9 +106,12 @@ def filtered_rows(nj):
 def kernel_bytes(name, ni, nj, nk, members):
     """Algorithmic bytes of one launch of `name` (None if not tabulated)."""
     base = name.split("<")[0]  # template arguments do not change the bytes
-    if base not in KERNEL_BYTES:
+    if name in KERNEL_BYTES:  # ... except where the table names them
+        per_cell, fields = KERNEL_BYTES[name]
+    elif base in KERNEL_BYTES:
+        per_cell, fields = KERNEL_BYTES[base]
+    else:
         return None
-    per_cell, fields = KERNEL_BYTES[base]
     if base == "k_fft_filter":
         cells = ni * filtered_rows(nj) * nk
     else:

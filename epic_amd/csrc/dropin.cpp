@@ -265,7 +265,7 @@ extern "C" void timestep(struct epic_planet_abi *planet, int action, double **Bu
     memcpy(g_di.planes, planes, sizeof(planes));
   } else {
     if (epic_engine_step(g_di.E, 1) != 0) fail(engine_errbuf(g_di.E));
-    // the step's k_rotate: (it0, itm1, itm2) <- (itm2, it0, itm1), itime + 1
+    // the step's closing rotation: (it0, itm1, itm2) <- (itm2, it0, itm1), itime + 1
     const int rot[4] = {planes[2], planes[0], planes[1], planes[3] + 1};
     memcpy(g_di.planes, rot, sizeof(rot));
   }

@@ -201,23 +201,21 @@ static int rebuild_buckets(epic_engine *E) {
 // writes every output again, so the first forms those only and leaves out
 // P2, H3, EXNER3, T2, RHO2 and DZDT3 -- the step's results are the same.
 static void rec_p2etc_pgrad_vars(hipStream_t s, const Bucket &b, int full) {
-  L_set_p2etc(s, b, full);
-  L_thermo(s, b, full);
+  L_set_p2etc(s, b, full);  // full = 0: with store_pgrad_vars' T3, RHO3
+  if (full) L_thermo(s, b);
   L_gz_from_u(s, b);
   L_gz3(s, b);
   if (full) L_dzdt3(s, b);
 }
 
 // store_diag (src/shared/epic_funcs_diag.c:3963-4053); the filtered copies
-// of V and U are formed row by row inside the PV and divergence kernels
-static void rec_store_diag(hipStream_t s, const Bucket &b) {
-  L_vorticity(s, b, F_U, F_V, 0, b.nk);
-  L_divergence(s, b, F_U, F_V, F_DIV, 0, b.nk);
-}
+// of V and U are formed row by row inside the PV and divergence blocks.
+// rotate: the step's closing rotation of the time planes rides along.
+static void rec_store_diag(hipStream_t s, const Bucket &b, int rotate) { L_store_diag(s, b, rotate); }
 
 static void rec_sync(hipStream_t s, const Bucket &b, bool adjust) {
   rec_p2etc_pgrad_vars(s, b, 1);
-  rec_store_diag(s, b);
+  rec_store_diag(s, b, 0);
   L_heating(s, b, adjust, false);
   L_calc_w(s, b, 0);
 }
@@ -225,8 +223,7 @@ static void rec_sync(hipStream_t s, const Bucket &b, bool adjust) {
 // timestep(STEP_PROGS_AND_SYNC_DIAGS) in the order of
 // src/shared/epic_timestep.c:104-327
 static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool adjust) {
-  L_zero_tend(s, b);
-  L_calc_w(s, b, 1);  // source_sink + calc_w
+  L_calc_w(s, b, 1);  // source_sink + calc_w, and the step's tendency planes cleared
   L_uv_core(s, b);
   L_uv_drag(s, b);
   // hdry and theta take their hyperviscosity together (neither reads the
@@ -255,9 +252,10 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
     L_merid_hv(s, b, uv);
     L_zonal_hv(s, b, uv);
   }
-  rec_store_diag(s, b);
-  L_heating(s, b, adjust, true);  // the relaxation itself was formed in k_thermo
-  L_rotate(s, b);
+  // store_diag, then the time planes rotated (in the same launch: nothing
+  // after the Adams-Bashforth update reads them)
+  rec_store_diag(s, b, 1);
+  L_heating(s, b, adjust, true);  // the relaxation itself was formed with T3 (thermo_cell)
 }
 
 void member_record(hipStream_t s, const Bucket &b, bool sync_only, bool hv, bool divd, bool adjust) {

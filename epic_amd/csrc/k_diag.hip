@@ -6,7 +6,7 @@
 //   calc_heating      src/shared/epic_funcs_diag.c:4793-4869 +
 //                     src/shared/epic_sensible_heating.c:51-463 (Held-Suarez,
 //                     and the hot Jupiter's day-night relaxation; inside the
-//                     step it is formed by the second k_thermo)
+//                     step it is formed with T3 by thermo_cell)
 //   calc_w            src/shared/epic_flux.c:123-335           (column scan)
 //   vorticity         src/shared/epic_funcs_diag.c:4131-4445
 //   divergence        src/shared/epic_funcs_diag.c:4064-4117
@@ -15,6 +15,8 @@
 // the K recurrences in one lane per column); zonal means stage the row in
 // LDS and sum it in one lane in the reference's order.  The arithmetic of
 // each expression follows the reference's operation order.
+#include <algorithm>
+
 #include "kernels.h"
 #include "filter_row.h"
 
@@ -117,11 +119,14 @@ static inline dim3 tile_grid(const Bucket &b, int C) {
 // P3 from the top down (prefix of dsgth*g*h), the sigma-region pressures,
 // h = max(h_min, dp/(g dsgth)), and the layer diagnostics P2, H3, THETA2
 // (and the theta of the sigma region when update_theta)
+// The FULL = 0 launch then forms store_pgrad_vars' T3 and RHO3 of its
+// columns (thermo_cell).
 // FULL = 1: set_p2etc(UPDATE_THETA) with P2, H3, THETA2.  FULL = 0: the
 // call that follows the advection inside the step forms P3 and h (and the
 // pressure-range check) only: the P2 and H3 it would form are written again
 // by the step's second set_p2etc before anything reads them, so the values
 // the step leaves are the same.
+__device__ __forceinline__ void thermo_cell(const MemberDesc &d, int K, int J, int I, bool full, double th, double p);
 template <int FULL>
 __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, int C) {
   constexpr bool update_theta = FULL;
@@ -231,6 +236,12 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
     d.H3[q0] = aux[1];
     if (update_theta) d.THETA2[ix3(d, ks - 1, J, I)] = .5 * (d.THETA[ix3(d, ks - 1, J, I)] + aux[0]);
   }
+  // FULL = 0: store_pgrad_vars' per-cell part (T3, RHO3) on the column's
+  // levels from the P3 in LDS.  FULL = 1 leaves its larger per-cell part to
+  // k_thermo, whose one lane per cell hides the latency of its pow chains
+  // better than the tile's few lanes per column.
+  if (!FULL && T.live)
+    for (int K = T.kk; K <= nk; K += T.KS) thermo_cell(d, K, J, I, false, d.THETA[ix3(d, K, J, I)], a[K]);
 }
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
   const int C = b.knob[KN_COLS_P2ETC] ? b.knob[KN_COLS_P2ETC] : tile_cols(b, 2);
@@ -253,21 +264,10 @@ __device__ __forceinline__ double hj_relax(const MemberDesc &d, int J, int I, do
 // nothing changes between here and the end of the step, where the reference
 // forms it, and it takes the p^kappa formed here for T3 (a pad takes the
 // value of the column it copies)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_thermo(const MemberDesc *D, int mb, int full) {
-  MEMBER;
-  if (full && d.dyn->itime <= 1) {
-    // no lane of this kernel writes DZDT3 while itime <= 1
-    const size_t nthr = (size_t)gridDim.x * gridDim.y * blockDim.x;
-    for (size_t i = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; i < (size_t)d.N3;
-         i += nthr)
-      d.DZDT3[i] = 0.;
-  }
-  FLAT2(0, d.nj + 1, 0, d.ni + 2);
-  const int K = blockIdx.y;
+__device__ __forceinline__ void thermo_cell(const MemberDesc &d, int K, int J, int I, bool full, double th, double p) {
   const size_t q = ix3(d, K, J, I);
   if (full && K >= 1 && d.dyn->itime > 1 && I >= 1 && I <= d.ni) d.DZDT3[q] = d.GZ3[q];
   const double R_GAS = 8.314472e+3;
-  double th = d.THETA[q], p = d.P3[q];
   const double pk = crm_pow(p / d.press0, d.kappa);
   double t = th * pk;
   double rho3 = p * d.mu3 / (R_GAS * t);
@@ -302,8 +302,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     d.T2[q] = t;
   }
 }
-void L_thermo(hipStream_t s, const Bucket &b, int full) {
-  LAUNCH(k_thermo, grid2(b.nj + 1, b.ni + 2, b.nk + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb, full);
+
+// store_pgrad_vars' per-cell part with full = 1, one lane per (K,J,I),
+// K = 0..nk, J = 0..nj, I = 0..ni+1 (after the FULL set_p2etc)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_thermo(const MemberDesc *D, int mb) {
+  MEMBER;
+  if (d.dyn->itime <= 1) {
+    // no lane of this kernel writes DZDT3 while itime <= 1
+    const size_t nthr = (size_t)gridDim.x * gridDim.y * blockDim.x;
+    for (size_t i = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; i < (size_t)d.N3;
+         i += nthr)
+      d.DZDT3[i] = 0.;
+  }
+  FLAT2(0, d.nj + 1, 0, d.ni + 2);
+  const int K = blockIdx.y;
+  const size_t q = ix3(d, K, J, I);
+  thermo_cell(d, K, J, I, true, d.THETA[q], d.P3[q]);
+}
+void L_thermo(hipStream_t s, const Bucket &b) {
+  LAUNCH(k_thermo, grid2(b.nj + 1, b.ni + 2, b.nk + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
 // ------------------------------------------------------------ RI2
@@ -377,9 +394,11 @@ __device__ double kin_uniform(const double *mn, const double *mni, int nj, const
 }
 
 // Gas giants: GZ_SURFACE from gradient balance with the zonal-mean bottom
-// wind, in three launches: the zonal means of u, p, rho on the bottom
+// wind, in two launches: the zonal means of u, p, rho on the bottom
 // interface (one wavefront per row; lanes 0-2 sum in the reference's
-// sequential order), the meridional integral (one block), the fills.
+// sequential order), the meridional integral (one block); the column
+// kernel k_gz3 that must follow copies the results into GZ_SURFACE and
+// PV3's plane nk+1 as it starts each column.
 __global__ void __launch_bounds__(256) k_gzu_means(const MemberDesc *D, int mb) {
   MEMBER;
   if (!d.is_gas) return;
@@ -497,21 +516,12 @@ __global__ void __launch_bounds__(256) k_gzu_core(const MemberDesc *D, int mb) {
     if (J <= nj) d.rowbuf[4 * R + J] = bern[J] - kn[J];
   }
 }
-// PV3 plane nk+1 holds the absolute vorticity of the uniform rows; GZ_SURFACE
-__global__ void __launch_bounds__(256) k_gzu_fill(const MemberDesc *D, int mb) {
-  MEMBER;
-  if (!d.is_gas) return;
-  FLAT2(0, d.nj + 2, 0, d.ni + 2);
-  const int R = d.nj + 3;
-  d.PV3[(size_t)(d.nk + 1) * d.N2 + ix2(d, J, I)] = d.rowbuf[3 * R + J];
-  if (J <= d.nj) d.GZS[ix2(d, J, I)] = d.rowbuf[4 * R + J];
-}
 void L_gz_from_u(hipStream_t s, const Bucket &b) {
   if (!b.any_gas) return;
   LAUNCH(k_gzu_means, dim3(b.nj + 1, 1, b.nm), dim3(64), sizeof(double) * 3 * b.ni, s, b.D, b.mb);
   LAUNCH(k_gzu_core, dim3(1, 1, b.nm), dim3(256), sizeof(double) * (7 * (b.nj + 3) + 5 * (2 * b.nj + 3)), s, b.D,
          b.mb);
-  LAUNCH(k_gzu_fill, grid2(b.nj + 2, b.ni + 2, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  // GZ_SURFACE and PV3's plane nk+1 are filled in by the k_gz3 that follows
 }
 
 // hydrostatic GZ3 and MONT3 from the bottom up (Leslie-Purser 4-point
@@ -525,6 +535,23 @@ __global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb, int C)
   const double *P = d.P3, *RH = d.RHO3;
 #define PP(k) P[ix3(d, k, J, I)]
 #define RR(k) RH[ix3(d, k, J, I)]
+  // the column's surface geopotential, read ahead by the lane that runs the
+  // chain; on a gas giant gz_from_u's results for this column: GZ_SURFACE,
+  // and plane nk+1 of PV3 (the absolute vorticity of the uniform rows; row
+  // nj+1 by the columns of row nj)
+  double gs = 0.;
+  if (T.live && T.kk == 0) {
+    if (d.is_gas) {
+      const int R = d.nj + 3;
+      gs = d.rowbuf[4 * R + J];
+      d.GZS[ix2(d, J, I)] = gs;
+      double *pv = d.PV3 + (size_t)(d.nk + 1) * d.N2;
+      pv[ix2(d, J, I)] = d.rowbuf[3 * R + J];
+      if (J == d.nj) pv[ix2(d, J + 1, I)] = d.rowbuf[3 * R + J + 1];
+    } else {
+      gs = d.GZS[ix2(d, J, I)];
+    }
+  }
   // lg[k] = log(P3(k)/P3(k-1)): every ln-p interval the integral uses, once
   if (T.live)
     for (int K = T.kk; K <= nk; K += T.KS) {
@@ -556,7 +583,7 @@ __global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb, int C)
   if (T.live && T.kk == 0) {
     const double cpr = d.cpr, rgas = d.rgas;
     double *G = d.GZ3, *M = d.MONT3;
-    double gprev = d.topo_scale * d.GZS[ix2(d, J, I)];
+    double gprev = d.topo_scale * gs;
     G[ix3(d, nk, J, I)] = gprev;
     M[ix3(d, nk, J, I)] = gprev + cpr * tt[nk] * rgas;
     const bool inner = (I >= 1 && I <= d.ni);
@@ -728,7 +755,7 @@ __global__ void __launch_bounds__(256) k_heat_adjust_apply(const MemberDesc *D, 
   const size_t q = ix3(d, K, J, I);
   d.HEAT3[q] = 0. + (d.HEAT3[q] - d.sD[K]);
 }
-// in_thermo: the relaxation was formed by this step's k_thermo(full = 1),
+// in_thermo: the relaxation was formed by this step's thermo_cell (full = 1),
 // and every other cell of HEAT3 is zero (a member's blob is cleared when it
 // is created, a drop-in upload brings the caller's array, which the
 // reference zeroes there, and no kernel writes outside K = 1..nk, J = 0..nj),
@@ -751,8 +778,21 @@ void L_heating(hipStream_t s, const Bucket &b, int adjust, bool in_thermo) {
 // SUM_DIV in one lane per column.  SUM_DIV(1) and SUM_DIV(nk) are zero (the
 // reference's top/bottom statements write stale-index pad cells), so the
 // dF/dpbot term vanishes and is not formed.
-__global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int with_source, int C) {
+// With with_source (the step's call) the blocks past the ncb column tiles
+// first clear the step's tendency planes of u and v (the FP64 plane IT_ZERO,
+// or the mixed mode's one FP64 plane), which nothing in this launch reads.
+__global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int with_source, int C, int ncb) {
   MEMBER;
+  if ((int)blockIdx.x >= ncb) {
+    const size_t off = d.mixed ? 0 : (size_t)d.dyn->it0 * d.N3;
+    double *du = d.DUDT + off, *dv = d.DVDT + off;
+    const size_t st = (size_t)(gridDim.x - ncb) * blockDim.x;
+    for (size_t i = (size_t)(blockIdx.x - ncb) * blockDim.x + threadIdx.x; i < (size_t)d.N3; i += st) {
+      du[i] = 0.;
+      dv[i] = 0.;
+    }
+    return;  // a whole block: no barrier below waits for it
+  }
   extern __shared__ double lds[];
   const Tile T = tile_of(d, C);
   const int nk = d.nk, ks = d.ks, W = nk + 1, J = T.J, I = T.I;
@@ -842,18 +882,23 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
 }
 void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
   const int C = b.knob[KN_COLS_CALCW] ? b.knob[KN_COLS_CALCW] : tile_cols(b, 4);
-  LAUNCH(k_calc_w, tile_grid(b, C), dim3(TPB), sizeof(double) * 4 * C * (b.nk + 1), s, b.D, b.mb, with_source, C);
+  dim3 g = tile_grid(b, C);
+  const int ncb = (int)g.x;
+  if (with_source) {
+    // blocks that clear the tendency planes: about 16 cells per lane
+    const size_t n3 = (size_t)(b.ni + 2) * (b.nj + 3) * (b.nk + 2);
+    g.x += (unsigned)std::max<size_t>(1, std::min<size_t>(1024, n3 / (TPB * 16)));
+  }
+  LAUNCH(k_calc_w, g, dim3(TPB), sizeof(double) * 4 * C * (b.nk + 1), s, b.D, b.mb, with_source, C, ncb);
 }
 
 // ------------------------------------------------ vorticity / divergence
 // PV3 planes k0..k1 from U (uu) and the zonally filtered V (vv), with H3.
-// One block per (K,J) pv-row: row J of vv is filtered in LDS (rows of the
-// v-grid filter band), then one lane per I.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_vort_interior(const MemberDesc *D, int mb, int uf, int vf, int k0) {
-  MEMBER;
-  extern __shared__ double lds[];
+// One block per (K,J) pv-row group: row J of vv is filtered in LDS (rows of
+// the v-grid filter band), then one lane per I.  bx is the row group.
+__device__ __forceinline__ void vort_interior_rows(const MemberDesc &d, double *lds, int bx, int K, int uf, int vf) {
   const RowMap m = row_map(d.ni);
-  const int Jr = 1 + blockIdx.x * m.rpb + m.g, K = k0 + blockIdx.y, ni = d.ni;
+  const int Jr = 1 + bx * m.rpb + m.g, ni = d.ni;
   const bool live = Jr <= d.nj;
   const int J = live ? Jr : 1;
   const double *uu = fld_ptr(d, uf) + (size_t)K * d.N2, *vv = fld_ptr(d, vf) + (size_t)K * d.N2;
@@ -882,13 +927,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     if (I == 1) pv[ix2(d, J, ni + 1)] = v;
   }
 }
-// pole PV from the circulation of the adjacent u-row: one wavefront per
-// (pole, K); the row is staged in LDS and lanes 0/1 form the two zonal sums
-// in the reference's sequential order
-__global__ void __launch_bounds__(256) k_vort_poles(const MemberDesc *D, int mb, int uf, int k0) {
-  MEMBER;
-  extern __shared__ double row[];
-  const int K = k0 + (blockIdx.x >> 1), north = blockIdx.x & 1;
+// pole PV from the circulation of the adjacent u-row (north = 1: J = nj+1
+// from row nj, else J = 0 from row 0): the row is staged in LDS and lanes
+// 0/1 form the two zonal sums in the reference's sequential order.  Every
+// lane of the block reaches both barriers.
+__device__ __forceinline__ void vort_pole(const MemberDesc &d, double *row, int K, int north, int uf) {
   const double *uu = fld_ptr(d, uf) + (size_t)K * d.N2;
   const double *hh = d.H3 + (size_t)K * d.N2;
   double *pv = d.PV3 + (size_t)K * d.N2;
@@ -919,21 +962,11 @@ __global__ void __launch_bounds__(256) k_vort_poles(const MemberDesc *D, int mb,
   const int Jd = north ? nj + 1 : 0;
   for (int I = threadIdx.x; I <= ni + 1; I += blockDim.x) pv[ix2(d, Jd, I)] = val;
 }
-void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1) {
-  int nK = k1 - k0 + 1;
-  const int rpb = rows_per_block(b.ni);
-  LAUNCH(k_vort_interior, dim3((b.nj + rpb - 1) / rpb, nK, b.nm), dim3(256), sizeof(double) * rpb * 2 * b.ni, s, b.D,
-         b.mb, uu, vv, k0);
-  LAUNCH(k_vort_poles, dim3(2 * nK, 1, b.nm), dim3(64), sizeof(double) * (2 * b.ni + 2), s, b.D, b.mb, uu, k0);
-}
-
-// DIV of (uu zonally filtered, vv) on planes k0..: one block per (K,J)
-// h-row, row J of uu filtered in LDS (rows of the h-grid filter band)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_divergence(const MemberDesc *D, int mb, int uf, int vf, int df, int k0) {
-  MEMBER;
-  extern __shared__ double lds[];
+// DIV of (uu zonally filtered, vv) on plane K: one block per (K,J) h-row
+// group, row J of uu filtered in LDS (rows of the h-grid filter band)
+__device__ __forceinline__ void divergence_rows(const MemberDesc &d, double *lds, int bx, int K, int uf, int vf, int df) {
   const RowMap m = row_map(d.ni);
-  const int Jr = blockIdx.x * m.rpb + m.g, K = k0 + blockIdx.y, ni = d.ni;
+  const int Jr = bx * m.rpb + m.g, ni = d.ni;
   const bool live = Jr <= d.nj;
   const int J = live ? Jr : 0;
   const double *uu = fld_ptr(d, uf) + (size_t)K * d.N2, *vv = fld_ptr(d, vf) + (size_t)K * d.N2;
@@ -956,6 +989,60 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     if (I == ni) di[ix2(d, J, 0)] = v;
     if (I == 1) di[ix2(d, J, ni + 1)] = v;
   }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_vort_interior(const MemberDesc *D, int mb, int uf, int vf, int k0) {
+  MEMBER;
+  extern __shared__ double lds[];
+  vort_interior_rows(d, lds, blockIdx.x, k0 + blockIdx.y, uf, vf);
+}
+__global__ void __launch_bounds__(256) k_vort_poles(const MemberDesc *D, int mb, int uf, int k0) {
+  MEMBER;
+  extern __shared__ double row[];
+  vort_pole(d, row, k0 + (blockIdx.x >> 1), blockIdx.x & 1, uf);
+}
+void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1) {
+  int nK = k1 - k0 + 1;
+  const int rpb = rows_per_block(b.ni);
+  LAUNCH(k_vort_interior, dim3((b.nj + rpb - 1) / rpb, nK, b.nm), dim3(256), sizeof(double) * rpb * 2 * b.ni, s, b.D,
+         b.mb, uu, vv, k0);
+  LAUNCH(k_vort_poles, dim3(2 * nK, 1, b.nm), dim3(64), sizeof(double) * (2 * b.ni + 2), s, b.D, b.mb, uu, k0);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_divergence(const MemberDesc *D, int mb, int uf, int vf, int df, int k0) {
+  MEMBER;
+  extern __shared__ double lds[];
+  divergence_rows(d, lds, blockIdx.x, k0 + blockIdx.y, uf, vf, df);
+}
+
+// store_diag in one launch: on planes K = 0..nk, blocks [0, nv) form the PV
+// rows, [nv, nv + nd) the divergence rows, and the last two the pole PV of
+// the plane (the three are independent: each reads U, V and H3 only).  With
+// rotate set, the first block of plane 0 of each member then rotates its
+// time planes (IT_ZERO <- IT_MINUS2 <- IT_MINUS1 <- IT_ZERO) and advances
+// itime: the step's last use of them was its Adams-Bashforth update, and no
+// block of this launch reads them.
+__global__ void __launch_bounds__(256) k_store_diag(const MemberDesc *D, int mb, int nv, int nd, int rotate) {
+  MEMBER;
+  extern __shared__ double lds[];
+  const int bx = blockIdx.x, K = blockIdx.y;
+  if (bx < nv) vort_interior_rows(d, lds, bx, K, F_U, F_V);
+  else if (bx < nv + nd) divergence_rows(d, lds, bx - nv, K, F_U, F_V, F_DIV);
+  else vort_pole(d, lds, K, bx - nv - nd, F_U);
+  if (rotate && bx == 0 && K == 0 && threadIdx.x == 0) {
+    MemberDyn *y = d.dyn;
+    int t = y->itm2;
+    y->itm2 = y->itm1;
+    y->itm1 = y->it0;
+    y->it0 = t;
+    y->itime += 1;
+  }
+}
+void L_store_diag(hipStream_t s, const Bucket &b, int rotate) {
+  const int rpb = rows_per_block(b.ni);
+  const int nv = (b.nj + rpb - 1) / rpb, nd = (b.nj + 1 + rpb - 1) / rpb;
+  const size_t shm = sizeof(double) * std::max(rpb * 2 * b.ni, 2 * b.ni + 2);
+  LAUNCH(k_store_diag, dim3(nv + nd + 2, b.nk + 1, b.nm), dim3(256), shm, s, b.D, b.mb, nv, nd, rotate);
 }
 void L_divergence(hipStream_t s, const Bucket &b, int uu, int vv, int dst, int k0, int k1) {
   const int rpb = rows_per_block(b.ni);

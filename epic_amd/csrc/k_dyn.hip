@@ -497,38 +497,9 @@ __global__ void __launch_bounds__(256) k_ab(const MemberDesc *D, int mb) {
     if (I == 1) val[ix3(d, K, J, d.ni + 1)] = y;
   }
 }
-// rotate IT_ZERO <- IT_MINUS2 <- IT_MINUS1 <- IT_ZERO and advance itime
-__global__ void __launch_bounds__(256) k_rotate(const MemberDesc *D, int mb) {
-  MEMBER;
-  if (threadIdx.x) return;
-  MemberDyn *y = d.dyn;
-  int t = y->itm2;
-  y->itm2 = y->itm1;
-  y->itm1 = y->it0;
-  y->it0 = t;
-  y->itime += 1;
-}
 void L_ab(hipStream_t s, const Bucket &b) {
   if (b.mixed) { LM_ab(s, b); return; }
   LAUNCH(k_ab, grid2(b.nj + 1, b.ni, b.nk - 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
-}
-void L_rotate(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_rotate, dim3(1, 1, b.nm), dim3(64), 0, s, b.D, b.mb);
-}
-
-// zero the current tendency planes
-__global__ void __launch_bounds__(256) k_zero_tend(const MemberDesc *D, int mb) {
-  MEMBER;
-  double *du = tend(d, d.DUDT, d.dyn->it0), *dv = tend(d, d.DVDT, d.dyn->it0);
-  size_t i = (size_t)blockIdx.x * TPB + threadIdx.x, st = (size_t)gridDim.x * TPB;
-  for (; i < (size_t)d.N3; i += st) {
-    du[i] = 0.;
-    dv[i] = 0.;
-  }
-}
-void L_zero_tend(hipStream_t s, const Bucket &b) {
-  if (b.mixed) { LM_zero_tend(s, b); return; }
-  LAUNCH(k_zero_tend, dim3(256, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
 // monotone filtered geopotential, K = nk-1 down to 0 (src/shared/epic_timestep.c:176-196)

@@ -41,16 +41,6 @@ __device__ __forceinline__ float *sBf(const MemberDesc &d) { return reinterpret_
 __device__ __forceinline__ float *sCf(const MemberDesc &d) { return reinterpret_cast<float *>(d.sC); }
 
 // ------------------------------------------------------------ tendencies
-__global__ void __launch_bounds__(256) km_zero_tend(const MemberDesc *D, int mb) {
-  MEMBER;
-  double *du = curU(d), *dv = curV(d);
-  size_t i = (size_t)blockIdx.x * TPB + threadIdx.x, st = (size_t)gridDim.x * TPB;
-  for (; i < (size_t)d.N3; i += st) {
-    du[i] = 0.;
-    dv[i] = 0.;
-  }
-}
-
 // UH, VH, KIN (FP32 views of sA, sB, sC) on planes 1..nk-1 with zonal pads
 __global__ void __launch_bounds__(256) km_uvh_kin(const MemberDesc *D, int mb) {
   MEMBER;
@@ -295,9 +285,6 @@ __global__ void __launch_bounds__(256) km_ab(const MemberDesc *D, int mb) {
 }
 
 // ------------------------------------------------------------- launchers
-void LM_zero_tend(hipStream_t s, const Bucket &b) {
-  LAUNCH(km_zero_tend, dim3(256, 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
-}
 void LM_uv_core(hipStream_t s, const Bucket &b) {
   const int nK = b.nk - 1;
   LAUNCH(km_uvh_kin, grid2(b.nj + 2, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);

@@ -55,7 +55,6 @@ struct Bucket {
 // ---- diagnostics (k_diag.hip)
 void L_fill(hipStream_t s, const Bucket &b, int fld, double v);
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta);
-void L_thermo(hipStream_t s, const Bucket &b, int full);
 void L_richardson(hipStream_t s, const Bucket &b);
 void L_gz_from_u(hipStream_t s, const Bucket &b);
 void L_gz3(hipStream_t s, const Bucket &b);
@@ -66,6 +65,11 @@ void L_calc_w(hipStream_t s, const Bucket &b, int with_source);
 void L_copy_planes(hipStream_t s, const Bucket &b, int src, int dst, int k0, int k1);
 void L_vorticity(hipStream_t s, const Bucket &b, int uu, int vv, int k0, int k1);
 void L_divergence(hipStream_t s, const Bucket &b, int uu, int vv, int dst, int k0, int k1);
+// store_diag (PV3 and DIV_UV3 of U, V on K = 0..nk) in one launch; rotate:
+// also rotate the time planes and advance itime (the step's last action)
+void L_store_diag(hipStream_t s, const Bucket &b, int rotate);
+// store_pgrad_vars' per-cell values after the FULL set_p2etc
+void L_thermo(hipStream_t s, const Bucket &b);
 
 // ---- zonal filter (k_filter.hip)
 // 2-D filter of planes k0..k1 of fld; vgrid selects the v/pv rows; mode 0
@@ -82,8 +86,6 @@ void L_uv_drag(hipStream_t s, const Bucket &b);
 void L_pgrad(hipStream_t s, const Bucket &b);
 void L_divdamp(hipStream_t s, const Bucket &b);
 void L_ab(hipStream_t s, const Bucket &b);
-void L_rotate(hipStream_t s, const Bucket &b);
-void L_zero_tend(hipStream_t s, const Bucket &b);
 void L_gz_mono(hipStream_t s, const Bucket &b);
 void L_gz_filtercopy(hipStream_t s, const Bucket &b);
 // hyperviscosity on every field of the set (at most two: the meridional
@@ -92,7 +94,6 @@ void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs);
 void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs);
 
 // ---- mixed-precision momentum kernels (k_mixed.hip)
-void LM_zero_tend(hipStream_t s, const Bucket &b);
 void LM_uv_core(hipStream_t s, const Bucket &b);
 void LM_uv_drag(hipStream_t s, const Bucket &b);
 void LM_divdamp_apply(hipStream_t s, const Bucket &b);
