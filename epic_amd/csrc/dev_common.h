@@ -50,6 +50,8 @@ struct MemberDesc {
   // Newtonian cooling toward the sounding (is_tab): PCHIP tables against
   // -log p of the ironed sounding temperature and of the cooling time
   int is_tab, n_snd, n_cool;
+  // the traditional pressure gradient in place of the finite-volume one
+  int pgrad_trad;
   const double *snd_x, *snd_y, *snd_d, *cool_x, *cool_y, *cool_d;
   // host-libm tables: cos(lat), cos(lat)^4 per h-row J, cos(lon) per column I
   const double *coslat_h, *coslat4_h, *coslon_h;

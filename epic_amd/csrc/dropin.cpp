@@ -45,6 +45,7 @@ struct DropIn {
   epic_engine *E = nullptr;
   int slot = -1;
   int lazy = 0;
+  int pgrad_traditional = 0;
   bool resident = false;  // the device copy is newer than or equal to var.*
   bool dirty_host = false;  // var.* lags the device copy (lazy mode)
   int ni = 0, nj = 0, nk = 0;
@@ -176,6 +177,7 @@ void create_member(const struct epic_planet_abi *planet) {
   c.k_sigma = grid.k_sigma; c.k_sponge = grid.k_sponge;
   c.newt_cool_on = grid.newt_cool_on; c.newt_cool_adjust = grid.newt_cool_adjust;
   c.drag_v = grid.drag_v; c.drag_zonal_avg = grid.drag_zonal_avg;
+  c.pgrad_traditional = g_di.pgrad_traditional;
   c.dln = grid.dln; c.dlt = grid.dlt; c.dy0 = grid.dy0;
   c.globe_lonbot = grid.globe_lonbot; c.globe_lontop = grid.globe_lontop;
   c.globe_latbot = grid.globe_latbot; c.globe_lattop = grid.globe_lattop;
@@ -223,6 +225,8 @@ void create_member(const struct epic_planet_abi *planet) {
 }  // namespace
 
 extern "C" void epic_amd_set_sync_mode(int lazy) { g_di.lazy = lazy ? 1 : 0; }
+
+extern "C" void epic_amd_set_pgrad_traditional(int on) { g_di.pgrad_traditional = on ? 1 : 0; }
 
 extern "C" void epic_amd_flush(void) {
   if (!g_di.E || g_di.slot < 0 || !g_di.dirty_host) return;

@@ -515,6 +515,7 @@ void member_plan(const epic_case *c, bool mixed, const epic_tables &T, MemberDes
   d.hj_lnref = d.is_hj ? log(c->pbot / c->teq_p_contrast) : 0.;
   d.n_snd = T.n_snd;
   d.n_cool = T.n_cool;
+  d.pgrad_trad = c->pgrad_traditional ? 1 : 0;
   d.heat_area = 0.;
   for (int J = 0; J <= c->nj; J++) {
     const double da = 1. / c->mn[2 * J + 1];

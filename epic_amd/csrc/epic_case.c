@@ -423,6 +423,7 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   c->drag_v = s->drag_v ? 1 : 0;
   c->drag_zonal_avg = s->drag_zonal_avg ? 1 : 0;
   c->newt_cool_adjust = s->newt_cool_adjust ? 1 : 0;
+  c->pgrad_traditional = s->pgrad_traditional ? 1 : 0;
   c->nudiv_nondim = s->nudiv_nondim;
   /* Newtonian cooling switched on for a planet with no relaxation of its
    * own relaxes toward a sounding with a cooling-time table, as the

@@ -534,7 +534,7 @@ int epic_nc_read(const char *path, epic_case *c) {
     r_text(&r, need_text[q][0], s, sizeof(s));
     if (r.bad) break;
     if (!text_is(s, need_text[q][1])) {
-      snprintf(g_err, sizeof(g_err), "%s = \"%s\" is not implemented (only \"%s\")", need_text[q][0], s,
+      snprintf(g_err, sizeof(g_err), "%.40s = \"%.60s\" is not implemented (only \"%.60s\")", need_text[q][0], s,
                need_text[q][1]);
       epic_nc_header_free(f);
       return -4;

@@ -279,6 +279,7 @@ void L_uv_drag(hipStream_t s, const Bucket &b) {
 // one wavefront per (K, pole), the row staged in LDS and summed in order
 __global__ void __launch_bounds__(256) k_gz_polefill(const MemberDesc *D, int mb) {
   MEMBER;
+  if (d.pgrad_trad) return;  // the traditional form reads no pole row
   extern __shared__ double row[];
   const int K = blockIdx.x >> 1, north = blockIdx.x & 1, ni = d.ni;
   const int Js = north ? d.nj : 0, Jd = north ? d.nj + 1 : -1;
@@ -335,6 +336,36 @@ __device__ __forceinline__ double face_lat(const MemberDesc &d, int K, int J, in
   return rlnn * (-t1 * (pbn - ptn) + t2 * (ztn * pbn - zbn * ptn)) / (ztn - zbn);
 }
 
+// uv_pgrad_traditional (src/shared/epic_timestep.c:806-884, the reference's
+// PGRAD_TRADITIONAL build): on K = 1..k_sigma-2 grad(MONT3) less the mean
+// EXNER3 times grad(THETA); on K = k_sigma-1..nk-1 the h3-weighted GZ3
+// difference less the p-weighted mean of the GZ3 differences of the
+// interfaces above and below.  u at (K,J,I), v at J >= 1.
+__device__ __forceinline__ void pgrad_traditional(const MemberDesc &d, int K, int J, int I, double *du, double *dv) {
+  const double *MO = d.MONT3, *EX = d.EXNER3, *TH = d.THETA, *G = d.GZ3, *H = d.H3, *P = d.P3;
+  const size_t q = ix3(d, K, J, I), w = ix3(d, K, J, I - 1), s = ix3(d, K, J - 1, I);
+  if (K <= d.ks - 2) {
+    du[q] -= d.m[2 * J + 1] * (MO[q] - MO[w] - .5 * (EX[q] + EX[w]) * (TH[q] - TH[w]));
+    if (J >= 1) dv[q] -= d.n[2 * J] * (MO[q] - MO[s] - .5 * (EX[q] + EX[s]) * (TH[q] - TH[s]));
+    return;
+  }
+  const size_t qu = q + d.N2, qd = q - d.N2;
+  {
+    const size_t wu = w + d.N2, wd = w - d.N2;
+    const double hw = (G[q] * H[q] - G[w] * H[w]) * 2. / (H[q] + H[w]);
+    const double up = (G[qu] + G[wu]) * (P[qu] - P[wu]) - (G[qd] + G[wd]) * (P[qd] - P[wd]);
+    const double dn = P[qu] + P[wu] - P[qd] - P[wd];
+    du[q] -= d.m[2 * J + 1] * (hw - up / dn);
+  }
+  if (J >= 1) {
+    const size_t su = s + d.N2, sd = s - d.N2;
+    const double hs = (G[q] * H[q] - G[s] * H[s]) * 2. / (H[q] + H[s]);
+    const double up = (G[qu] + G[su]) * (P[qu] - P[su]) - (G[qd] + G[sd]) * (P[qd] - P[sd]);
+    const double dn = P[qu] + P[su] - P[qd] - P[sd];
+    dv[q] -= d.n[2 * J] * (hs - up / dn);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_pgrad(const MemberDesc *D, int mb) {
   MEMBER;
   FLAT2(0, d.nj + 1, 1, d.ni);
@@ -342,6 +373,10 @@ __global__ void __launch_bounds__(256) k_pgrad(const MemberDesc *D, int mb) {
   const double TOL = 1.e-2, DG = M_PI / 180.;
   const double *G = d.GZ3, *P2 = d.P2;
   double *du = cur_tend(d, d.DUDT), *dv = cur_tend(d, d.DVDT);
+  if (d.pgrad_trad) {
+    pgrad_traditional(d, K, J, I, du, dv);
+    return;
+  }
 #define Z(k, j, i) G[ix3(d, k, j, i)]
 #define PZ(k, j, i) P2[ix3(d, k, j, i)]
   {

@@ -181,6 +181,12 @@ void epic_amd_set_sync_mode(int lazy);
 void epic_amd_flush(void);
 /* Forget the device copy; the next timestep() re-uploads var.* and grid.*. */
 void epic_amd_invalidate(void);
+/* The pressure gradient of a caller built with PGRAD_TYPE =
+ * PGRAD_TRADITIONAL (a compile-time switch of the reference's
+ * src/shared/epic_timestep.c:772-774, invisible to the library): 1 selects
+ * uv_pgrad_traditional, 0 the default finite-volume form.  Call before the
+ * first timestep() (or invalidate after). */
+void epic_amd_set_pgrad_traditional(int on);
 
 /* ------------------------------------------------------------------------
  * Synthetic cases: planet + grid + seeded initial state, laid out exactly
@@ -229,6 +235,10 @@ typedef struct {
                              Newtonian cooling on); 0: none                 */
   double nu6_nondim;      /* del^6 hyperviscosity, fraction of maximum     */
   double nu8_nondim;      /* del^8 hyperviscosity, fraction of maximum     */
+  int    pgrad_traditional; /* 1: the reference's PGRAD_TRADITIONAL
+                             pressure gradient (uv_pgrad_traditional,
+                             src/shared/epic_timestep.c:806-884) in place of
+                             the finite-volume one of its default build    */
 } epic_case_spec;
 
 typedef struct {
@@ -278,6 +288,8 @@ typedef struct {
   double *dudt_hist, *dvdt_hist;
   double model_time, start_time;
   void   *nc_header;
+  /* 1: the traditional pressure gradient (epic_case_spec.pgrad_traditional) */
+  int    pgrad_traditional;
 } epic_case;
 
 /* Build a case from a spec: metric arrays (set_lonlat/set_fmn/set_gravity/

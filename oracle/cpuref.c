@@ -1083,6 +1083,65 @@ static void uv_drag(cpuref *R, double *dudt, double *dvdt) {
   }
 }
 
+/* uv_pgrad_traditional, src/shared/epic_timestep.c:806-884 (the
+ * reference's PGRAD_TRADITIONAL build; no fpara): the Montgomery-potential
+ * form, grad(MONT3) - <EXNER3> grad(THETA), on K = 1..k_sigma-2, and on
+ * K = k_sigma-1..nk-1 the h3-weighted GZ3 difference less the p-weighted
+ * mean of the GZ3 differences of the interfaces above and below; u on
+ * J = 0..nj, v on J = 1..nj, I = 1..ni (no pads) */
+static void uv_pgrad_traditional(cpuref *R, double *dudt, double *dvdt) {
+  epic_case *c = &R->c;
+  const int ni = R->ni, nj = R->nj, nk = R->nk, ks = R->k_sigma;
+  double *MO = R->MONT3, *EX = R->EXNER3, *TH = R->THETA, *G = R->GZ3, *H = R->H3, *P = R->P3;
+  for (int K = 1; K <= ks - 2; K++) {
+    for (int J = 0; J <= nj; J++) {
+      const double m = c->m[2 * J + 1];
+      for (int I = 1; I <= ni; I++)
+        A3(dudt, K, J, I) -= m * (A3(MO, K, J, I) - A3(MO, K, J, I - 1) -
+                                  .5 * (A3(EX, K, J, I) + A3(EX, K, J, I - 1)) * (A3(TH, K, J, I) - A3(TH, K, J, I - 1)));
+    }
+    for (int J = 1; J <= nj; J++) {
+      const double n = c->n[2 * J];
+      for (int I = 1; I <= ni; I++)
+        A3(dvdt, K, J, I) -= n * (A3(MO, K, J, I) - A3(MO, K, J - 1, I) -
+                                  .5 * (A3(EX, K, J, I) + A3(EX, K, J - 1, I)) * (A3(TH, K, J, I) - A3(TH, K, J - 1, I)));
+    }
+  }
+  for (int K = ks - 1; K < nk; K++) {
+    for (int J = 0; J <= nj; J++) {
+      const double m = c->m[2 * J + 1];
+      for (int I = 1; I <= ni; I++) {
+        const double hw = (A3(G, K, J, I) * A3(H, K, J, I) - A3(G, K, J, I - 1) * A3(H, K, J, I - 1)) * 2. /
+                          (A3(H, K, J, I) + A3(H, K, J, I - 1));
+        const double up = (A3(G, K + 1, J, I) + A3(G, K + 1, J, I - 1)) * (A3(P, K + 1, J, I) - A3(P, K + 1, J, I - 1)) -
+                          (A3(G, K - 1, J, I) + A3(G, K - 1, J, I - 1)) * (A3(P, K - 1, J, I) - A3(P, K - 1, J, I - 1));
+        const double dn = A3(P, K + 1, J, I) + A3(P, K + 1, J, I - 1) - A3(P, K - 1, J, I) - A3(P, K - 1, J, I - 1);
+        A3(dudt, K, J, I) -= m * (hw - up / dn);
+      }
+    }
+    for (int J = 1; J <= nj; J++) {
+      const double n = c->n[2 * J];
+      for (int I = 1; I <= ni; I++) {
+        const double hs = (A3(G, K, J, I) * A3(H, K, J, I) - A3(G, K, J - 1, I) * A3(H, K, J - 1, I)) * 2. /
+                          (A3(H, K, J, I) + A3(H, K, J - 1, I));
+        const double up = (A3(G, K + 1, J, I) + A3(G, K + 1, J - 1, I)) * (A3(P, K + 1, J, I) - A3(P, K + 1, J - 1, I)) -
+                          (A3(G, K - 1, J, I) + A3(G, K - 1, J - 1, I)) * (A3(P, K - 1, J, I) - A3(P, K - 1, J - 1, I));
+        const double dn = A3(P, K + 1, J, I) + A3(P, K + 1, J - 1, I) - A3(P, K - 1, J, I) - A3(P, K - 1, J - 1, I);
+        A3(dvdt, K, J, I) -= n * (hs - up / dn);
+      }
+    }
+  }
+}
+
+/* the traditional pressure gradient alone on the current state, into a
+ * cleared plane IT_ZERO of the tendencies (the function-level check against
+ * the reference's own uv_pgrad_traditional) */
+void cpuref_pgrad_traditional(cpuref *R) {
+  memset(R->DUDT[R->it0], 0, sizeof(double) * R->N3);
+  memset(R->DVDT[R->it0], 0, sizeof(double) * R->N3);
+  uv_pgrad_traditional(R, R->DUDT[R->it0], R->DVDT[R->it0]);
+}
+
 /* uv_pgrad_finite_volume, src/shared/epic_timestep.c:909-1273 */
 static void uv_pgrad(cpuref *R, double *dudt, double *dvdt) {
   epic_case *c = &R->c;
@@ -1745,7 +1804,8 @@ int cpuref_step(cpuref *R) {
   filter3(R, R->THETA, FK_THETA);
   set_p2etc(R, 1);
   store_pgrad_vars(R);
-  uv_pgrad(R, dudt, dvdt);
+  if (R->c.pgrad_traditional) uv_pgrad_traditional(R, dudt, dvdt);
+  else uv_pgrad(R, dudt, dvdt);
   divergence_damping(R, dudt, dvdt);
   ab_step(R, R->U, R->DUDT, 0);
   ab_step(R, R->V, R->DVDT, 1);

@@ -15,6 +15,8 @@ typedef struct cpuref cpuref;
 cpuref *cpuref_create(const epic_case_spec *spec);
 /* the state of an epic.nc file (epic_nc_read), Adams-Bashforth history included */
 cpuref *cpuref_create_nc(const char *path);
+/* the traditional pressure gradient alone into a cleared plane IT_ZERO */
+void cpuref_pgrad_traditional(cpuref *R);
 void    cpuref_destroy(cpuref *r);
 /* timestep(SYNC_DIAGS_ONLY) and timestep(STEP_PROGS_AND_SYNC_DIAGS) */
 int     cpuref_sync(cpuref *r);

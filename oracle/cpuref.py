@@ -30,6 +30,7 @@ def lib(nudged=False):
         L.cpuref_create.argtypes = [vp]
         L.cpuref_create_nc.restype = vp
         L.cpuref_create_nc.argtypes = [ctypes.c_char_p]
+        L.cpuref_pgrad_traditional.argtypes = [vp]
         for f in ("cpuref_sync", "cpuref_step", "cpuref_error"):
             getattr(L, f).argtypes = [vp]
         L.cpuref_destroy.argtypes = [vp]
@@ -76,6 +77,11 @@ class CpuRef:
 
     def sync(self):
         return self._L.cpuref_sync(self._r)
+
+    def pgrad_traditional(self):
+        """The traditional pressure gradient alone into a cleared plane
+        IT_ZERO of the tendencies."""
+        self._L.cpuref_pgrad_traditional(self._r)
 
     def step(self, n=1):
         rc = 0

@@ -22,7 +22,9 @@
  * Usage: epic_ref [--OPTION=VALUE ...] PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT [DUMP_STEP ...]
  * OPTION is one of the epic_case_spec switches k_sponge, newt_cool,
  * newt_cool_adjust, drag_v, drag_zonal_avg, tau_drag, prandtl, nu6_nondim,
- * nu8_nondim.  With k_sponge > 0 the reference's own set_sponge() fills
+ * nu8_nondim, pgrad_traditional (then a last dump, step -1, holds the
+ * reference's uv_pgrad_traditional alone on the final state in a cleared
+ * tendency plane IT_ZERO).  With k_sponge > 0 the reference's own set_sponge() fills
  * t_sponge_inv, and the case builder's copy must equal it bit for bit.
  * A DUMP_STEP of 0 dumps the state after the SYNC call; s > 0 after s steps.
  * A negative DUMP_STEP -s dumps at s and prints the metric comparison only.
@@ -147,6 +149,11 @@ static void write_fields(FILE *fo, int step) {
 }
 
 /* --name=value switches of epic_case_spec; returns 0 or -1 */
+/* the reference's traditional pressure gradient (src/shared/epic_timestep.c:
+ * 806-884), compiled into its object beside the finite-volume one its build
+ * calls */
+void uv_pgrad_traditional(planetspec *planet);
+
 static int spec_option(epic_case_spec *sp, const char *arg) {
   const char *eq = strchr(arg, '=');
   if (strncmp(arg, "--", 2) != 0 || !eq) return -1;
@@ -155,7 +162,7 @@ static int spec_option(epic_case_spec *sp, const char *arg) {
 #define OPT_I(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atoi(v); return 0; }
 #define OPT_D(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atof(v); return 0; }
   OPT_I(k_sponge) OPT_I(newt_cool) OPT_I(newt_cool_adjust) OPT_I(drag_v) OPT_I(drag_zonal_avg)
-  OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim)
+  OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim) OPT_I(pgrad_traditional)
 #undef OPT_I
 #undef OPT_D
   return -1;
@@ -348,7 +355,11 @@ int main(int argc, char **argv) {
 
   FILE *fo = fopen(out, "wb");
   if (!fo) return 4;
-  int hdr[9] = {MAGIC, grid.ni, grid.nj, grid.nk, ndump, EPIC_F_COUNT, Nelem3d, Nelem2d, mism};
+  /* with pgrad_traditional the build's own (finite-volume) steps run, and
+   * one more dump follows them: the reference's uv_pgrad_traditional alone,
+   * on the final state, into a cleared tendency plane IT_ZERO */
+  int hdr[9] = {MAGIC, grid.ni, grid.nj, grid.nk, ndump + (sp.pgrad_traditional ? 1 : 0), EPIC_F_COUNT,
+                Nelem3d, Nelem2d, mism};
   fwrite(hdr, sizeof(int), 9, fo);
 
   grid.itime = 0;
@@ -367,6 +378,13 @@ int main(int argc, char **argv) {
   }
   clock_gettime(CLOCK_MONOTONIC, &t1);
   printf("timed %d steps in %.6f s\n", nsteps, (double)(t1.tv_sec - t0.tv_sec) + 1.e-9 * (double)(t1.tv_nsec - t0.tv_nsec));
+  if (sp.pgrad_traditional) {
+    grid.it_uv_tend = IT_ZERO;
+    memset(var.u.tendency + (size_t)IT_ZERO * Nelem3d, 0, sizeof(double) * Nelem3d);
+    memset(var.v.tendency + (size_t)IT_ZERO * Nelem3d, 0, sizeof(double) * Nelem3d);
+    uv_pgrad_traditional(planet);
+    write_fields(fo, -1);
+  }
   fclose(fo);
   epic_case_free(&c);
   return mism ? 5 : 0;

@@ -45,6 +45,11 @@ CASES = {
     "venus_newt_adjust_prandtl_32x16x6": ea.spec(ea.VENUS_LIKE, 32, 16, 6, 600, 27, 0.5, 0.1, 0.2, 0.5,
                                                  newt_cool=1, newt_cool_adjust=1, prandtl=0.4,
                                                  drag_zonal_avg=1),
+    # the reference's traditional pressure gradient (PGRAD_TRADITIONAL)
+    "hs_pgrad_traditional_32x16x8": ea.spec(ea.HELD_SUAREZ, 32, 16, 8, 600, 31, 0.5, 0.1, 0.3, 0.4,
+                                            pgrad_traditional=1),
+    "jupiter_pgrad_traditional_32x16x10": ea.spec(ea.JUPITER_LIKE, 32, 16, 10, 60, 32, 0.5, 0.0, 0.0, 1.0,
+                                                  pgrad_traditional=1),
 }
 
 

@@ -134,6 +134,37 @@ def test_oracle_bitwise_vs_reference_build(args, opts, tmp_path):
         assert r.time_planes()[:3] == [int(x) for x in hdr[1:4]]
 
 
+@pytest.mark.skipif(not os.path.exists(REF_EXE), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("args,opts", [
+    ((0, 32, 16, 8, 600, 31, 0.5, 0.1, 0.3, 0.4), {}),
+    ((1, 32, 16, 10, 60, 32, 0.5, 0.0, 0.0, 1.0), {"k_sponge": -1}),
+    ((2, 32, 12, 12, 60, 33, 0.4, 0.0, 0.0, 0.5), {"k_sponge": 2}),
+], ids=["held_suarez", "jupiter", "saturn"])
+def test_traditional_pressure_gradient_is_the_reference_function(args, opts, tmp_path):
+    """The reference's build compiles uv_pgrad_traditional
+    (src/shared/epic_timestep.c:806-884) beside the finite-volume pressure
+    gradient it calls (its PGRAD_TYPE is fixed in the source), so the
+    traditional form is checked as a function: after 5 steps of the build,
+    the reference's uv_pgrad_traditional alone on the final state equals the
+    oracle's on the same state, bit for bit, in both tendency planes."""
+    ref = _ref_run(args, [5], tmp_path, dict(opts, pgrad_traditional=1))
+    (h5, f5), (hp, fp) = ref
+    assert int(h5[0]) == 5 and int(hp[0]) == -1
+    r = cr.CpuRef(case_spec(args, opts))
+    r.sync()
+    r.step(5)
+    it0 = r.time_planes()[0]
+    assert it0 == int(hp[1])
+    for nm in ("U", "V", "GZ3", "MONT3"):
+        assert same_bits(r.read(nm), f5[nm]), nm
+    r.pgrad_traditional()
+    for w in ("DUDT", "DVDT"):
+        got, want = r.read(f"{w}{it0}"), fp[f"{w}{it0}"]
+        assert same_bits(got, want), w
+        assert np.count_nonzero(want) > 0
+    r.close()
+
+
 def test_richardson_of_given_fields_reproduces_store_diag():
     """The function form the GPU tests use for RI2 (oracle fields written in,
     RI2 formed again) gives store_diag's RI2 bit for bit, and the golden
