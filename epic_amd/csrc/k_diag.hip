@@ -30,6 +30,16 @@ namespace epic {
   if (t_ >= (nJ) * (nI)) return;                       \
   const int J = (j0) + t_ / (nI);                      \
   const int I = (i0) + t_ % (nI);
+// the same with the block renumbered for the XCDs (xcd_block): bx_.y and
+// bx_.z stand for blockIdx.y and blockIdx.z
+#define MEMBER_X                                       \
+  const uint3 bx_ = xcd_block();                       \
+  const MemberDesc &d = D[mb + bx_.z]
+#define FLAT2_X(j0, nJ, i0, nI)                        \
+  int t_ = bx_.x * TPB + threadIdx.x;                  \
+  if (t_ >= (nJ) * (nI)) return;                       \
+  const int J = (j0) + t_ / (nI);                      \
+  const int I = (i0) + t_ % (nI);
 
 static inline dim3 grid2(int nJ, int nI, int nK, int nm) {
   return dim3((unsigned)((nJ * nI + TPB - 1) / TPB), (unsigned)nK, (unsigned)nm);
@@ -624,9 +634,9 @@ void L_gz3(hipStream_t s, const Bucket &b) {
 
 // finish DZDT3 (diagnostic vertical velocity)
 __global__ void __launch_bounds__(256) k_dzdt3(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(0, d.nj + 1, 1, d.ni);
-  const int K = 1 + blockIdx.y;
+  MEMBER_X;
+  FLAT2_X(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + bx_.y;
   size_t q = ix3(d, K, J, I);
   const double *G = d.GZ3;
   double z = d.DZDT3[q];

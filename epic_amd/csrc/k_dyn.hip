@@ -27,6 +27,16 @@ namespace epic {
   if (t_ >= (nJ) * (nI)) return;                       \
   const int J = (j0) + t_ / (nI);                      \
   const int I = (i0) + t_ % (nI);
+// the same with the block renumbered for the XCDs (xcd_block): bx_.y and
+// bx_.z stand for blockIdx.y and blockIdx.z
+#define MEMBER_X                                       \
+  const uint3 bx_ = xcd_block();                       \
+  const MemberDesc &d = D[mb + bx_.z]
+#define FLAT2_X(j0, nJ, i0, nI)                        \
+  int t_ = bx_.x * TPB + threadIdx.x;                  \
+  if (t_ >= (nJ) * (nI)) return;                       \
+  const int J = (j0) + t_ / (nI);                      \
+  const int I = (i0) + t_ % (nI);
 
 static inline dim3 grid2(int nJ, int nI, int nK, int nm) {
   return dim3((unsigned)((nJ * nI + TPB - 1) / TPB), (unsigned)nK, (unsigned)nm);
@@ -45,9 +55,9 @@ __device__ __forceinline__ double *cur_tend(const MemberDesc &d, double *base) {
 // UH (sA), VH (sB), KIN (sC) on planes K = 1..nk-1 with zonal pads; VH rows
 // 0 and nj+1 are zero as in the reference's cleared buffers
 __global__ void __launch_bounds__(256) k_uvh_kin(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(0, d.nj + 2, 1, d.ni);
-  const int K = 1 + blockIdx.y;
+  MEMBER_X;
+  FLAT2_X(0, d.nj + 2, 1, d.ni);
+  const int K = 1 + bx_.y;
   const double *u = d.U + (size_t)K * d.N2, *v = d.V + (size_t)K * d.N2, *h3 = d.H3 + (size_t)K * d.N2;
   double *uh = d.sA + (size_t)K * d.N2, *vh = d.sB + (size_t)K * d.N2, *kin = d.sC + (size_t)K * d.N2;
   const int ni = d.ni, nj = d.nj;
@@ -88,9 +98,9 @@ __global__ void __launch_bounds__(256) k_uvh_kin(const MemberDesc *D, int mb) {
 
 // DVDT = (DVDT + PV*UH terms + dKIN/dy) - vertical advection
 __global__ void __launch_bounds__(256) k_dvdt_core(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(1, d.nj, 1, d.ni);
-  const int K = 1 + blockIdx.y;
+  MEMBER_X;
+  FLAT2_X(1, d.nj, 1, d.ni);
+  const int K = 1 + bx_.y;
   const size_t o = (size_t)K * d.N2;
   const double *uh = d.sA + o, *vh = d.sB + o, *kin = d.sC + o, *pv = d.PV3 + o;
   double *dv = cur_tend(d, d.DVDT) + o;
@@ -367,9 +377,9 @@ __device__ __forceinline__ void pgrad_traditional(const MemberDesc &d, int K, in
 }
 
 __global__ void __launch_bounds__(256) k_pgrad(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(0, d.nj + 1, 1, d.ni);
-  const int K = 1 + blockIdx.y;
+  MEMBER_X;
+  FLAT2_X(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + bx_.y;
   const double TOL = 1.e-2, DG = M_PI / 180.;
   const double *G = d.GZ3, *P2 = d.P2;
   double *du = cur_tend(d, d.DUDT), *dv = cur_tend(d, d.DVDT);
@@ -483,9 +493,9 @@ void L_pgrad(hipStream_t s, const Bucket &b) {
 
 // ------------------------------------------------------ divergence damping
 __global__ void __launch_bounds__(256) k_divdamp(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(0, d.nj + 1, 1, d.ni);
-  const int K = 1 + blockIdx.y;
+  MEMBER_X;
+  FLAT2_X(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + bx_.y;
   if (d.nudiv == 0.) return;
   const double *div = d.sB + (size_t)K * d.N2;
   double *du = cur_tend(d, d.DUDT), *dv = cur_tend(d, d.DVDT);

@@ -28,6 +28,16 @@ namespace epic {
   if (t_ >= (nJ) * (nI)) return;                       \
   const int J = (j0) + t_ / (nI);                      \
   const int I = (i0) + t_ % (nI);
+// the same with the block renumbered for the XCDs (xcd_block): bx_.y and
+// bx_.z stand for blockIdx.y and blockIdx.z
+#define MEMBER_X                                       \
+  const uint3 bx_ = xcd_block();                       \
+  const MemberDesc &d = D[mb + bx_.z]
+#define FLAT2_X(j0, nJ, i0, nI)                        \
+  int t_ = bx_.x * TPB + threadIdx.x;                  \
+  if (t_ >= (nJ) * (nI)) return;                       \
+  const int J = (j0) + t_ / (nI);                      \
+  const int I = (i0) + t_ % (nI);
 
 static inline dim3 grid2(int nJ, int nI, int nK, int nm) {
   return dim3((unsigned)((nJ * nI + TPB - 1) / TPB), (unsigned)nK, (unsigned)nm);
@@ -43,9 +53,9 @@ __device__ __forceinline__ float *sCf(const MemberDesc &d) { return reinterpret_
 // ------------------------------------------------------------ tendencies
 // UH, VH, KIN (FP32 views of sA, sB, sC) on planes 1..nk-1 with zonal pads
 __global__ void __launch_bounds__(256) km_uvh_kin(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(0, d.nj + 2, 1, d.ni);
-  const int K = 1 + blockIdx.y;
+  MEMBER_X;
+  FLAT2_X(0, d.nj + 2, 1, d.ni);
+  const int K = 1 + bx_.y;
   const size_t o = (size_t)K * d.N2;
   const double *u = d.U + o, *v = d.V + o, *h3 = d.H3 + o;
   float *uh = sAf(d) + o, *vh = sBf(d) + o, *kin = sCf(d) + o;
@@ -86,9 +96,9 @@ __global__ void __launch_bounds__(256) km_uvh_kin(const MemberDesc *D, int mb) {
 }
 
 __global__ void __launch_bounds__(256) km_dvdt_core(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(1, d.nj, 1, d.ni);
-  const int K = 1 + blockIdx.y;
+  MEMBER_X;
+  FLAT2_X(1, d.nj, 1, d.ni);
+  const int K = 1 + bx_.y;
   const size_t o = (size_t)K * d.N2;
   const float *uh = sAf(d) + o, *kin = sCf(d) + o;
   const double *pv = d.PV3 + o;
@@ -111,9 +121,9 @@ __global__ void __launch_bounds__(256) km_dvdt_core(const MemberDesc *D, int mb)
 }
 
 __global__ void __launch_bounds__(256) km_dudt_core(const MemberDesc *D, int mb) {
-  MEMBER;
-  FLAT2(0, d.nj + 1, 1, d.ni);
-  const int K = 1 + blockIdx.y;
+  MEMBER_X;
+  FLAT2_X(0, d.nj + 1, 1, d.ni);
+  const int K = 1 + bx_.y;
   const size_t o = (size_t)K * d.N2;
   const float *vh = sBf(d) + o, *kin = sCf(d) + o;
   const double *pv = d.PV3 + o;

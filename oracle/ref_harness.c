@@ -171,7 +171,19 @@ static int spec_option(epic_case_spec *sp, const char *arg) {
 int main(int argc, char **argv) {
   epic_case_spec sp;
   memset(&sp, 0, sizeof(sp));
+  /* --leapfrog=1: the reference's "Leapfrog (Asselin filtered)" scheme,
+   * started as its `initial` program starts it (U and V of IT_MINUS1 flagged
+   * FLOAT_MAX on K = 1..nk, then timeplane_bookkeeping(),
+   * src/shared/epic_initial.c:1316-1339); the run reports the first step
+   * whose state is not finite */
+  int leapfrog = 0;
   while (argc > 1 && strncmp(argv[1], "--", 2) == 0) {
+    if (!strncmp(argv[1], "--leapfrog=", 11)) {
+      leapfrog = atoi(argv[1] + 11);
+      argv++;
+      argc--;
+      continue;
+    }
     if (spec_option(&sp, argv[1])) {
       fprintf(stderr, "unknown option %s\n", argv[1]);
       return 2;
@@ -235,7 +247,7 @@ int main(int argc, char **argv) {
   memset(&grid, 0, sizeof(grid));
   memset(&var, 0, sizeof(var));
   strcpy(grid.geometry, "globe");
-  strcpy(grid.uv_timestep_scheme, "3rd-order Adams-Bashforth");
+  strcpy(grid.uv_timestep_scheme, leapfrog ? "Leapfrog (Asselin filtered)" : "3rd-order Adams-Bashforth");
   strcpy(grid.turbulence_scheme, "off");
   strcpy(grid.eos, "ideal");
   strcpy(grid.f_plane_map, "none");
@@ -340,6 +352,31 @@ int main(int argc, char **argv) {
     for (int J = JLO; J <= JHI; J++)
       for (int I = ILO; I <= IHI; I++) HDRY(K, J, I) = get_h(planet, 2 * K, J, I, DRY);
   bc_lateral(var.hdry.value, THREEDIM);
+  if (leapfrog) {
+    for (int K = 1; K <= KHI; K++)
+      for (int J = JLO; J <= JHI; J++)
+        for (int I = ILO; I <= IHI; I++) {
+          U(IT_MINUS1, K, J, I) = FLOAT_MAX;
+          V(IT_MINUS1, K, J, I) = FLOAT_MAX;
+        }
+    timeplane_bookkeeping();
+    EPIC_FLOAT *B2[NUM_WORKING_BUFFERS];
+    for (int q = 0; q < NUM_WORKING_BUFFERS; q++) B2[q] = fvector(0, Nelem2d - 1, "harness");
+    grid.itime = 0;
+    timestep(planet, SYNC_DIAGS_ONLY, B2);
+    for (int s = 1; s <= nsteps; s++) {
+      grid.itime = s - 1;
+      timestep(planet, STEP_PROGS_AND_SYNC_DIAGS, B2);
+      long bad = 0;
+      for (int K = 0; K <= KHI; K++)
+        for (int J = JLO; J <= JHI; J++)
+          for (int I = ILO; I <= IHI; I++)
+            if (!(fabs(U(IT_ZERO, K, J, I)) < 1.e30) || !(fabs(V(IT_ZERO, K, J, I)) < 1.e30)) bad++;
+      printf("leapfrog step %d: %ld cells of the current u, v plane not finite or flagged\n", s, bad);
+      fflush(stdout);
+    }
+    return 0;
+  }
   /* AB3 start-up flags */
   for (int K = 1; K <= KHI; K++)
     for (int J = JLO; J <= JHI; J++)

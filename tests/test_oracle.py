@@ -165,6 +165,25 @@ def test_traditional_pressure_gradient_is_the_reference_function(args, opts, tmp
     r.close()
 
 
+@pytest.mark.skipif(not os.path.exists(REF_EXE), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("args", [(1, 32, 16, 10, 60, 5, 0.5, 0.0, 0.0, 1.0), (0, 32, 16, 8, 600, 5, 0.5, 0.0, 0.3, 0.3)],
+                         ids=["jupiter", "held_suarez"])
+def test_reference_leapfrog_does_not_survive_its_start_up_flags(args, tmp_path):
+    """Why the GPU step refuses "Leapfrog (Asselin filtered)": the reference's
+    own build, started as its `initial` program starts that scheme (U and V
+    of IT_MINUS1 flagged FLOAT_MAX on K = 1..nk), leaves flags in the
+    current plane after its first (forward) step -- the bottom interface
+    K = nk, which no step updates, and the v pole row; they spread through
+    the second step, and the third stops with a NaN pressure (calc_w's
+    check)."""
+    cmd = [REF_EXE, "--leapfrog=1"] + [str(a) for a in args] + ["4", str(tmp_path / "lf.bin")]
+    r = subprocess.run(cmd, cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    out = r.stdout + r.stderr
+    bad = [int(out.split(f"leapfrog step {s}: ")[1].split()[0]) for s in (1, 2)]
+    assert 0 < bad[0] < bad[1], bad
+    assert r.returncode != 0 and "nan" in out.lower() and "leapfrog step 4:" not in out
+
+
 def test_richardson_of_given_fields_reproduces_store_diag():
     """The function form the GPU tests use for RI2 (oracle fields written in,
     RI2 formed again) gives store_diag's RI2 bit for bit, and the golden
