@@ -232,11 +232,13 @@ class Engine:
         self.specs[slot] = sp
         return slot
 
-    def add_case(self, c):
+    def add_case(self, c, spec=None):
         """A member from a built or read case (epic_engine_add); the engine
-        copies what it needs, the caller keeps and frees the case."""
+        copies what it needs, the caller keeps and frees the case.  With
+        `spec` (the spec the case was built from) the engine keeps that
+        instead of the case, which the caller may then free at once."""
         slot = self._check(lib().epic_engine_add(self._e, ctypes.byref(c)), "add")
-        self.specs[slot] = c
+        self.specs[slot] = c if spec is None else spec
         return slot
 
     def write_nc(self, slot, path, case=None):

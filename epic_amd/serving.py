@@ -17,6 +17,8 @@ import dataclasses
 import math
 import random
 import time
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor
+from concurrent.futures import wait as wait_futures
 
 import numpy as np
 
@@ -93,10 +95,19 @@ def poisson_mix(n, rate, seed, grids=GRIDS, planets=PLANETS, steps=(200, 2000)):
 
 
 class Server:
-    """Admission, stepping and retirement of resident integrations."""
+    """Admission, stepping and retirement of resident integrations.
+
+    Host work overlaps the device's: the initial state of an arrived request
+    is built (epic_case_build) on a worker thread while the admitted members
+    step, and the request joins at the first tick after its case is ready
+    (the device is never held idle for a build while others can step).  A
+    tick advances every member by up to `chunk` steps, cut short where a
+    member finishes or where the next request that would fit arrives (from
+    the measured time per step of the current mix), so an arrival waits
+    about one step, not a whole chunk, for its tick."""
 
     def __init__(self, engine, hbm_budget, max_members, precision=ea.FP64, chunk=50, keep_fields=False,
-                 footprint=None, clock=time.perf_counter, sleep=time.sleep):
+                 footprint=None, clock=time.perf_counter, sleep=time.sleep, prebuild=2):
         self.eng = engine
         self.budget = hbm_budget
         self.max_members = max_members
@@ -106,9 +117,46 @@ class Server:
         self.footprint = footprint or (lambda r: ea.footprint(r.ni, r.nj, r.nk, precision))
         self.clock = clock
         self.sleep = sleep
-        self.active = {}   # slot -> [request, steps_left, admitted_time]
+        # cases built ahead on a worker thread (engines that take built cases)
+        self.depth = prebuild if hasattr(engine, "add_case") else 0
+        self.pool = ThreadPoolExecutor(1) if self.depth > 0 else None
+        self.built = {}    # rid -> Future of the request's built case
+        self.pending = {}  # rid -> request, for the builds in self.built
+        self.active = {}   # slot -> [request, steps_left, admitted_time, footprint]
         self.in_use = 0
         self.point_steps = 0
+        self.t_step = None  # seconds per step of the current mix (last tick)
+        self.ticks = 0
+        self.host_s = 0.0   # time in admission and retirement (device idle)
+
+    def _fits(self, r):
+        return len(self.active) < self.max_members and self.in_use + self.footprint(r) <= self.budget
+
+    def _prebuild(self, waiting, now):
+        """Start the builds of arrived requests, in arrival order, that would
+        fit beside the admitted members and the builds already started."""
+        n, fp_sum = len(self.built), sum(self.footprint(r) for r in self.pending.values())
+        for r in waiting:
+            if n >= self.depth or r.arrival > now:
+                break
+            if r.rid in self.built:
+                continue
+            fp = self.footprint(r)
+            if len(self.active) + n < self.max_members and self.in_use + fp_sum + fp <= self.budget:
+                self.built[r.rid] = self.pool.submit(ea.build_case, r.spec())
+                self.pending[r.rid] = r
+                n += 1
+                fp_sum += fp
+
+    def _add(self, r):
+        if self.pool is None:
+            return self.eng.add(r.spec())
+        c = self.built.pop(r.rid).result()
+        self.pending.pop(r.rid)
+        try:
+            return self.eng.add_case(c, spec=r.spec())
+        finally:
+            ea.free_case(c)
 
     def _admit(self, waiting, now):
         admitted = []
@@ -118,7 +166,11 @@ class Server:
             fp = self.footprint(r)
             if self.in_use + fp > self.budget:
                 continue
-            slot = self.eng.add(r.spec())
+            if self.pool is not None:
+                f = self.built.get(r.rid)
+                if f is None or (not f.done() and self.active):
+                    continue  # joins at a later tick; the others keep stepping
+            slot = self._add(r)
             self.active[slot] = [r, r.steps, now, fp]
             self.in_use += fp
             waiting.remove(r)
@@ -139,6 +191,23 @@ class Server:
             status = "ok" if err == 0 else f"failed: {err}"
             results.append(Result(r.rid, status, r.arrival, t_adm, now, r.steps - left, checksum, fields))
 
+    def _tick_steps(self, waiting, now):
+        """Steps of the next tick: up to `chunk`, no member past its end,
+        ending about when the next request that would fit arrives, or soon
+        after an arrived request's case is built."""
+        n = min(self.chunk, min(v[1] for v in self.active.values()))
+        if not self.t_step:
+            return n
+        cut = []
+        nxt = next((r for r in waiting if r.arrival > now), None)
+        if nxt is not None and self._fits(nxt):
+            cut.append(nxt.arrival - now)
+        if self.built:
+            cut.append(0.05)
+        if cut:
+            n = min(n, max(1, math.ceil(min(cut) / self.t_step)))
+        return n
+
     def run(self, requests, time_limit=None):
         """Serve `requests` (arrival times relative to the call); returns the
         results in completion order."""
@@ -150,23 +219,43 @@ class Server:
             results.append(Result(r.rid, "rejected: footprint exceeds the HBM budget", r.arrival, r.arrival,
                                   r.arrival, 0, 0.0))
         t0 = self.clock()
-        while waiting or self.active:
-            now = self.clock() - t0
-            if time_limit is not None and now > time_limit:
-                break
-            self._admit(waiting, now)
-            if not self.active:
-                # idle until the next arrival
-                self.sleep(max(0.0, waiting[0].arrival - (self.clock() - t0)))
-                continue
-            n = min(self.chunk, min(v[1] for v in self.active.values()))
-            self.eng.step(n)
-            self.eng.wait()
-            for v in self.active.values():
-                v[1] -= n
-                self.point_steps += v[0].points * n
-            now = self.clock() - t0
-            self._retire([s for s, v in self.active.items() if v[1] == 0], now, results)
+        try:
+            while waiting or self.active:
+                now = self.clock() - t0
+                if time_limit is not None and now > time_limit:
+                    break
+                if self.pool is not None:
+                    self._prebuild(waiting, now)
+                h0 = self.clock()
+                self._admit(waiting, now)
+                self.host_s += self.clock() - h0
+                if not self.active:
+                    # idle until the next arrival (or the next built case)
+                    if self.built:
+                        wait_futures(list(self.built.values()), return_when=FIRST_COMPLETED)
+                    elif waiting:
+                        self.sleep(max(0.0, waiting[0].arrival - (self.clock() - t0)))
+                    continue
+                n = self._tick_steps(waiting, now)
+                s0 = self.clock()
+                self.eng.step(n)
+                self.eng.wait()
+                self.t_step = (self.clock() - s0) / n
+                self.ticks += 1
+                for v in self.active.values():
+                    v[1] -= n
+                    self.point_steps += v[0].points * n
+                now = self.clock() - t0
+                h0 = self.clock()
+                self._retire([s for s, v in self.active.items() if v[1] == 0], now, results)
+                self.host_s += self.clock() - h0
+        finally:
+            if self.pool is not None:
+                self.pool.shutdown(wait=True)
+                for f in self.built.values():
+                    ea.free_case(f.result())
+                self.built.clear()
+                self.pending.clear()
         self.elapsed = self.clock() - t0
         return results
 
@@ -191,4 +280,7 @@ def serve(requests, precision=ea.FP64, hbm_budget=64 << 30, max_members=64, chun
     srv = Server(eng, hbm_budget, max_members, precision, chunk, keep_fields)
     res = srv.run(requests)
     eng.close()
-    return res, summarize(res, srv.elapsed, srv.point_steps)
+    summ = summarize(res, srv.elapsed, srv.point_steps)
+    summ["ticks"] = srv.ticks
+    summ["host_s"] = srv.host_s  # admission and retirement, device idle
+    return res, summ

@@ -2,6 +2,8 @@
 and free slots, continuous batching (work joins while other work is on
 the device), retirement, latency bookkeeping.  The engine is a host-side
 stand-in with the same methods; the GPU test runs the real one."""
+import time
+
 import numpy as np
 
 import epic_amd as ea
@@ -105,3 +107,48 @@ def test_poisson_mix_is_seeded():
     a = sv.poisson_mix(10, 5.0, 7)
     b = sv.poisson_mix(10, 5.0, 7)
     assert a == b and all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
+
+
+class FakeCaseEngine(FakeEngine):
+    """Takes cases built ahead (on the server's worker thread) as the real
+    engine's add_case does."""
+
+    def __init__(self, clock):
+        super().__init__(clock)
+        self.from_cases = 0
+
+    def step(self, n):
+        super().step(n)
+        time.sleep(n * 1e-3)  # device time passes while the worker builds
+
+    def add_case(self, c, spec=None):
+        assert (c.ni, c.nj, c.nk, c.dt) == (spec.ni, spec.nj, spec.nk, spec.dt)
+        self.from_cases += 1
+        return self.add(spec)
+
+
+def test_cases_built_ahead_join_while_others_step():
+    reqs = sv.poisson_mix(12, rate=5000.0, seed=5, grids=[(64, 32, 10), (128, 64, 16)], steps=(20, 200))
+    clk = Clock()
+    eng = FakeCaseEngine(clk)
+    srv = sv.Server(eng, 1 << 40, max_members=4, chunk=25, footprint=footprint, clock=clk, sleep=clk.sleep)
+    res = srv.run(reqs)
+    assert len(res) == 12 and all(r.status == "ok" for r in res)
+    assert eng.from_cases == 12 and not srv.built and not srv.pending
+    assert eng.max_resident <= 4 and eng.joined_while_busy > 0
+    assert srv.point_steps == sum(r.steps * r.points for r in reqs)
+    for r in res:
+        req = reqs[r.rid]
+        assert r.steps == req.steps and r.checksum == req.steps * req.ni * (req.nj + 1) * req.nk
+
+
+def test_arrival_joins_within_a_step_not_a_chunk():
+    clk = Clock()
+    eng = FakeEngine(clk)
+    first = sv.Request(0, ea.HELD_SUAREZ, 64, 32, 10, 100000, 1, 0.0)
+    late = sv.Request(1, ea.HELD_SUAREZ, 64, 32, 10, 10, 2, 0.01)
+    srv = sv.Server(eng, 1 << 40, max_members=4, chunk=500, footprint=footprint, clock=clk, sleep=clk.sleep)
+    res = {r.rid: r for r in srv.run([first, late])}
+    step = first.points * 1e-10  # the fake engine's time per step with one member
+    assert 0 <= res[1].admitted - late.arrival <= 1.01 * step
+    assert res[0].steps == first.steps and res[1].steps == late.steps
