@@ -23,6 +23,8 @@ FIELDS = ["U", "V", "HDRY", "THETA", "P3", "P2", "H3", "THETA2", "T2", "T3", "RH
           "EXNER3", "GZ3", "MONT3", "PV3", "DIV_UV3", "W3", "HEAT3", "DZDT3", "DUDT0", "DUDT1",
           "DUDT2", "DVDT0", "DVDT1", "DVDT2", "GZ_SURFACE", "RI2"]
 FIELD = {n: i for i, n in enumerate(FIELDS)}
+# launch-shape knobs of the autotuner (epic_engine_set_shape's numbering)
+SHAPE_KNOBS = ["cols_p2etc", "cols_gz3", "cols_calcw", "cols_hsuv", "rows_zhv", "rows_mer"]
 
 
 K_SPONGE_DEFAULT = -1  # the reference `initial` program's choice of k_sponge
@@ -188,6 +190,7 @@ def lib():
         L.epic_engine_time_planes.argtypes = [vp, ci, ctypes.POINTER(ci)]
         L.epic_engine_profile.argtypes = [vp, ci, vp, ctypes.POINTER(cd), ctypes.POINTER(ci), ci]
         L.epic_engine_autotune.argtypes = [vp, ci, vp, ci]
+        L.epic_engine_set_shape.argtypes = [vp, ci, ci]
         L.epic_copy_probe.argtypes = [ctypes.c_size_t, ci, ctypes.POINTER(cd)]
         L.epic_member_footprint.argtypes = [ci, ci, ci, ci]
         L.epic_member_footprint.restype = ctypes.c_size_t
@@ -344,6 +347,12 @@ class Engine:
         buf = ctypes.create_string_buffer(4096)
         self._check(lib().epic_engine_autotune(self._e, steps, buf, len(buf)), "autotune")
         return buf.value.decode()
+
+    def set_shape(self, knob, value):
+        """One launch shape for every bucket by hand (epic_engine_set_shape);
+        knob by name or number."""
+        k = SHAPE_KNOBS.index(knob) if isinstance(knob, str) else knob
+        self._check(lib().epic_engine_set_shape(self._e, k, value), "set_shape")
 
     def close(self):
         if self._e:

@@ -1095,6 +1095,12 @@ static const char *const knob_name[KN_COUNT] = {"cols_p2etc", "cols_gz3", "cols_
 
 static bool knob_fits(const Bucket &b, int k, int v) {
   const size_t W = (size_t)b.nk + 1, L = (size_t)b.nk + 2, lim = 128 * 1024;
+  if (k <= KN_COLS_HSUV) {
+    // lanes per column beyond twice the column's length would idle
+    const int C = knob_cols(v);
+    if ((size_t)(knob_threads(v) / C) > 2 * L) return false;
+    v = C;
+  }
   switch (k) {
     case KN_COLS_P2ETC: return sizeof(double) * (2 * v * W + 3 * v) <= lim;
     case KN_COLS_GZ3: return sizeof(double) * 3 * v * W <= lim;
@@ -1104,6 +1110,16 @@ static bool knob_fits(const Bucket &b, int k, int v) {
     default:  // KN_ROWS_ZHV: one row array per row, and at most 16 points per lane
       return sizeof(double) * ((size_t)v * ((b.ni + 2) | 1) + v) <= lim && v / 4 * (b.ni >= 64 ? b.ni / 64 : 1) <= 16;
   }
+}
+
+// the candidates: column tiles of 8-64 columns per block of 256, 512 or
+// 1024 threads; 4-16 rows per zonal-hyperviscosity block; 2-8 rows per
+// meridional-advection tile
+static std::vector<int> knob_candidates(int k) {
+  static const int cols[12] = {8, 16, 32, 64, 2008, 2016, 2032, 2064, 4008, 4016, 4032, 4064};
+  static const int zhv[3] = {4, 8, 16}, mer[3] = {2, 4, 8};
+  if (k <= KN_COLS_HSUV) return std::vector<int>(cols, cols + 12);
+  return k == KN_ROWS_ZHV ? std::vector<int>(zhv, zhv + 3) : std::vector<int>(mer, mer + 3);
 }
 
 static bool knob_times(const char *kernel, int k) {
@@ -1142,8 +1158,6 @@ extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int
       M.steps = done[i];
     }
   };
-  static const int cand[KN_COUNT][4] = {{8, 16, 32, 64}, {8, 16, 32, 64}, {8, 16, 32, 64},
-                                        {8, 16, 32, 64}, {4, 8, 16},      {2, 4, 8}};
   size_t used = 0;
   for (Bucket &b : E->buckets) {
     bool hv, divd, adj;
@@ -1151,7 +1165,7 @@ extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int
     for (int k = 0; k < KN_COUNT; k++) {
       double best = 1e300;
       int bestv = b.knob[k];
-      for (int v : cand[k]) {
+      for (int v : knob_candidates(k)) {
         if (!v || !knob_fits(b, k, v)) continue;
         b.knob[k] = v;
         restore();
@@ -1177,9 +1191,14 @@ extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int
       }
       b.knob[k] = bestv;
       if (report && used < (size_t)cap) {
+        char shape[32];
+        if (k <= KN_COLS_HSUV && knob_threads(bestv) != 256)
+          snprintf(shape, sizeof(shape), "%d/%dthr", knob_cols(bestv), knob_threads(bestv));
+        else
+          snprintf(shape, sizeof(shape), "%d", bestv);
         if (best < 1e300)
-          used += snprintf(report + used, cap - used, "%dx%dx%d x%d %s=%d (%.1f us/step)\n", b.ni, b.nj, b.nk,
-                           b.nm, knob_name[k], bestv, best / steps * 1e3);
+          used += snprintf(report + used, cap - used, "%dx%dx%d x%d %s=%s (%.1f us/step)\n", b.ni, b.nj, b.nk,
+                           b.nm, knob_name[k], shape, best / steps * 1e3);
         else
           used += snprintf(report + used, cap - used, "%dx%dx%d x%d %s=built-in (no candidate fits)\n", b.ni,
                            b.nj, b.nk, b.nm, knob_name[k]);
@@ -1193,6 +1212,38 @@ extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int
   HIPCHK(hipStreamSynchronize(E->stream));
   release();
   HIPCHK(hipGetLastError());
+  drop_graph(E);
+  return 0;
+}
+
+extern "C" int epic_engine_set_shape(epic_engine *E, int knob, int value) {
+  if (!E || knob < 0 || knob >= KN_COUNT) return -1;
+  if (E->dirty && rebuild_buckets(E)) return -5;
+  if (value) {
+    const std::vector<int> c = knob_candidates(knob);
+    if (std::find(c.begin(), c.end(), value) == c.end()) {
+      snprintf(E->err, sizeof(E->err), "set_shape: %d is not a candidate of %s", value, knob_name[knob]);
+      return -1;
+    }
+    for (Bucket &b : E->buckets)
+      if (!knob_fits(b, knob, value)) {
+        snprintf(E->err, sizeof(E->err), "set_shape: %s=%d does not fit the %dx%dx%d bucket", knob_name[knob],
+                 value, b.ni, b.nj, b.nk);
+        return -1;
+      }
+  }
+  HIPCHK(hipStreamSynchronize(E->stream));
+  for (Bucket &b : E->buckets) {
+    b.knob[knob] = value;
+    auto t = E->tuned.find({b.ni, b.nj, b.nk, b.nm});
+    if (t == E->tuned.end()) {
+      std::array<int, KN_COUNT> kv{};
+      kv[knob] = value;
+      E->tuned[{b.ni, b.nj, b.nk, b.nm}] = kv;
+    } else {
+      t->second[knob] = value;
+    }
+  }
   drop_graph(E);
   return 0;
 }

@@ -75,7 +75,7 @@ __device__ __forceinline__ double old_value(double staged, const double *) { ret
 __device__ __forceinline__ double old_value(float, const double *p) { return *p; }
 
 template <class T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_hsu_vert(const MemberDesc *D, int mb, int nm, int C) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_hsu_vert(const MemberDesc *D, int mb, int nm, int C) {
   ADV_MEMBER;
   extern __shared__ double lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
@@ -391,9 +391,10 @@ static void hsu3_t(hipStream_t s, const Bucket &b) {
   // columns per block: as many as keep the four LDS columns within 40 KB
   int C = 64;
   while (C > 16 && (size_t)4 * C * (b.nk + 2) * sizeof(double) > 40 * 1024) C /= 2;
-  if (b.knob[KN_COLS_HSUV]) C = b.knob[KN_COLS_HSUV];
-  LAUNCH(k_hsu_vert<T>, dim3((ncol + C - 1) / C, 1, nz), dim3(256), sizeof(T) * 4 * C * (b.nk + 2), s, b.D, b.mb,
-         b.nm, C);
+  const int v = b.knob[KN_COLS_HSUV];
+  if (v) C = knob_cols(v);
+  LAUNCH(k_hsu_vert<T>, dim3((ncol + C - 1) / C, 1, nz), dim3(knob_threads(v)), sizeof(T) * 4 * C * (b.nk + 2), s,
+         b.D, b.mb, b.nm, C);
   const int TJ = b.knob[KN_ROWS_MER] ? b.knob[KN_ROWS_MER] : 4;
   const int CW = b.ni < 64 ? b.ni : 64, tiles = (b.nj + 1 + TJ - 1) / TJ;
   LAUNCH(k_hsu_mer<T>, dim3(tiles * (b.ni / CW), b.nk, nz), dim3(256),

@@ -138,7 +138,7 @@ static inline dim3 tile_grid(const Bucket &b, int C) {
 // the step leaves are the same.
 __device__ __forceinline__ void thermo_cell(const MemberDesc &d, int K, int J, int I, bool full, double th, double p);
 template <int FULL>
-__global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, int C) {
+__global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb, int C) {
   constexpr bool update_theta = FULL;
   MEMBER;
   extern __shared__ double lds[];
@@ -254,10 +254,10 @@ __global__ void __launch_bounds__(256) k_set_p2etc(const MemberDesc *D, int mb, 
     for (int K = T.kk; K <= nk; K += T.KS) thermo_cell(d, K, J, I, false, d.THETA[ix3(d, K, J, I)], a[K]);
 }
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
-  const int C = b.knob[KN_COLS_P2ETC] ? b.knob[KN_COLS_P2ETC] : tile_cols(b, 2);
+  const int v = b.knob[KN_COLS_P2ETC], C = v ? knob_cols(v) : tile_cols(b, 2), nt = knob_threads(v);
   size_t shm = sizeof(double) * (2 * C * (b.nk + 1) + 3 * C);
-  if (update_theta) LAUNCH(k_set_p2etc<1>, tile_grid(b, C), dim3(TPB), shm, s, b.D, b.mb, C);
-  else LAUNCH(k_set_p2etc<0>, tile_grid(b, C), dim3(TPB), shm, s, b.D, b.mb, C);
+  if (update_theta) LAUNCH(k_set_p2etc<1>, tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
+  else LAUNCH(k_set_p2etc<0>, tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
 }
 
 // ------------------------------------------------- store_pgrad_vars parts
@@ -536,7 +536,7 @@ void L_gz_from_u(hipStream_t s, const Bucket &b) {
 
 // hydrostatic GZ3 and MONT3 from the bottom up (Leslie-Purser 4-point
 // integral in ln p) and the monotonicity screen
-__global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb, int C) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_gz3(const MemberDesc *D, int mb, int C) {
   MEMBER;
   extern __shared__ double lds[];
   const Tile T = tile_of(d, C);
@@ -628,8 +628,8 @@ __global__ void __launch_bounds__(256) k_gz3(const MemberDesc *D, int mb, int C)
   }
 }
 void L_gz3(hipStream_t s, const Bucket &b) {
-  const int C = b.knob[KN_COLS_GZ3] ? b.knob[KN_COLS_GZ3] : tile_cols(b, 3);
-  LAUNCH(k_gz3, tile_grid(b, C), dim3(TPB), sizeof(double) * 3 * C * (b.nk + 1), s, b.D, b.mb, C);
+  const int v = b.knob[KN_COLS_GZ3], C = v ? knob_cols(v) : tile_cols(b, 3);
+  LAUNCH(k_gz3, tile_grid(b, C), dim3(knob_threads(v)), sizeof(double) * 3 * C * (b.nk + 1), s, b.D, b.mb, C);
 }
 
 // finish DZDT3 (diagnostic vertical velocity)
@@ -791,7 +791,7 @@ void L_heating(hipStream_t s, const Bucket &b, int adjust, bool in_thermo) {
 // With with_source (the step's call) the blocks past the ncb column tiles
 // first clear the step's tendency planes of u and v (the FP64 plane IT_ZERO,
 // or the mixed mode's one FP64 plane), which nothing in this launch reads.
-__global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int with_source, int C, int ncb) {
+__global__ void __launch_bounds__(1024) k_calc_w(const MemberDesc *D, int mb, int with_source, int C, int ncb) {
   MEMBER;
   if ((int)blockIdx.x >= ncb) {
     const size_t off = d.mixed ? 0 : (size_t)d.dyn->it0 * d.N3;
@@ -891,15 +891,15 @@ __global__ void __launch_bounds__(256) k_calc_w(const MemberDesc *D, int mb, int
   if (bad) atomicOr(&d.dyn->err, 2);
 }
 void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
-  const int C = b.knob[KN_COLS_CALCW] ? b.knob[KN_COLS_CALCW] : tile_cols(b, 4);
+  const int v = b.knob[KN_COLS_CALCW], C = v ? knob_cols(v) : tile_cols(b, 4), nt = knob_threads(v);
   dim3 g = tile_grid(b, C);
   const int ncb = (int)g.x;
   if (with_source) {
     // blocks that clear the tendency planes: about 16 cells per lane
     const size_t n3 = (size_t)(b.ni + 2) * (b.nj + 3) * (b.nk + 2);
-    g.x += (unsigned)std::max<size_t>(1, std::min<size_t>(1024, n3 / (TPB * 16)));
+    g.x += (unsigned)std::max<size_t>(1, std::min<size_t>(1024, n3 / ((size_t)nt * 16)));
   }
-  LAUNCH(k_calc_w, g, dim3(TPB), sizeof(double) * 4 * C * (b.nk + 1), s, b.D, b.mb, with_source, C, ncb);
+  LAUNCH(k_calc_w, g, dim3(nt), sizeof(double) * 4 * C * (b.nk + 1), s, b.D, b.mb, with_source, C, ncb);
 }
 
 // ------------------------------------------------ vorticity / divergence

@@ -401,13 +401,21 @@ void epic_nc_header_free(void *header);
 /* Autotuner (no counterpart in the reference, whose loops have no launch
  * shapes): for each bucket, time every candidate launch shape of the tunable
  * kernels (columns per column tile of set_p2etc, gz3, calc_w and the
- * vertical advection pass; rows per zonal-hyperviscosity block and per
- * meridional-advection tile) over
+ * vertical advection pass, and the tile's threads; rows per
+ * zonal-hyperviscosity block and per meridional-advection tile) over
  * `steps` steps of the bucket's own members and keep the fastest.  The
  * members' state is copied aside and put back, so the call changes no
  * result and no state; it needs room for one copy of every member.  Writes
  * one line per choice into report (up to cap bytes).  0 or a negative code. */
 int  epic_engine_autotune(epic_engine *e, int steps, char *report, int cap);
+/* One launch shape set by hand for every bucket (knob: 0 columns per
+ * set_p2etc tile, 1 gz3, 2 calc_w, 3 vertical advection, 4 rows per
+ * zonal-hyperviscosity block, 5 rows per meridional-advection tile; a
+ * column-tile value is C, or 1000 n + C for blocks of 256 n threads, n = 2
+ * or 4; 0 restores the built-in choice).  Values outside the candidates the
+ * autotuner tries, or whose LDS would not fit, are refused (-1).  For tests
+ * that every shape gives the same bits. */
+int  epic_engine_set_shape(epic_engine *e, int knob, int value);
 const char *epic_engine_last_error(epic_engine *e);
 
 /* ------------------------------------------------------------------------

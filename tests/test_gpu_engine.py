@@ -118,6 +118,49 @@ def test_autotune_changes_no_state_and_no_result():
 
 
 @pytest.mark.parametrize("precision", [ea.FP64, ea.MIXED])
+def test_every_column_tile_shape_gives_the_same_bits(precision):
+    """Candidate shapes of the column-tile kernels (16-64 columns per block
+    of 256, 512 or 1024 threads), set by hand on one knob at a time, step a
+    Held-Suarez and a Jupiter-like member to the same bits as the built-in
+    shapes."""
+    members = (A, ea.spec(ea.JUPITER_LIKE, 64, 32, 20, 60, 5, 0.5, 0.0, 0.0, 0.5))
+
+    def run(knob=None, v=0):
+        e = ea.Engine(precision, 2)
+        sl = [e.add(sp) for sp in members]
+        if knob is not None:
+            e.set_shape(knob, v)
+        e.step(3)
+        out = [{f: e.read(s, f) for f in PROG + ("GZ3", "W3")} for s in sl]
+        e.close()
+        return out
+
+    ref = run()
+    bad = []
+    for knob in ("cols_p2etc", "cols_gz3", "cols_calcw", "cols_hsuv"):
+        for v in (16, 64, 2032, 2064, 4064):
+            say(precision, knob, v)
+            got = run(knob, v)
+            for m, (g, r) in enumerate(zip(got, ref)):
+                bad += [(knob, v, m, f) for f in r if g[f].tobytes() != r[f].tobytes()]
+    assert not bad
+
+
+def test_set_shape_refuses_what_is_not_a_candidate():
+    e = ea.Engine(ea.FP64, 1)
+    e.add(A)
+    # not candidates; a knob that does not exist; 32 lanes per 10-level
+    # column (more than twice its 12 points)
+    for knob, v in (("cols_p2etc", 48), ("cols_gz3", 3016), ("rows_zhv", 32), (9, 8), ("cols_p2etc", 8)):
+        with pytest.raises(ea.EpicError):
+            e.set_shape(knob, v)
+    e.set_shape("cols_gz3", 4064)
+    e.set_shape("cols_gz3", 0)
+    e.step(1)
+    e.close()
+
+
+@pytest.mark.parametrize("precision", [ea.FP64, ea.MIXED])
 def test_checkpoint_restart_is_exact(tmp_path, precision):
     """A member restored from a checkpoint (state, Adams-Bashforth history,
     time planes) continues bit for bit as the original, also when it lands
