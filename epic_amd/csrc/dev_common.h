@@ -52,6 +52,9 @@ struct MemberDesc {
   int is_tab, n_snd, n_cool;
   // the traditional pressure gradient in place of the finite-volume one
   int pgrad_trad;
+  // advection scheme per advected field (unity, theta, hdry): 1 Akima, 0
+  // the predictor-corrector of Hsu, Konor and Arakawa
+  int adv_akima[3];
   const double *snd_x, *snd_y, *snd_d, *cool_x, *cool_y, *cool_d;
   // host-libm tables: cos(lat), cos(lat)^4 per h-row J, cos(lon) per column I
   const double *coslat_h, *coslat4_h, *coslon_h;

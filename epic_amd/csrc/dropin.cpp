@@ -25,6 +25,7 @@
 
 #include <vector>
 
+#include "case_internal.h"
 #include "engine_internal.h"
 
 extern "C" {
@@ -125,6 +126,13 @@ void download(const MemberDesc &d) {
   need(hipStreamSynchronize(s) == hipSuccess, "download failed");
 }
 
+// index of an advection scheme's name (epic_case_spec.adv_mass), -1 if none
+static int adv_scheme(const char *name) {
+  if (strcmp(name, EPIC_ADV_HSU) == 0) return 0;
+  if (strcmp(name, EPIC_ADV_AKIMA) == 0) return 1;
+  return -1;
+}
+
 // The configurations the GPU step implements (the reference's defaults for
 // a dry globe).
 void check_config(const struct epic_planet_abi *planet) {
@@ -140,9 +148,8 @@ void check_config(const struct epic_planet_abi *planet) {
        "species, microphysics and NMT physics are not implemented");
   need(!var.fpara.on && !var.nu_turb.on, "fpara and nu_turb are not implemented");
   need(var.hdry.on && var.theta.on, "hdry and theta must be on");
-  need(strcmp(var.theta.advection_scheme, "Predictor-corrector (Hsu, Konor, and Arakawa)") == 0 &&
-           strcmp(var.hdry.advection_scheme, var.theta.advection_scheme) == 0,
-       "only the Hsu-Konor-Arakawa advection scheme is implemented");
+  need(adv_scheme(var.theta.advection_scheme) >= 0 && adv_scheme(var.hdry.advection_scheme) >= 0,
+       "only the Hsu-Konor-Arakawa and Akima advection schemes are implemented");
   need(var.p3.value == var.pdry3.value && var.h3.value == var.hdry3.value,
        "dry runs alias p3/pdry3 and h3/hdry3");
   need(strcmp(planet->type, "gas-giant") == 0 || strcmp(planet->type, "terrestrial") == 0,
@@ -178,6 +185,8 @@ void create_member(const struct epic_planet_abi *planet) {
   c.newt_cool_on = grid.newt_cool_on; c.newt_cool_adjust = grid.newt_cool_adjust;
   c.drag_v = grid.drag_v; c.drag_zonal_avg = grid.drag_zonal_avg;
   c.pgrad_traditional = g_di.pgrad_traditional;
+  c.adv_mass = adv_scheme(var.hdry.advection_scheme);
+  c.adv_nonmass = adv_scheme(var.theta.advection_scheme);
   c.dln = grid.dln; c.dlt = grid.dlt; c.dy0 = grid.dy0;
   c.globe_lonbot = grid.globe_lonbot; c.globe_lontop = grid.globe_lontop;
   c.globe_latbot = grid.globe_latbot; c.globe_lattop = grid.globe_lattop;

@@ -176,12 +176,14 @@ static int rebuild_buckets(epic_engine *E) {
     b.mixed = (E->precision == EPIC_MIXED);
     b.any_gas = b.any_hs = b.any_ray = 0;
     b.hv_passes = 0;
+    b.adv_schemes = 0;
     for (size_t k = i; k < j; k++) {
       const MemberDesc &h = hd[k];
       b.hv_passes = std::max(b.hv_passes, hv_passes(h));
       b.any_gas |= (h.is_gas != 0);
       b.any_hs |= (h.is_hs != 0);
       b.any_ray |= ((h.newt_cool_on && h.prandtl > 0.) || (h.tau_drag > 0. && h.tau_drag < 1.e+20));
+      b.adv_schemes |= adv_schemes(h);
     }
     auto t = E->tuned.find({b.ni, b.nj, b.nk, b.nm});
     for (int k = 0; k < KN_COUNT; k++) b.knob[k] = (t == E->tuned.end()) ? 0 : t->second[k];
@@ -516,6 +518,8 @@ void member_plan(const epic_case *c, bool mixed, const epic_tables &T, MemberDes
   d.n_snd = T.n_snd;
   d.n_cool = T.n_cool;
   d.pgrad_trad = c->pgrad_traditional ? 1 : 0;
+  d.adv_akima[0] = d.adv_akima[1] = c->adv_nonmass ? 1 : 0;  // unity and theta
+  d.adv_akima[2] = c->adv_mass ? 1 : 0;                      // hdry
   d.heat_area = 0.;
   for (int J = 0; J <= c->nj; J++) {
     const double da = 1. / c->mn[2 * J + 1];

@@ -310,6 +310,7 @@ int epic_case_dsgth(epic_case *c) { return build_dsgth(c) ? -3 : 0; }
 int epic_case_build(const epic_case_spec *s, epic_case *c) {
   memset(c, 0, sizeof(*c));
   if (s->ni < 8 || (s->ni & (s->ni - 1)) || s->nj < 4 || s->nk < 4 || s->dt <= 0) return -1;
+  if (s->adv_mass < 0 || s->adv_mass > 1 || s->adv_nonmass < 0 || s->adv_nonmass > 1) return -1;
   int ni = s->ni, nj = s->nj, nk = s->nk;
   c->ni = ni;
   c->nj = nj;
@@ -424,6 +425,8 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   c->drag_zonal_avg = s->drag_zonal_avg ? 1 : 0;
   c->newt_cool_adjust = s->newt_cool_adjust ? 1 : 0;
   c->pgrad_traditional = s->pgrad_traditional ? 1 : 0;
+  c->adv_mass = s->adv_mass;
+  c->adv_nonmass = s->adv_nonmass;
   c->nudiv_nondim = s->nudiv_nondim;
   /* Newtonian cooling switched on for a planet with no relaxation of its
    * own relaxes toward a sounding with a cooling-time table, as the

@@ -266,13 +266,16 @@ int epic_nc_write(const char *path, const epic_case *c, const epic_nc_state *st)
     const ncio_att *a = h ? ncio_att_of(h, -1, "grid_stability_factor") : NULL;
     if (!a) w_text(&f, "grid_stability_factor", "off", N_STR);
   }
-  w_text(&f, "var_hdry_advection_scheme", "Predictor-corrector (Hsu, Konor, and Arakawa)", N_STR);
-  w_text(&f, "var_theta_advection_scheme", "Predictor-corrector (Hsu, Konor, and Arakawa)", N_STR);
   {
+    // mass scheme: hdry and the species; non-mass: theta, fpara, nu_turb
+    // (as the reference's initial program assigns them)
+    const char *mass = c->adv_mass ? EPIC_ADV_AKIMA : EPIC_ADV_HSU, *nonmass = c->adv_nonmass ? EPIC_ADV_AKIMA : EPIC_ADV_HSU;
+    w_text(&f, "var_hdry_advection_scheme", mass, N_STR);
+    w_text(&f, "var_theta_advection_scheme", nonmass, N_STR);
     const char *other[3] = {"var_fpara_advection_scheme", "var_nu_turb_advection_scheme",
                             "var_species_advection_scheme"};
     for (int q = 0; q < 3; q++)
-      if (!(h && ncio_att_of(h, -1, other[q]))) w_text(&f, other[q], "Predictor-corrector (Hsu, Konor, and Arakawa)", N_STR);
+      if (!(h && ncio_att_of(h, -1, other[q]))) w_text(&f, other[q], q == 2 ? mass : nonmass, N_STR);
   }
   w_d1(&f, "grid_globe_lonbot", c->globe_lonbot);
   w_d1(&f, "grid_globe_lontop", c->globe_lontop);
@@ -527,8 +530,6 @@ int epic_nc_read(const char *path, epic_case *c) {
   const char *need_text[][2] = {{"grid_geometry", "globe"},
                                 {"grid_uv_timestep_scheme", "3rd-order Adams-Bashforth"},
                                 {"grid_turbulence_scheme", "off"},
-                                {"var_hdry_advection_scheme", "Predictor-corrector (Hsu, Konor, and Arakawa)"},
-                                {"var_theta_advection_scheme", "Predictor-corrector (Hsu, Konor, and Arakawa)"},
                                 {"grid_eos", "ideal"}};
   for (size_t q = 0; q < sizeof(need_text) / sizeof(need_text[0]); q++) {
     r_text(&r, need_text[q][0], s, sizeof(s));
@@ -540,6 +541,21 @@ int epic_nc_read(const char *path, epic_case *c) {
       return -4;
     }
   }
+  // the two advection schemes the GPU step implements
+  const char *adv_att[2] = {"var_hdry_advection_scheme", "var_theta_advection_scheme"};
+  int adv[2] = {0, 0};
+  for (int q = 0; q < 2 && !r.bad; q++) {
+    r_text(&r, adv_att[q], s, sizeof(s));
+    if (r.bad) break;
+    if (text_is(s, EPIC_ADV_AKIMA)) adv[q] = 1;
+    else if (!text_is(s, EPIC_ADV_HSU)) {
+      snprintf(g_err, sizeof(g_err), "%.40s = \"%.60s\" is not implemented", adv_att[q], s);
+      epic_nc_header_free(f);
+      return -4;
+    }
+  }
+  c->adv_mass = adv[0];
+  c->adv_nonmass = adv[1];
   r_text(&r, "planet_name", c->planet_name, sizeof(c->planet_name));
   r_text(&r, "planet_type", c->planet_type, sizeof(c->planet_type));
   c->planet_index = r_i(&r, "planet_index");

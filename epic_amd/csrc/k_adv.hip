@@ -77,6 +77,7 @@ __device__ __forceinline__ double old_value(float, const double *p) { return *p;
 template <class T>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_hsu_vert(const MemberDesc *D, int mb, int nm, int C) {
   ADV_MEMBER;
+  if (d.adv_akima[which]) return;  // the field takes the Akima scheme (uniform per block)
   extern __shared__ double lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
   const int c = threadIdx.x % C, kk = threadIdx.x / C, KS = blockDim.x / C;
@@ -210,7 +211,7 @@ template <class T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_hsu_mer(const MemberDesc *D, int mb, int nm, int CW, int MER_TJ) {
   ADV_MEMBER;
   const int K = 1 + blockIdx.y;
-  if (which == AV_THETA && K <= d.ks - 2) return;  // uniform per block
+  if (d.adv_akima[which] || (which == AV_THETA && K <= d.ks - 2)) return;  // uniform per block
   extern __shared__ double sh_raw[];
   T *sh = reinterpret_cast<T *>(sh_raw);
   const int nj = d.nj, nchunk = d.ni / CW;
@@ -307,7 +308,7 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   ADV_MEMBER;
   const RowMap m = row_map(d.ni);
   const int K = 1 + blockIdx.y, Jr = blockIdx.x * m.rpb + m.g;
-  if (which == AV_THETA && K <= d.ks - 2) return;  // uniform per block
+  if (d.adv_akima[which] || (which == AV_THETA && K <= d.ks - 2)) return;  // uniform per block
   const bool live = Jr <= d.nj;
   const int J = live ? Jr : 0;
   extern __shared__ double sh_raw[];
@@ -383,8 +384,207 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   }
 }
 
+// ------------------------------------------------------------- Akima
+// akima_advection (src/shared/epic_flux.c:1502-1924), the monotonized
+// fourth-order centred scheme, in the same three passes and tiles as the
+// predictor-corrector above: the difference dA of neighbouring values, the
+// harmonic-mean slope dela = 2 dA- dA+ / (dA- + dA+) where both have one
+// sign (0 otherwise), and the flux: the velocity times the edge value
+// .5 (a- + a+) - (dela+ - dela-) / 6, each by the reference's expression.
+// A block whose field takes the other scheme returns at once (uniform per
+// block); the two schemes share the filtered copy sB and its filter launch.
+// The reference also runs the column pass on the pad rows J = -1 and nj+1,
+// where W3 is zero and the pass changes nothing.
+template <class T>
+__device__ __forceinline__ T ak_dela(T d0, T d1) {
+  const T tmp = d0 * d1;
+  return (tmp > T(0.)) ? T(2.) * tmp / (d0 + d1) : T(0.);
+}
+
+// column pass: columns J = 0..nj, I = 0..ni+1 in LDS tiles; the stages
+// (differences, slopes, fluxes, update) run on all lanes over (column,
+// level) pairs, separated by barriers (no recurrence in K)
+template <class T>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_ak_vert(const MemberDesc *D, int mb, int nm, int C) {
+  ADV_MEMBER;
+  if (!d.adv_akima[which]) return;  // uniform per block
+  extern __shared__ double lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
+  const int c = threadIdx.x % C, kk = threadIdx.x / C, KS = blockDim.x / C;
+  const int nI = d.ni + 2, col = blockIdx.x * C + c;
+  const bool live = col < (d.nj + 1) * nI;
+  const int J = live ? col / nI : 0, I = live ? col % nI : 0;
+  const int nk = d.nk, L = nk + 2;
+  const int layer = (which == AV_HDRY);
+  const int kf = (which == AV_THETA) ? d.ks - 1 : 1;
+  const T DT = T(d.dt);
+  const double *dsi = d.dsgth_inv;
+  double *b = adv_field(d, which);
+  // the flux reuses the difference column once the slopes are formed
+  T *wa = lds + c * L, *wv = lds + (C + c) * L, *da = lds + (2 * C + c) * L, *dl = lds + (3 * C + c) * L, *ff = da;
+  if (live)
+    for (int K = kk; K <= nk + 1; K += KS) {
+      double x = 0., w = 0.;
+      if (layer) {
+        if (K >= kf && K <= nk) x = b[ix3(d, K, J, I)];
+        if (K >= kf - 1 && K <= nk) w = d.W3[ix3(d, K, J, I)];
+      } else {
+        if (K >= kf - 1 && K <= nk) x = b[ix3(d, K, J, I)];
+        if (K >= kf && K <= nk) w = (K == 1 || K == nk) ? 0. : .5 * (d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K - 1, J, I)]);
+      }
+      wa[K] = T(x);
+      wv[K] = T(w);
+    }
+  __syncthreads();
+  // differences: in a layer, da[K] = a[K] - a[K+1] on kf..nk-1 (0 at kf-1
+  // and nk); on interfaces, da[K] = a[K-1] - a[K] on kf..nk
+  if (live)
+    for (int K = kf - 1 + kk; K <= nk; K += KS) {
+      if (layer) da[K] = (K == kf - 1 || K == nk) ? T(0.) : wa[K] - wa[K + 1];
+      else if (K >= kf) da[K] = wa[K - 1] - wa[K];
+    }
+  __syncthreads();
+  if (live)
+    for (int K = kf - 1 + kk; K <= nk; K += KS) {
+      if (layer) {
+        if (K >= kf) dl[K] = ak_dela(da[K], da[K - 1]);
+      } else {
+        dl[K] = (K == kf - 1 || K == nk) ? T(0.) : ak_dela(da[K], da[K + 1]);
+      }
+    }
+  __syncthreads();
+  // fluxes (mass_flux_top and mass_flux_bot are zero)
+  if (live)
+    for (int K = kf - 1 + kk; K <= nk + 1; K += KS) {
+      if (layer) {
+        if (K <= nk)
+          ff[K] = (K == kf - 1 || K == nk) ? T(0.) : wv[K] * (T(.5) * (wa[K] + wa[K + 1]) - (dl[K] - dl[K + 1]) / T(6.));
+      } else if (K >= kf) {
+        ff[K] = (K == nk + 1) ? T(0.) : wv[K] * (T(.5) * (wa[K - 1] + wa[K]) - (dl[K - 1] - dl[K]) / T(6.));
+      }
+    }
+  __syncthreads();
+  if (live)
+    for (int K = kf + kk; K <= nk; K += KS) {
+      const size_t q = ix3(d, K, J, I);
+      if (layer) b[q] = old_value(wa[K], b + q) + (double)((ff[K] - ff[K - 1]) * T(dsi[2 * K]) * DT);
+      else b[q] = old_value(wa[K], b + q) + (double)((ff[K + 1] - ff[K]) * T(dsi[2 * K + 1]) * DT);
+    }
+}
+
+// meridional pass: a tile of TJ rows x CW columns of one plane, with the
+// field on rows J0-2..J0+TJ+1, the slopes on J0-1..J0+TJ, the velocity and
+// the fluxes on v-rows J0..J0+TJ; the update goes to sA and its copy sB
+// (with sB's zonal pads), as the predictor-corrector's does
+template <class T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_ak_mer(const MemberDesc *D, int mb, int nm, int CW, int MER_TJ) {
+  ADV_MEMBER;
+  const int K = 1 + blockIdx.y;
+  if (!d.adv_akima[which] || (which == AV_THETA && K <= d.ks - 2)) return;  // uniform per block
+  extern __shared__ double sh_raw[];
+  T *sh = reinterpret_cast<T *>(sh_raw);
+  const int nj = d.nj, nchunk = d.ni / CW;
+  const int tile = blockIdx.x / nchunk, chunk = blockIdx.x - tile * nchunk;
+  const int J0 = tile * MER_TJ, c = threadIdx.x % CW, r0 = threadIdx.x / CW, RS = blockDim.x / CW;
+  const int I = 1 + chunk * CW + c;
+  const int RA = MER_TJ + 4, RV = MER_TJ + 1, RD = MER_TJ + 2;
+  T *ta = sh, *tv = ta + RA * CW, *tdl = tv + RV * CW, *tff = tdl + RD * CW;
+  const size_t o = (size_t)K * d.N2;
+  const double *A = adv_field(d, which) + o;
+  const T DT = T(d.dt);
+  auto a = [&](int J) { return ta[(J - J0 + 2) * CW + c]; };
+  auto dl = [&](int J) { return tdl[(J - J0 + 1) * CW + c]; };
+  for (int r = r0; r < RA; r += RS) {
+    const int J = J0 - 2 + r;
+    ta[r * CW + c] = (J >= -1 && J <= nj + 1) ? T(A[ix2(d, J, I)]) : T(0.);
+  }
+  for (int r = r0; r < RV; r += RS) {
+    const int J = J0 + r;
+    double v2 = 0.;
+    if (J >= 1 && J <= nj)
+      v2 = (which == AV_HDRY) ? .5 * (d.V[ix3(d, K, J, I)] + d.V[ix3(d, K - 1, J, I)]) : d.V[ix3(d, K, J, I)];
+    tv[r * CW + c] = T(v2);
+  }
+  __syncthreads();
+  // slopes on rows 0..nj (zero elsewhere; the differences are zero outside
+  // the rows 1..nj)
+  auto dA = [&](int J) { return (J >= 1 && J <= nj) ? a(J) - a(J - 1) : T(0.); };
+  for (int r = r0; r < RD; r += RS) {
+    const int J = J0 - 1 + r;
+    tdl[r * CW + c] = (J >= 0 && J <= nj) ? ak_dela(dA(J), dA(J + 1)) : T(0.);
+  }
+  __syncthreads();
+  for (int r = r0; r < RV; r += RS) {
+    const int J = J0 + r;
+    T f = T(0.);
+    if (J >= 1 && J <= nj) {
+      const double mi = (fabs(d.lat[2 * J]) == 90.) ? 0. : 1. / d.m[2 * J];
+      f = tv[r * CW + c] * (T(.5) * (a(J) + a(J - 1)) - (dl(J) - dl(J - 1)) / T(6.)) * T(mi);
+    }
+    tff[r * CW + c] = f;
+  }
+  __syncthreads();
+  double *sA = d.sA + so + o;
+  T *sB = reinterpret_cast<T *>(d.sB) + so + o;
+  for (int r = r0; r < MER_TJ; r += RS) {
+    const int J = J0 + r;
+    if (J > nj) break;
+    const double v =
+        old_value(a(J), A + ix2(d, J, I)) + (double)((tff[r * CW + c] - tff[(r + 1) * CW + c]) * T(d.mn[2 * J + 1]) * DT);
+    sA[ix2(d, J, I)] = v;
+    sB[ix2(d, J, I)] = T(v);
+    if (I == d.ni) sB[ix2(d, J, 0)] = T(v);
+    if (I == 1) sB[ix2(d, J, d.ni + 1)] = T(v);
+  }
+}
+
+// zonal pass: rows of the filtered copy sB with its pads in LDS; slopes,
+// fluxes and the update sA + d(flux) on all lanes of the row, periodic in I
+template <class T>
+__global__ void __launch_bounds__(256) k_ak_zonal(const MemberDesc *D, int mb, int nm) {
+  ADV_MEMBER;
+  const RowMap m = row_map(d.ni);
+  const int K = 1 + blockIdx.y, Jr = blockIdx.x * m.rpb + m.g;
+  if (!d.adv_akima[which] || (which == AV_THETA && K <= d.ks - 2)) return;  // uniform per block
+  const bool live = Jr <= d.nj;
+  const int J = live ? Jr : 0;
+  extern __shared__ double sh_raw[];
+  T *sh = reinterpret_cast<T *>(sh_raw);
+  const int ni = d.ni, W = ni + 2;
+  T *A = sh + (size_t)m.g * 3 * W, *DL = A + W, *FF = A + 2 * W;
+  const T *af = reinterpret_cast<const T *>(d.sB) + so + (size_t)K * d.N2;
+  const double ninv = 1. / d.n[2 * J + 1];
+  const T mn = T(d.mn[2 * J + 1]), DT = T(d.dt);
+  auto wr = [ni](int I) { return I < 1 ? I + ni : (I > ni ? I - ni : I); };
+  if (live)
+    for (int I = m.lane; I < W; I += m.stride) A[I] = af[ix2(d, J, I)];
+  __syncthreads();
+  // differences A(I) - A(I-1) on I = 1..ni, the pad of I = ni+1 that of I = 1
+  auto dA = [&](int I) { return A[I] - A[I - 1]; };
+  if (live)
+    for (int I = 1 + m.lane; I <= ni; I += m.stride) DL[I] = ak_dela(dA(I), dA(wr(I + 1)));
+  __syncthreads();
+  if (live)
+    for (int I = 1 + m.lane; I <= ni; I += m.stride) {
+      const double u2 =
+          (which == AV_HDRY) ? .5 * (d.U[ix3(d, K, J, I)] + d.U[ix3(d, K - 1, J, I)]) : d.U[ix3(d, K, J, I)];
+      FF[I] = T(u2) * (T(.5) * (A[I] + A[I - 1]) - (DL[I] - DL[wr(I - 1)]) / T(6.)) * T(ninv);
+    }
+  __syncthreads();
+  if (!live) return;
+  double *out = adv_field(d, which);
+  for (int I = 1 + m.lane; I <= ni; I += m.stride) {
+    const size_t q = ix3(d, K, J, I);
+    const double y = d.sA[so + q] + (double)((FF[I] - FF[wr(I + 1)]) * mn * DT);
+    out[q] = y;
+    if (I == ni) out[ix3(d, K, J, 0)] = y;
+    if (I == 1) out[ix3(d, K, J, ni + 1)] = y;
+  }
+}
+
 // one advection pass over all three fields of every member; the mixed mode
-// forms the fluxes in FP32 (sB, the filtered copy, FP32 too)
+// forms the fluxes in FP32 (sB, the filtered copy, FP32 too).  Each scheme's
+// launches run where some field of the bucket takes it.
 template <class T>
 static void hsu3_t(hipStream_t s, const Bucket &b) {
   const int ncol = (b.nj + 1) * (b.ni + 2), nz = ADV_FIELDS * b.nm;
@@ -393,20 +593,29 @@ static void hsu3_t(hipStream_t s, const Bucket &b) {
   while (C > 16 && (size_t)4 * C * (b.nk + 2) * sizeof(double) > 40 * 1024) C /= 2;
   const int v = b.knob[KN_COLS_HSUV];
   if (v) C = knob_cols(v);
-  LAUNCH(k_hsu_vert<T>, dim3((ncol + C - 1) / C, 1, nz), dim3(knob_threads(v)), sizeof(T) * 4 * C * (b.nk + 2), s,
-         b.D, b.mb, b.nm, C);
+  const bool hsu = b.adv_schemes & 1, akima = b.adv_schemes & 2;
+  const dim3 gv((ncol + C - 1) / C, 1, nz);
+  if (hsu)
+    LAUNCH(k_hsu_vert<T>, gv, dim3(knob_threads(v)), sizeof(T) * 4 * C * (b.nk + 2), s, b.D, b.mb, b.nm, C);
+  if (akima) LAUNCH(k_ak_vert<T>, gv, dim3(knob_threads(v)), sizeof(T) * 4 * C * (b.nk + 2), s, b.D, b.mb, b.nm, C);
   const int TJ = b.knob[KN_ROWS_MER] ? b.knob[KN_ROWS_MER] : 4;
   const int CW = b.ni < 64 ? b.ni : 64, tiles = (b.nj + 1 + TJ - 1) / TJ;
-  LAUNCH(k_hsu_mer<T>, dim3(tiles * (b.ni / CW), b.nk, nz), dim3(256),
-         sizeof(T) * CW * ((TJ + 4) + 2 * (TJ + 3) + 2 * (TJ + 2) + (TJ + 1)), s, b.D, b.mb, b.nm, CW, TJ);
+  const dim3 gm(tiles * (b.ni / CW), b.nk, nz);
+  if (hsu)
+    LAUNCH(k_hsu_mer<T>, gm, dim3(256), sizeof(T) * CW * ((TJ + 4) + 2 * (TJ + 3) + 2 * (TJ + 2) + (TJ + 1)), s, b.D,
+           b.mb, b.nm, CW, TJ);
+  if (akima)
+    LAUNCH(k_ak_mer<T>, gm, dim3(256), sizeof(T) * CW * ((TJ + 4) + 2 * (TJ + 1) + (TJ + 2)), s, b.D, b.mb, b.nm, CW,
+           TJ);
   // the zonal pass reads a zonally filtered copy (the reference filters it
   // per level before the zonal sweep); a separate launch packs the rows of
   // the filter band, which is faster here than filtering inside the pass
   if (sizeof(T) == sizeof(double)) L_filter(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
   else L_filter_f32(s, b, F_SB, 1, b.nk, 0, 0, ADV_FIELDS);
   const int rpb = rows_per_block(b.ni);
-  LAUNCH(k_hsu_zonal<T>, dim3((b.nj + 1 + rpb - 1) / rpb, b.nk, nz), dim3(256), sizeof(T) * rpb * 3 * (b.ni + 2), s,
-         b.D, b.mb, b.nm);
+  const dim3 gz((b.nj + 1 + rpb - 1) / rpb, b.nk, nz);
+  if (hsu) LAUNCH(k_hsu_zonal<T>, gz, dim3(256), sizeof(T) * rpb * 3 * (b.ni + 2), s, b.D, b.mb, b.nm);
+  if (akima) LAUNCH(k_ak_zonal<T>, gz, dim3(256), sizeof(T) * rpb * 3 * (b.ni + 2), s, b.D, b.mb, b.nm);
 }
 static void hsu3(hipStream_t s, const Bucket &b) {
   if (b.mixed) hsu3_t<float>(s, b);

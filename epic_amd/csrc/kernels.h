@@ -54,9 +54,19 @@ struct Bucket {
   int any_gas, any_hs, any_ray;
   // hyperviscosity passes of the member that needs the most (hv_passes)
   int hv_passes;
+  // advection schemes some member's field takes: bit 0 Hsu-Konor-Arakawa,
+  // bit 1 Akima (the launches of a scheme no field takes are left out)
+  int adv_schemes;
 };
 
 
+
+// the Bucket::adv_schemes bits of one member
+inline int adv_schemes(const MemberDesc &d) {
+  int s = 0;
+  for (int f = 0; f < 3; f++) s |= d.adv_akima[f] ? 2 : 1;
+  return s;
+}
 
 // ---- diagnostics (k_diag.hip)
 void L_fill(hipStream_t s, const Bucket &b, int fld, double v);

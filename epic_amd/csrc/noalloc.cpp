@@ -167,6 +167,7 @@ extern "C" int epic_step_noalloc(const epic_case *c, double *const *fields, int 
   b.any_hs = d.is_hs;
   b.any_ray = (c->newt_cool_on && c->prandtl > 0.) || (c->tau_drag > 0. && c->tau_drag < 1.e+20);
   b.hv_passes = hv_passes(d);
+  b.adv_schemes = adv_schemes(d);
   const bool hv = b.hv_passes > 0, divd = T.nudiv != 0.;
   const bool adjust = (c->newt_cool_adjust || c->k_sponge >= 1) && c->newt_cool_on;
   if (action == EPIC_SYNC_DIAGS_ONLY) {

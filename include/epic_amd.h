@@ -239,6 +239,14 @@ typedef struct {
                              pressure gradient (uv_pgrad_traditional,
                              src/shared/epic_timestep.c:806-884) in place of
                              the finite-volume one of its default build    */
+  int    adv_mass;        /* advection scheme of hdry, and adv_nonmass of
+                             theta (and of the unity field its correction
+                             uses): the reference's list
+                             (src/shared/epic_initial.c:146-149), 0 its
+                             default predictor-corrector (Hsu, Konor and
+                             Arakawa), 1 the monotonized fourth-order
+                             centred scheme (Akima)                         */
+  int    adv_nonmass;
 } epic_case_spec;
 
 typedef struct {
@@ -290,6 +298,8 @@ typedef struct {
   void   *nc_header;
   /* 1: the traditional pressure gradient (epic_case_spec.pgrad_traditional) */
   int    pgrad_traditional;
+  /* advection schemes (epic_case_spec.adv_mass, adv_nonmass) */
+  int    adv_mass, adv_nonmass;
 } epic_case;
 
 /* Build a case from a spec: metric arrays (set_lonlat/set_fmn/set_gravity/

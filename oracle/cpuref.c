@@ -1722,22 +1722,147 @@ static void hsu(cpuref *R, int which, double *b) {
   bc3(R, b);
 }
 
-/* advection, src/shared/epic_flux.c:346-637 (dry: unity, theta, hdry) */
+/* akima_advection, src/shared/epic_flux.c:1502-1924: the monotonized
+ * fourth-order centred scheme, directions split as in hsu: vertical column
+ * pass, then meridional and zonal passes per layer or interface (the zonal
+ * one on a filtered copy).  Differences dA of neighbouring values, their
+ * harmonic-mean slopes dela = 2 dA- dA+ / (dA- + dA+) where the two agree
+ * in sign (else 0), the edge value .5 (a- + a+) - (dela+ - dela-)/6 times the
+ * velocity as the flux.  The vertical pass runs over the rows J = 0..nj
+ * (the reference also runs it on the pad rows J = -1 and nj+1, where W3 is
+ * zero and it changes nothing). */
+static double akima_dela(double d0, double d1) {
+  double tmp = d0 * d1;
+  return (tmp > 0.) ? 2. * tmp / (d0 + d1) : 0.;
+}
+static void akima(cpuref *R, int which, double *b) {
+  epic_case *c = &R->c;
+  int ni = R->ni, nj = R->nj, nk = R->nk;
+  const double DT = R->dt;
+  const int layer = (which == AV_HDRY);
+  const int kf = (which == AV_THETA) ? R->k_sigma - 1 : 1;
+  double *wa = R->w_a, *w = R->w_wm, *da = R->w_d1, *dl = R->w_d2, *ff = R->w_ff;
+  const double *dsi = c->dsgth_inv;
+  /* ---- vertical ---- */
+  for (int J = 0; J <= nj; J++)
+    for (int I = 0; I <= ni + 1; I++) {
+      if (layer) {
+        for (int K = kf; K <= nk; K++) wa[K] = A3(b, K, J, I);
+        for (int K = kf - 1; K <= nk; K++) w[K] = A3(R->W3, K, J, I);
+        da[kf - 1] = 0.;
+        for (int K = kf; K < nk; K++) da[K] = wa[K] - wa[K + 1];
+        da[nk] = 0.;
+        for (int K = kf; K <= nk; K++) dl[K] = akima_dela(da[K], da[K - 1]);
+        ff[kf - 1] = 0.; /* mass_flux_top */
+        for (int K = kf; K < nk; K++) ff[K] = w[K] * (.5 * (wa[K] + wa[K + 1]) - (dl[K] - dl[K + 1]) / 6.);
+        ff[nk] = 0.; /* mass_flux_bot */
+        for (int K = kf; K <= nk; K++) A3(b, K, J, I) += (ff[K] - ff[K - 1]) * dsi[2 * K] * DT;
+      } else {
+        for (int K = kf - 1; K <= nk; K++) wa[K] = A3(b, K, J, I);
+        for (int K = kf; K <= nk; K++)
+          w[K] = (K == 1 || K == nk) ? 0. : .5 * (A3(R->W3, K, J, I) + A3(R->W3, K - 1, J, I));
+        for (int K = kf; K <= nk; K++) da[K] = wa[K - 1] - wa[K];
+        dl[kf - 1] = 0.;
+        for (int K = kf; K < nk; K++) dl[K] = akima_dela(da[K], da[K + 1]);
+        dl[nk] = 0.;
+        for (int K = kf; K <= nk; K++) ff[K] = w[K] * (.5 * (wa[K - 1] + wa[K]) - (dl[K - 1] - dl[K]) / 6.);
+        ff[nk + 1] = 0.;
+        for (int K = kf; K <= nk; K++) A3(b, K, J, I) += (ff[K + 1] - ff[K]) * dsi[2 * K + 1] * DT;
+      }
+    }
+  /* ---- meridional, per layer or interface K ---- */
+  double *v2 = R->b2[0], *dA = R->b2[1], *dL = R->b2[2], *fl = R->b2[3];
+  for (int K = 1; K <= nk; K++) {
+    if (which == AV_THETA && K <= R->k_sigma - 2) continue;
+    double *a = b + (size_t)K * R->N2;
+    memset(v2, 0, sizeof(double) * R->N2);
+    memset(dA, 0, sizeof(double) * R->N2);
+    memset(dL, 0, sizeof(double) * R->N2);
+    memset(fl, 0, sizeof(double) * R->N2);
+    if (layer) {
+      for (int J = 1; J <= nj; J++)
+        for (int I = 1; I <= ni; I++) A2(v2, J, I) = .5 * (A3(R->V, K, J, I) + A3(R->V, K - 1, J, I));
+      bc2(R, v2);
+    } else {
+      for (int J = -1; J <= nj + 1; J++)
+        for (int I = 0; I <= ni + 1; I++) A2(v2, J, I) = A3(R->V, K, J, I);
+    }
+    for (int J = 1; J <= nj; J++)
+      for (int I = 1; I <= ni; I++) A2(dA, J, I) = A2(a, J, I) - A2(a, J - 1, I);
+    bc2(R, dA);
+    for (int J = 0; J <= nj; J++)
+      for (int I = 1; I <= ni; I++) A2(dL, J, I) = akima_dela(A2(dA, J, I), A2(dA, J + 1, I));
+    bc2(R, dL);
+    for (int J = 1; J <= nj; J++) {
+      double mi = (fabs(c->lat[2 * J]) == 90.) ? 0. : 1. / c->m[2 * J];
+      for (int I = 1; I <= ni; I++)
+        A2(fl, J, I) = A2(v2, J, I) * (.5 * (A2(a, J, I) + A2(a, J - 1, I)) - (A2(dL, J, I) - A2(dL, J - 1, I)) / 6.) * mi;
+    }
+    bc2(R, fl);
+    for (int J = 0; J <= nj; J++) {
+      double mn = c->mn[2 * J + 1];
+      for (int I = 1; I <= ni; I++) A2(a, J, I) += (A2(fl, J, I) - A2(fl, J + 1, I)) * mn * DT;
+    }
+    bc2(R, a);
+  }
+  /* ---- zonal, per K, on a filtered copy ---- */
+  double *u2 = R->b2[0], *af = R->b2[6];
+  for (int K = 1; K <= nk; K++) {
+    if (which == AV_THETA && K <= R->k_sigma - 2) continue;
+    memcpy(af, b + (size_t)K * R->N2, sizeof(double) * R->N2);
+    filter2(R, af, FK_H);
+    double *a = af;
+    memset(u2, 0, sizeof(double) * R->N2);
+    memset(dA, 0, sizeof(double) * R->N2);
+    memset(dL, 0, sizeof(double) * R->N2);
+    memset(fl, 0, sizeof(double) * R->N2);
+    if (layer) {
+      for (int J = 0; J <= nj; J++)
+        for (int I = 1; I <= ni; I++) A2(u2, J, I) = .5 * (A3(R->U, K, J, I) + A3(R->U, K - 1, J, I));
+      bc2(R, u2);
+    } else {
+      for (int J = -1; J <= nj + 1; J++)
+        for (int I = 0; I <= ni + 1; I++) A2(u2, J, I) = A3(R->U, K, J, I);
+    }
+    for (int J = 0; J <= nj; J++)
+      for (int I = 1; I <= ni; I++) A2(dA, J, I) = A2(a, J, I) - A2(a, J, I - 1);
+    bc2(R, dA);
+    for (int J = 0; J <= nj; J++)
+      for (int I = 1; I <= ni; I++) A2(dL, J, I) = akima_dela(A2(dA, J, I), A2(dA, J, I + 1));
+    bc2(R, dL);
+    for (int J = 0; J <= nj; J++) {
+      double n_inv = 1. / c->n[2 * J + 1];
+      for (int I = 1; I <= ni; I++)
+        A2(fl, J, I) = A2(u2, J, I) * (.5 * (A2(a, J, I) + A2(a, J, I - 1)) - (A2(dL, J, I) - A2(dL, J, I - 1)) / 6.) * n_inv;
+    }
+    bc2(R, fl);
+    for (int J = 0; J <= nj; J++) {
+      double mn = c->mn[2 * J + 1];
+      for (int I = 1; I <= ni; I++) A3(b, K, J, I) += (A2(fl, J, I) - A2(fl, J, I + 1)) * mn * DT;
+    }
+  }
+  bc3(R, b);
+}
+
+/* advection, src/shared/epic_flux.c:346-637 (dry: unity, theta, hdry); the
+ * unity field and theta take theta's scheme, hdry its own */
 static void advection(cpuref *R) {
   int nk = R->nk, nj = R->nj, ni = R->ni;
+  void (*nonmass)(cpuref *, int, double *) = R->c.adv_nonmass ? akima : hsu;
   for (int K = 0; K <= nk + 1; K++)
     for (int J = 0; J <= nj; J++)
       for (int I = 0; I <= ni + 1; I++) A3(R->unity, K, J, I) = 1.;
-  hsu(R, AV_P3, R->unity);
+  nonmass(R, AV_P3, R->unity);
   for (int K = R->k_sigma - 1; K <= nk + 1; K++)
     for (int J = 0; J <= nj; J++)
       for (int I = 0; I <= ni + 1; I++) A3(R->old, K, J, I) = A3(R->THETA, K, J, I);
-  hsu(R, AV_THETA, R->THETA);
+  nonmass(R, AV_THETA, R->THETA);
   for (int K = R->k_sigma - 1; K <= nk + 1; K++)
     for (int J = 0; J <= nj; J++)
       for (int I = 0; I <= ni + 1; I++)
         A3(R->THETA, K, J, I) -= A3(R->old, K, J, I) * (A3(R->unity, K, J, I) - 1.);
-  hsu(R, AV_HDRY, R->HDRY);
+  if (R->c.adv_mass) akima(R, AV_HDRY, R->HDRY);
+  else hsu(R, AV_HDRY, R->HDRY);
   restore_mass(R);
   set_p2etc(R, 0);
 }

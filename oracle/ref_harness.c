@@ -24,7 +24,8 @@
  * newt_cool_adjust, drag_v, drag_zonal_avg, tau_drag, prandtl, nu6_nondim,
  * nu8_nondim, pgrad_traditional (then a last dump, step -1, holds the
  * reference's uv_pgrad_traditional alone on the final state in a cleared
- * tendency plane IT_ZERO).  With k_sponge > 0 the reference's own set_sponge() fills
+ * tendency plane IT_ZERO), adv_mass, adv_nonmass (1: the reference's Akima
+ * scheme for hdry, for theta).  With k_sponge > 0 the reference's own set_sponge() fills
  * t_sponge_inv, and the case builder's copy must equal it bit for bit.
  * A DUMP_STEP of 0 dumps the state after the SYNC call; s > 0 after s steps.
  * A negative DUMP_STEP -s dumps at s and prints the metric comparison only.
@@ -163,6 +164,7 @@ static int spec_option(epic_case_spec *sp, const char *arg) {
 #define OPT_D(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atof(v); return 0; }
   OPT_I(k_sponge) OPT_I(newt_cool) OPT_I(newt_cool_adjust) OPT_I(drag_v) OPT_I(drag_zonal_avg)
   OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim) OPT_I(pgrad_traditional)
+  OPT_I(adv_mass) OPT_I(adv_nonmass)
 #undef OPT_I
 #undef OPT_D
   return -1;
@@ -289,8 +291,12 @@ int main(int argc, char **argv) {
   }
   var.on_list[U_INDEX] = var.on_list[V_INDEX] = 1;
   var.on_list[HDRY_INDEX] = var.on_list[THETA_INDEX] = 1;
-  strcpy(var.hdry.advection_scheme, "Predictor-corrector (Hsu, Konor, and Arakawa)");
-  strcpy(var.theta.advection_scheme, "Predictor-corrector (Hsu, Konor, and Arakawa)");
+  {
+    /* the advection schemes by their index in initial's list */
+    const char *adv[2] = {"Predictor-corrector (Hsu, Konor, and Arakawa)", "Monotonized, 4th-order centered (Akima)"};
+    strcpy(var.hdry.advection_scheme, adv[c.adv_mass]);
+    strcpy(var.theta.advection_scheme, adv[c.adv_nonmass]);
+  }
   var.time_fp_bar = 3.e8 * 1000. * 100.;
   /* the sounding and the cooling-time table (make_arrays sizes them) */
   var.ntp = c.ntp;

@@ -41,7 +41,7 @@ static int spec_option(epic_case_spec *sp, const char *arg) {
 #define OPT_I(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atoi(v); return 0; }
 #define OPT_D(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atof(v); return 0; }
   OPT_I(k_sponge) OPT_I(newt_cool) OPT_I(newt_cool_adjust) OPT_I(drag_v) OPT_I(drag_zonal_avg)
-  OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim)
+  OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim) OPT_I(adv_mass) OPT_I(adv_nonmass)
 #undef OPT_I
 #undef OPT_D
   return -1;
@@ -120,8 +120,11 @@ int main(int argc, char **argv) {
   grid.p_ref = c.p_ref; grid.theta_ref = c.theta_ref; grid.h_min = c.h_min; grid.t_sponge_inv = c.t_sponge_inv;
 
   var.u.on = var.v.on = var.hdry.on = var.theta.on = 1;
-  strcpy(var.hdry.advection_scheme, "Predictor-corrector (Hsu, Konor, and Arakawa)");
-  strcpy(var.theta.advection_scheme, "Predictor-corrector (Hsu, Konor, and Arakawa)");
+  {
+    const char *adv[2] = {"Predictor-corrector (Hsu, Konor, and Arakawa)", "Monotonized, 4th-order centered (Akima)"};
+    strcpy(var.hdry.advection_scheme, adv[c.adv_mass]);
+    strcpy(var.theta.advection_scheme, adv[c.adv_nonmass]);
+  }
   var.u.value = arr(N3); var.v.value = arr(N3);
   var.u.tendency = arr(3 * N3); var.v.tendency = arr(3 * N3);
   var.hdry.value = arr(N3); var.theta.value = arr(N3);

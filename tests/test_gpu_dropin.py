@@ -33,6 +33,8 @@ JUP = ((1, 32, 16, 10, 60, 23, 0.5, 0.0, 0.0, 1.0),
 # Newtonian cooling toward the t_vs_p sounding with t_cool_table's times,
 # and the prandtl drag that reads those times
 SAT_NEWT = ((2, 32, 12, 8, 60, 28, 0.4, 0.0, 0.0, 0.5), {"k_sponge": 2, "newt_cool": 1, "prandtl": 0.5, "drag_v": 1})
+# the Akima advection scheme for hdry and theta (var.*.advection_scheme)
+AKIMA = ((0, 32, 20, 8, 900, 31, 0.5, 0.1, 0.3, 0.4), {"adv_mass": 1, "adv_nonmass": 1})
 NSTEPS = 12
 
 
@@ -101,7 +103,8 @@ def ri2_of(case, f):
 
 
 @pytest.mark.skipif(not os.path.exists(HOST), reason="tests/native/_bin/dropin_host not built")
-@pytest.mark.parametrize("case", [HS, JUP, SAT_NEWT], ids=["held_suarez", "jupiter_sponge_drag", "saturn_sounding"])
+@pytest.mark.parametrize("case", [HS, JUP, SAT_NEWT, AKIMA],
+                         ids=["held_suarez", "jupiter_sponge_drag", "saturn_sounding", "held_suarez_akima"])
 def test_dropin_eager_and_lazy_match_oracle(case, tmp_path):
     h_e, f_e = run(HOST, case, [0], tmp_path, "eager.bin")
     h_l, f_l = run(HOST, case, [1], tmp_path, "lazy.bin")
@@ -117,7 +120,8 @@ def test_dropin_eager_and_lazy_match_oracle(case, tmp_path):
 
 @pytest.mark.skipif(not (os.path.exists(REF_DROPIN) and os.path.exists(REF)),
                     reason="reference programs (oracle/_ref) not built")
-@pytest.mark.parametrize("case", [HS, JUP, SAT_NEWT], ids=["held_suarez", "jupiter_sponge_drag", "saturn_sounding"])
+@pytest.mark.parametrize("case", [HS, JUP, SAT_NEWT, AKIMA],
+                         ids=["held_suarez", "jupiter_sponge_drag", "saturn_sounding", "held_suarez_akima"])
 def test_reference_program_with_dropin_matches_reference(case, tmp_path):
     h_g, f_g = run(REF_DROPIN, case, [NSTEPS], tmp_path, "gpu.bin")
     h_c, f_c = run(REF, case, [NSTEPS], tmp_path, "cpu.bin")

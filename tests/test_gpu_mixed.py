@@ -130,6 +130,18 @@ def test_mixed_with_sounding_relaxation():
         assert e <= 1e-5, (f, e)
 
 
+def test_mixed_with_akima_advection():
+    """The Akima advection passes in the mixed mode's FP32 fluxes: mixed
+    within 1e-5 of FP64 after 200 steps."""
+    sp = ea.spec(ea.JUPITER_LIKE, 64, 32, 10, 60, 7, 0.5, 0.0, 0.0, 0.5, adv_mass=1, adv_nonmass=1)
+    ref, _, _ = run(sp, ea.FP64, 200)
+    mix, _, _ = run(sp, ea.MIXED, 200)
+    for f in FIELDS:
+        e = float(np.linalg.norm(interior(mix[f], sp) - interior(ref[f], sp)) / np.linalg.norm(interior(ref[f], sp)))
+        say(f"mixed vs fp64, Akima advection, 200 steps {f}: rel L2 {e:.2e}")
+        assert e <= 1e-5, (f, e)
+
+
 def test_mixed_with_traditional_pressure_gradient():
     """The traditional pressure gradient (FP64 in both modes): mixed within
     1e-5 of FP64 after 200 steps."""

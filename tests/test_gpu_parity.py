@@ -50,6 +50,16 @@ CASES = {
                                             pgrad_traditional=1),
     "jupiter_pgrad_traditional_32x16x10": ea.spec(ea.JUPITER_LIKE, 32, 16, 10, 60, 32, 0.5, 0.0, 0.0, 1.0,
                                                   pgrad_traditional=1),
+    # the Akima advection scheme: for both fields; for theta beside the
+    # predictor-corrector's hdry; for hdry beside its theta, on 64- and
+    # 128-point rows (the filtered rows of the zonal pass)
+    "hs_akima_32x20x8": ea.spec(ea.HELD_SUAREZ, 32, 20, 8, 900, 31, 0.5, 0.1, 0.3, 0.4, adv_mass=1, adv_nonmass=1),
+    "jupiter_akima_theta_32x16x10": ea.spec(ea.JUPITER_LIKE, 32, 16, 10, 60, 32, 0.5, 0.0, 0.0, 1.0,
+                                            adv_nonmass=1),
+    "saturn_akima_hdry_64x24x12": ea.spec(ea.SATURN_LIKE, 64, 24, 12, 60, 33, 0.4, 0.0, 0.0, 0.5, k_sponge=0,
+                                          adv_mass=1),
+    "venus_akima_128x48x6": ea.spec(ea.VENUS_LIKE, 128, 48, 6, 150, 34, 0.5, 0.1, 0.2, 0.5, adv_mass=1,
+                                    adv_nonmass=1),
 }
 
 

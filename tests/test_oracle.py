@@ -106,6 +106,12 @@ REF_CASES = [
     ((3, 16, 12, 6, 600, 27, 0.5, 0.0, 0.2, 0.5), {"newt_cool": 1, "newt_cool_adjust": 1}),
     ((2, 32, 12, 8, 60, 28, 0.4, 0.0, 0.0, 0.5),
      {"k_sponge": 2, "newt_cool": 1, "prandtl": 0.5, "drag_v": 1}),
+    # the Akima advection scheme: for hdry and theta; for theta (and the
+    # unity field of its correction) beside the predictor-corrector's hdry;
+    # for hdry beside the predictor-corrector's theta (64-point rows)
+    ((0, 32, 20, 8, 900, 31, 0.5, 0.1, 0.3, 0.4), {"adv_mass": 1, "adv_nonmass": 1}),
+    ((1, 32, 16, 10, 60, 32, 0.5, 0.0, 0.0, 1.0), {"k_sponge": -1, "adv_nonmass": 1}),
+    ((2, 64, 24, 12, 60, 33, 0.4, 0.0, 0.0, 0.5), {"k_sponge": 0, "adv_mass": 1}),
 ]
 
 
