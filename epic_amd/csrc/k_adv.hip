@@ -1,6 +1,8 @@
 // k_adv.hip -- flux-form predictor-corrector advection of Hsu & Arakawa as
-// modified by Konor & Arakawa (hsu_advection, src/shared/epic_flux.c:652-1488)
-// and its driver (advection, src/shared/epic_flux.c:346-637).
+// modified by Konor & Arakawa (hsu_advection, src/shared/epic_flux.c:652-1488),
+// the monotonized fourth-order centred scheme offered beside it (Akima,
+// akima_advection, src/shared/epic_flux.c:1502-1924), and their driver
+// (advection, src/shared/epic_flux.c:346-637).
 //
 // Directions are split as in the reference: vertical column pass, then a
 // meridional pass on every level, then a zonal pass on a zonally filtered

@@ -321,6 +321,7 @@ def test_nc_reads_a_file_of_another_writer(tmp_path):
     ({"grid_turbulence_scheme": b"Spalart-Allmaras DES"}, "grid_turbulence_scheme"),
     ({"grid_uv_timestep_scheme": b"Leapfrog (Asselin filtered)"}, "grid_uv_timestep_scheme"),
     ({"grid_microphysics_on": np.int32(1)}, "microphysics"),
+    ({"var_theta_advection_scheme": b"Upwind"}, "var_theta_advection_scheme"),
 ])
 def test_nc_read_refuses_what_the_step_does_not_implement(edit, msg, tmp_path):
     src, dst = tmp_path / "a.nc", tmp_path / "b.nc"
@@ -328,6 +329,28 @@ def test_nc_read_refuses_what_the_step_does_not_implement(edit, msg, tmp_path):
     _rewrite_with_scipy(src, dst, edit)
     with pytest.raises(ea.EpicError, match=msg):
         ea.nc_read(dst)
+
+
+@pytest.mark.parametrize("mass,nonmass", [(1, 0), (0, 1), (1, 1)])
+def test_nc_carries_the_advection_schemes(mass, nonmass, tmp_path):
+    """The Akima scheme of hdry and of theta goes into var_hdry_ and
+    var_theta_advection_scheme by the reference's name and comes back."""
+    src, dst = tmp_path / "a.nc", tmp_path / "b.nc"
+    sp = ea.spec(ea.HELD_SUAREZ, 16, 8, 5, 600, 3, adv_mass=mass, adv_nonmass=nonmass)
+    _write_case(src, sp)
+    akima, hsu = b"Monotonized, 4th-order centered (Akima)", b"Predictor-corrector (Hsu, Konor, and Arakawa)"
+    f = netcdf_file(src, "r", mmap=False)
+    try:
+        assert f._attributes["var_hdry_advection_scheme"].rstrip(b"\0") == (akima if mass else hsu)
+        assert f._attributes["var_theta_advection_scheme"].rstrip(b"\0") == (akima if nonmass else hsu)
+    finally:
+        f.close()
+    _rewrite_with_scipy(src, dst)
+    r = ea.nc_read(dst)
+    try:
+        assert (r.adv_mass, r.adv_nonmass) == (mass, nonmass)
+    finally:
+        ea.free_case(r)
 
 
 def test_nc_read_names_a_missing_attribute(tmp_path):
