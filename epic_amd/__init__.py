@@ -41,12 +41,13 @@ class CaseSpec(ctypes.Structure):
                 ("drag_zonal_avg", ctypes.c_int), ("tau_drag", ctypes.c_double),
                 ("prandtl", ctypes.c_double), ("nu6_nondim", ctypes.c_double),
                 ("nu8_nondim", ctypes.c_double), ("pgrad_traditional", ctypes.c_int),
-                ("adv_mass", ctypes.c_int), ("adv_nonmass", ctypes.c_int)]
+                ("adv_mass", ctypes.c_int), ("adv_nonmass", ctypes.c_int), ("pv_scheme", ctypes.c_int)]
 
     # the reference's switches as keyword arguments of spec(), in the order
     # of the command-line options of oracle/ref_harness.c
     OPTIONS = ("k_sponge", "newt_cool", "newt_cool_adjust", "drag_v", "drag_zonal_avg", "tau_drag",
-               "prandtl", "nu6_nondim", "nu8_nondim", "pgrad_traditional", "adv_mass", "adv_nonmass")
+               "prandtl", "nu6_nondim", "nu8_nondim", "pgrad_traditional", "adv_mass", "adv_nonmass",
+               "pv_scheme")
 
     def options(self):
         """{option: value} of the switches that differ from a zeroed spec."""
@@ -65,14 +66,14 @@ class CaseSpec(ctypes.Structure):
 def spec(planet=HELD_SUAREZ, ni=64, nj=32, nk=10, dt=600, seed=1, nu4=0.5, nudiv=0.0,
          noise_theta=0.1, noise_uv=0.0, k_sponge=K_SPONGE_DEFAULT, newt_cool=0, newt_cool_adjust=0,
          drag_v=0, drag_zonal_avg=0, tau_drag=0.0, prandtl=0.0, nu6=0.0, nu8=0.0, pgrad_traditional=0,
-         adv_mass=0, adv_nonmass=0):
+         adv_mass=0, adv_nonmass=0, pv_scheme=0):
     """A case: planet, grid, dt and seed, the del^4 and divergence-damping
     fractions, the initial noise, and the reference's sponge, drag and
     Newtonian-cooling switches (k_sponge defaults to the reference `initial`
     program's choice: none for Held-Suarez, 1..4 layers otherwise)."""
     return CaseSpec(planet, ni, nj, nk, dt, seed, nu4, nudiv, noise_theta, noise_uv, k_sponge, newt_cool,
                     newt_cool_adjust, drag_v, drag_zonal_avg, tau_drag, prandtl, nu6, nu8, pgrad_traditional,
-                    adv_mass, adv_nonmass)
+                    adv_mass, adv_nonmass, pv_scheme)
 
 
 _D, _I, _P = ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
@@ -99,7 +100,7 @@ class Case(ctypes.Structure):
                 [(n, _P) for n in ("pdat", "tdat", "t_cool_x", "t_cool_y")] +
                 [("dudt_hist", _P), ("dvdt_hist", _P), ("model_time", _D), ("start_time", _D),
                  ("nc_header", ctypes.c_void_p), ("pgrad_traditional", _I), ("adv_mass", _I),
-                 ("adv_nonmass", _I)])
+                 ("adv_nonmass", _I), ("pv_scheme", _I)])
 
     def array(self, name, n):
         """A numpy copy of the first n values of one of the case's arrays."""

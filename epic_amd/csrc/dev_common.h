@@ -55,6 +55,8 @@ struct MemberDesc {
   // advection scheme per advected field (unity, theta, hdry): 1 Akima, 0
   // the predictor-corrector of Hsu, Konor and Arakawa
   int adv_akima[3];
+  // PV averaging of uv_core and gz_from_u: 1 Arakawa-Lamb (1981), 0 Sadourny
+  int pv_al81;
   const double *snd_x, *snd_y, *snd_d, *cool_x, *cool_y, *cool_d;
   // host-libm tables: cos(lat), cos(lat)^4 per h-row J, cos(lon) per column I
   const double *coslat_h, *coslat4_h, *coslon_h;
@@ -101,6 +103,58 @@ __host__ __device__ __forceinline__ double hv_coef(const MemberDesc &d, int pass
 __host__ __device__ __forceinline__ double hv_nu(const MemberDesc &d, int pass) {
   return pass == 1 ? d.nu4 : pass == 2 ? d.nu6 : pass == 3 ? d.nu8 : 0.;
 }
+// PV weights of uv_core's pv*uh (v grid) and pv*vh (u grid) terms
+// (include/epic_pv_schemes.h:34-62): Sadourny (1975), three-point sums over
+// 12 with the extra terms zero; Arakawa and Lamb (1981), weighted four-point
+// sums over 24 with the extra terms PH1, PH2 (v grid) or EP1, EP2 (u grid);
+// every sum in the header's order.  P(j, i) reads PV in the type T.
+template <class T>
+struct PvW {
+  T al, be, ga, de, e1, e2, coef;
+};
+template <class T, class F>
+__device__ __forceinline__ PvW<T> pv_weights_v(bool al81, const F &P, int J, int I) {
+  PvW<T> w;
+  if (al81) {
+    w.ga = T(2.) * P(J + 1, I + 1) + P(J + 1, I) + T(2.) * P(J, I) + P(J, I + 1);
+    w.de = P(J + 1, I + 1) + T(2.) * P(J + 1, I) + P(J, I) + T(2.) * P(J, I + 1);
+    w.al = T(2.) * P(J, I + 1) + P(J, I) + T(2.) * P(J - 1, I) + P(J - 1, I + 1);
+    w.be = P(J, I + 1) + T(2.) * P(J, I) + P(J - 1, I) + T(2.) * P(J - 1, I + 1);
+    w.e1 = -P(J + 1, I + 1) + P(J + 1, I) + P(J, I) - P(J, I + 1);
+    w.e2 = -P(J, I + 1) + P(J, I) + P(J - 1, I) - P(J - 1, I + 1);
+    w.coef = T(1.) / T(24.);
+  } else {
+    w.ga = (P(J, I) + P(J, I + 1) + P(J + 1, I + 1));
+    w.de = (P(J, I) + P(J + 1, I) + P(J, I + 1));
+    w.al = (P(J - 1, I) + P(J, I) + P(J, I + 1));
+    w.be = (P(J, I) + P(J, I + 1) + P(J - 1, I + 1));
+    w.e1 = w.e2 = T(0.);
+    w.coef = T(1.) / T(12.);
+  }
+  return w;
+}
+template <class T, class F>
+__device__ __forceinline__ PvW<T> pv_weights_u(bool al81, const F &P, int J, int I) {
+  PvW<T> w;
+  if (al81) {
+    w.al = T(2.) * P(J + 1, I + 1) + P(J + 1, I) + T(2.) * P(J, I) + P(J, I + 1);
+    w.be = P(J + 1, I) + T(2.) * P(J + 1, I - 1) + P(J, I - 1) + T(2.) * P(J, I);
+    w.ga = T(2.) * P(J + 1, I) + P(J + 1, I - 1) + T(2.) * P(J, I - 1) + P(J, I);
+    w.de = P(J + 1, I + 1) + T(2.) * P(J + 1, I) + P(J, I) + T(2.) * P(J, I + 1);
+    w.e1 = P(J + 1, I + 1) + P(J + 1, I) - P(J, I) - P(J, I + 1);
+    w.e2 = P(J + 1, I) + P(J + 1, I - 1) - P(J, I - 1) - P(J, I);
+    w.coef = T(1.) / T(24.);
+  } else {
+    w.al = (P(J, I) + P(J + 1, I) + P(J + 1, I + 1));
+    w.be = (P(J + 1, I - 1) + P(J + 1, I) + P(J, I));
+    w.ga = (P(J, I - 1) + P(J, I) + P(J + 1, I));
+    w.de = (P(J, I) + P(J + 1, I) + P(J, I + 1));
+    w.e1 = w.e2 = T(0.);
+    w.coef = T(1.) / T(12.);
+  }
+  return w;
+}
+
 __host__ __device__ __forceinline__ int hv_passes(const MemberDesc &d) {
   return d.nu8 > 0. ? 4 : d.nu6 > 0. ? 3 : d.nu4 > 0. ? 2 : 0;
 }

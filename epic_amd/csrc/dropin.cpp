@@ -47,6 +47,7 @@ struct DropIn {
   int slot = -1;
   int lazy = 0;
   int pgrad_traditional = 0;
+  int pv_al81 = 0;
   bool resident = false;  // the device copy is newer than or equal to var.*
   bool dirty_host = false;  // var.* lags the device copy (lazy mode)
   int ni = 0, nj = 0, nk = 0;
@@ -185,6 +186,7 @@ void create_member(const struct epic_planet_abi *planet) {
   c.newt_cool_on = grid.newt_cool_on; c.newt_cool_adjust = grid.newt_cool_adjust;
   c.drag_v = grid.drag_v; c.drag_zonal_avg = grid.drag_zonal_avg;
   c.pgrad_traditional = g_di.pgrad_traditional;
+  c.pv_scheme = g_di.pv_al81;
   c.adv_mass = adv_scheme(var.hdry.advection_scheme);
   c.adv_nonmass = adv_scheme(var.theta.advection_scheme);
   c.dln = grid.dln; c.dlt = grid.dlt; c.dy0 = grid.dy0;
@@ -236,6 +238,7 @@ void create_member(const struct epic_planet_abi *planet) {
 extern "C" void epic_amd_set_sync_mode(int lazy) { g_di.lazy = lazy ? 1 : 0; }
 
 extern "C" void epic_amd_set_pgrad_traditional(int on) { g_di.pgrad_traditional = on ? 1 : 0; }
+extern "C" void epic_amd_set_pv_scheme(int al81) { g_di.pv_al81 = al81 ? 1 : 0; }
 
 extern "C" void epic_amd_flush(void) {
   if (!g_di.E || g_di.slot < 0 || !g_di.dirty_host) return;

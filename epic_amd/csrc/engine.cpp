@@ -520,6 +520,7 @@ void member_plan(const epic_case *c, bool mixed, const epic_tables &T, MemberDes
   d.pgrad_trad = c->pgrad_traditional ? 1 : 0;
   d.adv_akima[0] = d.adv_akima[1] = c->adv_nonmass ? 1 : 0;  // unity and theta
   d.adv_akima[2] = c->adv_mass ? 1 : 0;                      // hdry
+  d.pv_al81 = c->pv_scheme ? 1 : 0;
   d.heat_area = 0.;
   for (int J = 0; J <= c->nj; J++) {
     const double da = 1. / c->mn[2 * J + 1];

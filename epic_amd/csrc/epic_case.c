@@ -311,6 +311,7 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   memset(c, 0, sizeof(*c));
   if (s->ni < 8 || (s->ni & (s->ni - 1)) || s->nj < 4 || s->nk < 4 || s->dt <= 0) return -1;
   if (s->adv_mass < 0 || s->adv_mass > 1 || s->adv_nonmass < 0 || s->adv_nonmass > 1) return -1;
+  if (s->pv_scheme < 0 || s->pv_scheme > 1) return -1;
   int ni = s->ni, nj = s->nj, nk = s->nk;
   c->ni = ni;
   c->nj = nj;
@@ -427,6 +428,7 @@ int epic_case_build(const epic_case_spec *s, epic_case *c) {
   c->pgrad_traditional = s->pgrad_traditional ? 1 : 0;
   c->adv_mass = s->adv_mass;
   c->adv_nonmass = s->adv_nonmass;
+  c->pv_scheme = s->pv_scheme;
   c->nudiv_nondim = s->nudiv_nondim;
   /* Newtonian cooling switched on for a planet with no relaxation of its
    * own relaxes toward a sounding with a cooling-time table, as the

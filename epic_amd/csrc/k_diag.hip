@@ -390,11 +390,11 @@ void L_richardson(hipStream_t s, const Bucket &b) {
 }
 
 // kinetic energy of a zonally uniform u row (get_kin with v = 0)
-__device__ double kin_uniform(const double *mn, const double *mni, int nj, const double *u1, int J) {
+__device__ double kin_uniform(const double *mn, const double *mni, int nj, const double *u1, int J, bool al81) {
   int jj = 2 * J;
   double u2 = u1[J], u4 = u1[J], v2 = 0., v4 = 0.;
   double kc = (u2 * u2 + u4 * u4 + mn[jj + 1] * (v2 * v2 * mni[jj] + v4 * v4 * mni[jj + 2])) * .25;
-  if (J == 0 || J == nj) return kc;
+  if (al81 || J == 0 || J == nj) return kc;
   u2 = .5 * (u1[J + 1] + u1[J - 1]);
   u4 = .5 * (u1[J + 1] + u1[J - 1]);
   v2 = .5 * (0. + 0.);
@@ -467,17 +467,15 @@ __global__ void __launch_bounds__(256) k_gzu_core(const MemberDesc *D, int mb) {
   for (int J = threadIdx.x; J <= nj + 1; J += blockDim.x) {
     double v = 0.;
     if (J >= 1 && J <= nj) {
-      const double PVC = 1. / 12.;
       double udyJ = u1[J] / n[2 * J + 1], udyM = u1[J - 1] / n[2 * J - 1];
-      double ga = (pvr[J] + pvr[J] + pvr[J + 1]);
-      double de = (pvr[J] + pvr[J + 1] + pvr[J]);
-      double al = (pvr[J - 1] + pvr[J] + pvr[J]);
-      double be = (pvr[J] + pvr[J] + pvr[J - 1]);
-      v = (ga * udyJ + de * udyJ + al * udyM + be * udyM) * PVC;
+      // the uniform rows' PV in the weights of the v grid
+      auto P = [&](int j, int) { return pvr[j]; };
+      const PvW<double> w = pv_weights_v<double>(d.pv_al81, P, J, 1);
+      v = (w.ga * udyJ + w.de * udyJ + w.al * udyM + w.be * udyM) * w.coef;
       v += (p1[J] - p1[J - 1]) * 2. / (r1[J] + r1[J - 1]);
     }
     pvh[J] = v;
-    if (J <= nj) kn[J] = kin_uniform(mn, mni, nj, u1, J);
+    if (J <= nj) kn[J] = kin_uniform(mn, mni, nj, u1, J, d.pv_al81);
   }
   __syncthreads();
   if (threadIdx.x == 0) {

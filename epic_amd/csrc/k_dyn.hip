@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) k_uvh_kin(const MemberDesc *D, int mb) {
     int jj = 2 * J;
     double u2 = u[ix2(d, J, I)], u4 = u[ix2(d, J, I + 1)], v2 = v[ix2(d, J, I)], v4 = v[ix2(d, J + 1, I)];
     double kc = (u2 * u2 + u4 * u4 + mn[jj + 1] * (v2 * v2 * mni[jj] + v4 * v4 * mni[jj + 2])) * .25, k;
-    if (J == 0 || J == nj) {
+    if (d.pv_al81 || J == 0 || J == nj) {
       k = kc;
     } else {
       u2 = .5 * (u[ix2(d, J + 1, I)] + u[ix2(d, J - 1, I)]);
@@ -104,16 +104,12 @@ __global__ void __launch_bounds__(256) k_dvdt_core(const MemberDesc *D, int mb) 
   const size_t o = (size_t)K * d.N2;
   const double *uh = d.sA + o, *vh = d.sB + o, *kin = d.sC + o, *pv = d.PV3 + o;
   double *dv = cur_tend(d, d.DVDT) + o;
-#define PV(j, i) pv[ix2(d, j, i)]
-  double ga = (PV(J, I) + PV(J, I + 1) + PV(J + 1, I + 1));
-  double de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
-  double al = (PV(J - 1, I) + PV(J, I) + PV(J, I + 1));
-  double be = (PV(J, I) + PV(J, I + 1) + PV(J - 1, I + 1));
-#undef PV
+  auto PV = [&](int j, int i) { return pv[ix2(d, j, i)]; };
+  const PvW<double> w = pv_weights_v<double>(d.pv_al81, PV, J, I);
   double x = dv[ix2(d, J, I)];
-  x += ((-ga * uh[ix2(d, J, I + 1)] - de * uh[ix2(d, J, I)] - al * uh[ix2(d, J - 1, I)] -
-         be * uh[ix2(d, J - 1, I + 1)] + 0. * vh[ix2(d, J - 1, I)] - 0. * vh[ix2(d, J + 1, I)]) *
-            (1. / 12.) +
+  x += ((-w.ga * uh[ix2(d, J, I + 1)] - w.de * uh[ix2(d, J, I)] - w.al * uh[ix2(d, J - 1, I)] -
+         w.be * uh[ix2(d, J - 1, I + 1)] + w.e2 * vh[ix2(d, J - 1, I)] - w.e1 * vh[ix2(d, J + 1, I)]) *
+            w.coef +
         kin[ix2(d, J - 1, I)] - kin[ix2(d, J, I)]) *
        d.n[2 * J];
   int kk = 2 * K + 1;
@@ -144,17 +140,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
   if (!live) return;
   double *du = cur_tend(d, d.DUDT) + o;
   for (int I = 1 + m.lane; I <= ni; I += m.stride) {
-#define PV(j, i) pv[ix2(d, j, i)]
-    double al = (PV(J, I) + PV(J + 1, I) + PV(J + 1, I + 1));
-    double be = (PV(J + 1, I - 1) + PV(J + 1, I) + PV(J, I));
-    double ga = (PV(J, I - 1) + PV(J, I) + PV(J + 1, I));
-    double de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
-#undef PV
+    auto PV = [&](int j, int i) { return pv[ix2(d, j, i)]; };
+    const PvW<double> w = pv_weights_u<double>(d.pv_al81, PV, J, I);
     const double kin_w = kr[I == 1 ? ni - 1 : I - 2], kin_c = kr[I - 1];
     double x = du[ix2(d, J, I)];
-    x += ((al * vh[ix2(d, J + 1, I)] + be * vh[ix2(d, J + 1, I - 1)] + ga * vh[ix2(d, J, I - 1)] +
-           de * vh[ix2(d, J, I)] + 0. * uh[ix2(d, J, I - 1)] - 0. * uh[ix2(d, J, I + 1)]) *
-              (1. / 12.) +
+    x += ((w.al * vh[ix2(d, J + 1, I)] + w.be * vh[ix2(d, J + 1, I - 1)] + w.ga * vh[ix2(d, J, I - 1)] +
+           w.de * vh[ix2(d, J, I)] + w.e2 * uh[ix2(d, J, I - 1)] - w.e1 * uh[ix2(d, J, I + 1)]) *
+              w.coef +
           kin_w - kin_c) *
          d.m[2 * J + 1];
     int kk = 2 * K + 1;

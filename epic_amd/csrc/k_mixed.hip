@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) km_uvh_kin(const MemberDesc *D, int mb) {
     float u2 = (float)u[ix2(d, J, I)], u4 = (float)u[ix2(d, J, I + 1)];
     float v2 = (float)v[ix2(d, J, I)], v4 = (float)v[ix2(d, J + 1, I)];
     float kc = (u2 * u2 + u4 * u4 + mn1 * (v2 * v2 * mi0 + v4 * v4 * mi2)) * .25f, k;
-    if (J == 0 || J == nj) {
+    if (d.pv_al81 || J == 0 || J == nj) {
       k = kc;
     } else {
       u2 = .5f * ((float)u[ix2(d, J + 1, I)] + (float)u[ix2(d, J - 1, I)]);
@@ -100,19 +100,15 @@ __global__ void __launch_bounds__(256) km_dvdt_core(const MemberDesc *D, int mb)
   FLAT2_X(1, d.nj, 1, d.ni);
   const int K = 1 + bx_.y;
   const size_t o = (size_t)K * d.N2;
-  const float *uh = sAf(d) + o, *kin = sCf(d) + o;
+  const float *uh = sAf(d) + o, *vh = sBf(d) + o, *kin = sCf(d) + o;
   const double *pv = d.PV3 + o;
   double *dv = curV(d) + o;
-#define PV(j, i) ((float)pv[ix2(d, j, i)])
-  float ga = (PV(J, I) + PV(J, I + 1) + PV(J + 1, I + 1));
-  float de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
-  float al = (PV(J - 1, I) + PV(J, I) + PV(J, I + 1));
-  float be = (PV(J, I) + PV(J, I + 1) + PV(J - 1, I + 1));
-#undef PV
-  float x = ((-ga * uh[ix2(d, J, I + 1)] - de * uh[ix2(d, J, I)] - al * uh[ix2(d, J - 1, I)] -
-              be * uh[ix2(d, J - 1, I + 1)]) * (1.f / 12.f) +
-             (kin[ix2(d, J - 1, I)] - kin[ix2(d, J, I)])) *
-            (float)d.n[2 * J];
+  auto PV = [&](int j, int i) { return (float)pv[ix2(d, j, i)]; };
+  const PvW<float> w = pv_weights_v<float>(d.pv_al81, PV, J, I);
+  float t = -w.ga * uh[ix2(d, J, I + 1)] - w.de * uh[ix2(d, J, I)] - w.al * uh[ix2(d, J - 1, I)] -
+            w.be * uh[ix2(d, J - 1, I + 1)];
+  if (d.pv_al81) t = t + w.e2 * vh[ix2(d, J - 1, I)] - w.e1 * vh[ix2(d, J + 1, I)];
+  float x = (t * w.coef + (kin[ix2(d, J - 1, I)] - kin[ix2(d, J, I)])) * (float)d.n[2 * J];
   const int kk = 2 * K + 1;
   const float ds_inv = (float)(1. / (d.sgth[kk - 2] - d.sgth[kk + 2]));
   const float dvz = (float)(d.V[ix3(d, K - 1, J, I)] - d.V[ix3(d, K + 1, J, I)]);
@@ -125,19 +121,15 @@ __global__ void __launch_bounds__(256) km_dudt_core(const MemberDesc *D, int mb)
   FLAT2_X(0, d.nj + 1, 1, d.ni);
   const int K = 1 + bx_.y;
   const size_t o = (size_t)K * d.N2;
-  const float *vh = sBf(d) + o, *kin = sCf(d) + o;
+  const float *uh = sAf(d) + o, *vh = sBf(d) + o, *kin = sCf(d) + o;
   const double *pv = d.PV3 + o;
   double *du = curU(d) + o;
-#define PV(j, i) ((float)pv[ix2(d, j, i)])
-  float al = (PV(J, I) + PV(J + 1, I) + PV(J + 1, I + 1));
-  float be = (PV(J + 1, I - 1) + PV(J + 1, I) + PV(J, I));
-  float ga = (PV(J, I - 1) + PV(J, I) + PV(J + 1, I));
-  float de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
-#undef PV
-  float x = ((al * vh[ix2(d, J + 1, I)] + be * vh[ix2(d, J + 1, I - 1)] + ga * vh[ix2(d, J, I - 1)] +
-              de * vh[ix2(d, J, I)]) * (1.f / 12.f) +
-             (kin[ix2(d, J, I - 1)] - kin[ix2(d, J, I)])) *
-            (float)d.m[2 * J + 1];
+  auto PV = [&](int j, int i) { return (float)pv[ix2(d, j, i)]; };
+  const PvW<float> w = pv_weights_u<float>(d.pv_al81, PV, J, I);
+  float t = w.al * vh[ix2(d, J + 1, I)] + w.be * vh[ix2(d, J + 1, I - 1)] + w.ga * vh[ix2(d, J, I - 1)] +
+            w.de * vh[ix2(d, J, I)];
+  if (d.pv_al81) t = t + w.e2 * uh[ix2(d, J, I - 1)] - w.e1 * uh[ix2(d, J, I + 1)];
+  float x = (t * w.coef + (kin[ix2(d, J, I - 1)] - kin[ix2(d, J, I)])) * (float)d.m[2 * J + 1];
   const int kk = 2 * K + 1;
   const float ds_inv = (float)(1. / (d.sgth[kk - 2] - d.sgth[kk + 2]));
   const float duz = (float)(d.U[ix3(d, K - 1, J, I)] - d.U[ix3(d, K + 1, J, I)]);

@@ -187,6 +187,12 @@ void epic_amd_invalidate(void);
  * uv_pgrad_traditional, 0 the default finite-volume form.  Call before the
  * first timestep() (or invalidate after). */
 void epic_amd_set_pgrad_traditional(int on);
+/* The PV averaging of a caller built with PV_SCHEME = ARAKAWA_LAMB_1981
+ * (the other compile-time value of the reference's include/epic.h:79-86,
+ * invisible to the library): 1 selects Arakawa and Lamb (1981), 0 the
+ * default Sadourny (1975).  Call before the first timestep() (or
+ * invalidate after). */
+void epic_amd_set_pv_scheme(int al81);
 
 /* ------------------------------------------------------------------------
  * Synthetic cases: planet + grid + seeded initial state, laid out exactly
@@ -247,6 +253,11 @@ typedef struct {
                              Arakawa), 1 the monotonized fourth-order
                              centred scheme (Akima)                         */
   int    adv_nonmass;
+  int    pv_scheme;       /* how PV is averaged onto the u and v grids: 0
+                             Sadourny (1975), the reference's default
+                             PV_SCHEME; 1 Arakawa and Lamb (1981), the other
+                             value include/epic.h:79-86 offers
+                             (include/epic_pv_schemes.h:34-47)             */
 } epic_case_spec;
 
 typedef struct {
@@ -300,6 +311,8 @@ typedef struct {
   int    pgrad_traditional;
   /* advection schemes (epic_case_spec.adv_mass, adv_nonmass) */
   int    adv_mass, adv_nonmass;
+  /* PV averaging (epic_case_spec.pv_scheme) */
+  int    pv_scheme;
 } epic_case;
 
 /* Build a case from a spec: metric arrays (set_lonlat/set_fmn/set_gravity/

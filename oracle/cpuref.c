@@ -466,13 +466,14 @@ static void set_p2etc(cpuref *R, int update_theta) {
   }
 }
 
-/* get_kin, src/shared/epic_funcs_diag.c:2694-2847 (Sadourny/Suarez-Takacs) */
+/* get_kin, src/shared/epic_funcs_diag.c:2694-2847: Sadourny with the
+ * Suarez-Takacs correction off the poles; Arakawa-Lamb the centred value */
 static double kin_at(cpuref *R, const double *u, const double *v, int J, int I) {
   const double *mn = R->c.mn, *mni = R->mn_inv;
   int jj = 2 * J;
   double u2 = A2(u, J, I), u4 = A2(u, J, I + 1), v2 = A2(v, J, I), v4 = A2(v, J + 1, I);
   double kc = (u2 * u2 + u4 * u4 + mn[jj + 1] * (v2 * v2 * mni[jj] + v4 * v4 * mni[jj + 2])) * .25;
-  if (J == 0 || J == R->nj) return kc;
+  if (R->c.pv_scheme || J == 0 || J == R->nj) return kc;
   u2 = .5 * (A2(u, J + 1, I) + A2(u, J - 1, I));
   u4 = .5 * (A2(u, J + 1, I + 1) + A2(u, J - 1, I + 1));
   v2 = .5 * (A2(v, J, I + 1) + A2(v, J, I - 1));
@@ -548,6 +549,55 @@ static void divergence(cpuref *R, const double *uu, const double *vv, double *di
   bc2(R, di);
 }
 
+/* PV weights of uv_core's pv*uh (v grid) and pv*vh (u grid) terms,
+ * include/epic_pv_schemes.h:34-62: Sadourny (1975), three-point sums over
+ * 12, the extra terms zero; Arakawa and Lamb (1981), weighted four-point
+ * sums over 24 with the extra terms PH1, PH2 (v grid) and EP1, EP2 (u
+ * grid); each sum in the header's order */
+typedef struct { double al, be, ga, de, e1, e2, coef; } pvw;
+static pvw pv_v(const cpuref *R, const double *pv, int J, int I) {
+#define P(J_, I_) A2(pv, J_, I_)
+  pvw w;
+  if (R->c.pv_scheme) {
+    w.ga = 2. * P(J + 1, I + 1) + P(J + 1, I) + 2. * P(J, I) + P(J, I + 1);
+    w.de = P(J + 1, I + 1) + 2. * P(J + 1, I) + P(J, I) + 2. * P(J, I + 1);
+    w.al = 2. * P(J, I + 1) + P(J, I) + 2. * P(J - 1, I) + P(J - 1, I + 1);
+    w.be = P(J, I + 1) + 2. * P(J, I) + P(J - 1, I) + 2. * P(J - 1, I + 1);
+    w.e1 = -P(J + 1, I + 1) + P(J + 1, I) + P(J, I) - P(J, I + 1);
+    w.e2 = -P(J, I + 1) + P(J, I) + P(J - 1, I) - P(J - 1, I + 1);
+    w.coef = 1. / 24.;
+  } else {
+    w.ga = (P(J, I) + P(J, I + 1) + P(J + 1, I + 1));
+    w.de = (P(J, I) + P(J + 1, I) + P(J, I + 1));
+    w.al = (P(J - 1, I) + P(J, I) + P(J, I + 1));
+    w.be = (P(J, I) + P(J, I + 1) + P(J - 1, I + 1));
+    w.e1 = w.e2 = 0.;
+    w.coef = PV_COEF;
+  }
+  return w;
+}
+static pvw pv_u(const cpuref *R, const double *pv, int J, int I) {
+  pvw w;
+  if (R->c.pv_scheme) {
+    w.al = 2. * P(J + 1, I + 1) + P(J + 1, I) + 2. * P(J, I) + P(J, I + 1);
+    w.be = P(J + 1, I) + 2. * P(J + 1, I - 1) + P(J, I - 1) + 2. * P(J, I);
+    w.ga = 2. * P(J + 1, I) + P(J + 1, I - 1) + 2. * P(J, I - 1) + P(J, I);
+    w.de = P(J + 1, I + 1) + 2. * P(J + 1, I) + P(J, I) + 2. * P(J, I + 1);
+    w.e1 = P(J + 1, I + 1) + P(J + 1, I) - P(J, I) - P(J, I + 1);
+    w.e2 = P(J + 1, I) + P(J + 1, I - 1) - P(J, I - 1) - P(J, I);
+    w.coef = 1. / 24.;
+  } else {
+    w.al = (P(J, I) + P(J + 1, I) + P(J + 1, I + 1));
+    w.be = (P(J + 1, I - 1) + P(J + 1, I) + P(J, I));
+    w.ga = (P(J, I - 1) + P(J, I) + P(J + 1, I));
+    w.de = (P(J, I) + P(J + 1, I) + P(J, I + 1));
+    w.e1 = w.e2 = 0.;
+    w.coef = PV_COEF;
+  }
+  return w;
+#undef P
+}
+
 /* gz_from_u for gas giants, src/shared/epic_funcs_diag.c:4460-4610 */
 static void gz_from_u(cpuref *R, const double *u1d, const double *p1d, const double *r1d, double *gz1d) {
   epic_case *c = &R->c;
@@ -561,16 +611,11 @@ static void gz_from_u(cpuref *R, const double *u1d, const double *p1d, const dou
   for (int J = 0; J <= nj; J++) udy[J] = A2(u2d, J, 1) / c->n[2 * J + 1];
   memset(pvh, 0, sizeof(double) * (nj + 2));
   const int I = 1;
-#define PVK(J_, I_) A2(pv, J_, I_)
   for (int J = 1; J <= nj; J++) {
-    double ga = (PVK(J, I) + PVK(J, I + 1) + PVK(J + 1, I + 1));
-    double de = (PVK(J, I) + PVK(J + 1, I) + PVK(J, I + 1));
-    double al = (PVK(J - 1, I) + PVK(J, I) + PVK(J, I + 1));
-    double be = (PVK(J, I) + PVK(J, I + 1) + PVK(J - 1, I + 1));
-    pvh[J] = (ga * udy[J] + de * udy[J] + al * udy[J - 1] + be * udy[J - 1]) * PV_COEF;
+    const pvw w = pv_v(R, pv, J, I);
+    pvh[J] = (w.ga * udy[J] + w.de * udy[J] + w.al * udy[J - 1] + w.be * udy[J - 1]) * w.coef;
     pvh[J] += (p1d[J] - p1d[J - 1]) * 2. / (r1d[J] + r1d[J - 1]);
   }
-#undef PVK
   double kin0 = 0.;
   if (j0 >= 0 && j0 <= nj) kin0 = kin_at(R, u2d, v2d, j0, 1);
   bern[j0] = 0. + kin0;
@@ -959,17 +1004,13 @@ static void uv_core(cpuref *R, double *dudt, double *dvdt) {
     for (int J = 0; J <= nj; J++)
       for (int I = 1; I <= ni; I++) A2(kin, J, I) = kin_at(R, u, v, J, I);
     bc2(R, kin);
-#define PV(J_, I_) A2(pv, J_, I_)
     for (int J = 1; J <= nj; J++) {
       double n2j = c->n[2 * J];
       for (int I = 1; I <= ni; I++) {
-        double ga = (PV(J, I) + PV(J, I + 1) + PV(J + 1, I + 1));
-        double de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
-        double al = (PV(J - 1, I) + PV(J, I) + PV(J, I + 1));
-        double be = (PV(J, I) + PV(J, I + 1) + PV(J - 1, I + 1));
-        A3(dvdt, K, J, I) += ((-ga * A2(uh, J, I + 1) - de * A2(uh, J, I) - al * A2(uh, J - 1, I) -
-                               be * A2(uh, J - 1, I + 1) + 0. * A2(vh, J - 1, I) - 0. * A2(vh, J + 1, I)) *
-                                  PV_COEF +
+        const pvw w = pv_v(R, pv, J, I);
+        A3(dvdt, K, J, I) += ((-w.ga * A2(uh, J, I + 1) - w.de * A2(uh, J, I) - w.al * A2(uh, J - 1, I) -
+                               w.be * A2(uh, J - 1, I + 1) + w.e2 * A2(vh, J - 1, I) - w.e1 * A2(vh, J + 1, I)) *
+                                  w.coef +
                               A2(kin, J - 1, I) - A2(kin, J, I)) *
                              n2j;
       }
@@ -978,18 +1019,14 @@ static void uv_core(cpuref *R, double *dudt, double *dvdt) {
     for (int J = 0; J <= nj; J++) {
       double m2 = c->m[2 * J + 1];
       for (int I = 1; I <= ni; I++) {
-        double al = (PV(J, I) + PV(J + 1, I) + PV(J + 1, I + 1));
-        double be = (PV(J + 1, I - 1) + PV(J + 1, I) + PV(J, I));
-        double ga = (PV(J, I - 1) + PV(J, I) + PV(J + 1, I));
-        double de = (PV(J, I) + PV(J + 1, I) + PV(J, I + 1));
-        A3(dudt, K, J, I) += ((al * A2(vh, J + 1, I) + be * A2(vh, J + 1, I - 1) + ga * A2(vh, J, I - 1) +
-                               de * A2(vh, J, I) + 0. * A2(uh, J, I - 1) - 0. * A2(uh, J, I + 1)) *
-                                  PV_COEF +
+        const pvw w = pv_u(R, pv, J, I);
+        A3(dudt, K, J, I) += ((w.al * A2(vh, J + 1, I) + w.be * A2(vh, J + 1, I - 1) + w.ga * A2(vh, J, I - 1) +
+                               w.de * A2(vh, J, I) + w.e2 * A2(uh, J, I - 1) - w.e1 * A2(uh, J, I + 1)) *
+                                  w.coef +
                               A2(kin, J, I - 1) - A2(kin, J, I)) *
                              m2;
       }
     }
-#undef PV
   }
   for (int K = 1; K < R->nk; K++) {
     int kk = 2 * K + 1;

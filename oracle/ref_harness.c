@@ -25,7 +25,8 @@
  * nu8_nondim, pgrad_traditional (then a last dump, step -1, holds the
  * reference's uv_pgrad_traditional alone on the final state in a cleared
  * tendency plane IT_ZERO), adv_mass, adv_nonmass (1: the reference's Akima
- * scheme for hdry, for theta).  With k_sponge > 0 the reference's own set_sponge() fills
+ * scheme for hdry, for theta), pv_scheme (1: Arakawa-Lamb, only in the
+ * build with that PV_SCHEME, epic_ref_al81; 0 only in the others).  With k_sponge > 0 the reference's own set_sponge() fills
  * t_sponge_inv, and the case builder's copy must equal it bit for bit.
  * A DUMP_STEP of 0 dumps the state after the SYNC call; s > 0 after s steps.
  * A negative DUMP_STEP -s dumps at s and prints the metric comparison only.
@@ -164,7 +165,7 @@ static int spec_option(epic_case_spec *sp, const char *arg) {
 #define OPT_D(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atof(v); return 0; }
   OPT_I(k_sponge) OPT_I(newt_cool) OPT_I(newt_cool_adjust) OPT_I(drag_v) OPT_I(drag_zonal_avg)
   OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim) OPT_I(pgrad_traditional)
-  OPT_I(adv_mass) OPT_I(adv_nonmass)
+  OPT_I(adv_mass) OPT_I(adv_nonmass) OPT_I(pv_scheme)
 #undef OPT_I
 #undef OPT_D
   return -1;
@@ -192,6 +193,17 @@ int main(int argc, char **argv) {
     }
     argv++;
     argc--;
+  }
+  /* the PV averaging is fixed when the reference is compiled */
+#ifdef REF_PV_AL81
+  const int pv_built = 1;
+#else
+  const int pv_built = 0;
+#endif
+  if (sp.pv_scheme != pv_built) {
+    fprintf(stderr, "this build averages PV by %s; pv_scheme=%d needs the other build\n",
+            pv_built ? "Arakawa-Lamb (1981)" : "Sadourny (1975)", sp.pv_scheme);
+    return 2;
   }
   if (argc < 13) {
     fprintf(stderr, "usage: %s [--OPTION=VALUE ...] PLANET NI NJ NK DT SEED NU4 NUDIV NOISE_TH NOISE_UV NSTEPS OUT [DUMP ...]\n",

@@ -41,7 +41,7 @@ static int spec_option(epic_case_spec *sp, const char *arg) {
 #define OPT_I(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atoi(v); return 0; }
 #define OPT_D(name) if (n == strlen(#name) && !strncmp(k, #name, n)) { sp->name = atof(v); return 0; }
   OPT_I(k_sponge) OPT_I(newt_cool) OPT_I(newt_cool_adjust) OPT_I(drag_v) OPT_I(drag_zonal_avg)
-  OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim) OPT_I(adv_mass) OPT_I(adv_nonmass)
+  OPT_D(tau_drag) OPT_D(prandtl) OPT_D(nu6_nondim) OPT_D(nu8_nondim) OPT_I(adv_mass) OPT_I(adv_nonmass) OPT_I(pv_scheme)
 #undef OPT_I
 #undef OPT_D
   return -1;
@@ -178,6 +178,8 @@ int main(int argc, char **argv) {
         }
 
   if (lazy) epic_amd_set_sync_mode(1);
+  /* a caller built with the Arakawa-Lamb PV_SCHEME says so */
+  epic_amd_set_pv_scheme(sp.pv_scheme);
   grid.itime = 0;
   timestep(&pl, EPIC_SYNC_DIAGS_ONLY, NULL);
   struct timespec t0, t1, t2, t3;

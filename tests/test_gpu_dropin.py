@@ -35,6 +35,8 @@ JUP = ((1, 32, 16, 10, 60, 23, 0.5, 0.0, 0.0, 1.0),
 SAT_NEWT = ((2, 32, 12, 8, 60, 28, 0.4, 0.0, 0.0, 0.5), {"k_sponge": 2, "newt_cool": 1, "prandtl": 0.5, "drag_v": 1})
 # the Akima advection scheme for hdry and theta (var.*.advection_scheme)
 AKIMA = ((0, 32, 20, 8, 900, 31, 0.5, 0.1, 0.3, 0.4), {"adv_mass": 1, "adv_nonmass": 1})
+# a caller built with the Arakawa-Lamb PV averaging (epic_amd_set_pv_scheme)
+PV_AL81 = ((1, 32, 16, 10, 60, 35, 0.5, 0.0, 0.0, 1.0), {"k_sponge": -1, "pv_scheme": 1})
 NSTEPS = 12
 
 
@@ -103,8 +105,9 @@ def ri2_of(case, f):
 
 
 @pytest.mark.skipif(not os.path.exists(HOST), reason="tests/native/_bin/dropin_host not built")
-@pytest.mark.parametrize("case", [HS, JUP, SAT_NEWT, AKIMA],
-                         ids=["held_suarez", "jupiter_sponge_drag", "saturn_sounding", "held_suarez_akima"])
+@pytest.mark.parametrize("case", [HS, JUP, SAT_NEWT, AKIMA, PV_AL81],
+                         ids=["held_suarez", "jupiter_sponge_drag", "saturn_sounding", "held_suarez_akima",
+                              "jupiter_pv_al81"])
 def test_dropin_eager_and_lazy_match_oracle(case, tmp_path):
     h_e, f_e = run(HOST, case, [0], tmp_path, "eager.bin")
     h_l, f_l = run(HOST, case, [1], tmp_path, "lazy.bin")

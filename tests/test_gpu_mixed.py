@@ -142,6 +142,18 @@ def test_mixed_with_akima_advection():
         assert e <= 1e-5, (f, e)
 
 
+def test_mixed_with_arakawa_lamb_pv():
+    """The Arakawa-Lamb PV weights in the mixed mode's FP32 momentum terms:
+    mixed within 1e-5 of FP64 after 200 steps."""
+    sp = ea.spec(ea.JUPITER_LIKE, 64, 32, 10, 60, 8, 0.5, 0.0, 0.0, 0.5, pv_scheme=1)
+    ref, _, _ = run(sp, ea.FP64, 200)
+    mix, _, _ = run(sp, ea.MIXED, 200)
+    for f in FIELDS:
+        e = float(np.linalg.norm(interior(mix[f], sp) - interior(ref[f], sp)) / np.linalg.norm(interior(ref[f], sp)))
+        say(f"mixed vs fp64, Arakawa-Lamb PV, 200 steps {f}: rel L2 {e:.2e}")
+        assert e <= 1e-5, (f, e)
+
+
 def test_mixed_with_traditional_pressure_gradient():
     """The traditional pressure gradient (FP64 in both modes): mixed within
     1e-5 of FP64 after 200 steps."""

@@ -60,6 +60,10 @@ CASES = {
                                           adv_mass=1),
     "venus_akima_128x48x6": ea.spec(ea.VENUS_LIKE, 128, 48, 6, 150, 34, 0.5, 0.1, 0.2, 0.5, adv_mass=1,
                                     adv_nonmass=1),
+    # the Arakawa-Lamb PV averaging: Held-Suarez, and a gas giant (gz_from_u
+    # takes the weights too) with the top sponge
+    "hs_pv_al81_32x20x8": ea.spec(ea.HELD_SUAREZ, 32, 20, 8, 900, 34, 0.5, 0.1, 0.3, 0.4, pv_scheme=1),
+    "jupiter_pv_al81_64x32x10": ea.spec(ea.JUPITER_LIKE, 64, 32, 10, 60, 35, 0.5, 0.0, 0.0, 1.0, pv_scheme=1),
 }
 
 

@@ -18,6 +18,8 @@ from oracle import cpuref as cr
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
 REF_EXE = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "epic_ref")
+# the reference's step built with its Arakawa-Lamb PV averaging
+REF_AL81_EXE = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "epic_ref_al81")
 GJ = json.load(open(os.path.join(GOLD, "golden.json")))
 
 
@@ -65,7 +67,8 @@ def test_oracle_matches_golden_c1_100_steps():
 
 def _ref_run(args, dumps, tmp_path, opts=None):
     fn = str(tmp_path / "ref.bin")
-    cmd = ([REF_EXE] + [f"--{k}={v}" for k, v in (opts or {}).items()] + [str(a) for a in args] +
+    exe = REF_AL81_EXE if (opts or {}).get("pv_scheme") else REF_EXE
+    cmd = ([exe] + [f"--{k}={v}" for k, v in (opts or {}).items()] + [str(a) for a in args] +
            [str(max(dumps)), fn] + [str(d) for d in dumps])
     rc = subprocess.run(cmd, cwd=str(tmp_path)).returncode
     assert rc == 0, "reference run failed or its metric arrays (t_sponge_inv included) differ from the case builder's"
@@ -112,7 +115,29 @@ REF_CASES = [
     ((0, 32, 20, 8, 900, 31, 0.5, 0.1, 0.3, 0.4), {"adv_mass": 1, "adv_nonmass": 1}),
     ((1, 32, 16, 10, 60, 32, 0.5, 0.0, 0.0, 1.0), {"k_sponge": -1, "adv_nonmass": 1}),
     ((2, 64, 24, 12, 60, 33, 0.4, 0.0, 0.0, 0.5), {"k_sponge": 0, "adv_mass": 1}),
+    # the Arakawa-Lamb PV averaging (the reference built with that
+    # PV_SCHEME): Held-Suarez; a gas giant, where gz_from_u takes the
+    # weights and get_kin too
+    ((0, 32, 20, 8, 900, 34, 0.5, 0.1, 0.3, 0.4), {"pv_scheme": 1}),
+    ((1, 32, 16, 10, 60, 35, 0.5, 0.0, 0.0, 1.0), {"k_sponge": -1, "pv_scheme": 1}),
+    ((2, 64, 24, 12, 60, 36, 0.4, 0.0, 0.0, 0.5), {"k_sponge": 0, "pv_scheme": 1, "adv_nonmass": 1}),
 ]
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_EXE) and os.path.exists(REF_AL81_EXE)),
+                    reason="reference build (oracle/_ref) not present")
+def test_reference_pv_schemes_differ(tmp_path):
+    """The two reference builds average PV differently (so the Arakawa-Lamb
+    cases above pin that choice), and each refuses the other's scheme."""
+    args = (1, 32, 16, 10, 60, 35, 0.5, 0.0, 0.0, 1.0)
+    a = _ref_run(args, [3], tmp_path, {"k_sponge": -1})
+    b = _ref_run(args, [3], tmp_path, {"k_sponge": -1, "pv_scheme": 1})
+    for nm in ("U", "V", "GZ_SURFACE"):
+        x, y = a[0][1][nm], b[0][1][nm]
+        assert np.abs(x - y).max() > 1e-9 * np.abs(x).max(), nm
+    for exe, pv in ((REF_EXE, 1), (REF_AL81_EXE, 0)):
+        cmd = [exe, f"--pv_scheme={pv}"] + [str(v) for v in args] + ["1", str(tmp_path / "x.bin"), "1"]
+        assert subprocess.run(cmd, cwd=str(tmp_path), capture_output=True).returncode == 2
 
 
 def case_spec(args, opts):
@@ -123,7 +148,8 @@ def case_spec(args, opts):
     return sp
 
 
-@pytest.mark.skipif(not os.path.exists(REF_EXE), reason="reference build (oracle/_ref) not present")
+@pytest.mark.skipif(not (os.path.exists(REF_EXE) and os.path.exists(REF_AL81_EXE)),
+                    reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("args,opts", REF_CASES, ids=[f"{a[:4]}{o}" for a, o in REF_CASES])
 def test_oracle_bitwise_vs_reference_build(args, opts, tmp_path):
     dumps = [0, 1, 2, 3, 7]
