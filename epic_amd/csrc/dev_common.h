@@ -60,6 +60,10 @@ struct MemberDesc {
   const double *snd_x, *snd_y, *snd_d, *cool_x, *cool_y, *cool_d;
   // host-libm tables: cos(lat), cos(lat)^4 per h-row J, cos(lon) per column I
   const double *coslat_h, *coslat4_h, *coslon_h;
+  // reciprocals formed once on the host (the same IEEE quotients the
+  // kernels would form per cell): 1/m on v-rows J = 0..nj+1 (0 at the
+  // poles, as the meridional advection takes it) and 1/n on h-rows 0..nj
+  const double *minv_v, *ninv_h;
   // per layer K >= k_sigma: THETA2's interval in the sigmatheta table and its width
   const int *t2_kay;
   const double *t2_h;

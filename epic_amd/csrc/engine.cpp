@@ -428,6 +428,12 @@ void member_plan(const epic_case *c, bool mixed, const epic_tables &T, MemberDes
   }
   add(&d.t2_kay, sizeof(int) * (c->nk + 2), hold.t2_kay.data());
   add(&d.t2_h, sizeof(double) * (c->nk + 2), hold.t2_h.data());
+  hold.minv_v.assign(c->nj + 2, 0.);
+  hold.ninv_h.assign(c->nj + 2, 0.);
+  for (int J = 0; J <= c->nj + 1; J++) hold.minv_v[J] = (fabs(c->lat[2 * J]) == 90.) ? 0. : 1. / c->m[2 * J];
+  for (int J = 0; J <= c->nj; J++) hold.ninv_h[J] = 1. / c->n[2 * J + 1];
+  add(&d.minv_v, sizeof(double) * (c->nj + 2), hold.minv_v.data());
+  add(&d.ninv_h, sizeof(double) * (c->nj + 2), hold.ninv_h.data());
   add(&d.coslat_h, sizeof(double) * (c->nj + 2), hold.cl_h.data());
   add(&d.coslat4_h, sizeof(double) * (c->nj + 2), hold.cl4_h.data());
   add(&d.coslon_h, sizeof(double) * (c->ni + 2), hold.clon_h.data());

@@ -31,7 +31,7 @@ struct MemberPart {
 };
 // Host tables a member's parts point into until they are copied.
 struct MemberHold {
-  std::vector<double> cl_h, cl4_h, clon_h, t2_h;
+  std::vector<double> cl_h, cl4_h, clon_h, t2_h, minv_v, ninv_h;
   std::vector<int> t2_kay, flag_h, flag_v;
   std::vector<float> twf32, twi32, trf32, tri32;
 };

@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     double v2;
     if (which == AV_HDRY) v2 = (J < 1 || J > nj) ? 0. : .5 * (d.V[ix3(d, K, J, I)] + d.V[ix3(d, K - 1, J, I)]);
     else v2 = d.V[ix3(d, K, J, I)];
-    const double mi = (fabs(d.lat[2 * J]) == 90.) ? 0. : 1. / d.m[2 * J];
+    const double mi = d.minv_v[J];
     tvm[r * CW + c] = T(dmin(0., v2 * mi));
     tvp[r * CW + c] = T(dmax(0., v2 * mi));
   }
@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(256) k_hsu_zonal(const MemberDesc *D, int mb, 
   T *A = sh + (size_t)m.g * 3 * W;
   T *UV = A + W, *AP = A + 2 * W, *FF = UV;
   const T *af = reinterpret_cast<const T *>(d.sB) + so + (size_t)K * d.N2;
-  const double ninv = 1. / d.n[2 * J + 1];
+  const double ninv = d.ninv_h[J];
   const T mn = T(d.mn[2 * J + 1]), DT = T(d.dt);
   const T amin = (which == AV_HDRY) ? T(d.h_min[K]) : T(0.);
   auto wr = [ni](int I) { return I < 1 ? I + ni : (I > ni ? I - ni : I); };
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     const int J = J0 + r;
     T f = T(0.);
     if (J >= 1 && J <= nj) {
-      const double mi = (fabs(d.lat[2 * J]) == 90.) ? 0. : 1. / d.m[2 * J];
+      const double mi = d.minv_v[J];
       f = tv[r * CW + c] * (T(.5) * (a(J) + a(J - 1)) - (dl(J) - dl(J - 1)) / T(6.)) * T(mi);
     }
     tff[r * CW + c] = f;
@@ -555,7 +555,7 @@ __global__ void __launch_bounds__(256) k_ak_zonal(const MemberDesc *D, int mb, i
   const int ni = d.ni, W = ni + 2;
   T *A = sh + (size_t)m.g * 3 * W, *DL = A + W, *FF = A + 2 * W;
   const T *af = reinterpret_cast<const T *>(d.sB) + so + (size_t)K * d.N2;
-  const double ninv = 1. / d.n[2 * J + 1];
+  const double ninv = d.ninv_h[J];
   const T mn = T(d.mn[2 * J + 1]), DT = T(d.dt);
   auto wr = [ni](int I) { return I < 1 ? I + ni : (I > ni ? I - ni : I); };
   if (live)

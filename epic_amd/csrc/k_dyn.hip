@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) k_uvh_kin(const MemberDesc *D, int mb) {
   const int ni = d.ni, nj = d.nj;
   if (J <= nj) {
     double havg = .5 * (h3[ix2(d, J, I)] + h3[ix2(d, J, I - 1)]);
-    double x = u[ix2(d, J, I)] * havg * (1. / d.n[2 * J + 1]);
+    double x = u[ix2(d, J, I)] * havg * d.ninv_h[J];
     uh[ix2(d, J, I)] = x;
     if (I == ni) uh[ix2(d, J, 0)] = x;
     if (I == 1) uh[ix2(d, J, ni + 1)] = x;
@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(256) k_uvh_kin(const MemberDesc *D, int mb) {
   double y = 0.;
   if (J >= 1 && J <= nj) {
     double havg = .5 * (h3[ix2(d, J, I)] + h3[ix2(d, J - 1, I)]);
-    y = v[ix2(d, J, I)] * havg * (1. / d.m[2 * J]);
+    y = v[ix2(d, J, I)] * havg * d.minv_v[J];
   }
   vh[ix2(d, J, I)] = y;
   if (I == ni) vh[ix2(d, J, 0)] = y;

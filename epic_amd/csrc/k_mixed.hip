@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256) km_uvh_kin(const MemberDesc *D, int mb) {
   const int ni = d.ni, nj = d.nj;
   if (J <= nj) {
     float havg = .5f * ((float)h3[ix2(d, J, I)] + (float)h3[ix2(d, J, I - 1)]);
-    float x = (float)u[ix2(d, J, I)] * havg * (float)(1. / d.n[2 * J + 1]);
+    float x = (float)u[ix2(d, J, I)] * havg * (float)d.ninv_h[J];
     uh[ix2(d, J, I)] = x;
     if (I == ni) uh[ix2(d, J, 0)] = x;
     if (I == 1) uh[ix2(d, J, ni + 1)] = x;
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) km_uvh_kin(const MemberDesc *D, int mb) {
   float y = 0.f;
   if (J >= 1 && J <= nj) {
     float havg = .5f * ((float)h3[ix2(d, J, I)] + (float)h3[ix2(d, J - 1, I)]);
-    y = (float)v[ix2(d, J, I)] * havg * (float)(1. / d.m[2 * J]);
+    y = (float)v[ix2(d, J, I)] * havg * (float)d.minv_v[J];
   }
   vh[ix2(d, J, I)] = y;
   if (I == ni) vh[ix2(d, J, 0)] = y;
