@@ -109,12 +109,17 @@ def test_held_suarez_64x32x10_100_steps():
 
 
 def test_jupiter_256x128x20_full_size():
-    """The benchmarked workload itself, 3 steps against the oracle."""
+    """The benchmarked workload itself against the oracle, bit for bit with
+    the GPU's exp/log/pow, after 3 and after 10 steps (the Adams-Bashforth
+    start-up and the third-order steps that follow, every phase of the step
+    at the bench's size)."""
     sp = ea.spec(ea.JUPITER_LIKE, 256, 128, 20, 60, 1, 0.5, 0.0, 0.0, 0.5)
     p = Pair(sp)
     try:
         p.step(3)
         p.check("jupiter_256x128x20 step3")
+        p.step(7)
+        p.check("jupiter_256x128x20 step10")
     finally:
         p.close()
 
