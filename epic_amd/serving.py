@@ -253,7 +253,8 @@ class Server:
             if self.pool is not None:
                 self.pool.shutdown(wait=True)
                 for f in self.built.values():
-                    ea.free_case(f.result())
+                    if f.exception() is None:  # a failed build holds no case
+                        ea.free_case(f.result())
                 self.built.clear()
                 self.pending.clear()
         self.elapsed = self.clock() - t0
