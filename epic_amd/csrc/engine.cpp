@@ -1108,8 +1108,10 @@ static bool knob_fits(const Bucket &b, int k, int v) {
   const size_t W = (size_t)b.nk + 1, L = (size_t)b.nk + 2, lim = 128 * 1024;
   if (k <= KN_COLS_HSUV) {
     // lanes per column beyond twice the column's length would idle
-    const int C = knob_cols(v);
-    if ((size_t)(knob_threads(v) / C) > 2 * L) return false;
+    // the kernel's levels per column: nk+2 for the vertical advection pass,
+    // nk+1 for the others
+    const int C = knob_cols(v), lev = (k == KN_COLS_HSUV) ? (int)L : (int)W, nt = knob_threads(v, lev);
+    if (nt > 1024 || (size_t)(nt / C) > 2 * L) return false;
     v = C;
   }
   switch (k) {
@@ -1124,12 +1126,13 @@ static bool knob_fits(const Bucket &b, int k, int v) {
 }
 
 // the candidates: column tiles of 8-64 columns per block of 256, 512 or
-// 1024 threads; 4-16 rows per zonal-hyperviscosity block; 2-8 rows per
+// 1024 threads, or of 4-16 columns with one lane per level; 4-16 rows per zonal-hyperviscosity block; 2-8 rows per
 // meridional-advection tile
 static std::vector<int> knob_candidates(int k) {
-  static const int cols[12] = {8, 16, 32, 64, 2008, 2016, 2032, 2064, 4008, 4016, 4032, 4064};
+  static const int cols[16] = {8,    16,   32,   64,   2008,  2016,  2032,  2064,
+                               4008, 4016, 4032, 4064, 10004, 10008, 10012, 10016};
   static const int zhv[3] = {4, 8, 16}, mer[3] = {2, 4, 8};
-  if (k <= KN_COLS_HSUV) return std::vector<int>(cols, cols + 12);
+  if (k <= KN_COLS_HSUV) return std::vector<int>(cols, cols + 16);
   return k == KN_ROWS_ZHV ? std::vector<int>(zhv, zhv + 3) : std::vector<int>(mer, mer + 3);
 }
 
@@ -1203,8 +1206,9 @@ extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int
       b.knob[k] = bestv;
       if (report && used < (size_t)cap) {
         char shape[32];
-        if (k <= KN_COLS_HSUV && knob_threads(bestv) != 256)
-          snprintf(shape, sizeof(shape), "%d/%dthr", knob_cols(bestv), knob_threads(bestv));
+        const int lev = (k == KN_COLS_HSUV) ? b.nk + 2 : b.nk + 1;
+        if (k <= KN_COLS_HSUV && knob_threads(bestv, lev) != 256)
+          snprintf(shape, sizeof(shape), "%d/%dthr", knob_cols(bestv), knob_threads(bestv, lev));
         else
           snprintf(shape, sizeof(shape), "%d", bestv);
         if (best < 1e300)

@@ -598,8 +598,8 @@ static void hsu3_t(hipStream_t s, const Bucket &b) {
   const bool hsu = b.adv_schemes & 1, akima = b.adv_schemes & 2;
   const dim3 gv((ncol + C - 1) / C, 1, nz);
   if (hsu)
-    LAUNCH(k_hsu_vert<T>, gv, dim3(knob_threads(v)), sizeof(T) * 4 * C * (b.nk + 2), s, b.D, b.mb, b.nm, C);
-  if (akima) LAUNCH(k_ak_vert<T>, gv, dim3(knob_threads(v)), sizeof(T) * 4 * C * (b.nk + 2), s, b.D, b.mb, b.nm, C);
+    LAUNCH(k_hsu_vert<T>, gv, dim3(knob_threads(v, b.nk + 2)), sizeof(T) * 4 * C * (b.nk + 2), s, b.D, b.mb, b.nm, C);
+  if (akima) LAUNCH(k_ak_vert<T>, gv, dim3(knob_threads(v, b.nk + 2)), sizeof(T) * 4 * C * (b.nk + 2), s, b.D, b.mb, b.nm, C);
   const int TJ = b.knob[KN_ROWS_MER] ? b.knob[KN_ROWS_MER] : 4;
   const int CW = b.ni < 64 ? b.ni : 64, tiles = (b.nj + 1 + TJ - 1) / TJ;
   const dim3 gm(tiles * (b.ni / CW), b.nk, nz);

@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
     for (int K = T.kk; K <= nk; K += T.KS) thermo_cell(d, K, J, I, false, d.THETA[ix3(d, K, J, I)], a[K]);
 }
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
-  const int v = b.knob[KN_COLS_P2ETC], C = v ? knob_cols(v) : tile_cols(b, 2), nt = knob_threads(v);
+  const int v = b.knob[KN_COLS_P2ETC], C = v ? knob_cols(v) : tile_cols(b, 2), nt = knob_threads(v, b.nk + 1);
   size_t shm = sizeof(double) * (2 * C * (b.nk + 1) + 3 * C);
   if (update_theta) LAUNCH(k_set_p2etc<1>, tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
   else LAUNCH(k_set_p2etc<0>, tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
@@ -627,7 +627,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
 }
 void L_gz3(hipStream_t s, const Bucket &b) {
   const int v = b.knob[KN_COLS_GZ3], C = v ? knob_cols(v) : tile_cols(b, 3);
-  LAUNCH(k_gz3, tile_grid(b, C), dim3(knob_threads(v)), sizeof(double) * 3 * C * (b.nk + 1), s, b.D, b.mb, C);
+  LAUNCH(k_gz3, tile_grid(b, C), dim3(knob_threads(v, b.nk + 1)), sizeof(double) * 3 * C * (b.nk + 1), s, b.D,
+         b.mb, C);
 }
 
 // finish DZDT3 (diagnostic vertical velocity)
@@ -889,7 +890,7 @@ __global__ void __launch_bounds__(1024) k_calc_w(const MemberDesc *D, int mb, in
   if (bad) atomicOr(&d.dyn->err, 2);
 }
 void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
-  const int v = b.knob[KN_COLS_CALCW], C = v ? knob_cols(v) : tile_cols(b, 4), nt = knob_threads(v);
+  const int v = b.knob[KN_COLS_CALCW], C = v ? knob_cols(v) : tile_cols(b, 4), nt = knob_threads(v, b.nk + 1);
   dim3 g = tile_grid(b, C);
   const int ncb = (int)g.x;
   if (with_source) {

@@ -36,11 +36,14 @@ struct FieldSet {
 // by the same operations whatever the tile.
 enum Knob { KN_COLS_P2ETC, KN_COLS_GZ3, KN_COLS_CALCW, KN_COLS_HSUV, KN_ROWS_ZHV, KN_ROWS_MER, KN_COUNT };
 // A column-tile knob holds the columns per block C and, from 1000 up, the
-// block's threads: v = C (256 threads) or v = 1000 n + C (256 n threads,
+// block's threads: v = C (256 threads), v = 1000 n + C (256 n threads,
 // n = 2 or 4: more lanes per column, more waves in flight per CU for the
-// same LDS).
+// same LDS), or v = 10000 + C (one lane per level: C times the kernel's
+// levels per column, `levels`).
 static inline int knob_cols(int v) { return v % 1000; }
-static inline int knob_threads(int v) { return v >= 1000 ? (v / 1000) * 256 : 256; }
+static inline int knob_threads(int v, int levels) {
+  return v >= 10000 ? (v % 1000) * levels : v >= 1000 ? (v / 1000) * 256 : 256;
+}
 
 struct Bucket {
   const MemberDesc *D;  // device array of descriptors

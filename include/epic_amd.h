@@ -434,8 +434,9 @@ int  epic_engine_autotune(epic_engine *e, int steps, char *report, int cap);
 /* One launch shape set by hand for every bucket (knob: 0 columns per
  * set_p2etc tile, 1 gz3, 2 calc_w, 3 vertical advection, 4 rows per
  * zonal-hyperviscosity block, 5 rows per meridional-advection tile; a
- * column-tile value is C, or 1000 n + C for blocks of 256 n threads, n = 2
- * or 4; 0 restores the built-in choice).  Values outside the candidates the
+ * column-tile value is C, 1000 n + C for blocks of 256 n threads, n = 2
+ * or 4, or 10000 + C for one thread per level of each of the C columns; 0
+ * restores the built-in choice).  Values outside the candidates the
  * autotuner tries, or whose LDS would not fit, are refused (-1).  For tests
  * that every shape gives the same bits. */
 int  epic_engine_set_shape(epic_engine *e, int knob, int value);

@@ -120,9 +120,9 @@ def test_autotune_changes_no_state_and_no_result():
 @pytest.mark.parametrize("precision", [ea.FP64, ea.MIXED])
 def test_every_column_tile_shape_gives_the_same_bits(precision):
     """Candidate shapes of the column-tile kernels (16-64 columns per block
-    of 256, 512 or 1024 threads), set by hand on one knob at a time, step a
-    Held-Suarez and a Jupiter-like member to the same bits as the built-in
-    shapes."""
+    of 256, 512 or 1024 threads, or 4-12 columns with one thread per level),
+    set by hand on one knob at a time, step a Held-Suarez and a Jupiter-like
+    member to the same bits as the built-in shapes."""
     members = (A, ea.spec(ea.JUPITER_LIKE, 64, 32, 20, 60, 5, 0.5, 0.0, 0.0, 0.5))
 
     def run(knob=None, v=0):
@@ -138,7 +138,7 @@ def test_every_column_tile_shape_gives_the_same_bits(precision):
     ref = run()
     bad = []
     for knob in ("cols_p2etc", "cols_gz3", "cols_calcw", "cols_hsuv"):
-        for v in (16, 64, 2032, 2064, 4064):
+        for v in (16, 64, 2032, 2064, 4064, 10004, 10012):
             say(precision, knob, v)
             got = run(knob, v)
             for m, (g, r) in enumerate(zip(got, ref)):
