@@ -1115,9 +1115,9 @@ static bool knob_fits(const Bucket &b, int k, int v) {
     v = C;
   }
   switch (k) {
-    case KN_COLS_P2ETC: return sizeof(double) * (2 * v * W + 3 * v) <= lim;
+    case KN_COLS_P2ETC: return sizeof(double) * (2 * v * W + 4 * v) <= lim;
     case KN_COLS_GZ3: return sizeof(double) * 3 * v * W <= lim;
-    case KN_COLS_CALCW: return sizeof(double) * 4 * v * W <= lim;
+    case KN_COLS_CALCW: return sizeof(double) * (4 * v * W + v) <= lim;
     case KN_COLS_HSUV: return sizeof(double) * 4 * v * L <= lim;
     case KN_ROWS_MER: return sizeof(double) * (size_t)(b.ni < 64 ? b.ni : 64) * (6 * v + 15) <= lim;
     default:  // KN_ROWS_ZHV: one row array per row, and at most 16 points per lane

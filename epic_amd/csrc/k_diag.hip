@@ -145,7 +145,8 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
   const Tile T = tile_of(d, C);
   const int nk = d.nk, ks = d.ks, W = nk + 1, J = T.J, I = T.I;
   double *a = lds + T.c * W, *pw = lds + C * W + T.c * W;
-  double *aux = lds + 2 * C * W + 3 * T.c;  // theta(ks-2), H3(1), P2(1)
+  // theta(ks-2), H3(1), P2(1), log(ptop/pbot) of the column
+  double *aux = lds + 2 * C * W + 4 * T.c;
   double *P3 = d.P3, *H = d.HDRY;
   const double g = d.g[2 * J + 1];
   if (T.live)
@@ -171,10 +172,12 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
       acc = acc + a[K];
       a[K] = acc;
     }
+    // log(ptop/pbot) once per column (the sigma-region formulas read it)
+    aux[3] = crm_log(a[0] / a[nk]);
   }
   __syncthreads();
   if (T.live) {
-    const double ptop = a[0], pbot = a[nk], lr = crm_log(ptop / pbot);
+    const double pbot = a[nk], lr = aux[3];
     for (int K = ks - 1 + T.kk; K < nk; K += T.KS) {
       double sigma = (d.sgth[2 * K + 1] - d.zeta0) / (d.zeta1 - d.zeta0);
       a[K] = get_p_sigma_l(pbot, sigma, lr);
@@ -187,7 +190,7 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
     for (int K = T.kk; K <= nk; K += T.KS) pw[K] = crm_pow(a[K], d.kappa + 1.);
   __syncthreads();
   if (T.live) {
-    const double ptop = a[0], pbot = a[nk], kp1 = d.kappa + 1., lr = crm_log(ptop / pbot);
+    const double pbot = a[nk], kp1 = d.kappa + 1., lr = aux[3];
     const double *tx = d.sgth + 2 * nk + 1;  // tx[kay*(-2)] = sgth[2*(nk-kay)+1]
     for (int K = T.kk; K <= nk; K += T.KS) {
       const size_t q = ix3(d, K, J, I);
@@ -255,7 +258,7 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
 }
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
   const int v = b.knob[KN_COLS_P2ETC], C = v ? knob_cols(v) : tile_cols(b, 2), nt = knob_threads(v, b.nk + 1);
-  size_t shm = sizeof(double) * (2 * C * (b.nk + 1) + 3 * C);
+  size_t shm = sizeof(double) * (2 * C * (b.nk + 1) + 4 * C);
   if (update_theta) LAUNCH(k_set_p2etc<1>, tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
   else LAUNCH(k_set_p2etc<0>, tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
 }
@@ -806,19 +809,22 @@ __global__ void __launch_bounds__(1024) k_calc_w(const MemberDesc *D, int mb, in
   const Tile T = tile_of(d, C);
   const int nk = d.nk, ks = d.ks, W = nk + 1, J = T.J, I = T.I;
   double *dsv = lds + T.c * W, *dfv = lds + C * W + T.c * W, *wbv = lds + 2 * C * W + T.c * W;
-  double *lg = lds + 3 * C * W + T.c * W;
+  double *lg = lds + 3 * C * W + T.c * W, *lrv = lds + 4 * C * W + T.c;
   const double *P = d.P3;
   double *TH = d.THETA;
   int bad = 0;
 #define PP(k) P[ix3(d, k, J, I)]
 #define DV(k) d.DIV[ix3(d, k, J, I)]
   // lg[k] = log(P3(k)/P3(k-1)), the ln-p intervals of the Leslie-Purser weights
-  if (T.live)
+  // and log(ptop/pbot) once per column
+  if (T.live) {
     for (int K = 1 + T.kk; K <= nk; K += T.KS) lg[K] = crm_log(PP(K) / PP(K - 1));
+    if (T.kk == 0) lrv[0] = crm_log(PP(0) / PP(nk));
+  }
   __syncthreads();
   if (T.live) {
     const double ptop = P[ix3(d, 0, J, I)], pbot = P[ix3(d, nk, J, I)], psig = P[ix3(d, ks - 1, J, I)];
-    const double eps = d.eps13, lr = crm_log(ptop / pbot);
+    const double eps = d.eps13, lr = lrv[0];
     for (int K = 1 + T.kk; K <= nk; K += T.KS) {
       const size_t q = ix3(d, K, J, I);
       double th = TH[q];
@@ -898,7 +904,7 @@ void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
     const size_t n3 = (size_t)(b.ni + 2) * (b.nj + 3) * (b.nk + 2);
     g.x += (unsigned)std::max<size_t>(1, std::min<size_t>(1024, n3 / ((size_t)nt * 16)));
   }
-  LAUNCH(k_calc_w, g, dim3(nt), sizeof(double) * 4 * C * (b.nk + 1), s, b.D, b.mb, with_source, C, ncb);
+  LAUNCH(k_calc_w, g, dim3(nt), sizeof(double) * (4 * C * (b.nk + 1) + C), s, b.D, b.mb, with_source, C, ncb);
 }
 
 // ------------------------------------------------ vorticity / divergence
