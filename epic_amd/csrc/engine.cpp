@@ -94,6 +94,9 @@ struct epic_engine {
   std::map<std::array<int, 4>, std::array<int, KN_COUNT>> tuned;
   bool dirty;
   hipStream_t stream;
+  // a second stream (default priority) for the step's one independent
+  // branch, forked from and joined back into `stream` by two events
+  SideStream side;
   hipGraph_t graph;
   hipGraphExec_t gexec;
   bool have_graph;
@@ -116,6 +119,13 @@ extern "C" epic_engine *epic_engine_create(int precision_mode, int max_members, 
   E->have_graph = false;
   E->err[0] = 0;
   if (hipStreamCreate(&E->stream) != hipSuccess) {
+    delete E;
+    return nullptr;
+  }
+  if (hipStreamCreate(&E->side.s2) != hipSuccess ||
+      hipEventCreateWithFlags(&E->side.fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&E->side.join, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(E->stream);
     delete E;
     return nullptr;
   }
@@ -144,6 +154,10 @@ extern "C" void epic_engine_destroy(epic_engine *E) {
   for (auto &x : E->m)
     if (x.active) (void)hipFree(x.blob);
   (void)hipFree(E->dD);
+  (void)hipStreamSynchronize(E->side.s2);
+  (void)hipEventDestroy(E->side.fork);
+  (void)hipEventDestroy(E->side.join);
+  (void)hipStreamDestroy(E->side.s2);
   (void)hipStreamDestroy(E->stream);
   delete E;
 }
@@ -223,15 +237,34 @@ static void rec_sync(hipStream_t s, const Bucket &b, bool adjust) {
 }
 
 // timestep(STEP_PROGS_AND_SYNC_DIAGS) in the order of
-// src/shared/epic_timestep.c:104-327
-static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool adjust) {
+// src/shared/epic_timestep.c:104-327.  With `side`, the hyperviscosity of
+// hdry and theta runs on the second stream beside uv_core and uv_drag: the
+// two branches read what calc_w leaves and write disjoint arrays (the
+// tendencies and scratch copy 0; hdry, theta and scratch copies 1, 2), so
+// the results are those of the reference's order.
+static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool adjust, const SideStream *side) {
   L_calc_w(s, b, 1);  // source_sink + calc_w, and the step's tendency planes cleared
-  L_uv_core(s, b);
-  L_uv_drag(s, b);
   // hdry and theta take their hyperviscosity together (neither reads the
   // other, and restore_mass touches hdry only)
-  const FieldSet hth = {2, {F_HDRY, F_THETA}, {2, 3}};
-  if (hv) {
+  const FieldSet hth = {2, {F_HDRY, F_THETA}, {2, 3}, 1};
+  // forked only for buckets of 2^18 grid points or more: at 256x128x20 the
+  // branch saves 3% of the FP64 step, at 64x32x10 the cross-queue join
+  // costs more than the branch (+9%)
+  const bool fork = hv && side && (size_t)b.ni * (b.nj + 1) * b.nk * b.nm >= ((size_t)1 << 18);
+  // (a failure of the event calls shows in the caller's hipGetLastError,
+  // as a failed launch does)
+  if (fork) {
+    (void)hipEventRecord(side->fork, s);
+    (void)hipStreamWaitEvent(side->s2, side->fork, 0);
+    L_merid_hv(side->s2, b, hth);
+    L_zonal_hv(side->s2, b, hth);
+    (void)hipEventRecord(side->join, side->s2);
+  }
+  L_uv_core(s, b);
+  L_uv_drag(s, b);
+  if (fork) {
+    (void)hipStreamWaitEvent(s, side->join, 0);
+  } else if (hv) {
     L_merid_hv(s, b, hth);
     L_zonal_hv(s, b, hth);
   }
@@ -262,7 +295,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
 
 void member_record(hipStream_t s, const Bucket &b, bool sync_only, bool hv, bool divd, bool adjust) {
   if (sync_only) rec_sync(s, b, adjust);
-  else rec_step(s, b, hv, divd, adjust);
+  else rec_step(s, b, hv, divd, adjust, nullptr);
 }
 
 static void bucket_flags(epic_engine *E, const Bucket &b, bool &hv, bool &divd, bool &adj) {
@@ -288,11 +321,11 @@ static void record_all(epic_engine *E, bool sync_only, const std::vector<int> *o
         one.mb = b.mb + i;
         one.nm = 1;
         if (sync_only) rec_sync(E->stream, one, adj);
-        else rec_step(E->stream, one, hv, divd, adj);
+        else rec_step(E->stream, one, hv, divd, adj, &E->side);
       }
     } else {
       if (sync_only) rec_sync(E->stream, b, adj);
-      else rec_step(E->stream, b, hv, divd, adj);
+      else rec_step(E->stream, b, hv, divd, adj, &E->side);
     }
   }
 }
@@ -1183,12 +1216,12 @@ extern "C" int epic_engine_autotune(epic_engine *E, int steps, char *report, int
         if (!v || !knob_fits(b, k, v)) continue;
         b.knob[k] = v;
         restore();
-        rec_step(E->stream, b, hv, divd, adj);  // warm-up, not timed
+        rec_step(E->stream, b, hv, divd, adj, &E->side);  // warm-up, not timed
         HIPCHK(hipStreamSynchronize(E->stream));
         g_prof.clear();
         g_evnext = 0;
         g_prof_on = true;
-        for (int n = 0; n < steps; n++) rec_step(E->stream, b, hv, divd, adj);
+        for (int n = 0; n < steps; n++) rec_step(E->stream, b, hv, divd, adj, &E->side);
         g_prof_on = false;
         HIPCHK(hipStreamSynchronize(E->stream));
         double t = 0.;

@@ -35,6 +35,12 @@ struct MemberHold {
   std::vector<int> t2_kay, flag_h, flag_v;
   std::vector<float> twf32, twi32, trf32, tri32;
 };
+// A second stream (default priority) and the two events that fork a branch
+// of the step onto it and join it back.
+struct SideStream {
+  hipStream_t s2;
+  hipEvent_t fork, join;
+};
 // 0 when the engine implements the case, else -1 with the reason in err.
 int member_check(const epic_case *c, int precision, char *err, size_t cap);
 // The descriptor's scalars and the list of its arrays, in the blob's order.

@@ -646,7 +646,7 @@ __global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, in
   if (J0 > jmax) return;  // uniform per block (the grid covers the v rows)
   const int c = threadIdx.x % CW, r0 = threadIdx.x / CW, RS = blockDim.x / CW, I = 1 + chunk * CW + c;
   const double *hh = fld_ptr(d, f) + (size_t)K * d.N2;
-  T *out = reinterpret_cast<T *>(d.sB) + (size_t)fi * d.N3 + (size_t)K * d.N2;
+  T *out = reinterpret_cast<T *>(d.sB) + (size_t)(fs.copy0 + fi) * d.N3 + (size_t)K * d.N2;
   const double *cx = vg ? d.vcx : d.hcx, *cy = vg ? d.vcy : d.hcy, *cz = vg ? d.vcz : d.hcz;
   auto rows = [jmax](int J, int &j0, int &j1, int &j2) {
     if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
@@ -701,7 +701,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
   const int J = live ? Jr : 0, ni = d.ni, vg = (kind == 1);
   T *x = lds + (size_t)m.g * 2 * ni, *work = x + ni;
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
-  T *dd = reinterpret_cast<T *>(src ? d.sC : d.sB) + (size_t)fi * d.N3 + (size_t)K * d.N2;
+  T *dd = reinterpret_cast<T *>(src ? d.sC : d.sB) + (size_t)(fs.copy0 + fi) * d.N3 + (size_t)K * d.N2;
   const bool upd = live && J >= r.jbot && J <= d.nj;
   if (upd)
     for (int I = 1 + m.lane; I <= ni; I += m.stride) x[I - 1] = dd[ix2(d, J, I)];
@@ -737,7 +737,7 @@ __global__ void __launch_bounds__(256) k_d2dy2_pass(const MemberDesc *D, int mb,
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (jmax + 1) * d.ni) return;
   const int J = t / d.ni, I = 1 + t - J * d.ni;
-  const size_t o = (size_t)fi * d.N3 + (size_t)K * d.N2;
+  const size_t o = (size_t)(fs.copy0 + fi) * d.N3 + (size_t)K * d.N2;
   const T *hh = reinterpret_cast<const T *>(src ? d.sC : d.sB) + o;
   T *out = reinterpret_cast<T *>(src ? d.sB : d.sC) + o;
   const double *cx = vg ? d.vcx : d.hcx, *cy = vg ? d.vcy : d.hcy, *cz = vg ? d.vcz : d.hcz;

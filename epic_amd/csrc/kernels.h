@@ -25,10 +25,13 @@ extern bool g_prof_on;
 
 // Up to four fields processed by one launch of a batched kernel, each with
 // its kind (0 U, 1 V, 2 HDRY, 3 THETA); blockIdx.z = field index * members
-// + member, and field i uses scratch copy i.
+// + member, and field i uses scratch copy copy0 + i (the hyperviscosity
+// launches of hdry and theta take copies 1 and 2, so that they can run beside
+// uv_core, which uses copy 0).
 struct FieldSet {
   int n;
   int f[4], kind[4];
+  int copy0;
 };
 
 // Launch shapes a bucket can be tuned over (epic_engine_autotune); 0 takes
