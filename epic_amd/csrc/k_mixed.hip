@@ -19,6 +19,7 @@
 // and v are formed in FP64 before they are rounded.  This file is selected
 // per bucket when the engine runs in the mixed mode.
 #include "kernels.h"
+#include "filter_row.h"
 
 namespace epic {
 
@@ -116,25 +117,42 @@ __global__ void __launch_bounds__(256) km_dvdt_core(const MemberDesc *D, int mb)
   dv[ix2(d, J, I)] += (double)x;
 }
 
-__global__ void __launch_bounds__(256) km_dudt_core(const MemberDesc *D, int mb) {
-  MEMBER_X;
-  FLAT2_X(0, d.nj + 1, 1, d.ni);
-  const int K = 1 + bx_.y;
+// One block per (K,J) u-row group: the row of KIN is zonally filtered in
+// LDS (rows of the filter band, the FP32 filter of the 2-D zonal filter),
+// then one lane per I forms the u tendency, as k_dudt_core does in FP64.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) km_dudt_core(const MemberDesc *D, int mb) {
+  MEMBER;
+  extern __shared__ double lds_raw[];
+  float *lds = reinterpret_cast<float *>(lds_raw);
+  const RowMap m = row_map(d.ni);
+  const int Jr = blockIdx.x * m.rpb + m.g, K = 1 + blockIdx.y, ni = d.ni;
+  const bool live = Jr <= d.nj;
+  const int J = live ? Jr : 0;
   const size_t o = (size_t)K * d.N2;
   const float *uh = sAf(d) + o, *vh = sBf(d) + o, *kin = sCf(d) + o;
+  float *kr = lds + (size_t)m.g * 2 * ni, *work = kr + ni;  // kr[I-1] = KIN(J,I)
+  if (live)
+    for (int I = 1 + m.lane; I <= ni; I += m.stride) kr[I - 1] = kin[ix2(d, J, I)];
+  __syncthreads();
+  filter_row_lds(d, kr, work, J, 0, live && d.rflag_h[J] != 0, m.lane);
+  __syncthreads();
+  if (!live) return;
   const double *pv = d.PV3 + o;
   double *du = curU(d) + o;
-  auto PV = [&](int j, int i) { return (float)pv[ix2(d, j, i)]; };
-  const PvW<float> w = pv_weights_u<float>(d.pv_al81, PV, J, I);
-  float t = w.al * vh[ix2(d, J + 1, I)] + w.be * vh[ix2(d, J + 1, I - 1)] + w.ga * vh[ix2(d, J, I - 1)] +
-            w.de * vh[ix2(d, J, I)];
-  if (d.pv_al81) t = t + w.e2 * uh[ix2(d, J, I - 1)] - w.e1 * uh[ix2(d, J, I + 1)];
-  float x = (t * w.coef + (kin[ix2(d, J, I - 1)] - kin[ix2(d, J, I)])) * (float)d.m[2 * J + 1];
   const int kk = 2 * K + 1;
-  const float ds_inv = (float)(1. / (d.sgth[kk - 2] - d.sgth[kk + 2]));
-  const float duz = (float)(d.U[ix3(d, K - 1, J, I)] - d.U[ix3(d, K + 1, J, I)]);
-  x -= .5f * (float)(d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K, J, I - 1)]) * duz * ds_inv;
-  du[ix2(d, J, I)] += (double)x;
+  const float ds_inv = (float)(1. / (d.sgth[kk - 2] - d.sgth[kk + 2])), mj = (float)d.m[2 * J + 1];
+  for (int I = 1 + m.lane; I <= ni; I += m.stride) {
+    auto PV = [&](int j, int i) { return (float)pv[ix2(d, j, i)]; };
+    const PvW<float> w = pv_weights_u<float>(d.pv_al81, PV, J, I);
+    float t = w.al * vh[ix2(d, J + 1, I)] + w.be * vh[ix2(d, J + 1, I - 1)] + w.ga * vh[ix2(d, J, I - 1)] +
+              w.de * vh[ix2(d, J, I)];
+    if (d.pv_al81) t = t + w.e2 * uh[ix2(d, J, I - 1)] - w.e1 * uh[ix2(d, J, I + 1)];
+    const float kin_w = kr[I == 1 ? ni - 1 : I - 2], kin_c = kr[I - 1];
+    float x = (t * w.coef + (kin_w - kin_c)) * mj;
+    const float duz = (float)(d.U[ix3(d, K - 1, J, I)] - d.U[ix3(d, K + 1, J, I)]);
+    x -= .5f * (float)(d.W3[ix3(d, K, J, I)] + d.W3[ix3(d, K, J, I - 1)]) * duz * ds_inv;
+    du[ix2(d, J, I)] += (double)x;
+  }
 }
 
 // ------------------------------------------------------------- uv_drag
@@ -291,8 +309,9 @@ void LM_uv_core(hipStream_t s, const Bucket &b) {
   const int nK = b.nk - 1;
   LAUNCH(km_uvh_kin, grid2(b.nj + 2, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
   LAUNCH(km_dvdt_core, grid2(b.nj, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
-  L_filter_f32(s, b, F_SC, 1, b.nk - 1, 0, 0);
-  LAUNCH(km_dudt_core, grid2(b.nj + 1, b.ni, nK, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  const int rpb = rows_per_block(b.ni);
+  LAUNCH(km_dudt_core, dim3((b.nj + 1 + rpb - 1) / rpb, nK, b.nm), dim3(256), sizeof(float) * rpb * 2 * b.ni, s,
+         b.D, b.mb);
 }
 void LM_uv_drag(hipStream_t s, const Bucket &b) {
   LAUNCH(km_sponge, grid2(b.nj + 1, b.ni, b.nk, b.nm), dim3(TPB), 0, s, b.D, b.mb);
