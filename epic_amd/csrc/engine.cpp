@@ -216,10 +216,23 @@ static int rebuild_buckets(epic_engine *E) {
 // (through the filtered GZ3 copy of the 3-D filter) before the second pair
 // writes every output again, so the first forms those only and leaves out
 // P2, H3, EXNER3, T2, RHO2 and DZDT3 -- the step's results are the same.
-static void rec_p2etc_pgrad_vars(hipStream_t s, const Bucket &b, int full) {
+// With `side` (full = 1, gas giants), gz_from_u runs on the second stream
+// beside k_thermo: it reads U, P3 and theta of the bottom interface (forming
+// RHO3 there itself) and writes its own row buffer, k_thermo writes the
+// per-cell diagnostics; k_gz3 waits for both.
+static bool fork_worth(const Bucket &b) { return (size_t)b.ni * (b.nj + 1) * b.nk * b.nm >= ((size_t)1 << 18); }
+static void rec_p2etc_pgrad_vars(hipStream_t s, const Bucket &b, int full, const SideStream *side) {
   L_set_p2etc(s, b, full);  // full = 0: with store_pgrad_vars' T3, RHO3
+  const bool fork = full && side && b.any_gas && fork_worth(b);
+  if (fork) {
+    (void)hipEventRecord(side->fork, s);
+    (void)hipStreamWaitEvent(side->s2, side->fork, 0);
+    L_gz_from_u(side->s2, b);
+    (void)hipEventRecord(side->join, side->s2);
+  }
   if (full) L_thermo(s, b);
-  L_gz_from_u(s, b);
+  if (fork) (void)hipStreamWaitEvent(s, side->join, 0);
+  else L_gz_from_u(s, b);
   L_gz3(s, b);
   if (full) L_dzdt3(s, b);
 }
@@ -229,8 +242,8 @@ static void rec_p2etc_pgrad_vars(hipStream_t s, const Bucket &b, int full) {
 // rotate: the step's closing rotation of the time planes rides along.
 static void rec_store_diag(hipStream_t s, const Bucket &b, int rotate) { L_store_diag(s, b, rotate); }
 
-static void rec_sync(hipStream_t s, const Bucket &b, bool adjust) {
-  rec_p2etc_pgrad_vars(s, b, 1);
+static void rec_sync(hipStream_t s, const Bucket &b, bool adjust, const SideStream *side) {
+  rec_p2etc_pgrad_vars(s, b, 1, side);
   rec_store_diag(s, b, 0);
   L_heating(s, b, adjust, false);
   L_calc_w(s, b, 0);
@@ -250,7 +263,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
   // forked only for buckets of 2^18 grid points or more: at 256x128x20 the
   // branch saves 3% of the FP64 step, at 64x32x10 the cross-queue join
   // costs more than the branch (+9%)
-  const bool fork = hv && side && (size_t)b.ni * (b.nj + 1) * b.nk * b.nm >= ((size_t)1 << 18);
+  const bool fork = hv && side && fork_worth(b);
   // (a failure of the event calls shows in the caller's hipGetLastError,
   // as a failed launch does)
   if (fork) {
@@ -269,7 +282,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
     L_zonal_hv(s, b, hth);
   }
   L_advection(s, b);  // restore_mass first, inside
-  rec_p2etc_pgrad_vars(s, b, 0);
+  rec_p2etc_pgrad_vars(s, b, 0, side);
   // filtered, monotone copy of GZ3
   L_gz_filtercopy(s, b);
   L_gz_mono(s, b);
@@ -278,7 +291,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
   // them, is applied to the filtered hdry as it is stored
   const FieldSet f4 = {4, {F_DUDT_T, F_DVDT_T, F_HDRY, F_THETA}, {0, 1, 2, 3}};
   L_filter3(s, b, f4);
-  rec_p2etc_pgrad_vars(s, b, 1);
+  rec_p2etc_pgrad_vars(s, b, 1, side);
   L_pgrad(s, b);
   if (divd) L_divdamp(s, b);
   L_ab(s, b);
@@ -294,7 +307,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
 }
 
 void member_record(hipStream_t s, const Bucket &b, bool sync_only, bool hv, bool divd, bool adjust) {
-  if (sync_only) rec_sync(s, b, adjust);
+  if (sync_only) rec_sync(s, b, adjust, nullptr);
   else rec_step(s, b, hv, divd, adjust, nullptr);
 }
 
@@ -320,11 +333,11 @@ static void record_all(epic_engine *E, bool sync_only, const std::vector<int> *o
         Bucket one = b;
         one.mb = b.mb + i;
         one.nm = 1;
-        if (sync_only) rec_sync(E->stream, one, adj);
+        if (sync_only) rec_sync(E->stream, one, adj, &E->side);
         else rec_step(E->stream, one, hv, divd, adj, &E->side);
       }
     } else {
-      if (sync_only) rec_sync(E->stream, b, adj);
+      if (sync_only) rec_sync(E->stream, b, adj, &E->side);
       else rec_step(E->stream, b, hv, divd, adj, &E->side);
     }
   }

@@ -277,13 +277,23 @@ __device__ __forceinline__ double hj_relax(const MemberDesc &d, int J, int I, do
 // nothing changes between here and the end of the step, where the reference
 // forms it, and it takes the p^kappa formed here for T3 (a pad takes the
 // value of the column it copies)
+// T3 and RHO3 of an interface cell from its theta and p (return_temp and
+// return_density with x_h2 = 0); pk = (p/press0)^kappa
+__device__ __forceinline__ double t3_of(const MemberDesc &d, double th, double p, double *pk) {
+  *pk = crm_pow(p / d.press0, d.kappa);
+  return th * *pk;
+}
+__device__ __forceinline__ double rho3_of(const MemberDesc &d, double p, double t) {
+  const double R_GAS = 8.314472e+3;
+  return p * d.mu3 / (R_GAS * t);
+}
 __device__ __forceinline__ void thermo_cell(const MemberDesc &d, int K, int J, int I, bool full, double th, double p) {
   const size_t q = ix3(d, K, J, I);
   if (full && K >= 1 && d.dyn->itime > 1 && I >= 1 && I <= d.ni) d.DZDT3[q] = d.GZ3[q];
   const double R_GAS = 8.314472e+3;
-  const double pk = crm_pow(p / d.press0, d.kappa);
-  double t = th * pk;
-  double rho3 = p * d.mu3 / (R_GAS * t);
+  double pk;
+  double t = t3_of(d, th, p, &pk);
+  double rho3 = rho3_of(d, p, t);
   d.T3[q] = t;
   d.RHO3[q] = rho3;
   if (!full) return;
@@ -417,11 +427,16 @@ __global__ void __launch_bounds__(256) k_gzu_means(const MemberDesc *D, int mb) 
   if (!d.is_gas) return;
   extern __shared__ double row[];
   const int J = blockIdx.x, ni = d.ni, nk = d.nk, R = d.nj + 3;
+  // RHO3 of the bottom interface formed here from P3 and theta, as
+  // store_pgrad_vars forms it (the same values), so that this launch need
+  // not wait for the kernel that writes RHO3
   for (int i = threadIdx.x; i < ni; i += blockDim.x) {
     const size_t q = ix3(d, nk, J, i + 1);
+    const double p = d.P3[q];
+    double pk;
     row[i] = d.U[q];
-    row[ni + i] = d.P3[q];
-    row[2 * ni + i] = d.RHO3[q];
+    row[ni + i] = p;
+    row[2 * ni + i] = rho3_of(d, p, t3_of(d, d.THETA[q], p, &pk));
   }
   __syncthreads();
   if (threadIdx.x < 3) d.rowbuf[threadIdx.x * R + J] = seq_acc(row + threadIdx.x * ni, ni) / ni;
