@@ -318,6 +318,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", type=int, default=16,
                     help="requests of the serving mix to run after the step benchmark (0: skip)")
+    ap.add_argument("--serve-policy", choices=["shortest", "lockstep"], default="shortest",
+                    help="serving ticks step the shape ending soonest, or every shape together")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in launch shapes instead of timing the candidates first")
     ap.add_argument("--copy-probe", action="store_true",
@@ -388,7 +390,8 @@ def main():
     if args.serve > 0:
         from epic_amd import serving as sv
         reqs = sv.poisson_mix(args.serve, rate=4.0, seed=2024)
-        res, summ = sv.serve(reqs, precision=prec, hbm_budget=128 << 30, max_members=32, chunk=50)
+        res, summ = sv.serve(reqs, precision=prec, hbm_budget=128 << 30, max_members=32, chunk=50,
+                             policy=args.serve_policy)
         summ["mix"] = "Poisson arrivals at 4/s of {64x32x10,128x64x16,256x128x20,512x256x32} x " \
                       "{held_suarez, jupiter, saturn, venus} x U(200,2000) steps, seed 2024"
         summ["precision"] = pname

@@ -181,6 +181,7 @@ def lib():
         L.epic_engine_remove.argtypes = [vp, ci]
         L.epic_engine_step.argtypes = [vp, ci]
         L.epic_engine_step_members.argtypes = [vp, ctypes.POINTER(ci), ci, ci]
+        L.epic_engine_step_shape.argtypes = [vp, ci, ci, ci, ci]
         L.epic_engine_read.argtypes = [vp, ci, ci, vp]
         L.epic_engine_write.argtypes = [vp, ci, ci, vp]
         L.epic_engine_sync.argtypes = [vp]
@@ -275,6 +276,11 @@ class Engine:
     def step_members(self, slots, n=1):
         arr = (ctypes.c_int * len(slots))(*slots)
         self._check(lib().epic_engine_step_members(self._e, arr, len(slots), n), "step_members")
+
+    def step_shape(self, ni, nj, nk, n=1):
+        """Advance every member of grid shape ni x nj x nk by n steps (the
+        other members wait): the same bits as stepping the whole engine."""
+        self._check(lib().epic_engine_step_shape(self._e, ni, nj, nk, n), "step_shape")
 
     def sync(self):
         self._check(lib().epic_engine_sync(self._e), "sync")

@@ -100,6 +100,9 @@ struct epic_engine {
   hipGraph_t graph;
   hipGraphExec_t gexec;
   bool have_graph;
+  // one graph per bucket (epic_engine_step_shape), captured when first used
+  std::vector<hipGraph_t> bgraph;
+  std::vector<hipGraphExec_t> bgexec;
   char err[512];
 };
 
@@ -145,6 +148,13 @@ static void drop_graph(epic_engine *E) {
     (void)hipGraphDestroy(E->graph);
     E->have_graph = false;
   }
+  for (size_t i = 0; i < E->bgexec.size(); i++)
+    if (E->bgexec[i]) {
+      (void)hipGraphExecDestroy(E->bgexec[i]);
+      (void)hipGraphDestroy(E->bgraph[i]);
+    }
+  E->bgexec.clear();
+  E->bgraph.clear();
 }
 
 extern "C" void epic_engine_destroy(epic_engine *E) {
@@ -792,6 +802,43 @@ extern "C" int epic_engine_step(epic_engine *E, int nsteps) {
   }
   HIPCHK(hipGetLastError());
   for (int s : E->order) E->m[s].steps += nsteps;
+  return 0;
+}
+
+extern "C" int epic_engine_step_shape(epic_engine *E, int ni, int nj, int nk, int nsteps) {
+  if (!E) return -1;
+  if (E->dirty && rebuild_buckets(E)) return -5;
+  size_t bi = 0;
+  while (bi < E->buckets.size() &&
+         !(E->buckets[bi].ni == ni && E->buckets[bi].nj == nj && E->buckets[bi].nk == nk))
+    bi++;
+  if (bi == E->buckets.size()) {
+    snprintf(E->err, sizeof(E->err), "no member of shape %dx%dx%d", ni, nj, nk);
+    return -1;
+  }
+  const Bucket &b = E->buckets[bi];
+  bool hv, divd, adj;
+  bucket_flags(E, b, hv, divd, adj);
+  if (g_prof_on || getenv("EPIC_AMD_NO_GRAPH") != nullptr) {
+    for (int n = 0; n < nsteps; n++) rec_step(E->stream, b, hv, divd, adj, &E->side);
+    HIPCHK(hipGetLastError());
+    for (int i = 0; i < b.nm; i++) E->m[E->order[b.mb + i]].steps += nsteps;
+    return 0;
+  }
+  if (E->bgexec.size() != E->buckets.size()) {
+    E->bgexec.assign(E->buckets.size(), nullptr);
+    E->bgraph.assign(E->buckets.size(), nullptr);
+  }
+  if (!E->bgexec[bi]) {
+    HIPCHK(hipStreamSynchronize(E->stream));
+    HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+    rec_step(E->stream, b, hv, divd, adj, &E->side);
+    HIPCHK(hipStreamEndCapture(E->stream, &E->bgraph[bi]));
+    HIPCHK(hipGraphInstantiate(&E->bgexec[bi], E->bgraph[bi], nullptr, nullptr, 0));
+  }
+  for (int n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(E->bgexec[bi], E->stream));
+  HIPCHK(hipGetLastError());
+  for (int i = 0; i < b.nm; i++) E->m[E->order[b.mb + i]].steps += nsteps;
   return 0;
 }
 

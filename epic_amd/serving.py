@@ -2,8 +2,12 @@
 
 Requests are whole integrations: (planet, grid, steps, seed) arriving over
 time.  The loop keeps every admitted integration resident on the device as
-one engine member and advances all of them together (members of one grid
-shape share every kernel launch).  A request is admitted when it has
+one engine member; members of one grid shape share every kernel launch.
+Each tick advances one grid shape, the one whose first member would finish
+soonest (shortest remaining time first, from the measured time per step of
+each shape), so a short integration is not held behind the steps of every
+larger one; `policy="lockstep"` advances all shapes together instead.  A
+request is admitted when it has
 arrived, a member slot is free and its HBM footprint fits the budget
 (first come, first served; a request that does not fit blocks only itself,
 later smaller requests may pass it).  Between ticks, finished integrations
@@ -104,10 +108,14 @@ class Server:
     tick advances every member by up to `chunk` steps, cut short where a
     member finishes or where the next request that would fit arrives (from
     the measured time per step of the current mix), so an arrival waits
-    about one step, not a whole chunk, for its tick."""
+    about one step, not a whole chunk, for its tick.
+
+    policy "shortest" (engines with step_shape): a tick steps only the grid
+    shape with the least estimated time to its first member's end;
+    "lockstep": a tick steps every member."""
 
     def __init__(self, engine, hbm_budget, max_members, precision=ea.FP64, chunk=50, keep_fields=False,
-                 footprint=None, clock=time.perf_counter, sleep=time.sleep, prebuild=2):
+                 footprint=None, clock=time.perf_counter, sleep=time.sleep, prebuild=2, policy="shortest"):
         self.eng = engine
         self.budget = hbm_budget
         self.max_members = max_members
@@ -125,7 +133,12 @@ class Server:
         self.active = {}   # slot -> [request, steps_left, admitted_time, footprint]
         self.in_use = 0
         self.point_steps = 0
+        if policy not in ("shortest", "lockstep"):
+            raise ValueError(f"policy {policy!r}: shortest or lockstep")
+        self.by_shape = policy == "shortest" and hasattr(engine, "step_shape")
         self.t_step = None  # seconds per step of the current mix (last tick)
+        self.t_shape = {}   # (ni, nj, nk) -> seconds per step of that shape's members
+        self.s_per_pt = None  # seconds per resident point-step (last tick)
         self.ticks = 0
         self.host_s = 0.0   # time in admission and retirement (device idle)
 
@@ -191,13 +204,40 @@ class Server:
             status = "ok" if err == 0 else f"failed: {err}"
             results.append(Result(r.rid, status, r.arrival, t_adm, now, r.steps - left, checksum, fields))
 
-    def _tick_steps(self, waiting, now):
+    def _est(self, shape, pts):
+        """Estimated seconds per step of one shape's members (pts points)."""
+        t = self.t_shape.get(shape)
+        if t is None and self.s_per_pt:
+            t = pts * self.s_per_pt
+        return t
+
+    def _pick(self):
+        """The shape whose first member would finish soonest, and the
+        members' fewest steps left.  Unmeasured shapes are ranked by
+        points x steps left."""
+        groups = {}
+        for v in self.active.values():
+            r = v[0]
+            g = groups.setdefault((r.ni, r.nj, r.nk), [0, v[1]])
+            g[0] += r.points
+            g[1] = min(g[1], v[1])
+        def cost(item):
+            shape, (pts, left) = item
+            t = self._est(shape, pts)
+            return left * (t if t is not None else pts * 1e-9)
+        shape, (pts, left) = min(groups.items(), key=cost)
+        return shape, left, self._est(shape, pts)
+
+    def _tick_steps(self, waiting, now, left=None, t_step=None):
         """Steps of the next tick: up to `chunk`, no member past its end,
         ending about when the next request that would fit arrives, or soon
-        after an arrived request's case is built."""
-        n = min(self.chunk, min(v[1] for v in self.active.values()))
-        if not self.t_step:
-            return n
+        after an arrived request's case is built.  `left` and `t_step` are
+        the stepped members' fewest steps left and seconds per step."""
+        if left is None:
+            left, t_step = min(v[1] for v in self.active.values()), self.t_step
+        n = min(self.chunk, left)
+        if not t_step:
+            return 1  # unmeasured: one step, which measures it
         cut = []
         nxt = next((r for r in waiting if r.arrival > now), None)
         if nxt is not None and self._fits(nxt):
@@ -205,7 +245,7 @@ class Server:
         if self.built:
             cut.append(0.05)
         if cut:
-            n = min(n, max(1, math.ceil(min(cut) / self.t_step)))
+            n = min(n, max(1, math.ceil(min(cut) / t_step)))
         return n
 
     def run(self, requests, time_limit=None):
@@ -236,13 +276,25 @@ class Server:
                     elif waiting:
                         self.sleep(max(0.0, waiting[0].arrival - (self.clock() - t0)))
                     continue
-                n = self._tick_steps(waiting, now)
-                s0 = self.clock()
-                self.eng.step(n)
-                self.eng.wait()
-                self.t_step = (self.clock() - s0) / n
+                if self.by_shape:
+                    shape, left, t_est = self._pick()
+                    n = self._tick_steps(waiting, now, left, t_est)
+                    s0 = self.clock()
+                    self.eng.step_shape(*shape, n)
+                    self.eng.wait()
+                    stepped = [v for v in self.active.values() if (v[0].ni, v[0].nj, v[0].nk) == shape]
+                    t = (self.clock() - s0) / n
+                    self.t_shape[shape] = t
+                    self.s_per_pt = t / sum(v[0].points for v in stepped)
+                else:
+                    n = self._tick_steps(waiting, now)
+                    s0 = self.clock()
+                    self.eng.step(n)
+                    self.eng.wait()
+                    self.t_step = (self.clock() - s0) / n
+                    stepped = list(self.active.values())
                 self.ticks += 1
-                for v in self.active.values():
+                for v in stepped:
                     v[1] -= n
                     self.point_steps += v[0].points * n
                 now = self.clock() - t0
@@ -275,13 +327,15 @@ def summarize(results, elapsed, point_steps):
     }
 
 
-def serve(requests, precision=ea.FP64, hbm_budget=64 << 30, max_members=64, chunk=50, keep_fields=False):
+def serve(requests, precision=ea.FP64, hbm_budget=64 << 30, max_members=64, chunk=50, keep_fields=False,
+          policy="shortest"):
     """Run a request list on the GPU; returns (results, summary)."""
     eng = ea.Engine(precision, max_members, hbm_budget)
-    srv = Server(eng, hbm_budget, max_members, precision, chunk, keep_fields)
+    srv = Server(eng, hbm_budget, max_members, precision, chunk, keep_fields, policy=policy)
     res = srv.run(requests)
     eng.close()
     summ = summarize(res, srv.elapsed, srv.point_steps)
     summ["ticks"] = srv.ticks
+    summ["policy"] = policy
     summ["host_s"] = srv.host_s  # admission and retirement, device idle
     return res, summ

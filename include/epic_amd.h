@@ -358,6 +358,11 @@ int  epic_engine_set_time_planes(epic_engine *e, int slot, const int *in4);
 int  epic_engine_step(epic_engine *e, int nsteps);
 /* Advance only the listed members. */
 int  epic_engine_step_members(epic_engine *e, const int *slots, int nslots, int nsteps);
+/* Advance every member of one grid shape (one bucket) by nsteps, from a
+ * graph of that bucket alone: a scheduler can give each shape its own
+ * number of steps (no counterpart in the reference, which steps one
+ * domain).  -1 if no member has that shape. */
+int  epic_engine_step_shape(epic_engine *e, int ni, int nj, int nk, int nsteps);
 /* Copy one field of one member into host memory (Nelem3d doubles, or
  * Nelem2d for EPIC_F_GZ_SURFACE; tendencies are indexed by time plane). */
 int  epic_engine_read(epic_engine *e, int slot, int field, double *out);

@@ -61,6 +61,38 @@ def test_step_members_subset():
     e.close()
 
 
+@pytest.mark.parametrize("precision", [ea.FP64, ea.MIXED])
+def test_step_shape_steps_one_shape_with_the_same_bits(precision):
+    """Stepping one grid shape at a time (its own graph, captured on first
+    use and again after a member leaves) gives each member the bits it
+    gets from the whole engine's steps."""
+    say("step_shape", precision)
+    e = ea.Engine(precision, 4)
+    sa, sb, sc = e.add(A), e.add(B), e.add(C)
+    e.step_shape(64, 32, 10, 5)      # A and C
+    e.step_shape(32, 16, 8, 3)       # B
+    e.step_shape(64, 32, 10, 2)
+    assert [e.steps_done(s) for s in (sa, sb, sc)] == [7, 3, 7]
+    with pytest.raises(RuntimeError):
+        e.step_shape(16, 8, 4, 1)
+    r = ea.Engine(precision, 4)
+    ra, rb, rc = r.add(A), r.add(B), r.add(C)
+    r.step(3)
+    r.step_members([ra, rc], 4)
+    for k, (s, t) in {"A": (sa, ra), "B": (sb, rb), "C": (sc, rc)}.items():
+        for f in PROG:
+            assert e.read(s, f).tobytes() == r.read(t, f).tobytes(), (k, f)
+    # after a member leaves, the shape's graph is captured again
+    e.remove(sa)
+    r.remove(ra)
+    e.step_shape(64, 32, 10, 3)
+    r.step_members([rc], 3)
+    for f in PROG:
+        assert e.read(sc, f).tobytes() == r.read(rc, f).tobytes(), f
+    e.close()
+    r.close()
+
+
 def test_hbm_budget_is_enforced():
     fp = ea.footprint(64, 32, 10, ea.FP64)
     e = ea.Engine(ea.FP64, 4, hbm_budget=int(fp * 1.5))

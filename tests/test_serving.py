@@ -20,6 +20,7 @@ class FakeEngine:
         self.next = 0
         self.max_resident = 0
         self.joined_while_busy = 0
+        self.shape_ticks = 0
 
     def add(self, sp):
         if self.members:
@@ -35,6 +36,14 @@ class FakeEngine:
         for m in self.members.values():
             m[1] += n
         self.clock.t += n * pts * 1e-10
+
+    def step_shape(self, ni, nj, nk, n):
+        hit = [m for m in self.members.values() if (m[0].ni, m[0].nj, m[0].nk) == (ni, nj, nk)]
+        assert hit, "no member of that shape"
+        for m in hit:
+            m[1] += n
+        self.clock.t += n * sum(sp.ni * (sp.nj + 1) * sp.nk for sp, _ in hit) * 1e-10
+        self.shape_ticks += 1
 
     def wait(self):
         pass
@@ -121,6 +130,10 @@ class FakeCaseEngine(FakeEngine):
         super().step(n)
         time.sleep(n * 1e-3)  # device time passes while the worker builds
 
+    def step_shape(self, ni, nj, nk, n):
+        super().step_shape(ni, nj, nk, n)
+        time.sleep(n * 1e-3)
+
     def add_case(self, c, spec=None):
         assert (c.ni, c.nj, c.nk, c.dt) == (spec.ni, spec.nj, spec.nk, spec.dt)
         self.from_cases += 1
@@ -152,3 +165,54 @@ def test_arrival_joins_within_a_step_not_a_chunk():
     step = first.points * 1e-10  # the fake engine's time per step with one member
     assert 0 <= res[1].admitted - late.arrival <= 1.01 * step
     assert res[0].steps == first.steps and res[1].steps == late.steps
+
+
+def _run(policy, reqs):
+    clk = Clock()
+    eng = FakeEngine(clk)
+    srv = sv.Server(eng, 1 << 40, max_members=16, chunk=50, footprint=footprint, clock=clk, sleep=clk.sleep,
+                    policy=policy)
+    res = srv.run(reqs)
+    return eng, srv, {r.rid: r for r in res}
+
+
+def test_shortest_first_steps_one_shape_and_cuts_mean_latency():
+    """Stepping the shape whose first member ends soonest finishes the same
+    work (every request, all its steps, the same point-steps) with a lower
+    mean latency than stepping every member together; the last one ends at
+    about the same time (the device does the same work)."""
+    reqs = sv.poisson_mix(24, rate=400.0, seed=11, steps=(50, 500))
+    eng_s, srv_s, res_s = _run("shortest", reqs)
+    eng_l, srv_l, res_l = _run("lockstep", reqs)
+    assert eng_s.shape_ticks == srv_s.ticks > 0 and eng_l.shape_ticks == 0
+    for res in (res_s, res_l):
+        assert len(res) == 24 and all(r.status == "ok" for r in res.values())
+        for r in res.values():
+            req = reqs[r.rid]
+            assert r.steps == req.steps and r.checksum == req.steps * req.ni * (req.nj + 1) * req.nk
+    assert srv_s.point_steps == srv_l.point_steps == sum(r.steps * r.points for r in reqs)
+    mean = lambda res: np.mean([r.latency for r in res.values()])
+    assert mean(res_s) < 0.8 * mean(res_l)
+    last = lambda res: max(r.finished for r in res.values())
+    assert abs(last(res_s) - last(res_l)) <= 0.05 * last(res_l)
+
+
+def test_short_request_of_a_small_shape_passes_a_long_large_one():
+    big = sv.Request(0, ea.JUPITER_LIKE, 512, 256, 32, 2000, 1, 0.0)
+    small = sv.Request(1, ea.HELD_SUAREZ, 64, 32, 10, 200, 2, 0.001)
+    eng, srv, res = _run("shortest", [big, small])
+    step_small = small.points * 1e-10
+    # the small one waits at most about one step of the big one to join,
+    # then runs alone: its latency is its own work, not the big one's
+    assert res[1].latency <= 200 * step_small + 2 * big.points * 1e-10
+    assert res[1].finished < res[0].finished
+    assert res[0].steps == 2000 and res[1].steps == 200
+
+
+def test_unknown_policy_is_refused():
+    try:
+        sv.Server(FakeEngine(Clock()), 1 << 30, 4, policy="fifo")
+    except ValueError as e:
+        assert "fifo" in str(e)
+    else:
+        raise AssertionError("accepted an unknown policy")
