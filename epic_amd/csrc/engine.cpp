@@ -172,6 +172,14 @@ extern "C" void epic_engine_destroy(epic_engine *E) {
   delete E;
 }
 
+// Members share a bucket when their grids are the same; in the mixed mode
+// also only when both or neither apply a hyperviscosity order above del^4:
+// a del^4 member's FP32 zonal means are formed by the fused del^4 row
+// kernel, in its order, whatever the other members of its grid apply.
+static int hv_class(const epic_engine *E, const MemberDesc &h) {
+  return (E->precision == EPIC_MIXED && hv_passes(h) > 2) ? 1 : 0;
+}
+
 static int rebuild_buckets(epic_engine *E) {
   E->order.clear();
   for (int s = 0; s < E->max_members; s++)
@@ -180,7 +188,8 @@ static int rebuild_buckets(epic_engine *E) {
     const MemberDesc &x = E->m[a].h, &y = E->m[b].h;
     if (x.ni != y.ni) return x.ni < y.ni;
     if (x.nj != y.nj) return x.nj < y.nj;
-    return x.nk < y.nk;
+    if (x.nk != y.nk) return x.nk < y.nk;
+    return hv_class(E, x) < hv_class(E, y);
   });
   std::vector<MemberDesc> hd;
   for (int s : E->order) hd.push_back(E->m[s].h);
@@ -189,7 +198,9 @@ static int rebuild_buckets(epic_engine *E) {
   E->buckets.clear();
   for (size_t i = 0; i < hd.size();) {
     size_t j = i;
-    while (j < hd.size() && hd[j].ni == hd[i].ni && hd[j].nj == hd[i].nj && hd[j].nk == hd[i].nk) j++;
+    while (j < hd.size() && hd[j].ni == hd[i].ni && hd[j].nj == hd[i].nj && hd[j].nk == hd[i].nk &&
+           hv_class(E, hd[j]) == hv_class(E, hd[i]))
+      j++;
     Bucket b;
     b.D = E->dD;
     b.mb = (int)i;
@@ -279,8 +290,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
   if (fork) {
     (void)hipEventRecord(side->fork, s);
     (void)hipStreamWaitEvent(side->s2, side->fork, 0);
-    L_merid_hv(side->s2, b, hth);
-    L_zonal_hv(side->s2, b, hth);
+    L_hv(side->s2, b, hth);
     (void)hipEventRecord(side->join, side->s2);
   }
   L_uv_core(s, b);
@@ -288,8 +298,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
   if (fork) {
     (void)hipStreamWaitEvent(s, side->join, 0);
   } else if (hv) {
-    L_merid_hv(s, b, hth);
-    L_zonal_hv(s, b, hth);
+    L_hv(s, b, hth);
   }
   L_advection(s, b);  // restore_mass first, inside
   rec_p2etc_pgrad_vars(s, b, 0, side);
@@ -307,8 +316,7 @@ static void rec_step(hipStream_t s, const Bucket &b, bool hv, bool divd, bool ad
   L_ab(s, b);
   if (hv) {
     const FieldSet uv = {2, {F_U, F_V}, {0, 1}};
-    L_merid_hv(s, b, uv);
-    L_zonal_hv(s, b, uv);
+    L_hv(s, b, uv);
   }
   // store_diag, then the time planes rotated (in the same launch: nothing
   // after the Adams-Bashforth update reads them)
@@ -808,37 +816,43 @@ extern "C" int epic_engine_step(epic_engine *E, int nsteps) {
 extern "C" int epic_engine_step_shape(epic_engine *E, int ni, int nj, int nk, int nsteps) {
   if (!E) return -1;
   if (E->dirty && rebuild_buckets(E)) return -5;
-  size_t bi = 0;
-  while (bi < E->buckets.size() &&
-         !(E->buckets[bi].ni == ni && E->buckets[bi].nj == nj && E->buckets[bi].nk == nk))
-    bi++;
-  if (bi == E->buckets.size()) {
+  std::vector<size_t> sel;  // the buckets of the shape (two in the mixed mode when hv orders differ)
+  for (size_t bi = 0; bi < E->buckets.size(); bi++)
+    if (E->buckets[bi].ni == ni && E->buckets[bi].nj == nj && E->buckets[bi].nk == nk) sel.push_back(bi);
+  if (sel.empty()) {
     snprintf(E->err, sizeof(E->err), "no member of shape %dx%dx%d", ni, nj, nk);
     return -1;
   }
-  const Bucket &b = E->buckets[bi];
-  bool hv, divd, adj;
-  bucket_flags(E, b, hv, divd, adj);
-  if (g_prof_on || getenv("EPIC_AMD_NO_GRAPH") != nullptr) {
-    for (int n = 0; n < nsteps; n++) rec_step(E->stream, b, hv, divd, adj, &E->side);
-    HIPCHK(hipGetLastError());
-    for (int i = 0; i < b.nm; i++) E->m[E->order[b.mb + i]].steps += nsteps;
-    return 0;
-  }
-  if (E->bgexec.size() != E->buckets.size()) {
+  const bool use_graph = !g_prof_on && getenv("EPIC_AMD_NO_GRAPH") == nullptr;
+  if (use_graph && E->bgexec.size() != E->buckets.size()) {
     E->bgexec.assign(E->buckets.size(), nullptr);
     E->bgraph.assign(E->buckets.size(), nullptr);
   }
-  if (!E->bgexec[bi]) {
+  for (size_t bi : sel) {
+    if (!use_graph || E->bgexec[bi]) continue;
+    const Bucket &b = E->buckets[bi];
+    bool hv, divd, adj;
+    bucket_flags(E, b, hv, divd, adj);
     HIPCHK(hipStreamSynchronize(E->stream));
     HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
     rec_step(E->stream, b, hv, divd, adj, &E->side);
     HIPCHK(hipStreamEndCapture(E->stream, &E->bgraph[bi]));
     HIPCHK(hipGraphInstantiate(&E->bgexec[bi], E->bgraph[bi], nullptr, nullptr, 0));
   }
-  for (int n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(E->bgexec[bi], E->stream));
+  for (int n = 0; n < nsteps; n++)
+    for (size_t bi : sel) {
+      if (use_graph) {
+        HIPCHK(hipGraphLaunch(E->bgexec[bi], E->stream));
+      } else {
+        const Bucket &b = E->buckets[bi];
+        bool hv, divd, adj;
+        bucket_flags(E, b, hv, divd, adj);
+        rec_step(E->stream, b, hv, divd, adj, &E->side);
+      }
+    }
   HIPCHK(hipGetLastError());
-  for (int i = 0; i < b.nm; i++) E->m[E->order[b.mb + i]].steps += nsteps;
+  for (size_t bi : sel)
+    for (int i = 0; i < E->buckets[bi].nm; i++) E->m[E->order[E->buckets[bi].mb + i]].steps += nsteps;
   return 0;
 }
 

@@ -923,4 +923,159 @@ void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   else zonal_hv_t<double>(s, b, fs);
 }
 
+// Zonal mean of a row of 256 points or more in the mixed mode (FP32): the
+// row's ni values are summed as 64 partial sums, partial l = y[l] + y[l+64]
+// + y[l+128] + ... left to right, which then meet pairwise, l with l+32,
+// then l+16, ..., l+1 (a wavefront butterfly; IEEE addition commutes, so
+// every lane ends with the same sum), and the sum is divided by ni.  Called
+// by every lane of a wavefront, lane l holding partial l.
+__device__ __forceinline__ float mean_butterfly(float p, int ni) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o);
+  return p / float(ni);
+}
+
+// del^4 after its meridional second differences, one row at a time (rows
+// of members whose highest order is del^4): the row of D (copy i of sB) is
+// zonally filtered and applied, A += DT*(-1)*D, then the zonal del^4 of the
+// updated row (two second differences, each less its zonal mean) is
+// applied, A += DT*(-1)*D_zonal; the zonal pads of every row 0..nj+1 hold
+// copies of points ni and 1.  The steps and expressions are those of
+// k_hv_apply (pass 1) followed by k_zonal_hv, with the row read and written
+// once instead of twice.  A row's ni/4 lanes (RowMap) each own the points
+// I = 1 + lane + (ni/4) q, q < 4; three row buffers in LDS (ni + 2 values,
+// pads included) carry the row, the first second difference and the input
+// of the second.  The zonal means: BFLY, the mixed mode's order for rows of
+// 256 points or more (mean_butterfly; a row's lanes are whole wavefronts);
+// otherwise sums left to right, as the reference and k_zonal_hv form them,
+// each lane of a row forming the same sum itself (which spares a barrier
+// to hand it round).
+template <class T, bool BFLY>
+__global__ void __launch_bounds__(256) k_hv4_rows(const MemberDesc *D, int mb, int nm, FieldSet fs) {
+  SET_MEMBER;
+  extern __shared__ double lds_raw[];
+  const RowMap m = row_map(d.ni);
+  const int ni = d.ni, WS = ni + 2;
+  T *z0 = reinterpret_cast<T *>(lds_raw) + (size_t)m.g * 3 * WS, *z1 = z0 + WS, *z2 = z1 + WS;
+  HvRange r = hv_range(d, kind);
+  const int K = r.k0 + blockIdx.y, Jr = blockIdx.x * m.rpb + m.g;
+  if (K > r.k1 || hv_passes(d) != 2) return;  // uniform per block
+  const bool live = Jr <= d.nj + 1;
+  const int J = live ? Jr : 0, vg = (kind == 1);
+  double *a = fld_ptr(d, f) + (size_t)K * d.N2;
+  const T *dd = reinterpret_cast<const T *>(d.sB) + (size_t)(fs.copy0 + fi) * d.N3 + (size_t)K * d.N2;
+  const bool upd = live && J >= r.jbot && J <= d.nj;
+  // the meridional part: D filtered in place (x = z0[1..ni], work = z1)
+  T *x = z0 + 1;
+  if (upd)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int I = 1 + m.lane + m.stride * q;
+      x[I - 1] = dd[ix2(d, J, I)];
+    }
+  __syncthreads();
+  filter_row_lds(d, x, z1, J, vg, upd && (vg ? d.rflag_v[J] : d.rflag_h[J]) != 0, m.lane);
+  __syncthreads();
+  double v[4];
+  const double sign = -1.;  // del^4
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int I = 1 + m.lane + m.stride * q;
+    v[q] = live ? a[ix2(d, J, I)] : 0.;
+    if (upd) v[q] += d.dt * sign * (double)x[I - 1];
+  }
+  // the updated row, pads included, as the zonal part reads it (each lane
+  // rewrites only the points it read)
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int I = 1 + m.lane + m.stride * q;
+    const T t = T(v[q]);
+    z0[I] = t;
+    if (I == ni) z0[0] = t;
+    if (I == 1) z0[ni + 1] = t;
+  }
+  const int Jc = min(J, d.nj);
+  const double mm = fabs(d.lat[2 * Jc + r.jh]) > 45. ? d.m0z : d.m[2 * Jc + r.jh];
+  T y[4];
+  __syncthreads();
+  for (int pass = 0; pass < 2; pass++) {
+    // pass 0 reads z0 and leaves its raw result in z1; pass 1 reads z2 (pass
+    // 0 less its mean) and leaves its raw result in z0
+    const T *xin = pass ? z2 : z0;
+    T *raw = pass ? z0 : z1;
+    const T dcu = T(hv_coef(d, pass) * mm * mm);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int I = 1 + m.lane + m.stride * q;
+      y[q] = dcu * (xin[I - 1] - T(2.) * xin[I] + xin[I + 1]);
+      raw[I] = y[q];
+    }
+    __syncthreads();
+    T fm;
+    if constexpr (BFLY) {
+      // each wavefront of the row's lanes forms the 64 partials itself
+      const int l = m.lane & 63;
+      T p = raw[1 + l];
+      for (int k = l + 64; k < ni; k += 64) p += raw[1 + k];
+      fm = mean_butterfly(p, ni);
+    } else {
+      fm = seq_acc(raw + 1, ni) / T(ni);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) y[q] -= fm;
+    if (pass == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int I = 1 + m.lane + m.stride * q;
+        z2[I] = y[q];
+        if (I == ni) z2[0] = y[q];
+        if (I == 1) z2[ni + 1] = y[q];
+      }
+      __syncthreads();
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int I = 1 + m.lane + m.stride * q;
+    if (upd) {
+      v[q] += d.dt * sign * (double)y[q];
+      a[ix2(d, J, I)] = v[q];
+    }
+    if (I == ni) a[ix2(d, J, 0)] = v[q];
+    if (I == 1) a[ix2(d, J, ni + 1)] = v[q];
+  }
+}
+
+// del^4 (and higher orders) of a field set: the meridional part, then the
+// zonal part.  In the mixed mode with del^4 the highest order of the bucket
+// (the engine keeps such members apart from those with higher orders), the
+// filtered meridional update and the zonal part run as one row kernel
+// (k_hv4_rows); the FP64 path keeps the separate kernels (faster there: its
+// zonal means are the reference's serial sums, which k_zonal_hv spreads
+// over more rows at once).
+static void hv4_mixed(hipStream_t s, const Bucket &b, const FieldSet &fs) {
+  using T = float;
+  bool any_v = false;
+  for (int i = 0; i < fs.n; i++) any_v |= (fs.kind[i] == 1);
+  const int jrows = any_v ? b.nj + 2 : b.nj + 1;
+  int CW = 64;
+  while (b.ni % CW) CW >>= 1;
+  dim3 g((jrows + D2_TJ - 1) / D2_TJ * (b.ni / CW), b.nk, fs.n * b.nm);
+  LAUNCH(k_d2dy2x2<T>, g, dim3(256), sizeof(T) * (2 * D2_TJ + 10) * CW, s, b.D, b.mb, b.nm, fs, CW);
+  const int rpb = rows_per_block(b.ni);
+  const dim3 ga((b.nj + 2 + rpb - 1) / rpb, b.nk, fs.n * b.nm);
+  const size_t shm = sizeof(T) * rpb * 3 * (b.ni + 2);
+  if (b.ni >= 256) LAUNCH_AS("k_hv4_rows", (k_hv4_rows<T, true>), ga, dim3(256), shm, s, b.D, b.mb, b.nm, fs);
+  else LAUNCH_AS("k_hv4_rows", (k_hv4_rows<T, false>), ga, dim3(256), shm, s, b.D, b.mb, b.nm, fs);
+}
+void L_hv(hipStream_t s, const Bucket &b, const FieldSet &fs) {
+  if (b.mixed && b.hv_passes == 2 && b.ni >= 16 && b.ni <= 1024) {
+    hv4_mixed(s, b, fs);
+    return;
+  }
+  L_merid_hv(s, b, fs);
+  L_zonal_hv(s, b, fs);
+}
+
 }  // namespace epic

@@ -114,6 +114,7 @@ void L_gz_filtercopy(hipStream_t s, const Bucket &b);
 // part keeps its del^4 result in scratch copy i of sB)
 void L_merid_hv(hipStream_t s, const Bucket &b, const FieldSet &fs);
 void L_zonal_hv(hipStream_t s, const Bucket &b, const FieldSet &fs);
+void L_hv(hipStream_t s, const Bucket &b, const FieldSet &fs);
 
 // ---- mixed-precision momentum kernels (k_mixed.hip)
 void LM_uv_core(hipStream_t s, const Bucket &b);
