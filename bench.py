@@ -67,7 +67,9 @@ KERNEL_BYTES = {
     "k_gz3": (40, 1), "k_calc_w": (56, 1), "k_dzdt3": (72, 1),
     "k_hsu_vert": (24, 3), "k_hsu_mer": (40, 3),
     "k_hsu_zonal": (32, 3), "k_fft_filter": (16, 3),
-    "k_filter3": (24, 4), "k_zonal_hv": (16, 2), "k_d2dy2x2": (16, 2), "k_hv_apply": (24, 2),
+    "k_filter3": (24, 4), "k_zonal_hv": (16, 2), "k_d2dy2_cols": (16, 2), "k_hv_apply": (24, 2),
+    # the mixed mode's fused del^4 rows: D (FP32) read, the field read and written
+    "k_hv4_rows": (20, 2),
     "k_pgrad": (48, 1), "k_uvh_kin": (48, 1), "k_dvdt_core": (40, 1), "k_dudt_core": (40, 1), "k_ab": (80, 1),
     "k_divdamp": (40, 1), "km_uvh_kin": (36, 1), "km_dvdt_core": (32, 1), "km_dudt_core": (32, 1), "km_ab": (64, 1),
     "km_divdamp": (40, 1), "k_gz_mono": (16, 1),

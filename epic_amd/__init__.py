@@ -24,7 +24,7 @@ FIELDS = ["U", "V", "HDRY", "THETA", "P3", "P2", "H3", "THETA2", "T2", "T3", "RH
           "DUDT2", "DVDT0", "DVDT1", "DVDT2", "GZ_SURFACE", "RI2"]
 FIELD = {n: i for i, n in enumerate(FIELDS)}
 # launch-shape knobs of the autotuner (epic_engine_set_shape's numbering)
-SHAPE_KNOBS = ["cols_p2etc", "cols_gz3", "cols_calcw", "cols_hsuv", "rows_zhv", "rows_mer"]
+SHAPE_KNOBS = ["cols_p2etc", "cols_gz3", "cols_calcw", "cols_hsuv", "rows_zhv", "rows_mer", "rows_d2"]
 
 
 K_SPONGE_DEFAULT = -1  # the reference `initial` program's choice of k_sponge

@@ -1209,7 +1209,7 @@ extern "C" int epic_engine_restore(epic_engine *E, const epic_case *c, const cha
 // them; the shapes never change a result.  The graph is re-captured with
 // the chosen shapes at the next step.
 static const char *const knob_name[KN_COUNT] = {"cols_p2etc", "cols_gz3", "cols_calcw",
-                                                "cols_hsuv",  "rows_zhv", "rows_mer"};
+                                                "cols_hsuv",  "rows_zhv", "rows_mer", "rows_d2"};
 
 static bool knob_fits(const Bucket &b, int k, int v) {
   const size_t W = (size_t)b.nk + 1, L = (size_t)b.nk + 2, lim = 128 * 1024;
@@ -1227,6 +1227,7 @@ static bool knob_fits(const Bucket &b, int k, int v) {
     case KN_COLS_CALCW: return sizeof(double) * (4 * v * W + v) <= lim;
     case KN_COLS_HSUV: return sizeof(double) * 4 * v * L <= lim;
     case KN_ROWS_MER: return sizeof(double) * (size_t)(b.ni < 64 ? b.ni : 64) * (6 * v + 15) <= lim;
+    case KN_ROWS_D2: return true;  // registers only
     default:  // KN_ROWS_ZHV: one row array per row, and at most 16 points per lane
       return sizeof(double) * ((size_t)v * ((b.ni + 2) | 1) + v) <= lim && v / 4 * (b.ni >= 64 ? b.ni / 64 : 1) <= 16;
   }
@@ -1234,18 +1235,20 @@ static bool knob_fits(const Bucket &b, int k, int v) {
 
 // the candidates: column tiles of 8-64 columns per block of 256, 512 or
 // 1024 threads, or of 4-16 columns with one lane per level; 4-16 rows per zonal-hyperviscosity block; 2-8 rows per
-// meridional-advection tile
+// meridional-advection tile; 8-32 rows per column tile of the del^4
+// meridional second differences
 static std::vector<int> knob_candidates(int k) {
   static const int cols[16] = {8,    16,   32,   64,   2008,  2016,  2032,  2064,
                                4008, 4016, 4032, 4064, 10004, 10008, 10012, 10016};
-  static const int zhv[3] = {4, 8, 16}, mer[3] = {2, 4, 8};
+  static const int zhv[3] = {4, 8, 16}, mer[3] = {2, 4, 8}, d2[3] = {8, 16, 32};
   if (k <= KN_COLS_HSUV) return std::vector<int>(cols, cols + 16);
+  if (k == KN_ROWS_D2) return std::vector<int>(d2, d2 + 3);
   return k == KN_ROWS_ZHV ? std::vector<int>(zhv, zhv + 3) : std::vector<int>(mer, mer + 3);
 }
 
 static bool knob_times(const char *kernel, int k) {
   static const char *const pfx[KN_COUNT] = {"k_set_p2etc", "k_gz3", "k_calc_w", "k_hsu_vert", "k_zonal_hv",
-                                            "k_hsu_mer"};
+                                            "k_hsu_mer", "k_d2dy2_cols"};
   return strncmp(kernel, pfx[k], strlen(pfx[k])) == 0;
 }
 

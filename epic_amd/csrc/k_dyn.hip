@@ -624,58 +624,62 @@ __device__ __forceinline__ HvRange hv_range(const MemberDesc &d, int kind) {
   const MemberDesc &d = D[mb + (int)blockIdx.z - fi * nm];      \
   const int f = fs.f[fi], kind = fs.kind[fi]
 
-// The two d2dy2 passes of del^4 (coef2, then coef4) in one launch: a block
-// holds a tile of D2_TJ rows x CW columns of one plane; the field on rows
-// J0-3..J0+D2_TJ+2 and the first pass on rows J0-2..J0+D2_TJ+1 (each row
-// clipped to 0..jmax) are staged in LDS, so each first-pass value is formed
-// once per tile (the halo rows again by the neighbouring tile: the same
-// values) by the same expression.
-constexpr int D2_TJ = 8;
-template <class T>
-__global__ void __launch_bounds__(256) k_d2dy2x2(const MemberDesc *D, int mb, int nm, FieldSet fs, int CW) {
+// The two d2dy2 passes of del^4 (coef2, then coef4) in one launch, by
+// column: a thread owns column I of a tile of TJ rows of one plane, loads
+// the field on rows J0-2..J0+TJ+1 (clipped to 0..jmax) into registers at
+// once (all the loads in flight together), and forms each first-pass value
+// it needs from them (three per output row, rather than staging them: the
+// same expressions, so the same values).  Rows 0, 1, jmax-1 and jmax, whose
+// stencils are shifted, read the field again (from cache).
+template <class T, int TJ>
+__global__ void __launch_bounds__(256) k_d2dy2_cols(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
-  extern __shared__ double lds_raw[];
-  T *th = reinterpret_cast<T *>(lds_raw), *tp = th + (D2_TJ + 6) * CW;
   const int vg = (kind == 1);
   const int jmax = vg ? d.nj + 1 : d.nj;
   HvRange r = hv_range(d, kind);
   const int K = r.k0 + blockIdx.y;
   if (K > r.k1) return;  // uniform per block
-  const int nchunk = d.ni / CW, tile = blockIdx.x / nchunk, chunk = blockIdx.x - tile * nchunk;
-  const int J0 = tile * D2_TJ;
+  const int nchunk = d.ni / (int)blockDim.x, tile = blockIdx.x / nchunk, chunk = blockIdx.x - tile * nchunk;
+  const int J0 = tile * TJ;
   if (J0 > jmax) return;  // uniform per block (the grid covers the v rows)
-  const int c = threadIdx.x % CW, r0 = threadIdx.x / CW, RS = blockDim.x / CW, I = 1 + chunk * CW + c;
+  const int I = 1 + chunk * (int)blockDim.x + (int)threadIdx.x;
   const double *hh = fld_ptr(d, f) + (size_t)K * d.N2;
   T *out = reinterpret_cast<T *>(d.sB) + (size_t)(fs.copy0 + fi) * d.N3 + (size_t)K * d.N2;
   const double *cx = vg ? d.vcx : d.hcx, *cy = vg ? d.vcy : d.hcy, *cz = vg ? d.vcz : d.hcz;
-  auto rows = [jmax](int J, int &j0, int &j1, int &j2) {
-    if (J == 0) { j0 = 0; j1 = 1; j2 = 2; }
-    else if (J == jmax) { j0 = J - 2; j1 = J - 1; j2 = J; }
-    else { j0 = J - 1; j1 = J; j2 = J + 1; }
-  };
-  for (int q = r0; q < D2_TJ + 6; q += RS) {
-    const int J = J0 - 3 + q;
-    th[q * CW + c] = (J >= 0 && J <= jmax) ? T(hh[ix2(d, J, I)]) : T(0.);
-  }
-  __syncthreads();
   const T c2 = T(d.coef2), c4 = T(d.coef4);
-  for (int q = r0; q < D2_TJ + 4; q += RS) {
-    const int J = J0 - 2 + q;
-    if (J < 0 || J > jmax) continue;
-    int j0, j1, j2;
-    rows(J, j0, j1, j2);
-    T a0 = c2 * T(cx[J]), a1 = c2 * T(cy[J]), a2 = c2 * T(cz[J]);
-    tp[q * CW + c] = a0 * th[(j0 - J0 + 3) * CW + c] + a1 * th[(j1 - J0 + 3) * CW + c] + a2 * th[(j2 - J0 + 3) * CW + c];
+  T w[TJ + 4];  // rows J0-2 .. J0+TJ+1
+#pragma unroll
+  for (int t = 0; t < TJ + 4; t++) {
+    const int J = J0 - 2 + t;
+    w[t] = (J >= 0 && J <= jmax) ? T(hh[ix2(d, J, I)]) : T(0.);
   }
-  __syncthreads();
-  for (int q = r0; q < D2_TJ; q += RS) {
-    const int J = J0 + q;
+  // first pass at row j from rows (j0, j1, j2)
+  auto p1 = [&](int j, T h0, T h1, T h2) {
+    T a0 = c2 * T(cx[j]), a1 = c2 * T(cy[j]), a2 = c2 * T(cz[j]);
+    return a0 * h0 + a1 * h1 + a2 * h2;
+  };
+  auto fld = [&](int j) { return T(hh[ix2(d, j, I)]); };
+  auto p1_at = [&](int j) {  // any row, stencil shifted at the ends
+    const int c = min(max(j, 1), jmax - 1);
+    return p1(j, fld(c - 1), fld(c), fld(c + 1));
+  };
+#pragma unroll
+  for (int t = 0; t < TJ; t++) {
+    const int J = J0 + t;
     if (J > jmax) break;
-    int j0, j1, j2;
-    rows(J, j0, j1, j2);
-    const T p0 = tp[(j0 - J0 + 2) * CW + c], p1 = tp[(j1 - J0 + 2) * CW + c], p2 = tp[(j2 - J0 + 2) * CW + c];
+    T q0, q1, q2;
+    if (J >= 2 && J <= jmax - 2) {
+      q0 = p1(J - 1, w[t], w[t + 1], w[t + 2]);
+      q1 = p1(J, w[t + 1], w[t + 2], w[t + 3]);
+      q2 = p1(J + 1, w[t + 2], w[t + 3], w[t + 4]);
+    } else {
+      const int c = min(max(J, 1), jmax - 1);
+      q0 = p1_at(c - 1);
+      q1 = p1_at(c);
+      q2 = p1_at(c + 1);
+    }
     T a0 = c4 * T(cx[J]), a1 = c4 * T(cy[J]), a2 = c4 * T(cz[J]);
-    out[ix2(d, J, I)] = a0 * p0 + a1 * p1 + a2 * p2;
+    out[ix2(d, J, I)] = a0 * q0 + a1 * q1 + a2 * q2;
   }
 }
 
@@ -750,17 +754,34 @@ __global__ void __launch_bounds__(256) k_d2dy2_pass(const MemberDesc *D, int mb,
   out[ix2(d, J, I)] = a0 * hh[ix2(d, j0, I)] + a1 * hh[ix2(d, j1, I)] + a2 * hh[ix2(d, j2, I)];
 }
 
+// the two d2dy2 passes of del^4 into copy i of sB, by column tiles of up to
+// 256 columns and TJ rows (the rows_d2 knob; built in: 32 rows where that
+// still gives 2048 blocks, else 16 or 8)
+template <class T>
+static void d2dy2_del4(hipStream_t s, const Bucket &b, const FieldSet &fs) {
+  bool any_v = false;
+  for (int i = 0; i < fs.n; i++) any_v |= (fs.kind[i] == 1);
+  // rows 0..nj (+ nj+1 on the v grid); the kernel drops rows past its kind's
+  const int jrows = any_v ? b.nj + 2 : b.nj + 1;
+  const int CW = b.ni < 256 ? b.ni : 256;
+  const int per = (b.ni / CW) * b.nk * fs.n * b.nm;  // blocks per row tile
+  const int TJ = b.knob[KN_ROWS_D2]                           ? b.knob[KN_ROWS_D2]
+                 : ((jrows + 31) / 32 * per >= 2048) ? 32
+                 : ((jrows + 15) / 16 * per >= 2048) ? 16
+                                                     : 8;
+  dim3 g((jrows + TJ - 1) / TJ * (b.ni / CW), b.nk, fs.n * b.nm);
+  if (TJ == 32) LAUNCH_AS("k_d2dy2_cols<T>", (k_d2dy2_cols<T, 32>), g, dim3(CW), 0, s, b.D, b.mb, b.nm, fs);
+  else if (TJ == 16) LAUNCH_AS("k_d2dy2_cols<T>", (k_d2dy2_cols<T, 16>), g, dim3(CW), 0, s, b.D, b.mb, b.nm, fs);
+  else LAUNCH_AS("k_d2dy2_cols<T>", (k_d2dy2_cols<T, 8>), g, dim3(CW), 0, s, b.D, b.mb, b.nm, fs);
+}
+
 // the mixed mode forms del^4 in FP32 (the field and its update stay FP64)
 template <class T>
 static void merid_hv_t(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   bool any_v = false;
   for (int i = 0; i < fs.n; i++) any_v |= (fs.kind[i] == 1);
-  // rows 0..nj (+ nj+1 on the v grid); the kernel drops rows past its kind's
   const int jrows = any_v ? b.nj + 2 : b.nj + 1;
-  int CW = 64;
-  while (b.ni % CW) CW >>= 1;
-  dim3 g((jrows + D2_TJ - 1) / D2_TJ * (b.ni / CW), b.nk, fs.n * b.nm);
-  LAUNCH(k_d2dy2x2<T>, g, dim3(256), sizeof(T) * (2 * D2_TJ + 10) * CW, s, b.D, b.mb, b.nm, fs, CW);
+  d2dy2_del4<T>(s, b, fs);
   const int rpb = rows_per_block(b.ni);
   const dim3 ga((b.nj + 2 + rpb - 1) / rpb, b.nk, fs.n * b.nm);
   LAUNCH(k_hv_apply<T>, ga, dim3(256), sizeof(T) * rpb * 2 * b.ni, s, b.D, b.mb, b.nm, fs, 1, 0);
@@ -965,25 +986,35 @@ __global__ void __launch_bounds__(256) k_hv4_rows(const MemberDesc *D, int mb, i
   double *a = fld_ptr(d, f) + (size_t)K * d.N2;
   const T *dd = reinterpret_cast<const T *>(d.sB) + (size_t)(fs.copy0 + fi) * d.N3 + (size_t)K * d.N2;
   const bool upd = live && J >= r.jbot && J <= d.nj;
-  // the meridional part: D filtered in place (x = z0[1..ni], work = z1)
-  T *x = z0 + 1;
-  if (upd)
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int I = 1 + m.lane + m.stride * q;
-      x[I - 1] = dd[ix2(d, J, I)];
-    }
-  __syncthreads();
-  filter_row_lds(d, x, z1, J, vg, upd && (vg ? d.rflag_v[J] : d.rflag_h[J]) != 0, m.lane);
-  __syncthreads();
+  // the row of A and the row of D, both loads in flight together
   double v[4];
-  const double sign = -1.;  // del^4
+  T dq[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int I = 1 + m.lane + m.stride * q;
     v[q] = live ? a[ix2(d, J, I)] : 0.;
-    if (upd) v[q] += d.dt * sign * (double)x[I - 1];
+    dq[q] = upd ? dd[ix2(d, J, I)] : T(0.);
   }
+  // the meridional part: D filtered in place (x = z0[1..ni], work = z1) in
+  // blocks that hold a row of the filter band (a block-uniform test)
+  const int *rflag = vg ? d.rflag_v : d.rflag_h;
+  bool anyf = false;
+  for (int g = 0, J1 = blockIdx.x * m.rpb; g < m.rpb; g++, J1++)
+    anyf |= (J1 >= r.jbot && J1 <= d.nj && rflag[J1] != 0);
+  T *x = z0 + 1;
+  if (anyf) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) x[m.lane + m.stride * q] = dq[q];
+    __syncthreads();
+    filter_row_lds(d, x, z1, J, vg, upd && rflag[J] != 0, m.lane);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++) dq[q] = x[m.lane + m.stride * q];
+  }
+  const double sign = -1.;  // del^4
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (upd) v[q] += d.dt * sign * (double)dq[q];
   // the updated row, pads included, as the zonal part reads it (each lane
   // rewrites only the points it read)
 #pragma unroll
@@ -1056,13 +1087,7 @@ __global__ void __launch_bounds__(256) k_hv4_rows(const MemberDesc *D, int mb, i
 // over more rows at once).
 static void hv4_mixed(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   using T = float;
-  bool any_v = false;
-  for (int i = 0; i < fs.n; i++) any_v |= (fs.kind[i] == 1);
-  const int jrows = any_v ? b.nj + 2 : b.nj + 1;
-  int CW = 64;
-  while (b.ni % CW) CW >>= 1;
-  dim3 g((jrows + D2_TJ - 1) / D2_TJ * (b.ni / CW), b.nk, fs.n * b.nm);
-  LAUNCH(k_d2dy2x2<T>, g, dim3(256), sizeof(T) * (2 * D2_TJ + 10) * CW, s, b.D, b.mb, b.nm, fs, CW);
+  d2dy2_del4<T>(s, b, fs);
   const int rpb = rows_per_block(b.ni);
   const dim3 ga((b.nj + 2 + rpb - 1) / rpb, b.nk, fs.n * b.nm);
   const size_t shm = sizeof(T) * rpb * 3 * (b.ni + 2);

@@ -37,7 +37,7 @@ struct FieldSet {
 // Launch shapes a bucket can be tuned over (epic_engine_autotune); 0 takes
 // the built-in choice.  None of them changes a result: every cell is formed
 // by the same operations whatever the tile.
-enum Knob { KN_COLS_P2ETC, KN_COLS_GZ3, KN_COLS_CALCW, KN_COLS_HSUV, KN_ROWS_ZHV, KN_ROWS_MER, KN_COUNT };
+enum Knob { KN_COLS_P2ETC, KN_COLS_GZ3, KN_COLS_CALCW, KN_COLS_HSUV, KN_ROWS_ZHV, KN_ROWS_MER, KN_ROWS_D2, KN_COUNT };
 // A column-tile knob holds the columns per block C and, from 1000 up, the
 // block's threads: v = C (256 threads), v = 1000 n + C (256 n threads,
 // n = 2 or 4: more lanes per column, more waves in flight per CU for the

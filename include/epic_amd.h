@@ -430,7 +430,8 @@ void epic_nc_header_free(void *header);
  * shapes): for each bucket, time every candidate launch shape of the tunable
  * kernels (columns per column tile of set_p2etc, gz3, calc_w and the
  * vertical advection pass, and the tile's threads; rows per
- * zonal-hyperviscosity block and per meridional-advection tile) over
+ * zonal-hyperviscosity block, per meridional-advection tile and per column
+ * tile of the del^4 meridional second differences) over
  * `steps` steps of the bucket's own members and keep the fastest.  The
  * members' state is copied aside and put back, so the call changes no
  * result and no state; it needs room for one copy of every member.  Writes
@@ -438,7 +439,8 @@ void epic_nc_header_free(void *header);
 int  epic_engine_autotune(epic_engine *e, int steps, char *report, int cap);
 /* One launch shape set by hand for every bucket (knob: 0 columns per
  * set_p2etc tile, 1 gz3, 2 calc_w, 3 vertical advection, 4 rows per
- * zonal-hyperviscosity block, 5 rows per meridional-advection tile; a
+ * zonal-hyperviscosity block, 5 rows per meridional-advection tile, 6 rows
+ * per column tile of the del^4 meridional second differences (8, 16, 32); a
  * column-tile value is C, 1000 n + C for blocks of 256 n threads, n = 2
  * or 4, or 10000 + C for one thread per level of each of the C columns; 0
  * restores the built-in choice).  Values outside the candidates the

@@ -136,7 +136,7 @@ def test_autotune_changes_no_state_and_no_result():
     before = {k: {f: e.read(s, f) for f in PROG} for k, s in slots.items()}
     rep = e.autotune(1)
     say(rep)
-    assert rep.count("\n") == 3 * 6  # three buckets, six shapes each
+    assert rep.count("\n") == 3 * 7  # three buckets, seven shapes each
     for k, s in slots.items():
         assert e.steps_done(s) == 2
         for f in PROG:
@@ -153,8 +153,9 @@ def test_autotune_changes_no_state_and_no_result():
 def test_every_column_tile_shape_gives_the_same_bits(precision):
     """Candidate shapes of the column-tile kernels (16-64 columns per block
     of 256, 512 or 1024 threads, or 4-12 columns with one thread per level),
-    set by hand on one knob at a time, step a Held-Suarez and a Jupiter-like
-    member to the same bits as the built-in shapes."""
+    set by hand on one knob at a time, and 8-32 rows per column tile of the
+    del^4 meridional second differences, step a Held-Suarez and a
+    Jupiter-like member to the same bits as the built-in shapes."""
     members = (A, ea.spec(ea.JUPITER_LIKE, 64, 32, 20, 60, 5, 0.5, 0.0, 0.0, 0.5))
 
     def run(knob=None, v=0):
@@ -175,6 +176,12 @@ def test_every_column_tile_shape_gives_the_same_bits(precision):
             got = run(knob, v)
             for m, (g, r) in enumerate(zip(got, ref)):
                 bad += [(knob, v, m, f) for f in r if g[f].tobytes() != r[f].tobytes()]
+    # rows per column tile of the del^4 meridional second differences
+    for v in (8, 16, 32):
+        say(precision, "rows_d2", v)
+        got = run("rows_d2", v)
+        for m, (g, r) in enumerate(zip(got, ref)):
+            bad += [("rows_d2", v, m, f) for f in r if g[f].tobytes() != r[f].tobytes()]
     assert not bad
 
 
