@@ -36,9 +36,12 @@ def test_measured_traffic_lookup():
     # the zonal hyperviscosity's profile name carries its register layout
     t, _ = bench.measured_traffic("hot_jupiter_1024x512x64", "fp64", "k_zonal_hv<T>")
     assert t and t > 0
-    # the mixed mode's FP32 instantiation is found by its base name
-    t, src = bench.measured_traffic("jupiter_256x128x20", "mixed", "k_hv_apply<T>")
+    # the mixed mode's FP32 instantiation is found by its base name, and so
+    # is the fused del^4 row kernel, whose profile name carries its template
+    t, src = bench.measured_traffic("jupiter_256x128x20", "mixed", "k_hsu_mer<T>")
     assert t and t > 0 and src.endswith("traffic_jupiter_256x128x20_mixed.json")
+    t, _ = bench.measured_traffic("jupiter_ensemble_64x128x64x16", "mixed", "k_hv4_rows")
+    assert t and t > 0
     assert bench.measured_traffic("no_such_workload", "fp64", "k_hv_apply<T>") == (None, None)
 
 
