@@ -175,30 +175,53 @@ __device__ __forceinline__ int fcmp_d(double x1, double x2) {
   return 0;
 }
 
+// exp, log and pow of the FP64 hydrostatics: correctly rounded (crmath.h)
+// in the FP64 path, whose results are the oracle's bit for bit; the device
+// library's own (within an ulp or two) when F, the mixed mode, whose fields
+// are held to FP32's accuracy and which gains the time of the double-double
+// evaluation
+template <bool F> __device__ __forceinline__ double m_exp(double x) {
+  if constexpr (F) return exp(x);
+  else return crm_exp(x);
+}
+template <bool F> __device__ __forceinline__ double m_log(double x) {
+  if constexpr (F) return log(x);
+  else return crm_log(x);
+}
+template <bool F> __device__ __forceinline__ double m_pow(double x, double y) {
+  if constexpr (F) return pow(x, y);
+  else return crm_pow(x, y);
+}
+
 // hybrid sigma-theta coordinate (src/shared/epic_funcs_diag.c:1238-1314,1751-1770)
+template <bool F = false>
 __device__ __forceinline__ double get_sigma(double pbot, double p, double ptop) {
-  return crm_log(p / pbot) / crm_log(ptop / pbot);
+  return m_log<F>(p / pbot) / m_log<F>(ptop / pbot);
 }
+template <bool F = false>
 __device__ __forceinline__ double get_p_sigma(double pbot, double s, double ptop) {
-  return pbot * crm_exp(s * crm_log(ptop / pbot));
+  return pbot * m_exp<F>(s * m_log<F>(ptop / pbot));
 }
+template <bool F = false>
 __device__ __forceinline__ double g_sig(const MemberDesc &d, double s) {
   if (s <= d.sigma_sigma) return 0.;
   if (s >= 1.) return 1.;
-  return d.g_coeff * (1. - crm_exp(-d.alpha * (s - d.sigma_sigma)));
+  return d.g_coeff * (1. - m_exp<F>(-d.alpha * (s - d.sigma_sigma)));
 }
+template <bool F = false>
 __device__ __forceinline__ double f_sig(const MemberDesc &d, double s) {
   if (s < 0.) return d.zeta0;
   if (s <= d.sigma_sigma) return d.zeta0 + s * (d.zeta1 - d.zeta0);
-  if (s < 1.) return (1. - g_sig(d, s)) * (d.zeta0 + s * (d.zeta1 - d.zeta0));
+  if (s < 1.) return (1. - g_sig<F>(d, s)) * (d.zeta0 + s * (d.zeta1 - d.zeta0));
   return 0.;
 }
 // (sigmatheta - f_sigma(s)) / g_sigma(s), the theta of a hybrid level at
 // sigma s, with g_sigma formed once for both terms (the same operations as
 // calling f_sig and g_sig, so the same result)
+template <bool F = false>
 __device__ __forceinline__ double theta_of_sigma(const MemberDesc &d, double sgth, double s) {
   const bool mid = (s > d.sigma_sigma && s < 1.);
-  const double gm = mid ? d.g_coeff * (1. - crm_exp(-d.alpha * (s - d.sigma_sigma))) : 0.;
+  const double gm = mid ? d.g_coeff * (1. - m_exp<F>(-d.alpha * (s - d.sigma_sigma))) : 0.;
   const double lin = d.zeta0 + s * (d.zeta1 - d.zeta0);
   const double g = (s <= d.sigma_sigma) ? 0. : (s >= 1. ? 1. : gm);
   const double f = (s < 0.) ? d.zeta0 : (s <= d.sigma_sigma ? lin : (s < 1. ? (1. - gm) * lin : 0.));
@@ -219,17 +242,20 @@ __device__ __forceinline__ double sgth_of(const MemberDesc &d, double theta, dou
 // The same three functions with the column's log(ptop/pbot) formed once by
 // the caller: identical operations on identical operands, so identical
 // results, with one logarithm fewer per call.
-__device__ __forceinline__ double get_sigma_l(double pbot, double p, double lr) { return crm_log(p / pbot) / lr; }
-__device__ __forceinline__ double get_p_sigma_l(double pbot, double s, double lr) { return pbot * crm_exp(s * lr); }
+template <bool F = false>
+__device__ __forceinline__ double get_sigma_l(double pbot, double p, double lr) { return m_log<F>(p / pbot) / lr; }
+template <bool F = false>
+__device__ __forceinline__ double get_p_sigma_l(double pbot, double s, double lr) { return pbot * m_exp<F>(s * lr); }
+template <bool F = false>
 __device__ __forceinline__ double sgth_of_l(const MemberDesc &d, double theta, double p, double pbot, double ptop,
                                             double lr, int *bad) {
   if (p <= ptop) return theta;
   if (p <= pbot) {
-    const double s = get_sigma_l(pbot, p, lr);
+    const double s = get_sigma_l<F>(pbot, p, lr);
     // f_sig(s) + g_sig(s) * theta with g_sigma's exponential formed once (the
     // same operations as the two calls, so the same result)
     const bool mid = (s > d.sigma_sigma && s < 1.);
-    const double gm = mid ? d.g_coeff * (1. - crm_exp(-d.alpha * (s - d.sigma_sigma))) : 0.;
+    const double gm = mid ? d.g_coeff * (1. - m_exp<F>(-d.alpha * (s - d.sigma_sigma))) : 0.;
     const double g = (s <= d.sigma_sigma) ? 0. : (s >= 1. ? 1. : gm);
     const double f = (s < 0.) ? d.zeta0 : (s <= d.sigma_sigma ? d.zeta0 + s * (d.zeta1 - d.zeta0)
                                                               : (s < 1. ? (1. - gm) * (d.zeta0 + s * (d.zeta1 - d.zeta0)) : 0.));

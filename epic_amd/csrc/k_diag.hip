@@ -136,8 +136,10 @@ static inline dim3 tile_grid(const Bucket &b, int C) {
 // pressure-range check) only: the P2 and H3 it would form are written again
 // by the step's second set_p2etc before anything reads them, so the values
 // the step leaves are the same.
+template <bool F>
 __device__ __forceinline__ void thermo_cell(const MemberDesc &d, int K, int J, int I, bool full, double th, double p);
-template <int FULL>
+// F: the mixed mode's exp/log/pow (m_exp)
+template <int FULL, bool F>
 __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb, int C) {
   constexpr bool update_theta = FULL;
   MEMBER;
@@ -173,21 +175,21 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
       a[K] = acc;
     }
     // log(ptop/pbot) once per column (the sigma-region formulas read it)
-    aux[3] = crm_log(a[0] / a[nk]);
+    aux[3] = m_log<F>(a[0] / a[nk]);
   }
   __syncthreads();
   if (T.live) {
     const double pbot = a[nk], lr = aux[3];
     for (int K = ks - 1 + T.kk; K < nk; K += T.KS) {
       double sigma = (d.sgth[2 * K + 1] - d.zeta0) / (d.zeta1 - d.zeta0);
-      a[K] = get_p_sigma_l(pbot, sigma, lr);
+      a[K] = get_p_sigma_l<F>(pbot, sigma, lr);
     }
   }
   __syncthreads();
   // p^(kappa+1) once per interface (each is the top of one layer and the
   // bottom of the next)
   if (FULL && T.live)
-    for (int K = T.kk; K <= nk; K += T.KS) pw[K] = crm_pow(a[K], d.kappa + 1.);
+    for (int K = T.kk; K <= nk; K += T.KS) pw[K] = m_pow<F>(a[K], d.kappa + 1.);
   __syncthreads();
   if (T.live) {
     const double pbot = a[nk], kp1 = d.kappa + 1., lr = aux[3];
@@ -196,8 +198,8 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
       const size_t q = ix3(d, K, J, I);
       P3[q] = a[K];
       if (update_theta && K < ks - 1) {
-        double sigma = get_sigma_l(pbot, a[K], lr);
-        double th = theta_of_sigma(d, d.sgth[2 * K + 1], sigma);
+        double sigma = get_sigma_l<F>(pbot, a[K], lr);
+        double th = theta_of_sigma<F>(d, d.sgth[2 * K + 1], sigma);
         d.THETA[q] = th;
         if (K == ks - 2) aux[0] = th;
       }
@@ -211,7 +213,7 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
       double top = a[K - 1], bot = a[K], pp;
       if (fcmp_d(bot, top) == 0) pp = .5 * (bot + top);
       else if (top < 0. || bot < 0.) { atomicOr(&d.dyn->err, 2); pp = 0.; }
-      else pp = crm_pow((pw[K] - pw[K - 1]) / (kp1 * (bot - top)), 1. / d.kappa);
+      else pp = m_pow<F>((pw[K] - pw[K - 1]) / (kp1 * (bot - top)), 1. / d.kappa);
       if (K == 1) aux[2] = pp;
       d.P2[q] = pp;
       double h3;
@@ -227,8 +229,8 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
       if (K == 1) aux[1] = h3;
       if (update_theta) {
         if (K < ks - 1) {
-          double sigma = get_sigma_l(pbot, pp, lr);
-          d.THETA2[q] = theta_of_sigma(d, d.sgth[2 * K], sigma);
+          double sigma = get_sigma_l<F>(pbot, pp, lr);
+          d.THETA2[q] = theta_of_sigma<F>(d, d.sgth[2 * K], sigma);
         } else if (K >= ks) {
           // linear interpolation of theta against sigmatheta, table ordered
           // from the bottom interface up (src/shared/epic_funcs_diag.c:3522-3562);
@@ -254,13 +256,18 @@ __global__ void __launch_bounds__(1024) k_set_p2etc(const MemberDesc *D, int mb,
   // k_thermo, whose one lane per cell hides the latency of its pow chains
   // better than the tile's few lanes per column.
   if (!FULL && T.live)
-    for (int K = T.kk; K <= nk; K += T.KS) thermo_cell(d, K, J, I, false, d.THETA[ix3(d, K, J, I)], a[K]);
+    for (int K = T.kk; K <= nk; K += T.KS) thermo_cell<F>(d, K, J, I, false, d.THETA[ix3(d, K, J, I)], a[K]);
 }
 void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
   const int v = b.knob[KN_COLS_P2ETC], C = v ? knob_cols(v) : tile_cols(b, 2), nt = knob_threads(v, b.nk + 1);
   size_t shm = sizeof(double) * (2 * C * (b.nk + 1) + 4 * C);
-  if (update_theta) LAUNCH(k_set_p2etc<1>, tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
-  else LAUNCH(k_set_p2etc<0>, tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
+  if (b.mixed) {
+    if (update_theta) LAUNCH_AS("k_set_p2etc<1>", (k_set_p2etc<1, true>), tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
+    else LAUNCH_AS("k_set_p2etc<0>", (k_set_p2etc<0, true>), tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
+  } else {
+    if (update_theta) LAUNCH_AS("k_set_p2etc<1>", (k_set_p2etc<1, false>), tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
+    else LAUNCH_AS("k_set_p2etc<0>", (k_set_p2etc<0, false>), tile_grid(b, C), dim3(nt), shm, s, b.D, b.mb, C);
+  }
 }
 
 // ------------------------------------------------- store_pgrad_vars parts
@@ -270,7 +277,9 @@ void L_set_p2etc(hipStream_t s, const Bucket &b, int update_theta) {
 // full = 0 (the step's first store_pgrad_vars): T3 and RHO3 only, all that
 // the geopotential integration and the filtered GZ3 copy read before the
 // second call writes every output again.
+template <bool F = false>
 __device__ __forceinline__ double hs_relax(const MemberDesc &d, int J, double p, double t, double pbot, double pk0);
+template <bool F = false>
 __device__ __forceinline__ double hj_relax(const MemberDesc &d, int J, int I, double p, double t, double pk);
 // full = 1 also forms the Newtonian relaxation HEAT3 of calc_heating on
 // K = 1..nk, J = 0..nj and the zonal pads: it reads only P3 and T3, which
@@ -279,20 +288,22 @@ __device__ __forceinline__ double hj_relax(const MemberDesc &d, int J, int I, do
 // value of the column it copies)
 // T3 and RHO3 of an interface cell from its theta and p (return_temp and
 // return_density with x_h2 = 0); pk = (p/press0)^kappa
+template <bool F = false>
 __device__ __forceinline__ double t3_of(const MemberDesc &d, double th, double p, double *pk) {
-  *pk = crm_pow(p / d.press0, d.kappa);
+  *pk = m_pow<F>(p / d.press0, d.kappa);
   return th * *pk;
 }
 __device__ __forceinline__ double rho3_of(const MemberDesc &d, double p, double t) {
   const double R_GAS = 8.314472e+3;
   return p * d.mu3 / (R_GAS * t);
 }
+template <bool F>
 __device__ __forceinline__ void thermo_cell(const MemberDesc &d, int K, int J, int I, bool full, double th, double p) {
   const size_t q = ix3(d, K, J, I);
   if (full && K >= 1 && d.dyn->itime > 1 && I >= 1 && I <= d.ni) d.DZDT3[q] = d.GZ3[q];
   const double R_GAS = 8.314472e+3;
   double pk;
-  double t = t3_of(d, th, p, &pk);
+  double t = t3_of<F>(d, th, p, &pk);
   double rho3 = rho3_of(d, p, t);
   d.T3[q] = t;
   d.RHO3[q] = rho3;
@@ -302,23 +313,23 @@ __device__ __forceinline__ void thermo_cell(const MemberDesc &d, int K, int J, i
     if (d.is_tab) {
       h = 0. + tab_relax(d, p, t);
     } else if (d.is_hj) {
-      h = hj_relax(d, J, (I == 0) ? d.ni : (I == d.ni + 1 ? 1 : I), p, t, pk);
+      h = hj_relax<F>(d, J, (I == 0) ? d.ni : (I == d.ni + 1 ? 1 : I), p, t, pk);
     } else {
-      const double pk0 = (d.press0 == 1000. * 100.) ? pk : crm_pow(p / (1000. * 100.), d.kappa);
-      h = 0. + hs_relax(d, J, p, t, d.P3[ix3(d, d.nk, J, I)], pk0);
+      const double pk0 = (d.press0 == 1000. * 100.) ? pk : m_pow<F>(p / (1000. * 100.), d.kappa);
+      h = 0. + hs_relax<F>(d, J, p, t, d.P3[ix3(d, d.nk, J, I)], pk0);
     }
     d.HEAT3[q] = h;
   }
   d.EXNER3[q] = d.cp * t / th;
   if (K >= 1) {
     double p2 = d.P2[q], th2 = d.THETA2[q];
-    double t2 = th2 * crm_pow(p2 / d.press0, d.kappa);
+    double t2 = th2 * m_pow<F>(p2 / d.press0, d.kappa);
     d.T2[q] = t2;
     d.RHO2[q] = p2 * d.mu2 / (R_GAS * t2);
   } else {
     const size_t q1 = ix3(d, 1, J, I);
     double p21 = d.P2[q1], th21 = d.THETA2[q1];
-    double t21 = th21 * crm_pow(p21 / d.press0, d.kappa);
+    double t21 = th21 * m_pow<F>(p21 / d.press0, d.kappa);
     double rho21 = p21 * d.mu2 / (R_GAS * t21);
     d.RHO2[q] = rho3 * rho3 / rho21;
     d.THETA2[q] = d.sgth[0];
@@ -328,6 +339,7 @@ __device__ __forceinline__ void thermo_cell(const MemberDesc &d, int K, int J, i
 
 // store_pgrad_vars' per-cell part with full = 1, one lane per (K,J,I),
 // K = 0..nk, J = 0..nj, I = 0..ni+1 (after the FULL set_p2etc)
+template <bool F>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_thermo(const MemberDesc *D, int mb) {
   MEMBER;
   if (d.dyn->itime <= 1) {
@@ -340,10 +352,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
   FLAT2(0, d.nj + 1, 0, d.ni + 2);
   const int K = blockIdx.y;
   const size_t q = ix3(d, K, J, I);
-  thermo_cell(d, K, J, I, true, d.THETA[q], d.P3[q]);
+  thermo_cell<F>(d, K, J, I, true, d.THETA[q], d.P3[q]);
 }
 void L_thermo(hipStream_t s, const Bucket &b) {
-  LAUNCH(k_thermo, grid2(b.nj + 1, b.ni + 2, b.nk + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  if (b.mixed) LAUNCH_AS("k_thermo", k_thermo<true>, grid2(b.nj + 1, b.ni + 2, b.nk + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
+  else LAUNCH_AS("k_thermo", k_thermo<false>, grid2(b.nj + 1, b.ni + 2, b.nk + 1, b.nm), dim3(TPB), 0, s, b.D, b.mb);
 }
 
 // ------------------------------------------------------------ RI2
@@ -422,6 +435,7 @@ __device__ double kin_uniform(const double *mn, const double *mni, int nj, const
 // sequential order), the meridional integral (one block); the column
 // kernel k_gz3 that must follow copies the results into GZ_SURFACE and
 // PV3's plane nk+1 as it starts each column.
+template <bool F>
 __global__ void __launch_bounds__(256) k_gzu_means(const MemberDesc *D, int mb) {
   MEMBER;
   if (!d.is_gas) return;
@@ -436,7 +450,7 @@ __global__ void __launch_bounds__(256) k_gzu_means(const MemberDesc *D, int mb) 
     double pk;
     row[i] = d.U[q];
     row[ni + i] = p;
-    row[2 * ni + i] = rho3_of(d, p, t3_of(d, d.THETA[q], p, &pk));
+    row[2 * ni + i] = rho3_of(d, p, t3_of<F>(d, d.THETA[q], p, &pk));
   }
   __syncthreads();
   if (threadIdx.x < 3) d.rowbuf[threadIdx.x * R + J] = seq_acc(row + threadIdx.x * ni, ni) / ni;
@@ -544,7 +558,8 @@ __global__ void __launch_bounds__(256) k_gzu_core(const MemberDesc *D, int mb) {
 }
 void L_gz_from_u(hipStream_t s, const Bucket &b) {
   if (!b.any_gas) return;
-  LAUNCH(k_gzu_means, dim3(b.nj + 1, 1, b.nm), dim3(64), sizeof(double) * 3 * b.ni, s, b.D, b.mb);
+  if (b.mixed) LAUNCH_AS("k_gzu_means", k_gzu_means<true>, dim3(b.nj + 1, 1, b.nm), dim3(64), sizeof(double) * 3 * b.ni, s, b.D, b.mb);
+  else LAUNCH_AS("k_gzu_means", k_gzu_means<false>, dim3(b.nj + 1, 1, b.nm), dim3(64), sizeof(double) * 3 * b.ni, s, b.D, b.mb);
   LAUNCH(k_gzu_core, dim3(1, 1, b.nm), dim3(256), sizeof(double) * (7 * (b.nj + 3) + 5 * (2 * b.nj + 3)), s, b.D,
          b.mb);
   // GZ_SURFACE and PV3's plane nk+1 are filled in by the k_gz3 that follows
@@ -552,6 +567,7 @@ void L_gz_from_u(hipStream_t s, const Bucket &b) {
 
 // hydrostatic GZ3 and MONT3 from the bottom up (Leslie-Purser 4-point
 // integral in ln p) and the monotonicity screen
+template <bool F>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_gz3(const MemberDesc *D, int mb, int C) {
   MEMBER;
   extern __shared__ double lds[];
@@ -582,7 +598,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
   if (T.live)
     for (int K = T.kk; K <= nk; K += T.KS) {
       tt[K] = d.T3[ix3(d, K, J, I)];
-      if (K >= 1) lg[K] = crm_log(PP(K) / PP(K - 1));
+      if (K >= 1) lg[K] = m_log<F>(PP(K) / PP(K - 1));
     }
   __syncthreads();
   if (T.live)
@@ -645,8 +661,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
 }
 void L_gz3(hipStream_t s, const Bucket &b) {
   const int v = b.knob[KN_COLS_GZ3], C = v ? knob_cols(v) : tile_cols(b, 3);
-  LAUNCH(k_gz3, tile_grid(b, C), dim3(knob_threads(v, b.nk + 1)), sizeof(double) * 3 * C * (b.nk + 1), s, b.D,
-         b.mb, C);
+  const size_t shm = sizeof(double) * 3 * C * (b.nk + 1);
+  if (b.mixed) LAUNCH_AS("k_gz3", k_gz3<true>, tile_grid(b, C), dim3(knob_threads(v, b.nk + 1)), shm, s, b.D, b.mb, C);
+  else LAUNCH_AS("k_gz3", k_gz3<false>, tile_grid(b, C), dim3(knob_threads(v, b.nk + 1)), shm, s, b.D, b.mb, C);
 }
 
 // finish DZDT3 (diagnostic vertical velocity)
@@ -682,6 +699,7 @@ void L_dzdt3(hipStream_t s, const Bucket &b) {
 // ------------------------------------------------------------ heating
 // the Held-Suarez relaxation of a cell from its p, T, the column's bottom
 // pressure and (p/p0)^kappa, p0 = 1000 hPa
+template <bool F>
 __device__ __forceinline__ double hs_relax(const MemberDesc &d, int J, double p, double t, double pbot, double pk0) {
   const double ka = 1. / (40. * 60. * 60. * 24.), ks = 1. / (4. * 60. * 60. * 24.);
   const double dt_y = 60., dth_z = 10., p0 = 1000. * 100.;
@@ -695,7 +713,7 @@ __device__ __forceinline__ double hs_relax(const MemberDesc &d, int J, double p,
   double c2 = d.coslat_h[J];
   c2 *= c2;
   double s2 = 1. - c2, pp0 = p / p0;
-  double teq = (315. - dt_y * s2 - dth_z * crm_log(pp0) * c2) * pk0;
+  double teq = (315. - dt_y * s2 - dth_z * m_log<F>(pp0) * c2) * pk0;
   teq = dmax(200., teq);
   return -cpt * (t - teq);
 }
@@ -710,12 +728,13 @@ __device__ double hs_newton(const MemberDesc &d, int K, int J, int I) {
 // contrast above teq_p_contrast and linear in ln p to 0 at pbot.
 // the day-night relaxation of a cell from its p, T, (p/press0)^kappa and the
 // column I whose longitude it takes
+template <bool F>
 __device__ __forceinline__ double hj_relax(const MemberDesc &d, int J, int I, double p, double t, double pk) {
   const double tref = (1. / (1. / d.ref_theta_b + d.ref_a_over_kappa * (pk - d.ref_pbk))) * pk;
   double dT = d.teq_contrast;
   // the per-member log and the per-row/column cosines come from host tables
   // formed with the host libm, as the oracle forms them
-  if (p > d.teq_p_contrast) dT *= dmax(0., crm_log(d.pbot / p) / d.hj_lnref);
+  if (p > d.teq_p_contrast) dT *= dmax(0., m_log<F>(d.pbot / p) / d.hj_lnref);
   const double cl = d.coslat_h[J];
   const double day = dmax(0., cl * d.coslon_h[I]);
   const double teq = tref + dT * (day - .25);
@@ -808,6 +827,7 @@ void L_heating(hipStream_t s, const Bucket &b, int adjust, bool in_thermo) {
 // With with_source (the step's call) the blocks past the ncb column tiles
 // first clear the step's tendency planes of u and v (the FP64 plane IT_ZERO,
 // or the mixed mode's one FP64 plane), which nothing in this launch reads.
+template <bool F>
 __global__ void __launch_bounds__(1024) k_calc_w(const MemberDesc *D, int mb, int with_source, int C, int ncb) {
   MEMBER;
   if ((int)blockIdx.x >= ncb) {
@@ -833,8 +853,8 @@ __global__ void __launch_bounds__(1024) k_calc_w(const MemberDesc *D, int mb, in
   // lg[k] = log(P3(k)/P3(k-1)), the ln-p intervals of the Leslie-Purser weights
   // and log(ptop/pbot) once per column
   if (T.live) {
-    for (int K = 1 + T.kk; K <= nk; K += T.KS) lg[K] = crm_log(PP(K) / PP(K - 1));
-    if (T.kk == 0) lrv[0] = crm_log(PP(0) / PP(nk));
+    for (int K = 1 + T.kk; K <= nk; K += T.KS) lg[K] = m_log<F>(PP(K) / PP(K - 1));
+    if (T.kk == 0) lrv[0] = m_log<F>(PP(0) / PP(nk));
   }
   __syncthreads();
   if (T.live) {
@@ -850,8 +870,8 @@ __global__ void __launch_bounds__(1024) k_calc_w(const MemberDesc *D, int mb, in
       if (K == nk) continue;
       double wb = 0.;
       if (K < ks - 1) {
-        double sigma = get_sigma_l(pbot, P[q], lr);
-        wb = g_sig(d, sigma) * d.HEAT3[q] / d.EXNER3[q];
+        double sigma = get_sigma_l<F>(pbot, P[q], lr);
+        wb = g_sig<F>(d, sigma) * d.HEAT3[q] / d.EXNER3[q];
       }
       double ds = 0.;
       if (K >= 2) {
@@ -871,7 +891,7 @@ __global__ void __launch_bounds__(1024) k_calc_w(const MemberDesc *D, int mb, in
       if (p2 < psig && p4 >= psig) p2 = p3;
       double dFdp = 0.;
       if (p2 < p4)
-        dFdp = (sgth_of_l(d, th, p2, pb3, ptop, lr, &bad) - sgth_of_l(d, th, p4, pb3, ptop, lr, &bad)) / (p2 - p4);
+        dFdp = (sgth_of_l<F>(d, th, p2, pb3, ptop, lr, &bad) - sgth_of_l<F>(d, th, p4, pb3, ptop, lr, &bad)) / (p2 - p4);
       else
         bad = 1;
       dsv[K] = ds;
@@ -919,7 +939,9 @@ void L_calc_w(hipStream_t s, const Bucket &b, int with_source) {
     const size_t n3 = (size_t)(b.ni + 2) * (b.nj + 3) * (b.nk + 2);
     g.x += (unsigned)std::max<size_t>(1, std::min<size_t>(1024, n3 / ((size_t)nt * 16)));
   }
-  LAUNCH(k_calc_w, g, dim3(nt), sizeof(double) * (4 * C * (b.nk + 1) + C), s, b.D, b.mb, with_source, C, ncb);
+  const size_t shm = sizeof(double) * (4 * C * (b.nk + 1) + C);
+  if (b.mixed) LAUNCH_AS("k_calc_w", k_calc_w<true>, g, dim3(nt), shm, s, b.D, b.mb, with_source, C, ncb);
+  else LAUNCH_AS("k_calc_w", k_calc_w<false>, g, dim3(nt), shm, s, b.D, b.mb, with_source, C, ncb);
 }
 
 // ------------------------------------------------ vorticity / divergence
