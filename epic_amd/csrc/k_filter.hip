@@ -203,7 +203,8 @@ __device__ int pchip_wave(int n, const double *x, const double *y, double *dd, i
   const MemberDesc &d = D[mb + (int)blockIdx.z - fi * nm];      \
   const int f = fs.f[fi], kind = fs.kind[fi]
 
-template <class T>
+// F: the mixed mode's exp/log (m_exp)
+template <class T, bool F>
 __global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
   F3 r = f3_range(d, kind);
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb
     const int K = r.kend - e;
     sx[e] = geez(d, kind, K, J, Imin);
     double v = (double)b[ix3(d, K, J, Imin)];
-    sy[e] = (kind == 2) ? crm_log(v) : v;
+    sy[e] = (kind == 2) ? m_log<F>(v) : v;
   }
   wave_sync();
   if (pchip_wave(n, sx, sy, sz, lane) && lane == 0) atomicOr(&d.dyn->err, 4);
@@ -254,7 +255,7 @@ __global__ void __launch_bounds__(256) k_spline_prep(const MemberDesc *D, int mb
 // band come out as (a - v) + v, as in the reference, which subtracts and adds
 // the profile on every row.  HDRY is stored with the restore_mass that
 // follows the filters in the step.  RB rows per block, ni/4 lanes per row.
-template <class T>
+template <class T, bool F>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_filter3(const MemberDesc *D, int mb, int nm, FieldSet fs) {
   SET_MEMBER;
   extern __shared__ double lds_raw[];
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
       int k = near ? table_place_near(nt, tx, xx, r.kend - K, &h) : table_place(nt, tx, xx, &h);
       double xr = xx - tx[k];
       double val = ty[k] + xr * (tz[k] + xr * (tc2[k] + xr * tc3[k]));
-      if (kind == 2) val = crm_exp(val);
+      if (kind == 2) val = m_exp<F>(val);
       v[q] = val;
       x[q] = T((double)a[ix2(d, J, I)] - val);
     }
@@ -366,11 +367,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
 // cover 0..nj and the kernels drop the rows outside the kind's range)
 void L_filter3(hipStream_t s, const Bucket &b, const FieldSet &fs) {
   const int nrow = b.nj + 1, nz = fs.n * b.nm;
-  LAUNCH(k_spline_prep<double>, dim3(nrow, 1, nz), dim3(64), 0, s, b.D, b.mb, b.nm, fs);
+  if (b.mixed) LAUNCH_AS("k_spline_prep<double>", (k_spline_prep<double, true>), dim3(nrow, 1, nz), dim3(64), 0, s, b.D, b.mb, b.nm, fs);
+  else LAUNCH_AS("k_spline_prep<double>", (k_spline_prep<double, false>), dim3(nrow, 1, nz), dim3(64), 0, s, b.D, b.mb, b.nm, fs);
   const int nq = b.ni / 4, RB = (256 / nq) < 16 ? (256 / nq) : 16;
   dim3 g((nrow + RB - 1) / RB, b.nk, nz);
   size_t shm = sizeof(double) * (size_t)RB * 5 * (b.nk + 2) + sizeof(double) * (size_t)RB * b.ni;
-  LAUNCH(k_filter3<double>, g, dim3(RB * nq), shm, s, b.D, b.mb, b.nm, fs);
+  if (b.mixed) LAUNCH_AS("k_filter3<double>", (k_filter3<double, true>), g, dim3(RB * nq), shm, s, b.D, b.mb, b.nm, fs);
+  else LAUNCH_AS("k_filter3<double>", (k_filter3<double, false>), g, dim3(RB * nq), shm, s, b.D, b.mb, b.nm, fs);
 }
 
 }  // namespace epic
