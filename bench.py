@@ -91,6 +91,10 @@ def measured_traffic(workload, precision, name):
         if cand in k:
             return k[cand]["traffic_bytes"], os.path.relpath(fn, HERE)
     # a kernel with more template arguments than its launch name shows
+    if name.endswith(">"):
+        more = [c for c in k if c.startswith(name[:-1] + ",")]
+        if len(more) == 1:
+            return k[more[0]]["traffic_bytes"], os.path.relpath(fn, HERE)
     base = name.split("<")[0]
     same = [c for c in k if c.split("<")[0] == base]
     if len(same) == 1:

@@ -42,6 +42,11 @@ def test_measured_traffic_lookup():
     assert t and t > 0 and src.endswith("traffic_jupiter_256x128x20_mixed.json")
     t, _ = bench.measured_traffic("jupiter_ensemble_64x128x64x16", "mixed", "k_hv4_rows")
     assert t and t > 0
+    # launch name k_set_p2etc<1>, profile name k_set_p2etc<1, true> (the
+    # mixed mode's math library)
+    t1, _ = bench.measured_traffic("jupiter_256x128x20", "mixed", "k_set_p2etc<1>")
+    t0, _ = bench.measured_traffic("jupiter_256x128x20", "mixed", "k_set_p2etc<0>")
+    assert t1 and t0 and t1 != t0
     assert bench.measured_traffic("no_such_workload", "fp64", "k_hv_apply<T>") == (None, None)
 
 
