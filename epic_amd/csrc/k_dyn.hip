@@ -976,6 +976,12 @@ __global__ void __launch_bounds__(256) k_hv4_rows(const MemberDesc *D, int mb, i
   SET_MEMBER;
   extern __shared__ double lds_raw[];
   const RowMap m = row_map(d.ni);
+  // a row's lanes exchange data only among themselves: within one
+  // wavefront when ni <= 256, so a wavefront barrier suffices there
+  auto row_bar = [&]() {
+    if (m.stride <= 64) wave_sync();
+    else __syncthreads();
+  };
   const int ni = d.ni, WS = ni + 2;
   T *z0 = reinterpret_cast<T *>(lds_raw) + (size_t)m.g * 3 * WS, *z1 = z0 + WS, *z2 = z1 + WS;
   HvRange r = hv_range(d, kind);
@@ -1005,9 +1011,9 @@ __global__ void __launch_bounds__(256) k_hv4_rows(const MemberDesc *D, int mb, i
   if (anyf) {
 #pragma unroll
     for (int q = 0; q < 4; q++) x[m.lane + m.stride * q] = dq[q];
-    __syncthreads();
+    row_bar();
     filter_row_lds(d, x, z1, J, vg, upd && rflag[J] != 0, m.lane);
-    __syncthreads();
+    row_bar();
 #pragma unroll
     for (int q = 0; q < 4; q++) dq[q] = x[m.lane + m.stride * q];
   }
@@ -1028,7 +1034,7 @@ __global__ void __launch_bounds__(256) k_hv4_rows(const MemberDesc *D, int mb, i
   const int Jc = min(J, d.nj);
   const double mm = fabs(d.lat[2 * Jc + r.jh]) > 45. ? d.m0z : d.m[2 * Jc + r.jh];
   T y[4];
-  __syncthreads();
+  row_bar();
   for (int pass = 0; pass < 2; pass++) {
     // pass 0 reads z0 and leaves its raw result in z1; pass 1 reads z2 (pass
     // 0 less its mean) and leaves its raw result in z0
@@ -1041,7 +1047,7 @@ __global__ void __launch_bounds__(256) k_hv4_rows(const MemberDesc *D, int mb, i
       y[q] = dcu * (xin[I - 1] - T(2.) * xin[I] + xin[I + 1]);
       raw[I] = y[q];
     }
-    __syncthreads();
+    row_bar();
     T fm;
     if constexpr (BFLY) {
       // each wavefront of the row's lanes forms the 64 partials itself
@@ -1062,7 +1068,7 @@ __global__ void __launch_bounds__(256) k_hv4_rows(const MemberDesc *D, int mb, i
         if (I == ni) z2[0] = y[q];
         if (I == 1) z2[ni + 1] = y[q];
       }
-      __syncthreads();
+      row_bar();
     }
   }
   if (!live) return;
