@@ -109,6 +109,33 @@ def test_fused_del4_rows_equal_the_separate_kernels(ni, nj, nk):
     e2.close()
 
 
+def test_del4_and_del8_members_of_one_grid_step_as_two_buckets():
+    """In the mixed mode a del^4 member and a del^8 member of the same grid
+    are two buckets (the del^4 member's zonal means come from the fused row
+    kernel, the del^8 member's from the separate kernels); stepping the
+    shape (both buckets, one graph each) or the whole engine gives each
+    member the bits it gets alone."""
+    a = ea.spec(ea.JUPITER_LIKE, 64, 32, 10, 60, 7, 0.5, 0.0, 0.0, 0.5)
+    b = ea.spec(ea.JUPITER_LIKE, 64, 32, 10, 60, 4, 0.0, 0.0, 0.0, 0.5, nu8=0.5)
+    say("del^4 beside del^8, mixed")
+    alone = {}
+    for k, sp in (("a", a), ("b", b)):
+        e = ea.Engine(ea.MIXED, 1)
+        s = e.add(sp)
+        e.step(6)
+        alone[k] = {f: e.read(s, f) for f in FIELDS}
+        e.close()
+    e = ea.Engine(ea.MIXED, 2)
+    sa, sb = e.add(a), e.add(b)
+    e.step_shape(64, 32, 10, 4)
+    e.step(2)
+    assert e.steps_done(sa) == 6 and e.steps_done(sb) == 6
+    for k, s in (("a", sa), ("b", sb)):
+        for f in FIELDS:
+            assert e.read(s, f).tobytes() == alone[k][f].tobytes(), (k, f)
+    e.close()
+
+
 @pytest.mark.parametrize("ni,nj,nk", [(256, 128, 16), (1024, 512, 64)])
 def test_mixed_against_fp64_hot_jupiter(ni, nj, nk):
     """The hot Jupiter in the mixed mode (at 1024x512x64: rows of 1024 points,
